@@ -1,0 +1,1187 @@
+"""CPU FP64 oracle for the finite-PEPS contraction engine.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  The product path (``lgt_amd``) never calls it.
+
+It restates, function by function, the Julia reference (NoaFeldman/LGT) in
+numpy/complex128, keeping the reference's column-major index conventions:
+every ``reshape`` of the reference is ``jr`` (``order='F'``) here and every
+``permutedims`` is ``pd`` with the same 1-based permutation.  Each function
+cites the reference file:line it follows.
+
+Generalisation beyond the reference (recorded in the design notes):
+  * the link dimension is ``gd`` (the reference has ``gd = 2dg+1``); flux
+    labels are stored as integers ``2E``.  For odd ``gd`` this is the
+    reference's model with every label doubled.
+  * a missing boundary link carries the label ``b0 = 0`` for odd ``gd`` (as
+    in the reference) and ``b0 = +1`` (E = +1/2) for even ``gd``, so that the
+    Gauss law stays satisfiable on 3-link edge sites of the spin-1/2 model.
+  * Gauss law in label units: lR + lU - fl - fd == 2 (n_f + q).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+
+FU_REG = 1e-10      # finite_peps_full_update.jl:336
+GI_REG = 1e-12      # finite_peps_gauge_invariant.jl:224
+V4_ALS_REG = 1e-10  # finite_peps_fullupdate_v4.jl:30
+SVD_CUT = 1e-12     # finite_peps_ground_state.jl:57
+CMPS_CUT = 1e-14    # finite_peps_boundary_mps.jl:93
+
+
+def jr(a, *shape):
+    """Julia ``reshape`` (column-major)."""
+    return np.reshape(a, shape, order="F")
+
+
+def pd(a, *perm):
+    """Julia ``permutedims`` with a 1-based permutation."""
+    return np.ascontiguousarray(np.transpose(a, [p - 1 for p in perm]))
+
+
+# ════════════════════════════════════════════════════════════════════════════
+#  Model (u1_lgt_hamiltonian.jl)
+# ════════════════════════════════════════════════════════════════════════════
+
+def link_labels(gd):
+    """Flux labels 2E of one link, index order (gauge_idx, u1_lgt_hamiltonian.jl:95)."""
+    return np.array([2 * i - (gd - 1) for i in range(gd)], dtype=np.int64)
+
+
+def boundary_label(gd):
+    """Label carried by a missing boundary link (0 in the reference, decode_phys :64)."""
+    return 0 if gd % 2 == 1 else 1
+
+
+def site_dims(ix, iy, nx, ny, gd):
+    """u1_lgt_hamiltonian.jl:605-611."""
+    d_gR = gd if ix < nx else 1
+    d_gU = gd if iy < ny else 1
+    return 2 * d_gR * d_gU, d_gR, d_gU
+
+
+def site_idx(nf, lR, lU, d_gR, d_gU, gd):
+    """u1_lgt_hamiltonian.jl:614-618 (labels 2E)."""
+    iR = 1 if d_gR == 1 else (lR + gd - 1) // 2 + 1
+    iU = 1 if d_gU == 1 else (lU + gd - 1) // 2 + 1
+    return nf * d_gR * d_gU + (iR - 1) * d_gU + iU
+
+
+def decode_phys(p, d_gR, d_gU, gd):
+    """finite_peps_gauge_invariant.jl:58-67 (labels 2E, missing -> b0)."""
+    p0 = p - 1
+    nf = p0 // (d_gR * d_gU)
+    rem = p0 % (d_gR * d_gU)
+    iR1 = rem // d_gU
+    iU0 = rem % d_gU
+    b0 = boundary_label(gd)
+    lR = b0 if d_gR == 1 else 2 * iR1 - (gd - 1)
+    lU = b0 if d_gU == 1 else 2 * iU0 - (gd - 1)
+    return nf, lR, lU
+
+
+def _I(n):
+    return np.eye(n, dtype=np.complex128)
+
+
+def op_E(gd):
+    """u1_lgt_hamiltonian.jl:110-117 (E = label/2)."""
+    return np.diag(link_labels(gd) / 2.0).astype(np.complex128)
+
+
+def op_E2(gd):
+    E = op_E(gd)
+    return E @ E
+
+
+def op_U_gauge(gd):
+    """u1_lgt_hamiltonian.jl:123-130: U|e> = |e+1>."""
+    U = np.zeros((gd, gd), dtype=np.complex128)
+    for i in range(gd - 1):
+        U[i + 1, i] = 1.0
+    return U
+
+
+def op_c():
+    """u1_lgt_hamiltonian.jl:138-144."""
+    c = np.zeros((2, 2), dtype=np.complex128)
+    c[0, 1] = 1.0
+    return c
+
+
+def op_nf():
+    """u1_lgt_hamiltonian.jl:150-155."""
+    n = np.zeros((2, 2), dtype=np.complex128)
+    n[1, 1] = 1.0
+    return n
+
+
+def H_onsite_site(ix, iy, nx, ny, gd, g, m, mu=0.0):
+    """u1_lgt_hamiltonian.jl:658-676."""
+    d_phys, d_gR, d_gU = site_dims(ix, iy, nx, ny, gd)
+    H = np.zeros((d_phys, d_phys), dtype=np.complex128)
+    if ix < nx:
+        H += (g ** 2 / 2) * np.kron(np.kron(_I(2), op_E2(gd)), _I(d_gU))
+    if iy < ny:
+        H += (g ** 2 / 2) * np.kron(np.kron(_I(2), _I(d_gR)), op_E2(gd))
+    stag = 1 if (ix + iy) % 2 == 0 else -1
+    H += m * stag * np.kron(np.kron(op_nf(), _I(d_gR)), _I(d_gU))
+    if mu != 0.0:
+        H += mu * np.kron(np.kron(op_nf(), _I(d_gR)), _I(d_gU))
+    return 0.5 * (H + H.conj().T)
+
+
+def H_hop_h_site(ix, iy, nx, ny, gd, t):
+    """u1_lgt_hamiltonian.jl:689-701."""
+    assert ix < nx
+    _, d_gR_L, d_gU_L = site_dims(ix, iy, nx, ny, gd)
+    _, d_gR_R, d_gU_R = site_dims(ix + 1, iy, nx, ny, gd)
+    cdag_U_L = np.kron(np.kron(op_c().conj().T, op_U_gauge(gd)), _I(d_gU_L))
+    c_R = np.kron(np.kron(op_c(), _I(d_gR_R)), _I(d_gU_R))
+    hop = np.kron(cdag_U_L, c_R)
+    H = -t * (hop + hop.conj().T)
+    return 0.5 * (H + H.conj().T)
+
+
+def H_hop_v_site(ix, iy, nx, ny, gd, t):
+    """u1_lgt_hamiltonian.jl:789-801."""
+    assert iy < ny
+    _, d_gR_D, d_gU_D = site_dims(ix, iy, nx, ny, gd)
+    _, d_gR_U, d_gU_U = site_dims(ix, iy + 1, nx, ny, gd)
+    cdag_U_D = np.kron(np.kron(op_c().conj().T, _I(d_gR_D)), op_U_gauge(gd))
+    c_U = np.kron(np.kron(op_c(), _I(d_gR_U)), _I(d_gU_U))
+    hop = np.kron(cdag_U_D, c_U)
+    H = -t * (hop + hop.conj().T)
+    return 0.5 * (H + H.conj().T)
+
+
+def onsite_weight(ix, iy, nx, ny):
+    """u1_lgt_hamiltonian.jl:815-818."""
+    nb = int(ix > 1) + int(ix < nx) + int(iy > 1) + int(iy < ny)
+    return 1.0 / nb
+
+
+def H_merged_h_site(ix, iy, nx, ny, gd, g, t, m, mu=0.0):
+    """u1_lgt_hamiltonian.jl:829-841."""
+    dL = site_dims(ix, iy, nx, ny, gd)[0]
+    dR = site_dims(ix + 1, iy, nx, ny, gd)[0]
+    HL = H_onsite_site(ix, iy, nx, ny, gd, g, m, mu)
+    HR = H_onsite_site(ix + 1, iy, nx, ny, gd, g, m, mu)
+    aL = onsite_weight(ix, iy, nx, ny)
+    aR = onsite_weight(ix + 1, iy, nx, ny)
+    return (H_hop_h_site(ix, iy, nx, ny, gd, t) + aL * np.kron(HL, _I(dR))
+            + aR * np.kron(_I(dL), HR))
+
+
+def H_merged_v_site(ix, iy, nx, ny, gd, g, t, m, mu=0.0):
+    """u1_lgt_hamiltonian.jl:851-863."""
+    dD = site_dims(ix, iy, nx, ny, gd)[0]
+    dU = site_dims(ix, iy + 1, nx, ny, gd)[0]
+    HD = H_onsite_site(ix, iy, nx, ny, gd, g, m, mu)
+    HU = H_onsite_site(ix, iy + 1, nx, ny, gd, g, m, mu)
+    aD = onsite_weight(ix, iy, nx, ny)
+    aU = onsite_weight(ix, iy + 1, nx, ny)
+    return (H_hop_v_site(ix, iy, nx, ny, gd, t) + aD * np.kron(HD, _I(dU))
+            + aU * np.kron(_I(dD), HU))
+
+
+def gate(H, tau):
+    """lgt_gate, u1_lgt_hamiltonian.jl:396: exp(-tau H) (Pade, scaling & squaring)."""
+    return sla.expm(-tau * H)
+
+
+# ════════════════════════════════════════════════════════════════════════════
+#  State (finite_peps_ground_state.jl:88-103, finite_peps_gauge_invariant.jl:82-97)
+# ════════════════════════════════════════════════════════════════════════════
+
+class GaugePEPS:
+    """GaugePEPS: dense masked tensors (p,l,r,u,d) + weights + flux labels.
+
+    Index convention: ``T(ix, iy)`` etc. take the reference's 1-based
+    coordinates.
+    """
+
+    def __init__(self, nx, ny, gd, gch):
+        self.nx, self.ny, self.gd = nx, ny, gd
+        self.gch = np.asarray(gch, dtype=np.int64).reshape(nx, ny)
+        self.tensors = [[None] * ny for _ in range(nx)]
+        self.lh = [[None] * ny for _ in range(nx - 1)]
+        self.lv = [[None] * (ny - 1) for _ in range(nx)]
+        self.qh = [[None] * ny for _ in range(nx - 1)]
+        self.qv = [[None] * (ny - 1) for _ in range(nx)]
+
+    def T(self, ix, iy):
+        return self.tensors[ix - 1][iy - 1]
+
+    def setT(self, ix, iy, A):
+        self.tensors[ix - 1][iy - 1] = A
+
+    def LH(self, ix, iy):
+        return self.lh[ix - 1][iy - 1]
+
+    def LV(self, ix, iy):
+        return self.lv[ix - 1][iy - 1]
+
+    def copy(self):
+        g = GaugePEPS(self.nx, self.ny, self.gd, self.gch)
+        g.tensors = [[t.copy() for t in col] for col in self.tensors]
+        g.lh = [[x.copy() for x in col] for col in self.lh]
+        g.lv = [[x.copy() for x in col] for col in self.lv]
+        g.qh = [[x.copy() for x in col] for col in self.qh]
+        g.qv = [[x.copy() for x in col] for col in self.qv]
+        return g
+
+
+def vdims(gp, ix, iy):
+    """finite_peps_ground_state.jl:97-103."""
+    D_l = len(gp.LH(ix - 1, iy)) if ix > 1 else 1
+    D_r = len(gp.LH(ix, iy)) if ix < gp.nx else 1
+    D_u = len(gp.LV(ix, iy)) if iy < gp.ny else 1
+    D_d = len(gp.LV(ix, iy - 1)) if iy > 1 else 1
+    return D_l, D_r, D_u, D_d
+
+
+def _wt_l(gp, ix, iy):
+    return gp.LH(ix - 1, iy) if ix > 1 else np.ones(1)
+
+
+def _wt_r(gp, ix, iy):
+    return gp.LH(ix, iy) if ix < gp.nx else np.ones(1)
+
+
+def _wt_u(gp, ix, iy):
+    return gp.LV(ix, iy) if iy < gp.ny else np.ones(1)
+
+
+def _wt_d(gp, ix, iy):
+    return gp.LV(ix, iy - 1) if iy > 1 else np.ones(1)
+
+
+def _flux_vec(gp, ix, iy):
+    """Flux labels of the (l, r, u, d) legs of site (ix,iy) (gauge_invariant.jl:94-97)."""
+    b0 = boundary_label(gp.gd)
+    fl = gp.qh[ix - 2][iy - 1] if ix > 1 else np.array([b0])
+    fr = gp.qh[ix - 1][iy - 1] if ix < gp.nx else np.array([b0])
+    fu = gp.qv[ix - 1][iy - 1] if iy < gp.ny else np.array([b0])
+    fd = gp.qv[ix - 1][iy - 2] if iy > 1 else np.array([b0])
+    return (np.asarray(fl), np.asarray(fr), np.asarray(fu), np.asarray(fd))
+
+
+def phys_labels(ix, iy, nx, ny, gd):
+    d, dR, dU = site_dims(ix, iy, nx, ny, gd)
+    nf = np.zeros(d, np.int64)
+    lR = np.zeros(d, np.int64)
+    lU = np.zeros(d, np.int64)
+    for p in range(1, d + 1):
+        nf[p - 1], lR[p - 1], lU[p - 1] = decode_phys(p, dR, dU, gd)
+    return nf, lR, lU
+
+
+def gauss_mask(gp, ix, iy):
+    """finite_peps_gauge_invariant.jl:109-136."""
+    nf, lR, lU = phys_labels(ix, iy, gp.nx, gp.ny, gp.gd)
+    fl, fr, fu, fd = _flux_vec(gp, ix, iy)
+    q = gp.gch[ix - 1, iy - 1]
+    P = nf[:, None, None, None, None]
+    R = lR[:, None, None, None, None]
+    U = lU[:, None, None, None, None]
+    L_ = fl[None, :, None, None, None]
+    Rr = fr[None, None, :, None, None]
+    Uu = fu[None, None, None, :, None]
+    Dd = fd[None, None, None, None, :]
+    return (Rr == R) & (Uu == U) & (R + U - L_ - Dd == 2 * (P + q))
+
+
+def apply_mask(gp, ix, iy):
+    """finite_peps_gauge_invariant.jl:139-146."""
+    T = gp.T(ix, iy)
+    T[~gauss_mask(gp, ix, iy)] = 0
+
+
+def forbidden_norm(gp):
+    """finite_peps_gauge_invariant.jl:154-164."""
+    tot = 0.0
+    for iy in range(1, gp.ny + 1):
+        for ix in range(1, gp.nx + 1):
+            T = gp.T(ix, iy)
+            tot += np.sum(np.abs(T[~gauss_mask(gp, ix, iy)]) ** 2)
+    return np.sqrt(tot)
+
+
+def default_bond_labels(gd, D):
+    """Flux labels of a fresh bond of dimension D (cycle through the link labels)."""
+    labs = link_labels(gd)
+    order = sorted(labs, key=lambda x: (abs(x), x))
+    return np.array([order[i % gd] for i in range(D)], dtype=np.int64)
+
+
+def staggered_charges(nx, ny):
+    """g = -1 on odd ix+iy (gs_benchmark_v4.jl:46-52, finite_peps_boundary_mps.jl:263-266)."""
+    g = np.zeros((nx, ny), np.int64)
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            if (ix + iy) % 2 == 1:
+                g[ix - 1, iy - 1] = -1
+    return g
+
+
+def init_gauge_peps_random(nx, ny, gd, gch, D, seed, complex_entries=False):
+    """Seeded random gauge-invariant PEPS at bond dimension D.
+
+    Analogue of ``init_gauge_peps`` (finite_peps_gauge_invariant.jl:180-218)
+    with the configs' initial state: i.i.d. N(0,1) entries from
+    ``numpy.random.default_rng(seed)`` (PCG64) on Gauss-allowed entries only,
+    each site normalised, all weights 1, bond labels from
+    ``default_bond_labels``.  Entries are drawn site by site, iy outer, ix
+    inner, over the full (p,l,r,u,d) array in column-major order.
+    """
+    gp = GaugePEPS(nx, ny, gd, gch)
+    rng = np.random.default_rng(seed)
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx):
+            gp.lh[ix - 1][iy - 1] = np.ones(D)
+            gp.qh[ix - 1][iy - 1] = default_bond_labels(gd, D)
+    for iy in range(1, ny):
+        for ix in range(1, nx + 1):
+            gp.lv[ix - 1][iy - 1] = np.ones(D)
+            gp.qv[ix - 1][iy - 1] = default_bond_labels(gd, D)
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d = site_dims(ix, iy, nx, ny, gd)[0]
+            shp = (d,) + vdims(gp, ix, iy)
+            n = int(np.prod(shp))
+            v = rng.standard_normal(n)
+            if complex_entries:
+                v = v + 1j * rng.standard_normal(n)
+            T = jr(v.astype(np.complex128), *shp).copy()
+            gp.setT(ix, iy, T)
+            apply_mask(gp, ix, iy)
+            nrm = np.sqrt(np.sum(np.abs(T) ** 2))
+            if nrm > 1e-15:
+                T /= nrm
+    return gp
+
+
+def init_gauge_peps(nx, ny, gd, gch, noise=0.0, seed=None):
+    """finite_peps_gauge_invariant.jl:180-218 (odd gd: product state at zero flux)."""
+    assert gd % 2 == 1, "zero-flux product state needs an odd link dimension"
+    gp = GaugePEPS(nx, ny, gd, gch)
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx):
+            gp.lh[ix - 1][iy - 1] = np.array([1.0])
+            gp.qh[ix - 1][iy - 1] = np.array([0], np.int64)
+    for iy in range(1, ny):
+        for ix in range(1, nx + 1):
+            gp.lv[ix - 1][iy - 1] = np.array([1.0])
+            gp.qv[ix - 1][iy - 1] = np.array([0], np.int64)
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d, dR, dU = site_dims(ix, iy, nx, ny, gd)
+            arr = np.zeros((d, 1, 1, 1, 1), np.complex128)
+            nf0 = -int(gp.gch[ix - 1, iy - 1])
+            assert 0 <= nf0 <= 1
+            arr[site_idx(nf0, 0, 0, dR, dU, gd) - 1, 0, 0, 0, 0] = 1.0
+            gp.setT(ix, iy, arr)
+    if noise > 0:
+        rng = np.random.default_rng(seed)
+        for iy in range(1, ny + 1):
+            for ix in range(1, nx + 1):
+                T = gp.T(ix, iy)
+                T += noise * (rng.standard_normal(T.shape) + 1j * rng.standard_normal(T.shape)) / np.sqrt(2)
+                apply_mask(gp, ix, iy)
+                nrm = np.sqrt(np.sum(np.abs(T) ** 2))
+                if nrm > 1e-15:
+                    T /= nrm
+    return gp
+
+
+# ════════════════════════════════════════════════════════════════════════════
+#  Charge-resolved SVD and the v3 simple update (finite_peps_gauge_invariant.jl)
+# ════════════════════════════════════════════════════════════════════════════
+
+def charge_svd(Th, row_charge, col_charge, D_max, reg=GI_REG):
+    """finite_peps_gauge_invariant.jl:236-281."""
+    nrow, ncol = Th.shape
+    row_charge = np.asarray(row_charge)
+    col_charge = np.asarray(col_charge)
+    sectors = sorted(set(row_charge.tolist()) & set(col_charge.tolist()))
+    u_short, v_short, rows_of, cols_of, svals, charges = [], [], [], [], [], []
+    for s in sectors:
+        Rs = np.nonzero(row_charge == s)[0]
+        Cs = np.nonzero(col_charge == s)[0]
+        if len(Rs) == 0 or len(Cs) == 0:
+            continue
+        U, S, Vt = np.linalg.svd(Th[np.ix_(Rs, Cs)], full_matrices=False)
+        kmax = min(D_max, len(S))
+        for k in range(kmax):
+            if S[k] <= reg:
+                continue
+            u_short.append(U[:, k]); v_short.append(Vt[k, :])
+            rows_of.append(Rs); cols_of.append(Cs)
+            svals.append(S[k]); charges.append(s)
+    if not svals:
+        raise RuntimeError("charge_svd: no singular values above reg in any sector")
+    svals = np.array(svals)
+    keep = np.argsort(-svals, kind="stable")[:min(D_max, len(svals))]
+    Dk = len(keep)
+    U = np.zeros((nrow, Dk), np.complex128)
+    Vt = np.zeros((Dk, ncol), np.complex128)
+    S = np.zeros(Dk)
+    bq = np.zeros(Dk, np.int64)
+    for j, idx in enumerate(keep):
+        U[rows_of[idx], j] = u_short[idx]
+        Vt[j, cols_of[idx]] = v_short[idx]
+        S[j] = svals[idx]
+        bq[j] = charges[idx]
+    return U, S, Vt, bq
+
+
+def simple_update_bond_h(gp, ix, iy, G, D_max):
+    """finite_peps_gauge_invariant.jl:296-404."""
+    nxv, nyv, gd = gp.nx, gp.ny, gp.gd
+    apply_mask(gp, ix, iy); apply_mask(gp, ix + 1, iy)
+    T_L, T_R = gp.T(ix, iy), gp.T(ix + 1, iy)
+    dp_L, dp_R = T_L.shape[0], T_R.shape[0]
+    _, dgRL, dgUL = site_dims(ix, iy, nxv, nyv, gd)
+    _, dgRR, dgUR = site_dims(ix + 1, iy, nxv, nyv, gd)
+    lam = gp.LH(ix, iy)
+    bond_dim = len(lam)
+    l_l = gp.LH(ix - 1, iy) if ix > 1 else np.ones(1)
+    l_r = gp.LH(ix + 1, iy) if ix + 1 < nxv else np.ones(1)
+    l_uL = gp.LV(ix, iy) if iy < nyv else np.ones(1)
+    l_dL = gp.LV(ix, iy - 1) if iy > 1 else np.ones(1)
+    l_uR = gp.LV(ix + 1, iy) if iy < nyv else np.ones(1)
+    l_dR = gp.LV(ix + 1, iy - 1) if iy > 1 else np.ones(1)
+    sqb = np.sqrt(lam)
+    sq_l, sq_r = np.sqrt(l_l), np.sqrt(l_r)
+    sq_uL, sq_dL, sq_uR, sq_dR = map(np.sqrt, (l_uL, l_dL, l_uR, l_dR))
+    L_w = T_L * sq_l[None, :, None, None, None] * sqb[None, None, :, None, None] \
+        * sq_uL[None, None, None, :, None] * sq_dL[None, None, None, None, :]
+    R_w = T_R * sqb[None, :, None, None, None] * sq_r[None, None, :, None, None] \
+        * sq_uR[None, None, None, :, None] * sq_dR[None, None, None, None, :]
+    szL, szR = L_w.shape, R_w.shape
+    L_mat = jr(pd(L_w, 1, 2, 4, 5, 3), dp_L * szL[1] * szL[3] * szL[4], bond_dim)
+    R_mat = jr(pd(R_w, 2, 1, 3, 4, 5), bond_dim, dp_R * szR[2] * szR[3] * szR[4])
+    Th = jr(L_mat @ R_mat, dp_L, szL[1], szL[3], szL[4], dp_R, szR[2], szR[3], szR[4])
+    sz = Th.shape
+    Tpp = jr(pd(Th, 5, 1, 2, 3, 4, 6, 7, 8), dp_R * dp_L, sz[1] * sz[2] * sz[3] * sz[5] * sz[6] * sz[7])
+    Tnew = pd(jr(G @ Tpp, dp_R, dp_L, sz[1], sz[2], sz[3], sz[5], sz[6], sz[7]), 2, 3, 4, 5, 1, 6, 7, 8)
+    Tm = jr(Tnew, dp_L * sz[1] * sz[2] * sz[3], dp_R * sz[5] * sz[6] * sz[7]).copy()
+    nrow, ncol = Tm.shape
+    row_charge = np.array([decode_phys(r0 % dp_L + 1, dgRL, dgUL, gd)[1] for r0 in range(nrow)])
+    b0 = boundary_label(gd)
+    q = gp.gch[ix, iy - 1]
+    col_charge = np.zeros(ncol, np.int64)
+    for c0 in range(ncol):
+        pR = c0 % dp_R + 1
+        rest = c0 // dp_R
+        dR0 = (rest // sz[5]) // sz[6]
+        f_dR = gp.qv[ix][iy - 2][dR0] if iy > 1 else b0
+        nfR, eRR, eUR = decode_phys(pR, dgRR, dgUR, gd)
+        col_charge[c0] = eRR + eUR - f_dR - 2 * nfR - 2 * q
+    Tm[row_charge[:, None] != col_charge[None, :]] = 0
+    U, S, Vt, newq = charge_svd(Tm, row_charge, col_charge, D_max)
+    Dk = len(S)
+    lam_new = S / max(np.linalg.norm(S), 1e-15)
+    sqS = np.sqrt(S)
+    L_new = pd(jr(U * sqS[None, :], dp_L, sz[1], sz[2], sz[3], Dk), 1, 2, 5, 3, 4)
+    R_new = pd(jr(sqS[:, None] * Vt, Dk, dp_R, sz[5], sz[6], sz[7]), 2, 1, 3, 4, 5)
+    inv = lambda s: 1.0 / (s + FU_REG)
+    L_new *= inv(sq_l)[None, :, None, None, None] * inv(sq_uL)[None, None, None, :, None] \
+        * inv(sq_dL)[None, None, None, None, :]
+    R_new *= inv(sq_r)[None, None, :, None, None] * inv(sq_uR)[None, None, None, :, None] \
+        * inv(sq_dR)[None, None, None, None, :]
+    gp.setT(ix, iy, L_new); gp.setT(ix + 1, iy, R_new)
+    gp.lh[ix - 1][iy - 1] = lam_new
+    gp.qh[ix - 1][iy - 1] = newq
+    apply_mask(gp, ix, iy); apply_mask(gp, ix + 1, iy)
+
+
+def simple_update_bond_v(gp, ix, iy, G, D_max):
+    """finite_peps_gauge_invariant.jl:412-516."""
+    nxv, nyv, gd = gp.nx, gp.ny, gp.gd
+    apply_mask(gp, ix, iy); apply_mask(gp, ix, iy + 1)
+    T_D, T_U = gp.T(ix, iy), gp.T(ix, iy + 1)
+    dp_D, dp_U = T_D.shape[0], T_U.shape[0]
+    _, dgRD, dgUD = site_dims(ix, iy, nxv, nyv, gd)
+    _, dgRU, dgUU = site_dims(ix, iy + 1, nxv, nyv, gd)
+    lam = gp.LV(ix, iy)
+    bond_dim = len(lam)
+    l_lD = gp.LH(ix - 1, iy) if ix > 1 else np.ones(1)
+    l_rD = gp.LH(ix, iy) if ix < nxv else np.ones(1)
+    l_dD = gp.LV(ix, iy - 1) if iy > 1 else np.ones(1)
+    l_lU = gp.LH(ix - 1, iy + 1) if ix > 1 else np.ones(1)
+    l_rU = gp.LH(ix, iy + 1) if ix < nxv else np.ones(1)
+    l_uU = gp.LV(ix, iy + 1) if iy + 1 < nyv else np.ones(1)
+    sqb = np.sqrt(lam)
+    sq_lD, sq_rD, sq_dD, sq_lU, sq_rU, sq_uU = map(np.sqrt, (l_lD, l_rD, l_dD, l_lU, l_rU, l_uU))
+    D_w = T_D * sq_lD[None, :, None, None, None] * sq_rD[None, None, :, None, None] \
+        * sqb[None, None, None, :, None] * sq_dD[None, None, None, None, :]
+    U_w = T_U * sq_lU[None, :, None, None, None] * sq_rU[None, None, :, None, None] \
+        * sq_uU[None, None, None, :, None] * sqb[None, None, None, None, :]
+    szD, szU = D_w.shape, U_w.shape
+    D_mat = jr(pd(D_w, 1, 2, 3, 5, 4), dp_D * szD[1] * szD[2] * szD[4], bond_dim)
+    U_mat = jr(pd(U_w, 5, 1, 2, 3, 4), bond_dim, dp_U * szU[1] * szU[2] * szU[3])
+    Th = jr(D_mat @ U_mat, dp_D, szD[1], szD[2], szD[4], dp_U, szU[1], szU[2], szU[3])
+    sz = Th.shape
+    Tpp = jr(pd(Th, 5, 1, 2, 3, 4, 6, 7, 8), dp_U * dp_D, sz[1] * sz[2] * sz[3] * sz[5] * sz[6] * sz[7])
+    Tnew = pd(jr(G @ Tpp, dp_U, dp_D, sz[1], sz[2], sz[3], sz[5], sz[6], sz[7]), 2, 3, 4, 5, 1, 6, 7, 8)
+    Tm = jr(Tnew, dp_D * sz[1] * sz[2] * sz[3], dp_U * sz[5] * sz[6] * sz[7]).copy()
+    nrow, ncol = Tm.shape
+    row_charge = np.array([decode_phys(r0 % dp_D + 1, dgRD, dgUD, gd)[2] for r0 in range(nrow)])
+    b0 = boundary_label(gd)
+    q = gp.gch[ix - 1, iy]
+    col_charge = np.zeros(ncol, np.int64)
+    for c0 in range(ncol):
+        pU = c0 % dp_U + 1
+        rest = c0 // dp_U
+        lU0 = rest % sz[5]
+        f_lU = gp.qh[ix - 2][iy][lU0] if ix > 1 else b0
+        nfU, eRU, eUU = decode_phys(pU, dgRU, dgUU, gd)
+        col_charge[c0] = eRU + eUU - f_lU - 2 * nfU - 2 * q
+    Tm[row_charge[:, None] != col_charge[None, :]] = 0
+    U, S, Vt, newq = charge_svd(Tm, row_charge, col_charge, D_max)
+    Dk = len(S)
+    lam_new = S / max(np.linalg.norm(S), 1e-15)
+    sqS = np.sqrt(S)
+    D_new = pd(jr(U * sqS[None, :], dp_D, sz[1], sz[2], sz[3], Dk), 1, 2, 3, 5, 4)
+    U_new = pd(jr(sqS[:, None] * Vt, Dk, dp_U, sz[5], sz[6], sz[7]), 2, 3, 4, 5, 1)
+    inv = lambda s: 1.0 / (s + FU_REG)
+    D_new *= inv(sq_lD)[None, :, None, None, None] * inv(sq_rD)[None, None, :, None, None] \
+        * inv(sq_dD)[None, None, None, None, :]
+    U_new *= inv(sq_lU)[None, :, None, None, None] * inv(sq_rU)[None, None, :, None, None] \
+        * inv(sq_uU)[None, None, None, :, None]
+    gp.setT(ix, iy, D_new); gp.setT(ix, iy + 1, U_new)
+    gp.lv[ix - 1][iy - 1] = lam_new
+    gp.qv[ix - 1][iy - 1] = newq
+    apply_mask(gp, ix, iy); apply_mask(gp, ix, iy + 1)
+
+
+def trotter_step_gauge(gp, g, t_hop, m, tau, D_max):
+    """finite_peps_gauge_invariant.jl:523-539."""
+    nxv, nyv, gd = gp.nx, gp.ny, gp.gd
+    for iy in range(1, nyv + 1):
+        for ix in range(1, nxv):
+            simple_update_bond_h(gp, ix, iy, gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2), D_max)
+    for iy in range(1, nyv):
+        for ix in range(1, nxv + 1):
+            simple_update_bond_v(gp, ix, iy, gate(H_merged_v_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau), D_max)
+    for iy in range(1, nyv + 1):
+        for ix in range(nxv - 1, 0, -1):
+            simple_update_bond_h(gp, ix, iy, gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2), D_max)
+
+
+# ════════════════════════════════════════════════════════════════════════════
+#  Boundary MPS (finite_peps_boundary_mps.jl)
+# ════════════════════════════════════════════════════════════════════════════
+
+def sqrt_absorbed_site(gp, ix, iy):
+    """finite_peps_boundary_mps.jl:41-53."""
+    T = gp.T(ix, iy)
+    sl, sr = np.sqrt(_wt_l(gp, ix, iy)), np.sqrt(_wt_r(gp, ix, iy))
+    su, sd = np.sqrt(_wt_u(gp, ix, iy)), np.sqrt(_wt_d(gp, ix, iy))
+    return T * sl[None, :, None, None, None] * sr[None, None, :, None, None] \
+        * su[None, None, None, :, None] * sd[None, None, None, None, :]
+
+
+def dl_traced(T):
+    """_dl_traced, finite_peps_full_update.jl:79-91 (bra is the fast sub-index)."""
+    dp, Dl, Dr, Du, Dd = T.shape
+    E8 = np.einsum("pabcd,pefgh->eafbgchd", T, T.conj(), optimize=True)
+    return jr(E8, Dl * Dl, Dr * Dr, Du * Du, Dd * Dd)
+
+
+def dl_traced_site(gp, ix, iy):
+    """finite_peps_boundary_mps.jl:56-57."""
+    return dl_traced(sqrt_absorbed_site(gp, ix, iy))
+
+
+def compress_mps(mps, chi, cutoff=CMPS_CUT):
+    """finite_peps_boundary_mps.jl:70-105 (in place, returns the list)."""
+    n = len(mps)
+    if n == 1:
+        return mps
+    for i in range(n - 1):
+        Dl, dp, Dr = mps[i].shape
+        Q, R = np.linalg.qr(jr(mps[i], Dl * dp, Dr))
+        k = Q.shape[1]
+        mps[i] = jr(Q, Dl, dp, k)
+        Dl2, dp2, Dr2 = mps[i + 1].shape
+        mps[i + 1] = jr(R @ jr(mps[i + 1], Dl2, dp2 * Dr2), k, dp2, Dr2)
+    for i in range(n - 1, 0, -1):
+        Dl, dp, Dr = mps[i].shape
+        U, S, Vt = np.linalg.svd(jr(mps[i], Dl, dp * Dr), full_matrices=False)
+        k = max(min(chi, int(np.sum(S > cutoff)), len(S)), 1)
+        mps[i] = jr(Vt[:k, :], k, dp, Dr)
+        Dl2, dp2, Dr2 = mps[i - 1].shape
+        mps[i - 1] = jr(jr(mps[i - 1], Dl2 * dp2, Dr2) @ (U[:, :k] * S[None, :k]), Dl2, dp2, k)
+    return mps
+
+
+def row_dl(gp, iy):
+    """finite_peps_boundary_mps.jl:112."""
+    return [dl_traced_site(gp, ix, iy) for ix in range(1, gp.nx + 1)]
+
+
+def row_as_mps_down(row):
+    """finite_peps_boundary_mps.jl:115-119."""
+    return [jr(pd(E, 1, 4, 2, 3), E.shape[0], E.shape[3], E.shape[1] * E.shape[2]) for E in row]
+
+
+def row_as_mps_up(row):
+    """finite_peps_boundary_mps.jl:196-200."""
+    return [jr(pd(E, 1, 3, 2, 4), E.shape[0], E.shape[2], E.shape[1] * E.shape[3]) for E in row]
+
+
+def absorb_row_from_top(mps, row):
+    """finite_peps_boundary_mps.jl:128-148: newM[(a,l),d,(b,r)], l fast."""
+    out = []
+    for M, E in zip(mps, row):
+        a, u, b = M.shape
+        L2, R2, U2, D2 = E.shape
+        assert U2 == u
+        T = np.einsum("aub,lrud->ladrb", M, E, optimize=True)
+        out.append(jr(T, L2 * a, D2, R2 * b))
+    return out
+
+
+def absorb_row_from_bottom(mps, row):
+    """finite_peps_boundary_mps.jl:156-176."""
+    out = []
+    for M, E in zip(mps, row):
+        a, d, b = M.shape
+        L2, R2, U2, D2 = E.shape
+        assert D2 == d
+        T = np.einsum("adb,lrud->laurb", M, E, optimize=True)
+        out.append(jr(T, L2 * a, U2, R2 * b))
+    return out
+
+
+def _trivial_mps(n):
+    return [np.ones((1, 1, 1), np.complex128) for _ in range(n)]
+
+
+def boundary_top(gp, stop_row, chi):
+    """finite_peps_boundary_mps.jl:183-193."""
+    if stop_row >= gp.ny:
+        return _trivial_mps(gp.nx)
+    mps = row_as_mps_down(row_dl(gp, gp.ny))
+    for r in range(gp.ny - 1, stop_row, -1):
+        mps = absorb_row_from_top(mps, row_dl(gp, r))
+        compress_mps(mps, chi)
+    return mps
+
+
+def boundary_bottom(gp, stop_row, chi):
+    """finite_peps_boundary_mps.jl:207-216."""
+    if stop_row <= 1:
+        return _trivial_mps(gp.nx)
+    mps = row_as_mps_up(row_dl(gp, 1))
+    for r in range(2, stop_row):
+        mps = absorb_row_from_bottom(mps, row_dl(gp, r))
+        compress_mps(mps, chi)
+    return mps
+
+
+def peps_norm2(gp, chi=64):
+    """finite_peps_boundary_mps.jl:228-245."""
+    mps = row_as_mps_down(row_dl(gp, gp.ny))
+    for r in range(gp.ny - 1, 0, -1):
+        mps = absorb_row_from_top(mps, row_dl(gp, r))
+        compress_mps(mps, chi)
+    acc = np.ones((1, 1), np.complex128)
+    for M in mps:
+        a, p, b = M.shape
+        assert p == 1
+        acc = acc @ jr(M, a, b)
+    return acc[0, 0]
+
+
+# ── bond environments ────────────────────────────────────────────────────────
+
+def split_dl(E, Dl, Dr, Du, Dd):
+    """finite_peps_boundary_mps.jl:308-310 (bra, ket) pairs."""
+    return jr(E, Dl, Dl, Dr, Dr, Du, Du, Dd, Dd)
+
+
+def split_mps(M, DX):
+    """finite_peps_boundary_mps.jl:313-314 (vL, x_bra, x_ket, vR)."""
+    return jr(M, M.shape[0], DX, DX, M.shape[2])
+
+
+def _qr(M):
+    Q, R = np.linalg.qr(M)
+    return Q, R
+
+
+def qr_reduce_h_left(A):
+    """finite_peps_boundary_mps.jl:325-334."""
+    dp, Dl, Db, Du, Dd = A.shape
+    Q, R = _qr(jr(pd(A, 2, 4, 5, 1, 3), Dl * Du * Dd, dp * Db))
+    a = Q.shape[1]
+    return jr(Q, Dl, Du, Dd, a), jr(R, a, dp, Db)
+
+
+def qr_reduce_h_right(B):
+    """finite_peps_boundary_mps.jl:342-351."""
+    dp, Db, Dr, Du, Dd = B.shape
+    Q, R = _qr(jr(pd(B, 3, 4, 5, 1, 2), Dr * Du * Dd, dp * Db))
+    b = Q.shape[1]
+    return jr(Q, Dr, Du, Dd, b), jr(R, b, dp, Db)
+
+
+def qr_reduce_v_down(A):
+    """finite_peps_boundary_mps.jl:448-453."""
+    dp, Dl, Dr, Db, Dd = A.shape
+    Q, R = _qr(jr(pd(A, 2, 3, 5, 1, 4), Dl * Dr * Dd, dp * Db))
+    a = Q.shape[1]
+    return jr(Q, Dl, Dr, Dd, a), jr(R, a, dp, Db)
+
+
+def qr_reduce_v_up(B):
+    """finite_peps_boundary_mps.jl:457-462."""
+    dp, Dl, Dr, Du, Db = B.shape
+    Q, R = _qr(jr(pd(B, 2, 3, 4, 1, 5), Dl * Dr * Du, dp * Db))
+    b = Q.shape[1]
+    return jr(Q, Dl, Dr, Du, b), jr(R, b, dp, Db)
+
+
+def _isqrt(n):
+    r = int(round(np.sqrt(n)))
+    assert r * r == n
+    return r
+
+
+def _dims4(E):
+    return tuple(_isqrt(s) for s in E.shape)
+
+
+def spectator_left(Lenv, topj, botj, Ej):
+    """finite_peps_boundary_mps.jl:357-365."""
+    Dl, Dr, Du, Dd = _dims4(Ej)
+    E8 = split_dl(Ej, Dl, Dr, Du, Dd)
+    tp = split_mps(topj, Du)
+    bt = split_mps(botj, Dd)
+    return np.einsum("ABkb,AvuC,BedF,bkxyvued->CFyx", Lenv, tp, bt, E8, optimize=True)
+
+
+def spectator_right(Renv, topj, botj, Ej):
+    """finite_peps_boundary_mps.jl:369-377."""
+    Dl, Dr, Du, Dd = _dims4(Ej)
+    E8 = split_dl(Ej, Dl, Dr, Du, Dd)
+    tp = split_mps(topj, Du)
+    bt = split_mps(botj, Dd)
+    return np.einsum("CFyx,AvuC,BedF,bkxyvued->ABkb", Renv, tp, bt, E8, optimize=True)
+
+
+def bond_environment_h(gp, ix, iy, chi=64, top=None, bot=None):
+    """finite_peps_boundary_mps.jl:389-436."""
+    nxv = gp.nx
+    if top is None:
+        top = boundary_top(gp, iy, chi)
+    if bot is None:
+        bot = boundary_bottom(gp, iy, chi)
+    row = row_dl(gp, iy)
+    A = sqrt_absorbed_site(gp, ix, iy)
+    B = sqrt_absorbed_site(gp, ix + 1, iy)
+    QA, RA = qr_reduce_h_left(A)
+    QB, RB = qr_reduce_h_right(B)
+    DuA, DdA = A.shape[3], A.shape[4]
+    DuB, DdB = B.shape[3], B.shape[4]
+    Lenv = np.ones((1, 1, 1, 1), np.complex128)
+    for j in range(1, ix):
+        Lenv = spectator_left(Lenv, top[j - 1], bot[j - 1], row[j - 1])
+    Renv = np.ones((1, 1, 1, 1), np.complex128)
+    for j in range(nxv, ix + 1, -1):
+        Renv = spectator_right(Renv, top[j - 1], bot[j - 1], row[j - 1])
+    tpA, btA = split_mps(top[ix - 1], DuA), split_mps(bot[ix - 1], DdA)
+    GA = np.einsum("ABkb,AvuC,BedF,kuda,bvec->acCF", Lenv, tpA, btA, QA, QA.conj(), optimize=True)
+    tpB, btB = split_mps(top[ix], DuB), split_mps(bot[ix], DdB)
+    GB = np.einsum("CFkb,AvuC,BedF,kuda,bvec->acAB", Renv, tpB, btB, QB, QB.conj(), optimize=True)
+    env = np.einsum("acTB,bdTB->abcd", GA, GB, optimize=True)
+    return dict(env=env, RA=RA, RB=RB, QA=QA, QB=QB)
+
+
+def spectator_left_v(Lenv, topj, botj, Eiy, Eiy1):
+    """finite_peps_boundary_mps.jl:468-482."""
+    Dl, Dr, Du, Dd = _dims4(Eiy)
+    Dl1, Dr1, Du1, Dd1 = _dims4(Eiy1)
+    e0 = split_dl(Eiy, Dl, Dr, Du, Dd)
+    e1 = split_dl(Eiy1, Dl1, Dr1, Du1, Dd1)
+    tp = split_mps(topj, Du1)
+    bt = split_mps(botj, Dd)
+    # Lenv[tL,bL,liyk,liyb,liy1k,liy1b]
+    return np.einsum("ABkbmn,BedF,bkxyvued,nmzwsrvu,AsrC->CFyxwz",
+                     Lenv, bt, e0, e1, tp, optimize=True)
+
+
+def spectator_right_v(Renv, topj, botj, Eiy, Eiy1):
+    """finite_peps_boundary_mps.jl:485-499."""
+    Dl, Dr, Du, Dd = _dims4(Eiy)
+    Dl1, Dr1, Du1, Dd1 = _dims4(Eiy1)
+    e0 = split_dl(Eiy, Dl, Dr, Du, Dd)
+    e1 = split_dl(Eiy1, Dl1, Dr1, Du1, Dd1)
+    tp = split_mps(topj, Du1)
+    bt = split_mps(botj, Dd)
+    return np.einsum("CFyxwz,BedF,bkxyvued,nmzwsrvu,AsrC->ABkbmn",
+                     Renv, bt, e0, e1, tp, optimize=True)
+
+
+def bond_environment_v(gp, ix, iy, chi=64, top=None, bot=None):
+    """finite_peps_boundary_mps.jl:507-547 (same contraction, optimal pairwise order)."""
+    nxv = gp.nx
+    if top is None:
+        top = boundary_top(gp, iy + 1, chi)
+    if bot is None:
+        bot = boundary_bottom(gp, iy, chi)
+    row0, row1 = row_dl(gp, iy), row_dl(gp, iy + 1)
+    A = sqrt_absorbed_site(gp, ix, iy)
+    B = sqrt_absorbed_site(gp, ix, iy + 1)
+    QA, RA = qr_reduce_v_down(A)
+    QB, RB = qr_reduce_v_up(B)
+    DdA, DuB = A.shape[4], B.shape[3]
+    Lenv = np.ones((1,) * 6, np.complex128)
+    for j in range(1, ix):
+        Lenv = spectator_left_v(Lenv, top[j - 1], bot[j - 1], row0[j - 1], row1[j - 1])
+    Renv = np.ones((1,) * 6, np.complex128)
+    for j in range(nxv, ix, -1):
+        Renv = spectator_right_v(Renv, top[j - 1], bot[j - 1], row0[j - 1], row1[j - 1])
+    tp = split_mps(top[ix - 1], DuB)
+    bt = split_mps(bot[ix - 1], DdA)
+    # Lenv[tL,bL,lk,lb,l1k,l1b] tp[tL,ut2,ut1,tR] Renv[tR,bR,rk,rb,r1k,r1b]
+    # bt[bL,d2,d1,bR] QA[lk,rk,d1,aA] QA*[lb,rb,d2,aAp] QB[l1k,r1k,ut1,bB] QB*[l1b,r1b,ut2,bBp]
+    env = np.einsum("ABkbmn,AvuC,CFxyzw,BedF,kxda,byec,mzuf,nwvg->afcg",
+                    Lenv, tp, Renv, bt, QA, QA.conj(), QB, QB.conj(), optimize=True)
+    return dict(env=env, RA=RA, RB=RB, QA=QA, QB=QB)
+
+
+# ════════════════════════════════════════════════════════════════════════════
+#  Full update v4 (finite_peps_fullupdate_v4.jl)
+# ════════════════════════════════════════════════════════════════════════════
+
+def solve_herm(N, S, reg):
+    """_solve_herm, finite_peps_fullupdate_v4.jl:37-45."""
+    Nh = 0.5 * (N + N.conj().T)
+    w, V = np.linalg.eigh(Nh)
+    wmax = max(np.max(w), 0.0) if len(w) else 0.0
+    floor = reg * max(wmax, 1e-30)
+    inv_w = np.where(w > floor, 1.0 / np.where(w > floor, w, 1.0), 0.0)
+    return V @ (inv_w[:, None] * (V.conj().T @ S))
+
+
+def apply_gate(th, G, dpL, dpR):
+    """_apply_gate, finite_peps_fullupdate_v4.jl:48-55."""
+    nA, _, _, nB = th.shape
+    M = jr(pd(th, 3, 2, 1, 4), dpR * dpL, nA * nB)
+    return pd(jr(G @ M, dpR, dpL, nA, nB), 3, 2, 1, 4)
+
+
+def col_charge_from_blocks(M, row_charge):
+    """_col_charge_from_blocks, finite_peps_fullupdate_v4.jl:59-71 (first strict max)."""
+    a2 = np.abs(M) ** 2
+    rbest = np.argmax(a2, axis=0)  # numpy argmax returns the first maximum
+    return np.asarray(row_charge)[rbest]
+
+
+def als_cost(th, env, X, Y):
+    """finite_peps_fullupdate_v4.jl:83-92."""
+    thperm = pd(th, 1, 4, 2, 3).conj()
+    Eth = np.einsum("abcd,apqb->cdpq", env, th, optimize=True)
+    thth = np.real(np.sum(Eth * thperm))
+    XY = np.einsum("apk,bqk->apqb", X, Y, optimize=True)
+    EXY = np.einsum("abcd,apqb->cdpq", env, XY, optimize=True)
+    XYth = np.sum(EXY * thperm)
+    XYXY = np.real(np.sum(EXY * pd(XY, 1, 4, 2, 3).conj()))
+    return thth - 2 * np.real(XYth) + XYXY
+
+
+def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=False):
+    """finite_peps_fullupdate_v4.jl:94-154."""
+    nA, dpL, dpR, nB = th.shape
+    thm = jr(pd(th, 1, 2, 4, 3), nA * dpL, nB * dpR)
+    row_charge = np.array([eRofpL[r // nA] for r in range(nA * dpL)])
+    col_charge = col_charge_from_blocks(thm, row_charge)
+    U, S, Vt, bflux = charge_svd(thm, row_charge, col_charge, D_max)
+    Dp = len(S)
+    sq = np.sqrt(S)
+    X = jr(U * sq[None, :], nA, dpL, Dp).copy()
+    Y = pd(jr(sq[:, None] * Vt, Dp, nB, dpR), 2, 3, 1)
+    Xmask = np.broadcast_to((np.asarray(eRofpL)[:, None] == bflux[None, :])[None, :, :], X.shape)
+    X0, Y0 = X.copy(), Y.copy()
+    cost0 = als_cost(th, env, X0, Y0)
+    for _ in range(n_als):
+        NX = np.einsum("bqk,abcd,dql->akcl", Y, env, Y.conj(), optimize=True)
+        SX = np.einsum("ebcd,epqb,dqk->cpk", env, th, Y.conj(), optimize=True)
+        Nm = jr(NX, nA * Dp, nA * Dp)
+        for pL in range(dpL):
+            s = jr(SX[:, pL, :], nA * Dp, 1)
+            X[:, pL, :] = jr(solve_herm(Nm, s, reg), nA, Dp)
+        X[~Xmask] = 0
+        NY = np.einsum("apk,abcd,cpl->bkdl", X, env, X.conj(), optimize=True)
+        SY = np.einsum("aecb,apqe,cpk->bqk", env, th, X.conj(), optimize=True)
+        Nm = jr(NY, nB * Dp, nB * Dp)
+        for pR in range(dpR):
+            s = jr(SY[:, pR, :], nB * Dp, 1)
+            Y[:, pR, :] = jr(solve_herm(Nm, s, reg), nB, Dp)
+    cost1 = als_cost(th, env, X, Y)
+    accepted = bool(np.isfinite(cost1) and cost1 < cost0)
+    if accepted:
+        out = (X, Y, bflux)
+    else:
+        out = (X0, Y0, bflux)
+    if return_info:
+        return out + (dict(cost0=cost0, cost1=cost1, accepted=accepted),)
+    return out
+
+
+def _isq(gp_lam):
+    """1/(sqrt(lam)+FU_REG) on an existing leg, exactly 1 on a missing one (v4 :249-255)."""
+    if gp_lam is None:
+        return np.ones(1)
+    return 1.0 / (np.sqrt(gp_lam) + FU_REG)
+
+
+def recanonicalise_h(gp, ix, iy, A, B, D_max):
+    """_recanonicalise_h!, finite_peps_fullupdate_v4.jl:215-267."""
+    gd = gp.gd
+    dpL, DlA, _, DuA, DdA = A.shape
+    dpR, _, DrB, DuB, DdB = B.shape
+    _, dgRL, dgUL = site_dims(ix, iy, gp.nx, gp.ny, gd)
+    Th = np.einsum("plkud,qkrvw->pludqrvw", A, B, optimize=True)
+    Tm = jr(Th, dpL * DlA * DuA * DdA, dpR * DrB * DuB * DdB)
+    row_charge = np.array([decode_phys(r0 % dpL + 1, dgRL, dgUL, gd)[1] for r0 in range(Tm.shape[0])])
+    col_charge = col_charge_from_blocks(Tm, row_charge)
+    U, S, Vt, newq = charge_svd(Tm, row_charge, col_charge, D_max)
+    Dk = len(S)
+    lam = S / max(np.linalg.norm(S), 1e-15)
+    Lnew = pd(jr(U, dpL, DlA, DuA, DdA, Dk), 1, 2, 5, 3, 4)
+    Rnew = pd(jr(Vt, Dk, dpR, DrB, DuB, DdB), 2, 1, 3, 4, 5)
+    nx, ny = gp.nx, gp.ny
+    il = _isq(gp.LH(ix - 1, iy) if ix > 1 else None)
+    iuL = _isq(gp.LV(ix, iy) if iy < ny else None)
+    idL = _isq(gp.LV(ix, iy - 1) if iy > 1 else None)
+    ir = _isq(gp.LH(ix + 1, iy) if ix + 1 < nx else None)
+    iuR = _isq(gp.LV(ix + 1, iy) if iy < ny else None)
+    idR = _isq(gp.LV(ix + 1, iy - 1) if iy > 1 else None)
+    Lnew = Lnew * il[None, :, None, None, None] * iuL[None, None, None, :, None] * idL[None, None, None, None, :]
+    Rnew = Rnew * ir[None, None, :, None, None] * iuR[None, None, None, :, None] * idR[None, None, None, None, :]
+    gp.setT(ix, iy, Lnew); gp.setT(ix + 1, iy, Rnew)
+    gp.lh[ix - 1][iy - 1] = lam
+    gp.qh[ix - 1][iy - 1] = newq
+
+
+def recanonicalise_v(gp, ix, iy, Dt, Ut, D_max):
+    """_recanonicalise_v!, finite_peps_fullupdate_v4.jl:318-367."""
+    gd = gp.gd
+    dpD, DlD, DrD, _, DdD = Dt.shape
+    dpU, DlU, DrU, DuU, _ = Ut.shape
+    _, dgRD, dgUD = site_dims(ix, iy, gp.nx, gp.ny, gd)
+    Th = np.einsum("plrkd,qxyzk->plrdqxyz", Dt, Ut, optimize=True)
+    Tm = jr(Th, dpD * DlD * DrD * DdD, dpU * DlU * DrU * DuU)
+    row_charge = np.array([decode_phys(r0 % dpD + 1, dgRD, dgUD, gd)[2] for r0 in range(Tm.shape[0])])
+    col_charge = col_charge_from_blocks(Tm, row_charge)
+    U, S, Vt, newq = charge_svd(Tm, row_charge, col_charge, D_max)
+    Dk = len(S)
+    lam = S / max(np.linalg.norm(S), 1e-15)
+    Dnew = pd(jr(U, dpD, DlD, DrD, DdD, Dk), 1, 2, 3, 5, 4)
+    Unew = pd(jr(Vt, Dk, dpU, DlU, DrU, DuU), 2, 3, 4, 5, 1)
+    nx, ny = gp.nx, gp.ny
+    ilD = _isq(gp.LH(ix - 1, iy) if ix > 1 else None)
+    irD = _isq(gp.LH(ix, iy) if ix < nx else None)
+    idD = _isq(gp.LV(ix, iy - 1) if iy > 1 else None)
+    ilU = _isq(gp.LH(ix - 1, iy + 1) if ix > 1 else None)
+    irU = _isq(gp.LH(ix, iy + 1) if ix < nx else None)
+    iuU = _isq(gp.LV(ix, iy + 1) if iy + 1 < ny else None)
+    Dnew = Dnew * ilD[None, :, None, None, None] * irD[None, None, :, None, None] * idD[None, None, None, None, :]
+    Unew = Unew * ilU[None, :, None, None, None] * irU[None, None, :, None, None] * iuU[None, None, None, :, None]
+    gp.setT(ix, iy, Dnew); gp.setT(ix, iy + 1, Unew)
+    gp.lv[ix - 1][iy - 1] = lam
+    gp.qv[ix - 1][iy - 1] = newq
+
+
+def full_update_bond_h_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=None):
+    """finite_peps_fullupdate_v4.jl:165-193."""
+    gd = gp.gd
+    apply_mask(gp, ix, iy); apply_mask(gp, ix + 1, iy)
+    E = bond_environment_h(gp, ix, iy, chi, top=top, bot=bot)
+    dpL, dpR = E["RA"].shape[1], E["RB"].shape[1]
+    _, dgRL, dgUL = site_dims(ix, iy, gp.nx, gp.ny, gd)
+    th = np.einsum("apk,bqk->apqb", E["RA"], E["RB"], optimize=True)
+    th = apply_gate(th, G, dpL, dpR)
+    eR = [decode_phys(p, dgRL, dgUL, gd)[1] for p in range(1, dpL + 1)]
+    X, Y, bflux = als_truncate(th, E["env"], D_max, eR, n_als=n_als)
+    Afull = np.einsum("luda,apr->plrud", E["QA"], X, optimize=True)
+    Bfull = np.einsum("rudb,bpl->plrud", E["QB"], Y, optimize=True)
+    recanonicalise_h(gp, ix, iy, Afull, Bfull, D_max)
+    apply_mask(gp, ix, iy); apply_mask(gp, ix + 1, iy)
+
+
+def full_update_bond_v_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=None):
+    """finite_peps_fullupdate_v4.jl:274-299."""
+    gd = gp.gd
+    apply_mask(gp, ix, iy); apply_mask(gp, ix, iy + 1)
+    E = bond_environment_v(gp, ix, iy, chi, top=top, bot=bot)
+    dpD, dpU = E["RA"].shape[1], E["RB"].shape[1]
+    _, dgRD, dgUD = site_dims(ix, iy, gp.nx, gp.ny, gd)
+    th = np.einsum("apk,bqk->apqb", E["RA"], E["RB"], optimize=True)
+    th = apply_gate(th, G, dpD, dpU)
+    eU = [decode_phys(p, dgRD, dgUD, gd)[2] for p in range(1, dpD + 1)]
+    X, Y, bflux = als_truncate(th, E["env"], D_max, eU, n_als=n_als)
+    Afull = np.einsum("lrda,apu->plrud", E["QA"], X, optimize=True)
+    Bfull = np.einsum("lrub,bpd->plrud", E["QB"], Y, optimize=True)
+    recanonicalise_v(gp, ix, iy, Afull, Bfull, D_max)
+    apply_mask(gp, ix, iy); apply_mask(gp, ix, iy + 1)
+
+
+def trotter_step_v4(gp, g, t_hop, m, tau, D_max, chi=64, n_als=30):
+    """trotter_step_v4!, finite_peps_fullupdate_v4.jl:415-432."""
+    nxv, nyv, gd = gp.nx, gp.ny, gp.gd
+    for iy in range(1, nyv + 1):
+        for ix in range(1, nxv):
+            G = gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2)
+            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als)
+    for iy in range(1, nyv):
+        for ix in range(1, nxv + 1):
+            G = gate(H_merged_v_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau)
+            full_update_bond_v_v4(gp, ix, iy, G, D_max, chi, n_als)
+    for iy in range(1, nyv + 1):
+        for ix in range(nxv - 1, 0, -1):
+            G = gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2)
+            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als)
+
+
+def bond_expect(E, h, dpL, dpR):
+    """_bond_expect, finite_peps_fullupdate_v4.jl:374-382."""
+    th0 = np.einsum("apk,bqk->apqb", E["RA"], E["RB"], optimize=True)
+    hth = apply_gate(th0, h, dpL, dpR)
+    c = pd(th0, 1, 4, 2, 3).conj()
+    Tn = np.einsum("abcd,apqb->cdpq", E["env"], th0, optimize=True)
+    Th = np.einsum("abcd,apqb->cdpq", E["env"], hth, optimize=True)
+    return np.real(np.sum(Th * c)), np.real(np.sum(Tn * c))
+
+
+def compute_energy_v4(gp, g, t_hop, m, chi=64):
+    """finite_peps_fullupdate_v4.jl:390-409 (total energy)."""
+    nxv, nyv, gd = gp.nx, gp.ny, gp.gd
+    E_tot = 0.0
+    for iy in range(1, nyv + 1):
+        top, bot = boundary_top(gp, iy, chi), boundary_bottom(gp, iy, chi)
+        for ix in range(1, nxv):
+            Eb = bond_environment_h(gp, ix, iy, chi, top=top, bot=bot)
+            h = H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m)
+            hv, nr = bond_expect(Eb, h, Eb["RA"].shape[1], Eb["RB"].shape[1])
+            E_tot += hv / nr
+    for iy in range(1, nyv):
+        top, bot = boundary_top(gp, iy + 1, chi), boundary_bottom(gp, iy, chi)
+        for ix in range(1, nxv + 1):
+            Eb = bond_environment_v(gp, ix, iy, chi, top=top, bot=bot)
+            h = H_merged_v_site(ix, iy, nxv, nyv, gd, g, t_hop, m)
+            hv, nr = bond_expect(Eb, h, Eb["RA"].shape[1], Eb["RB"].shape[1])
+            E_tot += hv / nr
+    return E_tot
+
+
+def mean_nf(gp):
+    """Simple-update-weight <n_f> averaged over sites (finite_peps_quench.jl:134-146)."""
+    vals = []
+    for iy in range(1, gp.ny + 1):
+        for ix in range(1, gp.nx + 1):
+            nf, _, _ = phys_labels(ix, iy, gp.nx, gp.ny, gp.gd)
+            T = gp.T(ix, iy)
+            w = T * _wt_l(gp, ix, iy)[None, :, None, None, None] * _wt_r(gp, ix, iy)[None, None, :, None, None] \
+                * _wt_u(gp, ix, iy)[None, None, None, :, None] * _wt_d(gp, ix, iy)[None, None, None, None, :]
+            rho = np.sum(np.abs(w) ** 2, axis=(1, 2, 3, 4))
+            vals.append(np.sum(rho * nf) / np.sum(rho))
+    return float(np.mean(vals))
+
+
+def bond_entropy(lam):
+    """finite_peps_quench.jl:121-127."""
+    p = np.asarray(lam) ** 2
+    Z = p.sum()
+    if Z < 1e-30:
+        return 0.0
+    p = p / Z
+    return float(-np.sum([x * np.log(x) if x > 1e-30 else 0.0 for x in p]))
+
+
+# ════════════════════════════════════════════════════════════════════════════
+#  Brute-force references (tiny lattices only): dense |psi>, <psi|psi>, <H>
+# ════════════════════════════════════════════════════════════════════════════
+
+def dense_psi(gp):
+    """Contract the sqrt-absorbed network into a dense state psi[p(1,1), p(2,1), ...].
+
+    Physical legs are ordered ix fastest, then iy (column-major over sites).
+    """
+    nx, ny = gp.nx, gp.ny
+    # contract row by row: a row tensor R[p_1..p_nx, u_1..u_nx] (down legs closed)
+    psi = None
+    for iy in range(1, ny + 1):
+        row = None
+        for ix in range(1, nx + 1):
+            A = sqrt_absorbed_site(gp, ix, iy)  # (p,l,r,u,d)
+            if row is None:
+                # (p, r, u, d) with l dim 1
+                row = A[:, 0, :, :, :]
+                row = row.reshape(A.shape[0], A.shape[2], A.shape[3], A.shape[4])
+                shape_p, shape_u, shape_d = [A.shape[0]], [A.shape[3]], [A.shape[4]]
+                row = np.transpose(row, (0, 2, 3, 1))  # (p, u, d, r)
+                row = row.reshape(A.shape[0], A.shape[3], A.shape[4], A.shape[2])
+            else:
+                # row[P, U, Dd, r] * A[p, r, r2, u, d] -> [P,p, U,u, Dd,d, r2]
+                P, U, Dd, r = row.shape
+                t = np.einsum("PUDr,prsud->PpUuDds", row, A, optimize=True)
+                shape_p.append(A.shape[0]); shape_u.append(A.shape[3]); shape_d.append(A.shape[4])
+                row = t.reshape(P * A.shape[0], U * A.shape[3], Dd * A.shape[4], A.shape[2], order="F")
+        assert row.shape[3] == 1
+        row = row[:, :, :, 0]  # (P_row, U_row, D_row)
+        if psi is None:
+            assert row.shape[2] == 1
+            psi = row[:, :, 0]  # (P_all, U)
+        else:
+            # psi[Pall, U] * row[P, U', Drow] with Drow == U
+            psi = np.einsum("Au,Bvu->ABv", psi, row, optimize=True)
+            psi = psi.reshape(psi.shape[0] * psi.shape[1], psi.shape[2], order="F")
+    assert psi.shape[1] == 1
+    return psi[:, 0]
+
+
+def site_phys_dims(gp):
+    return [site_dims(ix, iy, gp.nx, gp.ny, gp.gd)[0] for iy in range(1, gp.ny + 1) for ix in range(1, gp.nx + 1)]
+
+
+def apply_two_site(psi, dims, i, j, O):
+    """Apply O (acting on sites i<j... any order; kron(A_i, B_j) with j fast) to psi."""
+    n = len(dims)
+    T = psi.reshape(dims[::-1])  # C-order reshape of a column-major flat index
+    ai, aj = n - 1 - i, n - 1 - j
+    di, dj = dims[i], dims[j]
+    O4 = O.reshape(di, dj, di, dj)  # [pi', pj', pi, pj] (row-major kron: j fast)
+    out = np.tensordot(O4, T, axes=([2, 3], [ai, aj]))  # (pi', pj', rest...)
+    rest = [k for k in range(n) if k not in (ai, aj)]
+    perm_src = [ai, aj] + rest
+    inv = np.argsort(perm_src)
+    out = np.transpose(out, inv)
+    return out.reshape(-1)
+
+
+def dense_energy(gp, g, t_hop, m):
+    """<psi|H|psi>/<psi|psi> with H = sum of H_merged over all bonds (brute force)."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    psi = dense_psi(gp)
+    dims = site_phys_dims(gp)
+    idx = lambda ix, iy: (iy - 1) * nx + (ix - 1)
+    nrm = np.vdot(psi, psi).real
+    E = 0.0
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx):
+            h = H_merged_h_site(ix, iy, nx, ny, gd, g, t_hop, m)
+            E += np.vdot(psi, apply_two_site(psi, dims, idx(ix, iy), idx(ix + 1, iy), h)).real
+    for iy in range(1, ny):
+        for ix in range(1, nx + 1):
+            h = H_merged_v_site(ix, iy, nx, ny, gd, g, t_hop, m)
+            E += np.vdot(psi, apply_two_site(psi, dims, idx(ix, iy), idx(ix, iy + 1), h)).real
+    return E / nrm
