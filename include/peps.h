@@ -1,0 +1,186 @@
+/*
+ * peps.h — C ABI of libpeps_hip.so, the MI355X finite-PEPS contraction engine.
+ *
+ * Each entry point replaces one function of the Julia reference
+ * (NoaFeldman/LGT); the reference file:line is given beside it.  A Julia
+ * host reaches them through `ccall`; this repository's
+ * tests and bench reach them through Python ctypes.
+ *
+ * Conventions
+ *  - Every call returns an int32 status (PEPS_OK == 0).  On failure the
+ *    context's message is available from peps_last_error().  No exception or
+ *    longjmp crosses the ABI.  Shape @asserts of the reference map to
+ *    PEPS_ESHAPE, the charge-SVD "no sector" error() to PEPS_ENOSECTOR.
+ *  - Arrays are column-major, first index fastest (Julia layout).  Complex
+ *    arrays are interleaved (re, im) doubles: byte-compatible with Julia
+ *    ComplexF64 and C `double _Complex`.  Real arrays are plain doubles.  A
+ *    state is created real or complex; a real state runs the real fast path.
+ *  - Lattice coordinates are 1-based (ix in 1..nx, iy in 1..ny) as in Julia.
+ *  - Host buffers are read or written only during the call; the library
+ *    never retains host pointers.
+ *  - Flux labels are int64 integers 2E.  A link of dimension gd carries
+ *    labels -(gd-1), -(gd-3), ..., gd-1; a missing boundary link carries 0
+ *    for odd gd (the reference's convention) and +1 for even gd.
+ *  - A context is not thread-safe: use one per host thread.
+ */
+#ifndef LIBPEPS_HIP_PEPS_H_
+#define LIBPEPS_HIP_PEPS_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct peps_ctx peps_ctx;
+typedef struct peps_state peps_state;
+typedef struct peps_tensor_list peps_tensor_list;
+
+enum {
+  PEPS_OK = 0,
+  PEPS_EARG = 1,
+  PEPS_ESHAPE = 2,       /* reference @assert on shapes */
+  PEPS_ENOSECTOR = 3,    /* charge_svd: no singular value above reg (gauge_invariant.jl:265) */
+  PEPS_EHIP = 4,
+  PEPS_ENOMEM = 5,
+  PEPS_ESOLVER = 6,      /* factorisation did not converge */
+  PEPS_EUNSUPPORTED = 7
+};
+
+enum { PEPS_TIER_FP64 = 0, PEPS_TIER_FP32 = 1, PEPS_TIER_BF16 = 2 };
+enum { PEPS_H = 0, PEPS_V = 1 };
+enum { PEPS_STEP_V4 = 0, PEPS_STEP_GAUGE = 1 };
+
+typedef struct peps_params {
+  double g;        /* gauge coupling g (the reference takes g, not g^2) */
+  double t_hop;    /* hopping t */
+  double m;        /* staggered mass m */
+  double tau;      /* Trotter step tau */
+  int32_t D_max;   /* maximal PEPS bond dimension */
+  int32_t chi;     /* boundary-MPS bond dimension */
+  int32_t n_als;   /* ALS sweeps (exactly n_als when als_tol == 0) */
+  int32_t pad;
+  double als_tol;  /* > 0: stop ALS when the relative cost change < als_tol */
+} peps_params;
+
+/* ── context ──────────────────────────────────────────────────────────────── */
+int32_t peps_ctx_create(int32_t device, peps_ctx** out);
+int32_t peps_ctx_destroy(peps_ctx* ctx);
+const char* peps_last_error(const peps_ctx* ctx);
+/* number of reference-@warn conditions met (‖Θ‖≈0 in _recanonicalise_*!,
+   finite_peps_fullupdate_v4.jl:236-238,336-338) */
+int64_t peps_warning_count(const peps_ctx* ctx);
+/* contraction precision: FP64 MFMA, FP32 MFMA, BF16 MFMA (FP32 accumulate) */
+int32_t peps_set_tier(peps_ctx* ctx, int32_t tier);
+/* per-kernel-class HIP-event timing on/off, and the collected counters */
+int32_t peps_set_timing(peps_ctx* ctx, int32_t on);
+int32_t peps_stats_count(peps_ctx* ctx, int32_t* n);
+int32_t peps_stats_get(peps_ctx* ctx, int32_t i, char* name, int32_t name_len, double* flops,
+                       double* bytes, double* ms, int64_t* launches);
+int32_t peps_stats_reset(peps_ctx* ctx);
+int32_t peps_sync(peps_ctx* ctx);
+
+/* ── state: GaugePEPS (finite_peps_gauge_invariant.jl:82-88) ─────────────── */
+/* g_charges: nx*ny background charges, ix fastest */
+int32_t peps_state_create(peps_ctx* ctx, int32_t nx, int32_t ny, int32_t gd,
+                          const int64_t* g_charges, int32_t is_complex, peps_state** out);
+int32_t peps_state_destroy(peps_state* st);
+/* tensors[ix,iy] = T (p,l,r,u,d) */
+int32_t peps_state_set_site(peps_state* st, int32_t ix, int32_t iy, const int64_t dims[5],
+                            const double* data);
+int32_t peps_state_site_dims(peps_state* st, int32_t ix, int32_t iy, int64_t dims[5]);
+int32_t peps_state_get_site(peps_state* st, int32_t ix, int32_t iy, double* data);
+/* λh/qh (orient PEPS_H) or λv/qv (PEPS_V) of bond (ix,iy) */
+int32_t peps_state_set_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy, int64_t D,
+                            const double* lambda, const int64_t* flux);
+int32_t peps_state_bond_dim(peps_state* st, int32_t orient, int32_t ix, int32_t iy, int64_t* D);
+int32_t peps_state_get_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
+                            double* lambda, int64_t* flux);
+/* apply_mask! (finite_peps_gauge_invariant.jl:139) */
+int32_t peps_apply_mask(peps_state* st, int32_t ix, int32_t iy);
+/* forbidden_norm (finite_peps_gauge_invariant.jl:154) */
+int32_t peps_forbidden_norm(peps_state* st, double* out);
+/* mean_nf with simple-update weights (finite_peps_ground_state.jl:550-578) */
+int32_t peps_mean_nf(peps_state* st, double* out);
+
+/* ── result lists (boundary MPS, environment tuples, factors) ────────────── */
+int32_t peps_list_len(const peps_tensor_list* l, int32_t* n);
+int32_t peps_list_dims(const peps_tensor_list* l, int32_t i, int32_t* rank, int64_t dims[8],
+                       int32_t* is_complex);
+int32_t peps_list_get(const peps_tensor_list* l, int32_t i, double* data);
+int32_t peps_list_destroy(peps_tensor_list* l);
+
+/* ── kernel-level entry points ────────────────────────────────────────────── */
+/* _dl_traced (finite_peps_full_update.jl:79-91): T (p,l,r,u,d) -> E (l²,r²,u²,d²) */
+int32_t peps_dl_traced(peps_ctx* ctx, int32_t is_complex, const int64_t dims[5], const double* T,
+                       double* E);
+/* absorb_row_from_top / _bottom (finite_peps_boundary_mps.jl:128,156).
+   mps_dims: 3n, row_dims: 4n. */
+int32_t peps_absorb_row(peps_ctx* ctx, int32_t from_top, int32_t is_complex, int32_t n,
+                        const int64_t* mps_dims, const double* const* mps,
+                        const int64_t* row_dims, const double* const* row,
+                        peps_tensor_list** out);
+/* compress_mps! (finite_peps_boundary_mps.jl:70-105) */
+int32_t peps_compress_mps(peps_ctx* ctx, int32_t is_complex, int32_t n, const int64_t* dims,
+                          const double* const* mps, int32_t chi, double cutoff,
+                          peps_tensor_list** out);
+/* boundary_top / boundary_bottom (finite_peps_boundary_mps.jl:183,207) */
+int32_t peps_boundary(peps_state* st, int32_t from_top, int32_t stop_row, int32_t chi,
+                      peps_tensor_list** out);
+/* peps_norm2 (finite_peps_boundary_mps.jl:228): out = (re, im) */
+int32_t peps_norm2(peps_state* st, int32_t chi, double out[2]);
+/* bond_environment_h / _v (finite_peps_boundary_mps.jl:389,507): list = env, RA, RB, QA, QB */
+int32_t peps_bond_env(peps_state* st, int32_t orient, int32_t ix, int32_t iy, int32_t chi,
+                      peps_tensor_list** out);
+/* charge_svd (finite_peps_gauge_invariant.jl:236): list = U, Vt, S; charges[Dk] */
+int32_t peps_charge_svd(peps_ctx* ctx, int32_t is_complex, int64_t nrow, int64_t ncol,
+                        const double* M, const int64_t* row_charge, const int64_t* col_charge,
+                        int32_t D_max, double reg, int64_t* Dk, int64_t* charges,
+                        peps_tensor_list** out);
+/* als_truncate (finite_peps_fullupdate_v4.jl:94): theta (nA,dpL,dpR,nB),
+   env (nA,nB,nA,nB); list = X, Y; bond_flux[Dp]; info = cost0, cost1, accepted */
+int32_t peps_als_truncate(peps_ctx* ctx, int32_t is_complex, const int64_t th_dims[4],
+                          const double* theta, const double* env, int32_t D_max,
+                          const int64_t* eR_of_pL, int32_t n_als, double reg, int64_t* Dp,
+                          int64_t* bond_flux, double info[3], peps_tensor_list** out);
+/* simple_update_bond_h! / _v! (finite_peps_gauge_invariant.jl:296,412);
+   gate: (dL*dR)^2, kron(L,R) ordering (right site fast) */
+int32_t peps_su_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
+                     int32_t D_max);
+/* full_update_bond_h_v4! / _v_v4! (finite_peps_fullupdate_v4.jl:165,274) */
+int32_t peps_fu_bond_v4(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
+                        const double* gate, int32_t D_max, int32_t chi, int32_t n_als);
+/* trotter_step_v4! (finite_peps_fullupdate_v4.jl:415) or trotter_step_gauge!
+   (finite_peps_gauge_invariant.jl:523) */
+int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p);
+/* ALS statistics of the last PEPS_STEP_V4 step: bonds, accepted, total sweeps */
+int32_t peps_last_als_stats(peps_state* st, int64_t out[3]);
+/* compute_energy_v4 (finite_peps_fullupdate_v4.jl:390): total energy */
+int32_t peps_energy_v4(peps_state* st, const peps_params* p, double* E);
+
+/* The contraction GEMM itself (no counterpart in the reference; exposed so
+   callers can validate the MFMA path of the current tier):
+   C (M x N) = op(A) op(B); A stored (K x M) if a_kfast else (M x K);
+   B stored (K x N) if b_kfast else (N x K); conj flags conjugate an operand. */
+int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,
+                  const double* A, int32_t a_kfast, int32_t conjA, const double* B,
+                  int32_t b_kfast, int32_t conjB, double* Cout);
+
+/* Hermitian eigendecomposition used by the ALS solve (_solve_herm,
+   finite_peps_fullupdate_v4.jl:37-45): batched parallel-Jacobi kernel.
+   A: nb contiguous n x n Hermitian matrices; outputs V (same layout) and
+   w (nb*n, unsorted per matrix). */
+int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, const double* A,
+                  double* V, double* w);
+
+/* ── model (u1_lgt_hamiltonian.jl:829-863, :396) ─────────────────────────── */
+/* H_merged_{h,v}_site (tau == 0) or its gate exp(-tau H); pass out == NULL to
+   query n.  out is n*n real, column-major. */
+int32_t peps_bond_operator(int32_t orient, int32_t ix, int32_t iy, int32_t nx, int32_t ny,
+                           int32_t gd, double g, double t_hop, double m, double tau, int32_t* n,
+                           double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIBPEPS_HIP_PEPS_H_ */

@@ -1,0 +1,528 @@
+// C ABI of libpeps_hip.so (declared in include/peps.h).
+#include "peps.h"
+
+#include <cstring>
+
+#include "engine.h"
+
+// States and result lists hold a reference to the context, so device
+// buffers always return to a live allocator whatever order the caller
+// releases handles in.  `ref` is declared first: it is released last.
+struct peps_ctx {
+  explicit peps_ctx(int dev) : c(std::make_shared<peps::Ctx>(dev)) {}
+  std::shared_ptr<peps::Ctx> c;
+};
+
+struct peps_state {
+  std::shared_ptr<peps::Ctx> ref;
+  peps::State s;
+  peps::GateCache gc;
+  int64_t als_bonds = 0, als_accepted = 0, als_sweeps = 0;
+};
+
+struct peps_tensor_list {
+  std::shared_ptr<peps::Ctx> ref;
+  std::vector<peps::Tensor> t;
+};
+
+namespace {
+
+using peps::Tensor;
+
+template <typename F>
+int32_t guard(peps::Ctx* c, F&& f) {
+  try {
+    f();
+    return PEPS_OK;
+  } catch (const peps::Error& e) {
+    if (c) c->last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (c) c->last_error = "host allocation failed";
+    return PEPS_ENOMEM;
+  } catch (const std::exception& e) {
+    if (c) c->last_error = e.what();
+    return PEPS_EARG;
+  }
+}
+
+peps_tensor_list* make_list(const std::shared_ptr<peps::Ctx>& c, std::vector<Tensor> t) {
+  auto* l = new peps_tensor_list;
+  l->ref = c;
+  l->t = std::move(t);
+  return l;
+}
+
+void check_site(peps::State& s, int ix, int iy) {
+  PEPS_CHECK(ix >= 1 && ix <= s.nx && iy >= 1 && iy <= s.ny, peps::PS_EARG, "site out of range");
+}
+
+std::vector<double>& bond_lam(peps::State& s, int orient, int ix, int iy) {
+  if (orient == PEPS_H) {
+    PEPS_CHECK(ix >= 1 && ix < s.nx && iy >= 1 && iy <= s.ny, peps::PS_EARG, "h bond out of range");
+    return s.LH(ix, iy);
+  }
+  PEPS_CHECK(ix >= 1 && ix <= s.nx && iy >= 1 && iy < s.ny, peps::PS_EARG, "v bond out of range");
+  return s.LV(ix, iy);
+}
+std::vector<int64_t>& bond_q(peps::State& s, int orient, int ix, int iy) {
+  bond_lam(s, orient, ix, iy);
+  return orient == PEPS_H ? s.QH(ix, iy) : s.QV(ix, iy);
+}
+
+peps::Params to_params(const peps_params* p) {
+  PEPS_CHECK(p != nullptr, peps::PS_EARG, "null params");
+  PEPS_CHECK(p->D_max >= 1 && p->chi >= 1 && p->n_als >= 0, peps::PS_EARG, "invalid params");
+  peps::Params q;
+  q.g = p->g;
+  q.t = p->t_hop;
+  q.m = p->m;
+  q.tau = p->tau;
+  q.Dmax = p->D_max;
+  q.chi = p->chi;
+  q.n_als = p->n_als;
+  q.als_tol = p->als_tol;
+  return q;
+}
+
+Tensor gate_from_host(peps::State& s, int orient, int ix, int iy, const double* gate) {
+  int jx = orient == PEPS_H ? ix + 1 : ix, jy = orient == PEPS_H ? iy : iy + 1;
+  int64_t n = int64_t(peps::site_dims(ix, iy, s.nx, s.ny, s.gd).d) * peps::site_dims(jx, jy, s.nx, s.ny, s.gd).d;
+  PEPS_CHECK(gate != nullptr, peps::PS_EARG, "null gate");
+  return s.c->upload({n, n}, s.cplx, gate);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t peps_ctx_create(int32_t device, peps_ctx** out) {
+  if (!out) return PEPS_EARG;
+  *out = nullptr;
+  return guard(nullptr, [&] { *out = new peps_ctx(device); });
+}
+
+int32_t peps_ctx_destroy(peps_ctx* ctx) {
+  delete ctx;
+  return PEPS_OK;
+}
+
+const char* peps_last_error(const peps_ctx* ctx) { return ctx ? ctx->c->last_error.c_str() : "null context"; }
+int64_t peps_warning_count(const peps_ctx* ctx) { return ctx ? ctx->c->warnings : 0; }
+
+int32_t peps_set_tier(peps_ctx* ctx, int32_t tier) {
+  if (!ctx || tier < 0 || tier > 2) return PEPS_EARG;
+  ctx->c->tier = peps::Tier(tier);
+  return PEPS_OK;
+}
+
+int32_t peps_set_timing(peps_ctx* ctx, int32_t on) {
+  if (!ctx) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    ctx->c->resolve_timing();
+    ctx->c->timing = on != 0;
+  });
+}
+
+int32_t peps_stats_count(peps_ctx* ctx, int32_t* n) {
+  if (!ctx || !n) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    ctx->c->resolve_timing();
+    *n = int32_t(ctx->c->stats.size());
+  });
+}
+
+int32_t peps_stats_get(peps_ctx* ctx, int32_t i, char* name, int32_t name_len, double* flops,
+                       double* bytes, double* ms, int64_t* launches) {
+  if (!ctx || i < 0 || i >= int32_t(ctx->c->stats.size())) return PEPS_EARG;
+  auto it = ctx->c->stats.begin();
+  std::advance(it, i);
+  if (name && name_len > 0) {
+    std::strncpy(name, it->first.c_str(), size_t(name_len - 1));
+    name[name_len - 1] = 0;
+  }
+  if (flops) *flops = it->second.flops;
+  if (bytes) *bytes = it->second.bytes;
+  if (ms) *ms = it->second.ms;
+  if (launches) *launches = it->second.launches;
+  return PEPS_OK;
+}
+
+int32_t peps_stats_reset(peps_ctx* ctx) {
+  if (!ctx) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    ctx->c->resolve_timing();
+    ctx->c->stats.clear();
+  });
+}
+
+int32_t peps_sync(peps_ctx* ctx) {
+  if (!ctx) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] { ctx->c->sync(); });
+}
+
+// ── state ────────────────────────────────────────────────────────────────────
+
+int32_t peps_state_create(peps_ctx* ctx, int32_t nx, int32_t ny, int32_t gd,
+                          const int64_t* g_charges, int32_t is_complex, peps_state** out) {
+  if (!ctx || !out || !g_charges) return PEPS_EARG;
+  *out = nullptr;
+  return guard(ctx->c.get(), [&] {
+    auto* st = new peps_state;
+    st->ref = ctx->c;
+    try {
+      st->s.init(ctx->c.get(), nx, ny, gd, g_charges, is_complex != 0);
+    } catch (...) {
+      delete st;
+      throw;
+    }
+    *out = st;
+  });
+}
+
+int32_t peps_state_destroy(peps_state* st) {
+  delete st;
+  return PEPS_OK;
+}
+
+int32_t peps_state_set_site(peps_state* st, int32_t ix, int32_t iy, const int64_t dims[5],
+                            const double* data) {
+  if (!st || !dims || !data) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    check_site(st->s, ix, iy);
+    auto sd = peps::site_dims(ix, iy, st->s.nx, st->s.ny, st->s.gd);
+    PEPS_CHECK(dims[0] == sd.d, peps::PS_ESHAPE, "site tensor: physical dimension mismatch");
+    for (int i = 0; i < 5; ++i) PEPS_CHECK(dims[i] >= 1, peps::PS_ESHAPE, "site tensor: empty leg");
+    PEPS_CHECK((ix > 1 || dims[1] == 1) && (ix < st->s.nx || dims[2] == 1) &&
+                   (iy < st->s.ny || dims[3] == 1) && (iy > 1 || dims[4] == 1),
+               peps::PS_ESHAPE, "site tensor: boundary legs must have dimension 1");
+    st->s.t(ix, iy) = st->ref->upload(std::vector<int64_t>(dims, dims + 5), st->s.cplx, data);
+  });
+}
+
+int32_t peps_state_site_dims(peps_state* st, int32_t ix, int32_t iy, int64_t dims[5]) {
+  if (!st || !dims) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    check_site(st->s, ix, iy);
+    auto& t = st->s.t(ix, iy);
+    PEPS_CHECK(!t.empty(), peps::PS_EARG, "site not set");
+    for (int i = 0; i < 5; ++i) dims[i] = t.dims[i];
+  });
+}
+
+int32_t peps_state_get_site(peps_state* st, int32_t ix, int32_t iy, double* data) {
+  if (!st || !data) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    check_site(st->s, ix, iy);
+    auto& t = st->s.t(ix, iy);
+    PEPS_CHECK(!t.empty(), peps::PS_EARG, "site not set");
+    st->ref->download(t, data);
+  });
+}
+
+int32_t peps_state_set_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy, int64_t D,
+                            const double* lambda, const int64_t* flux) {
+  if (!st || !lambda || !flux || D < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    PEPS_CHECK(D <= 64, peps::PS_EUNSUPPORTED, "bond dimension > 64");
+    bond_lam(st->s, orient, ix, iy).assign(lambda, lambda + D);
+    bond_q(st->s, orient, ix, iy).assign(flux, flux + D);
+  });
+}
+
+int32_t peps_state_bond_dim(peps_state* st, int32_t orient, int32_t ix, int32_t iy, int64_t* D) {
+  if (!st || !D) return PEPS_EARG;
+  return guard(st->ref.get(), [&] { *D = int64_t(bond_lam(st->s, orient, ix, iy).size()); });
+}
+
+int32_t peps_state_get_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
+                            double* lambda, int64_t* flux) {
+  if (!st) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    auto& l = bond_lam(st->s, orient, ix, iy);
+    auto& q = bond_q(st->s, orient, ix, iy);
+    if (lambda) std::memcpy(lambda, l.data(), l.size() * sizeof(double));
+    if (flux) std::memcpy(flux, q.data(), q.size() * sizeof(int64_t));
+  });
+}
+
+int32_t peps_apply_mask(peps_state* st, int32_t ix, int32_t iy) {
+  if (!st) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    check_site(st->s, ix, iy);
+    peps::apply_mask(st->s, ix, iy);
+  });
+}
+
+int32_t peps_forbidden_norm(peps_state* st, double* out) {
+  if (!st || !out) return PEPS_EARG;
+  return guard(st->ref.get(), [&] { *out = peps::forbidden_norm(st->s); });
+}
+
+int32_t peps_mean_nf(peps_state* st, double* out) {
+  if (!st || !out) return PEPS_EARG;
+  return guard(st->ref.get(), [&] { *out = peps::mean_nf(st->s); });
+}
+
+// ── lists ────────────────────────────────────────────────────────────────────
+
+int32_t peps_list_len(const peps_tensor_list* l, int32_t* n) {
+  if (!l || !n) return PEPS_EARG;
+  *n = int32_t(l->t.size());
+  return PEPS_OK;
+}
+
+int32_t peps_list_dims(const peps_tensor_list* l, int32_t i, int32_t* rank, int64_t dims[8],
+                       int32_t* is_complex) {
+  if (!l || i < 0 || i >= int32_t(l->t.size()) || !rank || !dims) return PEPS_EARG;
+  const auto& t = l->t[i];
+  if (t.rank() > 8) return PEPS_EUNSUPPORTED;
+  *rank = t.rank();
+  for (int k = 0; k < t.rank(); ++k) dims[k] = t.dims[k];
+  if (is_complex) *is_complex = t.cplx ? 1 : 0;
+  return PEPS_OK;
+}
+
+int32_t peps_list_get(const peps_tensor_list* l, int32_t i, double* data) {
+  if (!l || i < 0 || i >= int32_t(l->t.size()) || !data) return PEPS_EARG;
+  return guard(l->ref.get(), [&] { l->ref->download(l->t[i], data); });
+}
+
+int32_t peps_list_destroy(peps_tensor_list* l) {
+  delete l;
+  return PEPS_OK;
+}
+
+// ── kernel-level ─────────────────────────────────────────────────────────────
+
+int32_t peps_dl_traced(peps_ctx* ctx, int32_t is_complex, const int64_t dims[5], const double* T,
+                       double* E) {
+  if (!ctx || !dims || !T || !E) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    Tensor t = ctx->c->upload(std::vector<int64_t>(dims, dims + 5), is_complex != 0, T);
+    ctx->c->download(peps::dl_traced(*ctx->c, t), E);
+  });
+}
+
+static peps::MPS upload_list(peps::Ctx& c, bool cplx, int n, int rank, const int64_t* dims,
+                             const double* const* data) {
+  peps::MPS m;
+  for (int i = 0; i < n; ++i) {
+    PEPS_CHECK(data[i] != nullptr, peps::PS_EARG, "null tensor pointer");
+    m.push_back(c.upload(std::vector<int64_t>(dims + rank * i, dims + rank * (i + 1)), cplx, data[i]));
+  }
+  return m;
+}
+
+int32_t peps_absorb_row(peps_ctx* ctx, int32_t from_top, int32_t is_complex, int32_t n,
+                        const int64_t* mps_dims, const double* const* mps,
+                        const int64_t* row_dims, const double* const* row,
+                        peps_tensor_list** out) {
+  if (!ctx || !mps_dims || !mps || !row_dims || !row || !out || n < 1) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    auto M = upload_list(*ctx->c, is_complex != 0, n, 3, mps_dims, mps);
+    auto R = upload_list(*ctx->c, is_complex != 0, n, 4, row_dims, row);
+    auto o = from_top ? peps::absorb_row_from_top(*ctx->c, M, R) : peps::absorb_row_from_bottom(*ctx->c, M, R);
+    *out = make_list(ctx->c, o);
+  });
+}
+
+int32_t peps_compress_mps(peps_ctx* ctx, int32_t is_complex, int32_t n, const int64_t* dims,
+                          const double* const* mps, int32_t chi, double cutoff,
+                          peps_tensor_list** out) {
+  if (!ctx || !dims || !mps || !out || n < 1 || chi < 1) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    auto M = upload_list(*ctx->c, is_complex != 0, n, 3, dims, mps);
+    peps::compress_mps(*ctx->c, M, chi, cutoff);
+    *out = make_list(ctx->c, M);
+  });
+}
+
+int32_t peps_boundary(peps_state* st, int32_t from_top, int32_t stop_row, int32_t chi,
+                      peps_tensor_list** out) {
+  if (!st || !out || chi < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    auto m = from_top ? peps::boundary_top(st->s, stop_row, chi) : peps::boundary_bottom(st->s, stop_row, chi);
+    *out = make_list(st->ref, m);
+  });
+}
+
+int32_t peps_norm2(peps_state* st, int32_t chi, double out[2]) {
+  if (!st || !out || chi < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    peps::cplx v = peps::peps_norm2(st->s, chi);
+    out[0] = v.re;
+    out[1] = v.im;
+  });
+}
+
+int32_t peps_bond_env(peps_state* st, int32_t orient, int32_t ix, int32_t iy, int32_t chi,
+                      peps_tensor_list** out) {
+  if (!st || !out || chi < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    bond_lam(st->s, orient, ix, iy);
+    peps::Env E = orient == PEPS_H ? peps::bond_environment_h(st->s, ix, iy, chi)
+                                   : peps::bond_environment_v(st->s, ix, iy, chi);
+    *out = make_list(st->ref, {E.env, E.RA, E.RB, E.QA, E.QB});
+  });
+}
+
+int32_t peps_charge_svd(peps_ctx* ctx, int32_t is_complex, int64_t nrow, int64_t ncol,
+                        const double* M, const int64_t* row_charge, const int64_t* col_charge,
+                        int32_t D_max, double reg, int64_t* Dk, int64_t* charges,
+                        peps_tensor_list** out) {
+  if (!ctx || !M || !row_charge || !col_charge || !Dk || !charges || !out || D_max < 1)
+    return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    Tensor t = ctx->c->upload({nrow, ncol}, is_complex != 0, M);
+    std::vector<int64_t> rq(row_charge, row_charge + nrow), cq(col_charge, col_charge + ncol);
+    peps::CSVD r = peps::charge_svd(*ctx->c, t, rq, cq, D_max, reg);
+    *Dk = int64_t(r.S.size());
+    std::memcpy(charges, r.q.data(), r.q.size() * sizeof(int64_t));
+    Tensor S = ctx->c->upload({int64_t(r.S.size())}, false, r.S.data());
+    *out = make_list(ctx->c, {r.U, r.Vt, S});
+  });
+}
+
+int32_t peps_als_truncate(peps_ctx* ctx, int32_t is_complex, const int64_t th_dims[4],
+                          const double* theta, const double* env, int32_t D_max,
+                          const int64_t* eR_of_pL, int32_t n_als, double reg, int64_t* Dp,
+                          int64_t* bond_flux, double info[3], peps_tensor_list** out) {
+  if (!ctx || !th_dims || !theta || !env || !eR_of_pL || !Dp || !bond_flux || !out || D_max < 1)
+    return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    bool cp = is_complex != 0;
+    int64_t nA = th_dims[0], nB = th_dims[3];
+    Tensor th = ctx->c->upload(std::vector<int64_t>(th_dims, th_dims + 4), cp, theta);
+    Tensor en = ctx->c->upload({nA, nB, nA, nB}, cp, env);
+    std::vector<int64_t> eR(eR_of_pL, eR_of_pL + th_dims[1]);
+    Tensor X, Y;
+    std::vector<int64_t> bf;
+    peps::ALSInfo inf;
+    peps::als_truncate(*ctx->c, th, en, D_max, eR, n_als, reg, 0.0, &X, &Y, &bf, &inf);
+    *Dp = int64_t(bf.size());
+    std::memcpy(bond_flux, bf.data(), bf.size() * sizeof(int64_t));
+    if (info) {
+      info[0] = inf.cost0;
+      info[1] = inf.cost1;
+      info[2] = inf.accepted ? 1.0 : 0.0;
+    }
+    *out = make_list(ctx->c, {X, Y});
+  });
+}
+
+int32_t peps_su_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
+                     int32_t D_max) {
+  if (!st || D_max < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    bond_lam(st->s, orient, ix, iy);
+    Tensor G = gate_from_host(st->s, orient, ix, iy, gate);
+    if (orient == PEPS_H) peps::su_bond_h(st->s, ix, iy, G, D_max);
+    else peps::su_bond_v(st->s, ix, iy, G, D_max);
+  });
+}
+
+int32_t peps_fu_bond_v4(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
+                        const double* gate, int32_t D_max, int32_t chi, int32_t n_als) {
+  if (!st || D_max < 1 || chi < 1 || n_als < 0) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    bond_lam(st->s, orient, ix, iy);
+    Tensor G = gate_from_host(st->s, orient, ix, iy, gate);
+    if (orient == PEPS_H) {
+      peps::apply_mask(st->s, ix, iy);
+      peps::apply_mask(st->s, ix + 1, iy);
+      peps::Env E = peps::bond_environment_h(st->s, ix, iy, chi);
+      peps::fu_bond_h(st->s, ix, iy, G, D_max, n_als, 0.0, E);
+    } else {
+      peps::apply_mask(st->s, ix, iy);
+      peps::apply_mask(st->s, ix, iy + 1);
+      peps::Env E = peps::bond_environment_v(st->s, ix, iy, chi);
+      peps::fu_bond_v(st->s, ix, iy, G, D_max, n_als, 0.0, E);
+    }
+  });
+}
+
+int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p) {
+  if (!st || !p) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    peps::Params q = to_params(p);
+    for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
+    if (kind == PEPS_STEP_V4) {
+      std::vector<peps::ALSInfo> infos;
+      peps::trotter_step_v4(st->s, q, st->gc, &infos);
+      st->als_bonds = int64_t(infos.size());
+      st->als_accepted = 0;
+      st->als_sweeps = 0;
+      for (auto& i : infos) {
+        st->als_accepted += i.accepted;
+        st->als_sweeps += i.sweeps;
+      }
+    } else if (kind == PEPS_STEP_GAUGE) {
+      peps::trotter_step_gauge(st->s, q, st->gc);
+    } else {
+      throw peps::Error(peps::PS_EARG, "unknown step kind");
+    }
+  });
+}
+
+int32_t peps_last_als_stats(peps_state* st, int64_t out[3]) {
+  if (!st || !out) return PEPS_EARG;
+  out[0] = st->als_bonds;
+  out[1] = st->als_accepted;
+  out[2] = st->als_sweeps;
+  return PEPS_OK;
+}
+
+int32_t peps_energy_v4(peps_state* st, const peps_params* p, double* E) {
+  if (!st || !p || !E) return PEPS_EARG;
+  return guard(st->ref.get(), [&] { *E = peps::energy_v4(st->s, to_params(p)); });
+}
+
+int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,
+                  const double* A, int32_t a_kfast, int32_t conjA, const double* B,
+                  int32_t b_kfast, int32_t conjB, double* Cout) {
+  if (!ctx || !A || !B || !Cout || M < 0 || N < 0 || K < 0) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    bool cp = is_complex != 0;
+    Tensor a = ctx->c->upload({std::max<int64_t>(1, M * K)}, cp, A);
+    Tensor b = ctx->c->upload({std::max<int64_t>(1, K * N)}, cp, B);
+    Tensor c = ctx->c->alloc({std::max<int64_t>(1, M * N)}, cp);
+    peps::gemm(*ctx->c, cp, M, N, K, a.data(), a_kfast != 0, conjA != 0, b.data(), b_kfast != 0,
+               conjB != 0, c.data(), 0.0);
+    ctx->c->download(c.reshaped({M * N}), Cout);
+  });
+}
+
+int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, const double* A,
+                  double* V, double* w) {
+  if (!ctx || !A || !V || !w || n < 1 || nb < 1) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    bool cp = is_complex != 0;
+    int64_t tot = int64_t(n) * n * nb;
+    Tensor a = ctx->c->upload({tot}, cp, A);
+    Tensor v = ctx->c->alloc({tot}, cp);
+    Tensor ww = ctx->c->alloc({int64_t(n) * nb}, false);
+    peps::jacobi_eigh(*ctx->c, a, v, ww, n, nb, 1e-15, 60);
+    ctx->c->download(v, V);
+    ctx->c->download(ww, w);
+  });
+}
+
+int32_t peps_bond_operator(int32_t orient, int32_t ix, int32_t iy, int32_t nx, int32_t ny,
+                           int32_t gd, double g, double t_hop, double m, double tau, int32_t* n,
+                           double* out) {
+  if (!n) return PEPS_EARG;
+  return guard(nullptr, [&] {
+    PEPS_CHECK(nx >= 2 && ny >= 2 && gd >= 2, peps::PS_EARG, "bad lattice");
+    PEPS_CHECK(orient == PEPS_H ? (ix >= 1 && ix < nx && iy >= 1 && iy <= ny)
+                                : (ix >= 1 && ix <= nx && iy >= 1 && iy < ny),
+               peps::PS_EARG, "bond out of range");
+    peps::Mat H = orient == PEPS_H ? peps::H_merged_h_site(ix, iy, nx, ny, gd, g, t_hop, m)
+                                   : peps::H_merged_v_site(ix, iy, nx, ny, gd, g, t_hop, m);
+    if (tau != 0.0) H = peps::expm_sym(H, tau);
+    *n = H.n;
+    if (out) std::memcpy(out, H.a.data(), H.a.size() * sizeof(double));
+  });
+}
+
+}  // extern "C"

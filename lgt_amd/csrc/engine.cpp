@@ -1,0 +1,1195 @@
+// Finite-PEPS contraction engine on the GPU.
+//
+// Function-by-function restatement of the reference hot path
+// (finite_peps_boundary_mps.jl, finite_peps_fullupdate_v4.jl,
+// finite_peps_gauge_invariant.jl) on device tensors.  Every @tensor becomes a
+// chain of pairwise `contract` calls in a cost-minimising order (the same
+// sums, never the reference's memory-bound χ²D¹⁰ vertical frame).
+//
+// Deliberate, math-preserving differences from the reference:
+//  * environments are cached exactly (trotter_step_v4 / energy_v4): a boundary
+//    or strip environment is reused only while none of the tensors or weights
+//    it was built from has changed, so results equal recompute-per-bond;
+//  * charge-resolved SVDs of rank-limited products are taken on thin QR
+//    factors (same non-zero spectrum, same singular subspaces), see
+//    charge_svd_lowrank and su_bond_*;
+//  * the ALS normal matrix is factored once per half-sweep and applied to all
+//    physical right-hand sides (the reference re-runs the identical `eigen`
+//    for every p).
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <set>
+#include <sstream>
+
+namespace peps {
+
+// ════════════════════════════════════════════════════════════════════════════
+//  State
+// ════════════════════════════════════════════════════════════════════════════
+
+void State::init(Ctx* ctx, int nx_, int ny_, int gd_, const int64_t* g, bool cplx_) {
+  PEPS_CHECK(nx_ >= 2 && ny_ >= 2, PS_EARG, "lattice must be at least 2x2");
+  PEPS_CHECK(gd_ >= 2 && gd_ <= 8, PS_EARG, "link dimension gd must be in [2, 8]");
+  c = ctx;
+  nx = nx_;
+  ny = ny_;
+  gd = gd_;
+  cplx = cplx_;
+  gch.assign(g, g + size_t(nx) * ny);
+  T.assign(size_t(nx) * ny, Tensor());
+  lh.assign(size_t(nx - 1) * ny, std::vector<double>{1.0});
+  lv.assign(size_t(nx) * (ny - 1), std::vector<double>{1.0});
+  qh.assign(size_t(nx - 1) * ny, std::vector<int64_t>{boundary_label(gd) == 0 ? 0 : 1});
+  qv.assign(size_t(nx) * (ny - 1), std::vector<int64_t>{boundary_label(gd) == 0 ? 0 : 1});
+}
+
+static std::vector<double> ones1() { return {1.0}; }
+
+// transverse weights of site (ix,iy), ones(1) on missing legs
+static std::vector<double> w_l(State& s, int ix, int iy) { return ix > 1 ? s.LH(ix - 1, iy) : ones1(); }
+static std::vector<double> w_r(State& s, int ix, int iy) { return ix < s.nx ? s.LH(ix, iy) : ones1(); }
+static std::vector<double> w_u(State& s, int ix, int iy) { return iy < s.ny ? s.LV(ix, iy) : ones1(); }
+static std::vector<double> w_d(State& s, int ix, int iy) { return iy > 1 ? s.LV(ix, iy - 1) : ones1(); }
+
+static std::vector<double> vsqrt(const std::vector<double>& w) {
+  std::vector<double> o(w.size());
+  for (size_t i = 0; i < w.size(); ++i) o[i] = std::sqrt(w[i]);
+  return o;
+}
+// 1/(sqrt(w) + FU_REG)
+static std::vector<double> visq(const std::vector<double>& w) {
+  std::vector<double> o(w.size());
+  for (size_t i = 0; i < w.size(); ++i) o[i] = 1.0 / (std::sqrt(w[i]) + FU_REG);
+  return o;
+}
+static std::vector<double> vones(size_t n) { return std::vector<double>(n, 1.0); }
+
+static void flux_legs(State& s, int ix, int iy, std::vector<int64_t>* fl, std::vector<int64_t>* fr,
+                      std::vector<int64_t>* fu, std::vector<int64_t>* fd) {
+  // finite_peps_gauge_invariant.jl:94-97
+  int64_t b0 = boundary_label(s.gd);
+  *fl = ix > 1 ? s.QH(ix - 1, iy) : std::vector<int64_t>{b0};
+  *fr = ix < s.nx ? s.QH(ix, iy) : std::vector<int64_t>{b0};
+  *fu = iy < s.ny ? s.QV(ix, iy) : std::vector<int64_t>{b0};
+  *fd = iy > 1 ? s.QV(ix, iy - 1) : std::vector<int64_t>{b0};
+}
+
+void apply_mask(State& s, int ix, int iy) {
+  // finite_peps_gauge_invariant.jl:109-146
+  std::vector<int64_t> nf, lR, lU, fl, fr, fu, fd;
+  phys_labels(ix, iy, s.nx, s.ny, s.gd, &nf, &lR, &lU);
+  flux_legs(s, ix, iy, &fl, &fr, &fu, &fd);
+  gauss_mask(*s.c, s.t(ix, iy), nf, lR, lU, fl, fr, fu, fd, s.q(ix, iy));
+}
+
+double forbidden_norm(State& s) {
+  // finite_peps_gauge_invariant.jl:154-164
+  double tot = 0.0;
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = 1; ix <= s.nx; ++ix) {
+      std::vector<int64_t> nf, lR, lU, fl, fr, fu, fd;
+      phys_labels(ix, iy, s.nx, s.ny, s.gd, &nf, &lR, &lU);
+      flux_legs(s, ix, iy, &fl, &fr, &fu, &fd);
+      tot += gauss_violation(*s.c, s.t(ix, iy), nf, lR, lU, fl, fr, fu, fd, s.q(ix, iy));
+    }
+  return std::sqrt(tot);
+}
+
+Tensor sqrt_absorbed(State& s, int ix, int iy) {
+  // finite_peps_boundary_mps.jl:41-53
+  Tensor T = copy_tensor(*s.c, s.t(ix, iy));
+  std::vector<double> w[4] = {vsqrt(w_l(s, ix, iy)), vsqrt(w_r(s, ix, iy)), vsqrt(w_u(s, ix, iy)),
+                              vsqrt(w_d(s, ix, iy))};
+  scale_legs(*s.c, T, w);
+  return T;
+}
+
+Tensor dl_traced(Ctx& c, const Tensor& T) {
+  // _dl_traced, finite_peps_full_update.jl:79-91: E[(ket,bra) merged, bra fast]
+  PEPS_CHECK(T.rank() == 5, PS_ESHAPE, "dl_traced: (p,l,r,u,d) tensor expected");
+  int64_t Dl = T.dims[1], Dr = T.dims[2], Du = T.dims[3], Dd = T.dims[4];
+  Tensor E8 = contract(c, T, "pabcd", T, "pefgh", "eafbgchd", false, true);
+  return E8.reshaped({Dl * Dl, Dr * Dr, Du * Du, Dd * Dd});
+}
+
+Tensor gate_tensor(Ctx& c, const Mat& G, bool cplx) {
+  std::vector<double> h;
+  if (cplx) {
+    h.assign(G.a.size() * 2, 0.0);
+    for (size_t i = 0; i < G.a.size(); ++i) h[2 * i] = G.a[i];
+  } else {
+    h = G.a;
+  }
+  return c.upload({G.n, G.n}, cplx, h.data());
+}
+
+Tensor ones_tensor(Ctx& c, int rank, bool cplx) {
+  double h[2] = {1.0, 0.0};
+  return c.upload(std::vector<int64_t>(rank, 1), cplx, h);
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Boundary MPS (finite_peps_boundary_mps.jl:70-245)
+// ════════════════════════════════════════════════════════════════════════════
+
+MPS row_dl(State& s, int iy) {
+  MPS r;
+  for (int ix = 1; ix <= s.nx; ++ix) r.push_back(dl_traced(*s.c, sqrt_absorbed(s, ix, iy)));
+  return r;
+}
+
+MPS row_as_mps_down(Ctx& c, const MPS& row) {
+  // :115-119  reshape(permutedims(E,(1,4,2,3)), L², D², R²·U²); U² == 1 on the top row
+  MPS out;
+  for (auto& E : row)
+    out.push_back(permute(c, E, {0, 3, 1, 2}).reshaped({E.dims[0], E.dims[3], E.dims[1] * E.dims[2]}));
+  return out;
+}
+
+MPS row_as_mps_up(Ctx& c, const MPS& row) {
+  // :196-200  reshape(permutedims(E,(1,3,2,4)), L², U², R²·D²); D² == 1 on the bottom row
+  MPS out;
+  for (auto& E : row)
+    out.push_back(permute(c, E, {0, 2, 1, 3}).reshaped({E.dims[0], E.dims[2], E.dims[1] * E.dims[3]}));
+  return out;
+}
+
+MPS absorb_row_from_top(Ctx& c, const MPS& mps, const MPS& row) {
+  // :128-148  newM[(a,l),d,(b,r)], l fast in the merged index
+  MPS out;
+  for (size_t i = 0; i < mps.size(); ++i) {
+    const Tensor &M = mps[i], &E = row[i];
+    PEPS_CHECK(E.dims[2] == M.dims[1], PS_ESHAPE, "top-absorb phys mismatch at col " + std::to_string(i + 1));
+    Tensor T = contract(c, M, "aub", E, "lrud", "ladrb");
+    out.push_back(T.reshaped({E.dims[0] * M.dims[0], E.dims[3], E.dims[1] * M.dims[2]}));
+  }
+  return out;
+}
+
+MPS absorb_row_from_bottom(Ctx& c, const MPS& mps, const MPS& row) {
+  // :156-176
+  MPS out;
+  for (size_t i = 0; i < mps.size(); ++i) {
+    const Tensor &M = mps[i], &E = row[i];
+    PEPS_CHECK(E.dims[3] == M.dims[1], PS_ESHAPE, "bottom-absorb phys mismatch at col " + std::to_string(i + 1));
+    Tensor T = contract(c, M, "adb", E, "lrud", "laurb");
+    out.push_back(T.reshaped({E.dims[0] * M.dims[0], E.dims[2], E.dims[1] * M.dims[2]}));
+  }
+  return out;
+}
+
+void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff) {
+  // :70-105  left->right QR, then right->left truncated SVD
+  const size_t n = mps.size();
+  if (n <= 1) return;
+  for (size_t i = 0; i + 1 < n; ++i) {
+    int64_t Dl = mps[i].dims[0], dp = mps[i].dims[1], Dr = mps[i].dims[2];
+    QR f = qr_thin(c, mps[i].reshaped({Dl * dp, Dr}));
+    int64_t k = f.Q.dims[1];
+    mps[i] = f.Q.reshaped({Dl, dp, k});
+    int64_t Dl2 = mps[i + 1].dims[0], dp2 = mps[i + 1].dims[1], Dr2 = mps[i + 1].dims[2];
+    mps[i + 1] = matmul(c, f.R, mps[i + 1].reshaped({Dl2, dp2 * Dr2})).reshaped({k, dp2, Dr2});
+  }
+  for (size_t i = n - 1; i >= 1; --i) {
+    int64_t Dl = mps[i].dims[0], dp = mps[i].dims[1], Dr = mps[i].dims[2];
+    SVD f = svd_thin(c, mps[i].reshaped({Dl, dp * Dr}));
+    int64_t cnt = 0;
+    for (double x : f.S) cnt += x > cutoff;
+    int64_t k = std::max<int64_t>(1, std::min<int64_t>({int64_t(chi), cnt, int64_t(f.S.size())}));
+    std::vector<int64_t> rows(k), cols(dp * Dr);
+    std::iota(rows.begin(), rows.end(), 0);
+    std::iota(cols.begin(), cols.end(), 0);
+    mps[i] = gather2d(c, f.Vt, rows, cols).reshaped({k, dp, Dr});
+    Tensor Uk = f.U;
+    Uk.dims = {f.U.dims[0], k};
+    Tensor US = copy_tensor(c, Uk);
+    scale_cols(c, US, std::vector<double>(f.S.begin(), f.S.begin() + k));
+    int64_t Dl2 = mps[i - 1].dims[0], dp2 = mps[i - 1].dims[1], Dr2 = mps[i - 1].dims[2];
+    mps[i - 1] = matmul(c, mps[i - 1].reshaped({Dl2 * dp2, Dr2}), US).reshaped({Dl2, dp2, k});
+  }
+}
+
+MPS trivial_mps(Ctx& c, int n, bool cplx) {
+  MPS m;
+  for (int i = 0; i < n; ++i) m.push_back(ones_tensor(c, 3, cplx));
+  return m;
+}
+
+MPS boundary_top(State& s, int stop_row, int chi) {
+  // :183-193
+  if (stop_row >= s.ny) return trivial_mps(*s.c, s.nx, s.cplx);
+  MPS mps = row_as_mps_down(*s.c, row_dl(s, s.ny));
+  for (int r = s.ny - 1; r > stop_row; --r) {
+    mps = absorb_row_from_top(*s.c, mps, row_dl(s, r));
+    compress_mps(*s.c, mps, chi);
+  }
+  return mps;
+}
+
+MPS boundary_bottom(State& s, int stop_row, int chi) {
+  // :207-216
+  if (stop_row <= 1) return trivial_mps(*s.c, s.nx, s.cplx);
+  MPS mps = row_as_mps_up(*s.c, row_dl(s, 1));
+  for (int r = 2; r < stop_row; ++r) {
+    mps = absorb_row_from_bottom(*s.c, mps, row_dl(s, r));
+    compress_mps(*s.c, mps, chi);
+  }
+  return mps;
+}
+
+cplx peps_norm2(State& s, int chi) {
+  // :228-245
+  Ctx& c = *s.c;
+  MPS mps = row_as_mps_down(*s.c, row_dl(s, s.ny));
+  for (int r = s.ny - 1; r >= 1; --r) {
+    mps = absorb_row_from_top(c, mps, row_dl(s, r));
+    compress_mps(c, mps, chi);
+  }
+  Tensor acc = ones_tensor(c, 2, s.cplx);
+  for (auto& M : mps) {
+    PEPS_CHECK(M.dims[1] == 1, PS_ESHAPE, "norm: bottom phys dim != 1");
+    acc = matmul(c, acc, M.reshaped({M.dims[0], M.dims[2]}));
+  }
+  auto h = c.to_host(acc);
+  return s.cplx ? cplx{h[0], h[1]} : cplx{h[0], 0.0};
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Bond environments (finite_peps_boundary_mps.jl:306-547)
+// ════════════════════════════════════════════════════════════════════════════
+
+static int64_t isqrt64(int64_t n) {
+  int64_t r = int64_t(std::llround(std::sqrt(double(n))));
+  PEPS_CHECK(r * r == n, PS_ESHAPE, "double-layer leg is not a square");
+  return r;
+}
+
+// (Dl²,Dr²,Du²,Dd²) -> (Dl,Dl,Dr,Dr,Du,Du,Dd,Dd) as (bra, ket) pairs (:308-310)
+static Tensor split_dl(const Tensor& E) {
+  int64_t l = isqrt64(E.dims[0]), r = isqrt64(E.dims[1]), u = isqrt64(E.dims[2]), d = isqrt64(E.dims[3]);
+  return E.reshaped({l, l, r, r, u, u, d, d});
+}
+// (vL, X², vR) -> (vL, x_bra, x_ket, vR) (:313-314)
+static Tensor split_mps(const Tensor& M, int64_t DX) {
+  return M.reshaped({M.dims[0], DX, DX, M.dims[2]});
+}
+
+Tensor spectator_left(Ctx& c, const Tensor& Lenv, const Tensor& top, const Tensor& bot, const Tensor& E) {
+  // :357-365  Lnew[tR,bR,rket,rbra]
+  Tensor e8 = split_dl(E);
+  Tensor tp = split_mps(top, e8.dims[4]);
+  Tensor bt = split_mps(bot, e8.dims[6]);
+  Tensor x1 = contract(c, Lenv, "ABkb", tp, "AvuC", "BkbvuC");
+  Tensor x2 = contract(c, x1, "BkbvuC", e8, "bkxyvued", "BCxyed");
+  return contract(c, x2, "BCxyed", bt, "BedF", "CFyx");
+}
+
+Tensor spectator_right(Ctx& c, const Tensor& Renv, const Tensor& top, const Tensor& bot, const Tensor& E) {
+  // :369-377  Rnew[tL,bL,lket,lbra]
+  Tensor e8 = split_dl(E);
+  Tensor tp = split_mps(top, e8.dims[4]);
+  Tensor bt = split_mps(bot, e8.dims[6]);
+  Tensor x1 = contract(c, Renv, "CFyx", tp, "AvuC", "FyxAvu");
+  Tensor x2 = contract(c, x1, "FyxAvu", e8, "bkxyvued", "FAbked");
+  return contract(c, x2, "FAbked", bt, "BedF", "ABkb");
+}
+
+Tensor spectator_left_v(Ctx& c, const Tensor& Lenv, const Tensor& top, const Tensor& bot,
+                        const Tensor& E0, const Tensor& E1) {
+  // :468-482
+  Tensor e0 = split_dl(E0), e1 = split_dl(E1);
+  Tensor tp = split_mps(top, e1.dims[4]);
+  Tensor bt = split_mps(bot, e0.dims[6]);
+  Tensor x1 = contract(c, Lenv, "ABkbmn", bt, "BedF", "AkbmnedF");
+  Tensor x2 = contract(c, x1, "AkbmnedF", e0, "bkxyvued", "AmnFxyvu");
+  Tensor x3 = contract(c, x2, "AmnFxyvu", e1, "nmzwsrvu", "AFxyzwsr");
+  return contract(c, x3, "AFxyzwsr", tp, "AsrC", "CFyxwz");
+}
+
+Tensor spectator_right_v(Ctx& c, const Tensor& Renv, const Tensor& top, const Tensor& bot,
+                         const Tensor& E0, const Tensor& E1) {
+  // :485-499
+  Tensor e0 = split_dl(E0), e1 = split_dl(E1);
+  Tensor tp = split_mps(top, e1.dims[4]);
+  Tensor bt = split_mps(bot, e0.dims[6]);
+  Tensor x1 = contract(c, Renv, "CFyxwz", bt, "BedF", "CyxwzBed");
+  Tensor x2 = contract(c, x1, "CyxwzBed", e0, "bkxyvued", "CwzBbkvu");
+  Tensor x3 = contract(c, x2, "CwzBbkvu", e1, "nmzwsrvu", "CBbknmsr");
+  return contract(c, x3, "CBbknmsr", tp, "AsrC", "ABkbmn");
+}
+
+// QR reductions (:325-351, :448-462)
+static void qr_reduce(Ctx& c, const Tensor& A, const std::vector<int>& perm, int n_outer,
+                      Tensor* Q, Tensor* R) {
+  Tensor Ap = permute(c, A, perm);
+  int64_t mo = 1, mi = 1;
+  for (int i = 0; i < 5; ++i) (i < n_outer ? mo : mi) *= Ap.dims[i];
+  QR f = qr_thin(c, Ap.reshaped({mo, mi}));
+  int64_t a = f.Q.dims[1];
+  *Q = f.Q.reshaped({Ap.dims[0], Ap.dims[1], Ap.dims[2], a});
+  *R = f.R.reshaped({a, Ap.dims[3], Ap.dims[4]});
+}
+
+Env bond_env_h(State& s, int ix, int iy, const MPS& top, const MPS& bot, const Tensor& Lenv,
+               const Tensor& Renv) {
+  // :389-436 with the row environments supplied by the caller
+  Ctx& c = *s.c;
+  Tensor A = sqrt_absorbed(s, ix, iy), B = sqrt_absorbed(s, ix + 1, iy);
+  Env E;
+  qr_reduce(c, A, {1, 3, 4, 0, 2}, 3, &E.QA, &E.RA);  // (l,u,d | p,bond)
+  qr_reduce(c, B, {2, 3, 4, 0, 1}, 3, &E.QB, &E.RB);  // (r,u,d | p,bond)
+  Tensor tpA = split_mps(top[ix - 1], A.dims[3]), btA = split_mps(bot[ix - 1], A.dims[4]);
+  Tensor tpB = split_mps(top[ix], B.dims[3]), btB = split_mps(bot[ix], B.dims[4]);
+  Tensor x1 = contract(c, Lenv, "ABkb", tpA, "AvuC", "BkbvuC");
+  Tensor x2 = contract(c, x1, "BkbvuC", E.QA, "kuda", "BbvCda");
+  Tensor x3 = contract(c, x2, "BbvCda", E.QA, "bvec", "BCdaec", false, true);
+  Tensor GA = contract(c, x3, "BCdaec", btA, "BedF", "acCF");
+  Tensor y1 = contract(c, Renv, "CFkb", tpB, "AvuC", "FkbAvu");
+  Tensor y2 = contract(c, y1, "FkbAvu", E.QB, "kuda", "FbAvda");
+  Tensor y3 = contract(c, y2, "FbAvda", E.QB, "bvec", "FAdaec", false, true);
+  Tensor GB = contract(c, y3, "FAdaec", btB, "BedF", "acAB");
+  E.env = contract(c, GA, "acTB", GB, "bdTB", "abcd");
+  return E;
+}
+
+Env bond_env_v(State& s, int ix, int iy, const MPS& top, const MPS& bot, const Tensor& Lenv,
+               const Tensor& Renv) {
+  // :507-547.  env = Σ Lenv tp Renv bt QA QA* QB QB*, closed as
+  // G1[A,m,n,F,x,a,y,c] (Lenv,bt,QA,QA*) · G2[F,x,y,A,m,f,n,g] (Renv,tp,QB,QB*),
+  // streamed over chunks of the bottom boundary bond F when large.
+  Ctx& c = *s.c;
+  Tensor A = sqrt_absorbed(s, ix, iy), B = sqrt_absorbed(s, ix, iy + 1);
+  Env E;
+  qr_reduce(c, A, {1, 2, 4, 0, 3}, 3, &E.QA, &E.RA);  // (l,r,d | p,bond)
+  qr_reduce(c, B, {1, 2, 3, 0, 4}, 3, &E.QB, &E.RB);  // (l,r,u | p,bond)
+  Tensor tp = split_mps(top[ix - 1], B.dims[3]);
+  Tensor bt = split_mps(bot[ix - 1], A.dims[4]);
+  const int64_t chiB = bt.dims[3];
+  const int64_t a = E.QA.dims[3], b = E.QB.dims[3];
+  // largest per-F intermediate of the two chains, bounded to a few GiB per chunk
+  const double esz = s.cplx ? 16.0 : 8.0;
+  const double A_ = double(Lenv.dims[0]), m_ = double(Lenv.dims[4]), n_ = double(Lenv.dims[5]);
+  const double k_ = double(Lenv.dims[2]), x_ = double(E.QA.dims[1]), e_ = double(bt.dims[1]);
+  const double fB = double(E.QB.dims[1]);
+  const double perF = esz * std::max({A_ * m_ * n_ * x_ * x_ * double(a) * double(a),
+                                      A_ * k_ * k_ * m_ * n_ * e_ * e_,
+                                      x_ * x_ * A_ * m_ * n_ * double(b) * double(b),
+                                      x_ * x_ * fB * fB * A_ * double(tp.dims[1]) * double(tp.dims[2])});
+  const double budget = 4.0 * double(1ull << 30);
+  int64_t fc = std::max<int64_t>(1, std::min<int64_t>(chiB, int64_t(budget / std::max(perF, 1.0))));
+  Tensor Rp = permute(c, Renv, {0, 2, 3, 4, 5, 1});  // (C,x,y,z,w,F)
+  Tensor env;
+  for (int64_t f0 = 0; f0 < chiB; f0 += fc) {
+    int64_t f1 = std::min(chiB, f0 + fc);
+    Tensor btc = bt.slice_last(f0, f1);
+    Tensor Rc = Rp.slice_last(f0, f1);
+    Tensor x1 = contract(c, Lenv, "ABkbmn", btc, "BedF", "AkbmnedF");
+    Tensor x2 = contract(c, x1, "AkbmnedF", E.QA, "kxda", "AbmneFxa");
+    Tensor G1 = contract(c, x2, "AbmneFxa", E.QA, "byec", "AmnFxayc", false, true);
+    Tensor y1 = contract(c, Rc, "CxyzwF", tp, "AvuC", "FxyzwAvu");
+    Tensor y2 = contract(c, y1, "FxyzwAvu", E.QB, "mzuf", "FxywAvmf");
+    Tensor G2 = contract(c, y2, "FxywAvmf", E.QB, "nwvg", "FxyAmfng", false, true);
+    Tensor part = contract(c, G1, "AmnFxayc", G2, "FxyAmfng", "afcg");
+    if (env.empty()) env = part;
+    else axpy_inplace(c, env, part, 1.0);
+  }
+  E.env = env;
+  return E;
+}
+
+Env bond_environment_h(State& s, int ix, int iy, int chi) {
+  Ctx& c = *s.c;
+  MPS top = boundary_top(s, iy, chi), bot = boundary_bottom(s, iy, chi);
+  MPS row = row_dl(s, iy);
+  Tensor L = ones_tensor(c, 4, s.cplx), R = ones_tensor(c, 4, s.cplx);
+  for (int j = 1; j <= ix - 1; ++j) L = spectator_left(c, L, top[j - 1], bot[j - 1], row[j - 1]);
+  for (int j = s.nx; j >= ix + 2; --j) R = spectator_right(c, R, top[j - 1], bot[j - 1], row[j - 1]);
+  return bond_env_h(s, ix, iy, top, bot, L, R);
+}
+
+Env bond_environment_v(State& s, int ix, int iy, int chi) {
+  Ctx& c = *s.c;
+  MPS top = boundary_top(s, iy + 1, chi), bot = boundary_bottom(s, iy, chi);
+  MPS r0 = row_dl(s, iy), r1 = row_dl(s, iy + 1);
+  Tensor L = ones_tensor(c, 6, s.cplx), R = ones_tensor(c, 6, s.cplx);
+  for (int j = 1; j <= ix - 1; ++j) L = spectator_left_v(c, L, top[j - 1], bot[j - 1], r0[j - 1], r1[j - 1]);
+  for (int j = s.nx; j >= ix + 1; --j) R = spectator_right_v(c, R, top[j - 1], bot[j - 1], r0[j - 1], r1[j - 1]);
+  return bond_env_v(s, ix, iy, top, bot, L, R);
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Charge-resolved SVD (finite_peps_gauge_invariant.jl:236-281)
+// ════════════════════════════════════════════════════════════════════════════
+
+namespace {
+struct SectorSVD {
+  int64_t s;
+  std::vector<int64_t> rows, cols;  // global row / column indices
+  Tensor U, Vt;                     // (|rows| x k), (k x |cols|)
+  std::vector<double> S;
+};
+
+CSVD assemble(Ctx& c, int64_t nrow, int64_t ncol, std::vector<SectorSVD>& secs, int Dmax,
+              double reg, bool cplx) {
+  struct Cand {
+    int sec;
+    int k;
+    double sv;
+  };
+  std::vector<Cand> cands;
+  for (int i = 0; i < int(secs.size()); ++i) {
+    int kmax = std::min<int>(Dmax, int(secs[i].S.size()));
+    for (int k = 0; k < kmax; ++k) {
+      if (secs[i].S[k] <= reg) continue;
+      cands.push_back({i, k, secs[i].S[k]});
+    }
+  }
+  PEPS_CHECK(!cands.empty(), PS_ENOSECTOR, "charge_svd: no singular values above reg in any sector");
+  std::vector<size_t> ord(cands.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return cands[x].sv > cands[y].sv; });
+  int Dk = int(std::min<size_t>(size_t(Dmax), ord.size()));
+  CSVD r;
+  r.U = c.zeros({nrow, Dk}, cplx);
+  r.Vt = c.zeros({Dk, ncol}, cplx);
+  r.S.resize(Dk);
+  r.q.resize(Dk);
+  // group kept columns by sector for batched scatters
+  std::map<int, std::pair<std::vector<int64_t>, std::vector<int64_t>>> bysec;
+  for (int j = 0; j < Dk; ++j) {
+    const Cand& cd = cands[ord[j]];
+    r.S[j] = cd.sv;
+    r.q[j] = secs[cd.sec].s;
+    bysec[cd.sec].first.push_back(cd.k);
+    bysec[cd.sec].second.push_back(j);
+  }
+  for (auto& kv : bysec) {
+    SectorSVD& sc = secs[kv.first];
+    scatter_cols(c, r.U, sc.U, sc.rows, kv.second.first, kv.second.second);
+    scatter_rows(c, r.Vt, sc.Vt, kv.second.first, kv.second.second, sc.cols);
+  }
+  return r;
+}
+
+std::vector<int64_t> sectors_of(const std::vector<int64_t>& rq, const std::vector<int64_t>& cq) {
+  std::set<int64_t> a(rq.begin(), rq.end()), b(cq.begin(), cq.end());
+  std::vector<int64_t> out;
+  for (auto x : a)
+    if (b.count(x)) out.push_back(x);
+  return out;
+}
+
+std::vector<int64_t> where_eq(const std::vector<int64_t>& v, int64_t s) {
+  std::vector<int64_t> o;
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i] == s) o.push_back(int64_t(i));
+  return o;
+}
+
+std::vector<int64_t> iota_v(int64_t n) {
+  std::vector<int64_t> v(n);
+  std::iota(v.begin(), v.end(), 0);
+  return v;
+}
+}  // namespace
+
+CSVD charge_svd(Ctx& c, const Tensor& M, const std::vector<int64_t>& rq,
+                const std::vector<int64_t>& cq, int Dmax, double reg) {
+  std::vector<SectorSVD> secs;
+  for (int64_t s : sectors_of(rq, cq)) {
+    SectorSVD sc;
+    sc.s = s;
+    sc.rows = where_eq(rq, s);
+    sc.cols = where_eq(cq, s);
+    if (sc.rows.empty() || sc.cols.empty()) continue;
+    SVD f = svd_thin(c, gather2d(c, M, sc.rows, sc.cols));
+    sc.U = f.U;
+    sc.Vt = f.Vt;
+    sc.S = f.S;
+    secs.push_back(std::move(sc));
+  }
+  return assemble(c, M.dims[0], M.dims[1], secs, Dmax, reg, M.cplx);
+}
+
+// charge_svd of M = Amat (nrow x r) * Bmat (r x ncol): each sector block is a
+// product of thin factors, decomposed as Qa (Ra Rb^T) Qb^T.  The block's
+// non-zero singular triplets are exactly those of the r x r core.
+static CSVD charge_svd_lowrank(Ctx& c, const Tensor& Amat, const Tensor& Bmat,
+                               const std::vector<int64_t>& rq, const std::vector<int64_t>& cq,
+                               int Dmax, double reg) {
+  const int64_t r = Amat.dims[1];
+  Tensor BT = permute(c, Bmat, {1, 0});
+  std::vector<SectorSVD> secs;
+  for (int64_t s : sectors_of(rq, cq)) {
+    SectorSVD sc;
+    sc.s = s;
+    sc.rows = where_eq(rq, s);
+    sc.cols = where_eq(cq, s);
+    if (sc.rows.empty() || sc.cols.empty()) continue;
+    QR qa = qr_thin(c, gather2d(c, Amat, sc.rows, iota_v(r)));
+    QR qb = qr_thin(c, gather2d(c, BT, sc.cols, iota_v(r)));
+    Tensor core = matmul(c, qa.R, qb.R, false, false, false, true);
+    SVD f = svd_thin(c, core);
+    sc.U = matmul(c, qa.Q, f.U);
+    sc.Vt = matmul(c, f.Vt, qb.Q, false, false, false, true);
+    sc.S = f.S;
+    secs.push_back(std::move(sc));
+  }
+  return assemble(c, Amat.dims[0], Bmat.dims[1], secs, Dmax, reg, Amat.cplx);
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  ALS truncation (finite_peps_fullupdate_v4.jl:37-154)
+// ════════════════════════════════════════════════════════════════════════════
+
+Tensor apply_gate(Ctx& c, const Tensor& th, const Tensor& G) {
+  // _apply_gate :48-55: gate rows/cols are (pR fast, pL slow)
+  int64_t dpL = th.dims[1], dpR = th.dims[2];
+  PEPS_CHECK(G.dims[0] == dpL * dpR && G.dims[1] == dpL * dpR, PS_ESHAPE, "apply_gate: gate size");
+  Tensor G4 = G.reshaped({dpR, dpL, dpR, dpL});
+  return contract(c, G4, "qpQP", th, "aPQb", "apqb");
+}
+
+// _solve_herm :37-45 applied to every column of S at once.
+static Tensor solve_herm_multi(Ctx& c, const Tensor& N, const Tensor& S, double reg) {
+  Tensor Nh = copy_tensor(c, N);
+  hermitize(c, Nh);
+  EIG e = eigh(c, Nh);
+  double wmax = 0.0;
+  for (double w : e.w) wmax = std::max(wmax, w);
+  double floor = reg * std::max(wmax, 1e-30);
+  std::vector<double> inv(e.w.size());
+  for (size_t i = 0; i < e.w.size(); ++i) inv[i] = e.w[i] > floor ? 1.0 / e.w[i] : 0.0;
+  Tensor T = matmul(c, e.V, S, true, false, true, false);  // V^H S
+  scale_rows(c, T, inv);
+  return matmul(c, e.V, T);
+}
+
+double als_cost(Ctx& c, const Tensor& th, const Tensor& env, const Tensor& X, const Tensor& Y,
+                const Tensor* Eth) {
+  // :83-92
+  Tensor Et = Eth ? *Eth : contract(c, env, "abcd", th, "apqb", "cdpq");
+  Tensor thp = permute(c, th, {0, 3, 1, 2});
+  double thth = dot(c, Et, thp, true).re;
+  Tensor XY = contract(c, X, "apk", Y, "bqk", "apqb");
+  Tensor EXY = contract(c, env, "abcd", XY, "apqb", "cdpq");
+  cplx xyth = dot(c, EXY, thp, true);
+  Tensor XYp = permute(c, XY, {0, 3, 1, 2});
+  double xyxy = dot(c, EXY, XYp, true).re;
+  return thth - 2.0 * xyth.re + xyxy;
+}
+
+void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,
+                  const std::vector<int64_t>& eR, int n_als, double reg, double tol, Tensor* Xo,
+                  Tensor* Yo, std::vector<int64_t>* bflux, ALSInfo* info, const Tensor* X_init,
+                  const Tensor* Y_init) {
+  // als_truncate :94-154
+  const int64_t nA = th.dims[0], dpL = th.dims[1], dpR = th.dims[2], nB = th.dims[3];
+  Tensor thm = permute(c, th, {0, 1, 3, 2}).reshaped({nA * dpL, nB * dpR});
+  std::vector<int64_t> rq(nA * dpL);
+  for (int64_t r = 0; r < nA * dpL; ++r) rq[r] = eR[r / nA];
+  std::vector<int64_t> am = col_argmax(c, thm), cq(am.size());
+  for (size_t i = 0; i < am.size(); ++i) cq[i] = rq[am[i]];
+  CSVD cs = charge_svd(c, thm, rq, cq, Dmax, GI_REG);
+  const int64_t Dp = int64_t(cs.S.size());
+  std::vector<double> sq(Dp);
+  for (int64_t i = 0; i < Dp; ++i) sq[i] = std::sqrt(cs.S[i]);
+  Tensor X = copy_tensor(c, cs.U);
+  scale_cols(c, X, sq);
+  X = X.reshaped({nA, dpL, Dp});
+  Tensor Yt = copy_tensor(c, cs.Vt);
+  scale_rows(c, Yt, sq);
+  Tensor Y = permute(c, Yt.reshaped({Dp, nB, dpR}), {1, 2, 0});
+  std::vector<int8_t> keep(dpL * Dp);
+  for (int64_t b = 0; b < Dp; ++b)
+    for (int64_t p = 0; p < dpL; ++p) keep[p + b * dpL] = eR[p] == cs.q[b];
+  Tensor X0 = X, Y0 = Y;
+  Tensor Eth = contract(c, env, "abcd", th, "apqb", "cdpq");  // Σ env·θ, shared by SX/SY/cost
+  double cost0 = als_cost(c, th, env, X0, Y0, &Eth);
+  if (X_init && Y_init) {  // warm start (same shapes and flux pattern as the SVD init)
+    PEPS_CHECK(X_init->dims == X.dims && Y_init->dims == Y.dims, PS_ESHAPE, "ALS warm start shape");
+    X = copy_tensor(c, *X_init);
+    Y = copy_tensor(c, *Y_init);
+    mask_x(c, X, keep);
+  } else {
+    X = copy_tensor(c, X0);
+    Y = copy_tensor(c, Y0);
+  }
+  double prev = tol > 0 ? als_cost(c, th, env, X, Y, &Eth) : 0.0;
+  int sweeps = 0;
+  double cost1 = std::nan("");
+  try {
+    for (int it = 0; it < n_als; ++it) {
+      ++sweeps;
+      Tensor t1 = contract(c, env, "abcd", Y, "bqk", "acdqk");
+      Tensor NX = contract(c, t1, "acdqk", Y, "dql", "akcl", false, true);
+      Tensor SX = contract(c, Eth, "cdpq", Y, "dqk", "ckp", false, true);
+      Tensor xs = solve_herm_multi(c, NX.reshaped({nA * Dp, nA * Dp}), SX.reshaped({nA * Dp, dpL}), reg);
+      X = permute(c, xs.reshaped({nA, Dp, dpL}), {0, 2, 1});
+      mask_x(c, X, keep);
+      Tensor t3 = contract(c, env, "abcd", X, "apk", "bcdpk");
+      Tensor NY = contract(c, t3, "bcdpk", X, "cpl", "bkdl", false, true);
+      Tensor SY = contract(c, Eth, "cbpq", X, "cpk", "bkq", false, true);
+      Tensor ys = solve_herm_multi(c, NY.reshaped({nB * Dp, nB * Dp}), SY.reshaped({nB * Dp, dpR}), reg);
+      Y = permute(c, ys.reshaped({nB, Dp, dpR}), {0, 2, 1});
+      if (tol > 0) {
+        double cur = als_cost(c, th, env, X, Y, &Eth);
+        double scale = std::max(std::fabs(prev), 1e-300);
+        bool conv = std::fabs(prev - cur) <= tol * scale;
+        prev = cur;
+        if (conv) break;
+      }
+    }
+    cost1 = als_cost(c, th, env, X, Y, &Eth);
+  } catch (const Error& e) {
+    // a non-finite normal matrix: the refinement is rejected, as the
+    // reference rejects a non-finite cost (:148-153)
+    if (e.code != PS_ESOLVER) throw;
+    cost1 = std::nan("");
+  }
+  bool ok = std::isfinite(cost1) && cost1 < cost0;
+  if (info) {
+    info->cost0 = cost0;
+    info->cost1 = cost1;
+    info->accepted = ok;
+    info->sweeps = sweeps;
+  }
+  *Xo = ok ? X : X0;
+  *Yo = ok ? Y : Y0;
+  *bflux = cs.q;
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Full update v4 (finite_peps_fullupdate_v4.jl:165-367)
+// ════════════════════════════════════════════════════════════════════════════
+
+static std::vector<double> isq_or_one(bool exists, const std::vector<double>& lam) {
+  return exists ? visq(lam) : ones1();
+}
+
+static void recanonicalise_h(State& s, int ix, int iy, const Tensor& A, const Tensor& B, int Dmax) {
+  // _recanonicalise_h! :215-267
+  Ctx& c = *s.c;
+  const int64_t dpL = A.dims[0], DlA = A.dims[1], Dkb = A.dims[2], DuA = A.dims[3], DdA = A.dims[4];
+  const int64_t dpR = B.dims[0], DrB = B.dims[2], DuB = B.dims[3], DdB = B.dims[4];
+  SiteDims sL = site_dims(ix, iy, s.nx, s.ny, s.gd);
+  Tensor Amat = permute(c, A, {0, 1, 3, 4, 2}).reshaped({dpL * DlA * DuA * DdA, Dkb});
+  Tensor Bmat = permute(c, B, {1, 0, 2, 3, 4}).reshaped({Dkb, dpR * DrB * DuB * DdB});
+  Tensor Tm = matmul(c, Amat, Bmat);
+  const int64_t nrow = Tm.dims[0];
+  std::vector<int64_t> rq(nrow);
+  for (int64_t r0 = 0; r0 < nrow; ++r0) {
+    int64_t nf, lR, lU;
+    decode_phys(int(r0 % dpL) + 1, sL.dR, sL.dU, s.gd, &nf, &lR, &lU);
+    rq[r0] = lR;
+  }
+  std::vector<int64_t> am = col_argmax(c, Tm), cq(am.size());
+  for (size_t i = 0; i < am.size(); ++i) cq[i] = rq[am[i]];
+  if (std::sqrt(norm2sq(c, Tm)) < 1e-12) ++c.warnings;
+  CSVD cs = charge_svd_lowrank(c, Amat, Bmat, rq, cq, Dmax, GI_REG);
+  const int64_t Dk = int64_t(cs.S.size());
+  double nS = 0;
+  for (double x : cs.S) nS += x * x;
+  nS = std::max(std::sqrt(nS), 1e-15);
+  std::vector<double> lam(Dk);
+  for (int64_t i = 0; i < Dk; ++i) lam[i] = cs.S[i] / nS;
+  Tensor Lnew = permute(c, cs.U.reshaped({dpL, DlA, DuA, DdA, Dk}), {0, 1, 4, 2, 3});
+  Tensor Rnew = permute(c, cs.Vt.reshaped({Dk, dpR, DrB, DuB, DdB}), {1, 0, 2, 3, 4});
+  std::vector<double> wl[4] = {isq_or_one(ix > 1, ix > 1 ? s.LH(ix - 1, iy) : ones1()), vones(Dk),
+                               isq_or_one(iy < s.ny, iy < s.ny ? s.LV(ix, iy) : ones1()),
+                               isq_or_one(iy > 1, iy > 1 ? s.LV(ix, iy - 1) : ones1())};
+  std::vector<double> wr[4] = {vones(Dk),
+                               isq_or_one(ix + 1 < s.nx, ix + 1 < s.nx ? s.LH(ix + 1, iy) : ones1()),
+                               isq_or_one(iy < s.ny, iy < s.ny ? s.LV(ix + 1, iy) : ones1()),
+                               isq_or_one(iy > 1, iy > 1 ? s.LV(ix + 1, iy - 1) : ones1())};
+  scale_legs(c, Lnew, wl);
+  scale_legs(c, Rnew, wr);
+  s.t(ix, iy) = Lnew;
+  s.t(ix + 1, iy) = Rnew;
+  s.LH(ix, iy) = lam;
+  s.QH(ix, iy) = cs.q;
+}
+
+static void recanonicalise_v(State& s, int ix, int iy, const Tensor& Dt, const Tensor& Ut, int Dmax) {
+  // _recanonicalise_v! :318-367
+  Ctx& c = *s.c;
+  const int64_t dpD = Dt.dims[0], DlD = Dt.dims[1], DrD = Dt.dims[2], Dkb = Dt.dims[3], DdD = Dt.dims[4];
+  const int64_t dpU = Ut.dims[0], DlU = Ut.dims[1], DrU = Ut.dims[2], DuU = Ut.dims[3];
+  SiteDims sD = site_dims(ix, iy, s.nx, s.ny, s.gd);
+  Tensor Amat = permute(c, Dt, {0, 1, 2, 4, 3}).reshaped({dpD * DlD * DrD * DdD, Dkb});
+  Tensor Bmat = permute(c, Ut, {4, 0, 1, 2, 3}).reshaped({Dkb, dpU * DlU * DrU * DuU});
+  Tensor Tm = matmul(c, Amat, Bmat);
+  const int64_t nrow = Tm.dims[0];
+  std::vector<int64_t> rq(nrow);
+  for (int64_t r0 = 0; r0 < nrow; ++r0) {
+    int64_t nf, lR, lU;
+    decode_phys(int(r0 % dpD) + 1, sD.dR, sD.dU, s.gd, &nf, &lR, &lU);
+    rq[r0] = lU;
+  }
+  std::vector<int64_t> am = col_argmax(c, Tm), cq(am.size());
+  for (size_t i = 0; i < am.size(); ++i) cq[i] = rq[am[i]];
+  if (std::sqrt(norm2sq(c, Tm)) < 1e-12) ++c.warnings;
+  CSVD cs = charge_svd_lowrank(c, Amat, Bmat, rq, cq, Dmax, GI_REG);
+  const int64_t Dk = int64_t(cs.S.size());
+  double nS = 0;
+  for (double x : cs.S) nS += x * x;
+  nS = std::max(std::sqrt(nS), 1e-15);
+  std::vector<double> lam(Dk);
+  for (int64_t i = 0; i < Dk; ++i) lam[i] = cs.S[i] / nS;
+  Tensor Dnew = permute(c, cs.U.reshaped({dpD, DlD, DrD, DdD, Dk}), {0, 1, 2, 4, 3});
+  Tensor Unew = permute(c, cs.Vt.reshaped({Dk, dpU, DlU, DrU, DuU}), {1, 2, 3, 4, 0});
+  std::vector<double> wd[4] = {isq_or_one(ix > 1, ix > 1 ? s.LH(ix - 1, iy) : ones1()),
+                               isq_or_one(ix < s.nx, ix < s.nx ? s.LH(ix, iy) : ones1()), vones(Dk),
+                               isq_or_one(iy > 1, iy > 1 ? s.LV(ix, iy - 1) : ones1())};
+  std::vector<double> wu[4] = {isq_or_one(ix > 1, ix > 1 ? s.LH(ix - 1, iy + 1) : ones1()),
+                               isq_or_one(ix < s.nx, ix < s.nx ? s.LH(ix, iy + 1) : ones1()),
+                               isq_or_one(iy + 1 < s.ny, iy + 1 < s.ny ? s.LV(ix, iy + 1) : ones1()),
+                               vones(Dk)};
+  scale_legs(c, Dnew, wd);
+  scale_legs(c, Unew, wu);
+  s.t(ix, iy) = Dnew;
+  s.t(ix, iy + 1) = Unew;
+  s.LV(ix, iy) = lam;
+  s.QV(ix, iy) = cs.q;
+}
+
+void fu_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, double tol,
+               const Env& E, ALSInfo* info) {
+  // full_update_bond_h_v4! :165-193 (environment supplied by the caller,
+  // built after apply_mask on both sites)
+  Ctx& c = *s.c;
+  Tensor th = contract(c, E.RA, "apk", E.RB, "bqk", "apqb");
+  th = apply_gate(c, th, G);
+  std::vector<int64_t> nf, lR, lU;
+  phys_labels(ix, iy, s.nx, s.ny, s.gd, &nf, &lR, &lU);
+  Tensor X, Y;
+  std::vector<int64_t> bflux;
+  als_truncate(c, th, E.env, Dmax, lR, n_als, V4_ALS_REG, tol, &X, &Y, &bflux, info);
+  Tensor Af = contract(c, E.QA, "luda", X, "apr", "plrud");
+  Tensor Bf = contract(c, E.QB, "rudb", Y, "bpl", "plrud");
+  recanonicalise_h(s, ix, iy, Af, Bf, Dmax);
+  apply_mask(s, ix, iy);
+  apply_mask(s, ix + 1, iy);
+}
+
+void fu_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, double tol,
+               const Env& E, ALSInfo* info) {
+  // full_update_bond_v_v4! :274-299
+  Ctx& c = *s.c;
+  Tensor th = contract(c, E.RA, "apk", E.RB, "bqk", "apqb");
+  th = apply_gate(c, th, G);
+  std::vector<int64_t> nf, lR, lU;
+  phys_labels(ix, iy, s.nx, s.ny, s.gd, &nf, &lR, &lU);
+  Tensor X, Y;
+  std::vector<int64_t> bflux;
+  als_truncate(c, th, E.env, Dmax, lU, n_als, V4_ALS_REG, tol, &X, &Y, &bflux, info);
+  Tensor Af = contract(c, E.QA, "lrda", X, "apu", "plrud");
+  Tensor Bf = contract(c, E.QB, "lrub", Y, "bpd", "plrud");
+  recanonicalise_v(s, ix, iy, Af, Bf, Dmax);
+  apply_mask(s, ix, iy);
+  apply_mask(s, ix, iy + 1);
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Simple update v3 (finite_peps_gauge_invariant.jl:296-516)
+// ════════════════════════════════════════════════════════════════════════════
+//
+// Θ = (I⊗Q_L) · C · (I⊗Q_R)^T with the gate-applied core C[(p_L,a),(p_R,b)]
+// from the QR-reduced sites.  Θ is block diagonal in the bond flux, and the
+// sector-s rows of Θ are (I⊗Q_L) times the sector-s rows of C, so the
+// sector's singular values are those of C[rows of s, :]; the left/right
+// singular vectors are mapped back through Q_L / Q_R and the right ones are
+// restricted to the analytically labelled sector columns, as the reference
+// does by zeroing cross-sector entries.
+
+static CSVD su_charge_svd(Ctx& c, const Tensor& core, int64_t dpL, int64_t a,
+                          const std::vector<int64_t>& eR_of_p, const std::vector<int64_t>& rq_full,
+                          const std::vector<int64_t>& cq_full, int Dmax) {
+  const int64_t ncc = core.dims[1];
+  std::vector<SectorSVD> secs;
+  std::vector<int64_t> rq_core(dpL * a);
+  for (int64_t r = 0; r < dpL * a; ++r) rq_core[r] = eR_of_p[r % dpL];
+  for (int64_t s : sectors_of(rq_full, cq_full)) {
+    SectorSVD sc;
+    sc.s = s;
+    sc.rows = where_eq(rq_core, s);
+    sc.cols = iota_v(ncc);
+    if (sc.rows.empty() || where_eq(cq_full, s).empty()) continue;
+    SVD f = svd_thin(c, gather2d(c, core, sc.rows, sc.cols));
+    sc.U = f.U;
+    sc.Vt = f.Vt;
+    sc.S = f.S;
+    secs.push_back(std::move(sc));
+  }
+  return assemble(c, dpL * a, ncc, secs, Dmax, GI_REG, core.cplx);
+}
+
+void su_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax) {
+  // simple_update_bond_h! :296-404
+  Ctx& c = *s.c;
+  apply_mask(s, ix, iy);
+  apply_mask(s, ix + 1, iy);
+  const int nx = s.nx, ny = s.ny;
+  std::vector<double> lam = s.LH(ix, iy);
+  std::vector<double> l_l = ix > 1 ? s.LH(ix - 1, iy) : ones1();
+  std::vector<double> l_r = ix + 1 < nx ? s.LH(ix + 1, iy) : ones1();
+  std::vector<double> l_uL = iy < ny ? s.LV(ix, iy) : ones1();
+  std::vector<double> l_dL = iy > 1 ? s.LV(ix, iy - 1) : ones1();
+  std::vector<double> l_uR = iy < ny ? s.LV(ix + 1, iy) : ones1();
+  std::vector<double> l_dR = iy > 1 ? s.LV(ix + 1, iy - 1) : ones1();
+  Tensor Lw = copy_tensor(c, s.t(ix, iy)), Rw = copy_tensor(c, s.t(ix + 1, iy));
+  std::vector<double> wL[4] = {vsqrt(l_l), vsqrt(lam), vsqrt(l_uL), vsqrt(l_dL)};
+  std::vector<double> wR[4] = {vsqrt(lam), vsqrt(l_r), vsqrt(l_uR), vsqrt(l_dR)};
+  scale_legs(c, Lw, wL);
+  scale_legs(c, Rw, wR);
+  const int64_t dpL = Lw.dims[0], Dl = Lw.dims[1], DuL = Lw.dims[3], DdL = Lw.dims[4];
+  const int64_t dpR = Rw.dims[0], Dr = Rw.dims[2], DuR = Rw.dims[3], DdR = Rw.dims[4];
+  Tensor QL, RL, QR_, RR;
+  qr_reduce(c, Lw, {1, 3, 4, 0, 2}, 3, &QL, &RL);   // QL (l,u,d,a)
+  qr_reduce(c, Rw, {2, 3, 4, 0, 1}, 3, &QR_, &RR);  // QR (r,u,d,b)
+  const int64_t a = QL.dims[3], b = QR_.dims[3];
+  Tensor th = apply_gate(c, contract(c, RL, "apk", RR, "bqk", "apqb"), G);
+  Tensor core = permute(c, th, {1, 0, 2, 3}).reshaped({dpL * a, dpR * b});
+  // sector labels of the full Θ (rows (pL,l,uL,dL), cols (pR,r,uR,dR))
+  SiteDims sL = site_dims(ix, iy, nx, ny, s.gd), sR = site_dims(ix + 1, iy, nx, ny, s.gd);
+  std::vector<int64_t> eRp(dpL);
+  for (int64_t p = 0; p < dpL; ++p) {
+    int64_t nf, lR, lU;
+    decode_phys(int(p) + 1, sL.dR, sL.dU, s.gd, &nf, &lR, &lU);
+    eRp[p] = lR;
+  }
+  const int64_t nrow = dpL * Dl * DuL * DdL, ncol = dpR * Dr * DuR * DdR;
+  std::vector<int64_t> rq(nrow), cq(ncol);
+  for (int64_t r0 = 0; r0 < nrow; ++r0) rq[r0] = eRp[r0 % dpL];
+  const int64_t b0 = boundary_label(s.gd);
+  const int64_t qR = s.q(ix + 1, iy);
+  for (int64_t c0 = 0; c0 < ncol; ++c0) {
+    int64_t pR = c0 % dpR, rest = c0 / dpR;
+    int64_t dR0 = (rest / Dr) / DuR;
+    int64_t f_dR = iy > 1 ? s.QV(ix + 1, iy - 1)[dR0] : b0;
+    int64_t nf, lR, lU;
+    decode_phys(int(pR) + 1, sR.dR, sR.dU, s.gd, &nf, &lR, &lU);
+    cq[c0] = lR + lU - f_dR - 2 * nf - 2 * qR;
+  }
+  CSVD cs = su_charge_svd(c, core, dpL, a, eRp, rq, cq, Dmax);
+  const int64_t Dk = int64_t(cs.S.size());
+  Tensor Ufull = contract(c, cs.U.reshaped({dpL, a, Dk}), "pak", QL, "luda", "pludk");
+  Tensor Vfull = contract(c, cs.Vt.reshaped({Dk, dpR, b}), "kqb", QR_, "rvwb", "kqrvw").reshaped({Dk, ncol});
+  zero_cross_sector(c, Vfull, cs.q, cq);
+  double nS = 0;
+  for (double x : cs.S) nS += x * x;
+  nS = std::max(std::sqrt(nS), 1e-15);
+  std::vector<double> lamn(Dk), sqS(Dk);
+  for (int64_t i = 0; i < Dk; ++i) {
+    lamn[i] = cs.S[i] / nS;
+    sqS[i] = std::sqrt(cs.S[i]);
+  }
+  Tensor U2 = Ufull.reshaped({nrow, Dk});
+  scale_cols(c, U2, sqS);
+  scale_rows(c, Vfull, sqS);
+  Tensor Lnew = permute(c, U2.reshaped({dpL, Dl, DuL, DdL, Dk}), {0, 1, 4, 2, 3});
+  Tensor Rnew = permute(c, Vfull.reshaped({Dk, dpR, Dr, DuR, DdR}), {1, 0, 2, 3, 4});
+  std::vector<double> pL[4] = {visq(l_l), vones(Dk), visq(l_uL), visq(l_dL)};
+  std::vector<double> pR[4] = {vones(Dk), visq(l_r), visq(l_uR), visq(l_dR)};
+  scale_legs(c, Lnew, pL);
+  scale_legs(c, Rnew, pR);
+  s.t(ix, iy) = Lnew;
+  s.t(ix + 1, iy) = Rnew;
+  s.LH(ix, iy) = lamn;
+  s.QH(ix, iy) = cs.q;
+  apply_mask(s, ix, iy);
+  apply_mask(s, ix + 1, iy);
+}
+
+void su_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax) {
+  // simple_update_bond_v! :412-516
+  Ctx& c = *s.c;
+  apply_mask(s, ix, iy);
+  apply_mask(s, ix, iy + 1);
+  const int nx = s.nx, ny = s.ny;
+  std::vector<double> lam = s.LV(ix, iy);
+  std::vector<double> l_lD = ix > 1 ? s.LH(ix - 1, iy) : ones1();
+  std::vector<double> l_rD = ix < nx ? s.LH(ix, iy) : ones1();
+  std::vector<double> l_dD = iy > 1 ? s.LV(ix, iy - 1) : ones1();
+  std::vector<double> l_lU = ix > 1 ? s.LH(ix - 1, iy + 1) : ones1();
+  std::vector<double> l_rU = ix < nx ? s.LH(ix, iy + 1) : ones1();
+  std::vector<double> l_uU = iy + 1 < ny ? s.LV(ix, iy + 1) : ones1();
+  Tensor Dw = copy_tensor(c, s.t(ix, iy)), Uw = copy_tensor(c, s.t(ix, iy + 1));
+  std::vector<double> wD[4] = {vsqrt(l_lD), vsqrt(l_rD), vsqrt(lam), vsqrt(l_dD)};
+  std::vector<double> wU[4] = {vsqrt(l_lU), vsqrt(l_rU), vsqrt(l_uU), vsqrt(lam)};
+  scale_legs(c, Dw, wD);
+  scale_legs(c, Uw, wU);
+  const int64_t dpD = Dw.dims[0], DlD = Dw.dims[1], DrD = Dw.dims[2], DdD = Dw.dims[4];
+  const int64_t dpU = Uw.dims[0], DlU = Uw.dims[1], DrU = Uw.dims[2], DuU = Uw.dims[3];
+  Tensor QD, RD, QU, RU;
+  qr_reduce(c, Dw, {1, 2, 4, 0, 3}, 3, &QD, &RD);  // QD (l,r,d,a)
+  qr_reduce(c, Uw, {1, 2, 3, 0, 4}, 3, &QU, &RU);  // QU (l,r,u,b)
+  const int64_t a = QD.dims[3], b = QU.dims[3];
+  Tensor th = apply_gate(c, contract(c, RD, "apk", RU, "bqk", "apqb"), G);
+  Tensor core = permute(c, th, {1, 0, 2, 3}).reshaped({dpD * a, dpU * b});
+  SiteDims sD = site_dims(ix, iy, nx, ny, s.gd), sU = site_dims(ix, iy + 1, nx, ny, s.gd);
+  std::vector<int64_t> eUp(dpD);
+  for (int64_t p = 0; p < dpD; ++p) {
+    int64_t nf, lR, lU;
+    decode_phys(int(p) + 1, sD.dR, sD.dU, s.gd, &nf, &lR, &lU);
+    eUp[p] = lU;
+  }
+  const int64_t nrow = dpD * DlD * DrD * DdD, ncol = dpU * DlU * DrU * DuU;
+  std::vector<int64_t> rq(nrow), cq(ncol);
+  for (int64_t r0 = 0; r0 < nrow; ++r0) rq[r0] = eUp[r0 % dpD];
+  const int64_t b0 = boundary_label(s.gd);
+  const int64_t qU = s.q(ix, iy + 1);
+  for (int64_t c0 = 0; c0 < ncol; ++c0) {
+    int64_t pU = c0 % dpU, rest = c0 / dpU;
+    int64_t lU0 = rest % DlU;
+    int64_t f_lU = ix > 1 ? s.QH(ix - 1, iy + 1)[lU0] : b0;
+    int64_t nf, lR, lU;
+    decode_phys(int(pU) + 1, sU.dR, sU.dU, s.gd, &nf, &lR, &lU);
+    cq[c0] = lR + lU - f_lU - 2 * nf - 2 * qU;
+  }
+  CSVD cs = su_charge_svd(c, core, dpD, a, eUp, rq, cq, Dmax);
+  const int64_t Dk = int64_t(cs.S.size());
+  Tensor Ufull = contract(c, cs.U.reshaped({dpD, a, Dk}), "pak", QD, "lrda", "plrdk");
+  Tensor Vfull = contract(c, cs.Vt.reshaped({Dk, dpU, b}), "kqb", QU, "xyzb", "kqxyz").reshaped({Dk, ncol});
+  zero_cross_sector(c, Vfull, cs.q, cq);
+  double nS = 0;
+  for (double x : cs.S) nS += x * x;
+  nS = std::max(std::sqrt(nS), 1e-15);
+  std::vector<double> lamn(Dk), sqS(Dk);
+  for (int64_t i = 0; i < Dk; ++i) {
+    lamn[i] = cs.S[i] / nS;
+    sqS[i] = std::sqrt(cs.S[i]);
+  }
+  Tensor U2 = Ufull.reshaped({nrow, Dk});
+  scale_cols(c, U2, sqS);
+  scale_rows(c, Vfull, sqS);
+  Tensor Dnew = permute(c, U2.reshaped({dpD, DlD, DrD, DdD, Dk}), {0, 1, 2, 4, 3});
+  Tensor Unew = permute(c, Vfull.reshaped({Dk, dpU, DlU, DrU, DuU}), {1, 2, 3, 4, 0});
+  std::vector<double> pD[4] = {visq(l_lD), visq(l_rD), vones(Dk), visq(l_dD)};
+  std::vector<double> pU[4] = {visq(l_lU), visq(l_rU), visq(l_uU), vones(Dk)};
+  scale_legs(c, Dnew, pD);
+  scale_legs(c, Unew, pU);
+  s.t(ix, iy) = Dnew;
+  s.t(ix, iy + 1) = Unew;
+  s.LV(ix, iy) = lamn;
+  s.QV(ix, iy) = cs.q;
+  apply_mask(s, ix, iy);
+  apply_mask(s, ix, iy + 1);
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Drivers
+// ════════════════════════════════════════════════════════════════════════════
+
+Tensor GateCache::get(State& s, char orient, int ix, int iy, const Params& p, double tau_factor) {
+  // the gate depends on the bond only through the two sites' dimensions,
+  // on-site weights and stagger signs
+  int jx = orient == 'h' ? ix + 1 : ix, jy = orient == 'h' ? iy : iy + 1;
+  SiteDims a = site_dims(ix, iy, s.nx, s.ny, s.gd), b = site_dims(jx, jy, s.nx, s.ny, s.gd);
+  std::ostringstream k;
+  k.precision(17);
+  k << orient << a.dR << a.dU << b.dR << b.dU << ((ix + iy) % 2) << ':'
+    << onsite_weight(ix, iy, s.nx, s.ny) << ':' << onsite_weight(jx, jy, s.nx, s.ny) << ':' << p.g
+    << ':' << p.t << ':' << p.m << ':' << p.tau * tau_factor << ':' << s.cplx;
+  auto it = g.find(k.str());
+  if (it != g.end()) return it->second;
+  Mat H = orient == 'h' ? H_merged_h_site(ix, iy, s.nx, s.ny, s.gd, p.g, p.t, p.m)
+                        : H_merged_v_site(ix, iy, s.nx, s.ny, s.gd, p.g, p.t, p.m);
+  Tensor G = gate_tensor(*s.c, expm_sym(H, p.tau * tau_factor), s.cplx);
+  g[k.str()] = G;
+  return G;
+}
+
+void trotter_step_gauge(State& s, const Params& p, GateCache& gc) {
+  // trotter_step_gauge! :523-539
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = 1; ix < s.nx; ++ix) su_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax);
+  for (int iy = 1; iy < s.ny; ++iy)
+    for (int ix = 1; ix <= s.nx; ++ix) su_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 1.0), p.Dmax);
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = s.nx - 1; ix >= 1; --ix) su_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax);
+}
+
+// All top boundaries of the current state: tops[r] = boundary_top(r), r=1..ny.
+static std::vector<MPS> all_tops(State& s, int chi) {
+  std::vector<MPS> tops(s.ny + 1);
+  tops[s.ny] = trivial_mps(*s.c, s.nx, s.cplx);
+  if (s.ny >= 2) tops[s.ny - 1] = row_as_mps_down(*s.c, row_dl(s, s.ny));
+  for (int r = s.ny - 2; r >= 1; --r) {
+    tops[r] = absorb_row_from_top(*s.c, tops[r + 1], row_dl(s, r + 1));
+    compress_mps(*s.c, tops[r], chi);
+  }
+  return tops;
+}
+
+// boundary_bottom(iy) from boundary_bottom(iy-1) and the current row iy-1.
+static MPS next_bottom(State& s, const MPS& prev, int iy, int chi) {
+  if (iy <= 1) return trivial_mps(*s.c, s.nx, s.cplx);
+  if (iy == 2) return row_as_mps_up(*s.c, row_dl(s, 1));
+  MPS b = absorb_row_from_bottom(*s.c, prev, row_dl(s, iy - 1));
+  compress_mps(*s.c, b, chi);
+  return b;
+}
+
+static void h_row(State& s, int iy, const MPS& top, const MPS& bot, bool l2r, const Params& p,
+                  GateCache& gc, std::vector<ALSInfo>* infos) {
+  Ctx& c = *s.c;
+  const int nx = s.nx;
+  auto dl = [&](int ix) { return dl_traced(c, sqrt_absorbed(s, ix, iy)); };
+  auto update = [&](int ix, const Tensor& L, const Tensor& R) {
+    apply_mask(s, ix, iy);
+    apply_mask(s, ix + 1, iy);
+    Env E = bond_env_h(s, ix, iy, top, bot, L, R);
+    ALSInfo inf;
+    fu_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, p.n_als, p.als_tol, E, &inf);
+    if (infos) infos->push_back(inf);
+  };
+  if (l2r) {
+    std::vector<Tensor> Renv(nx + 2);
+    Renv[nx + 1] = ones_tensor(c, 4, s.cplx);
+    for (int j = nx; j >= 3; --j) Renv[j] = spectator_right(c, Renv[j + 1], top[j - 1], bot[j - 1], dl(j));
+    Tensor L = ones_tensor(c, 4, s.cplx);
+    for (int ix = 1; ix <= nx - 1; ++ix) {
+      if (ix >= 2) L = spectator_left(c, L, top[ix - 2], bot[ix - 2], dl(ix - 1));
+      update(ix, L, Renv[ix + 2]);
+    }
+  } else {
+    std::vector<Tensor> Lenv(nx);
+    Lenv[0] = ones_tensor(c, 4, s.cplx);
+    for (int j = 1; j <= nx - 2; ++j) Lenv[j] = spectator_left(c, Lenv[j - 1], top[j - 1], bot[j - 1], dl(j));
+    Tensor R = ones_tensor(c, 4, s.cplx);
+    for (int ix = nx - 1; ix >= 1; --ix) {
+      if (ix <= nx - 2) R = spectator_right(c, R, top[ix + 1], bot[ix + 1], dl(ix + 2));
+      update(ix, Lenv[ix - 1], R);
+    }
+  }
+}
+
+static void h_sweep(State& s, bool l2r, const Params& p, GateCache& gc, std::vector<ALSInfo>* infos) {
+  std::vector<MPS> tops = all_tops(s, p.chi);
+  MPS bot;
+  for (int iy = 1; iy <= s.ny; ++iy) {
+    bot = next_bottom(s, bot, iy, p.chi);
+    h_row(s, iy, tops[iy], bot, l2r, p, gc, infos);
+  }
+}
+
+static void v_sweep(State& s, const Params& p, GateCache& gc, std::vector<ALSInfo>* infos) {
+  Ctx& c = *s.c;
+  const int nx = s.nx;
+  std::vector<MPS> tops = all_tops(s, p.chi);
+  MPS bot;
+  for (int iy = 1; iy < s.ny; ++iy) {
+    bot = next_bottom(s, bot, iy, p.chi);
+    const MPS& top = tops[iy + 1];
+    auto dl0 = [&](int ix) { return dl_traced(c, sqrt_absorbed(s, ix, iy)); };
+    auto dl1 = [&](int ix) { return dl_traced(c, sqrt_absorbed(s, ix, iy + 1)); };
+    std::vector<Tensor> Renv(nx + 2);
+    Renv[nx + 1] = ones_tensor(c, 6, s.cplx);
+    for (int j = nx; j >= 2; --j)
+      Renv[j] = spectator_right_v(c, Renv[j + 1], top[j - 1], bot[j - 1], dl0(j), dl1(j));
+    Tensor L = ones_tensor(c, 6, s.cplx);
+    for (int ix = 1; ix <= nx; ++ix) {
+      if (ix >= 2) L = spectator_left_v(c, L, top[ix - 2], bot[ix - 2], dl0(ix - 1), dl1(ix - 1));
+      apply_mask(s, ix, iy);
+      apply_mask(s, ix, iy + 1);
+      Env E = bond_env_v(s, ix, iy, top, bot, L, Renv[ix + 1]);
+      ALSInfo inf;
+      fu_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 1.0), p.Dmax, p.n_als, p.als_tol, E, &inf);
+      if (infos) infos->push_back(inf);
+    }
+  }
+}
+
+void trotter_step_v4(State& s, const Params& p, GateCache& gc, std::vector<ALSInfo>* infos) {
+  // trotter_step_v4! :415-432 — h½ (L→R), v, h½ (R→L)
+  h_sweep(s, true, p, gc, infos);
+  v_sweep(s, p, gc, infos);
+  h_sweep(s, false, p, gc, infos);
+}
+
+static double bond_expect(Ctx& c, const Env& E, const Tensor& h, double* nrm) {
+  // _bond_expect :374-382
+  Tensor th0 = contract(c, E.RA, "apk", E.RB, "bqk", "apqb");
+  Tensor hth = apply_gate(c, th0, h);
+  Tensor thp = permute(c, th0, {0, 3, 1, 2});
+  Tensor Tn = contract(c, E.env, "abcd", th0, "apqb", "cdpq");
+  Tensor Th = contract(c, E.env, "abcd", hth, "apqb", "cdpq");
+  *nrm = dot(c, Tn, thp, true).re;
+  return dot(c, Th, thp, true).re;
+}
+
+double energy_v4(State& s, const Params& p) {
+  // compute_energy_v4 :390-409 with exactly cached environments
+  Ctx& c = *s.c;
+  const int nx = s.nx, ny = s.ny;
+  std::vector<MPS> tops = all_tops(s, p.chi);
+  std::vector<MPS> bots(ny + 1);
+  for (int iy = 1; iy <= ny; ++iy) bots[iy] = next_bottom(s, iy >= 2 ? bots[iy - 1] : MPS(), iy, p.chi);
+  std::vector<std::vector<Tensor>> dls(ny + 1);
+  for (int iy = 1; iy <= ny; ++iy) dls[iy] = row_dl(s, iy);
+  double E = 0.0;
+  for (int iy = 1; iy <= ny; ++iy) {
+    const MPS &top = tops[iy], &bot = bots[iy];
+    std::vector<Tensor> Renv(nx + 2);
+    Renv[nx + 1] = ones_tensor(c, 4, s.cplx);
+    for (int j = nx; j >= 3; --j) Renv[j] = spectator_right(c, Renv[j + 1], top[j - 1], bot[j - 1], dls[iy][j - 1]);
+    Tensor L = ones_tensor(c, 4, s.cplx);
+    for (int ix = 1; ix < nx; ++ix) {
+      if (ix >= 2) L = spectator_left(c, L, top[ix - 2], bot[ix - 2], dls[iy][ix - 2]);
+      Env Eb = bond_env_h(s, ix, iy, top, bot, L, Renv[ix + 2]);
+      Tensor h = gate_tensor(c, H_merged_h_site(ix, iy, nx, ny, s.gd, p.g, p.t, p.m), s.cplx);
+      double nr;
+      double hv = bond_expect(c, Eb, h, &nr);
+      E += hv / nr;
+    }
+  }
+  for (int iy = 1; iy < ny; ++iy) {
+    const MPS &top = tops[iy + 1], &bot = bots[iy];
+    std::vector<Tensor> Renv(nx + 2);
+    Renv[nx + 1] = ones_tensor(c, 6, s.cplx);
+    for (int j = nx; j >= 2; --j)
+      Renv[j] = spectator_right_v(c, Renv[j + 1], top[j - 1], bot[j - 1], dls[iy][j - 1], dls[iy + 1][j - 1]);
+    Tensor L = ones_tensor(c, 6, s.cplx);
+    for (int ix = 1; ix <= nx; ++ix) {
+      if (ix >= 2) L = spectator_left_v(c, L, top[ix - 2], bot[ix - 2], dls[iy][ix - 2], dls[iy + 1][ix - 2]);
+      Env Eb = bond_env_v(s, ix, iy, top, bot, L, Renv[ix + 1]);
+      Tensor h = gate_tensor(c, H_merged_v_site(ix, iy, nx, ny, s.gd, p.g, p.t, p.m), s.cplx);
+      double nr;
+      double hv = bond_expect(c, Eb, h, &nr);
+      E += hv / nr;
+    }
+  }
+  return E;
+}
+
+double mean_nf(State& s) {
+  // measure_nf_grid with SU weights (finite_peps_quench.jl:134-146), averaged
+  Ctx& c = *s.c;
+  double acc = 0.0;
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = 1; ix <= s.nx; ++ix) {
+      Tensor T = copy_tensor(c, s.t(ix, iy));
+      std::vector<double> w[4] = {w_l(s, ix, iy), w_r(s, ix, iy), w_u(s, ix, iy), w_d(s, ix, iy)};
+      scale_legs(c, T, w);
+      auto h = c.to_host(T);
+      std::vector<int64_t> nf, lR, lU;
+      phys_labels(ix, iy, s.nx, s.ny, s.gd, &nf, &lR, &lU);
+      const int64_t d = T.dims[0], rest = T.numel() / d;
+      double num = 0, den = 0;
+      for (int64_t r = 0; r < rest; ++r)
+        for (int64_t p = 0; p < d; ++p) {
+          size_t i = size_t(p + r * d);
+          double a2 = s.cplx ? h[2 * i] * h[2 * i] + h[2 * i + 1] * h[2 * i + 1] : h[i] * h[i];
+          num += a2 * double(nf[p]);
+          den += a2;
+        }
+      acc += num / den;
+    }
+  return acc / (s.nx * s.ny);
+}
+
+}  // namespace peps

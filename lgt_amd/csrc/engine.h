@@ -1,0 +1,126 @@
+// The finite-PEPS contraction engine on the GPU: state, boundary MPS, bond
+// environments, ALS truncation, simple/full bond updates and the Trotter
+// drivers with exactly cached environments.
+#pragma once
+#include <map>
+#include <string>
+
+#include "model.h"
+#include "peps_internal.h"
+
+namespace peps {
+
+constexpr double FU_REG = 1e-10;      // finite_peps_full_update.jl:336
+constexpr double GI_REG = 1e-12;      // finite_peps_gauge_invariant.jl:224
+constexpr double V4_ALS_REG = 1e-10;  // finite_peps_fullupdate_v4.jl:30
+constexpr double CMPS_CUT = 1e-14;    // finite_peps_boundary_mps.jl:93
+
+using MPS = std::vector<Tensor>;
+
+struct Params {
+  double g = 1.0, t = 1.0, m = 0.5, tau = 0.05;
+  int Dmax = 2, chi = 16, n_als = 30;
+  double als_tol = 0.0;  // > 0: stop sweeping when the relative cost change < tol
+};
+
+// GaugePEPS (finite_peps_gauge_invariant.jl:82-88) held on the device.
+struct State {
+  Ctx* c = nullptr;
+  int nx = 0, ny = 0, gd = 3;
+  bool cplx = false;
+  std::vector<int64_t> gch;
+  std::vector<Tensor> T;
+  std::vector<std::vector<double>> lh, lv;
+  std::vector<std::vector<int64_t>> qh, qv;
+
+  Tensor& t(int ix, int iy) { return T.at(size_t(ix - 1) + size_t(iy - 1) * nx); }
+  std::vector<double>& LH(int ix, int iy) { return lh.at(size_t(ix - 1) + size_t(iy - 1) * (nx - 1)); }
+  std::vector<double>& LV(int ix, int iy) { return lv.at(size_t(ix - 1) + size_t(iy - 1) * nx); }
+  std::vector<int64_t>& QH(int ix, int iy) { return qh.at(size_t(ix - 1) + size_t(iy - 1) * (nx - 1)); }
+  std::vector<int64_t>& QV(int ix, int iy) { return qv.at(size_t(ix - 1) + size_t(iy - 1) * nx); }
+  int64_t q(int ix, int iy) const { return gch.at(size_t(ix - 1) + size_t(iy - 1) * nx); }
+  void init(Ctx* ctx, int nx_, int ny_, int gd_, const int64_t* g, bool cplx_);
+};
+
+struct Env {
+  Tensor env, RA, RB, QA, QB;
+};
+
+struct ALSInfo {
+  double cost0 = 0, cost1 = 0;
+  bool accepted = false;
+  int sweeps = 0;
+};
+
+struct CSVD {
+  Tensor U, Vt;
+  std::vector<double> S;
+  std::vector<int64_t> q;
+};
+
+// ── site-level pieces ────────────────────────────────────────────────────────
+void apply_mask(State& s, int ix, int iy);
+double forbidden_norm(State& s);
+Tensor sqrt_absorbed(State& s, int ix, int iy);
+Tensor dl_traced(Ctx& c, const Tensor& T);
+Tensor gate_tensor(Ctx& c, const Mat& G, bool cplx);
+
+// ── boundary MPS ─────────────────────────────────────────────────────────────
+MPS row_dl(State& s, int iy);
+MPS row_as_mps_down(Ctx& c, const MPS& row);
+MPS row_as_mps_up(Ctx& c, const MPS& row);
+MPS absorb_row_from_top(Ctx& c, const MPS& mps, const MPS& row);
+MPS absorb_row_from_bottom(Ctx& c, const MPS& mps, const MPS& row);
+void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT);
+MPS trivial_mps(Ctx& c, int n, bool cplx);
+MPS boundary_top(State& s, int stop_row, int chi);
+MPS boundary_bottom(State& s, int stop_row, int chi);
+cplx peps_norm2(State& s, int chi);
+
+// ── bond environments ────────────────────────────────────────────────────────
+Tensor spectator_left(Ctx& c, const Tensor& Lenv, const Tensor& top, const Tensor& bot, const Tensor& E);
+Tensor spectator_right(Ctx& c, const Tensor& Renv, const Tensor& top, const Tensor& bot, const Tensor& E);
+Tensor spectator_left_v(Ctx& c, const Tensor& Lenv, const Tensor& top, const Tensor& bot,
+                        const Tensor& E0, const Tensor& E1);
+Tensor spectator_right_v(Ctx& c, const Tensor& Renv, const Tensor& top, const Tensor& bot,
+                         const Tensor& E0, const Tensor& E1);
+Tensor ones_tensor(Ctx& c, int rank, bool cplx);
+Env bond_env_h(State& s, int ix, int iy, const MPS& top, const MPS& bot, const Tensor& Lenv,
+               const Tensor& Renv);
+Env bond_env_v(State& s, int ix, int iy, const MPS& top, const MPS& bot, const Tensor& Lenv,
+               const Tensor& Renv);
+// Recompute-from-scratch forms (finite_peps_boundary_mps.jl:389, :507).
+Env bond_environment_h(State& s, int ix, int iy, int chi);
+Env bond_environment_v(State& s, int ix, int iy, int chi);
+
+// ── truncations ──────────────────────────────────────────────────────────────
+CSVD charge_svd(Ctx& c, const Tensor& M, const std::vector<int64_t>& rq,
+                const std::vector<int64_t>& cq, int Dmax, double reg = GI_REG);
+Tensor apply_gate(Ctx& c, const Tensor& th, const Tensor& G);
+double als_cost(Ctx& c, const Tensor& th, const Tensor& env, const Tensor& X, const Tensor& Y,
+                const Tensor* Eth = nullptr);
+void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,
+                  const std::vector<int64_t>& eR, int n_als, double reg, double tol, Tensor* X,
+                  Tensor* Y, std::vector<int64_t>* bflux, ALSInfo* info,
+                  const Tensor* X_init = nullptr, const Tensor* Y_init = nullptr);
+
+// ── bond updates and drivers ────────────────────────────────────────────────
+void su_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax);
+void su_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax);
+void fu_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, double tol,
+               const Env& E, ALSInfo* info = nullptr);
+void fu_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, double tol,
+               const Env& E, ALSInfo* info = nullptr);
+
+struct GateCache {
+  std::map<std::string, Tensor> g;
+  Tensor get(State& s, char orient, int ix, int iy, const Params& p, double tau_factor);
+};
+
+void trotter_step_gauge(State& s, const Params& p, GateCache& gc);
+void trotter_step_v4(State& s, const Params& p, GateCache& gc, std::vector<ALSInfo>* infos = nullptr);
+double energy_v4(State& s, const Params& p);
+// <n_f> from simple-update weights, averaged over sites (finite_peps_quench.jl:134).
+double mean_nf(State& s);
+
+}  // namespace peps

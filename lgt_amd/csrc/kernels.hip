@@ -1,0 +1,1247 @@
+// HIP kernels of the finite-PEPS engine, written for CDNA4 (gfx950).
+//
+//   gemm      LDS-tiled MFMA GEMM.  Storage is FP64 (real or interleaved
+//             complex); the compute tier picks the MFMA: f64 16x16x4,
+//             f32 16x16x4 or bf16 16x16x16 (FP32 accumulation).  Tiles are
+//             converted while being staged into LDS.  Split-K with a
+//             deterministic (fixed-order) reduction fills the chip when the
+//             output is small and K is huge (environment closures).
+//   permute   HBM-streaming N-d transpose (coalesced writes).
+//   the rest  elementwise weights/masks and wave reductions.
+#include "peps_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace peps {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+static inline unsigned cdiv(int64_t a, int64_t b) { return unsigned((a + b - 1) / b); }
+
+// ════════════════════════════════════════════════════════════════════════════
+//  GEMM
+// ════════════════════════════════════════════════════════════════════════════
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+
+template <int PREC> struct LdsT;
+template <> struct LdsT<0> { typedef double T; };
+template <> struct LdsT<1> { typedef float T; };
+template <> struct LdsT<2> { typedef unsigned short T; };
+
+template <int PREC> struct AccT;
+template <> struct AccT<0> { typedef v4d T; };
+template <> struct AccT<1> { typedef v4f T; };
+template <> struct AccT<2> { typedef v4f T; };
+
+template <int PREC>
+__device__ __forceinline__ typename LdsT<PREC>::T cvt(double x) {
+  if constexpr (PREC == 0) return x;
+  else if constexpr (PREC == 1) return float(x);
+  else return f2bf(float(x));
+}
+
+// Block of 256 threads = 4 waves in a 2x2 arrangement; each wave owns
+// (16*WM) x (16*WN) outputs as WM x WN MFMA 16x16 tiles.  BK = 16.
+//
+// MFMA 16x16 operand layout (lane l): A row = l%16, k-group = l/16;
+// B col = l%16, k-group = l/16; output row = 4*(l/16)+v, col = l%16.
+template <bool CPLX, int PREC, bool AK, bool BKF, int WM, int WN>
+__global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t K,
+                                                   const double* __restrict__ A, int64_t lda,
+                                                   const double* __restrict__ B, int64_t ldb,
+                                                   double* __restrict__ C, int64_t ldc, int conjA,
+                                                   int conjB, double beta, int64_t kchunk,
+                                                   int64_t wstride) {
+  constexpr int BM = 32 * WM, BN = 32 * WN, BK = 16;
+  constexpr int NC = CPLX ? 2 : 1;
+  typedef typename LdsT<PREC>::T LT;
+  typedef typename AccT<PREC>::T ACC;
+  // PREC 0/1 store tiles k-major ([k][m]); bf16 stores m-major ([m][k]) so a
+  // lane's 4 consecutive k are one 8-byte read.
+  constexpr int APAD = (PREC == 2) ? (BK + 4) : (BM + 1);
+  constexpr int BPAD = (PREC == 2) ? (BK + 4) : (BN + 1);
+  constexpr int ASZ = (PREC == 2) ? BM * APAD : BK * APAD;
+  constexpr int BSZ = (PREC == 2) ? BN * BPAD : BK * BPAD;
+  __shared__ __align__(16) LT As[NC][ASZ];
+  __shared__ __align__(16) LT Bs[NC][BSZ];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave & 1) * (16 * WM);
+  const int wn = (wave >> 1) * (16 * WN);
+  const int64_t m0 = int64_t(blockIdx.x) * BM;
+  const int64_t n0 = int64_t(blockIdx.y) * BN;
+  const int64_t kbeg = int64_t(blockIdx.z) * kchunk;
+  const int64_t kend = min(K, kbeg + kchunk);
+  double* Cz = C + int64_t(blockIdx.z) * wstride;
+
+  ACC acc[NC][WM][WN];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j) acc[c][i][j] = ACC{0, 0, 0, 0};
+
+  constexpr int AE = BM * BK / 256;  // elements per thread per tile
+  constexpr int BE = BN * BK / 256;
+  double ra[AE][NC], rb[BE][NC];
+
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      int idx = tid + e * 256;
+      int kk, mm;
+      if (AK) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+      int64_t gm = m0 + mm, gk = k0 + kk;
+      bool ok = gm < M && gk < kend;
+      int64_t off = AK ? (gk + gm * lda) : (gm + gk * lda);
+      if (CPLX) {
+        ra[e][0] = ok ? A[2 * off] : 0.0;
+        ra[e][NC - 1] = ok ? A[2 * off + 1] : 0.0;
+      } else {
+        ra[e][0] = ok ? A[off] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      int idx = tid + e * 256;
+      int kk, nn;
+      if (BKF) { kk = idx % BK; nn = idx / BK; } else { nn = idx % BN; kk = idx / BN; }
+      int64_t gn = n0 + nn, gk = k0 + kk;
+      bool ok = gn < N && gk < kend;
+      int64_t off = BKF ? (gk + gn * ldb) : (gn + gk * ldb);
+      if (CPLX) {
+        rb[e][0] = ok ? B[2 * off] : 0.0;
+        rb[e][NC - 1] = ok ? B[2 * off + 1] : 0.0;
+      } else {
+        rb[e][0] = ok ? B[off] : 0.0;
+      }
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      int idx = tid + e * 256;
+      int kk, mm;
+      if (AK) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+      int li = (PREC == 2) ? (mm * APAD + kk) : (kk * APAD + mm);
+      As[0][li] = cvt<PREC>(ra[e][0]);
+      if (CPLX) As[NC - 1][li] = cvt<PREC>(conjA ? -ra[e][NC - 1] : ra[e][NC - 1]);
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      int idx = tid + e * 256;
+      int kk, nn;
+      if (BKF) { kk = idx % BK; nn = idx / BK; } else { nn = idx % BN; kk = idx / BN; }
+      int li = (PREC == 2) ? (nn * BPAD + kk) : (kk * BPAD + nn);
+      Bs[0][li] = cvt<PREC>(rb[e][0]);
+      if (CPLX) Bs[NC - 1][li] = cvt<PREC>(conjB ? -rb[e][NC - 1] : rb[e][NC - 1]);
+    }
+  };
+
+  const int lr = lane & 15;
+  const int lg = lane >> 4;
+
+  if (kbeg < kend) {
+    gload(kbeg);
+    lstore();
+  }
+  __syncthreads();
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = (k0 + BK) < kend;
+    if (more) gload(k0 + BK);
+    if constexpr (PREC == 2) {
+      v4s a[NC][WM], b[NC][WN];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+          a[c][i] = *reinterpret_cast<const v4s*>(&As[c][(wm + i * 16 + lr) * APAD + 4 * lg]);
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+          b[c][j] = *reinterpret_cast<const v4s*>(&Bs[c][(wn + j * 16 + lr) * BPAD + 4 * lg]);
+      }
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+          acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0][i], b[0][j], acc[0][i][j], 0, 0, 0);
+          if constexpr (CPLX) {
+            // bf16 negation flips the sign bit
+            const short sb = short(0x8000);
+            v4s nai = a[1][i] ^ v4s{sb, sb, sb, sb};
+            acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nai, b[1][j], acc[0][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0][i], b[1][j], acc[1][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1][i], b[0][j], acc[1][i][j], 0, 0, 0);
+          }
+        }
+    } else {
+#pragma unroll
+      for (int kq = 0; kq < BK / 4; ++kq) {
+        const int kr = kq * 4 + lg;
+        LT a[NC][WM], b[NC][WN];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+#pragma unroll
+          for (int i = 0; i < WM; ++i) a[c][i] = As[c][kr * APAD + wm + i * 16 + lr];
+#pragma unroll
+          for (int j = 0; j < WN; ++j) b[c][j] = Bs[c][kr * BPAD + wn + j * 16 + lr];
+        }
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+#pragma unroll
+          for (int j = 0; j < WN; ++j) {
+            if constexpr (PREC == 0) {
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i], b[0][j], acc[0][i][j], 0, 0, 0);
+              if constexpr (CPLX) {
+                acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[NC - 1][i], b[NC - 1][j], acc[0][i][j], 0, 0, 0);
+                acc[NC - 1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i], b[NC - 1][j], acc[NC - 1][i][j], 0, 0, 0);
+                acc[NC - 1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[NC - 1][i], b[0][j], acc[NC - 1][i][j], 0, 0, 0);
+              }
+            } else {
+              acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i], b[0][j], acc[0][i][j], 0, 0, 0);
+              if constexpr (CPLX) {
+                acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[NC - 1][i], b[NC - 1][j], acc[0][i][j], 0, 0, 0);
+                acc[NC - 1][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i], b[NC - 1][j], acc[NC - 1][i][j], 0, 0, 0);
+                acc[NC - 1][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[NC - 1][i], b[0][j], acc[NC - 1][i][j], 0, 0, 0);
+              }
+            }
+          }
+      }
+    }
+    __syncthreads();
+    if (more) {
+      lstore();
+      __syncthreads();
+    }
+  }
+
+  // epilogue: C = acc + beta * C
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      int64_t col = n0 + wn + j * 16 + lr;
+      if (col >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // f64 16x16x4 keeps rows 4v + l/16 per lane (group size 1); the
+        // f32/bf16 instructions keep rows 4(l/16) + v (group size 4)
+        int64_t row = m0 + wm + i * 16 + (PREC == 0 ? (lg + 4 * v) : (4 * lg + v));
+        if (row >= M) continue;
+        int64_t off = row + col * ldc;
+        if (CPLX) {
+          double re = double(acc[0][i][j][v]), im = double(acc[NC - 1][i][j][v]);
+          if (beta != 0.0) {
+            re += beta * Cz[2 * off];
+            im += beta * Cz[2 * off + 1];
+          }
+          Cz[2 * off] = re;
+          Cz[2 * off + 1] = im;
+        } else {
+          double x = double(acc[0][i][j][v]);
+          if (beta != 0.0) x += beta * Cz[off];
+          Cz[off] = x;
+        }
+      }
+    }
+}
+
+__global__ void splitk_reduce(double* __restrict__ C, const double* __restrict__ W, int64_t n,
+                              int splits, int64_t wstride, double beta) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double s = 0.0;
+    for (int z = 0; z < splits; ++z) s += W[z * wstride + i];
+    C[i] = (beta != 0.0 ? beta * C[i] : 0.0) + s;
+  }
+}
+
+template <bool CPLX, int PREC, bool AK, bool BKF, int WM, int WN>
+static void launch_gemm_t(hipStream_t st, int64_t M, int64_t N, int64_t K, const double* A,
+                          int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                          int cA, int cB, double beta, int splits, int64_t kchunk,
+                          int64_t wstride) {
+  dim3 grid(cdiv(M, 32 * WM), cdiv(N, 32 * WN), splits);
+  hipLaunchKernelGGL((gemm_kernel<CPLX, PREC, AK, BKF, WM, WN>), grid, dim3(256), 0, st, M, N, K,
+                     A, lda, B, ldb, C, ldc, cA, cB, beta, kchunk, wstride);
+}
+
+template <bool CPLX, int PREC, int WM, int WN>
+static void dispatch_layout(bool ak, bool bk, hipStream_t st, int64_t M, int64_t N, int64_t K,
+                            const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                            int64_t ldc, int cA, int cB, double beta, int splits, int64_t kchunk,
+                            int64_t wstride) {
+  if (ak && bk)
+    launch_gemm_t<CPLX, PREC, true, true, WM, WN>(st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
+  else if (ak && !bk)
+    launch_gemm_t<CPLX, PREC, true, false, WM, WN>(st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
+  else if (!ak && bk)
+    launch_gemm_t<CPLX, PREC, false, true, WM, WN>(st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
+  else
+    launch_gemm_t<CPLX, PREC, false, false, WM, WN>(st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
+}
+
+template <bool CPLX, int PREC>
+static void dispatch_tile(bool big, bool ak, bool bk, hipStream_t st, int64_t M, int64_t N,
+                          int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb,
+                          double* C, int64_t ldc, int cA, int cB, double beta, int splits,
+                          int64_t kchunk, int64_t wstride) {
+  if (big && !CPLX)
+    dispatch_layout<CPLX, PREC, 4, 4>(ak, bk, st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
+  else
+    dispatch_layout<CPLX, PREC, 2, 2>(ak, bk, st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
+}
+
+void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, bool a_kfast,
+          bool conjA, const double* B, bool b_kfast, bool conjB, double* C, double beta) {
+  if (M <= 0 || N <= 0) return;
+  const int64_t lda = a_kfast ? K : M;
+  const int64_t ldb = b_kfast ? K : N;
+  const int esz = cplx ? 2 : 1;
+  if (K <= 0) {
+    if (beta == 0.0) PEPS_HIP(hipMemsetAsync(C, 0, size_t(M * N) * esz * 8, c.stream));
+    else if (beta != 1.0) {
+      Tensor t;  // scale existing C
+      t.buf = std::make_shared<Buffer>();
+      t.buf->ptr = C;
+      t.dims = {M, N};
+      t.cplx = cplx;
+      scale(c, t, beta);
+      t.buf->ptr = nullptr;
+    }
+    return;
+  }
+  const bool big = !cplx && M >= 128 && N >= 128;
+  const int BMt = big ? 128 : 64, BNt = big ? 128 : 64;
+  const int64_t tiles = int64_t(cdiv(M, BMt)) * cdiv(N, BNt);
+  int splits = 1;
+  const int64_t target = int64_t(c.ncu) * 2;
+  if (tiles < target && K >= 1024) {
+    splits = int(std::min<int64_t>((target + tiles - 1) / tiles, K / 256));
+    splits = std::max(1, std::min(splits, 512));
+    // bound the workspace to 1 GiB
+    while (splits > 1 && double(splits) * M * N * esz * 8 > double(1ull << 30)) splits /= 2;
+  }
+  int64_t kchunk = K;
+  if (splits > 1) {
+    kchunk = ((K + splits - 1) / splits + 15) / 16 * 16;
+    splits = int((K + kchunk - 1) / kchunk);
+  }
+  const double flops = (cplx ? 8.0 : 2.0) * double(M) * double(N) * double(K);
+  const double bytes = double(esz * 8) * (double(M) * K + double(K) * N + double(M) * N);
+  const char* cls = c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16");
+  c.count(cls, flops, bytes);
+  hipEvent_t ev = nullptr;
+  c.tbegin(cls, &ev);
+  std::shared_ptr<Buffer> ws;
+  double* out = C;
+  int64_t wstride = 0;
+  double kbeta = beta;
+  if (splits > 1) {
+    wstride = M * N * esz;
+    ws = std::make_shared<Buffer>();
+    ws->pool = c.pool.get();
+    ws->bytes = size_t(splits) * wstride * 8;
+    ws->ptr = c.pool->get(ws->bytes);
+    out = static_cast<double*>(ws->ptr);
+    kbeta = 0.0;
+  }
+  auto go = [&](auto cplx_tag, auto prec_tag) {
+    constexpr bool CP = decltype(cplx_tag)::value;
+    constexpr int PR = decltype(prec_tag)::value;
+    dispatch_tile<CP, PR>(big, a_kfast, b_kfast, c.stream, M, N, K, A, lda, B, ldb, out, M,
+                          conjA, conjB, kbeta, splits, kchunk, wstride);
+  };
+  using T0 = std::integral_constant<int, 0>;
+  using T1 = std::integral_constant<int, 1>;
+  using T2 = std::integral_constant<int, 2>;
+  if (cplx) {
+    if (c.tier == TIER_FP64) go(std::true_type{}, T0{});
+    else if (c.tier == TIER_FP32) go(std::true_type{}, T1{});
+    else go(std::true_type{}, T2{});
+  } else {
+    if (c.tier == TIER_FP64) go(std::false_type{}, T0{});
+    else if (c.tier == TIER_FP32) go(std::false_type{}, T1{});
+    else go(std::false_type{}, T2{});
+  }
+  PEPS_HIP(hipGetLastError());
+  if (splits > 1) {
+    int64_t n = M * N * esz;
+    unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 8192));
+    hipLaunchKernelGGL(splitk_reduce, dim3(g), dim3(256), 0, c.stream, C, out, n, splits, wstride, beta);
+    PEPS_HIP(hipGetLastError());
+  }
+  c.tend(cls, ev);
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  permute
+// ════════════════════════════════════════════════════════════════════════════
+
+constexpr int PMAX = 16;
+struct PermArgs {
+  int nd;
+  int64_t odims[PMAX];
+  int64_t istr[PMAX];  // input stride (elements) of output dim d
+};
+
+template <typename E>
+__global__ void permute_kernel(const E* __restrict__ in, E* __restrict__ out, int64_t n,
+                               PermArgs p) {
+  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < n;
+       j += int64_t(gridDim.x) * blockDim.x) {
+    int64_t rem = j, off = 0;
+#pragma unroll 4
+    for (int d = 0; d < p.nd; ++d) {
+      int64_t q = rem / p.odims[d];
+      off += (rem - q * p.odims[d]) * p.istr[d];
+      rem = q;
+    }
+    out[j] = in[off];
+  }
+}
+
+// 2-D tiled transpose through LDS for the common "swap two blocks" case:
+// in is (R x C) col-major, out is (C x R) col-major, batched over `nb`.
+template <typename E>
+__global__ void transpose_kernel(const E* __restrict__ in, E* __restrict__ out, int64_t R,
+                                 int64_t Cc, int64_t nb) {
+  __shared__ E tile[32][33];
+  const int64_t b = blockIdx.z;
+  const E* inb = in + b * R * Cc;
+  E* outb = out + b * R * Cc;
+  for (int64_t r0 = int64_t(blockIdx.x) * 32; r0 < R; r0 += int64_t(gridDim.x) * 32)
+    for (int64_t c0 = int64_t(blockIdx.y) * 32; c0 < Cc; c0 += int64_t(gridDim.y) * 32) {
+      for (int k = threadIdx.y; k < 32; k += 8) {
+        int64_t r = r0 + threadIdx.x, cc = c0 + k;
+        if (r < R && cc < Cc) tile[k][threadIdx.x] = inb[r + cc * R];
+      }
+      __syncthreads();
+      for (int k = threadIdx.y; k < 32; k += 8) {
+        int64_t cc = c0 + threadIdx.x, r = r0 + k;
+        if (r < R && cc < Cc) outb[cc + r * Cc] = tile[threadIdx.x][k];
+      }
+      __syncthreads();
+    }
+}
+
+Tensor permute(Ctx& c, const Tensor& in, const std::vector<int>& perm) {
+  const int nd = in.rank();
+  PEPS_CHECK(int(perm.size()) == nd, PS_ESHAPE, "permute: rank mismatch");
+  std::vector<int64_t> od(nd);
+  for (int i = 0; i < nd; ++i) od[i] = in.dims[perm[i]];
+  bool ident = true;
+  for (int i = 0; i < nd; ++i) ident = ident && (perm[i] == i || in.dims[perm[i]] == 1);
+  // identity up to unit dims: pure reshape
+  {
+    std::vector<int> nz;
+    for (int i = 0; i < nd; ++i)
+      if (in.dims[perm[i]] != 1) nz.push_back(perm[i]);
+    bool sorted = true;
+    for (size_t i = 1; i < nz.size(); ++i) sorted = sorted && nz[i] > nz[i - 1];
+    if (sorted) return in.reshaped(od);
+  }
+  (void)ident;
+  Tensor out = c.alloc(od, in.cplx);
+  const int64_t n = in.numel();
+  if (n == 0) return out;
+  std::vector<int64_t> istride(nd);
+  {
+    int64_t s = 1;
+    for (int i = 0; i < nd; ++i) {
+      istride[i] = s;
+      s *= in.dims[i];
+    }
+  }
+  // collapse: drop unit dims, merge output dims that are contiguous in input
+  std::vector<int64_t> cd, cs;
+  for (int i = 0; i < nd; ++i) {
+    if (od[i] == 1) continue;
+    int64_t st = istride[perm[i]];
+    if (!cd.empty() && cs.back() * cd.back() == st) {
+      cd.back() *= od[i];
+    } else {
+      cd.push_back(od[i]);
+      cs.push_back(st);
+    }
+  }
+  c.count("permute", 0, 2.0 * in.bytes());
+  hipEvent_t ev = nullptr;
+  c.tbegin("permute", &ev);
+  // two-block swap => tiled transpose: output (d0, d1) with d0 contiguous at
+  // input stride d1-extent.. i.e. input is (d1 x d0) col-major.
+  bool tr = false;
+  int64_t R = 0, Cc = 0, nb = 1;
+  if (cd.size() == 2 && cs[1] == 1 && cs[0] == cd[1]) {
+    tr = true; R = cd[1]; Cc = cd[0];
+  } else if (cd.size() == 3 && cs[1] == 1 && cs[0] == cd[1] && cs[2] == cd[0] * cd[1]) {
+    tr = true; R = cd[1]; Cc = cd[0]; nb = cd[2];
+  }
+  if (tr && nb < 65536) {
+    dim3 grid(std::min<unsigned>(cdiv(R, 32), 1024u), std::min<unsigned>(cdiv(Cc, 32), 1024u), unsigned(nb));
+    if (in.cplx)
+      hipLaunchKernelGGL(transpose_kernel<double2>, grid, dim3(32, 8), 0, c.stream,
+                         (const double2*)in.data(), (double2*)out.data(), R, Cc, nb);
+    else
+      hipLaunchKernelGGL(transpose_kernel<double>, grid, dim3(32, 8), 0, c.stream, in.data(),
+                         out.data(), R, Cc, nb);
+  } else {
+    PEPS_CHECK(int(cd.size()) <= PMAX, PS_EUNSUPPORTED, "permute: too many dims");
+    PermArgs p{};
+    p.nd = int(cd.size());
+    for (int i = 0; i < p.nd; ++i) {
+      p.odims[i] = cd[i];
+      p.istr[i] = cs[i];
+    }
+    unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 65536));
+    if (in.cplx)
+      hipLaunchKernelGGL(permute_kernel<double2>, dim3(g), dim3(256), 0, c.stream,
+                         (const double2*)in.data(), (double2*)out.data(), n, p);
+    else
+      hipLaunchKernelGGL(permute_kernel<double>, dim3(g), dim3(256), 0, c.stream, in.data(),
+                         out.data(), n, p);
+  }
+  PEPS_HIP(hipGetLastError());
+  c.tend("permute", ev);
+  return out;
+}
+
+// ════════════════════════════════════════════════════════════════════════════
+//  elementwise
+// ════════════════════════════════════════════════════════════════════════════
+
+constexpr int WMAX = 64;
+struct LegW {
+  int64_t d[5];
+  double w[4][WMAX];
+};
+
+__global__ void scale_legs_kernel(double* T, int esz, int64_t n, LegW a) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i / a.d[0];
+    int64_t l = r % a.d[1]; r /= a.d[1];
+    int64_t rr = r % a.d[2]; r /= a.d[2];
+    int64_t u = r % a.d[3]; r /= a.d[3];
+    int64_t d = r;
+    double s = a.w[0][l] * a.w[1][rr] * a.w[2][u] * a.w[3][d];
+    T[i * esz] *= s;
+    if (esz == 2) T[i * esz + 1] *= s;
+  }
+}
+
+void scale_legs(Ctx& c, Tensor& T, const std::vector<double>* w) {
+  PEPS_CHECK(T.rank() == 5, PS_ESHAPE, "scale_legs: rank-5 tensor expected");
+  LegW a{};
+  for (int i = 0; i < 5; ++i) a.d[i] = T.dims[i];
+  for (int k = 0; k < 4; ++k) {
+    PEPS_CHECK(int64_t(w[k].size()) == T.dims[k + 1], PS_ESHAPE, "scale_legs: weight length");
+    PEPS_CHECK(w[k].size() <= size_t(WMAX), PS_EUNSUPPORTED, "scale_legs: bond dimension > 64");
+    for (size_t j = 0; j < w[k].size(); ++j) a.w[k][j] = w[k][j];
+  }
+  int64_t n = T.numel();
+  if (!n) return;
+  c.count("elementwise", 0, 2.0 * T.bytes());
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(scale_legs_kernel, dim3(g), dim3(256), 0, c.stream, T.data(),
+                     T.cplx ? 2 : 1, n, a);
+  PEPS_HIP(hipGetLastError());
+}
+
+__global__ void scale_kernel(double* x, int64_t n, double s) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] *= s;
+}
+void scale(Ctx& c, Tensor& T, double s) {
+  int64_t n = T.numel() * (T.cplx ? 2 : 1);
+  if (!n) return;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(scale_kernel, dim3(g), dim3(256), 0, c.stream, T.data(), n, s);
+  PEPS_HIP(hipGetLastError());
+}
+
+__global__ void axpy_kernel(double* y, const double* x, int64_t n, double a) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    y[i] += a * x[i];
+}
+void axpy_inplace(Ctx& c, Tensor& Y, const Tensor& X, double a) {
+  PEPS_CHECK(Y.numel() == X.numel() && Y.cplx == X.cplx, PS_ESHAPE, "axpy: shape");
+  int64_t n = Y.numel() * (Y.cplx ? 2 : 1);
+  if (!n) return;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(axpy_kernel, dim3(g), dim3(256), 0, c.stream, Y.data(), X.data(), n, a);
+  PEPS_HIP(hipGetLastError());
+}
+
+__global__ void conj_kernel(double* x, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[2 * i + 1] = -x[2 * i + 1];
+}
+void conj_inplace(Ctx& c, Tensor& T) {
+  if (!T.cplx || !T.numel()) return;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(T.numel(), 256), 16384));
+  hipLaunchKernelGGL(conj_kernel, dim3(g), dim3(256), 0, c.stream, T.data(), T.numel());
+  PEPS_HIP(hipGetLastError());
+}
+
+// ── Gauss mask ───────────────────────────────────────────────────────────────
+struct MaskArgs {
+  int64_t d[5];
+  int64_t q;
+  int64_t nf[WMAX], lR[WMAX], lU[WMAX];
+  int64_t fl[WMAX], fr[WMAX], fu[WMAX], fd[WMAX];
+};
+
+__device__ __forceinline__ bool mask_ok(const MaskArgs& a, int64_t i) {
+  int64_t p = i % a.d[0];
+  int64_t r = i / a.d[0];
+  int64_t l = r % a.d[1]; r /= a.d[1];
+  int64_t rr = r % a.d[2]; r /= a.d[2];
+  int64_t u = r % a.d[3]; r /= a.d[3];
+  int64_t d = r;
+  int64_t R = a.lR[p], U = a.lU[p];
+  return a.fr[rr] == R && a.fu[u] == U && (R + U - a.fl[l] - a.fd[d] == 2 * (a.nf[p] + a.q));
+}
+
+__global__ void mask_kernel(double* T, int esz, int64_t n, MaskArgs a) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (!mask_ok(a, i)) {
+      T[i * esz] = 0.0;
+      if (esz == 2) T[i * esz + 1] = 0.0;
+    }
+  }
+}
+
+__global__ void mask_viol_kernel(const double* T, int esz, int64_t n, MaskArgs a, double* part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (!mask_ok(a, i)) {
+      double x = T[i * esz];
+      s += x * x;
+      if (esz == 2) {
+        double y = T[i * esz + 1];
+        s += y * y;
+      }
+    }
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+static MaskArgs make_mask_args(const Tensor& T, const std::vector<int64_t>& nf,
+                               const std::vector<int64_t>& lR, const std::vector<int64_t>& lU,
+                               const std::vector<int64_t>& fl, const std::vector<int64_t>& fr,
+                               const std::vector<int64_t>& fu, const std::vector<int64_t>& fd,
+                               int64_t q) {
+  PEPS_CHECK(T.rank() == 5, PS_ESHAPE, "mask: rank-5 tensor expected");
+  MaskArgs a{};
+  for (int i = 0; i < 5; ++i) a.d[i] = T.dims[i];
+  PEPS_CHECK(int64_t(nf.size()) == T.dims[0] && int64_t(fl.size()) == T.dims[1] &&
+                 int64_t(fr.size()) == T.dims[2] && int64_t(fu.size()) == T.dims[3] &&
+                 int64_t(fd.size()) == T.dims[4],
+             PS_ESHAPE, "mask: label length mismatch");
+  PEPS_CHECK(T.dims[0] <= WMAX && T.dims[1] <= WMAX && T.dims[2] <= WMAX && T.dims[3] <= WMAX &&
+                 T.dims[4] <= WMAX,
+             PS_EUNSUPPORTED, "mask: dimension > 64");
+  a.q = q;
+  for (size_t i = 0; i < nf.size(); ++i) { a.nf[i] = nf[i]; a.lR[i] = lR[i]; a.lU[i] = lU[i]; }
+  for (size_t i = 0; i < fl.size(); ++i) a.fl[i] = fl[i];
+  for (size_t i = 0; i < fr.size(); ++i) a.fr[i] = fr[i];
+  for (size_t i = 0; i < fu.size(); ++i) a.fu[i] = fu[i];
+  for (size_t i = 0; i < fd.size(); ++i) a.fd[i] = fd[i];
+  return a;
+}
+
+void gauss_mask(Ctx& c, Tensor& T, const std::vector<int64_t>& nf, const std::vector<int64_t>& lR,
+                const std::vector<int64_t>& lU, const std::vector<int64_t>& fl,
+                const std::vector<int64_t>& fr, const std::vector<int64_t>& fu,
+                const std::vector<int64_t>& fd, int64_t q) {
+  MaskArgs a = make_mask_args(T, nf, lR, lU, fl, fr, fu, fd, q);
+  int64_t n = T.numel();
+  if (!n) return;
+  c.count("elementwise", 0, 2.0 * T.bytes());
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(mask_kernel, dim3(g), dim3(256), 0, c.stream, T.data(), T.cplx ? 2 : 1, n, a);
+  PEPS_HIP(hipGetLastError());
+}
+
+double gauss_violation(Ctx& c, const Tensor& T, const std::vector<int64_t>& nf,
+                       const std::vector<int64_t>& lR, const std::vector<int64_t>& lU,
+                       const std::vector<int64_t>& fl, const std::vector<int64_t>& fr,
+                       const std::vector<int64_t>& fu, const std::vector<int64_t>& fd, int64_t q) {
+  MaskArgs a = make_mask_args(T, nf, lR, lU, fl, fr, fu, fd, q);
+  int64_t n = T.numel();
+  if (!n) return 0.0;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 1024));
+  Tensor part = c.alloc({int64_t(g)}, false);
+  hipLaunchKernelGGL(mask_viol_kernel, dim3(g), dim3(256), 0, c.stream, T.data(),
+                     T.cplx ? 2 : 1, n, a, part.data());
+  PEPS_HIP(hipGetLastError());
+  auto h = c.to_host(part);
+  double s = 0.0;
+  for (double x : h) s += x;
+  return s;
+}
+
+// ── reductions ───────────────────────────────────────────────────────────────
+__global__ void dot_kernel(const double* a, const double* b, int64_t n, int cplx, int conjb,
+                           double* part) {
+  __shared__ double sre[256], sim[256];
+  double re = 0.0, im = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (cplx) {
+      double ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+      if (conjb) bi = -bi;
+      re += ar * br - ai * bi;
+      im += ar * bi + ai * br;
+    } else {
+      re += a[i] * b[i];
+    }
+  }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) {
+      sre[threadIdx.x] += sre[threadIdx.x + o];
+      sim[threadIdx.x] += sim[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sre[0];
+    part[2 * blockIdx.x + 1] = sim[0];
+  }
+}
+
+cplx dot(Ctx& c, const Tensor& a, const Tensor& b, bool conj_b) {
+  PEPS_CHECK(a.numel() == b.numel() && a.cplx == b.cplx, PS_ESHAPE, "dot: shape mismatch");
+  int64_t n = a.numel();
+  if (!n) return cplx{0, 0};
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 1024));
+  Tensor part = c.alloc({2 * int64_t(g)}, false);
+  c.count("reduce", 0, 2.0 * a.bytes());
+  hipLaunchKernelGGL(dot_kernel, dim3(g), dim3(256), 0, c.stream, a.data(), b.data(), n,
+                     a.cplx ? 1 : 0, conj_b ? 1 : 0, part.data());
+  PEPS_HIP(hipGetLastError());
+  auto h = c.to_host(part);
+  cplx s{0, 0};
+  for (unsigned i = 0; i < g; ++i) {
+    s.re += h[2 * i];
+    s.im += h[2 * i + 1];
+  }
+  return s;
+}
+
+double norm2sq(Ctx& c, const Tensor& a) { return dot(c, a, a, true).re; }
+
+// one wave per column: first row index of max |M|^2
+__global__ void col_argmax_kernel(const double* M, int64_t nrow, int64_t ncol, int cplx,
+                                  int64_t* out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t col = blockIdx.x * int64_t(blockDim.x / 64) + (threadIdx.x >> 6);
+  if (col >= ncol) return;
+  double best = -1.0;
+  int64_t bi = 0;
+  for (int64_t r = lane; r < nrow; r += 64) {
+    int64_t off = r + col * nrow;
+    double v = cplx ? (M[2 * off] * M[2 * off] + M[2 * off + 1] * M[2 * off + 1]) : M[off] * M[off];
+    if (v > best) {
+      best = v;
+      bi = r;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double ob = __shfl_xor(best, o);
+    int64_t oi = __shfl_xor(bi, o);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  if (lane == 0) out[col] = bi;
+}
+
+std::vector<int64_t> col_argmax(Ctx& c, const Tensor& M) {
+  PEPS_CHECK(M.rank() == 2, PS_ESHAPE, "col_argmax: matrix expected");
+  int64_t nrow = M.dims[0], ncol = M.dims[1];
+  std::vector<int64_t> h(ncol);
+  if (!ncol) return h;
+  auto buf = std::make_shared<Buffer>();
+  buf->pool = c.pool.get();
+  buf->bytes = size_t(ncol) * 8;
+  buf->ptr = c.pool->get(buf->bytes);
+  hipLaunchKernelGGL(col_argmax_kernel, dim3(cdiv(ncol, 4)), dim3(256), 0, c.stream, M.data(),
+                     nrow, ncol, M.cplx ? 1 : 0, static_cast<int64_t*>(buf->ptr));
+  PEPS_HIP(hipGetLastError());
+  PEPS_HIP(hipMemcpyAsync(h.data(), buf->ptr, buf->bytes, hipMemcpyDeviceToHost, c.stream));
+  PEPS_HIP(hipStreamSynchronize(c.stream));
+  return h;
+}
+
+__global__ void cross_zero_kernel(double* M, int64_t nrow, int64_t ncol, int esz,
+                                  const int64_t* rq, const int64_t* cq) {
+  int64_t n = nrow * ncol;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i % nrow, cc = i / nrow;
+    if (rq[r] != cq[cc]) {
+      M[i * esz] = 0.0;
+      if (esz == 2) M[i * esz + 1] = 0.0;
+    }
+  }
+}
+
+void zero_cross_sector(Ctx& c, Tensor& M, const std::vector<int64_t>& rq,
+                       const std::vector<int64_t>& cq) {
+  int64_t nrow = M.dims[0], ncol = M.dims[1];
+  PEPS_CHECK(int64_t(rq.size()) == nrow && int64_t(cq.size()) == ncol, PS_ESHAPE, "cross: labels");
+  std::shared_ptr<Buffer> k1, k2;
+  auto* drq = (const int64_t*)c.scratch_upload(rq.data(), rq.size() * 8, &k1);
+  auto* dcq = (const int64_t*)c.scratch_upload(cq.data(), cq.size() * 8, &k2);
+  unsigned g = unsigned(std::min<int64_t>(cdiv(nrow * ncol, 256), 16384));
+  hipLaunchKernelGGL(cross_zero_kernel, dim3(g), dim3(256), 0, c.stream, M.data(), nrow, ncol,
+                     M.cplx ? 2 : 1, drq, dcq);
+  PEPS_HIP(hipGetLastError());
+}
+
+__global__ void gather2d_kernel(const double* M, int64_t nrow, double* out, int64_t nr,
+                                int64_t nc, int esz, const int64_t* rows, const int64_t* cols) {
+  int64_t n = nr * nc;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i % nr, cc = i / nr;
+    int64_t src = rows[r] + cols[cc] * nrow;
+    out[i * esz] = M[src * esz];
+    if (esz == 2) out[i * esz + 1] = M[src * esz + 1];
+  }
+}
+
+Tensor gather2d(Ctx& c, const Tensor& M, const std::vector<int64_t>& rows,
+                const std::vector<int64_t>& cols) {
+  Tensor out = c.alloc({int64_t(rows.size()), int64_t(cols.size())}, M.cplx);
+  if (rows.empty() || cols.empty()) return out;
+  std::shared_ptr<Buffer> k1, k2;
+  auto* dr = (const int64_t*)c.scratch_upload(rows.data(), rows.size() * 8, &k1);
+  auto* dc = (const int64_t*)c.scratch_upload(cols.data(), cols.size() * 8, &k2);
+  int64_t n = int64_t(rows.size() * cols.size());
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(gather2d_kernel, dim3(g), dim3(256), 0, c.stream, M.data(), M.dims[0],
+                     out.data(), int64_t(rows.size()), int64_t(cols.size()), M.cplx ? 2 : 1, dr, dc);
+  PEPS_HIP(hipGetLastError());
+  return out;
+}
+
+// dst(rows[i], dst_cols[j]) = src(i, src_cols[j])
+__global__ void scatter_cols_kernel(double* dst, int64_t dnrow, const double* src, int64_t snrow,
+                                    int64_t nr, int64_t nc, int esz, const int64_t* rows,
+                                    const int64_t* scols, const int64_t* dcols) {
+  int64_t n = nr * nc;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i % nr, j = i / nr;
+    int64_t so = r + scols[j] * snrow;
+    int64_t d = rows[r] + dcols[j] * dnrow;
+    dst[d * esz] = src[so * esz];
+    if (esz == 2) dst[d * esz + 1] = src[so * esz + 1];
+  }
+}
+
+void scatter_cols(Ctx& c, Tensor& dst, const Tensor& src, const std::vector<int64_t>& rows,
+                  const std::vector<int64_t>& src_cols, const std::vector<int64_t>& dst_cols) {
+  PEPS_CHECK(src_cols.size() == dst_cols.size(), PS_EARG, "scatter_cols");
+  if (rows.empty() || src_cols.empty()) return;
+  std::shared_ptr<Buffer> k1, k2, k3;
+  auto* dr = (const int64_t*)c.scratch_upload(rows.data(), rows.size() * 8, &k1);
+  auto* ds = (const int64_t*)c.scratch_upload(src_cols.data(), src_cols.size() * 8, &k2);
+  auto* dd = (const int64_t*)c.scratch_upload(dst_cols.data(), dst_cols.size() * 8, &k3);
+  int64_t n = int64_t(rows.size() * src_cols.size());
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(scatter_cols_kernel, dim3(g), dim3(256), 0, c.stream, dst.data(), dst.dims[0],
+                     src.data(), src.dims[0], int64_t(rows.size()), int64_t(src_cols.size()),
+                     dst.cplx ? 2 : 1, dr, ds, dd);
+  PEPS_HIP(hipGetLastError());
+}
+
+// dst(dst_rows[i], cols[j]) = src(src_rows[i], j)
+__global__ void scatter_rows_kernel(double* dst, int64_t dnrow, const double* src, int64_t snrow,
+                                    int64_t nr, int64_t nc, int esz, const int64_t* srows,
+                                    const int64_t* drows, const int64_t* cols) {
+  int64_t n = nr * nc;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i % nr, j = i / nr;
+    int64_t so = srows[r] + j * snrow;
+    int64_t d = drows[r] + cols[j] * dnrow;
+    dst[d * esz] = src[so * esz];
+    if (esz == 2) dst[d * esz + 1] = src[so * esz + 1];
+  }
+}
+
+void scatter_rows(Ctx& c, Tensor& dst, const Tensor& src, const std::vector<int64_t>& src_rows,
+                  const std::vector<int64_t>& dst_rows, const std::vector<int64_t>& cols) {
+  PEPS_CHECK(src_rows.size() == dst_rows.size(), PS_EARG, "scatter_rows");
+  if (src_rows.empty() || cols.empty()) return;
+  std::shared_ptr<Buffer> k1, k2, k3;
+  auto* ds = (const int64_t*)c.scratch_upload(src_rows.data(), src_rows.size() * 8, &k1);
+  auto* dd = (const int64_t*)c.scratch_upload(dst_rows.data(), dst_rows.size() * 8, &k2);
+  auto* dc = (const int64_t*)c.scratch_upload(cols.data(), cols.size() * 8, &k3);
+  int64_t n = int64_t(src_rows.size() * cols.size());
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(scatter_rows_kernel, dim3(g), dim3(256), 0, c.stream, dst.data(), dst.dims[0],
+                     src.data(), src.dims[0], int64_t(src_rows.size()), int64_t(cols.size()),
+                     dst.cplx ? 2 : 1, ds, dd, dc);
+  PEPS_HIP(hipGetLastError());
+}
+
+__global__ void mask_x_kernel(double* X, int64_t nA, int64_t dp, int64_t Dp, int esz,
+                              const int8_t* keep) {
+  int64_t n = nA * dp * Dp;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i / nA;
+    int64_t p = r % dp, b = r / dp;
+    if (!keep[p + b * dp]) {
+      X[i * esz] = 0.0;
+      if (esz == 2) X[i * esz + 1] = 0.0;
+    }
+  }
+}
+
+void mask_x(Ctx& c, Tensor& X, const std::vector<int8_t>& keep) {
+  PEPS_CHECK(X.rank() == 3 && int64_t(keep.size()) == X.dims[1] * X.dims[2], PS_ESHAPE, "mask_x");
+  std::shared_ptr<Buffer> k1;
+  auto* dk = (const int8_t*)c.scratch_upload(keep.data(), keep.size(), &k1);
+  int64_t n = X.numel();
+  if (!n) return;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(mask_x_kernel, dim3(g), dim3(256), 0, c.stream, X.data(), X.dims[0],
+                     X.dims[1], X.dims[2], X.cplx ? 2 : 1, dk);
+  PEPS_HIP(hipGetLastError());
+}
+
+__global__ void upper_trap_kernel(const double* A, int64_t m, double* R, int64_t k, int64_t n,
+                                  int esz) {
+  int64_t tot = k * n;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < tot;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i % k, cc = i / k;
+    double re = 0.0, im = 0.0;
+    if (r <= cc) {
+      re = A[(r + cc * m) * esz];
+      if (esz == 2) im = A[(r + cc * m) * esz + 1];
+    }
+    R[i * esz] = re;
+    if (esz == 2) R[i * esz + 1] = im;
+  }
+}
+
+Tensor upper_trapezoid(Ctx& c, const Tensor& A, int64_t k) {
+  int64_t m = A.dims[0], n = A.dims[1];
+  Tensor R = c.alloc({k, n}, A.cplx);
+  if (!k || !n) return R;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(k * n, 256), 16384));
+  hipLaunchKernelGGL(upper_trap_kernel, dim3(g), dim3(256), 0, c.stream, A.data(), m, R.data(), k,
+                     n, A.cplx ? 2 : 1);
+  PEPS_HIP(hipGetLastError());
+  return R;
+}
+
+__global__ void scale_rc_kernel(double* M, int64_t nrow, int64_t ncol, int esz, const double* s,
+                                int by_col) {
+  int64_t n = nrow * ncol;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double f = by_col ? s[i / nrow] : s[i % nrow];
+    M[i * esz] *= f;
+    if (esz == 2) M[i * esz + 1] *= f;
+  }
+}
+
+static void scale_rc(Ctx& c, Tensor& M, const std::vector<double>& s, bool by_col) {
+  int64_t nrow = M.dims[0], ncol = M.numel() / std::max<int64_t>(1, nrow);
+  PEPS_CHECK(int64_t(s.size()) == (by_col ? ncol : nrow), PS_ESHAPE, "scale_rc: length");
+  if (!M.numel()) return;
+  std::shared_ptr<Buffer> k1;
+  auto* ds = (const double*)c.scratch_upload(s.data(), s.size() * 8, &k1);
+  unsigned g = unsigned(std::min<int64_t>(cdiv(M.numel(), 256), 16384));
+  hipLaunchKernelGGL(scale_rc_kernel, dim3(g), dim3(256), 0, c.stream, M.data(), nrow, ncol,
+                     M.cplx ? 2 : 1, ds, by_col ? 1 : 0);
+  PEPS_HIP(hipGetLastError());
+}
+void scale_cols(Ctx& c, Tensor& M, const std::vector<double>& s) { scale_rc(c, M, s, true); }
+void scale_rows(Ctx& c, Tensor& M, const std::vector<double>& s) { scale_rc(c, M, s, false); }
+
+__global__ void hermitize_kernel(double* N, int64_t n, int esz) {
+  int64_t tot = n * n;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < tot;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i % n, cc = i / n;
+    if (r > cc) continue;
+    int64_t j = cc + r * n;
+    if (esz == 1) {
+      double v = 0.5 * (N[i] + N[j]);
+      N[i] = v;
+      N[j] = v;
+    } else {
+      double re = 0.5 * (N[2 * i] + N[2 * j]);
+      double im = 0.5 * (N[2 * i + 1] - N[2 * j + 1]);
+      N[2 * i] = re;
+      N[2 * i + 1] = im;
+      N[2 * j] = re;
+      N[2 * j + 1] = -im;
+    }
+  }
+}
+void hermitize(Ctx& c, Tensor& N) {
+  int64_t n = N.dims[0];
+  PEPS_CHECK(N.rank() == 2 && N.dims[1] == n, PS_ESHAPE, "hermitize: square matrix expected");
+  if (!n) return;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n * n, 256), 16384));
+  hipLaunchKernelGGL(hermitize_kernel, dim3(g), dim3(256), 0, c.stream, N.data(), n,
+                     N.cplx ? 2 : 1);
+  PEPS_HIP(hipGetLastError());
+}
+
+Tensor copy_tensor(Ctx& c, const Tensor& t) {
+  Tensor o = c.alloc(t.dims, t.cplx);
+  if (t.bytes())
+    PEPS_HIP(hipMemcpyAsync(o.data(), t.data(), t.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  return o;
+}
+
+__global__ void finite_kernel(const double* x, int64_t n, int* bad) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    if (!isfinite(x[i])) *bad = 1;
+}
+bool all_finite(Ctx& c, const Tensor& t) {
+  int64_t n = t.numel() * (t.cplx ? 2 : 1);
+  if (!n) return true;
+  auto buf = std::make_shared<Buffer>();
+  buf->pool = c.pool.get();
+  buf->bytes = 8;
+  buf->ptr = c.pool->get(8);
+  PEPS_HIP(hipMemsetAsync(buf->ptr, 0, 4, c.stream));
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 4096));
+  hipLaunchKernelGGL(finite_kernel, dim3(g), dim3(256), 0, c.stream, t.data(), n, (int*)buf->ptr);
+  PEPS_HIP(hipGetLastError());
+  int h = 0;
+  PEPS_HIP(hipMemcpyAsync(&h, buf->ptr, 4, hipMemcpyDeviceToHost, c.stream));
+  PEPS_HIP(hipStreamSynchronize(c.stream));
+  return h == 0;
+}
+
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Hermitian eigensolver: parallel cyclic Jacobi, one workgroup per matrix
+// ════════════════════════════════════════════════════════════════════════════
+//
+// Round-robin ("circle") ordering: m = n rounded up to even, m/2 disjoint
+// (p,q) pairs per round, m-1 rounds per sweep.  Each pair's 2x2 problem
+// [[a, b],[conj b, d]] is zeroed by J = diag(1, conj(e)) R(c, s) with
+// e = b/|b| and the real Jacobi rotation t = sign(th)/(|th|+sqrt(th^2+1)),
+// th = (d-a)/(2|b|) (Golub & Van Loan, Alg. 8.5.2).  A <- A J (columns),
+// A <- J^H A (rows), V <- V J.  Sweeps stop when the off-diagonal
+// Frobenius norm falls below tol * ||A||_F.  A, V live in global memory.
+
+namespace {
+template <bool CP> struct JT;
+template <> struct JT<false> {
+  typedef double T;
+  static __device__ double re(T x) { return x; }
+  static __device__ double abs2(T x) { return x * x; }
+};
+template <> struct JT<true> {
+  typedef double2 T;
+  static __device__ double re(T x) { return x.x; }
+  static __device__ double abs2(T x) { return x.x * x.x + x.y * x.y; }
+};
+__device__ __forceinline__ double mulc(double a, double b) { return a * b; }
+__device__ __forceinline__ double2 mulc(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double conjv(double a) { return a; }
+__device__ __forceinline__ double2 conjv(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double scl(double a, double s) { return a * s; }
+__device__ __forceinline__ double2 scl(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double addv(double a, double b) { return a + b; }
+__device__ __forceinline__ double2 addv(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double unitv(double b, double ab) { return ab > 0 ? b / ab : 1.0; }
+__device__ __forceinline__ double2 unitv(double2 b, double ab) {
+  return ab > 0 ? make_double2(b.x / ab, b.y / ab) : make_double2(1.0, 0.0);
+}
+__device__ __forceinline__ void set_one(double& x) { x = 1.0; }
+__device__ __forceinline__ void set_one(double2& x) { x = make_double2(1.0, 0.0); }
+__device__ __forceinline__ void set_zero(double& x) { x = 0.0; }
+__device__ __forceinline__ void set_zero(double2& x) { x = make_double2(0.0, 0.0); }
+}  // namespace
+
+constexpr int JMAXP = 256;  // max pairs per round (n <= 512)
+
+template <bool CP>
+__global__ __launch_bounds__(1024) void jacobi_eigh_kernel(typename JT<CP>::T* __restrict__ Ab,
+                                                           typename JT<CP>::T* __restrict__ Vb,
+                                                           double* __restrict__ wb, int n,
+                                                           int64_t stride, double tol,
+                                                           int max_sweeps, int* __restrict__ sweeps_out) {
+  typedef typename JT<CP>::T T;
+  T* A = Ab + int64_t(blockIdx.x) * stride;
+  T* V = Vb + int64_t(blockIdx.x) * stride;
+  double* w = wb + int64_t(blockIdx.x) * n;
+  __shared__ double cs_c[JMAXP], cs_s[JMAXP];
+  __shared__ T cs_e[JMAXP];
+  __shared__ int pp[JMAXP], qq[JMAXP];
+  __shared__ double red[1024];
+  __shared__ int done;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int m = (n + 1) & ~1;
+  const int half = m / 2;
+  const int64_t nn = int64_t(n) * n;
+  for (int64_t i = tid; i < nn; i += nt) {
+    int64_t r = i % n, c = i / n;
+    if (r == c) set_one(V[i]); else set_zero(V[i]);
+  }
+  __syncthreads();
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    // convergence test: off(A)^2 <= tol^2 * ||A||^2
+    double off = 0.0, tot = 0.0;
+    for (int64_t i = tid; i < nn; i += nt) {
+      double a2 = JT<CP>::abs2(A[i]);
+      tot += a2;
+      if (i % n != i / n) off += a2;
+    }
+    red[tid] = off;
+    __syncthreads();
+    for (int o = nt / 2; o > 0; o >>= 1) {
+      if (tid < o) red[tid] += red[tid + o];
+      __syncthreads();
+    }
+    double offs = red[0];
+    __syncthreads();
+    red[tid] = tot;
+    __syncthreads();
+    for (int o = nt / 2; o > 0; o >>= 1) {
+      if (tid < o) red[tid] += red[tid + o];
+      __syncthreads();
+    }
+    double tots = red[0];
+    if (tid == 0) done = (offs <= tol * tol * tots) || offs == 0.0;
+    __syncthreads();
+    if (done) break;
+    for (int rnd = 0; rnd < m - 1; ++rnd) {
+      // pairs of this round
+      for (int k = tid; k < half; k += nt) {
+        int i0 = k, i1 = m - 1 - k;
+        int p = (i0 == 0) ? 0 : ((i0 - 1 + rnd) % (m - 1)) + 1;
+        int q = ((i1 - 1 + rnd) % (m - 1)) + 1;
+        if (p > q) { int t = p; p = q; q = t; }
+        double c = 1.0, s = 0.0;
+        T e;
+        set_one(e);
+        if (q < n) {
+          T b = A[p + int64_t(q) * n];
+          double ab = sqrt(JT<CP>::abs2(b));
+          double a = JT<CP>::re(A[p + int64_t(p) * n]), d = JT<CP>::re(A[q + int64_t(q) * n]);
+          if (ab > 1e-300 * (fabs(a) + fabs(d) + 1e-300)) {
+            double th = (d - a) / (2.0 * ab);
+            double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+            e = unitv(b, ab);
+          }
+        }
+        cs_c[k] = c; cs_s[k] = s; cs_e[k] = e; pp[k] = p; qq[k] = q;
+      }
+      __syncthreads();
+      // columns: A <- A J, V <- V J
+      for (int64_t idx = tid; idx < int64_t(half) * n; idx += nt) {
+        int k = int(idx / n), i = int(idx % n);
+        int p = pp[k], q = qq[k];
+        if (q >= n || cs_s[k] == 0.0) continue;
+        double c = cs_c[k], s = cs_s[k];
+        T ec = conjv(cs_e[k]);
+        T aip = A[i + int64_t(p) * n], aiq = A[i + int64_t(q) * n];
+        T aq = mulc(aiq, ec);
+        A[i + int64_t(p) * n] = addv(scl(aip, c), scl(aq, -s));
+        A[i + int64_t(q) * n] = addv(scl(aip, s), scl(aq, c));
+        T vip = V[i + int64_t(p) * n], viq = V[i + int64_t(q) * n];
+        T vq = mulc(viq, ec);
+        V[i + int64_t(p) * n] = addv(scl(vip, c), scl(vq, -s));
+        V[i + int64_t(q) * n] = addv(scl(vip, s), scl(vq, c));
+      }
+      __syncthreads();
+      // rows: A <- J^H A
+      for (int64_t idx = tid; idx < int64_t(half) * n; idx += nt) {
+        int k = int(idx / n), j = int(idx % n);
+        int p = pp[k], q = qq[k];
+        if (q >= n || cs_s[k] == 0.0) continue;
+        double c = cs_c[k], s = cs_s[k];
+        T e = cs_e[k];
+        T apj = A[p + int64_t(j) * n], aqj = A[q + int64_t(j) * n];
+        T aq = mulc(aqj, e);
+        A[p + int64_t(j) * n] = addv(scl(apj, c), scl(aq, -s));
+        A[q + int64_t(j) * n] = addv(scl(apj, s), scl(aq, c));
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < n; i += nt) w[i] = JT<CP>::re(A[i + int64_t(i) * n]);
+  if (tid == 0 && sweeps_out) sweeps_out[blockIdx.x] = sweep;
+}
+
+// Batched Hermitian eigendecomposition of `nb` n x n matrices stored
+// contiguously (A is overwritten).  Returns the sweep count of matrix 0.
+int jacobi_eigh(Ctx& c, Tensor& A, Tensor& V, Tensor& w, int n, int nb, double tol, int max_sweeps) {
+  PEPS_CHECK(n >= 1 && n <= 2 * JMAXP, PS_EUNSUPPORTED, "jacobi_eigh: n must be in [1, 512]");
+  PEPS_CHECK(A.numel() == int64_t(n) * n * nb && V.numel() == A.numel() && w.numel() == int64_t(n) * nb &&
+                 V.cplx == A.cplx && !w.cplx,
+             PS_ESHAPE, "jacobi_eigh: buffer sizes");
+  auto buf = std::make_shared<Buffer>();
+  buf->pool = c.pool.get();
+  buf->bytes = size_t(nb) * sizeof(int);
+  buf->ptr = c.pool->get(buf->bytes);
+  int threads = n <= 32 ? 64 : (n <= 128 ? 256 : 1024);
+  hipEvent_t ev = nullptr;
+  c.tbegin("eigh", &ev);
+  c.count("eigh", (A.cplx ? 4.0 : 1.0) * 6.0 * 8.0 * double(n) * n * n * nb, 0);
+  if (A.cplx)
+    hipLaunchKernelGGL(jacobi_eigh_kernel<true>, dim3(nb), dim3(threads), 0, c.stream,
+                       (double2*)A.data(), (double2*)V.data(), w.data(), n, int64_t(n) * n, tol,
+                       max_sweeps, (int*)buf->ptr);
+  else
+    hipLaunchKernelGGL(jacobi_eigh_kernel<false>, dim3(nb), dim3(threads), 0, c.stream, A.data(),
+                       V.data(), w.data(), n, int64_t(n) * n, tol, max_sweeps, (int*)buf->ptr);
+  PEPS_HIP(hipGetLastError());
+  c.tend("eigh", ev);
+  int sw = 0;
+  PEPS_HIP(hipMemcpyAsync(&sw, buf->ptr, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  PEPS_HIP(hipStreamSynchronize(c.stream));
+  return sw;
+}
+
+}  // namespace peps

@@ -1,0 +1,41 @@
+// U(1) lattice-gauge model on an open nx x ny lattice: site Hilbert spaces,
+// Gauss-law labels and the merged two-site Hamiltonians / Trotter gates.
+//
+// Flux labels are integers l = 2E.  A link of dimension gd carries
+// l in {-(gd-1), ..., gd-1} step 2; a missing boundary link carries b0
+// (0 for odd gd, +1 for even gd).  Gauss law: lR + lU - lL - lD = 2 (n_f + q).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace peps {
+
+struct SiteDims {
+  int d, dR, dU;
+};
+
+SiteDims site_dims(int ix, int iy, int nx, int ny, int gd);
+int64_t boundary_label(int gd);
+// decode_phys for a 1-based physical index p: (n_f, lR, lU)
+void decode_phys(int p, int dR, int dU, int gd, int64_t* nf, int64_t* lR, int64_t* lU);
+void phys_labels(int ix, int iy, int nx, int ny, int gd, std::vector<int64_t>* nf,
+                 std::vector<int64_t>* lR, std::vector<int64_t>* lU);
+
+// Dense column-major real matrices (the model's operators are real).
+struct Mat {
+  int n = 0;
+  std::vector<double> a;  // n*n, column-major
+  double& operator()(int i, int j) { return a[i + size_t(j) * n]; }
+  double operator()(int i, int j) const { return a[i + size_t(j) * n]; }
+};
+
+Mat H_onsite_site(int ix, int iy, int nx, int ny, int gd, double g, double m, double mu = 0.0);
+Mat H_hop_h_site(int ix, int iy, int nx, int ny, int gd, double t);
+Mat H_hop_v_site(int ix, int iy, int nx, int ny, int gd, double t);
+double onsite_weight(int ix, int iy, int nx, int ny);
+Mat H_merged_h_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m);
+Mat H_merged_v_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m);
+// exp(-tau H) for real symmetric H (Jacobi eigendecomposition, FP64).
+Mat expm_sym(const Mat& H, double tau);
+
+}  // namespace peps

@@ -1,0 +1,251 @@
+// Internal declarations of the MI355X finite-PEPS engine (libpeps_hip.so).
+//
+// Storage convention: every tensor is column-major with the first index
+// fastest, exactly like the Julia reference arrays, and holds either real
+// FP64 (8 B/entry) or complex FP64 interleaved (re,im; 16 B/entry, byte
+// compatible with Julia ComplexF64 / C double _Complex).  The contraction
+// precision (FP64 / FP32 / BF16 MFMA) is a property of the context, not of
+// the storage: GEMM tiles are converted while being staged into LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace peps {
+
+// ── status codes (mirror include/peps.h) ─────────────────────────────────────
+enum Status : int {
+  PS_OK = 0,
+  PS_EARG = 1,
+  PS_ESHAPE = 2,
+  PS_ENOSECTOR = 3,
+  PS_EHIP = 4,
+  PS_ENOMEM = 5,
+  PS_ESOLVER = 6,
+  PS_EUNSUPPORTED = 7,
+};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PEPS_HIP(call)                                                         \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw ::peps::Error(::peps::PS_EHIP, std::string(#call) + ": " +           \
+                                            hipGetErrorString(e_));           \
+  } while (0)
+
+#define PEPS_CHECK(cond, code, msg)                                            \
+  do {                                                                         \
+    if (!(cond)) throw ::peps::Error((code), (msg));                           \
+  } while (0)
+
+struct cplx {
+  double re, im;
+};
+
+// Contraction precision tiers.
+enum Tier : int { TIER_FP64 = 0, TIER_FP32 = 1, TIER_BF16 = 2 };
+
+// ── device memory: a caching allocator (no hipMalloc on the hot path) ────────
+class Pool {
+ public:
+  explicit Pool(int device) : device_(device) {}
+  ~Pool();
+  void* get(size_t bytes);
+  void put(void* p, size_t bytes);
+  void trim();
+  size_t bytes_in_use() const { return in_use_; }
+  size_t bytes_cached() const { return cached_; }
+
+ private:
+  int device_;
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> sizes_;
+  size_t in_use_ = 0, cached_ = 0;
+};
+
+struct Buffer {
+  Pool* pool = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  ~Buffer() {
+    if (pool && ptr) pool->put(ptr, bytes);
+  }
+};
+
+class Ctx;
+
+// A column-major device tensor.
+struct Tensor {
+  std::shared_ptr<Buffer> buf;
+  std::vector<int64_t> dims;
+  bool cplx = false;
+  int64_t off = 0;  // element offset into buf (views/slices)
+
+  int64_t numel() const {
+    int64_t n = 1;
+    for (auto d : dims) n *= d;
+    return n;
+  }
+  int esz() const { return cplx ? 16 : 8; }
+  size_t bytes() const { return size_t(numel()) * esz(); }
+  int rank() const { return int(dims.size()); }
+  int64_t dim(int i) const { return dims.at(i); }
+  double* data() const {
+    return buf ? static_cast<double*>(buf->ptr) + off * (cplx ? 2 : 1) : nullptr;
+  }
+  bool empty() const { return !buf; }
+  // Reinterpret the same storage with other column-major dimensions.
+  Tensor reshaped(std::vector<int64_t> nd) const;
+  // View of the index range [i0, i1) of the last (slowest) dimension.
+  Tensor slice_last(int64_t i0, int64_t i1) const;
+};
+
+struct KernelStat {
+  double flops = 0;    // algorithmic flops (real-equivalent)
+  double bytes = 0;    // algorithmic HBM bytes
+  double ms = 0;       // summed launch duration (HIP events), when timing is on
+  int64_t launches = 0;
+};
+
+class Ctx {
+ public:
+  explicit Ctx(int device);
+  ~Ctx();
+
+  int device;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::unique_ptr<Pool> pool;
+  Tier tier = TIER_FP64;
+  std::string last_error;
+  int64_t warnings = 0;
+  int ncu = 256;
+
+  // Per kernel-class accounting ("gemm", "permute", "svd", ...).
+  std::map<std::string, KernelStat> stats;
+  bool timing = false;
+  struct PendingEv {
+    std::string cls;
+    hipEvent_t a, b;
+  };
+  std::vector<PendingEv> pending;
+  std::vector<hipEvent_t> ev_free;
+
+  // Begin/end a timed region for kernel class `cls` (no-ops unless timing).
+  void tbegin(const std::string& cls, hipEvent_t* a);
+  void tend(const std::string& cls, hipEvent_t a);
+  void resolve_timing();
+  void count(const std::string& cls, double flops, double bytes) {
+    auto& s = stats[cls];
+    s.flops += flops;
+    s.bytes += bytes;
+    s.launches += 1;
+  }
+
+  Tensor alloc(std::vector<int64_t> dims, bool cplx);
+  Tensor zeros(std::vector<int64_t> dims, bool cplx);
+  Tensor upload(std::vector<int64_t> dims, bool cplx, const double* host);
+  void download(const Tensor& t, double* host);
+  std::vector<double> to_host(const Tensor& t);
+  void sync();
+
+  // Device scratch for small host→device uploads (label tables, weights).
+  void* scratch_upload(const void* host, size_t bytes, std::shared_ptr<Buffer>* keep);
+};
+
+// ── kernels (kernels.hip) ────────────────────────────────────────────────────
+// out = permute(in, perm): out.dims[i] = in.dims[perm[i]] (0-based perm).
+Tensor permute(Ctx& c, const Tensor& in, const std::vector<int>& perm);
+// C = op(A) * op(B), matrices column-major contiguous.
+//   A is (M x K) if !a_kfast (A[m + k*M]) or stored (K x M) if a_kfast (A[k + m*K])
+//   B is (K x N) if b_kfast (B[k + n*K]) or stored (N x K) if !b_kfast (B[n + k*N])
+void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, bool a_kfast,
+          bool conjA, const double* B, bool b_kfast, bool conjB, double* C, double beta);
+void scale_legs(Ctx& c, Tensor& T, const std::vector<double>* w /*[4]: l,r,u,d*/);
+void scale(Ctx& c, Tensor& T, double s);
+void axpy_inplace(Ctx& c, Tensor& Y, const Tensor& X, double a);  // Y += a X
+void conj_inplace(Ctx& c, Tensor& T);
+// Zero the entries of a (p,l,r,u,d) tensor that violate the Gauss law.
+void gauss_mask(Ctx& c, Tensor& T, const std::vector<int64_t>& nf, const std::vector<int64_t>& lR,
+                const std::vector<int64_t>& lU, const std::vector<int64_t>& fl,
+                const std::vector<int64_t>& fr, const std::vector<int64_t>& fu,
+                const std::vector<int64_t>& fd, int64_t q);
+// sum |entries of T outside the Gauss mask|^2
+double gauss_violation(Ctx& c, const Tensor& T, const std::vector<int64_t>& nf,
+                       const std::vector<int64_t>& lR, const std::vector<int64_t>& lU,
+                       const std::vector<int64_t>& fl, const std::vector<int64_t>& fr,
+                       const std::vector<int64_t>& fu, const std::vector<int64_t>& fd, int64_t q);
+// sum_i a_i * conj?(b_i), deterministic two-pass reduction, result (re, im).
+cplx dot(Ctx& c, const Tensor& a, const Tensor& b, bool conj_b);
+double norm2sq(Ctx& c, const Tensor& a);
+// M (nrow x ncol): per column, index of the first row with maximal |M|^2.
+std::vector<int64_t> col_argmax(Ctx& c, const Tensor& M);
+// M (nrow x ncol): zero entries with row_charge[r] != col_charge[c].
+void zero_cross_sector(Ctx& c, Tensor& M, const std::vector<int64_t>& rq,
+                       const std::vector<int64_t>& cq);
+// out(i,j) = M(rows[i], cols[j]).
+Tensor gather2d(Ctx& c, const Tensor& M, const std::vector<int64_t>& rows,
+                const std::vector<int64_t>& cols);
+// M(rows[i], col) = v(i) for column-vector sources; dst must be zeroed.
+void scatter_cols(Ctx& c, Tensor& dst, const Tensor& src, const std::vector<int64_t>& rows,
+                  const std::vector<int64_t>& src_cols, const std::vector<int64_t>& dst_cols);
+void scatter_rows(Ctx& c, Tensor& dst, const Tensor& src, const std::vector<int64_t>& src_rows,
+                  const std::vector<int64_t>& dst_rows, const std::vector<int64_t>& cols);
+// Zero X(aA, pL, beta) where keep(pL, beta) == 0.
+void mask_x(Ctx& c, Tensor& X, const std::vector<int8_t>& keep);
+// out = upper trapezoid of A (rows 0..k-1) as a dense (k x n) matrix.
+Tensor upper_trapezoid(Ctx& c, const Tensor& A, int64_t k);
+// Column scaling M(:, j) *= s[j]; row scaling M(i, :) *= s[i].
+void scale_cols(Ctx& c, Tensor& M, const std::vector<double>& s);
+void scale_rows(Ctx& c, Tensor& M, const std::vector<double>& s);
+// Hermitian part 0.5 (N + N^H) in place.
+void hermitize(Ctx& c, Tensor& N);
+// Copy a sub-block: dst(:, j0:j0+n) = src (column block).
+Tensor copy_tensor(Ctx& c, const Tensor& t);
+bool all_finite(Ctx& c, const Tensor& t);
+
+// ── contraction front end (contract.cpp) ────────────────────────────────────
+// Einstein-summation contraction of two tensors with single-character labels,
+// e.g. contract(c, A, "aub", B, "lrud", "ladrb").  Repeated labels on both
+// operands are summed; every output label appears in exactly one operand.
+Tensor contract(Ctx& c, const Tensor& A, const std::string& la, const Tensor& B,
+                const std::string& lb, const std::string& lc, bool conjA = false,
+                bool conjB = false);
+// Plain matrix product of column-major matrices: C = op(A) op(B).
+Tensor matmul(Ctx& c, const Tensor& A, const Tensor& B, bool conjA = false, bool conjB = false,
+              bool transA = false, bool transB = false);
+
+// ── dense linear algebra (linalg.cpp, rocSOLVER) ────────────────────────────
+struct QR {
+  Tensor Q, R;  // thin: Q (m x k), R (k x n), k = min(m, n)
+};
+QR qr_thin(Ctx& c, const Tensor& A);
+Tensor qr_r_only(Ctx& c, const Tensor& A);
+struct SVD {
+  Tensor U, Vt;            // U (m x k), Vt (k x n)
+  std::vector<double> S;   // descending
+};
+SVD svd_thin(Ctx& c, const Tensor& A);
+struct EIG {
+  std::vector<double> w;   // ascending
+  Tensor V;
+};
+EIG eigh(Ctx& c, const Tensor& A);
+// Batched parallel-Jacobi Hermitian eigensolver (kernels.hip): A (nb n x n
+// matrices, overwritten), V eigenvectors, w eigenvalues (unsorted).
+int jacobi_eigh(Ctx& c, Tensor& A, Tensor& V, Tensor& w, int n, int nb, double tol, int max_sweeps);
+
+}  // namespace peps

@@ -1,0 +1,94 @@
+"""Host-side lattice set-up: site dimensions, Gauss-law labels and the seeded
+random initial GaugePEPS used by the configs (small numpy arrays, uploaded
+once).  Mirrors u1_lgt_hamiltonian.jl:605-618, finite_peps_gauge_invariant.jl
+:58-67,109-136,180-218 with flux labels stored as integers 2E.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def boundary_label(gd):
+    """Label of a missing boundary link: 0 for odd gd (reference), +1 for even gd."""
+    return 0 if gd % 2 == 1 else 1
+
+
+def link_labels(gd):
+    return np.array([2 * i - (gd - 1) for i in range(gd)], dtype=np.int64)
+
+
+def site_dims(ix, iy, nx, ny, gd):
+    dR = gd if ix < nx else 1
+    dU = gd if iy < ny else 1
+    return 2 * dR * dU, dR, dU
+
+
+def phys_labels(ix, iy, nx, ny, gd):
+    """(n_f, lR, lU) of every physical index of site (ix, iy)."""
+    d, dR, dU = site_dims(ix, iy, nx, ny, gd)
+    p0 = np.arange(d)
+    nf = p0 // (dR * dU)
+    rem = p0 % (dR * dU)
+    b0 = boundary_label(gd)
+    lR = np.full(d, b0) if dR == 1 else 2 * (rem // dU) - (gd - 1)
+    lU = np.full(d, b0) if dU == 1 else 2 * (rem % dU) - (gd - 1)
+    return nf.astype(np.int64), np.asarray(lR, np.int64), np.asarray(lU, np.int64)
+
+
+def staggered_charges(nx, ny):
+    """g = -1 on odd ix+iy (gs_benchmark_v4.jl:46-52)."""
+    ix = np.arange(1, nx + 1)[:, None]
+    iy = np.arange(1, ny + 1)[None, :]
+    return np.where((ix + iy) % 2 == 1, -1, 0).astype(np.int64)
+
+
+def default_bond_labels(gd, D):
+    labs = sorted(link_labels(gd).tolist(), key=lambda x: (abs(x), x))
+    return np.array([labs[i % gd] for i in range(D)], dtype=np.int64)
+
+
+def gauss_mask(ix, iy, nx, ny, gd, q, fl, fr, fu, fd):
+    nf, lR, lU = phys_labels(ix, iy, nx, ny, gd)
+    P, R, U = nf[:, None, None, None, None], lR[:, None, None, None, None], lU[:, None, None, None, None]
+    L_ = np.asarray(fl)[None, :, None, None, None]
+    Rr = np.asarray(fr)[None, None, :, None, None]
+    Uu = np.asarray(fu)[None, None, None, :, None]
+    Dd = np.asarray(fd)[None, None, None, None, :]
+    return (Rr == R) & (Uu == U) & (R + U - L_ - Dd == 2 * (P + q))
+
+
+def random_gauge_peps_arrays(nx, ny, gd, gch, D, seed, complex_entries=False):
+    """Seeded random gauge-invariant PEPS at bond dimension D.
+
+    i.i.d. N(0,1) entries from numpy.random.default_rng(seed) (PCG64) drawn
+    site by site (iy outer, ix inner) over the full (p,l,r,u,d) array in
+    column-major order, Gauss-forbidden entries zeroed, each site normalised;
+    all weights 1, bond labels from default_bond_labels.
+    """
+    rng = np.random.default_rng(seed)
+    b0 = boundary_label(gd)
+    gch = np.asarray(gch).reshape(nx, ny)
+    lh = [[np.ones(D) for _ in range(ny)] for _ in range(nx - 1)]
+    lv = [[np.ones(D) for _ in range(ny - 1)] for _ in range(nx)]
+    qh = [[default_bond_labels(gd, D) for _ in range(ny)] for _ in range(nx - 1)]
+    qv = [[default_bond_labels(gd, D) for _ in range(ny - 1)] for _ in range(nx)]
+    tens = [[None] * ny for _ in range(nx)]
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d = site_dims(ix, iy, nx, ny, gd)[0]
+            shp = (d, D if ix > 1 else 1, D if ix < nx else 1, D if iy < ny else 1, D if iy > 1 else 1)
+            n = int(np.prod(shp))
+            v = rng.standard_normal(n)
+            if complex_entries:
+                v = v + 1j * rng.standard_normal(n)
+            T = np.reshape(v.astype(np.complex128 if complex_entries else np.float64), shp, order="F").copy()
+            fl = qh[ix - 2][iy - 1] if ix > 1 else [b0]
+            fr = qh[ix - 1][iy - 1] if ix < nx else [b0]
+            fu = qv[ix - 1][iy - 1] if iy < ny else [b0]
+            fd = qv[ix - 1][iy - 2] if iy > 1 else [b0]
+            T[~gauss_mask(ix, iy, nx, ny, gd, gch[ix - 1, iy - 1], fl, fr, fu, fd)] = 0
+            nrm = np.sqrt(np.sum(np.abs(T) ** 2))
+            if nrm > 1e-15:
+                T /= nrm
+            tens[ix - 1][iy - 1] = T
+    return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv, gch=gch, gd=gd)

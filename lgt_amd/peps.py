@@ -1,0 +1,547 @@
+"""Python host mirror of the reference's Julia calling surface over libpeps_hip.so.
+
+Function names and argument meaning follow finite_peps_boundary_mps.jl,
+finite_peps_fullupdate_v4.jl, finite_peps_gauge_invariant.jl and
+finite_peps_full_update.jl; every call goes through the C ABI in
+include/peps.h.  Arrays cross the boundary column-major (numpy order='F'),
+complex as interleaved complex128 — the byte layout of Julia ComplexF64.
+
+There is no CPU fallback: importing this module without a built
+libpeps_hip.so raises, and every failing call raises PepsError with the
+library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("PEPS_HIP_LIB", os.path.join(_HERE, "libpeps_hip.so"))
+
+PEPS_H, PEPS_V = 0, 1
+PEPS_STEP_V4, PEPS_STEP_GAUGE = 0, 1
+TIERS = {"fp64": 0, "fp32": 1, "bf16": 2}
+STATUS = {0: "PEPS_OK", 1: "PEPS_EARG", 2: "PEPS_ESHAPE", 3: "PEPS_ENOSECTOR", 4: "PEPS_EHIP",
+          5: "PEPS_ENOMEM", 6: "PEPS_ESOLVER", 7: "PEPS_EUNSUPPORTED"}
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+
+class PepsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("g", C.c_double), ("t_hop", C.c_double), ("m", C.c_double),
+                ("tau", C.c_double), ("D_max", C.c_int32), ("chi", C.c_int32),
+                ("n_als", C.c_int32), ("pad", C.c_int32), ("als_tol", C.c_double)]
+
+
+# name: (restype, argtypes)
+_SIGS = {
+    "peps_ctx_create": (C.c_int32, [C.c_int32, C.POINTER(_vp)]),
+    "peps_ctx_destroy": (C.c_int32, [_vp]),
+    "peps_last_error": (C.c_char_p, [_vp]),
+    "peps_warning_count": (C.c_int64, [_vp]),
+    "peps_set_tier": (C.c_int32, [_vp, C.c_int32]),
+    "peps_set_timing": (C.c_int32, [_vp, C.c_int32]),
+    "peps_stats_count": (C.c_int32, [_vp, _i32p]),
+    "peps_stats_get": (C.c_int32, [_vp, C.c_int32, C.c_char_p, C.c_int32, _dp, _dp, _dp, _i64p]),
+    "peps_stats_reset": (C.c_int32, [_vp]),
+    "peps_sync": (C.c_int32, [_vp]),
+    "peps_state_create": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _i64p, C.c_int32, C.POINTER(_vp)]),
+    "peps_state_destroy": (C.c_int32, [_vp]),
+    "peps_state_set_site": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p, _dp]),
+    "peps_state_site_dims": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p]),
+    "peps_state_get_site": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp]),
+    "peps_state_set_bond": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int64, _dp, _i64p]),
+    "peps_state_bond_dim": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _i64p]),
+    "peps_state_get_bond": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _i64p]),
+    "peps_apply_mask": (C.c_int32, [_vp, C.c_int32, C.c_int32]),
+    "peps_forbidden_norm": (C.c_int32, [_vp, _dp]),
+    "peps_mean_nf": (C.c_int32, [_vp, _dp]),
+    "peps_list_len": (C.c_int32, [_vp, _i32p]),
+    "peps_list_dims": (C.c_int32, [_vp, C.c_int32, _i32p, _i64p, _i32p]),
+    "peps_list_get": (C.c_int32, [_vp, C.c_int32, _dp]),
+    "peps_list_destroy": (C.c_int32, [_vp]),
+    "peps_dl_traced": (C.c_int32, [_vp, C.c_int32, _i64p, _dp, _dp]),
+    "peps_absorb_row": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _i64p, C.POINTER(_dp), _i64p, C.POINTER(_dp), C.POINTER(_vp)]),
+    "peps_compress_mps": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p, C.POINTER(_dp), C.c_int32, C.c_double, C.POINTER(_vp)]),
+    "peps_boundary": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "peps_norm2": (C.c_int32, [_vp, C.c_int32, _dp]),
+    "peps_bond_env": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "peps_charge_svd": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _i64p, _i64p, C.c_int32, C.c_double, _i64p, _i64p, C.POINTER(_vp)]),
+    "peps_als_truncate": (C.c_int32, [_vp, C.c_int32, _i64p, _dp, _dp, C.c_int32, _i64p, C.c_int32, C.c_double, _i64p, _i64p, _dp, C.POINTER(_vp)]),
+    "peps_su_bond": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32]),
+    "peps_fu_bond_v4": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, C.c_int32]),
+    "peps_trotter_step": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params)]),
+    "peps_last_als_stats": (C.c_int32, [_vp, _i64p]),
+    "peps_energy_v4": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
+    "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
+    "peps_eigh": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
+    "peps_bond_operator": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, _i32p, _dp]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def lib():
+    """Load libpeps_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libpeps_hip.so not found at {LIB_PATH}: run `python -m lgt_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_i64p)
+
+
+class Context:
+    """peps_ctx: one HIP stream, a caching allocator and the precision tier."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        st = lib().peps_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise PepsError(st, "peps_ctx_create failed")
+        self.h = h
+
+    def check(self, st):
+        if st != 0:
+            raise PepsError(st, lib().peps_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().peps_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tier(self, tier: str):
+        self.check(lib().peps_set_tier(self.h, TIERS[tier]))
+
+    def set_timing(self, on: bool):
+        self.check(lib().peps_set_timing(self.h, 1 if on else 0))
+
+    def sync(self):
+        self.check(lib().peps_sync(self.h))
+
+    def warning_count(self) -> int:
+        return int(lib().peps_warning_count(self.h))
+
+    def stats(self) -> dict:
+        n = C.c_int32()
+        self.check(lib().peps_stats_count(self.h, C.byref(n)))
+        out = {}
+        for i in range(n.value):
+            name = C.create_string_buffer(64)
+            f, b, ms = C.c_double(), C.c_double(), C.c_double()
+            k = C.c_int64()
+            self.check(lib().peps_stats_get(self.h, i, name, 64, C.byref(f), C.byref(b), C.byref(ms), C.byref(k)))
+            out[name.value.decode()] = dict(flops=f.value, bytes=b.value, ms=ms.value, launches=k.value)
+        return out
+
+    def stats_reset(self):
+        self.check(lib().peps_stats_reset(self.h))
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _as_array(a, cplx):
+    if cplx:
+        return np.asfortranarray(a, dtype=np.complex128)
+    a = np.asarray(a)
+    if np.iscomplexobj(a):
+        if np.any(a.imag != 0):
+            raise PepsError(1, "complex data passed to a real state/context call")
+        a = a.real
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _take_list(ctx: Context, h) -> list:
+    L = lib()
+    n = C.c_int32()
+    ctx.check(L.peps_list_len(h, C.byref(n)))
+    out = []
+    try:
+        for i in range(n.value):
+            rank = C.c_int32()
+            dims = (C.c_int64 * 8)()
+            cp = C.c_int32()
+            ctx.check(L.peps_list_dims(h, i, C.byref(rank), dims, C.byref(cp)))
+            shp = tuple(dims[k] for k in range(rank.value))
+            a = np.empty(shp, dtype=np.complex128 if cp.value else np.float64, order="F")
+            ctx.check(L.peps_list_get(h, i, a.ctypes.data_as(_dp)))
+            out.append(a)
+    finally:
+        L.peps_list_destroy(h)
+    return out
+
+
+# ════════════════════════════════════════════════════════════════════════════
+#  GaugePEPS on the device
+# ════════════════════════════════════════════════════════════════════════════
+
+class GaugePEPS:
+    """Device-resident GaugePEPS (finite_peps_gauge_invariant.jl:82-88).
+
+    ``tensors[ix-1][iy-1]`` (p,l,r,u,d); ``lh[ix-1][iy-1]``, ``lv[...]`` weights;
+    ``qh``, ``qv`` integer flux labels (2E); ``gch`` (nx, ny) background charges.
+    """
+
+    def __init__(self, nx, ny, gd, gch, is_complex=False, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.nx, self.ny, self.gd = nx, ny, gd
+        self.is_complex = bool(is_complex)
+        g = np.asfortranarray(np.asarray(gch, dtype=np.int64).reshape(nx, ny))
+        self.gch = g
+        h = _vp()
+        self.ctx.check(lib().peps_state_create(self.ctx.h, nx, ny, gd, _iptr(g), int(self.is_complex), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().peps_state_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # ── upload / download ────────────────────────────────────────────────────
+    @classmethod
+    def from_arrays(cls, tensors, lh, lv, qh, qv, gch, gd, is_complex=False, ctx=None):
+        nx, ny = len(tensors), len(tensors[0])
+        gp = cls(nx, ny, gd, gch, is_complex, ctx)
+        for ix in range(1, nx + 1):
+            for iy in range(1, ny + 1):
+                gp.set_site(ix, iy, tensors[ix - 1][iy - 1])
+        for iy in range(1, ny + 1):
+            for ix in range(1, nx):
+                gp.set_bond(PEPS_H, ix, iy, lh[ix - 1][iy - 1], qh[ix - 1][iy - 1])
+        for iy in range(1, ny):
+            for ix in range(1, nx + 1):
+                gp.set_bond(PEPS_V, ix, iy, lv[ix - 1][iy - 1], qv[ix - 1][iy - 1])
+        return gp
+
+    def set_site(self, ix, iy, T):
+        a = _as_array(T, self.is_complex)
+        assert a.ndim == 5
+        d = np.array(a.shape, dtype=np.int64)
+        self.ctx.check(lib().peps_state_set_site(self.h, ix, iy, _iptr(d), _dptr(a)))
+
+    def site(self, ix, iy):
+        d = np.zeros(5, np.int64)
+        self.ctx.check(lib().peps_state_site_dims(self.h, ix, iy, _iptr(d)))
+        a = np.empty(tuple(d), dtype=np.complex128 if self.is_complex else np.float64, order="F")
+        self.ctx.check(lib().peps_state_get_site(self.h, ix, iy, _dptr(a)))
+        return a
+
+    def set_bond(self, orient, ix, iy, lam, flux):
+        l = np.ascontiguousarray(lam, dtype=np.float64)
+        q = np.ascontiguousarray(flux, dtype=np.int64)
+        assert len(l) == len(q)
+        self.ctx.check(lib().peps_state_set_bond(self.h, orient, ix, iy, len(l), _dptr(l), _iptr(q)))
+
+    def bond(self, orient, ix, iy):
+        D = C.c_int64()
+        self.ctx.check(lib().peps_state_bond_dim(self.h, orient, ix, iy, C.byref(D)))
+        l = np.zeros(D.value)
+        q = np.zeros(D.value, np.int64)
+        self.ctx.check(lib().peps_state_get_bond(self.h, orient, ix, iy, _dptr(l), _iptr(q)))
+        return l, q
+
+    def to_arrays(self):
+        tens = [[self.site(ix, iy) for iy in range(1, self.ny + 1)] for ix in range(1, self.nx + 1)]
+        lh = [[None] * self.ny for _ in range(self.nx - 1)]
+        qh = [[None] * self.ny for _ in range(self.nx - 1)]
+        lv = [[None] * (self.ny - 1) for _ in range(self.nx)]
+        qv = [[None] * (self.ny - 1) for _ in range(self.nx)]
+        for iy in range(1, self.ny + 1):
+            for ix in range(1, self.nx):
+                lh[ix - 1][iy - 1], qh[ix - 1][iy - 1] = self.bond(PEPS_H, ix, iy)
+        for iy in range(1, self.ny):
+            for ix in range(1, self.nx + 1):
+                lv[ix - 1][iy - 1], qv[ix - 1][iy - 1] = self.bond(PEPS_V, ix, iy)
+        return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv)
+
+
+def _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol=0.0):
+    return Params(g=g, t_hop=t_hop, m=m, tau=tau, D_max=D_max, chi=chi, n_als=n_als, pad=0, als_tol=als_tol)
+
+
+# ── finite_peps_gauge_invariant.jl ───────────────────────────────────────────
+
+def apply_mask(gp: GaugePEPS, ix, iy):
+    gp.ctx.check(lib().peps_apply_mask(gp.h, ix, iy))
+
+
+def forbidden_norm(gp: GaugePEPS) -> float:
+    v = C.c_double()
+    gp.ctx.check(lib().peps_forbidden_norm(gp.h, C.byref(v)))
+    return v.value
+
+
+def mean_nf(gp: GaugePEPS) -> float:
+    v = C.c_double()
+    gp.ctx.check(lib().peps_mean_nf(gp.h, C.byref(v)))
+    return v.value
+
+
+def charge_svd(M, row_charge, col_charge, D_max, reg=1e-12, ctx=None):
+    """charge_svd (finite_peps_gauge_invariant.jl:236-281) -> (U, S, Vt, bond_charge)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(M)
+    a = _as_array(M, cp)
+    rq = np.ascontiguousarray(row_charge, dtype=np.int64)
+    cq = np.ascontiguousarray(col_charge, dtype=np.int64)
+    Dk = C.c_int64()
+    q = np.zeros(D_max, np.int64)
+    h = _vp()
+    ctx.check(lib().peps_charge_svd(ctx.h, int(cp), a.shape[0], a.shape[1], _dptr(a), _iptr(rq), _iptr(cq),
+                                    D_max, reg, C.byref(Dk), _iptr(q), C.byref(h)))
+    U, Vt, S = _take_list(ctx, h)
+    return U, S, Vt, q[:Dk.value]
+
+
+def simple_update_bond_h(gp: GaugePEPS, ix, iy, gate, D_max):
+    """simple_update_bond_h! (finite_peps_gauge_invariant.jl:296)."""
+    G = _as_array(gate, gp.is_complex)
+    gp.ctx.check(lib().peps_su_bond(gp.h, PEPS_H, ix, iy, _dptr(G), D_max))
+
+
+def simple_update_bond_v(gp: GaugePEPS, ix, iy, gate, D_max):
+    """simple_update_bond_v! (finite_peps_gauge_invariant.jl:412)."""
+    G = _as_array(gate, gp.is_complex)
+    gp.ctx.check(lib().peps_su_bond(gp.h, PEPS_V, ix, iy, _dptr(G), D_max))
+
+
+def trotter_step_gauge(gp: GaugePEPS, g, t_hop, m, tau, D_max):
+    """trotter_step_gauge! (finite_peps_gauge_invariant.jl:523)."""
+    p = _params(g, t_hop, m, tau, D_max, 1, 0)
+    gp.ctx.check(lib().peps_trotter_step(gp.h, PEPS_STEP_GAUGE, C.byref(p)))
+
+
+# ── finite_peps_full_update.jl / finite_peps_boundary_mps.jl ────────────────
+
+def dl_traced(T, ctx=None):
+    """_dl_traced (finite_peps_full_update.jl:79-91)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(T)
+    a = _as_array(T, cp)
+    d = np.array(a.shape, dtype=np.int64)
+    E = np.empty((d[1] ** 2, d[2] ** 2, d[3] ** 2, d[4] ** 2), dtype=a.dtype, order="F")
+    ctx.check(lib().peps_dl_traced(ctx.h, int(cp), _iptr(d), _dptr(a), _dptr(E)))
+    return E
+
+
+def _pack(arrs, cp):
+    arrs = [_as_array(x, cp) for x in arrs]
+    dims = np.array([s for x in arrs for s in x.shape], dtype=np.int64)
+    ptrs = (_dp * len(arrs))(*[_dptr(x) for x in arrs])
+    return arrs, dims, ptrs
+
+
+def _absorb(mps, row, from_top, ctx=None):
+    ctx = ctx or default_context()
+    cp = any(np.iscomplexobj(x) for x in list(mps) + list(row))
+    ma, md, mp = _pack(mps, cp)
+    ra, rd, rp = _pack(row, cp)
+    h = _vp()
+    ctx.check(lib().peps_absorb_row(ctx.h, int(from_top), int(cp), len(ma), _iptr(md), mp, _iptr(rd), rp, C.byref(h)))
+    return _take_list(ctx, h)
+
+
+def absorb_row_from_top(mps, row, ctx=None):
+    """absorb_row_from_top (finite_peps_boundary_mps.jl:128)."""
+    return _absorb(mps, row, True, ctx)
+
+
+def absorb_row_from_bottom(mps, row, ctx=None):
+    """absorb_row_from_bottom (finite_peps_boundary_mps.jl:156)."""
+    return _absorb(mps, row, False, ctx)
+
+
+def compress_mps(mps, chi, cutoff=1e-14, ctx=None):
+    """compress_mps! (finite_peps_boundary_mps.jl:70) — returns the new list."""
+    ctx = ctx or default_context()
+    cp = any(np.iscomplexobj(x) for x in mps)
+    a, d, p = _pack(mps, cp)
+    h = _vp()
+    ctx.check(lib().peps_compress_mps(ctx.h, int(cp), len(a), _iptr(d), p, chi, cutoff, C.byref(h)))
+    return _take_list(ctx, h)
+
+
+def boundary_top(gp: GaugePEPS, stop_row, chi):
+    """boundary_top (finite_peps_boundary_mps.jl:183)."""
+    h = _vp()
+    gp.ctx.check(lib().peps_boundary(gp.h, 1, stop_row, chi, C.byref(h)))
+    return _take_list(gp.ctx, h)
+
+
+def boundary_bottom(gp: GaugePEPS, stop_row, chi):
+    """boundary_bottom (finite_peps_boundary_mps.jl:207)."""
+    h = _vp()
+    gp.ctx.check(lib().peps_boundary(gp.h, 0, stop_row, chi, C.byref(h)))
+    return _take_list(gp.ctx, h)
+
+
+def peps_norm2(gp: GaugePEPS, chi=64) -> complex:
+    """peps_norm2 (finite_peps_boundary_mps.jl:228)."""
+    out = np.zeros(2)
+    gp.ctx.check(lib().peps_norm2(gp.h, chi, _dptr(out)))
+    return complex(out[0], out[1])
+
+
+def _env(gp, orient, ix, iy, chi):
+    h = _vp()
+    gp.ctx.check(lib().peps_bond_env(gp.h, orient, ix, iy, chi, C.byref(h)))
+    env, RA, RB, QA, QB = _take_list(gp.ctx, h)
+    return dict(env=env, RA=RA, RB=RB, QA=QA, QB=QB)
+
+
+def bond_environment_h(gp: GaugePEPS, ix, iy, chi=64):
+    """bond_environment_h (finite_peps_boundary_mps.jl:389)."""
+    return _env(gp, PEPS_H, ix, iy, chi)
+
+
+def bond_environment_v(gp: GaugePEPS, ix, iy, chi=64):
+    """bond_environment_v (finite_peps_boundary_mps.jl:507)."""
+    return _env(gp, PEPS_V, ix, iy, chi)
+
+
+# ── finite_peps_fullupdate_v4.jl ─────────────────────────────────────────────
+
+def als_truncate(theta, env, D_max, eRofpL, n_als=30, reg=1e-10, ctx=None, return_info=False):
+    """als_truncate (finite_peps_fullupdate_v4.jl:94) -> (X, Y, bflux)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(theta) or np.iscomplexobj(env)
+    th = _as_array(theta, cp)
+    en = _as_array(env, cp)
+    d = np.array(th.shape, dtype=np.int64)
+    eR = np.ascontiguousarray(eRofpL, dtype=np.int64)
+    Dp = C.c_int64()
+    bf = np.zeros(D_max, np.int64)
+    info = np.zeros(3)
+    h = _vp()
+    ctx.check(lib().peps_als_truncate(ctx.h, int(cp), _iptr(d), _dptr(th), _dptr(en), D_max, _iptr(eR), n_als,
+                                      reg, C.byref(Dp), _iptr(bf), _dptr(info), C.byref(h)))
+    X, Y = _take_list(ctx, h)
+    out = (X, Y, bf[:Dp.value])
+    if return_info:
+        return out + (dict(cost0=info[0], cost1=info[1], accepted=bool(info[2])),)
+    return out
+
+
+def full_update_bond_h_v4(gp: GaugePEPS, ix, iy, gate, D_max, chi=64, n_als=30):
+    """full_update_bond_h_v4! (finite_peps_fullupdate_v4.jl:165)."""
+    G = _as_array(gate, gp.is_complex)
+    gp.ctx.check(lib().peps_fu_bond_v4(gp.h, PEPS_H, ix, iy, _dptr(G), D_max, chi, n_als))
+
+
+def full_update_bond_v_v4(gp: GaugePEPS, ix, iy, gate, D_max, chi=64, n_als=30):
+    """full_update_bond_v_v4! (finite_peps_fullupdate_v4.jl:274)."""
+    G = _as_array(gate, gp.is_complex)
+    gp.ctx.check(lib().peps_fu_bond_v4(gp.h, PEPS_V, ix, iy, _dptr(G), D_max, chi, n_als))
+
+
+def trotter_step_v4(gp: GaugePEPS, g, t_hop, m, tau, D_max, chi=64, n_als=30, als_tol=0.0):
+    """trotter_step_v4! (finite_peps_fullupdate_v4.jl:415).  Returns ALS stats."""
+    p = _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol)
+    gp.ctx.check(lib().peps_trotter_step(gp.h, PEPS_STEP_V4, C.byref(p)))
+    st = np.zeros(3, np.int64)
+    gp.ctx.check(lib().peps_last_als_stats(gp.h, _iptr(st)))
+    return dict(bonds=int(st[0]), accepted=int(st[1]), sweeps=int(st[2]))
+
+
+def compute_energy_v4(gp: GaugePEPS, g, t_hop, m, chi=64) -> float:
+    """compute_energy_v4 (finite_peps_fullupdate_v4.jl:390): total energy."""
+    p = _params(g, t_hop, m, 0.0, 1, chi, 0)
+    E = C.c_double()
+    gp.ctx.check(lib().peps_energy_v4(gp.h, C.byref(p), C.byref(E)))
+    return E.value
+
+
+def gemm(A, B, a_kfast=False, conjA=False, b_kfast=True, conjB=False, ctx=None):
+    """The engine's contraction GEMM: returns op(A) @ op(B) as an (M, N) array.
+
+    A is given as the stored matrix: (M, K) if not a_kfast, (K, M) if a_kfast;
+    B as (K, N) if b_kfast, (N, K) if not.
+    """
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(A) or np.iscomplexobj(B)
+    a = _as_array(A, cp)
+    b = _as_array(B, cp)
+    M, K = (a.shape[1], a.shape[0]) if a_kfast else a.shape
+    K2, N = b.shape if b_kfast else (b.shape[1], b.shape[0])
+    assert K == K2
+    out = np.empty((M, N), dtype=a.dtype, order="F")
+    ctx.check(lib().peps_gemm(ctx.h, int(cp), M, N, K, _dptr(a), int(a_kfast), int(conjA), _dptr(b),
+                              int(b_kfast), int(conjB), _dptr(out)))
+    return out
+
+
+def eigh(A, ctx=None):
+    """Batched Hermitian eigendecomposition (Jacobi kernel): A (..., n, n) -> (w, V), w unsorted."""
+    ctx = ctx or default_context()
+    A = np.asarray(A)
+    cp = np.iscomplexobj(A)
+    n = A.shape[-1]
+    batch = A.reshape(-1, n, n)
+    nb = batch.shape[0]
+    # contiguous column-major matrices, one after the other
+    a = np.ascontiguousarray(np.transpose(batch, (0, 2, 1)), dtype=np.complex128 if cp else np.float64)
+    V = np.empty_like(a)
+    w = np.empty((nb, n))
+    ctx.check(lib().peps_eigh(ctx.h, int(cp), n, nb, _dptr(a), _dptr(V), _dptr(w)))
+    V = np.transpose(V, (0, 2, 1))
+    return w.reshape(A.shape[:-1]), V.reshape(A.shape)
+
+
+# ── u1_lgt_hamiltonian.jl ────────────────────────────────────────────────────
+
+def bond_operator(orient, ix, iy, nx, ny, gd, g, t_hop, m, tau=0.0):
+    """H_merged_{h,v}_site (tau == 0) or exp(-tau H) (u1_lgt_hamiltonian.jl:829,851,396)."""
+    n = C.c_int32()
+    L = lib()
+    st = L.peps_bond_operator(orient, ix, iy, nx, ny, gd, g, t_hop, m, tau, C.byref(n), None)
+    if st != 0:
+        raise PepsError(st, "peps_bond_operator failed")
+    out = np.zeros((n.value, n.value), order="F")
+    st = L.peps_bond_operator(orient, ix, iy, nx, ny, gd, g, t_hop, m, tau, C.byref(n), _dptr(out))
+    if st != 0:
+        raise PepsError(st, "peps_bond_operator failed")
+    return out

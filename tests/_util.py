@@ -1,0 +1,39 @@
+"""Shared helpers for the GPU parity tests: move states between the oracle
+and the device library."""
+import numpy as np
+
+from oracle import peps_oracle as O
+
+
+def to_device(gp_o, ctx):
+    from lgt_amd import peps
+    cp = any(np.iscomplexobj(t) and np.any(t.imag != 0) for col in gp_o.tensors for t in col)
+    return peps.GaugePEPS.from_arrays(gp_o.tensors, gp_o.lh, gp_o.lv, gp_o.qh, gp_o.qv, gp_o.gch,
+                                      gp_o.gd, is_complex=cp, ctx=ctx)
+
+
+def to_oracle(gp_d):
+    a = gp_d.to_arrays()
+    g = O.GaugePEPS(gp_d.nx, gp_d.ny, gp_d.gd, gp_d.gch)
+    g.tensors = [[np.asarray(t, dtype=np.complex128) for t in col] for col in a["tensors"]]
+    g.lh, g.lv, g.qh, g.qv = a["lh"], a["lv"], a["qh"], a["qv"]
+    return g
+
+
+def su_state(nx, ny, gd, D, steps, seed, cplx=False, tau=0.05, g=1.0, t=1.0, m=0.5):
+    gp = O.init_gauge_peps_random(nx, ny, gd, O.staggered_charges(nx, ny), D, seed, complex_entries=cplx)
+    for _ in range(steps):
+        O.trotter_step_gauge(gp, g, t, m, tau, D)
+    return gp
+
+
+def mps_dense(mps):
+    v = mps[0].reshape(mps[0].shape[0] * mps[0].shape[1], mps[0].shape[2], order="F")
+    for A in mps[1:]:
+        v = (v @ A.reshape(A.shape[0], -1, order="F")).reshape(-1, A.shape[2], order="F")
+    return v[:, 0]
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))

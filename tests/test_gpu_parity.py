@@ -1,0 +1,213 @@
+"""FP64 parity of libpeps_hip.so (through the C ABI) with the CPU oracle on
+identical seeded inputs.  Gauge-dependent outputs (singular-vector phases,
+QR completions) are compared through gauge-invariant contractions; integer
+flux labels must match exactly; energies to 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import peps_oracle as O
+from tests._util import mps_dense, rel, su_state, to_device, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+G, T_HOP, M = 1.0, 1.0, 0.5
+
+
+@pytest.fixture(scope="module")
+def s33():
+    return su_state(3, 3, 2, 2, 5, 11)
+
+
+@pytest.fixture(scope="module")
+def s34():
+    return su_state(3, 4, 2, 3, 4, 3)
+
+
+@pytest.fixture(scope="module")
+def s23c():
+    return su_state(2, 3, 3, 2, 3, 5, cplx=True)
+
+
+def test_absorb_rows(ctx):
+    from lgt_amd import peps
+    rng = np.random.default_rng(2)
+    for cp in (False, True):
+        mk = (lambda s: rng.standard_normal(s) + 1j * rng.standard_normal(s)) if cp else rng.standard_normal
+        mps = [mk((1, 4, 3)), mk((3, 4, 2)), mk((2, 9, 1))]
+        row = [mk((1, 4, 4, 9)), mk((4, 9, 4, 4)), mk((9, 1, 9, 16))]
+        row[2] = mk((4, 1, 9, 16))
+        mps[2] = mk((2, 9, 1))
+        out = peps.absorb_row_from_top(mps, row, ctx=ctx)
+        ref = O.absorb_row_from_top(mps, row)
+        for a, b in zip(out, ref):
+            assert rel(a, b) < 1e-14
+        mpsb = [mk((1, 9, 3)), mk((3, 4, 2)), mk((2, 16, 1))]
+        out = peps.absorb_row_from_bottom(mpsb, row, ctx=ctx)
+        ref = O.absorb_row_from_bottom(mpsb, row)
+        for a, b in zip(out, ref):
+            assert rel(a, b) < 1e-14
+
+
+@pytest.mark.parametrize("chi", [100, 3, 2])
+def test_compress_mps_state(ctx, chi):
+    from lgt_amd import peps
+    rng = np.random.default_rng(3)
+    dims, bonds = [3, 4, 3, 4, 2], [1, 3, 5, 6, 2, 1]
+    mps = [rng.standard_normal((bonds[i], dims[i], bonds[i + 1])) for i in range(5)]
+    out = peps.compress_mps(mps, chi, ctx=ctx)
+    ref = O.compress_mps([m.astype(complex) for m in mps], chi)
+    assert [m.shape for m in out] == [m.shape for m in ref]
+    assert rel(mps_dense(out), mps_dense(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("name,chi", [("s33", 64), ("s33", 6), ("s34", 12), ("s23c", 64)])
+def test_norm_and_boundaries(ctx, request, name, chi):
+    from lgt_amd import peps
+    go = request.getfixturevalue(name)
+    gd = to_device(go, ctx)
+    n_gpu = peps.peps_norm2(gd, chi)
+    n_ref = O.peps_norm2(go, chi)
+    assert abs(n_gpu - n_ref) < 1e-12 * abs(n_ref)
+    for r in range(1, go.ny + 1):
+        t_gpu, t_ref = peps.boundary_top(gd, r, chi), O.boundary_top(go, r, chi)
+        assert [m.shape for m in t_gpu] == [m.shape for m in t_ref]
+        assert rel(mps_dense(t_gpu), mps_dense(t_ref)) < 1e-11
+        b_gpu, b_ref = peps.boundary_bottom(gd, r, chi), O.boundary_bottom(go, r, chi)
+        assert rel(mps_dense(b_gpu), mps_dense(b_ref)) < 1e-11
+
+
+def _phys_env(E):
+    """env pulled back to (p,bond) space through RA, RB: gauge invariant."""
+    return np.einsum("apk,bql,abcd,cPK,dQL->pkqlPKQL", E["RA"], E["RB"], E["env"],
+                     E["RA"].conj(), E["RB"].conj(), optimize=True)
+
+
+@pytest.mark.parametrize("name,chi", [("s33", 64), ("s33", 6), ("s34", 12), ("s23c", 64)])
+def test_bond_environments(ctx, request, name, chi):
+    from lgt_amd import peps
+    go = request.getfixturevalue(name)
+    gd = to_device(go, ctx)
+    for orient, bonds in ((0, [(ix, iy) for iy in range(1, go.ny + 1) for ix in range(1, go.nx)]),
+                          (1, [(ix, iy) for iy in range(1, go.ny) for ix in range(1, go.nx + 1)])):
+        for ix, iy in bonds:
+            if orient == 0:
+                Eg, Er = peps.bond_environment_h(gd, ix, iy, chi), O.bond_environment_h(go, ix, iy, chi)
+            else:
+                Eg, Er = peps.bond_environment_v(gd, ix, iy, chi), O.bond_environment_v(go, ix, iy, chi)
+            assert Eg["env"].shape == Er["env"].shape
+            assert rel(_phys_env(Eg), _phys_env(Er)) < 1e-10, (orient, ix, iy)
+            # QA·RA reproduces the sqrt-absorbed site on both sides
+            if orient == 0:
+                Ag = np.einsum("luda,apr->plrud", Eg["QA"], Eg["RA"])
+                Ar = np.einsum("luda,apr->plrud", Er["QA"], Er["RA"])
+            else:
+                Ag = np.einsum("lrda,apu->plrud", Eg["QA"], Eg["RA"])
+                Ar = np.einsum("lrda,apu->plrud", Er["QA"], Er["RA"])
+            assert rel(Ag, Ar) < 1e-12
+
+
+def test_charge_svd(ctx):
+    from lgt_amd import peps
+    rng = np.random.default_rng(5)
+    rq = np.array([0, 1, 0, 2, 1, 0, 2, 1, 1, 0])
+    cq = np.array([1, 0, 2, 0, 1, 2, 0, 1])
+    for cp in (False, True):
+        Th = rng.standard_normal((10, 8)) + (1j * rng.standard_normal((10, 8)) if cp else 0)
+        Th[rq[:, None] != cq[None, :]] = 0
+        for D in (1, 3, 6, 20):
+            U, S, Vt, q = peps.charge_svd(Th, rq, cq, D, ctx=ctx)
+            Ur, Sr, Vr, qr = O.charge_svd(Th.astype(complex), rq, cq, D)
+            np.testing.assert_allclose(S, Sr, rtol=1e-13)
+            np.testing.assert_array_equal(q, qr)
+            assert rel((U * S) @ Vt, (Ur * Sr) @ Vr) < 1e-12
+
+
+def test_als_truncate(ctx, s33):
+    from lgt_amd import peps
+    for (ix, iy, orient) in [(1, 1, 0), (2, 2, 0), (2, 1, 1)]:
+        E = O.bond_environment_h(s33, ix, iy, 64) if orient == 0 else O.bond_environment_v(s33, ix, iy, 64)
+        th = np.einsum("apk,bqk->apqb", E["RA"], E["RB"])
+        Gt = O.gate(O.H_merged_h_site(ix, iy, 3, 3, 2, G, T_HOP, M) if orient == 0 else
+                    O.H_merged_v_site(ix, iy, 3, 3, 2, G, T_HOP, M), 0.05)
+        th = O.apply_gate(th, Gt, th.shape[1], th.shape[2]).real
+        env = E["env"].real
+        labels = O.phys_labels(ix, iy, 3, 3, 2)[1 if orient == 0 else 2]
+        X, Y, bf, info = peps.als_truncate(th, env, 2, labels, n_als=6, ctx=ctx, return_info=True)
+        Xr, Yr, bfr, infor = O.als_truncate(th.astype(complex), env.astype(complex), 2, labels, n_als=6,
+                                            return_info=True)
+        np.testing.assert_array_equal(bf, bfr)
+        assert info["accepted"] == infor["accepted"]
+        assert abs(info["cost0"] - infor["cost0"]) < 1e-10 * abs(infor["cost0"])
+        assert abs(info["cost1"] - infor["cost1"]) < 1e-9 * abs(infor["cost0"])
+        assert rel(np.einsum("apk,bqk->apqb", X, Y), np.einsum("apk,bqk->apqb", Xr, Yr)) < 1e-9
+
+
+def _compare_states(gd, go, chi=64, etol=1e-10):
+    back = to_oracle(gd)
+    for iy in range(1, go.ny + 1):
+        for ix in range(1, go.nx):
+            np.testing.assert_array_equal(back.qh[ix - 1][iy - 1], go.qh[ix - 1][iy - 1])
+            np.testing.assert_allclose(back.lh[ix - 1][iy - 1], go.lh[ix - 1][iy - 1], rtol=1e-9, atol=1e-12)
+    for iy in range(1, go.ny):
+        for ix in range(1, go.nx + 1):
+            np.testing.assert_array_equal(back.qv[ix - 1][iy - 1], go.qv[ix - 1][iy - 1])
+            np.testing.assert_allclose(back.lv[ix - 1][iy - 1], go.lv[ix - 1][iy - 1], rtol=1e-9, atol=1e-12)
+    assert O.forbidden_norm(back) < 1e-12
+    e_back = O.compute_energy_v4(back, G, T_HOP, M, chi)
+    e_ref = O.compute_energy_v4(go, G, T_HOP, M, chi)
+    assert abs(e_back - e_ref) < etol * abs(e_ref)
+    return back
+
+
+@pytest.mark.parametrize("name,D", [("s33", 2), ("s34", 3), ("s23c", 2)])
+def test_simple_update_step(ctx, request, name, D):
+    from lgt_amd import peps
+    go = request.getfixturevalue(name).copy()
+    gd = to_device(go, ctx)
+    peps.trotter_step_gauge(gd, G, T_HOP, M, 0.05, D)
+    O.trotter_step_gauge(go, G, T_HOP, M, 0.05, D)
+    _compare_states(gd, go)
+
+
+@pytest.mark.parametrize("name,D,chi", [("s33", 2, 64), ("s33", 2, 6), ("s23c", 2, 64)])
+def test_full_update_bonds(ctx, request, name, D, chi):
+    from lgt_amd import peps
+    go = request.getfixturevalue(name).copy()
+    gd = to_device(go, ctx)
+    gate_h = O.gate(O.H_merged_h_site(1, 2, go.nx, go.ny, go.gd, G, T_HOP, M), 0.025)
+    gate_v = O.gate(O.H_merged_v_site(2, 1, go.nx, go.ny, go.gd, G, T_HOP, M), 0.05)
+    gh = gate_h if gd.is_complex else gate_h.real
+    gv = gate_v if gd.is_complex else gate_v.real
+    peps.full_update_bond_h_v4(gd, 1, 2, gh, D, chi, 5)
+    O.full_update_bond_h_v4(go, 1, 2, gate_h, D, chi, 5)
+    peps.full_update_bond_v_v4(gd, 2, 1, gv, D, chi, 5)
+    O.full_update_bond_v_v4(go, 2, 1, gate_v, D, chi, 5)
+    _compare_states(gd, go, chi)
+
+
+@pytest.mark.parametrize("name,D,chi", [("s33", 2, 64), ("s33", 2, 5), ("s34", 3, 10)])
+def test_trotter_step_v4_cached_equals_recompute(ctx, request, name, D, chi):
+    """Exactly cached environments (GPU) reproduce recompute-per-bond (oracle)."""
+    from lgt_amd import peps
+    go = request.getfixturevalue(name).copy()
+    gd = to_device(go, ctx)
+    st = peps.trotter_step_v4(gd, G, T_HOP, M, 0.05, D, chi, 4)
+    O.trotter_step_v4(go, G, T_HOP, M, 0.05, D, chi, 4)
+    assert st["bonds"] == 2 * (go.nx - 1) * go.ny + go.nx * (go.ny - 1)
+    _compare_states(gd, go, chi)
+    e_gpu = peps.compute_energy_v4(gd, G, T_HOP, M, chi)
+    e_ref = O.compute_energy_v4(go, G, T_HOP, M, chi)
+    assert abs(e_gpu - e_ref) < 1e-10 * abs(e_ref)
+
+
+@pytest.mark.parametrize("name,chi", [("s33", 64), ("s34", 8), ("s23c", 64)])
+def test_energy_v4(ctx, request, name, chi):
+    from lgt_amd import peps
+    go = request.getfixturevalue(name)
+    gd = to_device(go, ctx)
+    e_gpu = peps.compute_energy_v4(gd, G, T_HOP, M, chi)
+    e_ref = O.compute_energy_v4(go, G, T_HOP, M, chi)
+    assert abs(e_gpu - e_ref) < 1e-10 * abs(e_ref)
+    assert abs(peps.forbidden_norm(gd)) < 1e-12
+    assert abs(peps.mean_nf(gd) - O.mean_nf(go)) < 1e-12
