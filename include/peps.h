@@ -72,6 +72,12 @@ const char* peps_last_error(const peps_ctx* ctx);
 int64_t peps_warning_count(const peps_ctx* ctx);
 /* contraction precision: FP64 MFMA, FP32 MFMA, BF16 MFMA (FP32 accumulate) */
 int32_t peps_set_tier(peps_ctx* ctx, int32_t tier);
+/* boundary-MPS compression (compress_mps!, finite_peps_boundary_mps.jl:70):
+   0 auto (Gram form when the exact product bond exceeds 1024), 1 the QR/SVD
+   sweep on the explicit product, 2 the Gram form: left Gram matrices of the
+   factored product and top-chi eigenvectors of Y^H G Y — the same truncated
+   MPS without forming the chi D^2 x D^2 x chi D^2 product. */
+int32_t peps_set_compress_mode(peps_ctx* ctx, int32_t mode);
 /* per-kernel-class HIP-event timing on/off, and the collected counters */
 int32_t peps_set_timing(peps_ctx* ctx, int32_t on);
 int32_t peps_stats_count(peps_ctx* ctx, int32_t* n);
@@ -172,6 +178,12 @@ int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64
    w (nb*n, unsorted per matrix). */
 int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, const double* A,
                   double* V, double* w);
+
+/* Thin SVD used by compress_mps!/charge_svd (LinearAlgebra.svd in the
+   reference): QR reduction + one-sided Jacobi kernel.  A (m x n); outputs
+   U (m x k), S (k, descending), Vt (k x n), k = min(m, n). */
+int32_t peps_svd(peps_ctx* ctx, int32_t is_complex, int64_t m, int64_t n, const double* A,
+                 double* U, double* S, double* Vt);
 
 /* ── model (u1_lgt_hamiltonian.jl:829-863, :396) ─────────────────────────── */
 /* H_merged_{h,v}_site (tau == 0) or its gate exp(-tau H); pass out == NULL to

@@ -116,6 +116,12 @@ int32_t peps_set_tier(peps_ctx* ctx, int32_t tier) {
   return PEPS_OK;
 }
 
+int32_t peps_set_compress_mode(peps_ctx* ctx, int32_t mode) {
+  if (!ctx || mode < 0 || mode > 2) return PEPS_EARG;
+  ctx->c->compress_mode = mode;
+  return PEPS_OK;
+}
+
 int32_t peps_set_timing(peps_ctx* ctx, int32_t on) {
   if (!ctx) return PEPS_EARG;
   return guard(ctx->c.get(), [&] {
@@ -333,7 +339,8 @@ int32_t peps_compress_mps(peps_ctx* ctx, int32_t is_complex, int32_t n, const in
   if (!ctx || !dims || !mps || !out || n < 1 || chi < 1) return PEPS_EARG;
   return guard(ctx->c.get(), [&] {
     auto M = upload_list(*ctx->c, is_complex != 0, n, 3, dims, mps);
-    peps::compress_mps(*ctx->c, M, chi, cutoff);
+    if (ctx->c->compress_mode == 2) peps::compress_mps_gram(*ctx->c, M, chi, cutoff);
+    else peps::compress_mps(*ctx->c, M, chi, cutoff);
     *out = make_list(ctx->c, M);
   });
 }
@@ -505,6 +512,18 @@ int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, cons
     peps::jacobi_eigh(*ctx->c, a, v, ww, n, nb, 1e-15, 60);
     ctx->c->download(v, V);
     ctx->c->download(ww, w);
+  });
+}
+
+int32_t peps_svd(peps_ctx* ctx, int32_t is_complex, int64_t m, int64_t n, const double* A,
+                 double* U, double* S, double* Vt) {
+  if (!ctx || !A || !U || !S || !Vt || m < 1 || n < 1) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    Tensor a = ctx->c->upload({m, n}, is_complex != 0, A);
+    peps::SVD f = peps::svd_thin(*ctx->c, a);
+    ctx->c->download(f.U, U);
+    ctx->c->download(f.Vt, Vt);
+    std::memcpy(S, f.S.data(), f.S.size() * sizeof(double));
   });
 }
 

@@ -48,6 +48,12 @@ void State::init(Ctx* ctx, int nx_, int ny_, int gd_, const int64_t* g, bool cpl
 
 static std::vector<double> ones1() { return {1.0}; }
 
+static std::vector<int64_t> seq(int64_t n) {
+  std::vector<int64_t> v(n);
+  std::iota(v.begin(), v.end(), 0);
+  return v;
+}
+
 // transverse weights of site (ix,iy), ones(1) on missing legs
 static std::vector<double> w_l(State& s, int ix, int iy) { return ix > 1 ? s.LH(ix - 1, iy) : ones1(); }
 static std::vector<double> w_r(State& s, int ix, int iy) { return ix < s.nx ? s.LH(ix, iy) : ones1(); }
@@ -212,6 +218,202 @@ void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff) {
   }
 }
 
+// ── Gram (density-matrix) form of compress_mps ──────────────────────────────
+//
+// The reference sweeps left->right with QR (keeping R) and right->left with a
+// truncated SVD of X_i = R_{i-1} Y_i, Y_i = M_i P_{i+1}, P_{i+1} = Y_{i+1} V.
+// Only R^H R = G (the left Gram matrix) enters the singular values and right
+// vectors of X_i, so the same truncated MPS follows from
+//   G_i = M_i^H (G_{i-1} ⊗ 1) M_i,   rho_i = Y_i^H G_{i-1} Y_i = V diag(S^2) V^H,
+// mps_i = V^H (top k),  P_i = Y_i V,  mps_1 = Y_1.
+// The reference's rank bookkeeping is kept: at site i at most
+// min(r_{i-1}, dp*k) singular values exist, r_i = min(r_{i-1} dp_i, N_y,i).
+// A column is either an explicit tensor M (N_x, dp, N_y) or the factored
+// product of a boundary-MPS tensor with a double-layer row tensor, so the
+// exact product (chi D^2 x D^2 x chi D^2) is never formed.
+
+namespace {
+
+struct Column {
+  // kind 0: explicit M (Nx, dp, Ny); 1: top absorption mps(a,u,b) E(l,r,u,d);
+  // 2: bottom absorption mps(a,d,b) E(l,r,u,d)
+  int kind = 0;
+  Tensor M, mps, E;
+  int64_t Nx() const { return kind == 0 ? M.dims[0] : E.dims[0] * mps.dims[0]; }
+  int64_t dp() const { return kind == 0 ? M.dims[1] : (kind == 1 ? E.dims[3] : E.dims[2]); }
+  int64_t Ny() const { return kind == 0 ? M.dims[2] : E.dims[1] * mps.dims[2]; }
+
+  Tensor gram_step(Ctx& c, const Tensor& G) const {
+    const int64_t nx = Nx(), ny = Ny();
+    if (kind == 0) {
+      Tensor T = contract(c, G.reshaped({nx, nx}), "xz", M, "zdy", "xdy");
+      return contract(c, M, "xdy", T, "xdw", "yw", true, false);
+    }
+    const int64_t L2 = E.dims[0], a = mps.dims[0];
+    Tensor g = G.reshaped({L2, a, L2, a});
+    Tensor Gn;
+    if (kind == 1) {
+      Tensor t1 = contract(c, g, "lamc", mps, "cub", "lamub");
+      Tensor t2 = contract(c, t1, "lamub", E, "mzud", "ladzb");
+      Tensor t3 = contract(c, t2, "ladzb", E, "lrvd", "arvzb", false, true);
+      Gn = contract(c, t3, "arvzb", mps, "avy", "ryzb", false, true);
+    } else {
+      Tensor t1 = contract(c, g, "lamc", mps, "cdb", "lamdb");
+      Tensor t2 = contract(c, t1, "lamdb", E, "mzud", "lauzb");
+      Tensor t3 = contract(c, t2, "lauzb", E, "lrue", "arezb", false, true);
+      Gn = contract(c, t3, "arezb", mps, "aey", "ryzb", false, true);
+    }
+    return Gn.reshaped({ny, ny});
+  }
+
+  // Y = M P, P (Ny x k) -> Y (Nx, dp, k)
+  Tensor apply_right(Ctx& c, const Tensor& P) const {
+    const int64_t k = P.dims[1];
+    if (kind == 0) return contract(c, M, "xdy", P, "yk", "xdk");
+    const int64_t R2 = E.dims[1], b = mps.dims[2];
+    Tensor p = P.reshaped({R2, b, k});
+    if (kind == 1) {
+      Tensor y1 = contract(c, mps, "aub", p, "rbk", "aurk");
+      return contract(c, y1, "aurk", E, "lrud", "ladk").reshaped({Nx(), dp(), k});
+    }
+    Tensor y1 = contract(c, mps, "adb", p, "rbk", "adrk");
+    return contract(c, y1, "adrk", E, "lrud", "lauk").reshaped({Nx(), dp(), k});
+  }
+};
+
+// Top-`want` eigenpairs (descending) of a Hermitian PSD matrix rho (N x N).
+void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam, Tensor* V) {
+  const int64_t N = rho.dims[0];
+  want = std::min(want, N);
+  Tensor Q, B;
+  if (N <= 512) {
+    Tensor A = copy_tensor(c, rho);
+    hermitize(c, A);
+    Tensor W = c.alloc({N, N}, rho.cplx), w = c.alloc({N}, false);
+    jacobi_eigh(c, A, W, w, int(N), 1, 1e-15, 60);
+    std::vector<double> hw = c.to_host(w);
+    std::vector<int64_t> ord(N);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return hw[x] > hw[y]; });
+    ord.resize(want);
+    lam->clear();
+    for (auto i : ord) lam->push_back(hw[i]);
+    *V = gather2d(c, W, seq(N), ord);
+    return;
+  }
+  // subspace iteration with Rayleigh-Ritz (p <= 512 columns)
+  const int64_t p = std::min<int64_t>(512, std::min<int64_t>(N, std::max<int64_t>(want + 64, 2 * want)));
+  Tensor Om = c.alloc({N, p}, rho.cplx);
+  fill_random(c, Om, 0x5eed1234abcdull + uint64_t(N));
+  Q = qr_thin(c, matmul(c, rho, Om)).Q;
+  std::vector<double> prev;
+  Tensor Wp, wv;
+  for (int it = 0; it < 60; ++it) {
+    Tensor Z = matmul(c, rho, Q);
+    if (it % 2 == 1 || it >= 4) {
+      // Rayleigh-Ritz on the current basis: convergence of the kept values
+      Tensor Bm = matmul(c, Q, Z, true, false, true, false);
+      hermitize(c, Bm);
+      Tensor Wm = c.alloc({p, p}, rho.cplx), wm = c.alloc({p}, false);
+      jacobi_eigh(c, Bm, Wm, wm, int(p), 1, 1e-15, 60);
+      std::vector<double> hw = c.to_host(wm);
+      std::sort(hw.begin(), hw.end(), std::greater<double>());
+      hw.resize(want);
+      bool conv = !prev.empty();
+      for (int64_t i = 0; conv && i < want; ++i)
+        conv = std::fabs(hw[i] - prev[i]) <= 1e-13 * std::fabs(hw[0]);
+      prev = hw;
+      if (conv) break;
+    }
+    Q = qr_thin(c, Z).Q;
+  }
+  Tensor Bm = matmul(c, Q, matmul(c, rho, Q), true, false, true, false);
+  hermitize(c, Bm);
+  Tensor Wm = c.alloc({p, p}, rho.cplx), wm = c.alloc({p}, false);
+  jacobi_eigh(c, Bm, Wm, wm, int(p), 1, 1e-15, 60);
+  std::vector<double> hw = c.to_host(wm);
+  std::vector<int64_t> ord(p);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return hw[x] > hw[y]; });
+  ord.resize(want);
+  lam->clear();
+  for (auto i : ord) lam->push_back(hw[i]);
+  *V = matmul(c, Q, gather2d(c, Wm, seq(p), ord));
+}
+
+MPS compress_columns(Ctx& c, const std::vector<Column>& cols, int chi, double cutoff, bool cplx) {
+  const size_t n = cols.size();
+  std::vector<Tensor> G(n);  // G[i] = left Gram on the left bond of column i
+  std::vector<int64_t> r(n + 1, 1);
+  double one[2] = {1.0, 0.0};
+  G[0] = c.upload({1, 1}, cplx, one);
+  PEPS_CHECK(cols[0].Nx() == 1 && cols[n - 1].Ny() == 1, PS_ESHAPE, "compress: open MPS expected");
+  for (size_t i = 0; i + 1 < n; ++i) {
+    G[i + 1] = cols[i].gram_step(c, G[i]);
+    r[i + 1] = std::min(r[i] * cols[i].dp(), cols[i].Ny());
+  }
+  MPS out(n);
+  Tensor P = c.upload({1, 1}, cplx, one);
+  for (size_t i = n - 1; i >= 1; --i) {
+    Tensor Y = cols[i].apply_right(c, P);
+    const int64_t nx = Y.dims[0], dp = Y.dims[1], k = Y.dims[2];
+    Tensor Ym = Y.reshaped({nx, dp * k});
+    Tensor GY = matmul(c, G[i], Ym);
+    Tensor rho = matmul(c, Ym, GY, true, false, true, false);
+    const int64_t length = std::min(r[i], dp * k);
+    std::vector<double> lam;
+    Tensor V;
+    top_eigs(c, rho, std::min<int64_t>(length, std::max<int64_t>(chi, 1)), &lam, &V);
+    int64_t cnt = 0;
+    for (double x : lam) cnt += std::sqrt(std::max(x, 0.0)) > cutoff;
+    int64_t kn = std::max<int64_t>(1, std::min<int64_t>({int64_t(chi), cnt, length}));
+    Tensor Vk = V;
+    Vk.dims = {dp * k, kn};  // column prefix
+    Vk = copy_tensor(c, Vk);
+    P = matmul(c, Ym, Vk);
+    // mps_i = V^H (a fresh copy: permute may return a view of Vk)
+    out[i] = copy_tensor(c, permute(c, Vk, {1, 0})).reshaped({kn, dp, k});
+    if (cplx) conj_inplace(c, out[i]);
+  }
+  out[0] = cols[0].apply_right(c, P);
+  return out;
+}
+
+}  // namespace
+
+void compress_mps_gram(Ctx& c, MPS& mps, int chi, double cutoff) {
+  if (mps.size() <= 1) return;
+  std::vector<Column> cols(mps.size());
+  for (size_t i = 0; i < mps.size(); ++i) {
+    cols[i].kind = 0;
+    cols[i].M = mps[i];
+  }
+  mps = compress_columns(c, cols, chi, cutoff, mps[0].cplx);
+}
+
+// absorb_row_from_top/bottom followed by compress_mps!, choosing the literal
+// QR/SVD sweep on the explicit product or the fused Gram form.
+MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int chi) {
+  int64_t maxbond = 1;
+  for (size_t i = 0; i < mps.size(); ++i)
+    maxbond = std::max(maxbond, mps[i].dims[2] * row[i].dims[1]);
+  bool gram = c.compress_mode == 2 || (c.compress_mode == 0 && maxbond > 1024);
+  if (!gram || mps.size() <= 1) {
+    MPS out = from_top ? absorb_row_from_top(c, mps, row) : absorb_row_from_bottom(c, mps, row);
+    compress_mps(c, out, chi);
+    return out;
+  }
+  std::vector<Column> cols(mps.size());
+  for (size_t i = 0; i < mps.size(); ++i) {
+    PEPS_CHECK((from_top ? row[i].dims[2] : row[i].dims[3]) == mps[i].dims[1], PS_ESHAPE,
+               "absorb: phys mismatch at col " + std::to_string(i + 1));
+    cols[i].kind = from_top ? 1 : 2;
+    cols[i].mps = mps[i];
+    cols[i].E = row[i];
+  }
+  return compress_columns(c, cols, chi, CMPS_CUT, mps[0].cplx);
+}
+
 MPS trivial_mps(Ctx& c, int n, bool cplx) {
   MPS m;
   for (int i = 0; i < n; ++i) m.push_back(ones_tensor(c, 3, cplx));
@@ -222,10 +424,7 @@ MPS boundary_top(State& s, int stop_row, int chi) {
   // :183-193
   if (stop_row >= s.ny) return trivial_mps(*s.c, s.nx, s.cplx);
   MPS mps = row_as_mps_down(*s.c, row_dl(s, s.ny));
-  for (int r = s.ny - 1; r > stop_row; --r) {
-    mps = absorb_row_from_top(*s.c, mps, row_dl(s, r));
-    compress_mps(*s.c, mps, chi);
-  }
+  for (int r = s.ny - 1; r > stop_row; --r) mps = absorb_compress(*s.c, mps, row_dl(s, r), true, chi);
   return mps;
 }
 
@@ -233,10 +432,7 @@ MPS boundary_bottom(State& s, int stop_row, int chi) {
   // :207-216
   if (stop_row <= 1) return trivial_mps(*s.c, s.nx, s.cplx);
   MPS mps = row_as_mps_up(*s.c, row_dl(s, 1));
-  for (int r = 2; r < stop_row; ++r) {
-    mps = absorb_row_from_bottom(*s.c, mps, row_dl(s, r));
-    compress_mps(*s.c, mps, chi);
-  }
+  for (int r = 2; r < stop_row; ++r) mps = absorb_compress(*s.c, mps, row_dl(s, r), false, chi);
   return mps;
 }
 
@@ -244,10 +440,7 @@ cplx peps_norm2(State& s, int chi) {
   // :228-245
   Ctx& c = *s.c;
   MPS mps = row_as_mps_down(*s.c, row_dl(s, s.ny));
-  for (int r = s.ny - 1; r >= 1; --r) {
-    mps = absorb_row_from_top(c, mps, row_dl(s, r));
-    compress_mps(c, mps, chi);
-  }
+  for (int r = s.ny - 1; r >= 1; --r) mps = absorb_compress(c, mps, row_dl(s, r), true, chi);
   Tensor acc = ones_tensor(c, 2, s.cplx);
   for (auto& M : mps) {
     PEPS_CHECK(M.dims[1] == 1, PS_ESHAPE, "norm: bottom phys dim != 1");
@@ -1018,10 +1211,7 @@ static std::vector<MPS> all_tops(State& s, int chi) {
   std::vector<MPS> tops(s.ny + 1);
   tops[s.ny] = trivial_mps(*s.c, s.nx, s.cplx);
   if (s.ny >= 2) tops[s.ny - 1] = row_as_mps_down(*s.c, row_dl(s, s.ny));
-  for (int r = s.ny - 2; r >= 1; --r) {
-    tops[r] = absorb_row_from_top(*s.c, tops[r + 1], row_dl(s, r + 1));
-    compress_mps(*s.c, tops[r], chi);
-  }
+  for (int r = s.ny - 2; r >= 1; --r) tops[r] = absorb_compress(*s.c, tops[r + 1], row_dl(s, r + 1), true, chi);
   return tops;
 }
 
@@ -1029,9 +1219,7 @@ static std::vector<MPS> all_tops(State& s, int chi) {
 static MPS next_bottom(State& s, const MPS& prev, int iy, int chi) {
   if (iy <= 1) return trivial_mps(*s.c, s.nx, s.cplx);
   if (iy == 2) return row_as_mps_up(*s.c, row_dl(s, 1));
-  MPS b = absorb_row_from_bottom(*s.c, prev, row_dl(s, iy - 1));
-  compress_mps(*s.c, b, chi);
-  return b;
+  return absorb_compress(*s.c, prev, row_dl(s, iy - 1), false, chi);
 }
 
 static void h_row(State& s, int iy, const MPS& top, const MPS& bot, bool l2r, const Params& p,

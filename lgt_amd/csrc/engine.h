@@ -72,6 +72,10 @@ MPS row_as_mps_up(Ctx& c, const MPS& row);
 MPS absorb_row_from_top(Ctx& c, const MPS& mps, const MPS& row);
 MPS absorb_row_from_bottom(Ctx& c, const MPS& mps, const MPS& row);
 void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT);
+// Same truncated MPS from left Gram matrices and top-chi eigenvectors.
+void compress_mps_gram(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT);
+// absorb_row_from_top/bottom + compress_mps!, literal or fused Gram form.
+MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int chi);
 MPS trivial_mps(Ctx& c, int n, bool cplx);
 MPS boundary_top(State& s, int stop_row, int chi);
 MPS boundary_bottom(State& s, int stop_row, int chi);

@@ -1245,3 +1245,159 @@ int jacobi_eigh(Ctx& c, Tensor& A, Tensor& V, Tensor& w, int n, int nb, double t
 }
 
 }  // namespace peps
+
+namespace peps {
+
+// Deterministic uniform(-1,1) fill (splitmix64 hash of the index).
+__global__ void fill_random_kernel(double* x, int64_t n, uint64_t seed) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * uint64_t(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    x[i] = double(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+void fill_random(Ctx& c, Tensor& T, uint64_t seed) {
+  int64_t n = T.numel() * (T.cplx ? 2 : 1);
+  if (!n) return;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
+  hipLaunchKernelGGL(fill_random_kernel, dim3(g), dim3(256), 0, c.stream, T.data(), n, seed);
+  PEPS_HIP(hipGetLastError());
+}
+
+}  // namespace peps
+
+namespace peps {
+
+// ════════════════════════════════════════════════════════════════════════════
+//  One-sided (Hestenes) Jacobi SVD, one workgroup per matrix
+// ════════════════════════════════════════════════════════════════════════════
+//
+// A (m x n, m >= n, column-major) is orthogonalised column pair by column
+// pair: the Gram block [[|a_p|^2, g],[conj g, |a_q|^2]], g = a_p^H a_q, is
+// diagonalised by the Jacobi rotation of the Hermitian eigensolver and the
+// rotation is applied to the columns of A and V.  Pairs follow the same
+// round-robin schedule; one wave owns one pair at a time, lanes stride the
+// rows, sums are reduced with an XOR butterfly (identical in every lane, so
+// the rotate/skip decision is wave-uniform).  Sweeps stop when a sweep
+// applies no rotation (|g| <= tol sqrt(|a_p|^2 |a_q|^2)).  On exit the
+// columns of A are u_j sigma_j, V holds the right vectors and sig the
+// column norms.
+
+template <bool CP>
+__global__ __launch_bounds__(1024) void jacobi_svd_kernel(typename JT<CP>::T* __restrict__ A,
+                                                          typename JT<CP>::T* __restrict__ V,
+                                                          double* __restrict__ sig, int m, int n,
+                                                          double tol, int max_sweeps,
+                                                          int* __restrict__ sweeps_out) {
+  typedef typename JT<CP>::T T;
+  __shared__ int rot;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  const int mm = (n + 1) & ~1;
+  const int half = mm / 2;
+  const int64_t nn = int64_t(n) * n;
+  for (int64_t i = tid; i < nn; i += blockDim.x) {
+    int64_t r = i % n, c = i / n;
+    if (r == c) set_one(V[i]); else set_zero(V[i]);
+  }
+  if (tid == 0) rot = 0;
+  __syncthreads();
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    for (int rnd = 0; rnd < mm - 1; ++rnd) {
+      for (int k = wave; k < half; k += nw) {
+        int i0 = k, i1 = mm - 1 - k;
+        int p = (i0 == 0) ? 0 : ((i0 - 1 + rnd) % (mm - 1)) + 1;
+        int q = ((i1 - 1 + rnd) % (mm - 1)) + 1;
+        if (p > q) { int t = p; p = q; q = t; }
+        if (q >= n) continue;
+        T* ap = A + int64_t(p) * m;
+        T* aq = A + int64_t(q) * m;
+        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+        for (int i = lane; i < m; i += 64) {
+          T x = ap[i], y = aq[i];
+          al += JT<CP>::abs2(x);
+          be += JT<CP>::abs2(y);
+          T g = mulc(conjv(x), y);
+          if constexpr (CP) { gr += g.x; gi += g.y; } else { gr += g; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          al += __shfl_xor(al, o);
+          be += __shfl_xor(be, o);
+          gr += __shfl_xor(gr, o);
+          gi += __shfl_xor(gi, o);
+        }
+        double ag = sqrt(gr * gr + gi * gi);
+        if (!(ag > tol * sqrt(al * be) && ag > 1e-300)) continue;
+        double th = (be - al) / (2.0 * ag);
+        double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        T ec;
+        if constexpr (CP) ec = make_double2(gr / ag, -gi / ag); else ec = gr / ag;
+        for (int i = lane; i < m; i += 64) {
+          T x = ap[i], y = mulc(aq[i], ec);
+          ap[i] = addv(scl(x, c), scl(y, -s));
+          aq[i] = addv(scl(x, s), scl(y, c));
+        }
+        T* vp = V + int64_t(p) * n;
+        T* vq = V + int64_t(q) * n;
+        for (int i = lane; i < n; i += 64) {
+          T x = vp[i], y = mulc(vq[i], ec);
+          vp[i] = addv(scl(x, c), scl(y, -s));
+          vq[i] = addv(scl(x, s), scl(y, c));
+        }
+        if (lane == 0) atomicAdd(&rot, 1);
+      }
+      __syncthreads();
+    }
+    const int r = rot;
+    __syncthreads();
+    if (r == 0) break;
+    if (tid == 0) rot = 0;
+    __syncthreads();
+  }
+  for (int j = wave; j < n; j += nw) {
+    const T* aj = A + int64_t(j) * m;
+    double s2 = 0.0;
+    for (int i = lane; i < m; i += 64) s2 += JT<CP>::abs2(aj[i]);
+    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
+    if (lane == 0) sig[j] = sqrt(s2);
+  }
+  if (tid == 0 && sweeps_out) *sweeps_out = sweep;
+}
+
+int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sweeps) {
+  PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "jacobi_svd: matrix expected");
+  const int64_t m = A.dims[0], n = A.dims[1];
+  PEPS_CHECK(m >= n && n >= 1 && n <= 4096 && m < (1ll << 31), PS_EUNSUPPORTED,
+             "jacobi_svd: need m >= n, 1 <= n <= 4096");
+  PEPS_CHECK(V.numel() == n * n && V.cplx == A.cplx && sig.numel() == n && !sig.cplx, PS_ESHAPE,
+             "jacobi_svd: buffer sizes");
+  auto buf = std::make_shared<Buffer>();
+  buf->pool = c.pool.get();
+  buf->bytes = sizeof(int);
+  buf->ptr = c.pool->get(buf->bytes);
+  const int half = int((n + 1) / 2);
+  const int waves = std::max(1, std::min(16, half));
+  hipEvent_t ev = nullptr;
+  c.tbegin("svd", &ev);
+  c.count("svd", (A.cplx ? 4.0 : 1.0) * 10.0 * 6.0 * double(m) * n * n, 0);
+  if (A.cplx)
+    hipLaunchKernelGGL(jacobi_svd_kernel<true>, dim3(1), dim3(64 * waves), 0, c.stream,
+                       (double2*)A.data(), (double2*)V.data(), sig.data(), int(m), int(n), tol,
+                       max_sweeps, (int*)buf->ptr);
+  else
+    hipLaunchKernelGGL(jacobi_svd_kernel<false>, dim3(1), dim3(64 * waves), 0, c.stream, A.data(),
+                       V.data(), sig.data(), int(m), int(n), tol, max_sweeps, (int*)buf->ptr);
+  PEPS_HIP(hipGetLastError());
+  c.tend("svd", ev);
+  int sw = 0;
+  PEPS_HIP(hipMemcpyAsync(&sw, buf->ptr, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  PEPS_HIP(hipStreamSynchronize(c.stream));
+  return sw;
+}
+
+}  // namespace peps

@@ -1,7 +1,7 @@
 // Dense FP64 factorisations on the GPU: thin QR (rocSOLVER Householder
-// geqrf + orgqr/ungqr, the LAPACK semantics of Julia's `qr`), thin SVD
-// (rocSOLVER gesvd) and the Hermitian eigendecomposition (the project's
-// Jacobi kernel).
+// geqrf + orgqr/ungqr, the LAPACK semantics of Julia's `qr`), and the
+// project's Jacobi kernels for the thin SVD (one-sided, after a QR
+// reduction) and the Hermitian eigendecomposition.
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -108,36 +108,64 @@ Tensor qr_r_only(Ctx& c, const Tensor& A) {
   return R;
 }
 
+static Tensor conj_transpose(Ctx& c, const Tensor& A) {
+  Tensor T = permute(c, A, {1, 0});
+  if (T.data() == A.data()) T = copy_tensor(c, T);  // vector-shaped: permute returned a view
+  conj_inplace(c, T);
+  return T;
+}
+
 SVD svd_thin(Ctx& c, const Tensor& A) {
+  // Thin SVD A = U diag(S) Vt with S descending: tall inputs are reduced by a
+  // Householder QR (rocSOLVER geqrf/orgqr), wide ones by their conjugate
+  // transpose, and the square core is diagonalised by the one-sided Jacobi
+  // kernel (kernels.hip).  Left vectors of zero singular values are zero.
   PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "svd: matrix expected");
-  int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);
+  const int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);
   check_int_range(m, n);
   SVD r;
-  r.U = c.alloc({m, k}, A.cplx);
-  r.Vt = c.alloc({k, n}, A.cplx);
-  if (k == 0) return r;
+  if (k == 0) {
+    r.U = c.alloc({m, 0}, A.cplx);
+    r.Vt = c.alloc({0, n}, A.cplx);
+    return r;
+  }
   check_finite(c, A, "svd");
-  Tensor W = copy_tensor(c, A);
-  Tensor S = c.alloc({k}, false);
-  Tensor E = c.alloc({std::max<int64_t>(k, 1)}, false);
-  DevInt info(c);
-  hipEvent_t ev = nullptr;
-  c.tbegin("svd", &ev);
-  c.count("svd", (A.cplx ? 4.0 : 1.0) * (4.0 * m * n * k + 8.0 * k * k * k), 0);
-  if (A.cplx)
-    ck(rocsolver_zgesvd(c.blas, rocblas_svect_singular, rocblas_svect_singular, int(m), int(n),
-                        (rocblas_double_complex*)W.data(), int(m), S.data(),
-                        (rocblas_double_complex*)r.U.data(), int(m),
-                        (rocblas_double_complex*)r.Vt.data(), int(k), E.data(),
-                        rocblas_outofplace, info.p), "zgesvd");
-  else
-    ck(rocsolver_dgesvd(c.blas, rocblas_svect_singular, rocblas_svect_singular, int(m), int(n),
-                        W.data(), int(m), S.data(), r.U.data(), int(m), r.Vt.data(), int(k),
-                        E.data(), rocblas_outofplace, info.p), "dgesvd");
-  c.tend("svd", ev);
-  int inf = info.get(c);
-  PEPS_CHECK(inf == 0, PS_ESOLVER, "gesvd did not converge (info=" + std::to_string(inf) + ")");
-  r.S = c.to_host(S);
+  const bool tr = m < n;
+  Tensor B = tr ? conj_transpose(c, A) : A;
+  const int64_t mb = B.dims[0], nb = B.dims[1];
+  const bool useq = mb > nb;
+  Tensor W, Q;
+  if (useq) {
+    QR f = qr_thin(c, B);
+    Q = f.Q;
+    W = f.R;
+  } else {
+    W = copy_tensor(c, B);
+  }
+  Tensor V = c.alloc({nb, nb}, A.cplx), sg = c.alloc({nb}, false);
+  jacobi_svd(c, W, V, sg, 1e-15, 80);
+  std::vector<double> s = c.to_host(sg);
+  std::vector<int64_t> ord(nb);
+  for (int64_t i = 0; i < nb; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return s[x] > s[y]; });
+  std::vector<double> inv(nb);
+  for (int64_t i = 0; i < nb; ++i) inv[i] = s[i] > 0.0 ? 1.0 / s[i] : 0.0;
+  scale_cols(c, W, inv);
+  Tensor Ub = useq ? matmul(c, Q, W) : W;
+  std::vector<int64_t> rows_u(mb), rows_v(nb);
+  for (int64_t i = 0; i < mb; ++i) rows_u[i] = i;
+  for (int64_t i = 0; i < nb; ++i) rows_v[i] = i;
+  Ub = gather2d(c, Ub, rows_u, ord);
+  Tensor Vb = gather2d(c, V, rows_v, ord);
+  r.S.resize(nb);
+  for (int64_t i = 0; i < nb; ++i) r.S[i] = s[ord[i]];
+  if (!tr) {
+    r.U = Ub;
+    r.Vt = conj_transpose(c, Vb);
+  } else {
+    r.U = Vb;
+    r.Vt = conj_transpose(c, Ub);
+  }
   return r;
 }
 

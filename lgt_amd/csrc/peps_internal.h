@@ -133,6 +133,9 @@ class Ctx {
   std::string last_error;
   int64_t warnings = 0;
   int ncu = 256;
+  // boundary-MPS compression: 0 auto (Gram form when the exact bond > 1024),
+  // 1 QR/SVD sweep on the exact product, 2 Gram form
+  int compress_mode = 0;
 
   // Per kernel-class accounting ("gemm", "permute", "svd", ...).
   std::map<std::string, KernelStat> stats;
@@ -216,6 +219,8 @@ void hermitize(Ctx& c, Tensor& N);
 // Copy a sub-block: dst(:, j0:j0+n) = src (column block).
 Tensor copy_tensor(Ctx& c, const Tensor& t);
 bool all_finite(Ctx& c, const Tensor& t);
+// Deterministic uniform(-1,1) entries (complex: both parts).
+void fill_random(Ctx& c, Tensor& T, uint64_t seed);
 
 // ── contraction front end (contract.cpp) ────────────────────────────────────
 // Einstein-summation contraction of two tensors with single-character labels,
@@ -247,5 +252,9 @@ EIG eigh(Ctx& c, const Tensor& A);
 // Batched parallel-Jacobi Hermitian eigensolver (kernels.hip): A (nb n x n
 // matrices, overwritten), V eigenvectors, w eigenvalues (unsorted).
 int jacobi_eigh(Ctx& c, Tensor& A, Tensor& V, Tensor& w, int n, int nb, double tol, int max_sweeps);
+// One-sided Jacobi SVD kernel (kernels.hip): A (m x n, m >= n) is overwritten
+// by its orthogonalised columns u_j sigma_j, V gets the right vectors and sig
+// the column norms (unsorted).
+int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sweeps);
 
 }  // namespace peps

@@ -51,6 +51,7 @@ _SIGS = {
     "peps_last_error": (C.c_char_p, [_vp]),
     "peps_warning_count": (C.c_int64, [_vp]),
     "peps_set_tier": (C.c_int32, [_vp, C.c_int32]),
+    "peps_set_compress_mode": (C.c_int32, [_vp, C.c_int32]),
     "peps_set_timing": (C.c_int32, [_vp, C.c_int32]),
     "peps_stats_count": (C.c_int32, [_vp, _i32p]),
     "peps_stats_get": (C.c_int32, [_vp, C.c_int32, C.c_char_p, C.c_int32, _dp, _dp, _dp, _i64p]),
@@ -86,6 +87,7 @@ _SIGS = {
     "peps_energy_v4": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
     "peps_eigh": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
+    "peps_svd": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]),
     "peps_bond_operator": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_double, C.c_double, C.c_double, C.c_double, _i32p, _dp]),
 }
@@ -145,6 +147,10 @@ class Context:
 
     def set_tier(self, tier: str):
         self.check(lib().peps_set_tier(self.h, TIERS[tier]))
+
+    def set_compress_mode(self, mode: str):
+        """'auto', 'qr_svd' (explicit product) or 'gram' (factored Gram form)."""
+        self.check(lib().peps_set_compress_mode(self.h, {"auto": 0, "qr_svd": 1, "gram": 2}[mode]))
 
     def set_timing(self, on: bool):
         self.check(lib().peps_set_timing(self.h, 1 if on else 0))
@@ -529,6 +535,20 @@ def eigh(A, ctx=None):
     ctx.check(lib().peps_eigh(ctx.h, int(cp), n, nb, _dptr(a), _dptr(V), _dptr(w)))
     V = np.transpose(V, (0, 2, 1))
     return w.reshape(A.shape[:-1]), V.reshape(A.shape)
+
+
+def svd(A, ctx=None):
+    """Thin SVD through the library (QR + one-sided Jacobi): (U, S, Vt)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(A)
+    a = _as_array(A, cp)
+    m, n = a.shape
+    k = min(m, n)
+    U = np.empty((m, k), dtype=a.dtype, order="F")
+    Vt = np.empty((k, n), dtype=a.dtype, order="F")
+    S = np.empty(k)
+    ctx.check(lib().peps_svd(ctx.h, int(cp), m, n, _dptr(a), _dptr(U), _dptr(S), _dptr(Vt)))
+    return U, S, Vt
 
 
 # ── u1_lgt_hamiltonian.jl ────────────────────────────────────────────────────

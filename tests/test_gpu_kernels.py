@@ -67,3 +67,29 @@ def test_jacobi_eigh(ctx, cplx):
             assert np.abs(rec - H[b]).max() < 1e-13 * n * scale
             assert np.abs(np.conj(V[b].T) @ V[b] - np.eye(n)).max() < 1e-13 * n
             np.testing.assert_allclose(np.sort(w[b]), np.linalg.eigvalsh(H[b]), atol=1e-13 * n * scale)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_svd(ctx, cplx):
+    """Thin SVD (QR + one-sided Jacobi) against numpy, including the
+    exact-zero columns and rank deficiencies of Gauss-masked blocks."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(8)
+    for (m, n) in [(1, 1), (1, 7), (7, 1), (3, 3), (8, 24), (24, 8), (16, 32), (40, 40), (130, 70),
+                   (70, 200), (300, 288)]:
+        A = rng.standard_normal((m, n)) + (1j * rng.standard_normal((m, n)) if cplx else 0)
+        if min(m, n) > 3:
+            A[:, 1] = 0
+            A[2, :] = 0
+            A[:, -1] = 3.0 * A[:, 0]
+        U, S, Vt = peps.svd(A, ctx=ctx)
+        Sr = np.linalg.svd(A, compute_uv=False)
+        scale = max(Sr.max(), 1e-300)
+        np.testing.assert_allclose(S, Sr, atol=1e-13 * scale * max(m, n) ** 0.5)
+        assert np.all(np.diff(S) <= 1e-12 * scale)
+        rec = (U * S) @ Vt
+        assert np.abs(rec - A).max() < 1e-13 * scale * max(m, n)
+        nz = S > 1e-10 * scale
+        Uk, Vk = U[:, nz], Vt[nz, :]
+        assert np.abs(np.conj(Uk.T) @ Uk - np.eye(nz.sum())).max() < 1e-12
+        assert np.abs(Vk @ np.conj(Vk.T) - np.eye(nz.sum())).max() < 1e-12

@@ -211,3 +211,41 @@ def test_energy_v4(ctx, request, name, chi):
     assert abs(e_gpu - e_ref) < 1e-10 * abs(e_ref)
     assert abs(peps.forbidden_norm(gd)) < 1e-12
     assert abs(peps.mean_nf(gd) - O.mean_nf(go)) < 1e-12
+
+
+@pytest.mark.parametrize("chi", [100, 3, 2])
+def test_compress_mps_gram_form(ctx, chi):
+    """Gram form of compress_mps!: same truncated state and bond shapes."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(3)
+    dims, bonds = [3, 4, 3, 4, 2], [1, 3, 5, 6, 2, 1]
+    mps = [rng.standard_normal((bonds[i], dims[i], bonds[i + 1])) for i in range(5)]
+    ctx.set_compress_mode("gram")
+    try:
+        out = peps.compress_mps(mps, chi, ctx=ctx)
+    finally:
+        ctx.set_compress_mode("auto")
+    ref = O.compress_mps([m.astype(complex) for m in mps], chi)
+    assert [m.shape for m in out] == [m.shape for m in ref]
+    assert rel(mps_dense(out), mps_dense(ref)) < 1e-10
+
+
+@pytest.mark.parametrize("name,chi", [("s34", 12), ("s34", 4), ("s23c", 64)])
+def test_boundaries_and_energy_gram_form(ctx, request, name, chi):
+    """Fused absorb+Gram compression vs the oracle's literal absorb+QR/SVD."""
+    from lgt_amd import peps
+    go = request.getfixturevalue(name)
+    gd = to_device(go, ctx)
+    ctx.set_compress_mode("gram")
+    try:
+        for r in range(1, go.ny + 1):
+            t_gpu, t_ref = peps.boundary_top(gd, r, chi), O.boundary_top(go, r, chi)
+            assert [m.shape for m in t_gpu] == [m.shape for m in t_ref]
+            assert rel(mps_dense(t_gpu), mps_dense(t_ref)) < 1e-9
+            b_gpu, b_ref = peps.boundary_bottom(gd, r, chi), O.boundary_bottom(go, r, chi)
+            assert rel(mps_dense(b_gpu), mps_dense(b_ref)) < 1e-9
+        e_gpu = peps.compute_energy_v4(gd, G, T_HOP, M, chi)
+    finally:
+        ctx.set_compress_mode("auto")
+    e_ref = O.compute_energy_v4(go, G, T_HOP, M, chi)
+    assert abs(e_gpu - e_ref) < 1e-9 * abs(e_ref)
