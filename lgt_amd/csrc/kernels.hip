@@ -658,7 +658,8 @@ static MaskArgs make_mask_args(const Tensor& T, const std::vector<int64_t>& nf,
   PEPS_CHECK(T.rank() == 5, PS_ESHAPE, "mask: rank-5 tensor expected");
   MaskArgs a{};
   for (int i = 0; i < 5; ++i) a.d[i] = T.dims[i];
-  PEPS_CHECK(int64_t(nf.size()) == T.dims[0] && int64_t(fl.size()) == T.dims[1] &&
+  PEPS_CHECK(int64_t(nf.size()) == T.dims[0] && lR.size() == nf.size() && lU.size() == nf.size() &&
+                 int64_t(fl.size()) == T.dims[1] &&
                  int64_t(fr.size()) == T.dims[2] && int64_t(fu.size()) == T.dims[3] &&
                  int64_t(fd.size()) == T.dims[4],
              PS_ESHAPE, "mask: label length mismatch");
@@ -818,7 +819,9 @@ __global__ void cross_zero_kernel(double* M, int64_t nrow, int64_t ncol, int esz
 void zero_cross_sector(Ctx& c, Tensor& M, const std::vector<int64_t>& rq,
                        const std::vector<int64_t>& cq) {
   int64_t nrow = M.dims[0], ncol = M.dims[1];
-  PEPS_CHECK(int64_t(rq.size()) == nrow && int64_t(cq.size()) == ncol, PS_ESHAPE, "cross: labels");
+  PEPS_CHECK(M.rank() == 2 && int64_t(rq.size()) == nrow && int64_t(cq.size()) == ncol, PS_ESHAPE,
+             "cross: labels");
+  if (nrow == 0 || ncol == 0) return;
   std::shared_ptr<Buffer> k1, k2;
   auto* drq = (const int64_t*)c.scratch_upload(rq.data(), rq.size() * 8, &k1);
   auto* dcq = (const int64_t*)c.scratch_upload(cq.data(), cq.size() * 8, &k2);
@@ -840,8 +843,19 @@ __global__ void gather2d_kernel(const double* M, int64_t nrow, double* out, int6
   }
 }
 
+// Host-side bounds check of an index list (the kernels below index with it).
+static void check_index(const std::vector<int64_t>& ix, int64_t bound, const char* what) {
+  for (int64_t v : ix)
+    PEPS_CHECK(v >= 0 && v < bound, PS_EARG, std::string(what) + ": index out of range");
+}
+
+static int64_t mat_cols(const Tensor& T) { return T.dims[0] ? T.numel() / T.dims[0] : 0; }
+
 Tensor gather2d(Ctx& c, const Tensor& M, const std::vector<int64_t>& rows,
                 const std::vector<int64_t>& cols) {
+  PEPS_CHECK(M.rank() >= 1, PS_ESHAPE, "gather2d: matrix expected");
+  check_index(rows, M.dims[0], "gather2d rows");
+  check_index(cols, mat_cols(M), "gather2d cols");
   Tensor out = c.alloc({int64_t(rows.size()), int64_t(cols.size())}, M.cplx);
   if (rows.empty() || cols.empty()) return out;
   std::shared_ptr<Buffer> k1, k2;
@@ -873,6 +887,10 @@ __global__ void scatter_cols_kernel(double* dst, int64_t dnrow, const double* sr
 void scatter_cols(Ctx& c, Tensor& dst, const Tensor& src, const std::vector<int64_t>& rows,
                   const std::vector<int64_t>& src_cols, const std::vector<int64_t>& dst_cols) {
   PEPS_CHECK(src_cols.size() == dst_cols.size(), PS_EARG, "scatter_cols");
+  PEPS_CHECK(int64_t(rows.size()) <= src.dims[0], PS_ESHAPE, "scatter_cols: rows");
+  check_index(rows, dst.dims[0], "scatter_cols rows");
+  check_index(src_cols, mat_cols(src), "scatter_cols src");
+  check_index(dst_cols, mat_cols(dst), "scatter_cols dst");
   if (rows.empty() || src_cols.empty()) return;
   std::shared_ptr<Buffer> k1, k2, k3;
   auto* dr = (const int64_t*)c.scratch_upload(rows.data(), rows.size() * 8, &k1);
@@ -904,6 +922,10 @@ __global__ void scatter_rows_kernel(double* dst, int64_t dnrow, const double* sr
 void scatter_rows(Ctx& c, Tensor& dst, const Tensor& src, const std::vector<int64_t>& src_rows,
                   const std::vector<int64_t>& dst_rows, const std::vector<int64_t>& cols) {
   PEPS_CHECK(src_rows.size() == dst_rows.size(), PS_EARG, "scatter_rows");
+  check_index(src_rows, src.dims[0], "scatter_rows src");
+  check_index(dst_rows, dst.dims[0], "scatter_rows dst");
+  check_index(cols, mat_cols(dst), "scatter_rows cols");
+  PEPS_CHECK(int64_t(cols.size()) <= mat_cols(src), PS_ESHAPE, "scatter_rows: cols");
   if (src_rows.empty() || cols.empty()) return;
   std::shared_ptr<Buffer> k1, k2, k3;
   auto* ds = (const int64_t*)c.scratch_upload(src_rows.data(), src_rows.size() * 8, &k1);
@@ -1033,26 +1055,63 @@ Tensor copy_tensor(Ctx& c, const Tensor& t) {
   return o;
 }
 
-__global__ void finite_kernel(const double* x, int64_t n, int* bad) {
+// flags[0] = 1 if any entry is non-finite, flags[1] = 1 if any entry is non-zero.
+__global__ void scan_kernel(const double* x, int64_t n, int* flags) {
+  int bad = 0, nz = 0;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    if (!isfinite(x[i])) *bad = 1;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double v = x[i];
+    if (!isfinite(v)) bad = 1;
+    else if (v != 0.0) nz = 1;
+  }
+  if (bad) flags[0] = 1;
+  if (nz) flags[1] = 1;
 }
-bool all_finite(Ctx& c, const Tensor& t) {
+
+void scan_values(Ctx& c, const Tensor& t, bool* finite, bool* nonzero) {
   int64_t n = t.numel() * (t.cplx ? 2 : 1);
-  if (!n) return true;
+  *finite = true;
+  *nonzero = false;
+  if (!n) return;
   auto buf = std::make_shared<Buffer>();
   buf->pool = c.pool.get();
   buf->bytes = 8;
   buf->ptr = c.pool->get(8);
-  PEPS_HIP(hipMemsetAsync(buf->ptr, 0, 4, c.stream));
+  PEPS_HIP(hipMemsetAsync(buf->ptr, 0, 8, c.stream));
   unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 4096));
-  hipLaunchKernelGGL(finite_kernel, dim3(g), dim3(256), 0, c.stream, t.data(), n, (int*)buf->ptr);
+  hipLaunchKernelGGL(scan_kernel, dim3(g), dim3(256), 0, c.stream, t.data(), n, (int*)buf->ptr);
   PEPS_HIP(hipGetLastError());
-  int h = 0;
-  PEPS_HIP(hipMemcpyAsync(&h, buf->ptr, 4, hipMemcpyDeviceToHost, c.stream));
+  int h[2] = {0, 0};
+  PEPS_HIP(hipMemcpyAsync(h, buf->ptr, 8, hipMemcpyDeviceToHost, c.stream));
   PEPS_HIP(hipStreamSynchronize(c.stream));
-  return h == 0;
+  *finite = h[0] == 0;
+  *nonzero = h[1] != 0;
+}
+
+// out = the first columns of the identity (m x k).
+__global__ void eye_kernel(double* out, int64_t m, int64_t k, int esz) {
+  int64_t n = m * k;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i % m, cc = i / m;
+    out[i * esz] = r == cc ? 1.0 : 0.0;
+    if (esz == 2) out[i * esz + 1] = 0.0;
+  }
+}
+
+Tensor eye(Ctx& c, int64_t m, int64_t k, bool cplx) {
+  Tensor out = c.alloc({m, k}, cplx);
+  if (m * k == 0) return out;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(m * k, 256), 16384));
+  hipLaunchKernelGGL(eye_kernel, dim3(g), dim3(256), 0, c.stream, out.data(), m, k, cplx ? 2 : 1);
+  PEPS_HIP(hipGetLastError());
+  return out;
+}
+
+bool all_finite(Ctx& c, const Tensor& t) {
+  bool f, nz;
+  scan_values(c, t, &f, &nz);
+  return f;
 }
 
 
@@ -1065,8 +1124,12 @@ bool all_finite(Ctx& c, const Tensor& t) {
 // [[a, b],[conj b, d]] is zeroed by J = diag(1, conj(e)) R(c, s) with
 // e = b/|b| and the real Jacobi rotation t = sign(th)/(|th|+sqrt(th^2+1)),
 // th = (d-a)/(2|b|) (Golub & Van Loan, Alg. 8.5.2).  A <- A J (columns),
-// A <- J^H A (rows), V <- V J.  Sweeps stop when the off-diagonal
-// Frobenius norm falls below tol * ||A||_F.  A, V live in global memory.
+// A <- J^H A (rows), V <- V J.  A pair is rotated only while
+// |b| > tol ||A||_F (the accuracy class of a Householder-based solver:
+// absolute error ~ eps ||A||; below it an off-diagonal entry is rounding
+// noise of the large entries); sweeps stop when a sweep rotates no pair.  |b| is formed without squaring and e = b/|b| is exactly unimodular
+// for real b, so every applied J is unitary to rounding.  A, V live in
+// global memory.
 
 namespace {
 template <bool CP> struct JT;
@@ -1074,11 +1137,13 @@ template <> struct JT<false> {
   typedef double T;
   static __device__ double re(T x) { return x; }
   static __device__ double abs2(T x) { return x * x; }
+  static __device__ double mag(T x) { return fabs(x); }
 };
 template <> struct JT<true> {
   typedef double2 T;
   static __device__ double re(T x) { return x.x; }
   static __device__ double abs2(T x) { return x.x * x.x + x.y * x.y; }
+  static __device__ double mag(T x) { return hypot(x.x, x.y); }
 };
 __device__ __forceinline__ double mulc(double a, double b) { return a * b; }
 __device__ __forceinline__ double2 mulc(double2 a, double2 b) {
@@ -1090,7 +1155,7 @@ __device__ __forceinline__ double scl(double a, double s) { return a * s; }
 __device__ __forceinline__ double2 scl(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __device__ __forceinline__ double addv(double a, double b) { return a + b; }
 __device__ __forceinline__ double2 addv(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double unitv(double b, double ab) { return ab > 0 ? b / ab : 1.0; }
+__device__ __forceinline__ double unitv(double b, double ab) { return b >= 0.0 ? 1.0 : -1.0; }
 __device__ __forceinline__ double2 unitv(double2 b, double ab) {
   return ab > 0 ? make_double2(b.x / ab, b.y / ab) : make_double2(1.0, 0.0);
 }
@@ -1115,8 +1180,9 @@ __global__ __launch_bounds__(1024) void jacobi_eigh_kernel(typename JT<CP>::T* _
   __shared__ double cs_c[JMAXP], cs_s[JMAXP];
   __shared__ T cs_e[JMAXP];
   __shared__ int pp[JMAXP], qq[JMAXP];
-  __shared__ double red[1024];
-  __shared__ int done;
+  __shared__ int rot;
+  __shared__ double part[16];
+  __shared__ double floor_abs;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int m = (n + 1) & ~1;
   const int half = m / 2;
@@ -1125,34 +1191,24 @@ __global__ __launch_bounds__(1024) void jacobi_eigh_kernel(typename JT<CP>::T* _
     int64_t r = i % n, c = i / n;
     if (r == c) set_one(V[i]); else set_zero(V[i]);
   }
+  // rotation threshold tol * ||A||_F (invariant under the rotations)
+  {
+    double f2 = 0.0;
+    for (int64_t i = tid; i < nn; i += nt) f2 += JT<CP>::abs2(A[i]);
+    for (int o = 32; o > 0; o >>= 1) f2 += __shfl_xor(f2, o);
+    if ((tid & 63) == 0) part[tid >> 6] = f2;
+  }
+  if (tid == 0) rot = 0;
   __syncthreads();
+  if (tid == 0) {
+    double f2 = 0.0;
+    for (int i = 0; i < (nt >> 6); ++i) f2 += part[i];
+    floor_abs = tol * sqrt(f2);
+  }
+  __syncthreads();
+  const double fl = floor_abs;
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
-    // convergence test: off(A)^2 <= tol^2 * ||A||^2
-    double off = 0.0, tot = 0.0;
-    for (int64_t i = tid; i < nn; i += nt) {
-      double a2 = JT<CP>::abs2(A[i]);
-      tot += a2;
-      if (i % n != i / n) off += a2;
-    }
-    red[tid] = off;
-    __syncthreads();
-    for (int o = nt / 2; o > 0; o >>= 1) {
-      if (tid < o) red[tid] += red[tid + o];
-      __syncthreads();
-    }
-    double offs = red[0];
-    __syncthreads();
-    red[tid] = tot;
-    __syncthreads();
-    for (int o = nt / 2; o > 0; o >>= 1) {
-      if (tid < o) red[tid] += red[tid + o];
-      __syncthreads();
-    }
-    double tots = red[0];
-    if (tid == 0) done = (offs <= tol * tol * tots) || offs == 0.0;
-    __syncthreads();
-    if (done) break;
     for (int rnd = 0; rnd < m - 1; ++rnd) {
       // pairs of this round
       for (int k = tid; k < half; k += nt) {
@@ -1165,14 +1221,18 @@ __global__ __launch_bounds__(1024) void jacobi_eigh_kernel(typename JT<CP>::T* _
         set_one(e);
         if (q < n) {
           T b = A[p + int64_t(q) * n];
-          double ab = sqrt(JT<CP>::abs2(b));
-          double a = JT<CP>::re(A[p + int64_t(p) * n]), d = JT<CP>::re(A[q + int64_t(q) * n]);
-          if (ab > 1e-300 * (fabs(a) + fabs(d) + 1e-300)) {
-            double th = (d - a) / (2.0 * ab);
-            double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
-            s = t * c;
-            e = unitv(b, ab);
+          const double ab = JT<CP>::mag(b);
+          const double a = JT<CP>::re(A[p + int64_t(p) * n]), d = JT<CP>::re(A[q + int64_t(q) * n]);
+          if (ab > fl && ab > 1e-300) {
+            const double th = (d - a) / (2.0 * ab);
+            const double t = fabs(th) > 1e150 ? 0.5 / th
+                                              : (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            if (t != 0.0) {
+              c = 1.0 / sqrt(t * t + 1.0);
+              s = t * c;
+              e = unitv(b, ab);
+              atomicAdd(&rot, 1);
+            }
           }
         }
         cs_c[k] = c; cs_s[k] = s; cs_e[k] = e; pp[k] = p; qq[k] = q;
@@ -1209,6 +1269,11 @@ __global__ __launch_bounds__(1024) void jacobi_eigh_kernel(typename JT<CP>::T* _
       }
       __syncthreads();
     }
+    const int r = rot;
+    __syncthreads();
+    if (r == 0) break;
+    if (tid == 0) rot = 0;
+    __syncthreads();
   }
   for (int i = tid; i < n; i += nt) w[i] = JT<CP>::re(A[i + int64_t(i) * n]);
   if (tid == 0 && sweeps_out) sweeps_out[blockIdx.x] = sweep;
@@ -1283,9 +1348,9 @@ namespace peps {
 // round-robin schedule; one wave owns one pair at a time, lanes stride the
 // rows, sums are reduced with an XOR butterfly (identical in every lane, so
 // the rotate/skip decision is wave-uniform).  Sweeps stop when a sweep
-// applies no rotation (|g| <= tol sqrt(|a_p|^2 |a_q|^2)).  On exit the
-// columns of A are u_j sigma_j, V holds the right vectors and sig the
-// column norms.
+// applies no rotation (|g| <= tol |a_p| |a_q|, or a column of norm below
+// 1e-140, which is treated as zero).  On exit the columns of A are
+// u_j sigma_j, V holds the right vectors and sig the column norms.
 
 template <bool CP>
 __global__ __launch_bounds__(1024) void jacobi_svd_kernel(typename JT<CP>::T* __restrict__ A,
@@ -1330,13 +1395,19 @@ __global__ __launch_bounds__(1024) void jacobi_svd_kernel(typename JT<CP>::T* __
           gr += __shfl_xor(gr, o);
           gi += __shfl_xor(gi, o);
         }
-        double ag = sqrt(gr * gr + gi * gi);
-        if (!(ag > tol * sqrt(al * be) && ag > 1e-300)) continue;
-        double th = (be - al) / (2.0 * ag);
-        double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        // |g| without squaring (gr^2 can underflow), so that the phase g/|g|
+        // is unimodular to rounding; columns of norm below 1e-140 count as
+        // zero and pairs orthogonal to working precision are skipped
+        const double na = sqrt(al), nb = sqrt(be);
+        const double ag = CP ? hypot(gr, gi) : fabs(gr);
+        if (!(na > 1e-140 && nb > 1e-140 && ag > tol * na * nb)) continue;
+        const double th = (be - al) / (2.0 * ag);
+        const double t = fabs(th) > 1e150 ? 0.5 / th
+                                          : (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        if (t == 0.0) continue;
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
         T ec;
-        if constexpr (CP) ec = make_double2(gr / ag, -gi / ag); else ec = gr / ag;
+        if constexpr (CP) ec = make_double2(gr / ag, -gi / ag); else ec = gr >= 0.0 ? 1.0 : -1.0;
         for (int i = lane; i < m; i += 64) {
           T x = ap[i], y = mulc(aq[i], ec);
           ap[i] = addv(scl(x, c), scl(y, -s));

@@ -17,28 +17,14 @@ void ck(rocblas_status s, const char* what) {
     throw Error(PS_ESOLVER, std::string(what) + " failed with rocblas status " + std::to_string(int(s)));
 }
 
-struct DevInt {
-  std::shared_ptr<Buffer> b;
-  rocblas_int* p;
-  explicit DevInt(Ctx& c) {
-    b = std::make_shared<Buffer>();
-    b->pool = c.pool.get();
-    b->bytes = 256;
-    b->ptr = c.pool->get(256);
-    p = static_cast<rocblas_int*>(b->ptr);
-  }
-  int get(Ctx& c) {
-    int h = 0;
-    PEPS_HIP(hipMemcpyAsync(&h, p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    PEPS_HIP(hipStreamSynchronize(c.stream));
-    return h;
-  }
-};
-
-// rocSOLVER is never handed NaN/Inf: a non-finite input is reported as a
-// solver failure instead.
-void check_finite(Ctx& c, const Tensor& A, const char* what) {
-  PEPS_CHECK(all_finite(c, A), PS_ESOLVER, std::string("non-finite input to ") + what);
+// rocSOLVER is never handed NaN/Inf (a non-finite input is reported as a
+// solver failure instead) nor an all-zero matrix (whose factorisations are
+// written down directly).  Returns whether A has a non-zero entry.
+bool check_input(Ctx& c, const Tensor& A, const char* what) {
+  bool finite, nonzero;
+  scan_values(c, A, &finite, &nonzero);
+  PEPS_CHECK(finite, PS_ESOLVER, std::string("non-finite input to ") + what);
+  return nonzero;
 }
 
 void check_int_range(int64_t m, int64_t n) {
@@ -52,7 +38,12 @@ QR qr_thin(Ctx& c, const Tensor& A) {
   PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "qr: matrix expected");
   int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);
   check_int_range(m, n);
-  check_finite(c, A, "qr");
+  if (!check_input(c, A, "qr")) {  // A = 0: Q = first k columns of I, R = 0
+    QR r;
+    r.Q = eye(c, m, k, A.cplx);
+    r.R = c.zeros({k, n}, A.cplx);
+    return r;
+  }
   Tensor W = copy_tensor(c, A);
   Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
   hipEvent_t ev = nullptr;
@@ -90,7 +81,7 @@ Tensor qr_r_only(Ctx& c, const Tensor& A) {
   PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "qr: matrix expected");
   int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);
   check_int_range(m, n);
-  check_finite(c, A, "qr");
+  if (!check_input(c, A, "qr")) return c.zeros({k, n}, A.cplx);
   Tensor W = copy_tensor(c, A);
   Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
   hipEvent_t ev = nullptr;
@@ -116,10 +107,11 @@ static Tensor conj_transpose(Ctx& c, const Tensor& A) {
 }
 
 SVD svd_thin(Ctx& c, const Tensor& A) {
-  // Thin SVD A = U diag(S) Vt with S descending: tall inputs are reduced by a
-  // Householder QR (rocSOLVER geqrf/orgqr), wide ones by their conjugate
-  // transpose, and the square core is diagonalised by the one-sided Jacobi
-  // kernel (kernels.hip).  Left vectors of zero singular values are zero.
+  // Thin SVD A = U diag(S) Vt with S descending.  Wide inputs are handled
+  // through their conjugate transpose; inputs much taller than wide are
+  // first reduced by a Householder QR (rocSOLVER geqrf/orgqr); the one-sided
+  // Jacobi kernel (kernels.hip) diagonalises what remains.  Left vectors of
+  // zero singular values are zero, except for A = 0 (identity vectors).
   PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "svd: matrix expected");
   const int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);
   check_int_range(m, n);
@@ -129,11 +121,17 @@ SVD svd_thin(Ctx& c, const Tensor& A) {
     r.Vt = c.alloc({0, n}, A.cplx);
     return r;
   }
-  check_finite(c, A, "svd");
+  if (!check_input(c, A, "svd")) {  // A = 0: S = 0 with identity vectors
+    r.U = eye(c, m, k, A.cplx);
+    r.S.assign(k, 0.0);
+    r.Vt = conj_transpose(c, eye(c, n, k, A.cplx));
+    return r;
+  }
   const bool tr = m < n;
   Tensor B = tr ? conj_transpose(c, A) : A;
   const int64_t mb = B.dims[0], nb = B.dims[1];
-  const bool useq = mb > nb;
+  // a Householder QR first only where it shrinks the Jacobi work markedly
+  const bool useq = mb > 2 * nb && mb * nb > 65536;
   Tensor W, Q;
   if (useq) {
     QR f = qr_thin(c, B);
@@ -149,7 +147,7 @@ SVD svd_thin(Ctx& c, const Tensor& A) {
   for (int64_t i = 0; i < nb; ++i) ord[i] = i;
   std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return s[x] > s[y]; });
   std::vector<double> inv(nb);
-  for (int64_t i = 0; i < nb; ++i) inv[i] = s[i] > 0.0 ? 1.0 / s[i] : 0.0;
+  for (int64_t i = 0; i < nb; ++i) inv[i] = s[i] > 1e-140 ? 1.0 / s[i] : 0.0;  // columns below 1e-140 are left unrotated: zero
   scale_cols(c, W, inv);
   Tensor Ub = useq ? matmul(c, Q, W) : W;
   std::vector<int64_t> rows_u(mb), rows_v(nb);
@@ -177,7 +175,7 @@ EIG eigh(Ctx& c, const Tensor& A) {
   EIG r;
   r.V = c.alloc({n, n}, A.cplx);
   if (n == 0) return r;
-  check_finite(c, A, "eigh");
+  check_input(c, A, "eigh");
   Tensor W = copy_tensor(c, A);
   Tensor w = c.alloc({n}, false);
   jacobi_eigh(c, W, r.V, w, int(n), 1, 1e-15, 60);

@@ -219,6 +219,10 @@ void hermitize(Ctx& c, Tensor& N);
 // Copy a sub-block: dst(:, j0:j0+n) = src (column block).
 Tensor copy_tensor(Ctx& c, const Tensor& t);
 bool all_finite(Ctx& c, const Tensor& t);
+// One pass: are all entries finite, is any entry non-zero (synchronises).
+void scan_values(Ctx& c, const Tensor& t, bool* finite, bool* nonzero);
+// The first k columns of the m x m identity.
+Tensor eye(Ctx& c, int64_t m, int64_t k, bool cplx);
 // Deterministic uniform(-1,1) entries (complex: both parts).
 void fill_random(Ctx& c, Tensor& T, uint64_t seed);
 

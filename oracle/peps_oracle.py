@@ -897,8 +897,14 @@ def als_cost(th, env, X, Y):
     return thth - 2 * np.real(XYth) + XYXY
 
 
-def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=False):
-    """finite_peps_fullupdate_v4.jl:94-154."""
+def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=False, init=None):
+    """finite_peps_fullupdate_v4.jl:94-154.
+
+    init=(X0, Y0) replaces the charge-SVD initial bond tensors (the SVD still
+    fixes the bond fluxes and the mask).  For complex tensors the sweeps are
+    not covariant under the phase freedom of the singular vectors (NX pairs
+    Y with conj(Y) in the reference's index order), so comparing two
+    implementations of the sweeps needs a shared init."""
     nA, dpL, dpR, nB = th.shape
     thm = jr(pd(th, 1, 2, 4, 3), nA * dpL, nB * dpR)
     row_charge = np.array([eRofpL[r // nA] for r in range(nA * dpL)])
@@ -909,6 +915,9 @@ def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=F
     X = jr(U * sq[None, :], nA, dpL, Dp).copy()
     Y = pd(jr(sq[:, None] * Vt, Dp, nB, dpR), 2, 3, 1)
     Xmask = np.broadcast_to((np.asarray(eRofpL)[:, None] == bflux[None, :])[None, :, :], X.shape)
+    if init is not None:
+        assert init[0].shape == X.shape and init[1].shape == Y.shape
+        X, Y = np.array(init[0], dtype=complex), np.array(init[1], dtype=complex)
     X0, Y0 = X.copy(), Y.copy()
     cost0 = als_cost(th, env, X0, Y0)
     for _ in range(n_als):

@@ -67,6 +67,18 @@ def test_jacobi_eigh(ctx, cplx):
             assert np.abs(rec - H[b]).max() < 1e-13 * n * scale
             assert np.abs(np.conj(V[b].T) @ V[b] - np.eye(n)).max() < 1e-13 * n
             np.testing.assert_allclose(np.sort(w[b]), np.linalg.eigvalsh(H[b]), atol=1e-13 * n * scale)
+    # graded spectrum down to 1e-14 with an exact null space (the ALS normal
+    # matrices of Gauss-masked bonds look like this)
+    for n in (5, 33, 128):
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0))
+        w0 = np.logspace(0, -14, n)
+        w0[: n // 5] = 0.0
+        H = (Q * w0) @ np.conj(Q.T)
+        H = 0.5 * (H + np.conj(H.T))
+        w, V = peps.eigh(H, ctx=ctx)
+        assert np.abs((V * w) @ np.conj(V.T) - H).max() < 1e-13 * n
+        assert np.abs(np.conj(V.T) @ V - np.eye(n)).max() < 1e-13 * n
+        np.testing.assert_allclose(np.sort(w), np.linalg.eigvalsh(H), atol=1e-13 * n)
 
 
 @pytest.mark.parametrize("cplx", [False, True])
@@ -76,7 +88,7 @@ def test_svd(ctx, cplx):
     from lgt_amd import peps
     rng = np.random.default_rng(8)
     for (m, n) in [(1, 1), (1, 7), (7, 1), (3, 3), (8, 24), (24, 8), (16, 32), (40, 40), (130, 70),
-                   (70, 200), (300, 288)]:
+                   (70, 200), (300, 288), (2000, 60)]:
         A = rng.standard_normal((m, n)) + (1j * rng.standard_normal((m, n)) if cplx else 0)
         if min(m, n) > 3:
             A[:, 1] = 0
@@ -93,3 +105,27 @@ def test_svd(ctx, cplx):
         Uk, Vk = U[:, nz], Vt[nz, :]
         assert np.abs(np.conj(Uk.T) @ Uk - np.eye(nz.sum())).max() < 1e-12
         assert np.abs(Vk @ np.conj(Vk.T) - np.eye(nz.sum())).max() < 1e-12
+    # a 4x3 sector block with two zero rows whose column inner products fall
+    # to ~1e-160 during the sweeps (their squares underflow), and columns
+    # scaled to 1e-120 .. 1e-200
+    blocks = [np.array([[-7.6510e-02, -4.6293e-03, -1.0015e+00], [0, 0, 0], [0, 0, 0],
+                        [6.7905e-03, 4.1073e-04, 8.8862e-02]])]
+    for sc in (1e-120, 1e-160, 1e-200):
+        A = rng.standard_normal((6, 4)) + (1j * rng.standard_normal((6, 4)) if cplx else 0)
+        A[:, 2] *= sc
+        A[:, 0] = A[:, 1] * 1e-9 + A[:, 0] * sc
+        blocks.append(A)
+    for A in blocks:
+        U, S, Vt = peps.svd(A, ctx=ctx)
+        Sr = np.linalg.svd(A, compute_uv=False)
+        np.testing.assert_allclose(S, Sr, atol=1e-14 * Sr.max())
+        assert np.abs((U * S) @ Vt - A).max() < 1e-14 * Sr.max()
+        assert np.abs(Vt @ np.conj(Vt.T) - np.eye(len(S))).max() < 1e-13
+    # the all-zero blocks of Gauss-masked matrices: S = 0, identity vectors
+    for (m, n) in [(4, 29), (29, 4), (1, 1)]:
+        A = np.zeros((m, n), complex if cplx else float)
+        U, S, Vt = peps.svd(A, ctx=ctx)
+        k = min(m, n)
+        assert S.shape == (k,) and np.all(S == 0)
+        assert np.abs(np.conj(U.T) @ U - np.eye(k)).max() < 1e-15
+        assert np.abs(Vt @ np.conj(Vt.T) - np.eye(k)).max() < 1e-15
