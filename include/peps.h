@@ -173,9 +173,10 @@ int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64
                   int32_t b_kfast, int32_t conjB, double* Cout);
 
 /* Hermitian eigendecomposition used by the ALS solve (_solve_herm,
-   finite_peps_fullupdate_v4.jl:37-45): batched parallel-Jacobi kernel.
-   A: nb contiguous n x n Hermitian matrices; outputs V (same layout) and
-   w (nb*n, unsorted per matrix). */
+   finite_peps_fullupdate_v4.jl:37-45): Householder tridiagonalisation and
+   divide and conquer (rocSOLVER syevd/heevd), one matrix after the other.
+   A: nb contiguous n x n Hermitian matrices (lower triangles read); outputs
+   V (same layout) and w (nb*n, ascending per matrix). */
 int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, const double* A,
                   double* V, double* w);
 

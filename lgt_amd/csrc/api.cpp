@@ -505,13 +505,13 @@ int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, cons
   if (!ctx || !A || !V || !w || n < 1 || nb < 1) return PEPS_EARG;
   return guard(ctx->c.get(), [&] {
     bool cp = is_complex != 0;
-    int64_t tot = int64_t(n) * n * nb;
-    Tensor a = ctx->c->upload({tot}, cp, A);
-    Tensor v = ctx->c->alloc({tot}, cp);
-    Tensor ww = ctx->c->alloc({int64_t(n) * nb}, false);
-    peps::jacobi_eigh(*ctx->c, a, v, ww, n, nb, 1e-15, 60);
-    ctx->c->download(v, V);
-    ctx->c->download(ww, w);
+    const int64_t nn = int64_t(n) * n, esz = cp ? 2 : 1;
+    for (int32_t b = 0; b < nb; ++b) {
+      Tensor a = ctx->c->upload({n, n}, cp, A + b * nn * esz);
+      peps::EIG f = peps::eigh(*ctx->c, a);
+      ctx->c->download(f.V, V + b * nn * esz);
+      std::copy(f.w.begin(), f.w.end(), w + int64_t(b) * n);
+    }
   });
 }
 

@@ -289,9 +289,9 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
   if (N <= 512) {
     Tensor A = copy_tensor(c, rho);
     hermitize(c, A);
-    Tensor W = c.alloc({N, N}, rho.cplx), w = c.alloc({N}, false);
-    jacobi_eigh(c, A, W, w, int(N), 1, 1e-15, 60);
-    std::vector<double> hw = c.to_host(w);
+    EIG f = eigh(c, A);
+    const std::vector<double>& hw = f.w;
+    const Tensor& W = f.V;
     std::vector<int64_t> ord(N);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return hw[x] > hw[y]; });
@@ -314,9 +314,7 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
       // Rayleigh-Ritz on the current basis: convergence of the kept values
       Tensor Bm = matmul(c, Q, Z, true, false, true, false);
       hermitize(c, Bm);
-      Tensor Wm = c.alloc({p, p}, rho.cplx), wm = c.alloc({p}, false);
-      jacobi_eigh(c, Bm, Wm, wm, int(p), 1, 1e-15, 60);
-      std::vector<double> hw = c.to_host(wm);
+      std::vector<double> hw = eigh(c, Bm).w;
       std::sort(hw.begin(), hw.end(), std::greater<double>());
       hw.resize(want);
       bool conv = !prev.empty();
@@ -329,9 +327,9 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
   }
   Tensor Bm = matmul(c, Q, matmul(c, rho, Q), true, false, true, false);
   hermitize(c, Bm);
-  Tensor Wm = c.alloc({p, p}, rho.cplx), wm = c.alloc({p}, false);
-  jacobi_eigh(c, Bm, Wm, wm, int(p), 1, 1e-15, 60);
-  std::vector<double> hw = c.to_host(wm);
+  EIG f = eigh(c, Bm);
+  const std::vector<double>& hw = f.w;
+  const Tensor& Wm = f.V;
   std::vector<int64_t> ord(p);
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return hw[x] > hw[y]; });

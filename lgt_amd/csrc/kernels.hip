@@ -1116,20 +1116,8 @@ bool all_finite(Ctx& c, const Tensor& t) {
 
 
 // ════════════════════════════════════════════════════════════════════════════
-//  Hermitian eigensolver: parallel cyclic Jacobi, one workgroup per matrix
+//  Complex/real helpers of the one-sided Jacobi SVD kernel
 // ════════════════════════════════════════════════════════════════════════════
-//
-// Round-robin ("circle") ordering: m = n rounded up to even, m/2 disjoint
-// (p,q) pairs per round, m-1 rounds per sweep.  Each pair's 2x2 problem
-// [[a, b],[conj b, d]] is zeroed by J = diag(1, conj(e)) R(c, s) with
-// e = b/|b| and the real Jacobi rotation t = sign(th)/(|th|+sqrt(th^2+1)),
-// th = (d-a)/(2|b|) (Golub & Van Loan, Alg. 8.5.2).  A <- A J (columns),
-// A <- J^H A (rows), V <- V J.  A pair is rotated only while
-// |b| > tol ||A||_F (the accuracy class of a Householder-based solver:
-// absolute error ~ eps ||A||; below it an off-diagonal entry is rounding
-// noise of the large entries); sweeps stop when a sweep rotates no pair.  |b| is formed without squaring and e = b/|b| is exactly unimodular
-// for real b, so every applied J is unitary to rounding.  A, V live in
-// global memory.
 
 namespace {
 template <bool CP> struct JT;
@@ -1164,150 +1152,6 @@ __device__ __forceinline__ void set_one(double2& x) { x = make_double2(1.0, 0.0)
 __device__ __forceinline__ void set_zero(double& x) { x = 0.0; }
 __device__ __forceinline__ void set_zero(double2& x) { x = make_double2(0.0, 0.0); }
 }  // namespace
-
-constexpr int JMAXP = 256;  // max pairs per round (n <= 512)
-
-template <bool CP>
-__global__ __launch_bounds__(1024) void jacobi_eigh_kernel(typename JT<CP>::T* __restrict__ Ab,
-                                                           typename JT<CP>::T* __restrict__ Vb,
-                                                           double* __restrict__ wb, int n,
-                                                           int64_t stride, double tol,
-                                                           int max_sweeps, int* __restrict__ sweeps_out) {
-  typedef typename JT<CP>::T T;
-  T* A = Ab + int64_t(blockIdx.x) * stride;
-  T* V = Vb + int64_t(blockIdx.x) * stride;
-  double* w = wb + int64_t(blockIdx.x) * n;
-  __shared__ double cs_c[JMAXP], cs_s[JMAXP];
-  __shared__ T cs_e[JMAXP];
-  __shared__ int pp[JMAXP], qq[JMAXP];
-  __shared__ int rot;
-  __shared__ double part[16];
-  __shared__ double floor_abs;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int m = (n + 1) & ~1;
-  const int half = m / 2;
-  const int64_t nn = int64_t(n) * n;
-  for (int64_t i = tid; i < nn; i += nt) {
-    int64_t r = i % n, c = i / n;
-    if (r == c) set_one(V[i]); else set_zero(V[i]);
-  }
-  // rotation threshold tol * ||A||_F (invariant under the rotations)
-  {
-    double f2 = 0.0;
-    for (int64_t i = tid; i < nn; i += nt) f2 += JT<CP>::abs2(A[i]);
-    for (int o = 32; o > 0; o >>= 1) f2 += __shfl_xor(f2, o);
-    if ((tid & 63) == 0) part[tid >> 6] = f2;
-  }
-  if (tid == 0) rot = 0;
-  __syncthreads();
-  if (tid == 0) {
-    double f2 = 0.0;
-    for (int i = 0; i < (nt >> 6); ++i) f2 += part[i];
-    floor_abs = tol * sqrt(f2);
-  }
-  __syncthreads();
-  const double fl = floor_abs;
-  int sweep = 0;
-  for (; sweep < max_sweeps; ++sweep) {
-    for (int rnd = 0; rnd < m - 1; ++rnd) {
-      // pairs of this round
-      for (int k = tid; k < half; k += nt) {
-        int i0 = k, i1 = m - 1 - k;
-        int p = (i0 == 0) ? 0 : ((i0 - 1 + rnd) % (m - 1)) + 1;
-        int q = ((i1 - 1 + rnd) % (m - 1)) + 1;
-        if (p > q) { int t = p; p = q; q = t; }
-        double c = 1.0, s = 0.0;
-        T e;
-        set_one(e);
-        if (q < n) {
-          T b = A[p + int64_t(q) * n];
-          const double ab = JT<CP>::mag(b);
-          const double a = JT<CP>::re(A[p + int64_t(p) * n]), d = JT<CP>::re(A[q + int64_t(q) * n]);
-          if (ab > fl && ab > 1e-300) {
-            const double th = (d - a) / (2.0 * ab);
-            const double t = fabs(th) > 1e150 ? 0.5 / th
-                                              : (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            if (t != 0.0) {
-              c = 1.0 / sqrt(t * t + 1.0);
-              s = t * c;
-              e = unitv(b, ab);
-              atomicAdd(&rot, 1);
-            }
-          }
-        }
-        cs_c[k] = c; cs_s[k] = s; cs_e[k] = e; pp[k] = p; qq[k] = q;
-      }
-      __syncthreads();
-      // columns: A <- A J, V <- V J
-      for (int64_t idx = tid; idx < int64_t(half) * n; idx += nt) {
-        int k = int(idx / n), i = int(idx % n);
-        int p = pp[k], q = qq[k];
-        if (q >= n || cs_s[k] == 0.0) continue;
-        double c = cs_c[k], s = cs_s[k];
-        T ec = conjv(cs_e[k]);
-        T aip = A[i + int64_t(p) * n], aiq = A[i + int64_t(q) * n];
-        T aq = mulc(aiq, ec);
-        A[i + int64_t(p) * n] = addv(scl(aip, c), scl(aq, -s));
-        A[i + int64_t(q) * n] = addv(scl(aip, s), scl(aq, c));
-        T vip = V[i + int64_t(p) * n], viq = V[i + int64_t(q) * n];
-        T vq = mulc(viq, ec);
-        V[i + int64_t(p) * n] = addv(scl(vip, c), scl(vq, -s));
-        V[i + int64_t(q) * n] = addv(scl(vip, s), scl(vq, c));
-      }
-      __syncthreads();
-      // rows: A <- J^H A
-      for (int64_t idx = tid; idx < int64_t(half) * n; idx += nt) {
-        int k = int(idx / n), j = int(idx % n);
-        int p = pp[k], q = qq[k];
-        if (q >= n || cs_s[k] == 0.0) continue;
-        double c = cs_c[k], s = cs_s[k];
-        T e = cs_e[k];
-        T apj = A[p + int64_t(j) * n], aqj = A[q + int64_t(j) * n];
-        T aq = mulc(aqj, e);
-        A[p + int64_t(j) * n] = addv(scl(apj, c), scl(aq, -s));
-        A[q + int64_t(j) * n] = addv(scl(apj, s), scl(aq, c));
-      }
-      __syncthreads();
-    }
-    const int r = rot;
-    __syncthreads();
-    if (r == 0) break;
-    if (tid == 0) rot = 0;
-    __syncthreads();
-  }
-  for (int i = tid; i < n; i += nt) w[i] = JT<CP>::re(A[i + int64_t(i) * n]);
-  if (tid == 0 && sweeps_out) sweeps_out[blockIdx.x] = sweep;
-}
-
-// Batched Hermitian eigendecomposition of `nb` n x n matrices stored
-// contiguously (A is overwritten).  Returns the sweep count of matrix 0.
-int jacobi_eigh(Ctx& c, Tensor& A, Tensor& V, Tensor& w, int n, int nb, double tol, int max_sweeps) {
-  PEPS_CHECK(n >= 1 && n <= 2 * JMAXP, PS_EUNSUPPORTED, "jacobi_eigh: n must be in [1, 512]");
-  PEPS_CHECK(A.numel() == int64_t(n) * n * nb && V.numel() == A.numel() && w.numel() == int64_t(n) * nb &&
-                 V.cplx == A.cplx && !w.cplx,
-             PS_ESHAPE, "jacobi_eigh: buffer sizes");
-  auto buf = std::make_shared<Buffer>();
-  buf->pool = c.pool.get();
-  buf->bytes = size_t(nb) * sizeof(int);
-  buf->ptr = c.pool->get(buf->bytes);
-  int threads = n <= 32 ? 64 : (n <= 128 ? 256 : 1024);
-  hipEvent_t ev = nullptr;
-  c.tbegin("eigh", &ev);
-  c.count("eigh", (A.cplx ? 4.0 : 1.0) * 6.0 * 8.0 * double(n) * n * n * nb, 0);
-  if (A.cplx)
-    hipLaunchKernelGGL(jacobi_eigh_kernel<true>, dim3(nb), dim3(threads), 0, c.stream,
-                       (double2*)A.data(), (double2*)V.data(), w.data(), n, int64_t(n) * n, tol,
-                       max_sweeps, (int*)buf->ptr);
-  else
-    hipLaunchKernelGGL(jacobi_eigh_kernel<false>, dim3(nb), dim3(threads), 0, c.stream, A.data(),
-                       V.data(), w.data(), n, int64_t(n) * n, tol, max_sweeps, (int*)buf->ptr);
-  PEPS_HIP(hipGetLastError());
-  c.tend("eigh", ev);
-  int sw = 0;
-  PEPS_HIP(hipMemcpyAsync(&sw, buf->ptr, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  PEPS_HIP(hipStreamSynchronize(c.stream));
-  return sw;
-}
 
 }  // namespace peps
 

@@ -1,7 +1,9 @@
 // Dense FP64 factorisations on the GPU: thin QR (rocSOLVER Householder
-// geqrf + orgqr/ungqr, the LAPACK semantics of Julia's `qr`), and the
-// project's Jacobi kernels for the thin SVD (one-sided, after a QR
-// reduction) and the Hermitian eigendecomposition.
+// geqrf + orgqr/ungqr, the LAPACK semantics of Julia's `qr`), the
+// Hermitian eigendecomposition (rocSOLVER syevd/heevd: Householder
+// tridiagonalisation and divide and conquer, the algorithm class of
+// Julia's `eigen(Hermitian)`), and the project's one-sided Jacobi kernel for
+// the thin SVD (after a QR reduction of very tall inputs).
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -16,6 +18,25 @@ void ck(rocblas_status s, const char* what) {
   if (s != rocblas_status_success)
     throw Error(PS_ESOLVER, std::string(what) + " failed with rocblas status " + std::to_string(int(s)));
 }
+
+// device-side info word of a rocSOLVER call
+struct DevInt {
+  std::shared_ptr<Buffer> b;
+  rocblas_int* p;
+  explicit DevInt(Ctx& c) {
+    b = std::make_shared<Buffer>();
+    b->pool = c.pool.get();
+    b->bytes = 256;
+    b->ptr = c.pool->get(256);
+    p = static_cast<rocblas_int*>(b->ptr);
+  }
+  int get(Ctx& c) {
+    int h = 0;
+    PEPS_HIP(hipMemcpyAsync(&h, p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PEPS_HIP(hipStreamSynchronize(c.stream));
+    return h;
+  }
+};
 
 // rocSOLVER is never handed NaN/Inf (a non-finite input is reported as a
 // solver failure instead) nor an all-zero matrix (whose factorisations are
@@ -168,17 +189,38 @@ SVD svd_thin(Ctx& c, const Tensor& A) {
 }
 
 EIG eigh(Ctx& c, const Tensor& A) {
-  // Hermitian eigendecomposition by the project's batched Jacobi kernel
-  // (kernels.hip); eigenvalues are returned unsorted.
+  // Hermitian eigendecomposition, eigenvalues ascending; only the lower
+  // triangle of A is read.
   PEPS_CHECK(A.rank() == 2 && A.dims[0] == A.dims[1], PS_ESHAPE, "eigh: square matrix expected");
-  int64_t n = A.dims[0];
+  const int64_t n = A.dims[0];
+  check_int_range(n, n);
   EIG r;
-  r.V = c.alloc({n, n}, A.cplx);
-  if (n == 0) return r;
-  check_input(c, A, "eigh");
-  Tensor W = copy_tensor(c, A);
-  Tensor w = c.alloc({n}, false);
-  jacobi_eigh(c, W, r.V, w, int(n), 1, 1e-15, 60);
+  if (n == 0) {
+    r.V = c.alloc({0, 0}, A.cplx);
+    return r;
+  }
+  if (!check_input(c, A, "eigh")) {  // A = 0
+    r.V = eye(c, n, n, A.cplx);
+    r.w.assign(n, 0.0);
+    return r;
+  }
+  r.V = copy_tensor(c, A);
+  Tensor w = c.alloc({n}, false), e = c.alloc({n}, false);
+  DevInt info(c);
+  hipEvent_t ev = nullptr;
+  c.tbegin("eigh", &ev);
+  c.count("eigh", (A.cplx ? 4.0 : 1.0) * 9.0 * double(n) * n * n, 0);
+  if (A.cplx)
+    ck(rocsolver_zheevd(c.blas, rocblas_evect_original, rocblas_fill_lower, int(n),
+                        (rocblas_double_complex*)r.V.data(), int(n), w.data(), e.data(), info.p),
+       "zheevd");
+  else
+    ck(rocsolver_dsyevd(c.blas, rocblas_evect_original, rocblas_fill_lower, int(n), r.V.data(),
+                        int(n), w.data(), e.data(), info.p),
+       "dsyevd");
+  c.tend("eigh", ev);
+  const int inf = info.get(c);
+  PEPS_CHECK(inf == 0, PS_ESOLVER, "syevd did not converge (info=" + std::to_string(inf) + ")");
   r.w = c.to_host(w);
   return r;
 }

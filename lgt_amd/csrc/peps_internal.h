@@ -253,9 +253,7 @@ struct EIG {
   Tensor V;
 };
 EIG eigh(Ctx& c, const Tensor& A);
-// Batched parallel-Jacobi Hermitian eigensolver (kernels.hip): A (nb n x n
-// matrices, overwritten), V eigenvectors, w eigenvalues (unsorted).
-int jacobi_eigh(Ctx& c, Tensor& A, Tensor& V, Tensor& w, int n, int nb, double tol, int max_sweeps);
+
 // One-sided Jacobi SVD kernel (kernels.hip): A (m x n, m >= n) is overwritten
 // by its orthogonalised columns u_j sigma_j, V gets the right vectors and sig
 // the column norms (unsorted).

@@ -49,8 +49,9 @@ def test_dl_traced_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("cplx", [False, True])
-def test_jacobi_eigh(ctx, cplx):
-    """The ALS eigensolver against numpy, including rank-deficient inputs."""
+def test_eigh(ctx, cplx):
+    """The ALS eigensolver (rocSOLVER syevd/heevd) against numpy, including
+    rank-deficient inputs."""
     from lgt_amd import peps
     rng = np.random.default_rng(4)
     for n in (1, 2, 3, 7, 16, 33, 64, 130, 288):
