@@ -43,6 +43,46 @@ def make_state(ctx, nx, ny, gd, D, seed, su_steps, g, t, m):
     return gp
 
 
+def normalise(gp, nx, ny, chi):
+    """Rescale the site tensors so that <psi|psi> = 1 (the physical state is
+    unchanged).  The boundary-MPS truncation keeps singular values above an
+    absolute 1e-14 and the charge SVD above 1e-12, as the reference does; on
+    an unnormalised 8x8 state <psi|psi> is ~1e-100 and every boundary would
+    collapse to bond dimension 1.  Each site is first scaled to unit
+    weighted norm, then all sites by the common factor <psi|psi>^(-1/2N),
+    repeated until <psi|psi> = 1 to 0.1%."""
+    from lgt_amd import peps
+
+    def lam(orient, ix, iy):
+        return np.sqrt(gp.bond(orient, ix, iy)[0])
+
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            T = gp.site(ix, iy)
+            W = T.copy()
+            if ix > 1: W = W * lam(peps.PEPS_H, ix - 1, iy)[None, :, None, None, None]
+            if ix < nx: W = W * lam(peps.PEPS_H, ix, iy)[None, None, :, None, None]
+            if iy < ny: W = W * lam(peps.PEPS_V, ix, iy)[None, None, None, :, None]
+            if iy > 1: W = W * lam(peps.PEPS_V, ix, iy - 1)[None, None, None, None, :]
+            gp.set_site(ix, iy, T / max(np.linalg.norm(W), 1e-300))
+    # while <psi|psi> is tiny the boundaries truncate hard and the computed
+    # norm is only an estimate; rescaling by it brings the next evaluation
+    # into the range where the cutoffs are harmless
+    first = None
+    for it in range(6):
+        n2 = peps.peps_norm2(gp, chi).real
+        if not (np.isfinite(n2) and n2 > 0):
+            raise RuntimeError(f"normalise: <psi|psi> = {n2}")
+        first = n2 if first is None else first
+        if abs(np.log10(n2)) < 1e-3:
+            break
+        f = n2 ** (-0.5 / (nx * ny))
+        for iy in range(1, ny + 1):
+            for ix in range(1, nx + 1):
+                gp.set_site(ix, iy, gp.site(ix, iy) * f)
+    return first, peps.peps_norm2(gp, chi).real
+
+
 def cpu_baseline(flops_per_step, seconds=12.0):
     """numpy (OpenBLAS, all host threads) on the dominant contraction shape of
     the workload — the vertical-bond environment closure, an (a^2 x K) by
@@ -93,13 +133,17 @@ def main():
               "is a design note, not a measured configuration", file=sys.stderr)
         sys.exit(2)
 
+    # one progress line per bond update on stderr (a C3 step runs for minutes)
+    os.environ.setdefault("PEPS_HIP_VERBOSE", "1")
     from lgt_amd import peps
     g, t, m = 1.0, 1.0, 0.5
     ctx = peps.Context(0)
     log(f"setup: {a.nx}x{a.ny} gd={a.gd} D={a.D} chi={a.chi}, {a.su_steps} SU steps")
     t0 = time.perf_counter()
     gp = make_state(ctx, a.nx, a.ny, a.gd, a.D, a.seed, a.su_steps, g, t, m)
-    log(f"setup done in {time.perf_counter() - t0:.1f} s")
+    n_before, n_after = normalise(gp, a.nx, a.ny, a.chi)
+    log(f"setup done in {time.perf_counter() - t0:.1f} s (<psi|psi> {n_before:.3e} -> {n_after:.6f} "
+        f"after site-uniform rescaling; bond dims h {[len(gp.bond(peps.PEPS_H, 1, iy)[0]) for iy in range(1, a.ny + 1)]})")
     ctx.set_tier(a.tier)
     for w in range(a.warmup):
         tw = time.perf_counter()

@@ -2,6 +2,9 @@
 #include "peps_internal.h"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 
 namespace peps {
 
@@ -80,6 +83,7 @@ Ctx::Ctx(int dev) : device(dev) {
   pool.reset(new Pool(dev));
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+  if (const char* v = std::getenv("PEPS_HIP_VERBOSE")) verbose = std::atoi(v);
 }
 
 Ctx::~Ctx() {
@@ -129,6 +133,14 @@ void Ctx::resolve_timing() {
     ev_free.push_back(p.b);
   }
   pending.clear();
+}
+
+void Ctx::note(const std::string& what) {
+  if (!verbose) return;
+  static const auto t0 = std::chrono::steady_clock::now();
+  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::fprintf(stderr, "[peps %9.2fs] %s\n", t, what.c_str());
+  std::fflush(stderr);
 }
 
 Tensor Ctx::alloc(std::vector<int64_t> dims, bool cplx) {

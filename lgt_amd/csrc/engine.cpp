@@ -18,6 +18,9 @@
 //    for every p).
 #include "engine.h"
 
+#include <chrono>
+#include <cstdio>
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -409,7 +412,13 @@ MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int c
     cols[i].mps = mps[i];
     cols[i].E = row[i];
   }
-  return compress_columns(c, cols, chi, CMPS_CUT, mps[0].cplx);
+  MPS out = compress_columns(c, cols, chi, CMPS_CUT, mps[0].cplx);
+  if (c.verbose) {
+    int64_t mx = 1;
+    for (auto& t : out) mx = std::max(mx, t.dims[2]);
+    c.note(std::string("boundary row absorbed (Gram form), max bond ") + std::to_string(mx));
+  }
+  return out;
 }
 
 MPS trivial_mps(Ctx& c, int n, bool cplx) {
@@ -548,46 +557,46 @@ Env bond_env_h(State& s, int ix, int iy, const MPS& top, const MPS& bot, const T
 
 Env bond_env_v(State& s, int ix, int iy, const MPS& top, const MPS& bot, const Tensor& Lenv,
                const Tensor& Renv) {
-  // :507-547.  env = Σ Lenv tp Renv bt QA QA* QB QB*, closed as
-  // G1[A,m,n,F,x,a,y,c] (Lenv,bt,QA,QA*) · G2[F,x,y,A,m,f,n,g] (Renv,tp,QB,QB*),
-  // streamed over chunks of the bottom boundary bond F when large.
+  // :507-547.  env = Σ Lenv tp Renv bt QA QA* QB QB*.  The four boundary
+  // pieces are first closed into the frame Fr[k,b,m,n,e,d | x,y,z,w,v,u]
+  // (the twelve D legs around the two sites): one GEMM over the cut (A, F)
+  // of the two boundary bonds, χ²D¹² MACs, accumulated over chunks of the
+  // bottom-right bond F.  The reduced sites are then absorbed one tensor at
+  // a time.  For D < 8 this is cheaper than closing two half-environments
+  // with open reduced legs (χ²D⁴a²b² MACs, a ~ dp·D).
   Ctx& c = *s.c;
   Tensor A = sqrt_absorbed(s, ix, iy), B = sqrt_absorbed(s, ix, iy + 1);
   Env E;
   qr_reduce(c, A, {1, 2, 4, 0, 3}, 3, &E.QA, &E.RA);  // (l,r,d | p,bond)
   qr_reduce(c, B, {1, 2, 3, 0, 4}, 3, &E.QB, &E.RB);  // (l,r,u | p,bond)
-  Tensor tp = split_mps(top[ix - 1], B.dims[3]);
-  Tensor bt = split_mps(bot[ix - 1], A.dims[4]);
-  const int64_t chiB = bt.dims[3];
-  const int64_t a = E.QA.dims[3], b = E.QB.dims[3];
-  // largest per-F intermediate of the two chains, bounded to a few GiB per chunk
+  Tensor tp = split_mps(top[ix - 1], B.dims[3]);       // (A, v, u, C)
+  Tensor bt = split_mps(bot[ix - 1], A.dims[4]);       // (B, e, d, F)
+  const int64_t chiA = Lenv.dims[0], chiF = bt.dims[3];
+  const int64_t M = Lenv.dims[2] * Lenv.dims[3] * Lenv.dims[4] * Lenv.dims[5] * bt.dims[1] * bt.dims[2];
+  const int64_t N = Renv.dims[2] * Renv.dims[3] * Renv.dims[4] * Renv.dims[5] * tp.dims[1] * tp.dims[2];
+  PEPS_CHECK(Renv.dims[1] == chiF && tp.dims[0] == chiA && tp.dims[3] == Renv.dims[0], PS_ESHAPE,
+             "bond_env_v: boundary bond mismatch");
+  // chunk of F such that the two chunk operands stay within 8 GiB
   const double esz = s.cplx ? 16.0 : 8.0;
-  const double A_ = double(Lenv.dims[0]), m_ = double(Lenv.dims[4]), n_ = double(Lenv.dims[5]);
-  const double k_ = double(Lenv.dims[2]), x_ = double(E.QA.dims[1]), e_ = double(bt.dims[1]);
-  const double fB = double(E.QB.dims[1]);
-  const double perF = esz * std::max({A_ * m_ * n_ * x_ * x_ * double(a) * double(a),
-                                      A_ * k_ * k_ * m_ * n_ * e_ * e_,
-                                      x_ * x_ * A_ * m_ * n_ * double(b) * double(b),
-                                      x_ * x_ * fB * fB * A_ * double(tp.dims[1]) * double(tp.dims[2])});
-  const double budget = 4.0 * double(1ull << 30);
-  int64_t fc = std::max<int64_t>(1, std::min<int64_t>(chiB, int64_t(budget / std::max(perF, 1.0))));
-  Tensor Rp = permute(c, Renv, {0, 2, 3, 4, 5, 1});  // (C,x,y,z,w,F)
-  Tensor env;
-  for (int64_t f0 = 0; f0 < chiB; f0 += fc) {
-    int64_t f1 = std::min(chiB, f0 + fc);
-    Tensor btc = bt.slice_last(f0, f1);
-    Tensor Rc = Rp.slice_last(f0, f1);
-    Tensor x1 = contract(c, Lenv, "ABkbmn", btc, "BedF", "AkbmnedF");
-    Tensor x2 = contract(c, x1, "AkbmnedF", E.QA, "kxda", "AbmneFxa");
-    Tensor G1 = contract(c, x2, "AbmneFxa", E.QA, "byec", "AmnFxayc", false, true);
-    Tensor y1 = contract(c, Rc, "CxyzwF", tp, "AvuC", "FxyzwAvu");
-    Tensor y2 = contract(c, y1, "FxyzwAvu", E.QB, "mzuf", "FxywAvmf");
-    Tensor G2 = contract(c, y2, "FxywAvmf", E.QB, "nwvg", "FxyAmfng", false, true);
-    Tensor part = contract(c, G1, "AmnFxayc", G2, "FxyAmfng", "afcg");
-    if (env.empty()) env = part;
-    else axpy_inplace(c, env, part, 1.0);
+  const double per_f = esz * double(M + N) * double(chiA) * 2.0;  // operands + contract's staging
+  int64_t fc = std::max<int64_t>(1, std::min<int64_t>(chiF, int64_t(8.0 * double(1ull << 30) / per_f)));
+  Tensor Rp = permute(c, Renv, {0, 2, 3, 4, 5, 1});  // (C, x, y, z, w, F)
+  Tensor frame = c.alloc({M, N}, s.cplx);
+  for (int64_t f0 = 0; f0 < chiF; f0 += fc) {
+    const int64_t f1 = std::min(chiF, f0 + fc);
+    Tensor LB = contract(c, Lenv, "ABkbmn", bt.slice_last(f0, f1), "BedF", "kbmnedAF");
+    Tensor TR = contract(c, Rp.slice_last(f0, f1), "CxyzwF", tp, "AvuC", "xyzwvuAF");
+    // frame (M x N) += LB (M x K) * TR (N x K)^T, K = (A, F-chunk)
+    gemm(c, s.cplx, M, N, chiA * (f1 - f0), LB.data(), false, false, TR.data(), false, false,
+         frame.data(), f0 == 0 ? 0.0 : 1.0);
   }
-  E.env = env;
+  Tensor fr = frame.reshaped({Lenv.dims[2], Lenv.dims[3], Lenv.dims[4], Lenv.dims[5], bt.dims[1],
+                              bt.dims[2], Renv.dims[2], Renv.dims[3], Renv.dims[4], Renv.dims[5],
+                              tp.dims[1], tp.dims[2]});
+  Tensor F1 = contract(c, fr, "kbmnedxyzwvu", E.QB, "mzuf", "kbnedxywvf");
+  Tensor F2 = contract(c, F1, "kbnedxywvf", E.QB, "nwvg", "kbedxyfg", false, true);
+  Tensor F3 = contract(c, F2, "kbedxyfg", E.QA, "kxda", "beyfga");
+  E.env = contract(c, F3, "beyfga", E.QA, "byec", "afcg", false, true);
   return E;
 }
 
@@ -753,7 +762,13 @@ static Tensor solve_herm_multi(Ctx& c, const Tensor& N, const Tensor& S, double 
   for (double w : e.w) wmax = std::max(wmax, w);
   double floor = reg * std::max(wmax, 1e-30);
   std::vector<double> inv(e.w.size());
-  for (size_t i = 0; i < e.w.size(); ++i) inv[i] = e.w[i] > floor ? 1.0 / e.w[i] : 0.0;
+  int64_t floored = 0;
+  for (size_t i = 0; i < e.w.size(); ++i) {
+    inv[i] = e.w[i] > floor ? 1.0 / e.w[i] : 0.0;
+    floored += e.w[i] > floor ? 0 : 1;
+  }
+  // solves with / without floored modes (statistics only)
+  c.count(floored ? "als_solve[floored]" : "als_solve[full rank]", double(floored), 0);
   Tensor T = matmul(c, e.V, S, true, false, true, false);  // V^H S
   scale_rows(c, T, inv);
   return matmul(c, e.V, T);
@@ -1220,6 +1235,14 @@ static MPS next_bottom(State& s, const MPS& prev, int iy, int chi) {
   return absorb_compress(*s.c, prev, row_dl(s, iy - 1), false, chi);
 }
 
+static void progress(Ctx& c, char orient, int ix, int iy, const ALSInfo& inf) {
+  if (!c.verbose) return;
+  c.note(std::string(1, orient) + "-bond (" + std::to_string(ix) + "," + std::to_string(iy) + "): " +
+         std::to_string(inf.sweeps) + " ALS sweeps, " + (inf.accepted ? "accepted" : "kept SVD init"));
+}
+
+static void note(Ctx& c, const char* what) { c.note(what); }
+
 static void h_row(State& s, int iy, const MPS& top, const MPS& bot, bool l2r, const Params& p,
                   GateCache& gc, std::vector<ALSInfo>* infos) {
   Ctx& c = *s.c;
@@ -1232,6 +1255,7 @@ static void h_row(State& s, int iy, const MPS& top, const MPS& bot, bool l2r, co
     ALSInfo inf;
     fu_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, p.n_als, p.als_tol, E, &inf);
     if (infos) infos->push_back(inf);
+    progress(c, 'h', ix, iy, inf);
   };
   if (l2r) {
     std::vector<Tensor> Renv(nx + 2);
@@ -1256,6 +1280,7 @@ static void h_row(State& s, int iy, const MPS& top, const MPS& bot, bool l2r, co
 
 static void h_sweep(State& s, bool l2r, const Params& p, GateCache& gc, std::vector<ALSInfo>* infos) {
   std::vector<MPS> tops = all_tops(s, p.chi);
+  note(*s.c, "top boundaries of the horizontal sweep built");
   MPS bot;
   for (int iy = 1; iy <= s.ny; ++iy) {
     bot = next_bottom(s, bot, iy, p.chi);
@@ -1267,6 +1292,7 @@ static void v_sweep(State& s, const Params& p, GateCache& gc, std::vector<ALSInf
   Ctx& c = *s.c;
   const int nx = s.nx;
   std::vector<MPS> tops = all_tops(s, p.chi);
+  note(c, "top boundaries of the vertical sweep built");
   MPS bot;
   for (int iy = 1; iy < s.ny; ++iy) {
     bot = next_bottom(s, bot, iy, p.chi);
@@ -1286,6 +1312,7 @@ static void v_sweep(State& s, const Params& p, GateCache& gc, std::vector<ALSInf
       ALSInfo inf;
       fu_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 1.0), p.Dmax, p.n_als, p.als_tol, E, &inf);
       if (infos) infos->push_back(inf);
+      progress(c, 'v', ix, iy, inf);
     }
   }
 }

@@ -1298,8 +1298,9 @@ int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sw
   const int half = int((n + 1) / 2);
   const int waves = std::max(1, std::min(16, half));
   hipEvent_t ev = nullptr;
-  c.tbegin("svd", &ev);
-  c.count("svd", (A.cplx ? 4.0 : 1.0) * 10.0 * 6.0 * double(m) * n * n, 0);
+  const std::string cls = Ctx::bucket("svd", n);
+  c.tbegin(cls, &ev);
+  c.count(cls, (A.cplx ? 4.0 : 1.0) * 10.0 * 6.0 * double(m) * n * n, 0);
   if (A.cplx)
     hipLaunchKernelGGL(jacobi_svd_kernel<true>, dim3(1), dim3(64 * waves), 0, c.stream,
                        (double2*)A.data(), (double2*)V.data(), sig.data(), int(m), int(n), tol,
@@ -1308,11 +1309,230 @@ int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sw
     hipLaunchKernelGGL(jacobi_svd_kernel<false>, dim3(1), dim3(64 * waves), 0, c.stream, A.data(),
                        V.data(), sig.data(), int(m), int(n), tol, max_sweeps, (int*)buf->ptr);
   PEPS_HIP(hipGetLastError());
-  c.tend("svd", ev);
+  c.tend(cls, ev);
   int sw = 0;
   PEPS_HIP(hipMemcpyAsync(&sw, buf->ptr, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   PEPS_HIP(hipStreamSynchronize(c.stream));
   return sw;
+}
+
+}  // namespace peps
+
+namespace peps {
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Zero lines of a matrix; Householder QR of narrow panels
+// ════════════════════════════════════════════════════════════════════════════
+
+// rf[r] = 1 / cf[c] = 1 when row r / column c of M (m x n) has a non-zero
+// entry (every writer stores the same value).
+__global__ void nonzero_lines_kernel(const double* __restrict__ M, int64_t m, int64_t n, int esz,
+                                     int* __restrict__ rf, int* __restrict__ cf) {
+  const int64_t tot = m * n;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < tot;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const bool nz = M[i * esz] != 0.0 || (esz == 2 && M[i * esz + 1] != 0.0);
+    if (nz) {
+      rf[i % m] = 1;
+      cf[i / m] = 1;
+    }
+  }
+}
+
+void nonzero_lines(Ctx& c, const Tensor& M, std::vector<int64_t>* rows, std::vector<int64_t>* cols) {
+  PEPS_CHECK(M.rank() == 2, PS_ESHAPE, "nonzero_lines: matrix expected");
+  const int64_t m = M.dims[0], n = M.dims[1];
+  rows->clear();
+  cols->clear();
+  if (m == 0 || n == 0) return;
+  auto buf = std::make_shared<Buffer>();
+  buf->pool = c.pool.get();
+  buf->bytes = size_t(m + n) * sizeof(int);
+  buf->ptr = c.pool->get(buf->bytes);
+  PEPS_HIP(hipMemsetAsync(buf->ptr, 0, buf->bytes, c.stream));
+  int* rf = static_cast<int*>(buf->ptr);
+  unsigned g = unsigned(std::min<int64_t>(cdiv(m * n, 256), 16384));
+  hipLaunchKernelGGL(nonzero_lines_kernel, dim3(g), dim3(256), 0, c.stream, M.data(), m, n,
+                     M.cplx ? 2 : 1, rf, rf + m);
+  PEPS_HIP(hipGetLastError());
+  std::vector<int> h(m + n);
+  PEPS_HIP(hipMemcpyAsync(h.data(), buf->ptr, buf->bytes, hipMemcpyDeviceToHost, c.stream));
+  PEPS_HIP(hipStreamSynchronize(c.stream));
+  for (int64_t i = 0; i < m; ++i)
+    if (h[i]) rows->push_back(i);
+  for (int64_t j = 0; j < n; ++j)
+    if (h[m + j]) cols->push_back(j);
+}
+
+// Householder QR of a narrow panel A (m x n, n <= QR_SMALL_N), one workgroup
+// of QR_THREADS threads, A in global memory and overwritten LAPACK-style: R in
+// the upper triangle, the reflectors v_j (v_j[j] = 1 implicit) below it,
+// tau[j] their scalars.  Reflector j (Golub & Van Loan 5.1.2 in the
+// convention of LAPACK's larfg): beta = -sign(Re a_jj) ||A[j:, j]||,
+// tau = (beta - a_jj) / beta, v = A[j:, j] / (a_jj - beta); tau = 0 when the
+// column below the diagonal is already zero and a_jj is real.  The trailing
+// columns are updated with H^H = I - conj(tau) v v^H.
+constexpr int QR_SMALL_N = 64;
+constexpr int QR_THREADS = 256;
+
+template <bool CP>
+__global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(typename JT<CP>::T* __restrict__ A,
+                                                              typename JT<CP>::T* __restrict__ tau,
+                                                              int64_t m, int n) {
+  typedef typename JT<CP>::T T;
+  __shared__ double part[QR_THREADS / 64];
+  __shared__ T s_tau, s_scale;
+  __shared__ int s_skip;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = QR_THREADS / 64;
+  const int k = int(std::min<int64_t>(m, n));
+  for (int j = 0; j < k; ++j) {
+    T* aj = A + int64_t(j) * m;
+    // sigma = sum_{i>j} |a_ij|^2
+    double sg = 0.0;
+    for (int64_t i = j + 1 + tid; i < m; i += QR_THREADS) sg += JT<CP>::abs2(aj[i]);
+    for (int o = 32; o > 0; o >>= 1) sg += __shfl_xor(sg, o);
+    if (lane == 0) part[wave] = sg;
+    __syncthreads();
+    if (tid == 0) {
+      double sigma = 0.0;
+      for (int w = 0; w < nw; ++w) sigma += part[w];
+      const T alpha = aj[j];
+      const double ar = JT<CP>::re(alpha);
+      double ai = 0.0;
+      if constexpr (CP) ai = alpha.y;
+      if (sigma == 0.0 && ai == 0.0) {
+        s_skip = 1;
+        set_zero(s_tau);
+      } else {
+        const double nrm = sqrt(ar * ar + ai * ai + sigma);
+        const double beta = ar >= 0.0 ? -nrm : nrm;
+        // tau = (beta - alpha) / beta, scale = 1 / (alpha - beta)
+        if constexpr (CP) {
+          s_tau = make_double2((beta - ar) / beta, -ai / beta);
+          const double dr = ar - beta, di = ai, dd = dr * dr + di * di;
+          s_scale = make_double2(dr / dd, -di / dd);
+          aj[j] = make_double2(beta, 0.0);
+        } else {
+          s_tau = (beta - ar) / beta;
+          s_scale = 1.0 / (ar - beta);
+          aj[j] = beta;
+        }
+        s_skip = 0;
+      }
+      tau[j] = s_tau;
+    }
+    __syncthreads();
+    if (s_skip) continue;  // uniform: H_j = I
+    const T scale = s_scale;
+    for (int64_t i = j + 1 + tid; i < m; i += QR_THREADS) aj[i] = mulc(aj[i], scale);
+    __syncthreads();
+    // trailing columns: w = v^H a_c ; a_c -= conj(tau) v w
+    const T ctau = conjv(s_tau);
+    for (int cix = j + 1 + wave; cix < n; cix += nw) {
+      T* ac = A + int64_t(cix) * m;
+      double wr = 0.0, wi = 0.0;
+      for (int64_t i = j + lane; i < m; i += 64) {
+        T v = i == j ? T{} : aj[i];
+        if (i == j) set_one(v);
+        const T p = mulc(conjv(v), ac[i]);
+        if constexpr (CP) { wr += p.x; wi += p.y; } else { wr += p; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        wr += __shfl_xor(wr, o);
+        wi += __shfl_xor(wi, o);
+      }
+      T w;
+      if constexpr (CP) w = make_double2(wr, wi); else w = wr;
+      const T f = mulc(ctau, w);
+      for (int64_t i = j + lane; i < m; i += 64) {
+        T v = i == j ? T{} : aj[i];
+        if (i == j) set_one(v);
+        const T d = mulc(v, f);
+        if constexpr (CP) ac[i] = make_double2(ac[i].x - d.x, ac[i].y - d.y); else ac[i] = ac[i] - d;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Q (m x k) = H_0 H_1 ... H_{k-1} [I; 0] from the reflectors left in A by
+// qr_panel_kernel (accumulated backwards, H_j = I - tau_j v_j v_j^H).
+template <bool CP>
+__global__ __launch_bounds__(QR_THREADS) void qr_form_q_kernel(const typename JT<CP>::T* __restrict__ A,
+                                                               const typename JT<CP>::T* __restrict__ tau,
+                                                               typename JT<CP>::T* __restrict__ Q,
+                                                               int64_t m, int k) {
+  typedef typename JT<CP>::T T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = QR_THREADS / 64;
+  const int64_t mk = m * k;
+  for (int64_t i = tid; i < mk; i += QR_THREADS) {
+    const int64_t r = i % m, cc = i / m;
+    if (r == cc) set_one(Q[i]); else set_zero(Q[i]);
+  }
+  __syncthreads();
+  for (int j = k - 1; j >= 0; --j) {
+    const T tj = tau[j];
+    bool zero_tau;
+    if constexpr (CP) zero_tau = tj.x == 0.0 && tj.y == 0.0; else zero_tau = tj == 0.0;
+    if (zero_tau) continue;  // uniform
+    const T* aj = A + int64_t(j) * m;
+    for (int cix = j + wave; cix < k; cix += nw) {
+      T* qc = Q + int64_t(cix) * m;
+      double wr = 0.0, wi = 0.0;
+      for (int64_t i = j + lane; i < m; i += 64) {
+        T v = i == j ? T{} : aj[i];
+        if (i == j) set_one(v);
+        const T p = mulc(conjv(v), qc[i]);
+        if constexpr (CP) { wr += p.x; wi += p.y; } else { wr += p; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        wr += __shfl_xor(wr, o);
+        wi += __shfl_xor(wi, o);
+      }
+      T w;
+      if constexpr (CP) w = make_double2(wr, wi); else w = wr;
+      const T f = mulc(tj, w);
+      for (int64_t i = j + lane; i < m; i += 64) {
+        T v = i == j ? T{} : aj[i];
+        if (i == j) set_one(v);
+        const T d = mulc(v, f);
+        if constexpr (CP) qc[i] = make_double2(qc[i].x - d.x, qc[i].y - d.y); else qc[i] = qc[i] - d;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+int qr_small_max_cols() { return QR_SMALL_N; }
+
+void qr_panel(Ctx& c, Tensor& W, Tensor& tau) {
+  PEPS_CHECK(W.rank() == 2 && W.dims[1] <= QR_SMALL_N && W.dims[0] < (1ll << 40), PS_EUNSUPPORTED,
+             "qr_panel: at most 64 columns");
+  const int64_t m = W.dims[0];
+  const int n = int(W.dims[1]);
+  PEPS_CHECK(tau.numel() >= std::min<int64_t>(m, n) && tau.cplx == W.cplx, PS_ESHAPE, "qr_panel: tau");
+  if (m == 0 || n == 0) return;
+  if (W.cplx)
+    hipLaunchKernelGGL(qr_panel_kernel<true>, dim3(1), dim3(QR_THREADS), 0, c.stream, (double2*)W.data(),
+                       (double2*)tau.data(), m, n);
+  else
+    hipLaunchKernelGGL(qr_panel_kernel<false>, dim3(1), dim3(QR_THREADS), 0, c.stream, W.data(),
+                       tau.data(), m, n);
+  PEPS_HIP(hipGetLastError());
+}
+
+void qr_form_q(Ctx& c, const Tensor& W, const Tensor& tau, Tensor& Q) {
+  const int64_t m = W.dims[0];
+  const int k = int(std::min<int64_t>(m, W.dims[1]));
+  PEPS_CHECK(Q.numel() == m * k && Q.cplx == W.cplx && k <= QR_SMALL_N, PS_ESHAPE, "qr_form_q: Q");
+  if (m == 0 || k == 0) return;
+  if (W.cplx)
+    hipLaunchKernelGGL(qr_form_q_kernel<true>, dim3(1), dim3(QR_THREADS), 0, c.stream,
+                       (const double2*)W.data(), (const double2*)tau.data(), (double2*)Q.data(), m, k);
+  else
+    hipLaunchKernelGGL(qr_form_q_kernel<false>, dim3(1), dim3(QR_THREADS), 0, c.stream, W.data(),
+                       tau.data(), Q.data(), m, k);
+  PEPS_HIP(hipGetLastError());
 }
 
 }  // namespace peps

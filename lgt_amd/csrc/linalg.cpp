@@ -68,8 +68,18 @@ QR qr_thin(Ctx& c, const Tensor& A) {
   Tensor W = copy_tensor(c, A);
   Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
   hipEvent_t ev = nullptr;
-  c.tbegin("qr", &ev);
-  c.count("qr", (A.cplx ? 4.0 : 1.0) * (4.0 * m * n * k - 2.0 * (m + n) * k * k + 4.0 * k * k * k / 3.0), 0);
+  const std::string cls = Ctx::bucket("qr", n);
+  c.tbegin(cls, &ev);
+  c.count(cls, (A.cplx ? 4.0 : 1.0) * (4.0 * m * n * k - 2.0 * (m + n) * k * k + 4.0 * k * k * k / 3.0), 0);
+  if (n <= qr_small_max_cols()) {  // narrow panel: one-workgroup Householder kernel
+    QR r;
+    if (k > 0) qr_panel(c, W, tau);
+    r.R = upper_trapezoid(c, W, k);
+    r.Q = c.alloc({m, k}, A.cplx);
+    if (k > 0) qr_form_q(c, W, tau, r.Q);
+    c.tend(cls, ev);
+    return r;
+  }
   if (k > 0) {
     if (A.cplx)
       ck(rocsolver_zgeqrf(c.blas, int(m), int(n), (rocblas_double_complex*)W.data(), int(m),
@@ -94,7 +104,7 @@ QR qr_thin(Ctx& c, const Tensor& A) {
     if (r.Q.bytes())
       PEPS_HIP(hipMemcpyAsync(r.Q.data(), W.data(), r.Q.bytes(), hipMemcpyDeviceToDevice, c.stream));
   }
-  c.tend("qr", ev);
+  c.tend(cls, ev);
   return r;
 }
 
@@ -106,8 +116,15 @@ Tensor qr_r_only(Ctx& c, const Tensor& A) {
   Tensor W = copy_tensor(c, A);
   Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
   hipEvent_t ev = nullptr;
-  c.tbegin("qr", &ev);
-  c.count("qr", (A.cplx ? 4.0 : 1.0) * (2.0 * m * n * k - (m + n) * k * k + 2.0 * k * k * k / 3.0), 0);
+  const std::string cls = Ctx::bucket("qr", n);
+  c.tbegin(cls, &ev);
+  c.count(cls, (A.cplx ? 4.0 : 1.0) * (2.0 * m * n * k - (m + n) * k * k + 2.0 * k * k * k / 3.0), 0);
+  if (n <= qr_small_max_cols()) {
+    if (k > 0) qr_panel(c, W, tau);
+    Tensor R = upper_trapezoid(c, W, k);
+    c.tend(cls, ev);
+    return R;
+  }
   if (k > 0) {
     if (A.cplx)
       ck(rocsolver_zgeqrf(c.blas, int(m), int(n), (rocblas_double_complex*)W.data(), int(m),
@@ -116,7 +133,7 @@ Tensor qr_r_only(Ctx& c, const Tensor& A) {
       ck(rocsolver_dgeqrf(c.blas, int(m), int(n), W.data(), int(m), tau.data()), "dgeqrf");
   }
   Tensor R = upper_trapezoid(c, W, k);
-  c.tend("qr", ev);
+  c.tend(cls, ev);
   return R;
 }
 
@@ -126,6 +143,8 @@ static Tensor conj_transpose(Ctx& c, const Tensor& A) {
   conj_inplace(c, T);
   return T;
 }
+
+static SVD svd_dense(Ctx& c, const Tensor& A);
 
 SVD svd_thin(Ctx& c, const Tensor& A) {
   // Thin SVD A = U diag(S) Vt with S descending.  Wide inputs are handled
@@ -148,6 +167,31 @@ SVD svd_thin(Ctx& c, const Tensor& A) {
     r.Vt = conj_transpose(c, eye(c, n, k, A.cplx));
     return r;
   }
+  // rows and columns that are exactly zero (Gauss-masked blocks have many)
+  // do not change the non-zero singular triplets: factorise the compacted
+  // block and pad with zero triplets
+  std::vector<int64_t> nzr, nzc;
+  nonzero_lines(c, A, &nzr, &nzc);
+  if (int64_t(nzr.size()) < m || int64_t(nzc.size()) < n) {
+    SVD f = svd_dense(c, gather2d(c, A, nzr, nzc));
+    const int64_t kc = int64_t(f.S.size());
+    std::vector<int64_t> jk(kc);
+    for (int64_t i = 0; i < kc; ++i) jk[i] = i;
+    r.U = c.zeros({m, k}, A.cplx);
+    r.Vt = c.zeros({k, n}, A.cplx);
+    scatter_cols(c, r.U, f.U, nzr, jk, jk);
+    scatter_rows(c, r.Vt, f.Vt, jk, jk, nzc);
+    r.S = f.S;
+    r.S.resize(k, 0.0);
+    return r;
+  }
+  return svd_dense(c, A);
+}
+
+static SVD svd_dense(Ctx& c, const Tensor& A) {
+  // A has no zero row or column
+  const int64_t m = A.dims[0], n = A.dims[1];
+  SVD r;
   const bool tr = m < n;
   Tensor B = tr ? conj_transpose(c, A) : A;
   const int64_t mb = B.dims[0], nb = B.dims[1];
@@ -208,8 +252,9 @@ EIG eigh(Ctx& c, const Tensor& A) {
   Tensor w = c.alloc({n}, false), e = c.alloc({n}, false);
   DevInt info(c);
   hipEvent_t ev = nullptr;
-  c.tbegin("eigh", &ev);
-  c.count("eigh", (A.cplx ? 4.0 : 1.0) * 9.0 * double(n) * n * n, 0);
+  const std::string cls = Ctx::bucket("eigh", n);
+  c.tbegin(cls, &ev);
+  c.count(cls, (A.cplx ? 4.0 : 1.0) * 9.0 * double(n) * n * n, 0);
   if (A.cplx)
     ck(rocsolver_zheevd(c.blas, rocblas_evect_original, rocblas_fill_lower, int(n),
                         (rocblas_double_complex*)r.V.data(), int(n), w.data(), e.data(), info.p),
@@ -218,7 +263,7 @@ EIG eigh(Ctx& c, const Tensor& A) {
     ck(rocsolver_dsyevd(c.blas, rocblas_evect_original, rocblas_fill_lower, int(n), r.V.data(),
                         int(n), w.data(), e.data(), info.p),
        "dsyevd");
-  c.tend("eigh", ev);
+  c.tend(cls, ev);
   const int inf = info.get(c);
   PEPS_CHECK(inf == 0, PS_ESOLVER, "syevd did not converge (info=" + std::to_string(inf) + ")");
   r.w = c.to_host(w);

@@ -136,6 +136,11 @@ class Ctx {
   // boundary-MPS compression: 0 auto (Gram form when the exact bond > 1024),
   // 1 QR/SVD sweep on the exact product, 2 Gram form
   int compress_mode = 0;
+  // progress lines on stderr, one per bond update (environment variable
+  // PEPS_HIP_VERBOSE=1 at context creation)
+  int verbose = 0;
+  // one progress line on stderr when verbose
+  void note(const std::string& what);
 
   // Per kernel-class accounting ("gemm", "permute", "svd", ...).
   std::map<std::string, KernelStat> stats;
@@ -151,6 +156,11 @@ class Ctx {
   void tbegin(const std::string& cls, hipEvent_t* a);
   void tend(const std::string& cls, hipEvent_t a);
   void resolve_timing();
+  // class name with a size bucket, e.g. "qr[<=64]" (factorisation statistics)
+  static std::string bucket(const char* cls, int64_t n) {
+    const char* b = n <= 16 ? "<=16" : n <= 64 ? "<=64" : n <= 256 ? "<=256" : n <= 1024 ? "<=1024" : ">1024";
+    return std::string(cls) + "[" + b + "]";
+  }
   void count(const std::string& cls, double flops, double bytes) {
     auto& s = stats[cls];
     s.flops += flops;
@@ -219,6 +229,14 @@ void hermitize(Ctx& c, Tensor& N);
 // Copy a sub-block: dst(:, j0:j0+n) = src (column block).
 Tensor copy_tensor(Ctx& c, const Tensor& t);
 bool all_finite(Ctx& c, const Tensor& t);
+// Indices of the rows / columns of M that hold a non-zero entry (synchronises).
+void nonzero_lines(Ctx& c, const Tensor& M, std::vector<int64_t>* rows, std::vector<int64_t>* cols);
+// Householder QR of a panel with at most qr_small_max_cols() columns (one
+// workgroup): W overwritten LAPACK-style (R above, reflectors below), tau
+// their scalars; qr_form_q builds the thin Q (m x min(m,n)).
+int qr_small_max_cols();
+void qr_panel(Ctx& c, Tensor& W, Tensor& tau);
+void qr_form_q(Ctx& c, const Tensor& W, const Tensor& tau, Tensor& Q);
 // One pass: are all entries finite, is any entry non-zero (synchronises).
 void scan_values(Ctx& c, const Tensor& t, bool* finite, bool* nonzero);
 // The first k columns of the m x m identity.

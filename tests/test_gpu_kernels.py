@@ -121,7 +121,9 @@ def test_svd(ctx, cplx):
         Sr = np.linalg.svd(A, compute_uv=False)
         np.testing.assert_allclose(S, Sr, atol=1e-14 * Sr.max())
         assert np.abs((U * S) @ Vt - A).max() < 1e-14 * Sr.max()
-        assert np.abs(Vt @ np.conj(Vt.T) - np.eye(len(S))).max() < 1e-13
+        nz = S > 0
+        Vk = Vt[nz, :]
+        assert np.abs(Vk @ np.conj(Vk.T) - np.eye(nz.sum())).max() < 1e-13
     # the all-zero blocks of Gauss-masked matrices: S = 0, identity vectors
     for (m, n) in [(4, 29), (29, 4), (1, 1)]:
         A = np.zeros((m, n), complex if cplx else float)
