@@ -8,7 +8,14 @@ to D=6 by simple-update steps (tau=0.05); set-up is not timed.
 
 One step = one trotter_step_v4 (h half-sweep, v sweep, h half-sweep: 168
 bond updates with their boundary-MPS environments and ALS solves), timed
-between device synchronisations.  Prints ONE JSON line.
+between device synchronisations.  The step runs with PEPS_STEP_RENORM (every
+boundary MPS is normalised after each row absorption, so the reference's
+absolute truncation thresholds do not collapse the chi=144 boundaries of an
+8x8 lattice whose <psi|psi> drifts by ~30 orders of magnitude per sweep) and
+PEPS_STEP_POSITIVE_COST (an ALS refinement is kept only if its cost is >= 0:
+the chi-truncated environments are not positive semidefinite, and accepting
+negative-cost bonds zeroes the 8x8 state within a sweep).  Prints ONE JSON
+line.
 """
 from __future__ import annotations
 
@@ -147,7 +154,8 @@ def main():
     ctx.set_tier(a.tier)
     for w in range(a.warmup):
         tw = time.perf_counter()
-        peps.trotter_step_v4(gp, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol)
+        peps.trotter_step_v4(gp, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
+                                 positive_cost=True)
         ctx.sync()
         log(f"warmup step {w + 1}/{a.warmup}: {time.perf_counter() - tw:.2f} s")
     ctx.stats_reset()
@@ -157,7 +165,8 @@ def main():
     als = dict(bonds=0, accepted=0, sweeps=0)
     for s in range(a.steps):
         ts = time.perf_counter()
-        st = peps.trotter_step_v4(gp, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol)
+        st = peps.trotter_step_v4(gp, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
+                                 positive_cost=True)
         for k in als:
             als[k] += st[k]
         ctx.sync()
@@ -189,6 +198,7 @@ def main():
         "config": {"workload": f"finite-PEPS full update, {a.nx}x{a.ny} U(1) LGT, spin-1/2 links",
                    "nx": a.nx, "ny": a.ny, "D": a.D, "chi": a.chi, "gd": a.gd, "tau": a.tau,
                    "n_als_max": a.n_als, "als_tol": a.als_tol, "tier": a.tier,
+                   "step_flags": "PEPS_STEP_RENORM|PEPS_STEP_POSITIVE_COST",
                    "bond_updates_per_step": 2 * (a.nx - 1) * a.ny + a.nx * (a.ny - 1),
                    "als_sweeps_per_step": als["sweeps"] / max(1, a.steps),
                    "als_accept_rate": als["accepted"] / max(1, als["bonds"]),
@@ -200,7 +210,7 @@ def main():
                      "avg_launch_ms": gk["ms"] / max(1, gk["launches"])},
     }
     if a.energy:
-        e = peps.compute_energy_v4(gp, g, t, m, a.chi)
+        e = peps.compute_energy_v4(gp, g, t, m, a.chi, renorm=True)
         out["config"]["energy_per_site"] = e / (a.nx * a.ny)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(total_flops / a.steps)

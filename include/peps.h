@@ -50,6 +50,25 @@ enum {
 enum { PEPS_TIER_FP64 = 0, PEPS_TIER_FP32 = 1, PEPS_TIER_BF16 = 2 };
 enum { PEPS_H = 0, PEPS_V = 1 };
 enum { PEPS_STEP_V4 = 0, PEPS_STEP_GAUGE = 1 };
+/* peps_params.flags bits.  PEPS_STEP_RENORM has no counterpart in the
+   reference: every boundary MPS built by a full-update step or an energy
+   evaluation is truncated at 1e-14 times its largest singular value (the
+   reference: 1e-14 absolute, compress_mps!) and divided by its norm after
+   each row absorption.  The environments enter the ALS solve and the
+   energy only through ratios, so the results are the reference's whenever
+   no boundary singular value lies between the two thresholds.  The v4
+   update rescales the state by 1/||S|| at every bond, so on an 8x8 lattice
+   <psi|psi> drifts by ~30 orders of magnitude within one sweep and, without
+   the flag, the boundaries collapse to bond dimension 1.  Off by default
+   (reference behaviour). */
+/* PEPS_STEP_POSITIVE_COST has no counterpart in the reference either: an ALS
+   refinement is accepted only if its cost is also >= 0.  With truncated
+   boundaries the environment need not be positive semidefinite, the cost
+   can then be driven below zero along its negative directions, and the
+   reference's test (cost < initial cost, finite_peps_fullupdate_v4.jl:145-153)
+   accepts such a bond, which on an 8x8 lattice zeroes the state within a
+   sweep.  Off by default (reference behaviour). */
+enum { PEPS_STEP_RENORM = 1, PEPS_STEP_POSITIVE_COST = 2 };
 
 typedef struct peps_params {
   double g;        /* gauge coupling g (the reference takes g, not g^2) */
@@ -59,7 +78,7 @@ typedef struct peps_params {
   int32_t D_max;   /* maximal PEPS bond dimension */
   int32_t chi;     /* boundary-MPS bond dimension */
   int32_t n_als;   /* ALS sweeps (exactly n_als when als_tol == 0) */
-  int32_t pad;
+  int32_t flags;    /* PEPS_STEP_* bits, 0 for the reference's behaviour */
   double als_tol;  /* > 0: stop ALS when the relative cost change < als_tol */
 } peps_params;
 

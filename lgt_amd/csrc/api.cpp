@@ -82,6 +82,10 @@ peps::Params to_params(const peps_params* p) {
   q.chi = p->chi;
   q.n_als = p->n_als;
   q.als_tol = p->als_tol;
+  PEPS_CHECK((p->flags & ~int32_t(PEPS_STEP_RENORM | PEPS_STEP_POSITIVE_COST)) == 0, peps::PS_EARG,
+             "unknown step flags");
+  q.renorm = (p->flags & PEPS_STEP_RENORM) != 0;
+  q.positive_cost = (p->flags & PEPS_STEP_POSITIVE_COST) != 0;
   return q;
 }
 

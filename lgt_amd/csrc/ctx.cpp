@@ -84,6 +84,7 @@ Ctx::Ctx(int dev) : device(dev) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   if (const char* v = std::getenv("PEPS_HIP_VERBOSE")) verbose = std::atoi(v);
+  if (const char* v = std::getenv("PEPS_HIP_EIG_DIRECT_MAX")) eig_direct_max = std::max(0, std::atoi(v));
 }
 
 Ctx::~Ctx() {
@@ -93,6 +94,8 @@ Ctx::~Ctx() {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : ev_free) (void)hipEventDestroy(e);
+  // cached tensors return their buffers to the pool: release them first
+  ritz_cache.clear();
   pool.reset();
   if (blas) rocblas_destroy_handle(blas);
   if (stream) (void)hipStreamDestroy(stream);

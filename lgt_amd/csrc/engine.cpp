@@ -190,7 +190,7 @@ MPS absorb_row_from_bottom(Ctx& c, const MPS& mps, const MPS& row) {
   return out;
 }
 
-void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff) {
+void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff, bool relative) {
   // :70-105  left->right QR, then right->left truncated SVD
   const size_t n = mps.size();
   if (n <= 1) return;
@@ -206,7 +206,8 @@ void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff) {
     int64_t Dl = mps[i].dims[0], dp = mps[i].dims[1], Dr = mps[i].dims[2];
     SVD f = svd_thin(c, mps[i].reshaped({Dl, dp * Dr}));
     int64_t cnt = 0;
-    for (double x : f.S) cnt += x > cutoff;
+    const double thr = relative && !f.S.empty() ? cutoff * f.S[0] : cutoff;
+    for (double x : f.S) cnt += x > thr;
     int64_t k = std::max<int64_t>(1, std::min<int64_t>({int64_t(chi), cnt, int64_t(f.S.size())}));
     std::vector<int64_t> rows(k), cols(dp * Dr);
     std::iota(rows.begin(), rows.end(), 0);
@@ -285,11 +286,19 @@ struct Column {
 };
 
 // Top-`want` eigenpairs (descending) of a Hermitian PSD matrix rho (N x N).
-void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam, Tensor* V) {
+// N <= eig_direct_max: full eigendecomposition.  Larger: Chebyshev-filtered
+// subspace iteration (degree-8 filter on [0, smallest Ritz value]) with
+// Rayleigh-Ritz on p = min(512, N, max(want+64, 2 want)) vectors, stopped
+// when the Frobenius norm of the residuals rho x_j - theta_j x_j of the
+// wanted Ritz pairs is <= 1e-12 theta_0.  With a cache key the previous
+// Ritz basis at the same key is the starting subspace.  All products here
+// run in FP64 whatever the contraction tier.
+void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam, Tensor* V,
+              const std::string* key) {
+  TierGuard fp64(c, TIER_FP64);
   const int64_t N = rho.dims[0];
   want = std::min(want, N);
-  Tensor Q, B;
-  if (N <= 512) {
+  if (N <= c.eig_direct_max) {
     Tensor A = copy_tensor(c, rho);
     hermitize(c, A);
     EIG f = eigh(c, A);
@@ -304,45 +313,90 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
     *V = gather2d(c, W, seq(N), ord);
     return;
   }
-  // subspace iteration with Rayleigh-Ritz (p <= 512 columns)
   const int64_t p = std::min<int64_t>(512, std::min<int64_t>(N, std::max<int64_t>(want + 64, 2 * want)));
-  Tensor Om = c.alloc({N, p}, rho.cplx);
-  fill_random(c, Om, 0x5eed1234abcdull + uint64_t(N));
-  Q = qr_thin(c, matmul(c, rho, Om)).Q;
-  std::vector<double> prev;
-  Tensor Wp, wv;
-  for (int it = 0; it < 60; ++it) {
-    Tensor Z = matmul(c, rho, Q);
-    if (it % 2 == 1 || it >= 4) {
-      // Rayleigh-Ritz on the current basis: convergence of the kept values
-      Tensor Bm = matmul(c, Q, Z, true, false, true, false);
-      hermitize(c, Bm);
-      std::vector<double> hw = eigh(c, Bm).w;
-      std::sort(hw.begin(), hw.end(), std::greater<double>());
-      hw.resize(want);
-      bool conv = !prev.empty();
-      for (int64_t i = 0; conv && i < want; ++i)
-        conv = std::fabs(hw[i] - prev[i]) <= 1e-13 * std::fabs(hw[0]);
-      prev = hw;
-      if (conv) break;
+  // rho comes from tier-precision products: make it exactly Hermitian, or the
+  // residuals stall at the level of its anti-Hermitian part
+  Tensor rh = copy_tensor(c, rho);
+  hermitize(c, rh);
+  // starting subspace: the cached Ritz basis (completed with random vectors)
+  Tensor Q0 = c.alloc({N, p}, rho.cplx);
+  int64_t have = 0;
+  if (key) {
+    auto it = c.ritz_cache.find(*key);
+    if (it != c.ritz_cache.end() && it->second.dims[0] == N && it->second.cplx == rho.cplx) {
+      have = std::min<int64_t>(p, it->second.dims[1]);
+      PEPS_HIP(hipMemcpyAsync(Q0.data(), it->second.data(), size_t(N * have) * Q0.esz(),
+                              hipMemcpyDeviceToDevice, c.stream));
     }
-    Q = qr_thin(c, Z).Q;
   }
-  Tensor Bm = matmul(c, Q, matmul(c, rho, Q), true, false, true, false);
-  hermitize(c, Bm);
-  EIG f = eigh(c, Bm);
-  const std::vector<double>& hw = f.w;
-  const Tensor& Wm = f.V;
-  std::vector<int64_t> ord(p);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return hw[x] > hw[y]; });
-  ord.resize(want);
-  lam->clear();
-  for (auto i : ord) lam->push_back(hw[i]);
-  *V = matmul(c, Q, gather2d(c, Wm, seq(p), ord));
+  if (have < p) {
+    Tensor rest = Q0.slice_last(have, p);
+    fill_random(c, rest, 0x5eed1234abcdull + uint64_t(N) * 7919u + uint64_t(have));
+  }
+  Tensor Q = qr_thin(c, have ? Q0 : matmul(c, rh, Q0)).Q;
+  std::vector<double> theta;
+  Tensor X;
+  constexpr int kCheb = 8;  // filter degree per outer iteration
+  for (int it = 0; it < 40; ++it) {
+    // Rayleigh-Ritz on span(Q)
+    Tensor Z = matmul(c, rh, Q);
+    Tensor T = matmul(c, Q, Z, true, false, true, false);
+    hermitize(c, T);
+    EIG f = eigh(c, T);
+    std::vector<int64_t> ord(p);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return f.w[x] > f.w[y]; });
+    Tensor W = gather2d(c, f.V, seq(p), ord);
+    theta.resize(p);
+    for (int64_t i = 0; i < p; ++i) theta[i] = f.w[ord[i]];
+    X = matmul(c, Q, W);
+    Tensor ZW = matmul(c, Z, W);  // rho X
+    // residuals of the wanted pairs: ZW[:, j] - theta_j X[:, j]
+    Tensor R = copy_tensor(c, ZW.slice_last(0, want));
+    Tensor Xs = copy_tensor(c, X.slice_last(0, want));
+    std::vector<double> mth(want);
+    for (int64_t j = 0; j < want; ++j) mth[j] = -theta[j];
+    scale_cols(c, Xs, mth);
+    axpy_inplace(c, R, Xs, 1.0);
+    const double res = std::sqrt(norm2sq(c, R));
+    c.count("top_eigs[iterations]", 1.0, 0);
+    if (res <= 1e-12 * std::max(std::fabs(theta[0]), 1e-300)) break;
+    if (it == 39) {
+      c.count("top_eigs[unconverged]", 1.0, 0);
+      break;
+    }
+    // Chebyshev filter of degree kCheb on the unwanted interval [0, theta_min]
+    // applied to the Ritz vectors (Zhou & Saad): eigencomponents above the
+    // interval grow like T_k, those inside stay bounded by 1.
+    // (the interval end is kept >= 1e-10 theta_0 so that T_8 stays far from overflow)
+    const double hi = std::max(theta[p - 1], 1e-10 * std::max(theta[0], 1e-300));
+    const double e = 0.5 * hi, c0 = 0.5 * hi;
+    Tensor Y0 = X;
+    Tensor Y1 = copy_tensor(c, ZW);  // (rho - c0) X / e
+    axpy_inplace(c, Y1, X, -c0);
+    scale(c, Y1, 1.0 / e);
+    for (int k = 2; k <= kCheb; ++k) {
+      Tensor Y2 = matmul(c, rh, Y1);  // 2 (rho - c0) Y1 / e - Y0
+      axpy_inplace(c, Y2, Y1, -c0);
+      scale(c, Y2, 2.0 / e);
+      axpy_inplace(c, Y2, Y0, -1.0);
+      Y0 = Y1;
+      Y1 = Y2;
+    }
+    // the filtered Ritz vectors are nearly orthogonal but of very different
+    // lengths: equalise them before the QR
+    std::vector<double> n2 = col_norm2(c, Y1), inv(p);
+    for (int64_t j = 0; j < p; ++j) inv[j] = n2[j] > 0.0 && std::isfinite(n2[j]) ? 1.0 / std::sqrt(n2[j]) : 0.0;
+    scale_cols(c, Y1, inv);
+    Q = qr_thin(c, Y1).Q;
+  }
+  if (key) c.ritz_cache[*key] = X;
+  lam->assign(theta.begin(), theta.begin() + want);
+  *V = copy_tensor(c, X.slice_last(0, want));
 }
 
-MPS compress_columns(Ctx& c, const std::vector<Column>& cols, int chi, double cutoff, bool cplx) {
+MPS compress_columns(Ctx& c, const std::vector<Column>& cols, int chi, double cutoff, bool cplx,
+                     const std::string* key_prefix, bool relative) {
   const size_t n = cols.size();
   std::vector<Tensor> G(n);  // G[i] = left Gram on the left bond of column i
   std::vector<int64_t> r(n + 1, 1);
@@ -364,9 +418,12 @@ MPS compress_columns(Ctx& c, const std::vector<Column>& cols, int chi, double cu
     const int64_t length = std::min(r[i], dp * k);
     std::vector<double> lam;
     Tensor V;
-    top_eigs(c, rho, std::min<int64_t>(length, std::max<int64_t>(chi, 1)), &lam, &V);
+    std::string key = key_prefix ? *key_prefix + "/" + std::to_string(i) : std::string();
+    top_eigs(c, rho, std::min<int64_t>(length, std::max<int64_t>(chi, 1)), &lam, &V,
+             key_prefix ? &key : nullptr);
     int64_t cnt = 0;
-    for (double x : lam) cnt += std::sqrt(std::max(x, 0.0)) > cutoff;
+    const double thr = relative && !lam.empty() ? cutoff * std::sqrt(std::max(lam[0], 0.0)) : cutoff;
+    for (double x : lam) cnt += std::sqrt(std::max(x, 0.0)) > thr;
     int64_t kn = std::max<int64_t>(1, std::min<int64_t>({int64_t(chi), cnt, length}));
     Tensor Vk = V;
     Vk.dims = {dp * k, kn};  // column prefix
@@ -389,19 +446,37 @@ void compress_mps_gram(Ctx& c, MPS& mps, int chi, double cutoff) {
     cols[i].kind = 0;
     cols[i].M = mps[i];
   }
-  mps = compress_columns(c, cols, chi, cutoff, mps[0].cplx);
+  mps = compress_columns(c, cols, chi, cutoff, mps[0].cplx, nullptr, false);
 }
 
 // absorb_row_from_top/bottom followed by compress_mps!, choosing the literal
 // QR/SVD sweep on the explicit product or the fused Gram form.
-MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int chi) {
+// Divide an MPS by a positive scale: right-canonical ones (every compressed
+// boundary) by ||mps[0]||, which is their norm; a raw row MPS tensor by
+// tensor.  Contractions of boundaries enter the full update and the energy
+// only through ratios, so the scale is irrelevant there.  With it the
+// compression cut is relative to each bond's largest singular value, the
+// scale-free form of the reference's absolute 1e-14 on an O(1) boundary.
+static void normalise_mps(Ctx& c, MPS& m, bool canonical) {
+  for (size_t i = 0; i < (canonical ? std::min<size_t>(1, m.size()) : m.size()); ++i) {
+    const double n = std::sqrt(norm2sq(c, m[i]));
+    if (std::isfinite(n) && n > 0.0) {
+      m[i] = copy_tensor(c, m[i]);
+      scale(c, m[i], 1.0 / n);
+    }
+  }
+}
+
+MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int chi, int key_row,
+                    bool normalise) {
   int64_t maxbond = 1;
   for (size_t i = 0; i < mps.size(); ++i)
     maxbond = std::max(maxbond, mps[i].dims[2] * row[i].dims[1]);
   bool gram = c.compress_mode == 2 || (c.compress_mode == 0 && maxbond > 1024);
   if (!gram || mps.size() <= 1) {
     MPS out = from_top ? absorb_row_from_top(c, mps, row) : absorb_row_from_bottom(c, mps, row);
-    compress_mps(c, out, chi);
+    compress_mps(c, out, chi, CMPS_CUT, normalise);
+    if (normalise) normalise_mps(c, out, true);
     return out;
   }
   std::vector<Column> cols(mps.size());
@@ -412,7 +487,11 @@ MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int c
     cols[i].mps = mps[i];
     cols[i].E = row[i];
   }
-  MPS out = compress_columns(c, cols, chi, CMPS_CUT, mps[0].cplx);
+  // the same boundary (side, row) is recomputed every sweep: key its columns
+  const std::string prefix = std::string(from_top ? "top" : "bottom") + std::to_string(key_row);
+  MPS out = compress_columns(c, cols, chi, CMPS_CUT, mps[0].cplx, key_row > 0 ? &prefix : nullptr,
+                             normalise);
+  if (normalise) normalise_mps(c, out, true);
   if (c.verbose) {
     int64_t mx = 1;
     for (auto& t : out) mx = std::max(mx, t.dims[2]);
@@ -431,7 +510,7 @@ MPS boundary_top(State& s, int stop_row, int chi) {
   // :183-193
   if (stop_row >= s.ny) return trivial_mps(*s.c, s.nx, s.cplx);
   MPS mps = row_as_mps_down(*s.c, row_dl(s, s.ny));
-  for (int r = s.ny - 1; r > stop_row; --r) mps = absorb_compress(*s.c, mps, row_dl(s, r), true, chi);
+  for (int r = s.ny - 1; r > stop_row; --r) mps = absorb_compress(*s.c, mps, row_dl(s, r), true, chi, r);
   return mps;
 }
 
@@ -439,7 +518,7 @@ MPS boundary_bottom(State& s, int stop_row, int chi) {
   // :207-216
   if (stop_row <= 1) return trivial_mps(*s.c, s.nx, s.cplx);
   MPS mps = row_as_mps_up(*s.c, row_dl(s, 1));
-  for (int r = 2; r < stop_row; ++r) mps = absorb_compress(*s.c, mps, row_dl(s, r), false, chi);
+  for (int r = 2; r < stop_row; ++r) mps = absorb_compress(*s.c, mps, row_dl(s, r), false, chi, r);
   return mps;
 }
 
@@ -447,7 +526,7 @@ cplx peps_norm2(State& s, int chi) {
   // :228-245
   Ctx& c = *s.c;
   MPS mps = row_as_mps_down(*s.c, row_dl(s, s.ny));
-  for (int r = s.ny - 1; r >= 1; --r) mps = absorb_compress(c, mps, row_dl(s, r), true, chi);
+  for (int r = s.ny - 1; r >= 1; --r) mps = absorb_compress(c, mps, row_dl(s, r), true, chi, r);
   Tensor acc = ones_tensor(c, 2, s.cplx);
   for (auto& M : mps) {
     PEPS_CHECK(M.dims[1] == 1, PS_ESHAPE, "norm: bottom phys dim != 1");
@@ -647,7 +726,16 @@ CSVD assemble(Ctx& c, int64_t nrow, int64_t ncol, std::vector<SectorSVD>& secs, 
       cands.push_back({i, k, secs[i].S[k]});
     }
   }
-  PEPS_CHECK(!cands.empty(), PS_ENOSECTOR, "charge_svd: no singular values above reg in any sector");
+  if (cands.empty()) {
+    double smax = 0.0;
+    for (auto& sc : secs)
+      for (double x : sc.S) smax = std::max(smax, x);
+    char buf[160];
+    std::snprintf(buf, sizeof buf,
+                  "charge_svd: no singular values above reg in any sector (largest %.3e, reg %.1e, %d sectors)",
+                  smax, reg, int(secs.size()));
+    throw Error(PS_ENOSECTOR, buf);
+  }
   std::vector<size_t> ord(cands.size());
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return cands[x].sv > cands[y].sv; });
@@ -720,6 +808,7 @@ CSVD charge_svd(Ctx& c, const Tensor& M, const std::vector<int64_t>& rq,
 static CSVD charge_svd_lowrank(Ctx& c, const Tensor& Amat, const Tensor& Bmat,
                                const std::vector<int64_t>& rq, const std::vector<int64_t>& cq,
                                int Dmax, double reg) {
+  TierGuard fp64(c, TIER_FP64);
   const int64_t r = Amat.dims[1];
   Tensor BT = permute(c, Bmat, {1, 0});
   std::vector<SectorSVD> secs;
@@ -755,6 +844,7 @@ Tensor apply_gate(Ctx& c, const Tensor& th, const Tensor& G) {
 
 // _solve_herm :37-45 applied to every column of S at once.
 static Tensor solve_herm_multi(Ctx& c, const Tensor& N, const Tensor& S, double reg) {
+  TierGuard fp64(c, TIER_FP64);
   Tensor Nh = copy_tensor(c, N);
   hermitize(c, Nh);
   EIG e = eigh(c, Nh);
@@ -791,7 +881,7 @@ double als_cost(Ctx& c, const Tensor& th, const Tensor& env, const Tensor& X, co
 void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,
                   const std::vector<int64_t>& eR, int n_als, double reg, double tol, Tensor* Xo,
                   Tensor* Yo, std::vector<int64_t>* bflux, ALSInfo* info, const Tensor* X_init,
-                  const Tensor* Y_init) {
+                  const Tensor* Y_init, bool positive_cost) {
   // als_truncate :94-154
   const int64_t nA = th.dims[0], dpL = th.dims[1], dpR = th.dims[2], nB = th.dims[3];
   Tensor thm = permute(c, th, {0, 1, 3, 2}).reshaped({nA * dpL, nB * dpR});
@@ -856,7 +946,7 @@ void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,
     if (e.code != PS_ESOLVER) throw;
     cost1 = std::nan("");
   }
-  bool ok = std::isfinite(cost1) && cost1 < cost0;
+  bool ok = std::isfinite(cost1) && cost1 < cost0 && (!positive_cost || cost1 >= 0.0);
   if (info) {
     info->cost0 = cost0;
     info->cost1 = cost1;
@@ -963,7 +1053,7 @@ static void recanonicalise_v(State& s, int ix, int iy, const Tensor& Dt, const T
 }
 
 void fu_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, double tol,
-               const Env& E, ALSInfo* info) {
+               const Env& E, ALSInfo* info, bool positive_cost) {
   // full_update_bond_h_v4! :165-193 (environment supplied by the caller,
   // built after apply_mask on both sites)
   Ctx& c = *s.c;
@@ -973,7 +1063,8 @@ void fu_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, d
   phys_labels(ix, iy, s.nx, s.ny, s.gd, &nf, &lR, &lU);
   Tensor X, Y;
   std::vector<int64_t> bflux;
-  als_truncate(c, th, E.env, Dmax, lR, n_als, V4_ALS_REG, tol, &X, &Y, &bflux, info);
+  als_truncate(c, th, E.env, Dmax, lR, n_als, V4_ALS_REG, tol, &X, &Y, &bflux, info, nullptr, nullptr,
+               positive_cost);
   Tensor Af = contract(c, E.QA, "luda", X, "apr", "plrud");
   Tensor Bf = contract(c, E.QB, "rudb", Y, "bpl", "plrud");
   recanonicalise_h(s, ix, iy, Af, Bf, Dmax);
@@ -982,7 +1073,7 @@ void fu_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, d
 }
 
 void fu_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, double tol,
-               const Env& E, ALSInfo* info) {
+               const Env& E, ALSInfo* info, bool positive_cost) {
   // full_update_bond_v_v4! :274-299
   Ctx& c = *s.c;
   Tensor th = contract(c, E.RA, "apk", E.RB, "bqk", "apqb");
@@ -991,7 +1082,8 @@ void fu_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, d
   phys_labels(ix, iy, s.nx, s.ny, s.gd, &nf, &lR, &lU);
   Tensor X, Y;
   std::vector<int64_t> bflux;
-  als_truncate(c, th, E.env, Dmax, lU, n_als, V4_ALS_REG, tol, &X, &Y, &bflux, info);
+  als_truncate(c, th, E.env, Dmax, lU, n_als, V4_ALS_REG, tol, &X, &Y, &bflux, info, nullptr, nullptr,
+               positive_cost);
   Tensor Af = contract(c, E.QA, "lrda", X, "apu", "plrud");
   Tensor Bf = contract(c, E.QB, "lrub", Y, "bpd", "plrud");
   recanonicalise_v(s, ix, iy, Af, Bf, Dmax);
@@ -1014,6 +1106,7 @@ void fu_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, d
 static CSVD su_charge_svd(Ctx& c, const Tensor& core, int64_t dpL, int64_t a,
                           const std::vector<int64_t>& eR_of_p, const std::vector<int64_t>& rq_full,
                           const std::vector<int64_t>& cq_full, int Dmax) {
+  TierGuard fp64(c, TIER_FP64);
   const int64_t ncc = core.dims[1];
   std::vector<SectorSVD> secs;
   std::vector<int64_t> rq_core(dpL * a);
@@ -1220,25 +1313,35 @@ void trotter_step_gauge(State& s, const Params& p, GateCache& gc) {
 }
 
 // All top boundaries of the current state: tops[r] = boundary_top(r), r=1..ny.
-static std::vector<MPS> all_tops(State& s, int chi) {
+static std::vector<MPS> all_tops(State& s, int chi, bool normalise = false) {
   std::vector<MPS> tops(s.ny + 1);
   tops[s.ny] = trivial_mps(*s.c, s.nx, s.cplx);
-  if (s.ny >= 2) tops[s.ny - 1] = row_as_mps_down(*s.c, row_dl(s, s.ny));
-  for (int r = s.ny - 2; r >= 1; --r) tops[r] = absorb_compress(*s.c, tops[r + 1], row_dl(s, r + 1), true, chi);
+  if (s.ny >= 2) {
+    tops[s.ny - 1] = row_as_mps_down(*s.c, row_dl(s, s.ny));
+    if (normalise) normalise_mps(*s.c, tops[s.ny - 1], false);
+  }
+  for (int r = s.ny - 2; r >= 1; --r)
+    tops[r] = absorb_compress(*s.c, tops[r + 1], row_dl(s, r + 1), true, chi, r + 1, normalise);
   return tops;
 }
 
 // boundary_bottom(iy) from boundary_bottom(iy-1) and the current row iy-1.
-static MPS next_bottom(State& s, const MPS& prev, int iy, int chi) {
+static MPS next_bottom(State& s, const MPS& prev, int iy, int chi, bool normalise = false) {
   if (iy <= 1) return trivial_mps(*s.c, s.nx, s.cplx);
-  if (iy == 2) return row_as_mps_up(*s.c, row_dl(s, 1));
-  return absorb_compress(*s.c, prev, row_dl(s, iy - 1), false, chi);
+  if (iy == 2) {
+    MPS m = row_as_mps_up(*s.c, row_dl(s, 1));
+    if (normalise) normalise_mps(*s.c, m, false);
+    return m;
+  }
+  return absorb_compress(*s.c, prev, row_dl(s, iy - 1), false, chi, iy - 1, normalise);
 }
 
 static void progress(Ctx& c, char orient, int ix, int iy, const ALSInfo& inf) {
   if (!c.verbose) return;
-  c.note(std::string(1, orient) + "-bond (" + std::to_string(ix) + "," + std::to_string(iy) + "): " +
-         std::to_string(inf.sweeps) + " ALS sweeps, " + (inf.accepted ? "accepted" : "kept SVD init"));
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "%c-bond (%d,%d): %d ALS sweeps, %s (cost %.3e -> %.3e)", orient, ix, iy,
+                inf.sweeps, inf.accepted ? "accepted" : "kept SVD init", inf.cost0, inf.cost1);
+  c.note(buf);
 }
 
 static void note(Ctx& c, const char* what) { c.note(what); }
@@ -1253,7 +1356,8 @@ static void h_row(State& s, int iy, const MPS& top, const MPS& bot, bool l2r, co
     apply_mask(s, ix + 1, iy);
     Env E = bond_env_h(s, ix, iy, top, bot, L, R);
     ALSInfo inf;
-    fu_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, p.n_als, p.als_tol, E, &inf);
+    fu_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, p.n_als, p.als_tol, E, &inf,
+              p.positive_cost);
     if (infos) infos->push_back(inf);
     progress(c, 'h', ix, iy, inf);
   };
@@ -1279,11 +1383,11 @@ static void h_row(State& s, int iy, const MPS& top, const MPS& bot, bool l2r, co
 }
 
 static void h_sweep(State& s, bool l2r, const Params& p, GateCache& gc, std::vector<ALSInfo>* infos) {
-  std::vector<MPS> tops = all_tops(s, p.chi);
+  std::vector<MPS> tops = all_tops(s, p.chi, p.renorm);
   note(*s.c, "top boundaries of the horizontal sweep built");
   MPS bot;
   for (int iy = 1; iy <= s.ny; ++iy) {
-    bot = next_bottom(s, bot, iy, p.chi);
+    bot = next_bottom(s, bot, iy, p.chi, p.renorm);
     h_row(s, iy, tops[iy], bot, l2r, p, gc, infos);
   }
 }
@@ -1291,11 +1395,11 @@ static void h_sweep(State& s, bool l2r, const Params& p, GateCache& gc, std::vec
 static void v_sweep(State& s, const Params& p, GateCache& gc, std::vector<ALSInfo>* infos) {
   Ctx& c = *s.c;
   const int nx = s.nx;
-  std::vector<MPS> tops = all_tops(s, p.chi);
+  std::vector<MPS> tops = all_tops(s, p.chi, p.renorm);
   note(c, "top boundaries of the vertical sweep built");
   MPS bot;
   for (int iy = 1; iy < s.ny; ++iy) {
-    bot = next_bottom(s, bot, iy, p.chi);
+    bot = next_bottom(s, bot, iy, p.chi, p.renorm);
     const MPS& top = tops[iy + 1];
     auto dl0 = [&](int ix) { return dl_traced(c, sqrt_absorbed(s, ix, iy)); };
     auto dl1 = [&](int ix) { return dl_traced(c, sqrt_absorbed(s, ix, iy + 1)); };
@@ -1310,7 +1414,8 @@ static void v_sweep(State& s, const Params& p, GateCache& gc, std::vector<ALSInf
       apply_mask(s, ix, iy + 1);
       Env E = bond_env_v(s, ix, iy, top, bot, L, Renv[ix + 1]);
       ALSInfo inf;
-      fu_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 1.0), p.Dmax, p.n_als, p.als_tol, E, &inf);
+      fu_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 1.0), p.Dmax, p.n_als, p.als_tol, E, &inf,
+                p.positive_cost);
       if (infos) infos->push_back(inf);
       progress(c, 'v', ix, iy, inf);
     }
@@ -1339,9 +1444,10 @@ double energy_v4(State& s, const Params& p) {
   // compute_energy_v4 :390-409 with exactly cached environments
   Ctx& c = *s.c;
   const int nx = s.nx, ny = s.ny;
-  std::vector<MPS> tops = all_tops(s, p.chi);
+  std::vector<MPS> tops = all_tops(s, p.chi, p.renorm);
   std::vector<MPS> bots(ny + 1);
-  for (int iy = 1; iy <= ny; ++iy) bots[iy] = next_bottom(s, iy >= 2 ? bots[iy - 1] : MPS(), iy, p.chi);
+  for (int iy = 1; iy <= ny; ++iy)
+    bots[iy] = next_bottom(s, iy >= 2 ? bots[iy - 1] : MPS(), iy, p.chi, p.renorm);
   std::vector<std::vector<Tensor>> dls(ny + 1);
   for (int iy = 1; iy <= ny; ++iy) dls[iy] = row_dl(s, iy);
   double E = 0.0;

@@ -21,6 +21,8 @@ struct Params {
   double g = 1.0, t = 1.0, m = 0.5, tau = 0.05;
   int Dmax = 2, chi = 16, n_als = 30;
   double als_tol = 0.0;  // > 0: stop sweeping when the relative cost change < tol
+  bool renorm = false;   // normalise every boundary MPS after each row absorption
+  bool positive_cost = false;  // accept an ALS refinement only if its cost is >= 0
 };
 
 // GaugePEPS (finite_peps_gauge_invariant.jl:82-88) held on the device.
@@ -71,11 +73,17 @@ MPS row_as_mps_down(Ctx& c, const MPS& row);
 MPS row_as_mps_up(Ctx& c, const MPS& row);
 MPS absorb_row_from_top(Ctx& c, const MPS& mps, const MPS& row);
 MPS absorb_row_from_bottom(Ctx& c, const MPS& mps, const MPS& row);
-void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT);
+// relative: keep singular values > cutoff * (largest) instead of > cutoff
+void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT, bool relative = false);
 // Same truncated MPS from left Gram matrices and top-chi eigenvectors.
 void compress_mps_gram(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT);
 // absorb_row_from_top/bottom + compress_mps!, literal or fused Gram form.
-MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int chi);
+// key_row > 0 names the row just absorbed: the Gram eigensolves of this
+// boundary warm-start from the previous computation at the same position.
+// normalise: relative compression cut and the result divided by its norm
+// (PEPS_STEP_RENORM).
+MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int chi, int key_row = 0,
+                    bool normalise = false);
 MPS trivial_mps(Ctx& c, int n, bool cplx);
 MPS boundary_top(State& s, int stop_row, int chi);
 MPS boundary_bottom(State& s, int stop_row, int chi);
@@ -106,15 +114,16 @@ double als_cost(Ctx& c, const Tensor& th, const Tensor& env, const Tensor& X, co
 void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,
                   const std::vector<int64_t>& eR, int n_als, double reg, double tol, Tensor* X,
                   Tensor* Y, std::vector<int64_t>* bflux, ALSInfo* info,
-                  const Tensor* X_init = nullptr, const Tensor* Y_init = nullptr);
+                  const Tensor* X_init = nullptr, const Tensor* Y_init = nullptr,
+                  bool positive_cost = false);
 
 // ── bond updates and drivers ────────────────────────────────────────────────
 void su_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax);
 void su_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax);
 void fu_bond_h(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, double tol,
-               const Env& E, ALSInfo* info = nullptr);
+               const Env& E, ALSInfo* info = nullptr, bool positive_cost = false);
 void fu_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, double tol,
-               const Env& E, ALSInfo* info = nullptr);
+               const Env& E, ALSInfo* info = nullptr, bool positive_cost = false);
 
 struct GateCache {
   std::map<std::string, Tensor> g;

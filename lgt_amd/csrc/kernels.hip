@@ -1536,3 +1536,34 @@ void qr_form_q(Ctx& c, const Tensor& W, const Tensor& tau, Tensor& Q) {
 }
 
 }  // namespace peps
+
+namespace peps {
+
+// out[j] = ||M[:, j]||^2 for a column-major matrix M (m x n): one wave per
+// column, an XOR-butterfly wave sum.
+__global__ void col_norm2_kernel(const double* __restrict__ M, int64_t m, int64_t n, int esz,
+                                 double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t col = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (col >= n) return;
+  const double* p = M + col * m * esz;
+  double s = 0.0;
+  for (int64_t i = lane; i < m * esz; i += 64) s += p[i] * p[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) out[col] = s;
+}
+
+std::vector<double> col_norm2(Ctx& c, const Tensor& M) {
+  PEPS_CHECK(M.rank() == 2, PS_ESHAPE, "col_norm2: matrix expected");
+  const int64_t m = M.dims[0], n = M.dims[1];
+  std::vector<double> h(n, 0.0);
+  if (n == 0 || m == 0) return h;
+  Tensor out = c.alloc({n}, false);
+  const unsigned blocks = unsigned(cdiv(n, 4));
+  hipLaunchKernelGGL(col_norm2_kernel, dim3(blocks), dim3(256), 0, c.stream, M.data(), m, n,
+                     M.cplx ? 2 : 1, out.data());
+  PEPS_HIP(hipGetLastError());
+  return c.to_host(out);
+}
+
+}  // namespace peps

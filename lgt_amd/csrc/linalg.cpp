@@ -190,6 +190,7 @@ SVD svd_thin(Ctx& c, const Tensor& A) {
 
 static SVD svd_dense(Ctx& c, const Tensor& A) {
   // A has no zero row or column
+  TierGuard fp64(c, TIER_FP64);
   const int64_t m = A.dims[0], n = A.dims[1];
   SVD r;
   const bool tr = m < n;

@@ -139,6 +139,12 @@ class Ctx {
   // progress lines on stderr, one per bond update (environment variable
   // PEPS_HIP_VERBOSE=1 at context creation)
   int verbose = 0;
+  // Hermitian top-k eigenproblems up to this size are solved directly, larger
+  // ones by subspace iteration (environment variable PEPS_HIP_EIG_DIRECT_MAX)
+  int eig_direct_max = 512;
+  // last Ritz basis of each boundary-MPS column position: the warm start of
+  // the next subspace iteration at the same position
+  std::map<std::string, Tensor> ritz_cache;
   // one progress line on stderr when verbose
   void note(const std::string& what);
 
@@ -177,6 +183,15 @@ class Ctx {
 
   // Device scratch for small host→device uploads (label tables, weights).
   void* scratch_upload(const void* host, size_t bytes, std::shared_ptr<Buffer>* keep);
+};
+
+// Runs the enclosed GEMMs at a fixed precision tier (the factorisations and
+// their internal products stay FP64 whatever tier the contractions use).
+struct TierGuard {
+  Ctx& c;
+  Tier old;
+  TierGuard(Ctx& ctx, Tier t) : c(ctx), old(ctx.tier) { c.tier = t; }
+  ~TierGuard() { c.tier = old; }
 };
 
 // ── kernels (kernels.hip) ────────────────────────────────────────────────────
@@ -229,6 +244,8 @@ void hermitize(Ctx& c, Tensor& N);
 // Copy a sub-block: dst(:, j0:j0+n) = src (column block).
 Tensor copy_tensor(Ctx& c, const Tensor& t);
 bool all_finite(Ctx& c, const Tensor& t);
+// Squared 2-norms of the columns of M (synchronises).
+std::vector<double> col_norm2(Ctx& c, const Tensor& M);
 // Indices of the rows / columns of M that hold a non-zero entry (synchronises).
 void nonzero_lines(Ctx& c, const Tensor& M, std::vector<int64_t>* rows, std::vector<int64_t>* cols);
 // Householder QR of a panel with at most qr_small_max_cols() columns (one

@@ -41,7 +41,7 @@ class PepsError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("g", C.c_double), ("t_hop", C.c_double), ("m", C.c_double),
                 ("tau", C.c_double), ("D_max", C.c_int32), ("chi", C.c_int32),
-                ("n_als", C.c_int32), ("pad", C.c_int32), ("als_tol", C.c_double)]
+                ("n_als", C.c_int32), ("flags", C.c_int32), ("als_tol", C.c_double)]
 
 
 # name: (restype, argtypes)
@@ -305,8 +305,13 @@ class GaugePEPS:
         return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv)
 
 
-def _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol=0.0):
-    return Params(g=g, t_hop=t_hop, m=m, tau=tau, D_max=D_max, chi=chi, n_als=n_als, pad=0, als_tol=als_tol)
+PEPS_STEP_RENORM = 1
+PEPS_STEP_POSITIVE_COST = 2
+
+
+def _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol=0.0, flags=0):
+    return Params(g=g, t_hop=t_hop, m=m, tau=tau, D_max=D_max, chi=chi, n_als=n_als, flags=flags,
+                  als_tol=als_tol)
 
 
 # ── finite_peps_gauge_invariant.jl ───────────────────────────────────────────
@@ -484,18 +489,27 @@ def full_update_bond_v_v4(gp: GaugePEPS, ix, iy, gate, D_max, chi=64, n_als=30):
     gp.ctx.check(lib().peps_fu_bond_v4(gp.h, PEPS_V, ix, iy, _dptr(G), D_max, chi, n_als))
 
 
-def trotter_step_v4(gp: GaugePEPS, g, t_hop, m, tau, D_max, chi=64, n_als=30, als_tol=0.0):
-    """trotter_step_v4! (finite_peps_fullupdate_v4.jl:415).  Returns ALS stats."""
-    p = _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol)
+def trotter_step_v4(gp: GaugePEPS, g, t_hop, m, tau, D_max, chi=64, n_als=30, als_tol=0.0, renorm=False,
+                    positive_cost=False):
+    """trotter_step_v4! (finite_peps_fullupdate_v4.jl:415).  Returns ALS stats.
+    Options without a reference counterpart (both off by default):
+    renorm normalises every boundary MPS after each row absorption
+    (PEPS_STEP_RENORM: keeps the absolute truncation thresholds meaningful
+    when <psi|psi> drifts far from 1); positive_cost accepts an ALS
+    refinement only if its cost is >= 0 (PEPS_STEP_POSITIVE_COST: guards
+    against environments that are not positive semidefinite)."""
+    flags = (PEPS_STEP_RENORM if renorm else 0) | (PEPS_STEP_POSITIVE_COST if positive_cost else 0)
+    p = _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol, flags)
     gp.ctx.check(lib().peps_trotter_step(gp.h, PEPS_STEP_V4, C.byref(p)))
     st = np.zeros(3, np.int64)
     gp.ctx.check(lib().peps_last_als_stats(gp.h, _iptr(st)))
     return dict(bonds=int(st[0]), accepted=int(st[1]), sweeps=int(st[2]))
 
 
-def compute_energy_v4(gp: GaugePEPS, g, t_hop, m, chi=64) -> float:
-    """compute_energy_v4 (finite_peps_fullupdate_v4.jl:390): total energy."""
-    p = _params(g, t_hop, m, 0.0, 1, chi, 0)
+def compute_energy_v4(gp: GaugePEPS, g, t_hop, m, chi=64, renorm=False) -> float:
+    """compute_energy_v4 (finite_peps_fullupdate_v4.jl:390): total energy
+    (renorm: normalised boundaries, as in trotter_step_v4)."""
+    p = _params(g, t_hop, m, 0.0, 1, chi, 0, 0.0, PEPS_STEP_RENORM if renorm else 0)
     E = C.c_double()
     gp.ctx.check(lib().peps_energy_v4(gp.h, C.byref(p), C.byref(E)))
     return E.value

@@ -340,3 +340,51 @@ def test_boundaries_and_energy_gram_form(ctx, request, name, chi):
         ctx.set_compress_mode("auto")
     e_ref = O.compute_energy_v4(go, G, T_HOP, M, chi)
     assert abs(e_gpu - e_ref) < 1e-9 * abs(e_ref)
+
+
+@pytest.mark.parametrize("name,chi", [("s34", 12), ("s34", 4), ("s23c", 64)])
+def test_gram_form_subspace_iteration(request, name, chi):
+    """The Gram form with every top-chi eigenproblem solved by the warm-started
+    subspace iteration (the path large boundaries take): boundaries and
+    energy vs the oracle, twice in a row (the second pass starts from the
+    cached Ritz bases)."""
+    import os
+    from lgt_amd import peps
+    os.environ["PEPS_HIP_EIG_DIRECT_MAX"] = "0"
+    try:
+        ctx2 = peps.Context(0)
+    finally:
+        del os.environ["PEPS_HIP_EIG_DIRECT_MAX"]
+    go = request.getfixturevalue(name)
+    gd = to_device(go, ctx2)
+    ctx2.set_compress_mode("gram")
+    e_ref = O.compute_energy_v4(go, G, T_HOP, M, chi)
+    for _ in range(2):
+        for r in range(1, go.ny + 1):
+            t_gpu, t_ref = peps.boundary_top(gd, r, chi), O.boundary_top(go, r, chi)
+            assert [m.shape for m in t_gpu] == [m.shape for m in t_ref]
+            assert rel(mps_dense(t_gpu), mps_dense(t_ref)) < 1e-9
+        e_gpu = peps.compute_energy_v4(gd, G, T_HOP, M, chi)
+        assert abs(e_gpu - e_ref) < 1e-9 * abs(e_ref)
+    assert ctx2.stats().get("top_eigs[iterations]", {}).get("launches", 0) > 0
+
+
+@pytest.mark.parametrize("name,D,chi,n_als", [("s33", 2, 64, 0), ("s33", 2, 5, 0), ("s34", 3, 10, 0),
+                                              ("s34", 3, 10, 4)])
+def test_trotter_step_v4_renorm_only_rescales_boundaries(ctx, request, name, D, chi, n_als):
+    """PEPS_STEP_RENORM only rescales the boundary MPS: the step and the
+    energy evaluation with it equal the reference's (oracle) on these states
+    (no singular value between the absolute thresholds and their rescaled
+    values), to the bounds of the corresponding tests without it."""
+    from lgt_amd import peps
+    go = request.getfixturevalue(name).copy()
+    O.trotter_step_v4(go, G, T_HOP, M, 0.05, D, chi, n_als)
+    gd = to_device(request.getfixturevalue(name), ctx)
+    peps.trotter_step_v4(gd, G, T_HOP, M, 0.05, D, chi, n_als, renorm=True)
+    e_ref = O.compute_energy_v4(go, G, T_HOP, M, chi)
+    e_gpu = peps.compute_energy_v4(gd, G, T_HOP, M, chi, renorm=True)
+    if n_als == 0:
+        _compare_states(gd, go, chi)
+        assert abs(e_gpu - e_ref) < 1e-10 * abs(e_ref)
+    else:
+        assert abs(e_gpu - e_ref) < 5e-9 * abs(e_ref)
