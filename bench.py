@@ -33,10 +33,21 @@ if ROOT not in sys.path:
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 GEMM_CLASS = {"fp64": "gemm_f64", "fp32": "gemm_f32", "bf16": "gemm_bf16"}
+# HBM bytes per gemm_kernel launch measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE over the timed step of this workload (copy-calibrated factors 2.0
+# and 1.0), keyed by (nx, ny, D, chi, gd, tier); profiles/ holds the summary.
+PROFILED_TRAFFIC = {(8, 8, 6, 144, 2, "fp32"): (2.0 * 8.3310e5 + 1.0 * 1.3208e5) * 1024.0}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _clocks():
+    """(CLOCK_BOOTTIME, CLOCK_MONOTONIC) in ns: logged around the timed region so
+    that a kernel trace of this script (rocprofv3 --kernel-trace) can be cut to
+    it by timestamp."""
+    return time.clock_gettime_ns(time.CLOCK_BOOTTIME), time.clock_gettime_ns(time.CLOCK_MONOTONIC)
 
 
 def make_state(ctx, nx, ny, gd, D, seed, su_steps, g, t, m):
@@ -116,6 +127,71 @@ def cpu_baseline(flops_per_step, seconds=12.0):
                         f"{flops_per_step:.3e} algorithmic flop/step / rate"))
 
 
+def summarize_rocprof(d, log_path=None):
+    """Summarise rocprofv3 output under directory d: per-kernel statistics of
+    the whole run (*kernel_stats.csv), of the timed region of this script cut
+    out of *kernel_trace.csv by the clock window it logs, and per-kernel means
+    of collected counters (*counter_collection.csv, timed window only).  gemm
+    instantiations are grouped by precision (PREC 0 f64, 1 f32, 2 bf16)."""
+    import collections
+    import csv
+    import glob
+    import re
+
+    def gname(name):
+        m = re.search(r"gemm_kernel<(\w+), (\d+),", name)
+        return "gemm_kernel PREC=" + m.group(2) if m else re.sub(r"\(.*", "", name).replace("void ", "")[:80]
+
+    win = None
+    for lf in ([log_path] if log_path else []):
+        for line in open(lf, errors="replace"):
+            m = re.search(r"clock_boottime_ns=(\d+)\.\.(\d+) clock_monotonic_ns=(\d+)\.\.(\d+)", line)
+            if m:
+                win = [(int(m.group(1)), int(m.group(2))), (int(m.group(3)), int(m.group(4)))]
+
+    def window(rows):
+        for t0, t1 in (win or []):
+            sel = [r for r in rows if int(r["Start_Timestamp"]) >= t0 and int(r["End_Timestamp"]) <= t1]
+            if sel:
+                return sel, (t1 - t0) / 1e9
+        return None, None
+
+    for f in sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)):
+        rows = sorted(csv.DictReader(open(f)), key=lambda r: -float(r["TotalDurationNs"]))
+        tot = sum(float(r["TotalDurationNs"]) for r in rows)
+        print(f"whole run ({os.path.basename(f)}): {len(rows)} kernels, {tot / 1e9:.3f} s kernel time")
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        for r in rows:
+            agg[gname(r["Name"])][0] += int(r["Calls"])
+            agg[gname(r["Name"])][1] += float(r["TotalDurationNs"])
+        for name, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:15]:
+            print("  %-80s calls %8d total %10.3f ms avg %9.4f ms %5.1f%%" % (name, n, t / 1e6, t / 1e6 / n, 100 * t / tot))
+    for f in sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)):
+        sel, wall = window(list(csv.DictReader(open(f))))
+        if not sel:
+            continue
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        for r in sel:
+            agg[gname(r["Kernel_Name"])][0] += 1
+            agg[gname(r["Kernel_Name"])][1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        tot = sum(v[1] for v in agg.values())
+        print(f"timed region: {len(sel)} dispatches, wall {wall:.3f} s, kernel time {tot / 1e9:.3f} s")
+        for name, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:30]:
+            print("  %-80s calls %8d total %10.3f ms avg %9.4f ms %5.1f%%" % (name, n, t / 1e6, t / 1e6 / n, 100 * t / tot))
+    for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+        rows = list(csv.DictReader(open(f)))
+        sel, _ = window(rows)
+        rows = sel or rows
+        agg = collections.defaultdict(lambda: [0.0, 0])
+        for r in rows:
+            key = (gname(r["Kernel_Name"]), r["Counter_Name"])
+            agg[key][0] += float(r["Counter_Value"])
+            agg[key][1] += 1
+        print(f"counters ({'timed region' if sel else 'whole run'}, {os.path.basename(f)}):")
+        for (name, cn), (v, n) in sorted(agg.items(), key=lambda kv: -kv[1][0]):
+            print("  %-80s %-10s dispatches %7d mean %.4e KiB total %.4e KiB" % (name, cn, n, v / n, v))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,7 +210,27 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--energy", action="store_true", help="also report E/site after the timed steps")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 5 times (the known-size kernel that "
+                         "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
+    ap.add_argument("--summarize-rocprof", metavar="DIR", default=None,
+                    help="only summarise the rocprofv3 output under DIR (with --bench-log: the log of "
+                         "the profiled run, whose clock window selects the timed region) and exit")
+    ap.add_argument("--bench-log", default=None)
     a = ap.parse_args()
+    if a.summarize_rocprof:
+        summarize_rocprof(a.summarize_rocprof, a.bench_log)
+        return
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_launch": 8 * n, "launches": 5}), flush=True)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU layout "
               "is a design note, not a measured configuration", file=sys.stderr)
@@ -161,6 +257,7 @@ def main():
     ctx.stats_reset()
     ctx.set_timing(True)
     ctx.sync()
+    c0 = _clocks()
     t0 = time.perf_counter()
     als = dict(bonds=0, accepted=0, sweeps=0)
     for s in range(a.steps):
@@ -173,6 +270,8 @@ def main():
         log(f"step {s + 1}/{a.steps}: {time.perf_counter() - ts:.2f} s  als={st}")
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    c1 = _clocks()
+    log(f"timed region clock_boottime_ns={c0[0]}..{c1[0]} clock_monotonic_ns={c0[1]}..{c1[1]}")
     stats = ctx.stats()
     ctx.set_timing(False)
     value = a.steps / elapsed
@@ -204,10 +303,13 @@ def main():
                    "als_accept_rate": als["accepted"] / max(1, als["bonds"]),
                    "flop_per_step": total_flops / a.steps},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[a.tier],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[a.tier], "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[a.tier],
+                     "traffic": PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier)),
                      "kernel": f"gemm_kernel ({GEMM_CLASS[a.tier]}, all launches)",
                      "launches": gk["launches"],
-                     "avg_launch_ms": gk["ms"] / max(1, gk["launches"])},
+                     "avg_launch_ms": gk["ms"] / max(1, gk["launches"]),
+                     "algorithmic_flop_per_launch": gk["flops"] / max(1, gk["launches"]),
+                     "algorithmic_bytes_per_launch": gk["bytes"] / max(1, gk["launches"])},
     }
     if a.energy:
         e = peps.compute_energy_v4(gp, g, t, m, a.chi, renorm=True)

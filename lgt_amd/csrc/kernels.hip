@@ -376,13 +376,17 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
     else go(std::false_type{}, T2{});
   }
   PEPS_HIP(hipGetLastError());
+  c.tend(cls, ev);  // the GEMM kernel alone (the split-K reduce is its own class)
   if (splits > 1) {
     int64_t n = M * N * esz;
     unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 8192));
+    hipEvent_t ev2 = nullptr;
+    c.tbegin("splitk_reduce", &ev2);
+    c.count("splitk_reduce", 0.0, double(n) * 8.0 * (splits + 2));
     hipLaunchKernelGGL(splitk_reduce, dim3(g), dim3(256), 0, c.stream, C, out, n, splits, wstride, beta);
     PEPS_HIP(hipGetLastError());
+    c.tend("splitk_reduce", ev2);
   }
-  c.tend(cls, ev);
 }
 
 // ════════════════════════════════════════════════════════════════════════════
