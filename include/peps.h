@@ -3,8 +3,8 @@
  *
  * Each entry point replaces one function of the Julia reference
  * (NoaFeldman/LGT); the reference file:line is given beside it.  A Julia
- * host reaches them through `ccall`; this repository's
- * tests and bench reach them through Python ctypes.
+ * host reaches them through `ccall`; this repository's tests and bench
+ * reach them through Python ctypes (lgt_amd/peps.py).
  *
  * Conventions
  *  - Every call returns an int32 status (PEPS_OK == 0).  On failure the

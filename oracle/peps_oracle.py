@@ -18,6 +18,16 @@ Generalisation beyond the reference (recorded in the design notes):
     in the reference) and ``b0 = +1`` (E = +1/2) for even ``gd``, so that the
     Gauss law stays satisfiable on 3-link edge sites of the spin-1/2 model.
   * Gauss law in label units: lR + lU - fl - fd == 2 (n_f + q).
+
+Pins (the reference ships no golden vectors and its Julia code cannot run
+here): each function is checked in tests/test_oracle.py against the
+reference's own loop formulas, dense brute force on tiny lattices, library
+routines (numpy/scipy SVD, eigh, pinvh) or invariants.  Parity unpinned:
+``trotter_step_v4`` / ``_recanonicalise`` beyond one bond — their multi-step
+trajectories depend on LAPACK phase and tie choices that only the Julia
+reference could fix; they are pinned only through the per-bond pieces
+(environment, charge SVD, ALS, solve) and the invariants (Gauss sector,
+half filling, environment Hermiticity).
 """
 from __future__ import annotations
 
