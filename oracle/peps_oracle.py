@@ -1127,6 +1127,299 @@ def bond_entropy(lam):
 
 
 # ════════════════════════════════════════════════════════════════════════════
+#  Dense simple update with plaquettes (finite_peps_ground_state.jl) and the
+#  plaquette gate split (u1_lgt_hamiltonian.jl).  A FinitePEPS is held in the
+#  GaugePEPS container; its flux labels are not used (nor kept up to date).
+# ════════════════════════════════════════════════════════════════════════════
+
+SU_PEEL = 1e-15  # finite_peps_ground_state.jl:264-269, :364-369
+
+
+def _bond_w(gp, orient, ix, iy, exists):
+    """Bond weight vector, or ones(1) for a missing bond."""
+    if not exists:
+        return np.ones(1)
+    return np.asarray(gp.LH(ix, iy) if orient == "h" else gp.LV(ix, iy), dtype=np.float64)
+
+
+def _scale5(T, wl, wr, wu, wd):
+    return T * wl[None, :, None, None, None] * wr[None, None, :, None, None] \
+        * wu[None, None, None, :, None] * wd[None, None, None, None, :]
+
+
+def _su_split(Thn, nrow, ncol, D_trunc):
+    """SVD split of the gated two-site tensor, finite_peps_ground_state.jl:241-246."""
+    U, S, Vt = np.linalg.svd(jr(Thn, nrow, ncol), full_matrices=False)
+    Dk = max(1, min(D_trunc, int(np.sum(S > SVD_CUT)), len(S)))
+    St = S[:Dk]
+    return U[:, :Dk], St, Vt[:Dk], St / np.linalg.norm(St), np.sqrt(St)
+
+
+def update_bond_h(gp, ix, iy, G, D_trunc):
+    """update_bond_h!, finite_peps_ground_state.jl:175-273: absorb sqrt(lambda)
+    on every leg, Theta = L.R, gate on (p_R fast, p_L slow), SVD, keep
+    <= D_trunc values above 1e-12, L = U sqrt(S), R = sqrt(S) V^H,
+    lambda = S/||S||, divide the transverse weights out with +1e-15."""
+    nx, ny = gp.nx, gp.ny
+    TL, TR = gp.T(ix, iy), gp.T(ix + 1, iy)
+    dpL, dpR = TL.shape[0], TR.shape[0]
+    lam = np.asarray(gp.LH(ix, iy), dtype=np.float64)
+    bd = len(lam)
+    assert TL.shape[2] == bd and TR.shape[1] == bd
+    sq_b = np.sqrt(lam)
+    sq_l = np.sqrt(_bond_w(gp, "h", ix - 1, iy, ix > 1))
+    sq_r = np.sqrt(_bond_w(gp, "h", ix + 1, iy, ix + 1 < nx))
+    sq_uL = np.sqrt(_bond_w(gp, "v", ix, iy, iy < ny))
+    sq_dL = np.sqrt(_bond_w(gp, "v", ix, iy - 1, iy > 1))
+    sq_uR = np.sqrt(_bond_w(gp, "v", ix + 1, iy, iy < ny))
+    sq_dR = np.sqrt(_bond_w(gp, "v", ix + 1, iy - 1, iy > 1))
+    Lw = _scale5(TL, sq_l, sq_b, sq_uL, sq_dL)
+    Rw = _scale5(TR, sq_b, sq_r, sq_uR, sq_dR)
+    _, Dl, _, DuL, DdL = Lw.shape
+    _, _, Dr, DuR, DdR = Rw.shape
+    Lm = jr(pd(Lw, 1, 2, 4, 5, 3), dpL * Dl * DuL * DdL, bd)
+    Rm = jr(pd(Rw, 2, 1, 3, 4, 5), bd, dpR * Dr * DuR * DdR)
+    Th = jr(Lm @ Rm, dpL, Dl, DuL, DdL, dpR, Dr, DuR, DdR)
+    Thpp = jr(pd(Th, 5, 1, 2, 3, 4, 6, 7, 8), dpR * dpL, Dl * DuL * DdL * Dr * DuR * DdR)
+    Thn = pd(jr(G @ Thpp, dpR, dpL, Dl, DuL, DdL, Dr, DuR, DdR), 2, 3, 4, 5, 1, 6, 7, 8)
+    U, St, Vt, lam_new, sqS = _su_split(Thn, dpL * Dl * DuL * DdL, dpR * Dr * DuR * DdR, D_trunc)
+    Dk = len(St)
+    Lnew = pd(jr(U * sqS[None, :], dpL, Dl, DuL, DdL, Dk), 1, 2, 5, 3, 4)
+    Rnew = pd(jr(sqS[:, None] * Vt, Dk, dpR, Dr, DuR, DdR), 2, 1, 3, 4, 5)
+    inv = lambda x: 1.0 / (x + SU_PEEL)
+    gp.setT(ix, iy, _scale5(Lnew, inv(sq_l), np.ones(Dk), inv(sq_uL), inv(sq_dL)))
+    gp.setT(ix + 1, iy, _scale5(Rnew, np.ones(Dk), inv(sq_r), inv(sq_uR), inv(sq_dR)))
+    gp.lh[ix - 1][iy - 1] = lam_new
+
+
+def update_bond_v(gp, ix, iy, G, D_trunc):
+    """update_bond_v!, finite_peps_ground_state.jl:280-382 (lower D=(ix,iy),
+    upper U=(ix,iy+1); gate on (p_U fast, p_D slow))."""
+    nx, ny = gp.nx, gp.ny
+    TD, TU = gp.T(ix, iy), gp.T(ix, iy + 1)
+    dpD, dpU = TD.shape[0], TU.shape[0]
+    lam = np.asarray(gp.LV(ix, iy), dtype=np.float64)
+    bd = len(lam)
+    assert TD.shape[3] == bd and TU.shape[4] == bd
+    sq_b = np.sqrt(lam)
+    sq_lD = np.sqrt(_bond_w(gp, "h", ix - 1, iy, ix > 1))
+    sq_rD = np.sqrt(_bond_w(gp, "h", ix, iy, ix < nx))
+    sq_dD = np.sqrt(_bond_w(gp, "v", ix, iy - 1, iy > 1))
+    sq_lU = np.sqrt(_bond_w(gp, "h", ix - 1, iy + 1, ix > 1))
+    sq_rU = np.sqrt(_bond_w(gp, "h", ix, iy + 1, ix < nx))
+    sq_uU = np.sqrt(_bond_w(gp, "v", ix, iy + 1, iy + 1 < ny))
+    Dw = _scale5(TD, sq_lD, sq_rD, sq_b, sq_dD)
+    Uw = _scale5(TU, sq_lU, sq_rU, sq_uU, sq_b)
+    _, DlD, DrD, _, DdD = Dw.shape
+    _, DlU, DrU, DuU, _ = Uw.shape
+    Dm = jr(pd(Dw, 1, 2, 3, 5, 4), dpD * DlD * DrD * DdD, bd)
+    Um = jr(pd(Uw, 5, 1, 2, 3, 4), bd, dpU * DlU * DrU * DuU)
+    Th = jr(Dm @ Um, dpD, DlD, DrD, DdD, dpU, DlU, DrU, DuU)
+    Thpp = jr(pd(Th, 5, 1, 2, 3, 4, 6, 7, 8), dpU * dpD, DlD * DrD * DdD * DlU * DrU * DuU)
+    Thn = pd(jr(G @ Thpp, dpU, dpD, DlD, DrD, DdD, DlU, DrU, DuU), 2, 3, 4, 5, 1, 6, 7, 8)
+    U, St, Vt, lam_new, sqS = _su_split(Thn, dpD * DlD * DrD * DdD, dpU * DlU * DrU * DuU, D_trunc)
+    Dk = len(St)
+    Dnew = pd(jr(U * sqS[None, :], dpD, DlD, DrD, DdD, Dk), 1, 2, 3, 5, 4)
+    Unew = pd(jr(sqS[:, None] * Vt, Dk, dpU, DlU, DrU, DuU), 2, 3, 4, 5, 1)
+    inv = lambda x: 1.0 / (x + SU_PEEL)
+    gp.setT(ix, iy, _scale5(Dnew, inv(sq_lD), inv(sq_rD), np.ones(Dk), inv(sq_dD)))
+    gp.setT(ix, iy + 1, _scale5(Unew, inv(sq_lU), inv(sq_rU), inv(sq_uU), np.ones(Dk)))
+    gp.lv[ix - 1][iy - 1] = lam_new
+
+
+def H_plaquette_gauge(gd, g):
+    """u1_lgt_hamiltonian.jl:283-289: -(1/2g^2)(W + W^H), W = Ud (x) U (x) Ud (x) U
+    (Julia kron: the last factor is the fastest index)."""
+    Ug = op_U_gauge(gd)
+    Ud = Ug.conj().T
+    W = np.kron(np.kron(np.kron(Ud, Ug), Ud), Ug)
+    H = -(1.0 / (2.0 * g ** 2)) * (W + W.conj().T)
+    return 0.5 * (H + H.conj().T)
+
+
+def decompose_plaquette_LShape(G, gd, D_aux_max=64, cutoff=1e-12):
+    """u1_lgt_hamiltonian.jl:728-768: two SVDs split the gd^4 x gd^4 gate into
+    GA (gd,gd,gd,gd,Da,Db), GB (gd,gd,Da), GC (gd,gd,Db)."""
+    G8 = jr(G, *([gd] * 8))
+    M1 = jr(pd(G8, 1, 2, 4, 5, 6, 8, 3, 7), gd ** 6, gd ** 2)
+    U1, S1, V1 = np.linalg.svd(M1, full_matrices=False)
+    Da = max(1, min(D_aux_max, int(np.sum(S1 > cutoff)), len(S1)))
+    sq1 = np.sqrt(S1[:Da])
+    GAC = U1[:, :Da] * sq1[None, :]
+    GB = pd(jr(sq1[:, None] * V1[:Da], Da, gd, gd), 2, 3, 1)
+    GAC5 = jr(GAC, gd, gd, gd, gd, gd, gd, Da)
+    M2 = jr(pd(GAC5, 1, 2, 4, 5, 7, 3, 6), gd ** 4 * Da, gd ** 2)
+    U2, S2, V2 = np.linalg.svd(M2, full_matrices=False)
+    Db = max(1, min(D_aux_max, int(np.sum(S2 > cutoff)), len(S2)))
+    sq2 = np.sqrt(S2[:Db])
+    GA = jr(U2[:, :Db] * sq2[None, :], gd, gd, gd, gd, Da, Db)
+    GC = pd(jr(sq2[:, None] * V2[:Db], Db, gd, gd), 2, 3, 1)
+    return GA, GB, GC
+
+
+def reconstruct_plaquette_LShape(GA, GB, GC):
+    """u1_lgt_hamiltonian.jl:776-781."""
+    gd = GA.shape[0]
+    Gr = np.einsum("blBLxy,rRx,tTy->blrtBLRT", GA, GB, GC, optimize=True)
+    return jr(Gr, gd ** 4, gd ** 4)
+
+
+def update_plaquette(gp, ix, iy, GA, GB, GC, D_trunc, recompress=True):
+    """update_plaquette!, finite_peps_ground_state.jl:405-483: GA on A's
+    (R, U) links, GB on B's U link, GC on C's R link; the A-B and A-C bonds
+    grow by Da, Db (lambda repeated) and are recompressed by the dense
+    update with identity gates (recompress=False stops before that, for the
+    checks of the enlargement).  GA/GB/GC are contracted through their first
+    (row) indices, so the operator applied to the links is G^T, as in the
+    reference."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    assert ix < nx and iy < ny
+    Da, Db = GB.shape[2], GC.shape[2]
+    TA, TB, TC = gp.T(ix, iy), gp.T(ix + 1, iy), gp.T(ix, iy + 1)
+    dpA, gRA, gUA = site_dims(ix, iy, nx, ny, gd)
+    dpB, gRB, gUB = site_dims(ix + 1, iy, nx, ny, gd)
+    dpC, gRC, gUC = site_dims(ix, iy + 1, nx, ny, gd)
+    assert gRA == gd and gUA == gd and gUB == gd and gRC == gd
+    _, DlA, DrA, DuA, DdA = TA.shape
+    _, DlB, DrB, DuB, DdB = TB.shape
+    _, DlC, DrC, DuC, DdC = TC.shape
+    TA3 = pd(jr(TA, gd, gd, 2, DlA, DrA, DuA, DdA), 3, 2, 1, 4, 5, 6, 7)
+    TAg = np.einsum("RUSVab,nRUlrud->nSVablrud", GA, TA3, optimize=True)
+    TAp = pd(pd(TAg, 1, 2, 3, 6, 7, 4, 8, 5, 9), 3, 2, 1, 4, 5, 6, 7, 8, 9)
+    TB3 = pd(jr(TB, gUB, gRB, 2, DlB, DrB, DuB, DdB), 3, 2, 1, 4, 5, 6, 7)
+    TBg = np.einsum("UVa,nRUlrud->nRValrud", GB, TB3, optimize=True)
+    TBp = pd(pd(TBg, 1, 2, 3, 5, 4, 6, 7, 8), 3, 2, 1, 4, 5, 6, 7, 8)
+    TC3 = pd(jr(TC, gUC, gRC, 2, DlC, DrC, DuC, DdC), 3, 2, 1, 4, 5, 6, 7)
+    TCg = np.einsum("RSb,nRUlrud->nSUblrud", GC, TC3, optimize=True)
+    TCp = pd(pd(TCg, 1, 2, 3, 5, 6, 7, 8, 4), 3, 2, 1, 4, 5, 6, 7, 8)
+    gp.setT(ix, iy, jr(TAp, dpA, DlA, DrA * Da, DuA * Db, DdA))
+    gp.setT(ix + 1, iy, jr(TBp, dpB, DlB * Da, DrB, DuB, DdB))
+    gp.setT(ix, iy + 1, jr(TCp, dpC, DlC, DrC, DuC, DdC * Db))
+    gp.lh[ix - 1][iy - 1] = np.tile(np.asarray(gp.LH(ix, iy)), Da)
+    gp.lv[ix - 1][iy - 1] = np.tile(np.asarray(gp.LV(ix, iy)), Db)
+    if not recompress:
+        return
+    update_bond_h(gp, ix, iy, np.eye(dpA * dpB, dtype=np.complex128), D_trunc)
+    update_bond_v(gp, ix, iy, np.eye(dpA * dpC, dtype=np.complex128), D_trunc)
+
+
+def trotter_step_dense(gp, g, t_hop, m, tau, D_trunc):
+    """trotter_step!, finite_peps_ground_state.jl:498-541:
+    h-half (L->R) -> v-half (B->T) -> plaquettes -> v-half (T->B) -> h-half (R->L)."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    GA, GB, GC = decompose_plaquette_LShape(gate(H_plaquette_gauge(gd, g), tau), gd, cutoff=SVD_CUT)
+    gh = lambda ix, iy: gate(H_merged_h_site(ix, iy, nx, ny, gd, g, t_hop, m), tau / 2)
+    gv = lambda ix, iy: gate(H_merged_v_site(ix, iy, nx, ny, gd, g, t_hop, m), tau / 2)
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx):
+            update_bond_h(gp, ix, iy, gh(ix, iy), D_trunc)
+    for iy in range(1, ny):
+        for ix in range(1, nx + 1):
+            update_bond_v(gp, ix, iy, gv(ix, iy), D_trunc)
+    for iy in range(1, ny):
+        for ix in range(1, nx):
+            update_plaquette(gp, ix, iy, GA, GB, GC, D_trunc)
+    for iy in range(ny - 1, 0, -1):
+        for ix in range(nx, 0, -1):
+            update_bond_v(gp, ix, iy, gv(ix, iy), D_trunc)
+    for iy in range(ny, 0, -1):
+        for ix in range(nx - 1, 0, -1):
+            update_bond_h(gp, ix, iy, gh(ix, iy), D_trunc)
+
+
+def expect_site(gp, ix, iy, O):
+    """expect_site, finite_peps_ground_state.jl:549-567: lambda (not its root) on
+    every leg, <O> = tr(T^H O T) / tr(T^H T), 0 when the norm is <= 1e-15."""
+    nx, ny = gp.nx, gp.ny
+    Tw = _scale5(gp.T(ix, iy), _bond_w(gp, "h", ix - 1, iy, ix > 1), _bond_w(gp, "h", ix, iy, ix < nx),
+                 _bond_w(gp, "v", ix, iy, iy < ny), _bond_w(gp, "v", ix, iy - 1, iy > 1))
+    Tf = jr(Tw, Tw.shape[0], Tw.size // Tw.shape[0])
+    num = np.trace(Tf.conj().T @ O @ Tf).real
+    den = np.trace(Tf.conj().T @ Tf).real
+    return num / den if den > 1e-15 else 0.0
+
+
+def plaquette_energy(gp, ix, iy, g):
+    """plaquette_energy, finite_peps_ground_state.jl:589-684: -(1/g^2) Re<W> on
+    the L-shape A=(ix,iy), B=(ix+1,iy), C=(ix,iy+1) with sqrt(lambda) on
+    every leg and the external legs closed diagonally."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    assert ix < nx and iy < ny
+    _, gRB, _ = site_dims(ix + 1, iy, nx, ny, gd)
+    _, _, gUC = site_dims(ix, iy + 1, nx, ny, gd)
+    sw = lambda o, jx, jy, e: np.sqrt(_bond_w(gp, o, jx, jy, e))
+    sq_AB, sq_AC = np.sqrt(np.asarray(gp.LH(ix, iy))), np.sqrt(np.asarray(gp.LV(ix, iy)))
+    TA = _scale5(gp.T(ix, iy), sw("h", ix - 1, iy, ix > 1), sq_AB, sq_AC, sw("v", ix, iy - 1, iy > 1))
+    TB = _scale5(gp.T(ix + 1, iy), sq_AB, sw("h", ix + 1, iy, ix + 1 < nx), sw("v", ix + 1, iy, iy < ny),
+                 sw("v", ix + 1, iy - 1, iy > 1))
+    TC = _scale5(gp.T(ix, iy + 1), sw("h", ix - 1, iy + 1, ix > 1), sw("h", ix, iy + 1, ix < nx),
+                 sw("v", ix, iy + 1, iy + 1 < ny), sq_AC)
+    Ug = op_U_gauge(gd)
+    Ud = Ug.conj().T
+    WA = np.kron(np.kron(_I(2), Ud), Ug)
+    WB = np.kron(_I(2), Ud) if gRB == 1 else np.kron(np.kron(_I(2), _I(gRB)), Ud)
+    WC = np.kron(_I(2), Ug) if gUC == 1 else np.kron(np.kron(_I(2), Ug), _I(gUC))
+    app = lambda T, O: jr(O @ jr(T, T.shape[0], T.size // T.shape[0]), *T.shape)
+    MA = lambda k, b: np.einsum("plrud,plsvd->rsuv", k, b.conj(), optimize=True)
+    MB = lambda k, b: np.einsum("plrud,pmrud->lm", k, b.conj(), optimize=True)
+    MC = lambda k, b: np.einsum("plrud,plrue->de", k, b.conj(), optimize=True)
+    sc = lambda a, b, c: np.einsum("rsuv,rs,uv->", a, b, c, optimize=True)
+    num = sc(MA(app(TA, WA), TA), MB(app(TB, WB), TB), MC(app(TC, WC), TC))
+    den = sc(MA(TA, TA), MB(TB, TB), MC(TC, TC))
+    return -(1.0 / g ** 2) * num.real / den.real if abs(den) > 1e-15 else 0.0
+
+
+def _su_hop_expect(Lw, Rw, Lperm, Rperm, H):
+    """The hopping term of total_energy (finite_peps_ground_state.jl:727-735,
+    :778-788): Theta[p1,p2,a,b] = sum_chi L[p1,chi,a] conj(R[p2,chi,b]), flattened
+    with p1 fastest, rho = Theta Theta^H, <H> = Re tr(H rho) / Re tr(rho)."""
+    dpL, dpR = Lw.shape[0], Rw.shape[0]
+    Lb = pd(Lw, *Lperm)
+    Rb = pd(Rw, *Rperm)
+    bd = Lb.shape[1]
+    Lb = jr(Lb, dpL, bd, Lb.size // (dpL * bd))
+    Rb = jr(Rb, dpR, bd, Rb.size // (dpR * bd))
+    Th = np.einsum("pca,qcb->pqab", Lb, Rb.conj(), optimize=True)
+    Tf = jr(Th, dpL * dpR, Th.size // (dpL * dpR))
+    rho = Tf @ Tf.conj().T
+    den = np.trace(rho).real
+    return np.trace(H @ rho).real / den if den > 1e-15 else 0.0
+
+
+def total_energy(gp, g, t_hop, m):
+    """total_energy, finite_peps_ground_state.jl:690-805: on-site terms by
+    expect_site, hopping terms by _su_hop_expect (the rho of each bond is
+    flattened with the left/lower site fastest while H_hop has the right/upper
+    site fastest; kept as written), plaquettes by plaquette_energy."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    E = 0.0
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            E += expect_site(gp, ix, iy, H_onsite_site(ix, iy, nx, ny, gd, g, m))
+    sw = lambda o, jx, jy, e: np.sqrt(_bond_w(gp, o, jx, jy, e))
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx):
+            sb = np.sqrt(np.asarray(gp.LH(ix, iy)))
+            Lw = _scale5(gp.T(ix, iy), sw("h", ix - 1, iy, ix > 1), sb, sw("v", ix, iy, iy < ny),
+                         sw("v", ix, iy - 1, iy > 1))
+            Rw = _scale5(gp.T(ix + 1, iy), sb, sw("h", ix + 1, iy, ix + 1 < nx), sw("v", ix + 1, iy, iy < ny),
+                         sw("v", ix + 1, iy - 1, iy > 1))
+            E += _su_hop_expect(Lw, Rw, (1, 3, 2, 4, 5), (1, 2, 3, 4, 5), H_hop_h_site(ix, iy, nx, ny, gd, t_hop))
+    for iy in range(1, ny):
+        for ix in range(1, nx + 1):
+            sb = np.sqrt(np.asarray(gp.LV(ix, iy)))
+            Dw = _scale5(gp.T(ix, iy), sw("h", ix - 1, iy, ix > 1), sw("h", ix, iy, ix < nx), sb,
+                         sw("v", ix, iy - 1, iy > 1))
+            Uw = _scale5(gp.T(ix, iy + 1), sw("h", ix - 1, iy + 1, ix > 1), sw("h", ix, iy + 1, ix < nx),
+                         sw("v", ix, iy + 1, iy + 1 < ny), sb)
+            E += _su_hop_expect(Dw, Uw, (1, 4, 2, 3, 5), (1, 5, 2, 3, 4), H_hop_v_site(ix, iy, nx, ny, gd, t_hop))
+    for iy in range(1, ny):
+        for ix in range(1, nx):
+            E += plaquette_energy(gp, ix, iy, g)
+    return E
+
+
+# ════════════════════════════════════════════════════════════════════════════
 #  Brute-force references (tiny lattices only): dense |psi>, <psi|psi>, <H>
 # ════════════════════════════════════════════════════════════════════════════
 
