@@ -4,7 +4,10 @@ Workload: the full-update configuration — 8x8 open lattice, spin-1/2 links
 (gd=2, site dim d=8 in the bulk), PEPS bond D=6, boundary-MPS bond chi=144,
 t=1, m=0.5, g^2=1, tau=0.02, ALS at most 20 sweeps with tolerance 1e-10.
 The initial state is the seeded random gauge-invariant PEPS (seed 0) brought
-to D=6 by simple-update steps (tau=0.05); set-up is not timed.
+to D=6 by 3 simple-update steps (tau=0.05); set-up is not timed.  After 3
+steps the bulk bonds are at D=6; after 10 most bonds have dropped to 1-3
+(singular values below the 1e-12 charge-SVD cut), which is a different,
+much smaller workload.  The JSON line reports the bond dimensions reached.
 
 One step = one trotter_step_v4 (h half-sweep, v sweep, h half-sweep: 168
 bond updates with their boundary-MPS environments and ALS solves), timed
@@ -203,7 +206,7 @@ def main():
     ap.add_argument("--D", type=int, default=6)
     ap.add_argument("--chi", type=int, default=144)
     ap.add_argument("--gd", type=int, default=2)
-    ap.add_argument("--su-steps", type=int, default=10)
+    ap.add_argument("--su-steps", type=int, default=3)
     ap.add_argument("--n-als", type=int, default=20)
     ap.add_argument("--als-tol", type=float, default=1e-10)
     ap.add_argument("--tau", type=float, default=0.02)
@@ -247,6 +250,11 @@ def main():
     n_before, n_after = normalise(gp, a.nx, a.ny, a.chi)
     log(f"setup done in {time.perf_counter() - t0:.1f} s (<psi|psi> {n_before:.3e} -> {n_after:.6f} "
         f"after site-uniform rescaling; bond dims h {[len(gp.bond(peps.PEPS_H, 1, iy)[0]) for iy in range(1, a.ny + 1)]})")
+    bd = [len(gp.bond(peps.PEPS_H, ix, iy)[0]) for iy in range(1, a.ny + 1) for ix in range(1, a.nx)] + \
+         [len(gp.bond(peps.PEPS_V, ix, iy)[0]) for iy in range(1, a.ny) for ix in range(1, a.nx + 1)]
+    bond_dims = {"mean": float(np.mean(bd)), "max": int(max(bd)), "at_D": int(sum(1 for x in bd if x == a.D)),
+                 "bonds": len(bd)}
+    log(f"bond dimensions: {bond_dims}")
     ctx.set_tier(a.tier)
     for w in range(a.warmup):
         tw = time.perf_counter()
@@ -301,7 +309,8 @@ def main():
                    "bond_updates_per_step": 2 * (a.nx - 1) * a.ny + a.nx * (a.ny - 1),
                    "als_sweeps_per_step": als["sweeps"] / max(1, a.steps),
                    "als_accept_rate": als["accepted"] / max(1, als["bonds"]),
-                   "flop_per_step": total_flops / a.steps},
+                   "flop_per_step": total_flops / a.steps,
+                   "bond_dims_at_start": bond_dims},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[a.tier],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[a.tier],
                      "traffic": PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier)),
