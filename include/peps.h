@@ -49,7 +49,7 @@ enum {
 
 enum { PEPS_TIER_FP64 = 0, PEPS_TIER_FP32 = 1, PEPS_TIER_BF16 = 2 };
 enum { PEPS_H = 0, PEPS_V = 1 };
-enum { PEPS_STEP_V4 = 0, PEPS_STEP_GAUGE = 1 };
+enum { PEPS_STEP_V4 = 0, PEPS_STEP_GAUGE = 1, PEPS_STEP_DENSE = 2 };
 /* peps_params.flags bits.  PEPS_STEP_RENORM has no counterpart in the
    reference: every boundary MPS built by a full-update step or an energy
    evaluation is truncated at 1e-14 times its largest singular value (the
@@ -175,13 +175,44 @@ int32_t peps_su_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy, con
 /* full_update_bond_h_v4! / _v_v4! (finite_peps_fullupdate_v4.jl:165,274) */
 int32_t peps_fu_bond_v4(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
                         const double* gate, int32_t D_max, int32_t chi, int32_t n_als);
-/* trotter_step_v4! (finite_peps_fullupdate_v4.jl:415) or trotter_step_gauge!
-   (finite_peps_gauge_invariant.jl:523) */
+/* trotter_step_v4! (finite_peps_fullupdate_v4.jl:415), trotter_step_gauge!
+   (finite_peps_gauge_invariant.jl:523) or, for PEPS_STEP_DENSE, the dense
+   trotter_step! with plaquettes (finite_peps_ground_state.jl:498-541; uses
+   g, t_hop, m, tau and D_max as D_trunc) */
 int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p);
 /* ALS statistics of the last PEPS_STEP_V4 step: bonds, accepted, total sweeps */
 int32_t peps_last_als_stats(peps_state* st, int64_t out[3]);
 /* compute_energy_v4 (finite_peps_fullupdate_v4.jl:390): total energy */
 int32_t peps_energy_v4(peps_state* st, const peps_params* p, double* E);
+
+/* ── dense simple update with plaquettes (finite_peps_ground_state.jl) ──── */
+/* These treat the state as a FinitePEPS: no Gauss mask, no flux labels.
+   After any dense update the bond labels are undefined (read back as 0)
+   and the gauge-invariant entries (peps_apply_mask, peps_forbidden_norm,
+   peps_su_bond, peps_fu_bond_v4, peps_trotter_step V4/GAUGE) return
+   PEPS_EARG for that state. */
+/* update_bond_h! / update_bond_v! (finite_peps_ground_state.jl:175-382):
+   gate (dL*dR)^2 with the right/upper site fastest, state element type */
+int32_t peps_su_bond_dense(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
+                           int32_t D_trunc);
+/* decompose_plaquette_LShape(exp(-tau H_plaquette_gauge(gd; g)), gd; cutoff)
+   (u1_lgt_hamiltonian.jl:283-289,728-768; auxiliary bonds capped at 64).
+   Pass GA == NULL to query Da, Db only.  Outputs real column-major
+   GA (gd,gd,gd,gd,Da,Db), GB (gd,gd,Da), GC (gd,gd,Db). */
+int32_t peps_plaquette_gate(peps_ctx* ctx, int32_t gd, double g, double tau, double cutoff,
+                            int32_t* Da, int32_t* Db, double* GA, double* GB, double* GC);
+/* update_plaquette! (finite_peps_ground_state.jl:405-483) on the L-shape
+   A=(ix,iy), B=(ix+1,iy), C=(ix,iy+1); GA/GB/GC shaped as above, in the
+   state's element type */
+int32_t peps_update_plaquette(peps_state* st, int32_t ix, int32_t iy, int32_t Da, int32_t Db,
+                              const double* GA, const double* GB, const double* GC, int32_t D_trunc);
+/* expect_site (finite_peps_ground_state.jl:549-567): O (dp x dp) in the
+   state's element type */
+int32_t peps_expect_site(peps_state* st, int32_t ix, int32_t iy, const double* O, double* out);
+/* plaquette_energy (finite_peps_ground_state.jl:589-684) */
+int32_t peps_plaquette_energy(peps_state* st, int32_t ix, int32_t iy, double g, double* out);
+/* total_energy (finite_peps_ground_state.jl:690-805): uses g, t_hop, m */
+int32_t peps_total_energy(peps_state* st, const peps_params* p, double* out);
 
 /* The contraction GEMM itself (no counterpart in the reference; exposed so
    callers can validate the MFMA path of the current tier):

@@ -70,6 +70,11 @@ std::vector<int64_t>& bond_q(peps::State& s, int orient, int ix, int iy) {
   return orient == PEPS_H ? s.QH(ix, iy) : s.QV(ix, iy);
 }
 
+void require_labels(const peps::State& s) {
+  PEPS_CHECK(s.labelled, peps::PS_EARG,
+             "the state's flux labels are undefined after a dense (label-free) update");
+}
+
 peps::Params to_params(const peps_params* p) {
   PEPS_CHECK(p != nullptr, peps::PS_EARG, "null params");
   PEPS_CHECK(p->D_max >= 1 && p->chi >= 1 && p->n_als >= 0, peps::PS_EARG, "invalid params");
@@ -260,13 +265,17 @@ int32_t peps_apply_mask(peps_state* st, int32_t ix, int32_t iy) {
   if (!st) return PEPS_EARG;
   return guard(st->ref.get(), [&] {
     check_site(st->s, ix, iy);
+    require_labels(st->s);
     peps::apply_mask(st->s, ix, iy);
   });
 }
 
 int32_t peps_forbidden_norm(peps_state* st, double* out) {
   if (!st || !out) return PEPS_EARG;
-  return guard(st->ref.get(), [&] { *out = peps::forbidden_norm(st->s); });
+  return guard(st->ref.get(), [&] {
+    require_labels(st->s);
+    *out = peps::forbidden_norm(st->s);
+  });
 }
 
 int32_t peps_mean_nf(peps_state* st, double* out) {
@@ -427,6 +436,7 @@ int32_t peps_su_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy, con
   if (!st || D_max < 1) return PEPS_EARG;
   return guard(st->ref.get(), [&] {
     bond_lam(st->s, orient, ix, iy);
+    require_labels(st->s);
     Tensor G = gate_from_host(st->s, orient, ix, iy, gate);
     if (orient == PEPS_H) peps::su_bond_h(st->s, ix, iy, G, D_max);
     else peps::su_bond_v(st->s, ix, iy, G, D_max);
@@ -438,6 +448,7 @@ int32_t peps_fu_bond_v4(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
   if (!st || D_max < 1 || chi < 1 || n_als < 0) return PEPS_EARG;
   return guard(st->ref.get(), [&] {
     bond_lam(st->s, orient, ix, iy);
+    require_labels(st->s);
     Tensor G = gate_from_host(st->s, orient, ix, iy, gate);
     if (orient == PEPS_H) {
       peps::apply_mask(st->s, ix, iy);
@@ -459,6 +470,7 @@ int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p) {
     peps::Params q = to_params(p);
     for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
     if (kind == PEPS_STEP_V4) {
+      require_labels(st->s);
       std::vector<peps::ALSInfo> infos;
       peps::trotter_step_v4(st->s, q, st->gc, &infos);
       st->als_bonds = int64_t(infos.size());
@@ -469,7 +481,10 @@ int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p) {
         st->als_sweeps += i.sweeps;
       }
     } else if (kind == PEPS_STEP_GAUGE) {
+      require_labels(st->s);
       peps::trotter_step_gauge(st->s, q, st->gc);
+    } else if (kind == PEPS_STEP_DENSE) {
+      peps::trotter_step_dense(st->s, q, st->gc);
     } else {
       throw peps::Error(peps::PS_EARG, "unknown step kind");
     }
@@ -487,6 +502,77 @@ int32_t peps_last_als_stats(peps_state* st, int64_t out[3]) {
 int32_t peps_energy_v4(peps_state* st, const peps_params* p, double* E) {
   if (!st || !p || !E) return PEPS_EARG;
   return guard(st->ref.get(), [&] { *E = peps::energy_v4(st->s, to_params(p)); });
+}
+
+// ── dense simple update with plaquettes ─────────────────────────────────────
+
+int32_t peps_su_bond_dense(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
+                           int32_t D_trunc) {
+  if (!st || D_trunc < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    bond_lam(st->s, orient, ix, iy);
+    Tensor G = gate_from_host(st->s, orient, ix, iy, gate);
+    if (orient == PEPS_H) peps::dense_bond_h(st->s, ix, iy, G, D_trunc);
+    else peps::dense_bond_v(st->s, ix, iy, G, D_trunc);
+  });
+}
+
+int32_t peps_plaquette_gate(peps_ctx* ctx, int32_t gd, double g, double tau, double cutoff,
+                            int32_t* Da, int32_t* Db, double* GA, double* GB, double* GC) {
+  if (!ctx || !Da || !Db || gd < 2 || gd > 8 || !(g != 0.0)) return PEPS_EARG;
+  if (GA && (!GB || !GC)) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    peps::PlaquetteGate pg = peps::plaquette_gate(*ctx->c, gd, g, tau, cutoff, 64, false);
+    *Da = pg.Da;
+    *Db = pg.Db;
+    if (GA) {
+      ctx->c->download(pg.GA, GA);
+      ctx->c->download(pg.GB, GB);
+      ctx->c->download(pg.GC, GC);
+    }
+  });
+}
+
+int32_t peps_update_plaquette(peps_state* st, int32_t ix, int32_t iy, int32_t Da, int32_t Db,
+                              const double* GA, const double* GB, const double* GC, int32_t D_trunc) {
+  if (!st || !GA || !GB || !GC || Da < 1 || Db < 1 || D_trunc < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    peps::State& s = st->s;
+    const int64_t gd = s.gd;
+    for (auto& t : s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
+    peps::PlaquetteGate pg;
+    pg.Da = Da;
+    pg.Db = Db;
+    pg.GA = s.c->upload({gd, gd, gd, gd, Da, Db}, s.cplx, GA);
+    pg.GB = s.c->upload({gd, gd, Da}, s.cplx, GB);
+    pg.GC = s.c->upload({gd, gd, Db}, s.cplx, GC);
+    peps::update_plaquette(s, ix, iy, pg, D_trunc);
+  });
+}
+
+int32_t peps_expect_site(peps_state* st, int32_t ix, int32_t iy, const double* O, double* out) {
+  if (!st || !O || !out) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    peps::State& s = st->s;
+    check_site(s, ix, iy);
+    const int64_t dp = s.t(ix, iy).dims.at(0);
+    std::vector<double> o(O, O + dp * dp * (s.cplx ? 2 : 1));
+    *out = peps::expect_site(s, ix, iy, o, s.cplx);
+  });
+}
+
+int32_t peps_plaquette_energy(peps_state* st, int32_t ix, int32_t iy, double g, double* out) {
+  if (!st || !out || !(g != 0.0)) return PEPS_EARG;
+  return guard(st->ref.get(), [&] { *out = peps::plaquette_energy(st->s, ix, iy, g); });
+}
+
+int32_t peps_total_energy(peps_state* st, const peps_params* p, double* out) {
+  if (!st || !p || !out) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
+    PEPS_CHECK(p->g != 0.0, peps::PS_EARG, "total_energy: g must be non-zero");
+    *out = peps::total_energy(st->s, p->g, p->t_hop, p->m);
+  });
 }
 
 int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,

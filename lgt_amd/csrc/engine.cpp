@@ -1280,6 +1280,372 @@ void su_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax) {
 }
 
 // ════════════════════════════════════════════════════════════════════════════
+//  Dense simple update with plaquettes (finite_peps_ground_state.jl:175-541)
+// ════════════════════════════════════════════════════════════════════════════
+//
+// Same reduced-core route as the gauge update: the weighted two-site tensor
+// Θ = (I⊗Q_L)·C·(I⊗Q_R)ᵀ with orthonormal Q's, so the SVD of the gated core
+// C has Θ's singular values and its vectors map back through Q_L, Q_R.  No
+// Gauss mask and no flux labels: after a dense update the bond labels are
+// undefined and the state is marked unlabelled.
+
+static std::vector<double> vpeel(const std::vector<double>& sq) {
+  std::vector<double> r(sq.size());
+  for (size_t i = 0; i < sq.size(); ++i) r[i] = 1.0 / (sq[i] + SU_PEEL);
+  return r;
+}
+
+namespace {
+struct DenseSplit {
+  Tensor U, Vt;                // (dpL·a x Dk), (Dk x dpR·b), sqrt(S) applied
+  std::vector<double> lam;     // S / ||S||
+};
+}  // namespace
+
+// SVD of the gated core, keep <= Dtrunc values above 1e-12 (at least one),
+// U·sqrt(S) and sqrt(S)·V^H (finite_peps_ground_state.jl:241-257).
+static DenseSplit dense_split(Ctx& c, const Tensor& core, int Dtrunc) {
+  TierGuard fp64(c, TIER_FP64);
+  SVD f = svd_thin(c, core);
+  int64_t cnt = 0;
+  for (double x : f.S) cnt += x > SVD_CUT;
+  const int64_t Dk = std::max<int64_t>(1, std::min<int64_t>({int64_t(Dtrunc), cnt, int64_t(f.S.size())}));
+  double nS = 0;
+  for (int64_t i = 0; i < Dk; ++i) nS += f.S[i] * f.S[i];
+  nS = std::sqrt(nS);
+  PEPS_CHECK(nS > 0, PS_ESOLVER, "dense simple update: the gated two-site tensor is zero");
+  DenseSplit d;
+  std::vector<double> sq(Dk);
+  d.lam.resize(Dk);
+  for (int64_t i = 0; i < Dk; ++i) {
+    d.lam[i] = f.S[i] / nS;
+    sq[i] = std::sqrt(f.S[i]);
+  }
+  d.U = copy_tensor(c, f.U.slice_last(0, Dk));
+  scale_cols(c, d.U, sq);
+  d.Vt = gather2d(c, f.Vt, iota_v(Dk), iota_v(f.Vt.dims[1]));
+  scale_rows(c, d.Vt, sq);
+  return d;
+}
+
+void dense_bond_h(State& s, int ix, int iy, const Tensor& G, int Dtrunc) {
+  // update_bond_h! :175-273
+  Ctx& c = *s.c;
+  const int nx = s.nx, ny = s.ny;
+  const std::vector<double> lam = s.LH(ix, iy);
+  PEPS_CHECK(s.t(ix, iy).dims[2] == int64_t(lam.size()) && s.t(ix + 1, iy).dims[1] == int64_t(lam.size()),
+             PS_ESHAPE, "update_bond_h: bond dimension mismatch");
+  std::vector<double> sq_l = vsqrt(ix > 1 ? s.LH(ix - 1, iy) : ones1());
+  std::vector<double> sq_r = vsqrt(ix + 1 < nx ? s.LH(ix + 1, iy) : ones1());
+  std::vector<double> sq_uL = vsqrt(iy < ny ? s.LV(ix, iy) : ones1());
+  std::vector<double> sq_dL = vsqrt(iy > 1 ? s.LV(ix, iy - 1) : ones1());
+  std::vector<double> sq_uR = vsqrt(iy < ny ? s.LV(ix + 1, iy) : ones1());
+  std::vector<double> sq_dR = vsqrt(iy > 1 ? s.LV(ix + 1, iy - 1) : ones1());
+  Tensor Lw = copy_tensor(c, s.t(ix, iy)), Rw = copy_tensor(c, s.t(ix + 1, iy));
+  std::vector<double> wL[4] = {sq_l, vsqrt(lam), sq_uL, sq_dL};
+  std::vector<double> wR[4] = {vsqrt(lam), sq_r, sq_uR, sq_dR};
+  scale_legs(c, Lw, wL);
+  scale_legs(c, Rw, wR);
+  const int64_t dpL = Lw.dims[0], Dl = Lw.dims[1], DuL = Lw.dims[3], DdL = Lw.dims[4];
+  const int64_t dpR = Rw.dims[0], Dr = Rw.dims[2], DuR = Rw.dims[3], DdR = Rw.dims[4];
+  Tensor QL, RL, QR_, RR;
+  qr_reduce(c, Lw, {1, 3, 4, 0, 2}, 3, &QL, &RL);   // QL (l,u,d,a)
+  qr_reduce(c, Rw, {2, 3, 4, 0, 1}, 3, &QR_, &RR);  // QR (r,u,d,b)
+  const int64_t a = QL.dims[3], b = QR_.dims[3];
+  Tensor th = apply_gate(c, contract(c, RL, "apk", RR, "bqk", "apqb"), G);
+  DenseSplit d = dense_split(c, permute(c, th, {1, 0, 2, 3}).reshaped({dpL * a, dpR * b}), Dtrunc);
+  const int64_t Dk = int64_t(d.lam.size());
+  Tensor Ufull = contract(c, d.U.reshaped({dpL, a, Dk}), "pak", QL, "luda", "pludk");
+  Tensor Vfull = contract(c, d.Vt.reshaped({Dk, dpR, b}), "kqb", QR_, "rvwb", "kqrvw");
+  Tensor Lnew = permute(c, Ufull, {0, 1, 4, 2, 3});
+  Tensor Rnew = permute(c, Vfull, {1, 0, 2, 3, 4});
+  std::vector<double> pL[4] = {vpeel(sq_l), vones(Dk), vpeel(sq_uL), vpeel(sq_dL)};
+  std::vector<double> pR[4] = {vones(Dk), vpeel(sq_r), vpeel(sq_uR), vpeel(sq_dR)};
+  scale_legs(c, Lnew, pL);
+  scale_legs(c, Rnew, pR);
+  (void)Dl; (void)DuL; (void)DdL; (void)Dr; (void)DuR; (void)DdR;
+  s.t(ix, iy) = Lnew;
+  s.t(ix + 1, iy) = Rnew;
+  s.LH(ix, iy) = d.lam;
+  s.QH(ix, iy).assign(size_t(Dk), 0);
+  s.labelled = false;
+}
+
+void dense_bond_v(State& s, int ix, int iy, const Tensor& G, int Dtrunc) {
+  // update_bond_v! :280-382
+  Ctx& c = *s.c;
+  const int nx = s.nx, ny = s.ny;
+  const std::vector<double> lam = s.LV(ix, iy);
+  PEPS_CHECK(s.t(ix, iy).dims[3] == int64_t(lam.size()) && s.t(ix, iy + 1).dims[4] == int64_t(lam.size()),
+             PS_ESHAPE, "update_bond_v: bond dimension mismatch");
+  std::vector<double> sq_lD = vsqrt(ix > 1 ? s.LH(ix - 1, iy) : ones1());
+  std::vector<double> sq_rD = vsqrt(ix < nx ? s.LH(ix, iy) : ones1());
+  std::vector<double> sq_dD = vsqrt(iy > 1 ? s.LV(ix, iy - 1) : ones1());
+  std::vector<double> sq_lU = vsqrt(ix > 1 ? s.LH(ix - 1, iy + 1) : ones1());
+  std::vector<double> sq_rU = vsqrt(ix < nx ? s.LH(ix, iy + 1) : ones1());
+  std::vector<double> sq_uU = vsqrt(iy + 1 < ny ? s.LV(ix, iy + 1) : ones1());
+  Tensor Dw = copy_tensor(c, s.t(ix, iy)), Uw = copy_tensor(c, s.t(ix, iy + 1));
+  std::vector<double> wD[4] = {sq_lD, sq_rD, vsqrt(lam), sq_dD};
+  std::vector<double> wU[4] = {sq_lU, sq_rU, sq_uU, vsqrt(lam)};
+  scale_legs(c, Dw, wD);
+  scale_legs(c, Uw, wU);
+  const int64_t dpD = Dw.dims[0], dpU = Uw.dims[0];
+  Tensor QD, RD, QU, RU;
+  qr_reduce(c, Dw, {1, 2, 4, 0, 3}, 3, &QD, &RD);  // QD (l,r,d,a)
+  qr_reduce(c, Uw, {1, 2, 3, 0, 4}, 3, &QU, &RU);  // QU (l,r,u,b)
+  const int64_t a = QD.dims[3], b = QU.dims[3];
+  Tensor th = apply_gate(c, contract(c, RD, "apk", RU, "bqk", "apqb"), G);
+  DenseSplit d = dense_split(c, permute(c, th, {1, 0, 2, 3}).reshaped({dpD * a, dpU * b}), Dtrunc);
+  const int64_t Dk = int64_t(d.lam.size());
+  Tensor Ufull = contract(c, d.U.reshaped({dpD, a, Dk}), "pak", QD, "lrda", "plrdk");
+  Tensor Vfull = contract(c, d.Vt.reshaped({Dk, dpU, b}), "kqb", QU, "xyzb", "kqxyz");
+  Tensor Dnew = permute(c, Ufull, {0, 1, 2, 4, 3});
+  Tensor Unew = permute(c, Vfull, {1, 2, 3, 4, 0});
+  std::vector<double> pD[4] = {vpeel(sq_lD), vpeel(sq_rD), vones(Dk), vpeel(sq_dD)};
+  std::vector<double> pU[4] = {vpeel(sq_lU), vpeel(sq_rU), vpeel(sq_uU), vones(Dk)};
+  scale_legs(c, Dnew, pD);
+  scale_legs(c, Unew, pU);
+  s.t(ix, iy) = Dnew;
+  s.t(ix, iy + 1) = Unew;
+  s.LV(ix, iy) = d.lam;
+  s.QV(ix, iy).assign(size_t(Dk), 0);
+  s.labelled = false;
+}
+
+PlaquetteGate plaquette_gate(Ctx& c, int gd, double g, double tau, double cutoff, int D_aux_max, bool cplx) {
+  // decompose_plaquette_LShape(exp(-tau H_plaquette_gauge)), u1_lgt_hamiltonian.jl:728-768
+  PEPS_CHECK(gd >= 2 && D_aux_max >= 1, PS_EARG, "plaquette_gate: invalid arguments");
+  TierGuard fp64(c, TIER_FP64);
+  const int64_t g2 = int64_t(gd) * gd, g4 = g2 * g2;
+  Tensor G = gate_tensor(c, expm_sym(H_plaquette_gauge(gd, g), tau), cplx);
+  auto keep = [&](const std::vector<double>& S) {
+    int64_t cnt = 0;
+    for (double x : S) cnt += x > cutoff;
+    return std::max<int64_t>(1, std::min<int64_t>({int64_t(D_aux_max), cnt, int64_t(S.size())}));
+  };
+  auto roots = [](const std::vector<double>& S, int64_t k) {
+    std::vector<double> r(k);
+    for (int64_t i = 0; i < k; ++i) r[i] = std::sqrt(S[i]);
+    return r;
+  };
+  // SVD 1: (b,l,t,b',l',t') x (r,r')
+  Tensor M1 = permute(c, G.reshaped({gd, gd, gd, gd, gd, gd, gd, gd}), {0, 1, 3, 4, 5, 7, 2, 6});
+  SVD f1 = svd_thin(c, M1.reshaped({g4 * g2, g2}));
+  const int64_t Da = keep(f1.S);
+  std::vector<double> sq1 = roots(f1.S, Da);
+  Tensor GAC = copy_tensor(c, f1.U.slice_last(0, Da));
+  scale_cols(c, GAC, sq1);
+  Tensor GBm = gather2d(c, f1.Vt, iota_v(Da), iota_v(g2));
+  scale_rows(c, GBm, sq1);
+  PlaquetteGate pg;
+  pg.GB = permute(c, GBm.reshaped({Da, gd, gd}), {1, 2, 0});
+  // SVD 2: (b,l,b',l',α) x (t,t')
+  Tensor M2 = permute(c, GAC.reshaped({gd, gd, gd, gd, gd, gd, Da}), {0, 1, 3, 4, 6, 2, 5});
+  SVD f2 = svd_thin(c, M2.reshaped({g2 * g2 * Da, g2}));
+  const int64_t Db = keep(f2.S);
+  std::vector<double> sq2 = roots(f2.S, Db);
+  pg.GA = copy_tensor(c, f2.U.slice_last(0, Db));
+  scale_cols(c, pg.GA, sq2);
+  pg.GA = pg.GA.reshaped({gd, gd, gd, gd, Da, Db});
+  Tensor GCm = gather2d(c, f2.Vt, iota_v(Db), iota_v(g2));
+  scale_rows(c, GCm, sq2);
+  pg.GC = permute(c, GCm.reshaped({Db, gd, gd}), {1, 2, 0});
+  pg.Da = int(Da);
+  pg.Db = int(Db);
+  return pg;
+}
+
+void update_plaquette(State& s, int ix, int iy, const PlaquetteGate& pg, int Dtrunc) {
+  // update_plaquette! :405-483.  GA/GB/GC are contracted through their first
+  // (row) indices, as in the reference.
+  Ctx& c = *s.c;
+  const int nx = s.nx, ny = s.ny, gd = s.gd;
+  PEPS_CHECK(ix >= 1 && ix < nx && iy >= 1 && iy < ny, PS_EARG, "update_plaquette: needs ix < nx and iy < ny");
+  const int64_t Da = pg.Da, Db = pg.Db;
+  PEPS_CHECK(pg.GA.rank() == 6 && pg.GA.dims[0] == gd && pg.GA.dims[4] == Da && pg.GA.dims[5] == Db &&
+                 pg.GB.dims[0] == gd && pg.GB.dims[2] == Da && pg.GC.dims[0] == gd && pg.GC.dims[2] == Db,
+             PS_ESHAPE, "update_plaquette: gate shapes");
+  SiteDims sA = site_dims(ix, iy, nx, ny, gd), sB = site_dims(ix + 1, iy, nx, ny, gd),
+           sC = site_dims(ix, iy + 1, nx, ny, gd);
+  PEPS_CHECK(sA.dR == gd && sA.dU == gd && sB.dU == gd && sC.dR == gd, PS_ESHAPE,
+             "update_plaquette: plaquette links missing");
+  const Tensor &TA = s.t(ix, iy), &TB = s.t(ix + 1, iy), &TC = s.t(ix, iy + 1);
+  PEPS_CHECK(TA.dims[2] == int64_t(s.LH(ix, iy).size()) && TA.dims[3] == int64_t(s.LV(ix, iy).size()),
+             PS_ESHAPE, "update_plaquette: bond dimension mismatch");
+  const std::vector<int64_t>& a = TA.dims;
+  const std::vector<int64_t>& b = TB.dims;
+  const std::vector<int64_t>& cc = TC.dims;
+  // physical index = (iU fastest, iR, nf)
+  Tensor A7 = TA.reshaped({gd, gd, 2, a[1], a[2], a[3], a[4]});
+  Tensor B7 = TB.reshaped({sB.dU, sB.dR, 2, b[1], b[2], b[3], b[4]});
+  Tensor C7 = TC.reshaped({sC.dU, sC.dR, 2, cc[1], cc[2], cc[3], cc[4]});
+  Tensor An = contract(c, pg.GA, "RUSVxy", A7, "URnlrud", "VSnlrxuyd");
+  Tensor Bn = contract(c, pg.GB, "UVx", B7, "URnlrud", "VRnlxrud");
+  Tensor Cn = contract(c, pg.GC, "RSy", C7, "URnlrud", "USnlrudy");
+  s.t(ix, iy) = An.reshaped({a[0], a[1], a[2] * Da, a[3] * Db, a[4]});
+  s.t(ix + 1, iy) = Bn.reshaped({b[0], b[1] * Da, b[2], b[3], b[4]});
+  s.t(ix, iy + 1) = Cn.reshaped({cc[0], cc[1], cc[2], cc[3], cc[4] * Db});
+  auto tile = [](const std::vector<double>& v, int64_t n) {
+    std::vector<double> r;
+    for (int64_t k = 0; k < n; ++k) r.insert(r.end(), v.begin(), v.end());
+    return r;
+  };
+  s.LH(ix, iy) = tile(s.LH(ix, iy), Da);
+  s.LV(ix, iy) = tile(s.LV(ix, iy), Db);
+  s.QH(ix, iy).assign(s.LH(ix, iy).size(), 0);
+  s.QV(ix, iy).assign(s.LV(ix, iy).size(), 0);
+  s.labelled = false;
+  dense_bond_h(s, ix, iy, gate_tensor(c, identity(int(a[0] * b[0])), s.cplx), Dtrunc);
+  dense_bond_v(s, ix, iy, gate_tensor(c, identity(int(a[0] * cc[0])), s.cplx), Dtrunc);
+}
+
+// ── simple-update observables (finite_peps_ground_state.jl:549-805) ─────────
+
+// rho = T T^H of a site tensor flattened as (p x rest), downloaded.
+static std::vector<double> site_rho(Ctx& c, const Tensor& T) {
+  const int64_t dp = T.dims[0], rest = T.numel() / dp;
+  Tensor Tf = T.reshaped({dp, rest});
+  return c.to_host(matmul(c, Tf, Tf, false, true, false, true));
+}
+
+// Re tr(O rho) and Re tr(rho) for an n x n rho (complex interleaved when cplx)
+// and a real or complex O with the same layout.
+static void trace_pair(const std::vector<double>& O, bool o_cplx, const std::vector<double>& rho, bool cplx,
+                       int64_t n, double* num, double* den) {
+  double nu = 0, de = 0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j < n; ++j) {
+      // O[i,j] * rho[j,i]
+      const int64_t ij = i + j * n, ji = j + i * n;
+      const double ore = o_cplx ? O[2 * ij] : O[ij], oim = o_cplx ? O[2 * ij + 1] : 0.0;
+      const double rre = cplx ? rho[2 * ji] : rho[ji], rim = cplx ? rho[2 * ji + 1] : 0.0;
+      nu += ore * rre - oim * rim;
+    }
+  for (int64_t i = 0; i < n; ++i) de += cplx ? rho[2 * (i + i * n)] : rho[i + i * n];
+  *num = nu;
+  *den = de;
+}
+
+double expect_site(State& s, int ix, int iy, const std::vector<double>& O, bool o_cplx) {
+  // expect_site :549-567: lambda (not its root) on every leg
+  Ctx& c = *s.c;
+  Tensor Tw = copy_tensor(c, s.t(ix, iy));
+  std::vector<double> w[4] = {w_l(s, ix, iy), w_r(s, ix, iy), w_u(s, ix, iy), w_d(s, ix, iy)};
+  scale_legs(c, Tw, w);
+  const int64_t dp = Tw.dims[0];
+  PEPS_CHECK(int64_t(O.size()) == dp * dp * (o_cplx ? 2 : 1), PS_ESHAPE, "expect_site: operator size");
+  double num, den;
+  trace_pair(O, o_cplx, site_rho(c, Tw), s.cplx, dp, &num, &den);
+  return den > 1e-15 ? num / den : 0.0;
+}
+
+double plaquette_energy(State& s, int ix, int iy, double g) {
+  // plaquette_energy :589-684: -(1/g^2) Re<W>, sqrt(lambda) on every leg,
+  // the external legs closed diagonally
+  Ctx& c = *s.c;
+  const int nx = s.nx, ny = s.ny, gd = s.gd;
+  PEPS_CHECK(ix >= 1 && ix < nx && iy >= 1 && iy < ny, PS_EARG, "plaquette_energy: needs ix < nx and iy < ny");
+  SiteDims sB = site_dims(ix + 1, iy, nx, ny, gd), sC = site_dims(ix, iy + 1, nx, ny, gd);
+  std::vector<double> sAB = vsqrt(s.LH(ix, iy)), sAC = vsqrt(s.LV(ix, iy));
+  Tensor TA = copy_tensor(c, s.t(ix, iy)), TB = copy_tensor(c, s.t(ix + 1, iy)), TC = copy_tensor(c, s.t(ix, iy + 1));
+  std::vector<double> wA[4] = {vsqrt(w_l(s, ix, iy)), sAB, sAC, vsqrt(w_d(s, ix, iy))};
+  std::vector<double> wB[4] = {sAB, vsqrt(ix + 1 < nx ? s.LH(ix + 1, iy) : ones1()), vsqrt(w_u(s, ix + 1, iy)),
+                               vsqrt(w_d(s, ix + 1, iy))};
+  std::vector<double> wC[4] = {vsqrt(w_l(s, ix, iy + 1)), vsqrt(w_r(s, ix, iy + 1)),
+                               vsqrt(iy + 1 < ny ? s.LV(ix, iy + 1) : ones1()), sAC};
+  scale_legs(c, TA, wA);
+  scale_legs(c, TB, wB);
+  scale_legs(c, TC, wC);
+  // single-site Wilson-loop factors: A Ud on R and U on U, B Ud on U, C U on R
+  Mat U = op_U_gauge(gd), Ud = U;
+  for (int i = 0; i < gd; ++i)
+    for (int j = 0; j < gd; ++j) Ud(i, j) = U(j, i);
+  Mat WA = kron(kron(identity(2), Ud), U);
+  Mat WB = sB.dR == 1 ? kron(identity(2), Ud) : kron(kron(identity(2), identity(sB.dR)), Ud);
+  Mat WC = sC.dU == 1 ? kron(identity(2), U) : kron(kron(identity(2), U), identity(sC.dU));
+  auto apply = [&](const Tensor& T, const Mat& W) {
+    const int64_t dp = T.dims[0];
+    PEPS_CHECK(W.n == dp, PS_ESHAPE, "plaquette_energy: operator size");
+    Tensor Wt = gate_tensor(c, W, s.cplx);
+    return matmul(c, Wt, T.reshaped({dp, T.numel() / dp})).reshaped(T.dims);
+  };
+  auto scalar = [&](const Tensor& Ak, const Tensor& Bk, const Tensor& Ck) {
+    Tensor MA = contract(c, Ak, "plrud", TA, "plsvd", "rsuv", false, true);
+    Tensor MB = contract(c, Bk, "plrud", TB, "pmrud", "lm", false, true);
+    Tensor MC = contract(c, Ck, "plrud", TC, "plrue", "de", false, true);
+    Tensor MAB = contract(c, MA, "rsuv", MB, "rs", "uv");
+    return dot(c, MAB, MC, false);
+  };
+  cplx num = scalar(apply(TA, WA), apply(TB, WB), apply(TC, WC));
+  cplx den = scalar(TA, TB, TC);
+  return std::hypot(den.re, den.im) > 1e-15 ? -(1.0 / (g * g)) * num.re / den.re : 0.0;
+}
+
+// Hopping term of total_energy (:727-735, :778-788): Theta[p1,p2,a,b] =
+// sum_chi L[p1,chi,a] conj(R[p2,chi,b]) flattened with p1 fastest, rho =
+// Theta Theta^H, Re tr(H rho) / Re tr(rho) (H has the second site fastest;
+// kept as written).
+static double su_hop_expect(Ctx& c, const Tensor& Lb, const Tensor& Rb, const Mat& H, bool cp) {
+  const int64_t dpL = Lb.dims[0], dpR = Rb.dims[0];
+  Tensor th = contract(c, Lb, "pca", Rb, "qcb", "pqab", false, true);
+  Tensor tf = th.reshaped({dpL * dpR, th.numel() / (dpL * dpR)});
+  std::vector<double> rho = c.to_host(matmul(c, tf, tf, false, true, false, true));
+  PEPS_CHECK(H.n == dpL * dpR, PS_ESHAPE, "total_energy: hopping operator size");
+  double num, den;
+  trace_pair(H.a, false, rho, cp, dpL * dpR, &num, &den);
+  return den > 1e-15 ? num / den : 0.0;
+}
+
+double total_energy(State& s, double g, double t, double m) {
+  // total_energy :690-805
+  Ctx& c = *s.c;
+  const int nx = s.nx, ny = s.ny, gd = s.gd;
+  double E = 0.0;
+  for (int iy = 1; iy <= ny; ++iy)
+    for (int ix = 1; ix <= nx; ++ix) E += expect_site(s, ix, iy, H_onsite_site(ix, iy, nx, ny, gd, g, m).a, false);
+  auto flat3 = [](const Tensor& T, int bond_axis) {
+    // (p, bond, rest) with the other three legs in their original order
+    std::vector<int> perm = {0, bond_axis};
+    for (int k = 1; k < 5; ++k)
+      if (k != bond_axis) perm.push_back(k);
+    return perm;
+  };
+  for (int iy = 1; iy <= ny; ++iy)
+    for (int ix = 1; ix < nx; ++ix) {
+      std::vector<double> sb = vsqrt(s.LH(ix, iy));
+      Tensor Lw = copy_tensor(c, s.t(ix, iy)), Rw = copy_tensor(c, s.t(ix + 1, iy));
+      std::vector<double> wL[4] = {vsqrt(w_l(s, ix, iy)), sb, vsqrt(w_u(s, ix, iy)), vsqrt(w_d(s, ix, iy))};
+      std::vector<double> wR[4] = {sb, vsqrt(ix + 1 < nx ? s.LH(ix + 1, iy) : ones1()), vsqrt(w_u(s, ix + 1, iy)),
+                                   vsqrt(w_d(s, ix + 1, iy))};
+      scale_legs(c, Lw, wL);
+      scale_legs(c, Rw, wR);
+      Tensor Lb = permute(c, Lw, flat3(Lw, 2)), Rb = permute(c, Rw, flat3(Rw, 1));
+      const int64_t bd = Lb.dims[1];
+      E += su_hop_expect(c, Lb.reshaped({Lb.dims[0], bd, Lb.numel() / (Lb.dims[0] * bd)}),
+                         Rb.reshaped({Rb.dims[0], bd, Rb.numel() / (Rb.dims[0] * bd)}),
+                         H_hop_h_site(ix, iy, nx, ny, gd, t), s.cplx);
+    }
+  for (int iy = 1; iy < ny; ++iy)
+    for (int ix = 1; ix <= nx; ++ix) {
+      std::vector<double> sb = vsqrt(s.LV(ix, iy));
+      Tensor Dw = copy_tensor(c, s.t(ix, iy)), Uw = copy_tensor(c, s.t(ix, iy + 1));
+      std::vector<double> wD[4] = {vsqrt(w_l(s, ix, iy)), vsqrt(w_r(s, ix, iy)), sb, vsqrt(w_d(s, ix, iy))};
+      std::vector<double> wU[4] = {vsqrt(w_l(s, ix, iy + 1)), vsqrt(w_r(s, ix, iy + 1)),
+                                   vsqrt(iy + 1 < ny ? s.LV(ix, iy + 1) : ones1()), sb};
+      scale_legs(c, Dw, wD);
+      scale_legs(c, Uw, wU);
+      Tensor Db = permute(c, Dw, flat3(Dw, 3)), Ub = permute(c, Uw, flat3(Uw, 4));
+      const int64_t bd = Db.dims[1];
+      E += su_hop_expect(c, Db.reshaped({Db.dims[0], bd, Db.numel() / (Db.dims[0] * bd)}),
+                         Ub.reshaped({Ub.dims[0], bd, Ub.numel() / (Ub.dims[0] * bd)}),
+                         H_hop_v_site(ix, iy, nx, ny, gd, t), s.cplx);
+    }
+  for (int iy = 1; iy < ny; ++iy)
+    for (int ix = 1; ix < nx; ++ix) E += plaquette_energy(s, ix, iy, g);
+  return E;
+}
+
+// ════════════════════════════════════════════════════════════════════════════
 //  Drivers
 // ════════════════════════════════════════════════════════════════════════════
 
@@ -1310,6 +1676,22 @@ void trotter_step_gauge(State& s, const Params& p, GateCache& gc) {
     for (int ix = 1; ix <= s.nx; ++ix) su_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 1.0), p.Dmax);
   for (int iy = 1; iy <= s.ny; ++iy)
     for (int ix = s.nx - 1; ix >= 1; --ix) su_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax);
+}
+
+void trotter_step_dense(State& s, const Params& p, GateCache& gc) {
+  // trotter_step! :498-541: h-half (L->R), v-half (B->T), plaquettes,
+  // v-half (T->B), h-half (R->L); the plaquette gate is split once per step
+  PlaquetteGate pg = plaquette_gate(*s.c, s.gd, p.g, p.tau, SVD_CUT, 64, s.cplx);
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = 1; ix < s.nx; ++ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax);
+  for (int iy = 1; iy < s.ny; ++iy)
+    for (int ix = 1; ix <= s.nx; ++ix) dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 0.5), p.Dmax);
+  for (int iy = 1; iy < s.ny; ++iy)
+    for (int ix = 1; ix < s.nx; ++ix) update_plaquette(s, ix, iy, pg, p.Dmax);
+  for (int iy = s.ny - 1; iy >= 1; --iy)
+    for (int ix = s.nx; ix >= 1; --ix) dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 0.5), p.Dmax);
+  for (int iy = s.ny; iy >= 1; --iy)
+    for (int ix = s.nx - 1; ix >= 1; --ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax);
 }
 
 // All top boundaries of the current state: tops[r] = boundary_top(r), r=1..ny.

@@ -14,6 +14,8 @@ constexpr double FU_REG = 1e-10;      // finite_peps_full_update.jl:336
 constexpr double GI_REG = 1e-12;      // finite_peps_gauge_invariant.jl:224
 constexpr double V4_ALS_REG = 1e-10;  // finite_peps_fullupdate_v4.jl:30
 constexpr double CMPS_CUT = 1e-14;    // finite_peps_boundary_mps.jl:93
+constexpr double SVD_CUT = 1e-12;     // finite_peps_ground_state.jl:57
+constexpr double SU_PEEL = 1e-15;     // finite_peps_ground_state.jl:264-269
 
 using MPS = std::vector<Tensor>;
 
@@ -34,6 +36,9 @@ struct State {
   std::vector<Tensor> T;
   std::vector<std::vector<double>> lh, lv;
   std::vector<std::vector<int64_t>> qh, qv;
+  // false once a dense (label-free) update has run: the flux labels are
+  // then undefined and the gauge-invariant operations refuse the state
+  bool labelled = true;
 
   Tensor& t(int ix, int iy) { return T.at(size_t(ix - 1) + size_t(iy - 1) * nx); }
   std::vector<double>& LH(int ix, int iy) { return lh.at(size_t(ix - 1) + size_t(iy - 1) * (nx - 1)); }
@@ -135,5 +140,25 @@ void trotter_step_v4(State& s, const Params& p, GateCache& gc, std::vector<ALSIn
 double energy_v4(State& s, const Params& p);
 // <n_f> from simple-update weights, averaged over sites (finite_peps_quench.jl:134).
 double mean_nf(State& s);
+
+// ── dense simple update with plaquettes (finite_peps_ground_state.jl) ───────
+struct PlaquetteGate {
+  Tensor GA, GB, GC;  // (gd,gd,gd,gd,Da,Db), (gd,gd,Da), (gd,gd,Db)
+  int Da = 1, Db = 1;
+};
+// update_bond_h! / update_bond_v! (:175-382)
+void dense_bond_h(State& s, int ix, int iy, const Tensor& G, int Dtrunc);
+void dense_bond_v(State& s, int ix, int iy, const Tensor& G, int Dtrunc);
+// decompose_plaquette_LShape(exp(-tau H_plaquette_gauge)) (u1_lgt_hamiltonian.jl:728-768)
+PlaquetteGate plaquette_gate(Ctx& c, int gd, double g, double tau, double cutoff, int D_aux_max, bool cplx);
+// update_plaquette! (:405-483)
+void update_plaquette(State& s, int ix, int iy, const PlaquetteGate& pg, int Dtrunc);
+// trotter_step! (:498-541)
+void trotter_step_dense(State& s, const Params& p, GateCache& gc);
+// expect_site (:549-567), O is dp x dp column-major (complex interleaved if o_cplx)
+double expect_site(State& s, int ix, int iy, const std::vector<double>& O, bool o_cplx);
+// plaquette_energy (:589-684) and total_energy (:690-805)
+double plaquette_energy(State& s, int ix, int iy, double g);
+double total_energy(State& s, double g, double t, double m);
 
 }  // namespace peps

@@ -51,7 +51,7 @@ static Mat eye(int n) {
   return m;
 }
 // Julia kron(A, B): (i-1)*p + k, B fastest.
-static Mat kron(const Mat& A, const Mat& B) {
+Mat kron(const Mat& A, const Mat& B) {
   Mat C = zeros(A.n * B.n);
   for (int j = 0; j < A.n; ++j)
     for (int i = 0; i < A.n; ++i) {
@@ -164,6 +164,20 @@ Mat H_merged_v_site(int ix, int iy, int nx, int ny, int gd, double g, double t, 
   axpy(H, onsite_weight(ix, iy + 1, nx, ny), kron(eye(dD), H_onsite_site(ix, iy + 1, nx, ny, gd, g, m)));
   return H;
 }
+
+Mat H_plaquette_gauge(int gd, double g) {
+  // u1_lgt_hamiltonian.jl:283-289: -(1/2g^2)(W + W'), W = Ud (x) U (x) Ud (x) U
+  Mat U = op_U(gd), Ud = transpose(op_U(gd));
+  Mat W = kron(kron3(Ud, U, Ud), U);
+  Mat H = zeros(W.n);
+  axpy(H, -1.0 / (2.0 * g * g), W);
+  axpy(H, -1.0 / (2.0 * g * g), transpose(W));
+  return hsym(H);
+}
+
+Mat identity(int n) { return eye(n); }
+
+Mat op_U_gauge(int gd) { return op_U(gd); }
 
 // Cyclic Jacobi eigendecomposition of a real symmetric matrix (Golub & Van
 // Loan, Alg. 8.5.3, with the classical rotation formulas).

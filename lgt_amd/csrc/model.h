@@ -35,6 +35,14 @@ Mat H_hop_v_site(int ix, int iy, int nx, int ny, int gd, double t);
 double onsite_weight(int ix, int iy, int nx, int ny);
 Mat H_merged_h_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m);
 Mat H_merged_v_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m);
+// Pure-gauge plaquette term on the links (b, l, r, t), b slowest
+// (u1_lgt_hamiltonian.jl:283-289).
+Mat H_plaquette_gauge(int gd, double g);
+Mat identity(int n);
+// Raising operator U|e> = |e+1> of one link (u1_lgt_hamiltonian.jl:123-130).
+Mat op_U_gauge(int gd);
+// Julia kron(A, B): the second factor is the fastest index.
+Mat kron(const Mat& A, const Mat& B);
 // exp(-tau H) for real symmetric H (Jacobi eigendecomposition, FP64).
 Mat expm_sym(const Mat& H, double tau);
 

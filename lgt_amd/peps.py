@@ -21,7 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PEPS_HIP_LIB", os.path.join(_HERE, "libpeps_hip.so"))
 
 PEPS_H, PEPS_V = 0, 1
-PEPS_STEP_V4, PEPS_STEP_GAUGE = 0, 1
+PEPS_STEP_V4, PEPS_STEP_GAUGE, PEPS_STEP_DENSE = 0, 1, 2
 TIERS = {"fp64": 0, "fp32": 1, "bf16": 2}
 STATUS = {0: "PEPS_OK", 1: "PEPS_EARG", 2: "PEPS_ESHAPE", 3: "PEPS_ENOSECTOR", 4: "PEPS_EHIP",
           5: "PEPS_ENOMEM", 6: "PEPS_ESOLVER", 7: "PEPS_EUNSUPPORTED"}
@@ -85,6 +85,14 @@ _SIGS = {
     "peps_trotter_step": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params)]),
     "peps_last_als_stats": (C.c_int32, [_vp, _i64p]),
     "peps_energy_v4": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
+    "peps_su_bond_dense": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32]),
+    "peps_plaquette_gate": (C.c_int32, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, _i32p, _i32p,
+                                        _dp, _dp, _dp]),
+    "peps_update_plaquette": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp,
+                                          C.c_int32]),
+    "peps_expect_site": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp, _dp]),
+    "peps_plaquette_energy": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double, _dp]),
+    "peps_total_energy": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
     "peps_eigh": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "peps_svd": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]),
@@ -364,6 +372,73 @@ def trotter_step_gauge(gp: GaugePEPS, g, t_hop, m, tau, D_max):
     """trotter_step_gauge! (finite_peps_gauge_invariant.jl:523)."""
     p = _params(g, t_hop, m, tau, D_max, 1, 0)
     gp.ctx.check(lib().peps_trotter_step(gp.h, PEPS_STEP_GAUGE, C.byref(p)))
+
+
+# ── finite_peps_ground_state.jl (dense simple update with plaquettes) ───────
+# These treat the state as a FinitePEPS (no mask, no flux labels): after a
+# dense update the gauge-invariant calls refuse the state.
+
+def update_bond_h(gp: GaugePEPS, ix, iy, gate, D_trunc):
+    """update_bond_h! (finite_peps_ground_state.jl:175)."""
+    G = _as_array(gate, gp.is_complex)
+    gp.ctx.check(lib().peps_su_bond_dense(gp.h, PEPS_H, ix, iy, _dptr(G), D_trunc))
+
+
+def update_bond_v(gp: GaugePEPS, ix, iy, gate, D_trunc):
+    """update_bond_v! (finite_peps_ground_state.jl:280)."""
+    G = _as_array(gate, gp.is_complex)
+    gp.ctx.check(lib().peps_su_bond_dense(gp.h, PEPS_V, ix, iy, _dptr(G), D_trunc))
+
+
+def plaquette_gate(gd, g, tau, cutoff=1e-12, ctx=None):
+    """decompose_plaquette_LShape(exp(-tau H_plaquette_gauge)) (u1_lgt_hamiltonian.jl:728)
+    -> (GA, GB, GC)."""
+    ctx = ctx or default_context()
+    Da, Db = C.c_int32(), C.c_int32()
+    nul = C.cast(None, _dp)
+    ctx.check(lib().peps_plaquette_gate(ctx.h, gd, g, tau, cutoff, C.byref(Da), C.byref(Db), nul, nul, nul))
+    GA = np.zeros((gd, gd, gd, gd, Da.value, Db.value), order="F")
+    GB = np.zeros((gd, gd, Da.value), order="F")
+    GC = np.zeros((gd, gd, Db.value), order="F")
+    ctx.check(lib().peps_plaquette_gate(ctx.h, gd, g, tau, cutoff, C.byref(Da), C.byref(Db), _dptr(GA),
+                                        _dptr(GB), _dptr(GC)))
+    return GA, GB, GC
+
+
+def update_plaquette(gp: GaugePEPS, ix, iy, GA, GB, GC, D_trunc):
+    """update_plaquette! (finite_peps_ground_state.jl:405)."""
+    A, B, Cc = (_as_array(x, gp.is_complex) for x in (GA, GB, GC))
+    gp.ctx.check(lib().peps_update_plaquette(gp.h, ix, iy, A.shape[4], A.shape[5], _dptr(A), _dptr(B), _dptr(Cc),
+                                             D_trunc))
+
+
+def trotter_step(gp: GaugePEPS, g, t_hop, m, tau, D_trunc):
+    """trotter_step! (finite_peps_ground_state.jl:498): h-half, v-half, plaquettes, v-half, h-half."""
+    p = _params(g, t_hop, m, tau, D_trunc, 1, 0)
+    gp.ctx.check(lib().peps_trotter_step(gp.h, PEPS_STEP_DENSE, C.byref(p)))
+
+
+def expect_site(gp: GaugePEPS, ix, iy, O):
+    """expect_site (finite_peps_ground_state.jl:549)."""
+    a = _as_array(O, gp.is_complex)
+    v = C.c_double()
+    gp.ctx.check(lib().peps_expect_site(gp.h, ix, iy, _dptr(a), C.byref(v)))
+    return v.value
+
+
+def plaquette_energy(gp: GaugePEPS, ix, iy, g):
+    """plaquette_energy (finite_peps_ground_state.jl:589)."""
+    v = C.c_double()
+    gp.ctx.check(lib().peps_plaquette_energy(gp.h, ix, iy, g, C.byref(v)))
+    return v.value
+
+
+def total_energy(gp: GaugePEPS, g, t_hop, m):
+    """total_energy (finite_peps_ground_state.jl:690)."""
+    p = _params(g, t_hop, m, 0.0, 1, 1, 0)
+    v = C.c_double()
+    gp.ctx.check(lib().peps_total_energy(gp.h, C.byref(p), C.byref(v)))
+    return v.value
 
 
 # ── finite_peps_full_update.jl / finite_peps_boundary_mps.jl ────────────────

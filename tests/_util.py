@@ -27,6 +27,30 @@ def su_state(nx, ny, gd, D, steps, seed, cplx=False, tau=0.05, g=1.0, t=1.0, m=0
     return gp
 
 
+def random_peps(nx, ny, gd, D, seed, cplx=False):
+    """Unmasked random FinitePEPS (bond labels all 0, unused) with random
+    positive decreasing weights."""
+    rng = np.random.default_rng(seed)
+    gp = O.GaugePEPS(nx, ny, gd, O.staggered_charges(nx, ny))
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d = O.site_dims(ix, iy, nx, ny, gd)[0]
+            dims = (d, D if ix > 1 else 1, D if ix < nx else 1, D if iy < ny else 1, D if iy > 1 else 1)
+            T = rng.standard_normal(dims)
+            if cplx:
+                T = T + 1j * rng.standard_normal(dims)
+            gp.setT(ix, iy, T.astype(np.complex128))
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx):
+            gp.lh[ix - 1][iy - 1] = np.sort(rng.uniform(0.2, 1.0, D))[::-1]
+            gp.qh[ix - 1][iy - 1] = np.zeros(D, np.int64)
+    for iy in range(1, ny):
+        for ix in range(1, nx + 1):
+            gp.lv[ix - 1][iy - 1] = np.sort(rng.uniform(0.2, 1.0, D))[::-1]
+            gp.qv[ix - 1][iy - 1] = np.zeros(D, np.int64)
+    return gp
+
+
 def mps_dense(mps):
     v = mps[0].reshape(mps[0].shape[0] * mps[0].shape[1], mps[0].shape[2], order="F")
     for A in mps[1:]:

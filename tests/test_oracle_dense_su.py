@@ -12,29 +12,9 @@ import numpy as np
 import pytest
 
 from oracle import peps_oracle as O
+from tests._util import random_peps as _random_peps
 
 G, T_HOP, M = 1.0, 1.0, 0.5
-
-
-def _random_peps(nx, ny, gd, D, seed, cplx=False):
-    """Unmasked random FinitePEPS (labels unused) with random positive weights."""
-    rng = np.random.default_rng(seed)
-    gp = O.GaugePEPS(nx, ny, gd, O.staggered_charges(nx, ny))
-    for iy in range(1, ny + 1):
-        for ix in range(1, nx + 1):
-            d = O.site_dims(ix, iy, nx, ny, gd)[0]
-            dims = (d, D if ix > 1 else 1, D if ix < nx else 1, D if iy < ny else 1, D if iy > 1 else 1)
-            T = rng.standard_normal(dims)
-            if cplx:
-                T = T + 1j * rng.standard_normal(dims)
-            gp.setT(ix, iy, T.astype(np.complex128))
-    for iy in range(1, ny + 1):
-        for ix in range(1, nx):
-            gp.lh[ix - 1][iy - 1] = np.sort(rng.uniform(0.2, 1.0, D))[::-1]
-    for iy in range(1, ny):
-        for ix in range(1, nx + 1):
-            gp.lv[ix - 1][iy - 1] = np.sort(rng.uniform(0.2, 1.0, D))[::-1]
-    return gp
 
 
 def _site_axes(psi, gp):
