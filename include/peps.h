@@ -49,7 +49,7 @@ enum {
 
 enum { PEPS_TIER_FP64 = 0, PEPS_TIER_FP32 = 1, PEPS_TIER_BF16 = 2 };
 enum { PEPS_H = 0, PEPS_V = 1 };
-enum { PEPS_STEP_V4 = 0, PEPS_STEP_GAUGE = 1, PEPS_STEP_DENSE = 2 };
+enum { PEPS_STEP_V4 = 0, PEPS_STEP_GAUGE = 1, PEPS_STEP_DENSE = 2, PEPS_STEP_FULL_V2 = 3 };
 /* peps_params.flags bits.  PEPS_STEP_RENORM has no counterpart in the
    reference: every boundary MPS built by a full-update step or an energy
    evaluation is truncated at 1e-14 times its largest singular value (the
@@ -178,7 +178,8 @@ int32_t peps_fu_bond_v4(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
 /* trotter_step_v4! (finite_peps_fullupdate_v4.jl:415), trotter_step_gauge!
    (finite_peps_gauge_invariant.jl:523) or, for PEPS_STEP_DENSE, the dense
    trotter_step! with plaquettes (finite_peps_ground_state.jl:498-541; uses
-   g, t_hop, m, tau and D_max as D_trunc) */
+   g, t_hop, m, tau and D_max as D_trunc) or, for PEPS_STEP_FULL_V2,
+   trotter_step_full! (finite_peps_full_update.jl:550-575, use_gauss=false) */
 int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p);
 /* ALS statistics of the last PEPS_STEP_V4 step: bonds, accepted, total sweeps */
 int32_t peps_last_als_stats(peps_state* st, int64_t out[3]);
@@ -195,6 +196,13 @@ int32_t peps_energy_v4(peps_state* st, const peps_params* p, double* E);
    gate (dL*dR)^2 with the right/upper site fastest, state element type */
 int32_t peps_su_bond_dense(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
                            int32_t D_trunc);
+/* full_update_bond_h! / full_update_bond_v! (finite_peps_full_update.jl:346-543):
+   the same label-free update with cut 1e-10, lambda = S/max(||S||, 1e-15)
+   and transverse weights divided out with +1e-10.  use_gauss must be 0
+   (PEPS_EUNSUPPORTED otherwise: the reference's projector path recovers dg
+   from the site dimension incorrectly; its driver never enables it). */
+int32_t peps_fu_bond_v2(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
+                        int32_t D_trunc, int32_t use_gauss);
 /* decompose_plaquette_LShape(exp(-tau H_plaquette_gauge(gd; g)), gd; cutoff)
    (u1_lgt_hamiltonian.jl:283-289,728-768; auxiliary bonds capped at 64).
    Pass GA == NULL to query Da, Db only.  Outputs real column-major
@@ -213,6 +221,14 @@ int32_t peps_expect_site(peps_state* st, int32_t ix, int32_t iy, const double* O
 int32_t peps_plaquette_energy(peps_state* st, int32_t ix, int32_t iy, double g, double* out);
 /* total_energy (finite_peps_ground_state.jl:690-805): uses g, t_hop, m */
 int32_t peps_total_energy(peps_state* st, const peps_params* p, double* out);
+
+/* bond_entropy (finite_peps_quench.jl:121-127): von Neumann entropy of
+   p_i = lambda_i^2 / sum lambda^2 (0 when the sum is below 1e-30) */
+int32_t peps_bond_entropy(const double* lambda, int64_t n, double* out);
+/* measure_all_finite (finite_peps_quench.jl:166-194) without the time
+   stamp: out = nf_mean, nf_even, nf_odd, E2_mean, S_h_mean, S_v_mean,
+   S_mean; nf_grid / E2_grid (nx x ny, ix fastest) may be NULL */
+int32_t peps_measure_all(peps_state* st, double out[7], double* nf_grid, double* E2_grid);
 
 /* The contraction GEMM itself (no counterpart in the reference; exposed so
    callers can validate the MFMA path of the current tier):

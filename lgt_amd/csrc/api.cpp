@@ -485,6 +485,8 @@ int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p) {
       peps::trotter_step_gauge(st->s, q, st->gc);
     } else if (kind == PEPS_STEP_DENSE) {
       peps::trotter_step_dense(st->s, q, st->gc);
+    } else if (kind == PEPS_STEP_FULL_V2) {
+      peps::trotter_step_full_v2(st->s, q, st->gc);
     } else {
       throw peps::Error(peps::PS_EARG, "unknown step kind");
     }
@@ -512,8 +514,22 @@ int32_t peps_su_bond_dense(peps_state* st, int32_t orient, int32_t ix, int32_t i
   return guard(st->ref.get(), [&] {
     bond_lam(st->s, orient, ix, iy);
     Tensor G = gate_from_host(st->s, orient, ix, iy, gate);
-    if (orient == PEPS_H) peps::dense_bond_h(st->s, ix, iy, G, D_trunc);
-    else peps::dense_bond_v(st->s, ix, iy, G, D_trunc);
+    if (orient == PEPS_H) peps::dense_bond_h(st->s, ix, iy, G, D_trunc, peps::SU_DENSE);
+    else peps::dense_bond_v(st->s, ix, iy, G, D_trunc, peps::SU_DENSE);
+  });
+}
+
+int32_t peps_fu_bond_v2(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
+                        int32_t D_trunc, int32_t use_gauss) {
+  if (!st || D_trunc < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    PEPS_CHECK(use_gauss == 0, peps::PS_EUNSUPPORTED,
+               "full_update_bond_h!: use_gauss=true is not supported (the reference recovers dg "
+               "from the site dimension incorrectly and projects the wrong index group)");
+    bond_lam(st->s, orient, ix, iy);
+    Tensor G = gate_from_host(st->s, orient, ix, iy, gate);
+    if (orient == PEPS_H) peps::dense_bond_h(st->s, ix, iy, G, D_trunc, peps::SU_V2);
+    else peps::dense_bond_v(st->s, ix, iy, G, D_trunc, peps::SU_V2);
   });
 }
 
@@ -572,6 +588,24 @@ int32_t peps_total_energy(peps_state* st, const peps_params* p, double* out) {
     for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
     PEPS_CHECK(p->g != 0.0, peps::PS_EARG, "total_energy: g must be non-zero");
     *out = peps::total_energy(st->s, p->g, p->t_hop, p->m);
+  });
+}
+
+int32_t peps_bond_entropy(const double* lambda, int64_t n, double* out) {
+  if ((!lambda && n > 0) || n < 0 || !out) return PEPS_EARG;
+  *out = peps::bond_entropy(std::vector<double>(lambda, lambda + n));
+  return PEPS_OK;
+}
+
+int32_t peps_measure_all(peps_state* st, double out[7], double* nf_grid, double* E2_grid) {
+  if (!st || !out) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
+    peps::Measurements m = peps::measure_all_finite(st->s);
+    const double v[7] = {m.nf_mean, m.nf_even, m.nf_odd, m.E2_mean, m.S_h_mean, m.S_v_mean, m.S_mean};
+    std::copy(v, v + 7, out);
+    if (nf_grid) std::copy(m.nf_grid.begin(), m.nf_grid.end(), nf_grid);
+    if (E2_grid) std::copy(m.E2_grid.begin(), m.E2_grid.end(), E2_grid);
   });
 }
 

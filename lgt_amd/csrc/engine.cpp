@@ -1289,9 +1289,9 @@ void su_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax) {
 // Gauss mask and no flux labels: after a dense update the bond labels are
 // undefined and the state is marked unlabelled.
 
-static std::vector<double> vpeel(const std::vector<double>& sq) {
+static std::vector<double> vpeel(const std::vector<double>& sq, double peel) {
   std::vector<double> r(sq.size());
-  for (size_t i = 0; i < sq.size(); ++i) r[i] = 1.0 / (sq[i] + SU_PEEL);
+  for (size_t i = 0; i < sq.size(); ++i) r[i] = 1.0 / (sq[i] + peel);
   return r;
 }
 
@@ -1302,17 +1302,18 @@ struct DenseSplit {
 };
 }  // namespace
 
-// SVD of the gated core, keep <= Dtrunc values above 1e-12 (at least one),
-// U·sqrt(S) and sqrt(S)·V^H (finite_peps_ground_state.jl:241-257).
-static DenseSplit dense_split(Ctx& c, const Tensor& core, int Dtrunc) {
+// SVD of the gated core, keep <= Dtrunc values above the rule's cut (at
+// least one), U·sqrt(S) and sqrt(S)·V^H, lambda = S / max(||S||, floor)
+// (finite_peps_ground_state.jl:241-257, finite_peps_full_update.jl:419-431).
+static DenseSplit dense_split(Ctx& c, const Tensor& core, int Dtrunc, const SURule& rule) {
   TierGuard fp64(c, TIER_FP64);
   SVD f = svd_thin(c, core);
   int64_t cnt = 0;
-  for (double x : f.S) cnt += x > SVD_CUT;
+  for (double x : f.S) cnt += x > rule.cut;
   const int64_t Dk = std::max<int64_t>(1, std::min<int64_t>({int64_t(Dtrunc), cnt, int64_t(f.S.size())}));
   double nS = 0;
   for (int64_t i = 0; i < Dk; ++i) nS += f.S[i] * f.S[i];
-  nS = std::sqrt(nS);
+  nS = std::max(std::sqrt(nS), rule.norm_floor);
   PEPS_CHECK(nS > 0, PS_ESOLVER, "dense simple update: the gated two-site tensor is zero");
   DenseSplit d;
   std::vector<double> sq(Dk);
@@ -1328,8 +1329,8 @@ static DenseSplit dense_split(Ctx& c, const Tensor& core, int Dtrunc) {
   return d;
 }
 
-void dense_bond_h(State& s, int ix, int iy, const Tensor& G, int Dtrunc) {
-  // update_bond_h! :175-273
+void dense_bond_h(State& s, int ix, int iy, const Tensor& G, int Dtrunc, const SURule& rule) {
+  // update_bond_h! :175-273 (full_update_bond_h!, finite_peps_full_update.jl:346-454, with SU_V2)
   Ctx& c = *s.c;
   const int nx = s.nx, ny = s.ny;
   const std::vector<double> lam = s.LH(ix, iy);
@@ -1353,14 +1354,15 @@ void dense_bond_h(State& s, int ix, int iy, const Tensor& G, int Dtrunc) {
   qr_reduce(c, Rw, {2, 3, 4, 0, 1}, 3, &QR_, &RR);  // QR (r,u,d,b)
   const int64_t a = QL.dims[3], b = QR_.dims[3];
   Tensor th = apply_gate(c, contract(c, RL, "apk", RR, "bqk", "apqb"), G);
-  DenseSplit d = dense_split(c, permute(c, th, {1, 0, 2, 3}).reshaped({dpL * a, dpR * b}), Dtrunc);
+  DenseSplit d = dense_split(c, permute(c, th, {1, 0, 2, 3}).reshaped({dpL * a, dpR * b}), Dtrunc, rule);
   const int64_t Dk = int64_t(d.lam.size());
   Tensor Ufull = contract(c, d.U.reshaped({dpL, a, Dk}), "pak", QL, "luda", "pludk");
   Tensor Vfull = contract(c, d.Vt.reshaped({Dk, dpR, b}), "kqb", QR_, "rvwb", "kqrvw");
   Tensor Lnew = permute(c, Ufull, {0, 1, 4, 2, 3});
   Tensor Rnew = permute(c, Vfull, {1, 0, 2, 3, 4});
-  std::vector<double> pL[4] = {vpeel(sq_l), vones(Dk), vpeel(sq_uL), vpeel(sq_dL)};
-  std::vector<double> pR[4] = {vones(Dk), vpeel(sq_r), vpeel(sq_uR), vpeel(sq_dR)};
+  const double pe = rule.peel;
+  std::vector<double> pL[4] = {vpeel(sq_l, pe), vones(Dk), vpeel(sq_uL, pe), vpeel(sq_dL, pe)};
+  std::vector<double> pR[4] = {vones(Dk), vpeel(sq_r, pe), vpeel(sq_uR, pe), vpeel(sq_dR, pe)};
   scale_legs(c, Lnew, pL);
   scale_legs(c, Rnew, pR);
   (void)Dl; (void)DuL; (void)DdL; (void)Dr; (void)DuR; (void)DdR;
@@ -1371,8 +1373,8 @@ void dense_bond_h(State& s, int ix, int iy, const Tensor& G, int Dtrunc) {
   s.labelled = false;
 }
 
-void dense_bond_v(State& s, int ix, int iy, const Tensor& G, int Dtrunc) {
-  // update_bond_v! :280-382
+void dense_bond_v(State& s, int ix, int iy, const Tensor& G, int Dtrunc, const SURule& rule) {
+  // update_bond_v! :280-382 (full_update_bond_v!, finite_peps_full_update.jl:462-543, with SU_V2)
   Ctx& c = *s.c;
   const int nx = s.nx, ny = s.ny;
   const std::vector<double> lam = s.LV(ix, iy);
@@ -1395,14 +1397,15 @@ void dense_bond_v(State& s, int ix, int iy, const Tensor& G, int Dtrunc) {
   qr_reduce(c, Uw, {1, 2, 3, 0, 4}, 3, &QU, &RU);  // QU (l,r,u,b)
   const int64_t a = QD.dims[3], b = QU.dims[3];
   Tensor th = apply_gate(c, contract(c, RD, "apk", RU, "bqk", "apqb"), G);
-  DenseSplit d = dense_split(c, permute(c, th, {1, 0, 2, 3}).reshaped({dpD * a, dpU * b}), Dtrunc);
+  DenseSplit d = dense_split(c, permute(c, th, {1, 0, 2, 3}).reshaped({dpD * a, dpU * b}), Dtrunc, rule);
   const int64_t Dk = int64_t(d.lam.size());
   Tensor Ufull = contract(c, d.U.reshaped({dpD, a, Dk}), "pak", QD, "lrda", "plrdk");
   Tensor Vfull = contract(c, d.Vt.reshaped({Dk, dpU, b}), "kqb", QU, "xyzb", "kqxyz");
   Tensor Dnew = permute(c, Ufull, {0, 1, 2, 4, 3});
   Tensor Unew = permute(c, Vfull, {1, 2, 3, 4, 0});
-  std::vector<double> pD[4] = {vpeel(sq_lD), vpeel(sq_rD), vones(Dk), vpeel(sq_dD)};
-  std::vector<double> pU[4] = {vpeel(sq_lU), vpeel(sq_rU), vpeel(sq_uU), vones(Dk)};
+  const double pe = rule.peel;
+  std::vector<double> pD[4] = {vpeel(sq_lD, pe), vpeel(sq_rD, pe), vones(Dk), vpeel(sq_dD, pe)};
+  std::vector<double> pU[4] = {vpeel(sq_lU, pe), vpeel(sq_rU, pe), vpeel(sq_uU, pe), vones(Dk)};
   scale_legs(c, Dnew, pD);
   scale_legs(c, Unew, pU);
   s.t(ix, iy) = Dnew;
@@ -1495,8 +1498,8 @@ void update_plaquette(State& s, int ix, int iy, const PlaquetteGate& pg, int Dtr
   s.QH(ix, iy).assign(s.LH(ix, iy).size(), 0);
   s.QV(ix, iy).assign(s.LV(ix, iy).size(), 0);
   s.labelled = false;
-  dense_bond_h(s, ix, iy, gate_tensor(c, identity(int(a[0] * b[0])), s.cplx), Dtrunc);
-  dense_bond_v(s, ix, iy, gate_tensor(c, identity(int(a[0] * cc[0])), s.cplx), Dtrunc);
+  dense_bond_h(s, ix, iy, gate_tensor(c, identity(int(a[0] * b[0])), s.cplx), Dtrunc, SU_DENSE);
+  dense_bond_v(s, ix, iy, gate_tensor(c, identity(int(a[0] * cc[0])), s.cplx), Dtrunc, SU_DENSE);
 }
 
 // ── simple-update observables (finite_peps_ground_state.jl:549-805) ─────────
@@ -1645,6 +1648,50 @@ double total_energy(State& s, double g, double t, double m) {
   return E;
 }
 
+double bond_entropy(const std::vector<double>& lam) {
+  // bond_entropy, finite_peps_quench.jl:121-127
+  double Z = 0;
+  for (double x : lam) Z += x * x;
+  if (Z < 1e-30) return 0.0;
+  double S = 0;
+  for (double x : lam) {
+    const double p = x * x / Z;
+    if (p > 1e-30) S -= p * std::log(p);
+  }
+  return S;
+}
+
+Measurements measure_all_finite(State& s) {
+  // measure_all_finite, finite_peps_quench.jl:166-194
+  const int nx = s.nx, ny = s.ny;
+  Measurements m;
+  m.nf_grid.assign(size_t(nx) * ny, 0.0);
+  m.E2_grid.assign(size_t(nx) * ny, 0.0);
+  double ev = 0, od = 0;
+  int nev = 0, nod = 0;
+  for (int iy = 1; iy <= ny; ++iy)
+    for (int ix = 1; ix <= nx; ++ix) {
+      const size_t k = size_t(ix - 1) + size_t(iy - 1) * nx;
+      m.nf_grid[k] = expect_site(s, ix, iy, site_nf(ix, iy, nx, ny, s.gd).a, false);
+      m.E2_grid[k] = expect_site(s, ix, iy, site_E2(ix, iy, nx, ny, s.gd).a, false);
+      if ((ix + iy) % 2 == 0) ev += m.nf_grid[k], ++nev;
+      else od += m.nf_grid[k], ++nod;
+    }
+  double nf = 0, e2 = 0;
+  for (size_t k = 0; k < m.nf_grid.size(); ++k) nf += m.nf_grid[k], e2 += m.E2_grid[k];
+  m.nf_mean = nf / double(nx * ny);
+  m.E2_mean = e2 / double(nx * ny);
+  m.nf_even = nev ? ev / nev : 0.0;
+  m.nf_odd = nod ? od / nod : 0.0;
+  double sh = 0, sv = 0;
+  for (auto& l : s.lh) sh += bond_entropy(l);
+  for (auto& l : s.lv) sv += bond_entropy(l);
+  m.S_h_mean = s.lh.empty() ? 0.0 : sh / double(s.lh.size());
+  m.S_v_mean = s.lv.empty() ? 0.0 : sv / double(s.lv.size());
+  m.S_mean = 0.5 * (m.S_h_mean + m.S_v_mean);
+  return m;
+}
+
 // ════════════════════════════════════════════════════════════════════════════
 //  Drivers
 // ════════════════════════════════════════════════════════════════════════════
@@ -1683,15 +1730,26 @@ void trotter_step_dense(State& s, const Params& p, GateCache& gc) {
   // v-half (T->B), h-half (R->L); the plaquette gate is split once per step
   PlaquetteGate pg = plaquette_gate(*s.c, s.gd, p.g, p.tau, SVD_CUT, 64, s.cplx);
   for (int iy = 1; iy <= s.ny; ++iy)
-    for (int ix = 1; ix < s.nx; ++ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax);
+    for (int ix = 1; ix < s.nx; ++ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, SU_DENSE);
   for (int iy = 1; iy < s.ny; ++iy)
-    for (int ix = 1; ix <= s.nx; ++ix) dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 0.5), p.Dmax);
+    for (int ix = 1; ix <= s.nx; ++ix) dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 0.5), p.Dmax, SU_DENSE);
   for (int iy = 1; iy < s.ny; ++iy)
     for (int ix = 1; ix < s.nx; ++ix) update_plaquette(s, ix, iy, pg, p.Dmax);
   for (int iy = s.ny - 1; iy >= 1; --iy)
-    for (int ix = s.nx; ix >= 1; --ix) dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 0.5), p.Dmax);
+    for (int ix = s.nx; ix >= 1; --ix) dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 0.5), p.Dmax, SU_DENSE);
   for (int iy = s.ny; iy >= 1; --iy)
-    for (int ix = s.nx - 1; ix >= 1; --ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax);
+    for (int ix = s.nx - 1; ix >= 1; --ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, SU_DENSE);
+}
+
+void trotter_step_full_v2(State& s, const Params& p, GateCache& gc) {
+  // trotter_step_full! (finite_peps_full_update.jl:550-575): h with tau/2
+  // (ix ascending), v with tau, h with tau/2 (iy ascending, ix descending)
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = 1; ix < s.nx; ++ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, SU_V2);
+  for (int iy = 1; iy < s.ny; ++iy)
+    for (int ix = 1; ix <= s.nx; ++ix) dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 1.0), p.Dmax, SU_V2);
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = s.nx - 1; ix >= 1; --ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, SU_V2);
 }
 
 // All top boundaries of the current state: tops[r] = boundary_top(r), r=1..ny.

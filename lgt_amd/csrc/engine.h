@@ -146,19 +146,36 @@ struct PlaquetteGate {
   Tensor GA, GB, GC;  // (gd,gd,gd,gd,Da,Db), (gd,gd,Da), (gd,gd,Db)
   int Da = 1, Db = 1;
 };
+// The constants that distinguish the label-free simple updates: singular
+// values kept above `cut`, lambda = S / max(||S||, norm_floor) (0: plain
+// norm), transverse weights divided out with 1 / (sqrt(lambda) + peel).
+struct SURule {
+  double cut, peel, norm_floor;
+};
+constexpr SURule SU_DENSE{SVD_CUT, SU_PEEL, 0.0};  // update_bond_h!/v!
+constexpr SURule SU_V2{FU_REG, FU_REG, 1e-15};     // full_update_bond_h!/v! (finite_peps_full_update.jl:336-454)
 // update_bond_h! / update_bond_v! (:175-382)
-void dense_bond_h(State& s, int ix, int iy, const Tensor& G, int Dtrunc);
-void dense_bond_v(State& s, int ix, int iy, const Tensor& G, int Dtrunc);
+void dense_bond_h(State& s, int ix, int iy, const Tensor& G, int Dtrunc, const SURule& rule);
+void dense_bond_v(State& s, int ix, int iy, const Tensor& G, int Dtrunc, const SURule& rule);
 // decompose_plaquette_LShape(exp(-tau H_plaquette_gauge)) (u1_lgt_hamiltonian.jl:728-768)
 PlaquetteGate plaquette_gate(Ctx& c, int gd, double g, double tau, double cutoff, int D_aux_max, bool cplx);
 // update_plaquette! (:405-483)
 void update_plaquette(State& s, int ix, int iy, const PlaquetteGate& pg, int Dtrunc);
 // trotter_step! (:498-541)
 void trotter_step_dense(State& s, const Params& p, GateCache& gc);
+// trotter_step_full! (finite_peps_full_update.jl:550-575, use_gauss = false)
+void trotter_step_full_v2(State& s, const Params& p, GateCache& gc);
 // expect_site (:549-567), O is dp x dp column-major (complex interleaved if o_cplx)
 double expect_site(State& s, int ix, int iy, const std::vector<double>& O, bool o_cplx);
 // plaquette_energy (:589-684) and total_energy (:690-805)
 double plaquette_energy(State& s, int ix, int iy, double g);
 double total_energy(State& s, double g, double t, double m);
+// bond_entropy and measure_all_finite (finite_peps_quench.jl:121-194)
+double bond_entropy(const std::vector<double>& lam);
+struct Measurements {
+  double nf_mean = 0, nf_even = 0, nf_odd = 0, E2_mean = 0, S_h_mean = 0, S_v_mean = 0, S_mean = 0;
+  std::vector<double> nf_grid, E2_grid;  // nx x ny, ix fastest
+};
+Measurements measure_all_finite(State& s);
 
 }  // namespace peps

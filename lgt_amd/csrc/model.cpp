@@ -179,6 +179,21 @@ Mat identity(int n) { return eye(n); }
 
 Mat op_U_gauge(int gd) { return op_U(gd); }
 
+Mat site_nf(int ix, int iy, int nx, int ny, int gd) {
+  // embed_f_site(op_nf(), d_gR, d_gU), u1_lgt_hamiltonian.jl:629
+  SiteDims s = site_dims(ix, iy, nx, ny, gd);
+  return kron3(op_nf(), eye(s.dR), eye(s.dU));
+}
+
+Mat site_E2(int ix, int iy, int nx, int ny, int gd) {
+  // E_R^2 + E_U^2 over the existing links (finite_peps_quench.jl:151-161)
+  SiteDims s = site_dims(ix, iy, nx, ny, gd);
+  Mat O = zeros(s.d);
+  if (ix < nx) axpy(O, 1.0, kron3(eye(2), op_E2(gd), eye(s.dU)));
+  if (iy < ny) axpy(O, 1.0, kron3(eye(2), eye(s.dR), op_E2(gd)));
+  return O;
+}
+
 // Cyclic Jacobi eigendecomposition of a real symmetric matrix (Golub & Van
 // Loan, Alg. 8.5.3, with the classical rotation formulas).
 static void jacobi_eig(Mat A, std::vector<double>* w, Mat* V) {

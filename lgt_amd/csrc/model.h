@@ -41,6 +41,9 @@ Mat H_plaquette_gauge(int gd, double g);
 Mat identity(int n);
 // Raising operator U|e> = |e+1> of one link (u1_lgt_hamiltonian.jl:123-130).
 Mat op_U_gauge(int gd);
+// n_f and E_R^2 + E_U^2 (existing links only) on one site's physical space.
+Mat site_nf(int ix, int iy, int nx, int ny, int gd);
+Mat site_E2(int ix, int iy, int nx, int ny, int gd);
 // Julia kron(A, B): the second factor is the fastest index.
 Mat kron(const Mat& A, const Mat& B);
 // exp(-tau H) for real symmetric H (Jacobi eigendecomposition, FP64).

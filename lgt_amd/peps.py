@@ -21,7 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PEPS_HIP_LIB", os.path.join(_HERE, "libpeps_hip.so"))
 
 PEPS_H, PEPS_V = 0, 1
-PEPS_STEP_V4, PEPS_STEP_GAUGE, PEPS_STEP_DENSE = 0, 1, 2
+PEPS_STEP_V4, PEPS_STEP_GAUGE, PEPS_STEP_DENSE, PEPS_STEP_FULL_V2 = 0, 1, 2, 3
 TIERS = {"fp64": 0, "fp32": 1, "bf16": 2}
 STATUS = {0: "PEPS_OK", 1: "PEPS_EARG", 2: "PEPS_ESHAPE", 3: "PEPS_ENOSECTOR", 4: "PEPS_EHIP",
           5: "PEPS_ENOMEM", 6: "PEPS_ESOLVER", 7: "PEPS_EUNSUPPORTED"}
@@ -86,6 +86,7 @@ _SIGS = {
     "peps_last_als_stats": (C.c_int32, [_vp, _i64p]),
     "peps_energy_v4": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
     "peps_su_bond_dense": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32]),
+    "peps_fu_bond_v2": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32]),
     "peps_plaquette_gate": (C.c_int32, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, _i32p, _i32p,
                                         _dp, _dp, _dp]),
     "peps_update_plaquette": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp,
@@ -93,6 +94,8 @@ _SIGS = {
     "peps_expect_site": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp, _dp]),
     "peps_plaquette_energy": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double, _dp]),
     "peps_total_energy": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
+    "peps_bond_entropy": (C.c_int32, [_dp, C.c_int64, _dp]),
+    "peps_measure_all": (C.c_int32, [_vp, _dp, _dp, _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
     "peps_eigh": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "peps_svd": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]),
@@ -390,6 +393,24 @@ def update_bond_v(gp: GaugePEPS, ix, iy, gate, D_trunc):
     gp.ctx.check(lib().peps_su_bond_dense(gp.h, PEPS_V, ix, iy, _dptr(G), D_trunc))
 
 
+def full_update_bond_h(gp: GaugePEPS, ix, iy, gate, D_trunc, use_gauss=False):
+    """full_update_bond_h! (finite_peps_full_update.jl:346); use_gauss=True is refused."""
+    G = _as_array(gate, gp.is_complex)
+    gp.ctx.check(lib().peps_fu_bond_v2(gp.h, PEPS_H, ix, iy, _dptr(G), D_trunc, int(use_gauss)))
+
+
+def full_update_bond_v(gp: GaugePEPS, ix, iy, gate, D_trunc, use_gauss=False):
+    """full_update_bond_v! (finite_peps_full_update.jl:462)."""
+    G = _as_array(gate, gp.is_complex)
+    gp.ctx.check(lib().peps_fu_bond_v2(gp.h, PEPS_V, ix, iy, _dptr(G), D_trunc, int(use_gauss)))
+
+
+def trotter_step_full(gp: GaugePEPS, g, t_hop, m, tau, D_trunc):
+    """trotter_step_full! (finite_peps_full_update.jl:550), use_gauss=false."""
+    p = _params(g, t_hop, m, tau, D_trunc, 1, 0)
+    gp.ctx.check(lib().peps_trotter_step(gp.h, PEPS_STEP_FULL_V2, C.byref(p)))
+
+
 def plaquette_gate(gd, g, tau, cutoff=1e-12, ctx=None):
     """decompose_plaquette_LShape(exp(-tau H_plaquette_gauge)) (u1_lgt_hamiltonian.jl:728)
     -> (GA, GB, GC)."""
@@ -439,6 +460,28 @@ def total_energy(gp: GaugePEPS, g, t_hop, m):
     v = C.c_double()
     gp.ctx.check(lib().peps_total_energy(gp.h, C.byref(p), C.byref(v)))
     return v.value
+
+
+# ── finite_peps_quench.jl (measurements) ────────────────────────────────────
+
+def bond_entropy(lam) -> float:
+    """bond_entropy (finite_peps_quench.jl:121)."""
+    a = np.ascontiguousarray(lam, dtype=np.float64)
+    v = C.c_double()
+    rc = lib().peps_bond_entropy(_dptr(a), len(a), C.byref(v))
+    if rc:
+        raise PepsError(rc, "bond_entropy")
+    return v.value
+
+
+def measure_all_finite(gp: GaugePEPS, t=0.0) -> dict:
+    """measure_all_finite (finite_peps_quench.jl:166)."""
+    out = np.zeros(7)
+    nf = np.zeros((gp.nx, gp.ny), order="F")
+    e2 = np.zeros((gp.nx, gp.ny), order="F")
+    gp.ctx.check(lib().peps_measure_all(gp.h, _dptr(out), _dptr(nf), _dptr(e2)))
+    keys = ("nf_mean", "nf_even", "nf_odd", "E2_mean", "S_h_mean", "S_v_mean", "S_mean")
+    return dict(t=t, **dict(zip(keys, out.tolist())), nf_grid=nf, E2_grid=e2)
 
 
 # ── finite_peps_full_update.jl / finite_peps_boundary_mps.jl ────────────────

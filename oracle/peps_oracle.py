@@ -1147,19 +1147,21 @@ def _scale5(T, wl, wr, wu, wd):
         * wu[None, None, None, :, None] * wd[None, None, None, None, :]
 
 
-def _su_split(Thn, nrow, ncol, D_trunc):
-    """SVD split of the gated two-site tensor, finite_peps_ground_state.jl:241-246."""
+def _su_split(Thn, nrow, ncol, D_trunc, cut=SVD_CUT, norm_floor=0.0):
+    """SVD split of the gated two-site tensor, finite_peps_ground_state.jl:241-246
+    (finite_peps_full_update.jl:419-424 with cut 1e-10 and norm floor 1e-15)."""
     U, S, Vt = np.linalg.svd(jr(Thn, nrow, ncol), full_matrices=False)
-    Dk = max(1, min(D_trunc, int(np.sum(S > SVD_CUT)), len(S)))
+    Dk = max(1, min(D_trunc, int(np.sum(S > cut)), len(S)))
     St = S[:Dk]
-    return U[:, :Dk], St, Vt[:Dk], St / np.linalg.norm(St), np.sqrt(St)
+    return U[:, :Dk], St, Vt[:Dk], St / max(np.linalg.norm(St), norm_floor), np.sqrt(St)
 
 
-def update_bond_h(gp, ix, iy, G, D_trunc):
+def update_bond_h(gp, ix, iy, G, D_trunc, cut=SVD_CUT, peel=SU_PEEL, norm_floor=0.0):
     """update_bond_h!, finite_peps_ground_state.jl:175-273: absorb sqrt(lambda)
     on every leg, Theta = L.R, gate on (p_R fast, p_L slow), SVD, keep
     <= D_trunc values above 1e-12, L = U sqrt(S), R = sqrt(S) V^H,
-    lambda = S/||S||, divide the transverse weights out with +1e-15."""
+    lambda = S/||S||, divide the transverse weights out with +1e-15.
+    (cut, peel, norm_floor) = (1e-10, 1e-10, 1e-15) is full_update_bond_h!."""
     nx, ny = gp.nx, gp.ny
     TL, TR = gp.T(ix, iy), gp.T(ix + 1, iy)
     dpL, dpR = TL.shape[0], TR.shape[0]
@@ -1182,19 +1184,20 @@ def update_bond_h(gp, ix, iy, G, D_trunc):
     Th = jr(Lm @ Rm, dpL, Dl, DuL, DdL, dpR, Dr, DuR, DdR)
     Thpp = jr(pd(Th, 5, 1, 2, 3, 4, 6, 7, 8), dpR * dpL, Dl * DuL * DdL * Dr * DuR * DdR)
     Thn = pd(jr(G @ Thpp, dpR, dpL, Dl, DuL, DdL, Dr, DuR, DdR), 2, 3, 4, 5, 1, 6, 7, 8)
-    U, St, Vt, lam_new, sqS = _su_split(Thn, dpL * Dl * DuL * DdL, dpR * Dr * DuR * DdR, D_trunc)
+    U, St, Vt, lam_new, sqS = _su_split(Thn, dpL * Dl * DuL * DdL, dpR * Dr * DuR * DdR, D_trunc, cut, norm_floor)
     Dk = len(St)
     Lnew = pd(jr(U * sqS[None, :], dpL, Dl, DuL, DdL, Dk), 1, 2, 5, 3, 4)
     Rnew = pd(jr(sqS[:, None] * Vt, Dk, dpR, Dr, DuR, DdR), 2, 1, 3, 4, 5)
-    inv = lambda x: 1.0 / (x + SU_PEEL)
+    inv = lambda x: 1.0 / (x + peel)
     gp.setT(ix, iy, _scale5(Lnew, inv(sq_l), np.ones(Dk), inv(sq_uL), inv(sq_dL)))
     gp.setT(ix + 1, iy, _scale5(Rnew, np.ones(Dk), inv(sq_r), inv(sq_uR), inv(sq_dR)))
     gp.lh[ix - 1][iy - 1] = lam_new
 
 
-def update_bond_v(gp, ix, iy, G, D_trunc):
+def update_bond_v(gp, ix, iy, G, D_trunc, cut=SVD_CUT, peel=SU_PEEL, norm_floor=0.0):
     """update_bond_v!, finite_peps_ground_state.jl:280-382 (lower D=(ix,iy),
-    upper U=(ix,iy+1); gate on (p_U fast, p_D slow))."""
+    upper U=(ix,iy+1); gate on (p_U fast, p_D slow)).  (cut, peel,
+    norm_floor) = (1e-10, 1e-10, 1e-15) is full_update_bond_v!."""
     nx, ny = gp.nx, gp.ny
     TD, TU = gp.T(ix, iy), gp.T(ix, iy + 1)
     dpD, dpU = TD.shape[0], TU.shape[0]
@@ -1217,11 +1220,11 @@ def update_bond_v(gp, ix, iy, G, D_trunc):
     Th = jr(Dm @ Um, dpD, DlD, DrD, DdD, dpU, DlU, DrU, DuU)
     Thpp = jr(pd(Th, 5, 1, 2, 3, 4, 6, 7, 8), dpU * dpD, DlD * DrD * DdD * DlU * DrU * DuU)
     Thn = pd(jr(G @ Thpp, dpU, dpD, DlD, DrD, DdD, DlU, DrU, DuU), 2, 3, 4, 5, 1, 6, 7, 8)
-    U, St, Vt, lam_new, sqS = _su_split(Thn, dpD * DlD * DrD * DdD, dpU * DlU * DrU * DuU, D_trunc)
+    U, St, Vt, lam_new, sqS = _su_split(Thn, dpD * DlD * DrD * DdD, dpU * DlU * DrU * DuU, D_trunc, cut, norm_floor)
     Dk = len(St)
     Dnew = pd(jr(U * sqS[None, :], dpD, DlD, DrD, DdD, Dk), 1, 2, 3, 5, 4)
     Unew = pd(jr(sqS[:, None] * Vt, Dk, dpU, DlU, DrU, DuU), 2, 3, 4, 5, 1)
-    inv = lambda x: 1.0 / (x + SU_PEEL)
+    inv = lambda x: 1.0 / (x + peel)
     gp.setT(ix, iy, _scale5(Dnew, inv(sq_lD), inv(sq_rD), np.ones(Dk), inv(sq_dD)))
     gp.setT(ix, iy + 1, _scale5(Unew, inv(sq_lU), inv(sq_rU), inv(sq_uU), np.ones(Dk)))
     gp.lv[ix - 1][iy - 1] = lam_new
@@ -1327,6 +1330,36 @@ def trotter_step_dense(gp, g, t_hop, m, tau, D_trunc):
             update_bond_h(gp, ix, iy, gh(ix, iy), D_trunc)
 
 
+def full_update_bond_h_v2(gp, ix, iy, G, D_trunc):
+    """full_update_bond_h!, finite_peps_full_update.jl:346-454 with
+    use_gauss=false (its driver's setting): the environment-weighted branch
+    was removed upstream, so this is the simple update with cut 1e-10,
+    lambda = S/max(||S||, 1e-15) and transverse weights peeled with +1e-10."""
+    update_bond_h(gp, ix, iy, G, D_trunc, cut=FU_REG, peel=FU_REG, norm_floor=1e-15)
+
+
+def full_update_bond_v_v2(gp, ix, iy, G, D_trunc):
+    """full_update_bond_v!, finite_peps_full_update.jl:462-543."""
+    update_bond_v(gp, ix, iy, G, D_trunc, cut=FU_REG, peel=FU_REG, norm_floor=1e-15)
+
+
+def trotter_step_full(gp, g, t_hop, m, tau, D_trunc):
+    """trotter_step_full!, finite_peps_full_update.jl:550-575: h with tau/2
+    (iy ascending, ix ascending), v with tau, h with tau/2 (iy ascending, ix
+    descending)."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    gh = lambda ix, iy: gate(H_merged_h_site(ix, iy, nx, ny, gd, g, t_hop, m), tau / 2)
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx):
+            full_update_bond_h_v2(gp, ix, iy, gh(ix, iy), D_trunc)
+    for iy in range(1, ny):
+        for ix in range(1, nx + 1):
+            full_update_bond_v_v2(gp, ix, iy, gate(H_merged_v_site(ix, iy, nx, ny, gd, g, t_hop, m), tau), D_trunc)
+    for iy in range(1, ny + 1):
+        for ix in range(nx - 1, 0, -1):
+            full_update_bond_h_v2(gp, ix, iy, gh(ix, iy), D_trunc)
+
+
 def expect_site(gp, ix, iy, O):
     """expect_site, finite_peps_ground_state.jl:549-567: lambda (not its root) on
     every leg, <O> = tr(T^H O T) / tr(T^H T), 0 when the norm is <= 1e-15."""
@@ -1417,6 +1450,48 @@ def total_energy(gp, g, t_hop, m):
         for ix in range(1, nx):
             E += plaquette_energy(gp, ix, iy, g)
     return E
+
+
+def measure_nf_grid(gp):
+    """measure_nf_grid, finite_peps_quench.jl:135-142: (nx, ny) grid of <n_f>."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    grid = np.zeros((nx, ny))
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            _, dR, dU = site_dims(ix, iy, nx, ny, gd)
+            grid[ix - 1, iy - 1] = expect_site(gp, ix, iy, np.kron(np.kron(op_nf(), _I(dR)), _I(dU)))
+    return grid
+
+
+def measure_E2_grid(gp):
+    """measure_E2_grid, finite_peps_quench.jl:149-163: <E_R^2 + E_U^2>, existing links only."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    grid = np.zeros((nx, ny))
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d, dR, dU = site_dims(ix, iy, nx, ny, gd)
+            Op = np.zeros((d, d), dtype=np.complex128)
+            if ix < nx:
+                Op += np.kron(np.kron(_I(2), op_E2(gd)), _I(dU))
+            if iy < ny:
+                Op += np.kron(np.kron(_I(2), _I(dR)), op_E2(gd))
+            grid[ix - 1, iy - 1] = expect_site(gp, ix, iy, Op)
+    return grid
+
+
+def measure_all_finite(gp, t=0.0):
+    """measure_all_finite, finite_peps_quench.jl:166-194."""
+    nx, ny = gp.nx, gp.ny
+    nf, E2 = measure_nf_grid(gp), measure_E2_grid(gp)
+    ev = [nf[ix - 1, iy - 1] for iy in range(1, ny + 1) for ix in range(1, nx + 1) if (ix + iy) % 2 == 0]
+    od = [nf[ix - 1, iy - 1] for iy in range(1, ny + 1) for ix in range(1, nx + 1) if (ix + iy) % 2 == 1]
+    Sh = [bond_entropy(gp.LH(ix, iy)) for iy in range(1, ny + 1) for ix in range(1, nx)]
+    Sv = [bond_entropy(gp.LV(ix, iy)) for iy in range(1, ny) for ix in range(1, nx + 1)]
+    Sh_m = float(np.mean(Sh)) if Sh else 0.0
+    Sv_m = float(np.mean(Sv)) if Sv else 0.0
+    return dict(t=t, nf_mean=float(np.mean(nf)), nf_even=float(np.mean(ev)) if ev else 0.0,
+                nf_odd=float(np.mean(od)) if od else 0.0, E2_mean=float(np.mean(E2)), S_h_mean=Sh_m,
+                S_v_mean=Sv_m, S_mean=(Sh_m + Sv_m) / 2, nf_grid=nf, E2_grid=E2)
 
 
 # ════════════════════════════════════════════════════════════════════════════

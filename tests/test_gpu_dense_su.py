@@ -126,3 +126,47 @@ def test_dense_update_drops_flux_labels(ctx):
     assert e.value.code == 1
     with pytest.raises(peps.PepsError):
         peps.trotter_step_gauge(gdv, G, T_HOP, M, 0.05, 2)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_full_update_v2_bonds_and_step(ctx, cplx):
+    """full_update_bond_h!/v! (finite_peps_full_update.jl): bond by bond, then
+    two trotter_step_full! steps."""
+    from lgt_amd import peps
+    nx, ny, gd = (3, 3, 2) if not cplx else (2, 3, 3)
+    print(f"v2 full update {nx}x{ny} gd={gd} cplx={cplx}", flush=True)
+    go = random_peps(nx, ny, gd, 2, 19, cplx)
+    go.lh[0][0] = np.array([1.0, 3e-11])  # below the 1e-10 cut after the update
+    gdv = to_device(go, ctx)
+    rng = np.random.default_rng(4)
+    for o, ix, iy in [("h", 1, 1), ("v", 1, 1), ("h", nx - 1, ny), ("v", nx, ny - 1)]:
+        Hb = (O.H_merged_h_site if o == "h" else O.H_merged_v_site)(ix, iy, nx, ny, gd, G, T_HOP, M)
+        Gt = O.gate(Hb, 0.05)
+        if cplx:
+            Gt = Gt + 0.1j * rng.standard_normal(Gt.shape)
+        (O.full_update_bond_h_v2 if o == "h" else O.full_update_bond_v_v2)(go, ix, iy, Gt, 3)
+        (peps.full_update_bond_h if o == "h" else peps.full_update_bond_v)(gdv, ix, iy, Gt, 3)
+    _compare(gdv, go, 1e-10, 1e-12)
+    for _ in range(2):
+        O.trotter_step_full(go, G, T_HOP, M, 0.05, 3)
+        peps.trotter_step_full(gdv, G, T_HOP, M, 0.05, 3)
+    _compare(gdv, go, 1e-9, 1e-10)
+    with pytest.raises(peps.PepsError) as e:
+        peps.full_update_bond_h(gdv, 1, 1, np.eye(O.site_dims(1, 1, nx, ny, gd)[0] * O.site_dims(2, 1, nx, ny, gd)[0]),
+                                3, use_gauss=True)
+    assert e.value.code == 7
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_measure_all_finite(ctx, cplx):
+    from lgt_amd import peps
+    nx, ny, gd = (3, 3, 2) if not cplx else (2, 3, 3)
+    go = random_peps(nx, ny, gd, 3, 51, cplx)
+    gdv = to_device(go, ctx)
+    mo, mg = O.measure_all_finite(go, 0.25), peps.measure_all_finite(gdv, 0.25)
+    for k in ("nf_mean", "nf_even", "nf_odd", "E2_mean", "S_h_mean", "S_v_mean", "S_mean"):
+        assert abs(mo[k] - mg[k]) < 1e-12 * max(1.0, abs(mo[k])), k
+    np.testing.assert_allclose(mg["nf_grid"], mo["nf_grid"], atol=1e-12)
+    np.testing.assert_allclose(mg["E2_grid"], mo["E2_grid"], atol=1e-12)
+    lam = go.LH(1, 1)
+    assert abs(peps.bond_entropy(lam) - O.bond_entropy(lam)) < 1e-14

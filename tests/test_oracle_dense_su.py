@@ -205,3 +205,77 @@ def test_expect_site_weights_bonds_by_lambda():
     Uk, Sk = U[:, :k], S[:k]
     expect = np.trace(Op @ (Uk * Sk ** 3) @ Uk.conj().T).real / np.sum(Sk ** 3)
     assert abs(O.expect_site(gp, 1, 1, Op) - expect) < 1e-12 * max(1, abs(expect))
+
+
+@pytest.mark.parametrize("variant", ["dense", "v2"])
+def test_su_variants_cut_and_peel(variant):
+    """Two-site closed form with a second singular value between 1e-12 and
+    1e-10: update_bond_h! (cut 1e-12) keeps it, full_update_bond_h! (cut
+    1e-10) drops it; both give U_k S_k^2 V_k^H / ||S_k||, times the peel of
+    the six missing legs (weight 1 divided by 1 + 1e-15, resp. 1 + 1e-10)."""
+    gp = _random_peps(2, 1, 3, 2, 13)
+    gp.lh[0][0] = np.array([1.0, 4e-11])
+    dims = O.site_phys_dims(gp)
+    Mg = O.dense_psi(gp).reshape(dims[0], dims[1], order="F")
+    U, S, Vh = np.linalg.svd(Mg)
+    assert len(S) == 2 and 1e-12 < S[1] < 1e-10
+    k, peel = (2, 1e-15) if variant == "dense" else (1, 1e-10)
+    expect = (U[:, :k] * (S[:k] ** 2 / np.linalg.norm(S[:k]))) @ Vh[:k] / (1 + peel) ** 6
+    Id = np.eye(dims[0] * dims[1])
+    (O.update_bond_h if variant == "dense" else O.full_update_bond_h_v2)(gp, 1, 1, Id, 4)
+    got = O.dense_psi(gp).reshape(dims[0], dims[1], order="F")
+    assert len(gp.LH(1, 1)) == k
+    assert np.abs(got - expect).max() < 1e-12 * np.abs(expect).max()
+
+
+def test_v2_peel_regulariser():
+    """full_update_bond_v! divides the transverse weights out with
+    1/(sqrt(lambda) + 1e-10): on a 2x2 lattice with the identity gate, the
+    left legs of the updated pair carry sqrt(l)/(sqrt(l) + 1e-10) relative to
+    the dense update's sqrt(l)/(sqrt(l) + 1e-15)."""
+    ga = _random_peps(2, 2, 2, 2, 6)
+    ga.lh[0][0] = np.array([1.0, 1e-16])
+    gb = ga.copy()
+    Id = np.eye(4 * 8)
+    O.update_bond_v(ga, 1, 1, Id, 8)
+    O.full_update_bond_v_v2(gb, 1, 1, Id, 8)
+    sq = np.sqrt(ga.LH(1, 1))
+    # the right leg (r) of the lower site (1,1) carries lambda_h(1,1)
+    ratio = (sq + 1e-15) / (sq + 1e-10)
+    Ta, Tb = ga.T(1, 1), gb.T(1, 1)
+    for r in range(2):
+        na, nb = np.linalg.norm(Ta[:, :, r]), np.linalg.norm(Tb[:, :, r])
+        assert abs(nb / na - ratio[r]) < 1e-9 * ratio[r]
+
+
+def test_bond_entropy_closed_forms():
+    assert O.bond_entropy([1.0]) == 0.0
+    assert abs(O.bond_entropy(np.ones(5) * 0.3) - np.log(5)) < 1e-14
+    p = np.array([0.7, 0.2, 0.1])
+    assert abs(O.bond_entropy(np.sqrt(p) * 3.0) + np.sum(p * np.log(p))) < 1e-14
+    assert O.bond_entropy([1e-20, 0.0]) == 0.0
+
+
+def test_measurements_on_basis_product_states():
+    """A D=1 product of Gauss-law basis states: <n_f> and <E_R^2 + E_U^2> are
+    the labels themselves, and every bond entropy is 0."""
+    nx, ny, gd = 3, 2, 3
+    gp = _random_peps(nx, ny, gd, 1, 2)
+    rng = np.random.default_rng(1)
+    nf_exp, e2_exp = np.zeros((nx, ny)), np.zeros((nx, ny))
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d, dR, dU = O.site_dims(ix, iy, nx, ny, gd)
+            p = int(rng.integers(1, d + 1))
+            T = np.zeros((d, 1, 1, 1, 1), dtype=np.complex128)
+            T[p - 1] = 1.0
+            gp.setT(ix, iy, T)
+            nf, lR, lU = O.decode_phys(p, dR, dU, gd)
+            nf_exp[ix - 1, iy - 1] = nf
+            e2_exp[ix - 1, iy - 1] = (lR / 2) ** 2 * (ix < nx) + (lU / 2) ** 2 * (iy < ny)
+    m = O.measure_all_finite(gp, 0.5)
+    np.testing.assert_allclose(m["nf_grid"], nf_exp, atol=1e-14)
+    np.testing.assert_allclose(m["E2_grid"], e2_exp, atol=1e-14)
+    assert abs(m["nf_mean"] - nf_exp.mean()) < 1e-14 and m["S_mean"] == 0.0 and m["t"] == 0.5
+    even = [nf_exp[ix - 1, iy - 1] for iy in range(1, ny + 1) for ix in range(1, nx + 1) if (ix + iy) % 2 == 0]
+    assert abs(m["nf_even"] - np.mean(even)) < 1e-14
