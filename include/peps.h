@@ -238,6 +238,12 @@ int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64
                   const double* A, int32_t a_kfast, int32_t conjA, const double* B,
                   int32_t b_kfast, int32_t conjB, double* Cout);
 
+/* The same GEMM accumulating into C: C (M x N) = op(A) op(B) + beta C (the
+   form the chunked contractions use). */
+int32_t peps_gemm_acc(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,
+                      const double* A, int32_t a_kfast, int32_t conjA, const double* B,
+                      int32_t b_kfast, int32_t conjB, double beta, double* Cinout);
+
 /* Hermitian eigendecomposition used by the ALS solve (_solve_herm,
    finite_peps_fullupdate_v4.jl:37-45): Householder tridiagonalisation and
    divide and conquer (rocSOLVER syevd/heevd), one matrix after the other.

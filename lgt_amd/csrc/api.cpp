@@ -624,6 +624,21 @@ int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64
   });
 }
 
+int32_t peps_gemm_acc(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,
+                      const double* A, int32_t a_kfast, int32_t conjA, const double* B,
+                      int32_t b_kfast, int32_t conjB, double beta, double* Cinout) {
+  if (!ctx || !A || !B || !Cinout || M < 0 || N < 0 || K < 0) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    bool cp = is_complex != 0;
+    Tensor a = ctx->c->upload({std::max<int64_t>(1, M * K)}, cp, A);
+    Tensor b = ctx->c->upload({std::max<int64_t>(1, K * N)}, cp, B);
+    Tensor c = ctx->c->upload({std::max<int64_t>(1, M * N)}, cp, Cinout);
+    peps::gemm(*ctx->c, cp, M, N, K, a.data(), a_kfast != 0, conjA != 0, b.data(), b_kfast != 0,
+               conjB != 0, c.data(), beta);
+    ctx->c->download(c.reshaped({M * N}), Cinout);
+  });
+}
+
 int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, const double* A,
                   double* V, double* w) {
   if (!ctx || !A || !V || !w || n < 1 || nb < 1) return PEPS_EARG;

@@ -267,6 +267,152 @@ __global__ void splitk_reduce(double* __restrict__ C, const double* __restrict__
   }
 }
 
+// Real GEMM of the FP32 tier for large outputs (M, N >= 256).
+//
+// 128x128 output tile per 256-thread block (2x2 waves of 64x64, 4x4 MFMA
+// f32 16x16x4 each), k-tiles of 32 staged FP64 -> FP32 through two LDS
+// buffers (one barrier per k-tile; the next tile's global loads are in
+// flight during the MFMAs).  LDS rows hold 32 consecutive k of one m (or n):
+// lane group g supplies k = 8g + s in MFMA step s, so a lane reads its eight
+// operands of a row as two float4.  Tiles are visited in groups of 8 tile
+// rows (L2 reuse of A and B panels) and the block index is remapped so that
+// the blocks the hardware sends to one XCD (round robin over 8) are
+// consecutive in that order.  The epilogue transposes the accumulators
+// through LDS so that C is read and written in whole column segments.
+template <bool AK, bool BKF>
+__global__ __launch_bounds__(256) void gemm_f32_tiled(int64_t M, int64_t N, int64_t K,
+                                                      const double* __restrict__ A, int64_t lda,
+                                                      const double* __restrict__ B, int64_t ldb,
+                                                      double* __restrict__ C, int64_t ldc, double beta,
+                                                      int64_t kchunk, int64_t wstride, int tilesM,
+                                                      int tilesN) {
+  constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 4, LDC = BM + 1;
+  constexpr int BUF = (BM + BN) * LDK;
+  static_assert(BN * LDC <= 2 * BUF, "epilogue staging must fit the operand buffers");
+  __shared__ __align__(16) float smem[2 * BUF];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+
+  // tile of this block: XCD-contiguous remap, then groups of 8 tile rows
+  const int nt = tilesM * tilesN;
+  int w = blockIdx.x;
+  {
+    const int q = nt >> 3, r = nt & 7, x = w & 7, i = w >> 3;
+    w = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+  }
+  constexpr int GM = 8;
+  const int per_group = GM * tilesN;
+  const int first = (w / per_group) * GM;
+  const int gs = min(tilesM - first, GM);
+  const int tm = first + (w % per_group) % gs;
+  const int tn = (w % per_group) / gs;
+  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+
+  const int64_t kbeg = int64_t(blockIdx.z) * kchunk;
+  const int64_t kend = min(K, kbeg + kchunk);
+  double* Cz = C + int64_t(blockIdx.z) * wstride;
+
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  double ra[16], rb[16];
+  // element e of this thread in a 128 x 32 operand tile: (row, k)
+  auto arow = [&](int e) { return AK ? (tid >> 5) + 8 * e : (tid & 127); };
+  auto acol = [&](int e) { return AK ? (tid & 31) : (tid >> 7) + 2 * e; };
+  auto brow = [&](int e) { return BKF ? (tid >> 5) + 8 * e : (tid & 127); };
+  auto bcol = [&](int e) { return BKF ? (tid & 31) : (tid >> 7) + 2 * e; };
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int64_t gm = m0 + arow(e), gk = k0 + acol(e);
+      ra[e] = (gm < M && gk < kend) ? A[AK ? gk + gm * lda : gm + gk * lda] : 0.0;
+      const int64_t gn = n0 + brow(e), gkb = k0 + bcol(e);
+      rb[e] = (gn < N && gkb < kend) ? B[BKF ? gkb + gn * ldb : gn + gkb * ldb] : 0.0;
+    }
+  };
+  auto lstore = [&](int buf) {
+    float* As = smem + buf * BUF;
+    float* Bs = As + BM * LDK;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      As[arow(e) * LDK + acol(e)] = float(ra[e]);
+      Bs[brow(e) * LDK + bcol(e)] = float(rb[e]);
+    }
+  };
+
+  const int64_t ntk = (kend - kbeg + BK - 1) / BK;
+  if (ntk > 0) {
+    gload(kbeg);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int64_t t = 0; t < ntk; ++t) {
+    const bool more = t + 1 < ntk;
+    if (more) gload(kbeg + (t + 1) * BK);
+    const float* As = smem + (t & 1) * BUF;
+    const float* Bs = As + BM * LDK;
+    v4f a[4][2], b[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float* pa = As + (wm + i * 16 + lr) * LDK + 8 * lg;
+      a[i][0] = *reinterpret_cast<const v4f*>(pa);
+      a[i][1] = *reinterpret_cast<const v4f*>(pa + 4);
+      const float* pb = Bs + (wn + i * 16 + lr) * LDK + 8 * lg;
+      b[i][0] = *reinterpret_cast<const v4f*>(pb);
+      b[i][1] = *reinterpret_cast<const v4f*>(pb + 4);
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s >> 2][s & 3], b[j][s >> 2][s & 3], acc[i][j],
+                                                             0, 0, 0);
+    if (more) lstore((t + 1) & 1);
+    __syncthreads();
+  }
+
+  // epilogue: accumulators -> LDS (column-major 128 x 128, padded) -> C
+  float* Cs = smem;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Cs[(wn + j * 16 + lr) * LDC + wm + i * 16 + 4 * lg + v] = acc[i][j][v];
+  __syncthreads();
+  for (int idx = tid; idx < BM * BN; idx += 256) {
+    const int mm = idx & (BM - 1), nn = idx / BM;
+    const int64_t gm = m0 + mm, gn = n0 + nn;
+    if (gm >= M || gn >= N) continue;
+    const int64_t off = gm + gn * ldc;
+    double x = double(Cs[nn * LDC + mm]);
+    if (beta != 0.0) x += beta * Cz[off];
+    Cz[off] = x;
+  }
+}
+
+static void launch_gemm_f32_tiled(bool ak, bool bk, hipStream_t st, int64_t M, int64_t N, int64_t K,
+                                  const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                                  int64_t ldc, double beta, int splits, int64_t kchunk, int64_t wstride) {
+  const int tm = int(cdiv(M, 128)), tn = int(cdiv(N, 128));
+  dim3 grid(unsigned(tm) * unsigned(tn), 1, splits);
+  if (ak && bk)
+    hipLaunchKernelGGL((gemm_f32_tiled<true, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+  else if (ak)
+    hipLaunchKernelGGL((gemm_f32_tiled<true, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+  else if (bk)
+    hipLaunchKernelGGL((gemm_f32_tiled<false, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+  else
+    hipLaunchKernelGGL((gemm_f32_tiled<false, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+}
+
 template <bool CPLX, int PREC, bool AK, bool BKF, int WM, int WN>
 static void launch_gemm_t(hipStream_t st, int64_t M, int64_t N, int64_t K, const double* A,
                           int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
@@ -340,7 +486,10 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
   }
   const double flops = (cplx ? 8.0 : 2.0) * double(M) * double(N) * double(K);
   const double bytes = double(esz * 8) * (double(M) * K + double(K) * N + double(M) * N);
-  const char* cls = c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16");
+  std::string cls = c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16");
+  if (c.gemm_shapes)
+    cls += " " + std::to_string(M) + "x" + std::to_string(N) + "x" + std::to_string(K) + (cplx ? "c" : "") +
+           (a_kfast ? " ak" : " am") + (b_kfast ? " bk" : " bn") + " s" + std::to_string(splits);
   c.count(cls, flops, bytes);
   hipEvent_t ev = nullptr;
   c.tbegin(cls, &ev);
@@ -357,6 +506,8 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
     out = static_cast<double*>(ws->ptr);
     kbeta = 0.0;
   }
+  const bool tiled = !cplx && c.tier == TIER_FP32 && M >= 256 && N >= 256 &&
+                     int64_t(cdiv(M, 128)) * cdiv(N, 128) < (int64_t(1) << 31);
   auto go = [&](auto cplx_tag, auto prec_tag) {
     constexpr bool CP = decltype(cplx_tag)::value;
     constexpr int PR = decltype(prec_tag)::value;
@@ -366,7 +517,10 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
   using T0 = std::integral_constant<int, 0>;
   using T1 = std::integral_constant<int, 1>;
   using T2 = std::integral_constant<int, 2>;
-  if (cplx) {
+  if (tiled) {
+    launch_gemm_f32_tiled(a_kfast, b_kfast, c.stream, M, N, K, A, lda, B, ldb, out, M, kbeta, splits, kchunk,
+                          wstride);
+  } else if (cplx) {
     if (c.tier == TIER_FP64) go(std::true_type{}, T0{});
     else if (c.tier == TIER_FP32) go(std::true_type{}, T1{});
     else go(std::true_type{}, T2{});

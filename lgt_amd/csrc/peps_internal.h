@@ -142,6 +142,9 @@ class Ctx {
   // Hermitian top-k eigenproblems up to this size are solved directly, larger
   // ones by subspace iteration (environment variable PEPS_HIP_EIG_DIRECT_MAX)
   int eig_direct_max = 512;
+  // account GEMMs per exact shape ("gemm_f32 MxNxK ...") instead of per
+  // precision (environment variable PEPS_HIP_GEMM_SHAPES=1)
+  bool gemm_shapes = false;
   // last Ritz basis of each boundary-MPS column position: the warm start of
   // the next subspace iteration at the same position
   std::map<std::string, Tensor> ritz_cache;

@@ -97,6 +97,8 @@ _SIGS = {
     "peps_bond_entropy": (C.c_int32, [_dp, C.c_int64, _dp]),
     "peps_measure_all": (C.c_int32, [_vp, _dp, _dp, _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
+    "peps_gemm_acc": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp,
+                                  C.c_int32, C.c_int32, C.c_double, _dp]),
     "peps_eigh": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "peps_svd": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]),
     "peps_bond_operator": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -649,6 +651,21 @@ def gemm(A, B, a_kfast=False, conjA=False, b_kfast=True, conjB=False, ctx=None):
     out = np.empty((M, N), dtype=a.dtype, order="F")
     ctx.check(lib().peps_gemm(ctx.h, int(cp), M, N, K, _dptr(a), int(a_kfast), int(conjA), _dptr(b),
                               int(b_kfast), int(conjB), _dptr(out)))
+    return out
+
+
+def gemm_acc(A, B, Cin, beta, a_kfast=False, conjA=False, b_kfast=True, conjB=False, ctx=None):
+    """op(A) @ op(B) + beta * Cin through the engine's GEMM (layouts as gemm)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(A) or np.iscomplexobj(B) or np.iscomplexobj(Cin)
+    a = _as_array(A, cp)
+    b = _as_array(B, cp)
+    M, K = (a.shape[1], a.shape[0]) if a_kfast else a.shape
+    K2, N = b.shape if b_kfast else (b.shape[1], b.shape[0])
+    assert K == K2 and Cin.shape == (M, N)
+    out = _as_array(Cin, cp).copy(order="F")
+    ctx.check(lib().peps_gemm_acc(ctx.h, int(cp), M, N, K, _dptr(a), int(a_kfast), int(conjA), _dptr(b),
+                                  int(b_kfast), int(conjB), beta, _dptr(out)))
     return out
 
 

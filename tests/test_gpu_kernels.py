@@ -5,7 +5,10 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(7, 5, 3), (64, 64, 16), (130, 129, 40), (200, 144, 37), (48, 40, 5000), (16, 300, 2048)]
+SHAPES = [(7, 5, 3), (64, 64, 16), (130, 129, 40), (200, 144, 37), (48, 40, 5000), (16, 300, 2048),
+          # large real outputs (the tiled FP32 kernel): ragged edges, K not a
+          # multiple of the k-tile, split-K, more tile rows than one group
+          (300, 260, 70), (257, 1000, 33), (512, 384, 4096), (1100, 300, 5)]
 TOL = {"fp64": 1e-13, "fp32": 3e-6, "bf16": 2e-2}
 
 
@@ -34,6 +37,30 @@ def test_gemm_layouts(ctx, tier, cplx):
                         scale = np.sqrt(K) * np.abs(A).max() * np.abs(B).max()
                         err = np.abs(out - ref).max() / scale
                         assert err < TOL[tier], (tier, cplx, M, N, K, ak, bk, cA, cB, err)
+    finally:
+        ctx.set_tier("fp64")
+
+
+@pytest.mark.parametrize("tier", ["fp64", "fp32"])
+def test_gemm_accumulate(ctx, tier):
+    """C = op(A) op(B) + beta C for the accumulating contractions, including
+    the split-K path (beta applied once, in the reduction)."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(4)
+    ctx.set_tier(tier)
+    try:
+        for (M, N, K, beta) in [(300, 260, 70, 1.0), (512, 384, 4096, 0.5), (40, 30, 20, -2.0),
+                                (1100, 300, 64, 1.0)]:
+            for ak in (False, True):
+                for bk in (False, True):
+                    A = rng.standard_normal((M, K))
+                    B = rng.standard_normal((K, N))
+                    C0 = rng.standard_normal((M, N))
+                    out = peps.gemm_acc(A.T.copy() if ak else A, B if bk else B.T.copy(), C0, beta,
+                                        a_kfast=ak, b_kfast=bk, ctx=ctx)
+                    ref = A @ B + beta * C0
+                    scale = np.sqrt(K) * np.abs(A).max() * np.abs(B).max() + abs(beta) * np.abs(C0).max()
+                    assert np.abs(out - ref).max() / scale < TOL[tier], (tier, M, N, K, beta, ak, bk)
     finally:
         ctx.set_tier("fp64")
 

@@ -388,3 +388,28 @@ def test_trotter_step_v4_renorm_only_rescales_boundaries(ctx, request, name, D, 
         assert abs(e_gpu - e_ref) < 1e-10 * abs(e_ref)
     else:
         assert abs(e_gpu - e_ref) < 5e-9 * abs(e_ref)
+
+
+def test_fp32_tier_matches_fp64_oracle(ctx, s34):
+    """The FP32 MFMA tier (the benchmark's tier; factorisations stay FP64)
+    against the FP64 oracle: bond environments within 2e-5 relative (the
+    3x4 D=3 vertical frames are 729 x 729, so the tiled FP32 GEMM runs), and
+    one n_als=0 step with identical flux labels and energy within 1e-5."""
+    from lgt_amd import peps
+    gd = to_device(s34, ctx)
+    ctx.set_tier("fp32")
+    try:
+        for orient, ix, iy in [(0, 1, 2), (0, 2, 3), (1, 2, 2), (1, 1, 1), (1, 3, 3)]:
+            if orient == 0:
+                Eg, Er = peps.bond_environment_h(gd, ix, iy, 12), O.bond_environment_h(s34, ix, iy, 12)
+            else:
+                Eg, Er = peps.bond_environment_v(gd, ix, iy, 12), O.bond_environment_v(s34, ix, iy, 12)
+            err = rel(_phys_env(Eg), _phys_env(Er))
+            print(f"fp32 tier env {orient} ({ix},{iy}) rel {err:.2e}", flush=True)
+            assert err < 2e-5, (orient, ix, iy, err)
+        go = s34.copy()
+        peps.trotter_step_v4(gd, G, T_HOP, M, 0.05, 3, 12, 0)
+        O.trotter_step_v4(go, G, T_HOP, M, 0.05, 3, 12, 0)
+        _compare_states(gd, go, 12, etol=1e-5, lam_atol=1e-4)
+    finally:
+        ctx.set_tier("fp64")
