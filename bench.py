@@ -287,7 +287,8 @@ def main():
     achieved = gk["flops"] / (gk["ms"] / 1e3) / 1e12 if gk["ms"] > 0 else 0.0
     total_flops = sum(v["flops"] for v in stats.values())
     for name, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
-        log(f"  {name:12s} launches={v['launches']:8d} ms={v['ms']:10.1f} flop={v['flops']:.3e}")
+        rate = f" {v['bytes'] / v['ms'] / 1e6:8.1f} GB/s" if v["ms"] > 0 and v["bytes"] > 0 else ""
+        log(f"  {name:12s} launches={v['launches']:8d} ms={v['ms']:10.1f} flop={v['flops']:.3e}{rate}")
     out = {
         "metric": "full-update steps/s at 8x8 D=6 chi=144",
         "value": value,

@@ -238,6 +238,11 @@ int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64
                   const double* A, int32_t a_kfast, int32_t conjA, const double* B,
                   int32_t b_kfast, int32_t conjB, double* Cout);
 
+/* The engine's tensor permutation (no counterpart in the reference; exposed
+   for validation): out.dims[i] = dims[perm[i]], 0-based perm. */
+int32_t peps_permute(peps_ctx* ctx, int32_t is_complex, int32_t rank, const int64_t* dims,
+                     const int32_t* perm, const double* in, double* out);
+
 /* The same GEMM accumulating into C: C (M x N) = op(A) op(B) + beta C (the
    form the chunked contractions use). */
 int32_t peps_gemm_acc(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,

@@ -624,6 +624,23 @@ int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64
   });
 }
 
+int32_t peps_permute(peps_ctx* ctx, int32_t is_complex, int32_t rank, const int64_t* dims,
+                     const int32_t* perm, const double* in, double* out) {
+  if (!ctx || !dims || !perm || !in || !out || rank < 1 || rank > 16) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    std::vector<int64_t> d(dims, dims + rank);
+    std::vector<int> pr(perm, perm + rank);
+    std::vector<int> seen(rank, 0);
+    for (int i = 0; i < rank; ++i) {
+      PEPS_CHECK(d[i] >= 1 && pr[i] >= 0 && pr[i] < rank && !seen[pr[i]], peps::PS_EARG, "permute: bad arguments");
+      seen[pr[i]] = 1;
+    }
+    Tensor t = ctx->c->upload(d, is_complex != 0, in);
+    Tensor o = peps::permute(*ctx->c, t, pr);
+    ctx->c->download(o, out);
+  });
+}
+
 int32_t peps_gemm_acc(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,
                       const double* A, int32_t a_kfast, int32_t conjA, const double* B,
                       int32_t b_kfast, int32_t conjB, double beta, double* Cinout) {

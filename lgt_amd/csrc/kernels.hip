@@ -594,6 +594,46 @@ __global__ void transpose_kernel(const E* __restrict__ in, E* __restrict__ out, 
     }
 }
 
+// General permutation as a batch of 2-D transposes through LDS: output dim 0
+// (output-contiguous, input stride s0) against the output dim whose input
+// stride is 1 (output stride o1); every other dim is a batch index whose
+// input/output offsets are decoded once per tile, not per element.  Reads
+// are coalesced along the input-contiguous dim, writes along output dim 0.
+struct PermTiled {
+  int nb;
+  int64_t bdims[PMAX], bistr[PMAX], bostr[PMAX];
+  int64_t n0, n1, s0, o1, nbatch;
+};
+
+template <typename E>
+__global__ __launch_bounds__(256) void permute_tiled_kernel(const E* __restrict__ in, E* __restrict__ out,
+                                                            PermTiled p) {
+  __shared__ E tile[32][33];
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (int64_t b = blockIdx.z; b < p.nbatch; b += gridDim.z) {
+    int64_t rem = b, ioff = 0, ooff = 0;
+    for (int d = 0; d < p.nb; ++d) {
+      const int64_t q = rem / p.bdims[d], r = rem - q * p.bdims[d];
+      ioff += r * p.bistr[d];
+      ooff += r * p.bostr[d];
+      rem = q;
+    }
+    for (int64_t t0 = int64_t(blockIdx.x) * 32; t0 < p.n0; t0 += int64_t(gridDim.x) * 32)
+      for (int64_t t1 = int64_t(blockIdx.y) * 32; t1 < p.n1; t1 += int64_t(gridDim.y) * 32) {
+        for (int k = ty; k < 32; k += 8) {
+          const int64_t i0 = t0 + k, i1 = t1 + tx;
+          if (i0 < p.n0 && i1 < p.n1) tile[k][tx] = in[ioff + i0 * p.s0 + i1];
+        }
+        __syncthreads();
+        for (int k = ty; k < 32; k += 8) {
+          const int64_t i0 = t0 + tx, i1 = t1 + k;
+          if (i0 < p.n0 && i1 < p.n1) out[ooff + i0 + i1 * p.o1] = tile[tx][k];
+        }
+        __syncthreads();
+      }
+  }
+}
+
 Tensor permute(Ctx& c, const Tensor& in, const std::vector<int>& perm) {
   const int nd = in.rank();
   PEPS_CHECK(int(perm.size()) == nd, PS_ESHAPE, "permute: rank mismatch");
@@ -634,9 +674,14 @@ Tensor permute(Ctx& c, const Tensor& in, const std::vector<int>& perm) {
       cs.push_back(st);
     }
   }
-  c.count("permute", 0, 2.0 * in.bytes());
+  std::string pcls = "permute";
+  if (c.gemm_shapes) {
+    pcls += in.cplx ? " c" : " r";
+    for (size_t i = 0; i < cd.size(); ++i) pcls += " " + std::to_string(cd[i]) + "@" + std::to_string(cs[i]);
+  }
+  c.count(pcls, 0, 2.0 * in.bytes());
   hipEvent_t ev = nullptr;
-  c.tbegin("permute", &ev);
+  c.tbegin(pcls, &ev);
   // two-block swap => tiled transpose: output (d0, d1) with d0 contiguous at
   // input stride d1-extent.. i.e. input is (d1 x d0) col-major.
   bool tr = false;
@@ -646,7 +691,38 @@ Tensor permute(Ctx& c, const Tensor& in, const std::vector<int>& perm) {
   } else if (cd.size() == 3 && cs[1] == 1 && cs[0] == cd[1] && cs[2] == cd[0] * cd[1]) {
     tr = true; R = cd[1]; Cc = cd[0]; nb = cd[2];
   }
-  if (tr && nb < 65536) {
+  // input-contiguous output dim, for the batched tiled transpose
+  int kc = -1;
+  for (size_t i = 1; i < cs.size(); ++i)
+    if (cs[i] == 1) kc = int(i);
+  const bool tiled = !tr && kc > 0 && cs[0] != 1 && cd[0] >= 16 && cd[size_t(kc)] >= 16 &&
+                     int(cd.size()) - 2 <= PMAX;
+  if (tiled) {
+    PermTiled p{};
+    p.n0 = cd[0];
+    p.s0 = cs[0];
+    p.n1 = cd[size_t(kc)];
+    int64_t ostr = 1;
+    p.nbatch = 1;
+    for (size_t i = 0; i < cd.size(); ++i) {
+      if (int(i) == kc) p.o1 = ostr;
+      if (i != 0 && int(i) != kc) {
+        p.bdims[p.nb] = cd[i];
+        p.bistr[p.nb] = cs[i];
+        p.bostr[p.nb] = ostr;
+        ++p.nb;
+        p.nbatch *= cd[i];
+      }
+      ostr *= cd[i];
+    }
+    dim3 grid(unsigned(std::min<int64_t>(cdiv(p.n0, 32), 64)), unsigned(std::min<int64_t>(cdiv(p.n1, 32), 64)),
+              unsigned(std::min<int64_t>(p.nbatch, 16384)));
+    if (in.cplx)
+      hipLaunchKernelGGL(permute_tiled_kernel<double2>, grid, dim3(32, 8), 0, c.stream,
+                         (const double2*)in.data(), (double2*)out.data(), p);
+    else
+      hipLaunchKernelGGL(permute_tiled_kernel<double>, grid, dim3(32, 8), 0, c.stream, in.data(), out.data(), p);
+  } else if (tr && nb < 65536) {
     dim3 grid(std::min<unsigned>(cdiv(R, 32), 1024u), std::min<unsigned>(cdiv(Cc, 32), 1024u), unsigned(nb));
     if (in.cplx)
       hipLaunchKernelGGL(transpose_kernel<double2>, grid, dim3(32, 8), 0, c.stream,
@@ -671,7 +747,7 @@ Tensor permute(Ctx& c, const Tensor& in, const std::vector<int>& perm) {
                          out.data(), n, p);
   }
   PEPS_HIP(hipGetLastError());
-  c.tend("permute", ev);
+  c.tend(pcls, ev);
   return out;
 }
 

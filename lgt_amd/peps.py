@@ -97,6 +97,7 @@ _SIGS = {
     "peps_bond_entropy": (C.c_int32, [_dp, C.c_int64, _dp]),
     "peps_measure_all": (C.c_int32, [_vp, _dp, _dp, _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
+    "peps_permute": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p, _i32p, _dp, _dp]),
     "peps_gemm_acc": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp,
                                   C.c_int32, C.c_int32, C.c_double, _dp]),
     "peps_eigh": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
@@ -651,6 +652,18 @@ def gemm(A, B, a_kfast=False, conjA=False, b_kfast=True, conjB=False, ctx=None):
     out = np.empty((M, N), dtype=a.dtype, order="F")
     ctx.check(lib().peps_gemm(ctx.h, int(cp), M, N, K, _dptr(a), int(a_kfast), int(conjA), _dptr(b),
                               int(b_kfast), int(conjB), _dptr(out)))
+    return out
+
+
+def permute(T, perm, ctx=None):
+    """The engine's permutation kernel: numpy.transpose(T, perm) (column-major storage)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(T)
+    a = _as_array(T, cp)
+    d = np.array(a.shape, dtype=np.int64)
+    pr = np.array(perm, dtype=np.int32)
+    out = np.empty(tuple(a.shape[i] for i in perm), dtype=a.dtype, order="F")
+    ctx.check(lib().peps_permute(ctx.h, int(cp), a.ndim, _iptr(d), pr.ctypes.data_as(_i32p), _dptr(a), _dptr(out)))
     return out
 
 

@@ -65,6 +65,23 @@ def test_gemm_accumulate(ctx, tier):
         ctx.set_tier("fp64")
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_permute(ctx, cplx):
+    """Every permutation path: pure reshape, 2-D/batched transpose, the
+    batched tiled transpose (output-fastest dim not input-contiguous, both
+    extents >= 16, ragged tiles) and the general gather."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(6)
+    cases = [((5, 7), (1, 0)), ((40, 33), (1, 0)), ((17, 20, 3), (1, 0, 2)),
+             ((18, 5, 40), (2, 0, 1)), ((36, 6, 6, 50), (3, 1, 0, 2)), ((20, 3, 17, 2, 19), (4, 2, 0, 3, 1)),
+             ((6, 6, 6, 6, 6, 6), (5, 3, 1, 0, 2, 4)), ((1, 48, 1, 33), (3, 0, 1, 2)), ((33, 1, 70), (2, 1, 0)),
+             ((3, 4, 5), (0, 1, 2))]
+    for shp, perm in cases:
+        T = rng.standard_normal(shp) + (1j * rng.standard_normal(shp) if cplx else 0)
+        out = peps.permute(T, perm, ctx=ctx)
+        np.testing.assert_array_equal(out, np.transpose(T, perm))
+
+
 def test_dl_traced_matches_oracle(ctx):
     from lgt_amd import peps
     from oracle import peps_oracle as O
