@@ -1518,6 +1518,109 @@ __global__ __launch_bounds__(1024) void jacobi_svd_kernel(typename JT<CP>::T* __
   if (tid == 0 && sweeps_out) *sweeps_out = sweep;
 }
 
+// Same one-sided Jacobi sweeps with each column pair handled by L lanes
+// (64/L pairs per wave) and the pair's four columns (A_p, A_q, V_p, V_q)
+// held in registers, R entries per lane: one load and one store per
+// rotation instead of three passes over global memory.  Needs m, n <= L*R.
+template <bool CP, int L, int R>
+__global__ __launch_bounds__(1024) void jacobi_svd_reg_kernel(typename JT<CP>::T* __restrict__ A,
+                                                              typename JT<CP>::T* __restrict__ V,
+                                                              double* __restrict__ sig, int m, int n,
+                                                              double tol, int max_sweeps,
+                                                              int* __restrict__ sweeps_out) {
+  typedef typename JT<CP>::T T;
+  __shared__ int rot;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  const int sub = lane & (L - 1);                          // lane within the pair group
+  const int slot = wave * (64 / L) + lane / L;             // pair slot of this group
+  const int nslots = nw * (64 / L);
+  const int mm = (n + 1) & ~1;
+  const int half = mm / 2;
+  const int64_t nn = int64_t(n) * n;
+  for (int64_t i = tid; i < nn; i += blockDim.x) {
+    int64_t r = i % n, c = i / n;
+    if (r == c) set_one(V[i]); else set_zero(V[i]);
+  }
+  if (tid == 0) rot = 0;
+  __syncthreads();
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    for (int rnd = 0; rnd < mm - 1; ++rnd) {
+      for (int k = slot; k < half; k += nslots) {
+        int i0 = k, i1 = mm - 1 - k;
+        int p = (i0 == 0) ? 0 : ((i0 - 1 + rnd) % (mm - 1)) + 1;
+        int q = ((i1 - 1 + rnd) % (mm - 1)) + 1;
+        if (p > q) { int t = p; p = q; q = t; }
+        if (q >= n) continue;
+        T* ap = A + int64_t(p) * m;
+        T* aq = A + int64_t(q) * m;
+        T* vp = V + int64_t(p) * n;
+        T* vq = V + int64_t(q) * n;
+        T xa[R], xb[R], ya[R], yb[R];
+        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = sub + L * r;
+          if (i < m) { xa[r] = ap[i]; xb[r] = aq[i]; } else { set_zero(xa[r]); set_zero(xb[r]); }
+          if (i < n) { ya[r] = vp[i]; yb[r] = vq[i]; } else { set_zero(ya[r]); set_zero(yb[r]); }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          al += JT<CP>::abs2(xa[r]);
+          be += JT<CP>::abs2(xb[r]);
+          T g = mulc(conjv(xa[r]), xb[r]);
+          if constexpr (CP) { gr += g.x; gi += g.y; } else { gr += g; }
+        }
+        for (int o = L / 2; o > 0; o >>= 1) {
+          al += __shfl_xor(al, o);
+          be += __shfl_xor(be, o);
+          gr += __shfl_xor(gr, o);
+          gi += __shfl_xor(gi, o);
+        }
+        const double na = sqrt(al), nb = sqrt(be);
+        const double ag = CP ? hypot(gr, gi) : fabs(gr);
+        if (!(na > 1e-140 && nb > 1e-140 && ag > tol * na * nb)) continue;
+        const double th = (be - al) / (2.0 * ag);
+        const double t = fabs(th) > 1e150 ? 0.5 / th
+                                          : (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        if (t == 0.0) continue;
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        T ec;
+        if constexpr (CP) ec = make_double2(gr / ag, -gi / ag); else ec = gr >= 0.0 ? 1.0 : -1.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = sub + L * r;
+          T x = xa[r], y = mulc(xb[r], ec);
+          if (i < m) {
+            ap[i] = addv(scl(x, c), scl(y, -s));
+            aq[i] = addv(scl(x, s), scl(y, c));
+          }
+          T u = ya[r], w = mulc(yb[r], ec);
+          if (i < n) {
+            vp[i] = addv(scl(u, c), scl(w, -s));
+            vq[i] = addv(scl(u, s), scl(w, c));
+          }
+        }
+        if (sub == 0) atomicAdd(&rot, 1);
+      }
+      __syncthreads();
+    }
+    const int r = rot;
+    __syncthreads();
+    if (r == 0) break;
+    if (tid == 0) rot = 0;
+    __syncthreads();
+  }
+  for (int j = wave; j < n; j += nw) {
+    const T* aj = A + int64_t(j) * m;
+    double s2 = 0.0;
+    for (int i = lane; i < m; i += 64) s2 += JT<CP>::abs2(aj[i]);
+    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
+    if (lane == 0) sig[j] = sqrt(s2);
+  }
+  if (tid == 0 && sweeps_out) *sweeps_out = sweep;
+}
+
 int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sweeps) {
   PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "jacobi_svd: matrix expected");
   const int64_t m = A.dims[0], n = A.dims[1];
@@ -1535,7 +1638,23 @@ int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sw
   const std::string cls = Ctx::bucket("svd", n);
   c.tbegin(cls, &ev);
   c.count(cls, (A.cplx ? 4.0 : 1.0) * 10.0 * 6.0 * double(m) * n * n, 0);
-  if (A.cplx)
+  // register-resident variants (4 columns x R entries per lane in VGPRs,
+  // within the 128-VGPR budget of 1024-thread blocks): real m, n <= 256 with
+  // 32 lanes per pair (two pairs per wave), real m, n <= 512 and complex
+  // m, n <= 256 with 64 lanes per pair
+  const int64_t mx = std::max(m, n);
+  const int rw2 = std::max(1, std::min(16, int(cdiv(half, 2)))), rw1 = std::max(1, std::min(16, half));
+  if (n >= 2 && !A.cplx && mx <= 256)
+    hipLaunchKernelGGL((jacobi_svd_reg_kernel<false, 32, 8>), dim3(1), dim3(64 * rw2), 0, c.stream, A.data(),
+                       V.data(), sig.data(), int(m), int(n), tol, max_sweeps, (int*)buf->ptr);
+  else if (n >= 2 && !A.cplx && mx <= 512)
+    hipLaunchKernelGGL((jacobi_svd_reg_kernel<false, 64, 8>), dim3(1), dim3(64 * rw1), 0, c.stream, A.data(),
+                       V.data(), sig.data(), int(m), int(n), tol, max_sweeps, (int*)buf->ptr);
+  else if (n >= 2 && A.cplx && mx <= 256)
+    hipLaunchKernelGGL((jacobi_svd_reg_kernel<true, 64, 4>), dim3(1), dim3(64 * rw1), 0, c.stream,
+                       (double2*)A.data(), (double2*)V.data(), sig.data(), int(m), int(n), tol, max_sweeps,
+                       (int*)buf->ptr);
+  else if (A.cplx)
     hipLaunchKernelGGL(jacobi_svd_kernel<true>, dim3(1), dim3(64 * waves), 0, c.stream,
                        (double2*)A.data(), (double2*)V.data(), sig.data(), int(m), int(n), tol,
                        max_sweeps, (int*)buf->ptr);
