@@ -267,7 +267,7 @@ __global__ void splitk_reduce(double* __restrict__ C, const double* __restrict__
   }
 }
 
-// Real GEMM of the FP32 tier for large outputs (M, N >= 256).
+// Real GEMM of the FP32 tier for large outputs (M, N >= 128).
 //
 // 128x128 output tile per 256-thread block (2x2 waves of 64x64, 4x4 MFMA
 // f32 16x16x4 each), k-tiles of 32 staged FP64 -> FP32 through two LDS
@@ -280,7 +280,7 @@ __global__ void splitk_reduce(double* __restrict__ C, const double* __restrict__
 // consecutive in that order.  The epilogue transposes the accumulators
 // through LDS so that C is read and written in whole column segments.
 template <bool AK, bool BKF>
-__global__ __launch_bounds__(256) void gemm_f32_tiled(int64_t M, int64_t N, int64_t K,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_f32_tiled(int64_t M, int64_t N, int64_t K,
                                                       const double* __restrict__ A, int64_t lda,
                                                       const double* __restrict__ B, int64_t ldb,
                                                       double* __restrict__ C, int64_t ldc, double beta,
@@ -320,28 +320,36 @@ __global__ __launch_bounds__(256) void gemm_f32_tiled(int64_t M, int64_t N, int6
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  double ra[16], rb[16];
-  // element e of this thread in a 128 x 32 operand tile: (row, k)
-  auto arow = [&](int e) { return AK ? (tid >> 5) + 8 * e : (tid & 127); };
-  auto acol = [&](int e) { return AK ? (tid & 31) : (tid >> 7) + 2 * e; };
-  auto brow = [&](int e) { return BKF ? (tid >> 5) + 8 * e : (tid & 127); };
-  auto bcol = [&](int e) { return BKF ? (tid & 31) : (tid >> 7) + 2 * e; };
+  // Each thread moves 4 groups of 4 consecutive k of one row (m or n) per
+  // operand and k-tile: four coalesced 8-byte loads per group (lanes run
+  // along the contiguous global dim) and one 16-byte LDS store.
+  double ra[4][4], rb[4][4];
+  // group e of this thread in a 128 x 32 operand tile: (row, first k)
+  auto arow = [&](int e) { return AK ? (tid >> 3) + 32 * e : (tid & 127); };
+  auto acol = [&](int e) { return AK ? 4 * (tid & 7) : 4 * ((tid >> 7) + 2 * e); };
+  auto brow = [&](int e) { return BKF ? (tid >> 3) + 32 * e : (tid & 127); };
+  auto bcol = [&](int e) { return BKF ? 4 * (tid & 7) : 4 * ((tid >> 7) + 2 * e); };
   auto gload = [&](int64_t k0) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int64_t gm = m0 + arow(e), gk = k0 + acol(e);
-      ra[e] = (gm < M && gk < kend) ? A[AK ? gk + gm * lda : gm + gk * lda] : 0.0;
-      const int64_t gn = n0 + brow(e), gkb = k0 + bcol(e);
-      rb[e] = (gn < N && gkb < kend) ? B[BKF ? gkb + gn * ldb : gn + gkb * ldb] : 0.0;
+    for (int e = 0; e < 4; ++e) {
+      const int64_t gm = m0 + arow(e), gn = n0 + brow(e);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t gk = k0 + acol(e) + j, gkb = k0 + bcol(e) + j;
+        ra[e][j] = (gm < M && gk < kend) ? A[AK ? gk + gm * lda : gm + gk * lda] : 0.0;
+        rb[e][j] = (gn < N && gkb < kend) ? B[BKF ? gkb + gn * ldb : gn + gkb * ldb] : 0.0;
+      }
     }
   };
   auto lstore = [&](int buf) {
     float* As = smem + buf * BUF;
     float* Bs = As + BM * LDK;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      As[arow(e) * LDK + acol(e)] = float(ra[e]);
-      Bs[brow(e) * LDK + bcol(e)] = float(rb[e]);
+    for (int e = 0; e < 4; ++e) {
+      *reinterpret_cast<v4f*>(As + arow(e) * LDK + acol(e)) =
+          v4f{float(ra[e][0]), float(ra[e][1]), float(ra[e][2]), float(ra[e][3])};
+      *reinterpret_cast<v4f*>(Bs + brow(e) * LDK + bcol(e)) =
+          v4f{float(rb[e][0]), float(rb[e][1]), float(rb[e][2]), float(rb[e][3])};
     }
   };
 
@@ -506,7 +514,7 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
     out = static_cast<double*>(ws->ptr);
     kbeta = 0.0;
   }
-  const bool tiled = !cplx && c.tier == TIER_FP32 && M >= 256 && N >= 256 &&
+  const bool tiled = !cplx && c.tier == TIER_FP32 && M >= 128 && N >= 128 &&
                      int64_t(cdiv(M, 128)) * cdiv(N, 128) < (int64_t(1) << 31);
   auto go = [&](auto cplx_tag, auto prec_tag) {
     constexpr bool CP = decltype(cplx_tag)::value;
@@ -1635,7 +1643,7 @@ int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sw
   const int half = int((n + 1) / 2);
   const int waves = std::max(1, std::min(16, half));
   hipEvent_t ev = nullptr;
-  const std::string cls = Ctx::bucket("svd", n);
+  const std::string cls = c.gemm_shapes ? "svd " + std::to_string(m) + "x" + std::to_string(n) : Ctx::bucket("svd", n);
   c.tbegin(cls, &ev);
   c.count(cls, (A.cplx ? 4.0 : 1.0) * 10.0 * 6.0 * double(m) * n * n, 0);
   // register-resident variants (4 columns x R entries per lane in VGPRs,
