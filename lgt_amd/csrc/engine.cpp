@@ -333,7 +333,7 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
     Tensor rest = Q0.slice_last(have, p);
     fill_random(c, rest, 0x5eed1234abcdull + uint64_t(N) * 7919u + uint64_t(have));
   }
-  Tensor Q = orthonormal_basis(c, have ? Q0 : matmul(c, rh, Q0));
+  Tensor Q = qr_thin(c, have ? Q0 : matmul(c, rh, Q0)).Q;
   std::vector<double> theta;
   Tensor X;
   constexpr int kCheb = 8;  // filter degree per outer iteration
@@ -388,7 +388,7 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
     std::vector<double> n2 = col_norm2(c, Y1), inv(p);
     for (int64_t j = 0; j < p; ++j) inv[j] = n2[j] > 0.0 && std::isfinite(n2[j]) ? 1.0 / std::sqrt(n2[j]) : 0.0;
     scale_cols(c, Y1, inv);
-    Q = orthonormal_basis(c, Y1);
+    Q = qr_thin(c, Y1).Q;
   }
   if (key) c.ritz_cache[*key] = X;
   lam->assign(theta.begin(), theta.begin() + want);

@@ -108,67 +108,6 @@ QR qr_thin(Ctx& c, const Tensor& A) {
   return r;
 }
 
-Tensor orthonormal_basis(Ctx& c, const Tensor& Y) {
-  PEPS_CHECK(Y.rank() == 2, PS_ESHAPE, "orthonormal_basis: matrix expected");
-  const int64_t m = Y.dims[0], p = Y.dims[1];
-  check_int_range(m, p);
-  if (p == 0 || m < p || !check_input(c, Y, "orthonormal_basis")) return qr_thin(c, Y).Q;
-  TierGuard fp64(c, TIER_FP64);
-  hipEvent_t ev = nullptr;
-  const std::string cls = c.gemm_shapes ? "cholqr " + std::to_string(m) + "x" + std::to_string(p)
-                                        : Ctx::bucket("cholqr", p);
-  c.count(cls, (Y.cplx ? 4.0 : 1.0) * 2.0 * (2.0 * m * p * p + p * p * p / 3.0 + m * p * p), 0);
-  Tensor Q = copy_tensor(c, Y);
-  DevInt info(c);
-  bool ok = true;
-  for (int pass = 0; pass < 2 && ok; ++pass) {
-    Tensor G = matmul(c, Q, Q, true, false, true, false);  // Q^H Q
-    hermitize(c, G);
-    c.tbegin(cls, &ev);
-    if (Y.cplx) {
-      ck(rocsolver_zpotrf(c.blas, rocblas_fill_upper, int(p), (rocblas_double_complex*)G.data(), int(p), info.p),
-         "zpotrf");
-    } else {
-      ck(rocsolver_dpotrf(c.blas, rocblas_fill_upper, int(p), G.data(), int(p), info.p), "dpotrf");
-    }
-    // Q <- Q R^{-1} (discarded below if the factorisation failed)
-    if (Y.cplx) {
-      const rocblas_double_complex one{1.0, 0.0};
-      ck(rocblas_ztrsm(c.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                       rocblas_diagonal_non_unit, int(m), int(p), &one, (rocblas_double_complex*)G.data(),
-                       int(p), (rocblas_double_complex*)Q.data(), int(m)),
-         "ztrsm");
-    } else {
-      const double one = 1.0;
-      ck(rocblas_dtrsm(c.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                       rocblas_diagonal_non_unit, int(m), int(p), &one, G.data(), int(p), Q.data(), int(m)),
-         "dtrsm");
-    }
-    c.tend(cls, ev);
-    ok = info.get(c) == 0;
-  }
-  if (ok) {
-    // accept only an orthonormal result
-    Tensor G = matmul(c, Q, Q, true, false, true, false);
-    std::vector<double> g = c.to_host(G);
-    double err = 0.0;
-    const int es = Y.cplx ? 2 : 1;
-    for (int64_t j = 0; j < p && std::isfinite(err); ++j)
-      for (int64_t i = 0; i < p; ++i) {
-        const double re = g[size_t(i + j * p) * es] - (i == j ? 1.0 : 0.0);
-        const double im = Y.cplx ? g[size_t(i + j * p) * es + 1] : 0.0;
-        err = std::max(err, std::fabs(re) + std::fabs(im));
-        if (!std::isfinite(re) || !std::isfinite(im)) err = INFINITY;
-      }
-    ok = err <= 1e-12;
-  }
-  if (!ok) {
-    c.count("cholqr[fallback]", 1.0, 0);
-    return qr_thin(c, Y).Q;
-  }
-  return Q;
-}
-
 Tensor qr_r_only(Ctx& c, const Tensor& A) {
   PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "qr: matrix expected");
   int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);

@@ -281,10 +281,6 @@ struct QR {
 };
 QR qr_thin(Ctx& c, const Tensor& A);
 Tensor qr_r_only(Ctx& c, const Tensor& A);
-// An orthonormal basis of span(Y) (m x p, m >= p): two Cholesky-QR passes,
-// or the Householder Q when a Gram matrix is not numerically positive
-// definite or the result is not orthonormal to 1e-12.
-Tensor orthonormal_basis(Ctx& c, const Tensor& Y);
 struct SVD {
   Tensor U, Vt;            // U (m x k), Vt (k x n)
   std::vector<double> S;   // descending
