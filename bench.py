@@ -35,11 +35,15 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
-GEMM_CLASS = {"fp64": "gemm_f64", "fp32": "gemm_f32", "bf16": "gemm_bf16"}
-# HBM bytes per gemm_kernel launch measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE over the timed step of this workload (copy-calibrated factors 2.0
-# and 1.0), keyed by (nx, ny, D, chi, gd, tier); profiles/ holds the summary.
-PROFILED_TRAFFIC = {(8, 8, 6, 144, 2, "fp32"): (2.0 * 8.3310e5 + 1.0 * 1.3208e5) * 1024.0}
+# the dominant kernel of each tier: its launches are timed as one class
+GEMM_CLASS = {"fp64": "gemm_f64", "fp32": "gemm_f32_tiled", "bf16": "gemm_bf16"}
+GEMM_KERNEL = {"fp64": "gemm_kernel (FP64 MFMA)", "fp32": "gemm_f32_tiled (FP32 MFMA, 128x128x32 tiles)",
+               "bf16": "gemm_kernel (BF16 MFMA)"}
+# HBM bytes per launch of the dominant kernel measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE over the timed step of this workload (copy-
+# calibrated factors 2.0 and 1.0), keyed by (nx, ny, D, chi, gd, tier);
+# profiles/ holds the summary.
+PROFILED_TRAFFIC = {}
 
 
 def log(msg):
@@ -315,7 +319,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[a.tier],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[a.tier],
                      "traffic": PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier)),
-                     "kernel": f"gemm_kernel ({GEMM_CLASS[a.tier]}, all launches)",
+                     "kernel": GEMM_KERNEL[a.tier],
                      "launches": gk["launches"],
                      "avg_launch_ms": gk["ms"] / max(1, gk["launches"]),
                      "algorithmic_flop_per_launch": gk["flops"] / max(1, gk["launches"]),

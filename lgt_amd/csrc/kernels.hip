@@ -494,7 +494,10 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
   }
   const double flops = (cplx ? 8.0 : 2.0) * double(M) * double(N) * double(K);
   const double bytes = double(esz * 8) * (double(M) * K + double(K) * N + double(M) * N);
-  std::string cls = c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16");
+  const bool tiled = !cplx && c.tier == TIER_FP32 && M >= 128 && N >= 128 &&
+                     int64_t(cdiv(M, 128)) * cdiv(N, 128) < (int64_t(1) << 31);
+  std::string cls = tiled ? "gemm_f32_tiled"
+                          : (c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16"));
   if (c.gemm_shapes)
     cls += " " + std::to_string(M) + "x" + std::to_string(N) + "x" + std::to_string(K) + (cplx ? "c" : "") +
            (a_kfast ? " ak" : " am") + (b_kfast ? " bk" : " bn") + " s" + std::to_string(splits);
@@ -514,8 +517,6 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
     out = static_cast<double*>(ws->ptr);
     kbeta = 0.0;
   }
-  const bool tiled = !cplx && c.tier == TIER_FP32 && M >= 128 && N >= 128 &&
-                     int64_t(cdiv(M, 128)) * cdiv(N, 128) < (int64_t(1) << 31);
   auto go = [&](auto cplx_tag, auto prec_tag) {
     constexpr bool CP = decltype(cplx_tag)::value;
     constexpr int PR = decltype(prec_tag)::value;
