@@ -222,6 +222,13 @@ int32_t peps_plaquette_energy(peps_state* st, int32_t ix, int32_t iy, double g, 
 /* total_energy (finite_peps_ground_state.jl:690-805): uses g, t_hop, m */
 int32_t peps_total_energy(peps_state* st, const peps_params* p, double* out);
 
+/* <W> = <psi| Ud_b U_l Ud_r U_t |psi> / <psi|psi> on the plaquette with
+   lower-left site (ix, iy), from boundary-MPS environments of bond
+   dimension chi (the two-row strip between boundary_top(iy+1) and
+   boundary_bottom(iy)); out = (re, im).  No reference counterpart: the
+   boundary-MPS form of plaquette_energy (finite_peps_ground_state.jl:589,
+   which uses simple-update weights); the plaquette energy is -Re<W>/g^2. */
+int32_t peps_plaquette_expect(peps_state* st, int32_t ix, int32_t iy, int32_t chi, double out[2]);
 /* bond_entropy (finite_peps_quench.jl:121-127): von Neumann entropy of
    p_i = lambda_i^2 / sum lambda^2 (0 when the sum is below 1e-30) */
 int32_t peps_bond_entropy(const double* lambda, int64_t n, double* out);

@@ -591,6 +591,16 @@ int32_t peps_total_energy(peps_state* st, const peps_params* p, double* out) {
   });
 }
 
+int32_t peps_plaquette_expect(peps_state* st, int32_t ix, int32_t iy, int32_t chi, double out[2]) {
+  if (!st || !out || chi < 1) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
+    peps::cplx w = peps::plaquette_expect(st->s, ix, iy, chi);
+    out[0] = w.re;
+    out[1] = w.im;
+  });
+}
+
 int32_t peps_bond_entropy(const double* lambda, int64_t n, double* out) {
   if ((!lambda && n > 0) || n < 0 || !out) return PEPS_EARG;
   *out = peps::bond_entropy(std::vector<double>(lambda, lambda + n));

@@ -1542,13 +1542,24 @@ double expect_site(State& s, int ix, int iy, const std::vector<double>& O, bool 
   return den > 1e-15 ? num / den : 0.0;
 }
 
+// Wilson-loop site factors of the L-shape (finite_peps_ground_state.jl:632-641):
+// A=(ix,iy) Ud on R and U on U, B=(ix+1,iy) Ud on U, C=(ix,iy+1) U on R.
+static void wilson_factors(int nx, int ny, int gd, int ix, int iy, Mat* WA, Mat* WB, Mat* WC) {
+  SiteDims sB = site_dims(ix + 1, iy, nx, ny, gd), sC = site_dims(ix, iy + 1, nx, ny, gd);
+  Mat U = op_U_gauge(gd), Ud = U;
+  for (int i = 0; i < gd; ++i)
+    for (int j = 0; j < gd; ++j) Ud(i, j) = U(j, i);
+  *WA = kron(kron(identity(2), Ud), U);
+  *WB = sB.dR == 1 ? kron(identity(2), Ud) : kron(kron(identity(2), identity(sB.dR)), Ud);
+  *WC = sC.dU == 1 ? kron(identity(2), U) : kron(kron(identity(2), U), identity(sC.dU));
+}
+
 double plaquette_energy(State& s, int ix, int iy, double g) {
   // plaquette_energy :589-684: -(1/g^2) Re<W>, sqrt(lambda) on every leg,
   // the external legs closed diagonally
   Ctx& c = *s.c;
   const int nx = s.nx, ny = s.ny, gd = s.gd;
   PEPS_CHECK(ix >= 1 && ix < nx && iy >= 1 && iy < ny, PS_EARG, "plaquette_energy: needs ix < nx and iy < ny");
-  SiteDims sB = site_dims(ix + 1, iy, nx, ny, gd), sC = site_dims(ix, iy + 1, nx, ny, gd);
   std::vector<double> sAB = vsqrt(s.LH(ix, iy)), sAC = vsqrt(s.LV(ix, iy));
   Tensor TA = copy_tensor(c, s.t(ix, iy)), TB = copy_tensor(c, s.t(ix + 1, iy)), TC = copy_tensor(c, s.t(ix, iy + 1));
   std::vector<double> wA[4] = {vsqrt(w_l(s, ix, iy)), sAB, sAC, vsqrt(w_d(s, ix, iy))};
@@ -1559,13 +1570,8 @@ double plaquette_energy(State& s, int ix, int iy, double g) {
   scale_legs(c, TA, wA);
   scale_legs(c, TB, wB);
   scale_legs(c, TC, wC);
-  // single-site Wilson-loop factors: A Ud on R and U on U, B Ud on U, C U on R
-  Mat U = op_U_gauge(gd), Ud = U;
-  for (int i = 0; i < gd; ++i)
-    for (int j = 0; j < gd; ++j) Ud(i, j) = U(j, i);
-  Mat WA = kron(kron(identity(2), Ud), U);
-  Mat WB = sB.dR == 1 ? kron(identity(2), Ud) : kron(kron(identity(2), identity(sB.dR)), Ud);
-  Mat WC = sC.dU == 1 ? kron(identity(2), U) : kron(kron(identity(2), U), identity(sC.dU));
+  Mat WA, WB, WC;
+  wilson_factors(nx, ny, gd, ix, iy, &WA, &WB, &WC);
   auto apply = [&](const Tensor& T, const Mat& W) {
     const int64_t dp = T.dims[0];
     PEPS_CHECK(W.n == dp, PS_ESHAPE, "plaquette_energy: operator size");
@@ -1646,6 +1652,43 @@ double total_energy(State& s, double g, double t, double m) {
   for (int iy = 1; iy < ny; ++iy)
     for (int ix = 1; ix < nx; ++ix) E += plaquette_energy(s, ix, iy, g);
   return E;
+}
+
+// Double layer with W on the ket's physical leg (layout of dl_traced).
+static Tensor dl_with_op(Ctx& c, const Tensor& T, const Mat& W, bool cplx) {
+  const int64_t dp = T.dims[0];
+  PEPS_CHECK(W.n == dp, PS_ESHAPE, "plaquette: operator size");
+  Tensor TO = matmul(c, gate_tensor(c, W, cplx), T.reshaped({dp, T.numel() / dp})).reshaped(T.dims);
+  const int64_t Dl = T.dims[1], Dr = T.dims[2], Du = T.dims[3], Dd = T.dims[4];
+  Tensor E8 = contract(c, TO, "pabcd", T, "pefgh", "eafbgchd", false, true);
+  return E8.reshaped({Dl * Dl, Dr * Dr, Du * Du, Dd * Dd});
+}
+
+cplx plaquette_expect(State& s, int ix, int iy, int chi) {
+  // <W> from boundary MPS: the two-row strip (rows iy, iy+1) between
+  // boundary_top(iy+1) and boundary_bottom(iy), closed column by column
+  Ctx& c = *s.c;
+  const int nx = s.nx, ny = s.ny;
+  PEPS_CHECK(ix >= 1 && ix < nx && iy >= 1 && iy < ny, PS_EARG, "plaquette_expect: needs ix < nx and iy < ny");
+  MPS top = boundary_top(s, iy + 1, chi), bot = boundary_bottom(s, iy, chi);
+  MPS r0 = row_dl(s, iy), r1 = row_dl(s, iy + 1);
+  Mat WA, WB, WC;
+  wilson_factors(nx, ny, s.gd, ix, iy, &WA, &WB, &WC);
+  MPS w0 = r0, w1 = r1;
+  w0[ix - 1] = dl_with_op(c, sqrt_absorbed(s, ix, iy), WA, s.cplx);
+  w0[ix] = dl_with_op(c, sqrt_absorbed(s, ix + 1, iy), WB, s.cplx);
+  w1[ix - 1] = dl_with_op(c, sqrt_absorbed(s, ix, iy + 1), WC, s.cplx);
+  auto close = [&](const MPS& a, const MPS& b) {
+    Tensor L = ones_tensor(c, 6, s.cplx);
+    for (int j = 1; j <= nx; ++j) L = spectator_left_v(c, L, top[j - 1], bot[j - 1], a[j - 1], b[j - 1]);
+    PEPS_CHECK(L.numel() == 1, PS_ESHAPE, "plaquette_expect: open legs at the right edge");
+    auto h = c.to_host(L);
+    return s.cplx ? cplx{h[0], h[1]} : cplx{h[0], 0.0};
+  };
+  const cplx num = close(w0, w1), den = close(r0, r1);
+  const double dd = den.re * den.re + den.im * den.im;
+  PEPS_CHECK(dd > 0 && std::isfinite(dd), PS_ESOLVER, "plaquette_expect: <psi|psi> is zero or not finite");
+  return cplx{(num.re * den.re + num.im * den.im) / dd, (num.im * den.re - num.re * den.im) / dd};
 }
 
 double bond_entropy(const std::vector<double>& lam) {

@@ -170,6 +170,9 @@ double expect_site(State& s, int ix, int iy, const std::vector<double>& O, bool 
 // plaquette_energy (:589-684) and total_energy (:690-805)
 double plaquette_energy(State& s, int ix, int iy, double g);
 double total_energy(State& s, double g, double t, double m);
+// <W> on the plaquette with lower-left site (ix, iy) from boundary MPS
+// (two-row strip; the boundary-MPS counterpart of plaquette_energy)
+cplx plaquette_expect(State& s, int ix, int iy, int chi);
 // bond_entropy and measure_all_finite (finite_peps_quench.jl:121-194)
 double bond_entropy(const std::vector<double>& lam);
 struct Measurements {

@@ -94,6 +94,7 @@ _SIGS = {
     "peps_expect_site": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp, _dp]),
     "peps_plaquette_energy": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double, _dp]),
     "peps_total_energy": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
+    "peps_plaquette_expect": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp]),
     "peps_bond_entropy": (C.c_int32, [_dp, C.c_int64, _dp]),
     "peps_measure_all": (C.c_int32, [_vp, _dp, _dp, _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
@@ -466,6 +467,13 @@ def total_energy(gp: GaugePEPS, g, t_hop, m):
 
 
 # ── finite_peps_quench.jl (measurements) ────────────────────────────────────
+
+def plaquette_expect(gp: GaugePEPS, ix, iy, chi) -> complex:
+    """<W> on the plaquette with lower-left site (ix, iy) from boundary MPS."""
+    out = np.zeros(2)
+    gp.ctx.check(lib().peps_plaquette_expect(gp.h, ix, iy, chi, _dptr(out)))
+    return complex(out[0], out[1])
+
 
 def bond_entropy(lam) -> float:
     """bond_entropy (finite_peps_quench.jl:121)."""

@@ -867,6 +867,41 @@ def bond_environment_v(gp, ix, iy, chi=64, top=None, bot=None):
     return dict(env=env, RA=RA, RB=RB, QA=QA, QB=QB)
 
 
+def dl_with_op(T, Op):
+    """Double layer with Op on the ket's physical leg, layout of dl_traced:
+    E = sum_{p,q} Op[p,q] T[q,...] conj(T[p,...]) (bra fast)."""
+    TO = np.einsum("pq,qabcd->pabcd", Op, T, optimize=True)
+    dp, Dl, Dr, Du, Dd = T.shape
+    E8 = np.einsum("pabcd,pefgh->eafbgchd", TO, T.conj(), optimize=True)
+    return jr(E8, Dl * Dl, Dr * Dr, Du * Du, Dd * Dd)
+
+
+def plaquette_expect(gp, ix, iy, chi=64):
+    """<W> = <psi|Ud_b U_l Ud_r U_t|psi> / <psi|psi> on the plaquette with lower-left
+    site (ix, iy), from boundary MPS: the two-row strip (rows iy, iy+1) between
+    boundary_top(iy+1) and boundary_bottom(iy), W's site factors on the ket
+    layer of A, B, C, closed column by column with spectator_left_v.  No
+    reference function: the boundary-MPS counterpart of plaquette_energy
+    (finite_peps_ground_state.jl:589-684), which uses simple-update weights."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    assert 1 <= ix < nx and 1 <= iy < ny
+    top, bot = boundary_top(gp, iy + 1, chi), boundary_bottom(gp, iy, chi)
+    row0, row1 = row_dl(gp, iy), row_dl(gp, iy + 1)
+    WA, WB, WC = wilson_site_factors(nx, ny, gd, ix, iy)
+    w0, w1 = list(row0), list(row1)
+    w0[ix - 1] = dl_with_op(sqrt_absorbed_site(gp, ix, iy), WA)
+    w0[ix] = dl_with_op(sqrt_absorbed_site(gp, ix + 1, iy), WB)
+    w1[ix - 1] = dl_with_op(sqrt_absorbed_site(gp, ix, iy + 1), WC)
+
+    def close(r0, r1):
+        L = np.ones((1,) * 6, np.complex128)
+        for j in range(1, nx + 1):
+            L = spectator_left_v(L, top[j - 1], bot[j - 1], r0[j - 1], r1[j - 1])
+        return L.reshape(-1)[0]
+
+    return close(w0, w1) / close(row0, row1)
+
+
 # ════════════════════════════════════════════════════════════════════════════
 #  Full update v4 (finite_peps_fullupdate_v4.jl)
 # ════════════════════════════════════════════════════════════════════════════
@@ -1372,6 +1407,20 @@ def expect_site(gp, ix, iy, O):
     return num / den if den > 1e-15 else 0.0
 
 
+def wilson_site_factors(nx, ny, gd, ix, iy):
+    """Single-site factors of W = Ud_bottom U_left Ud_right U_top on the L-shape
+    (finite_peps_ground_state.jl:632-641): A=(ix,iy) carries Ud on its R link and
+    U on its U link, B=(ix+1,iy) Ud on its U link, C=(ix,iy+1) U on its R link."""
+    _, gRB, _ = site_dims(ix + 1, iy, nx, ny, gd)
+    _, _, gUC = site_dims(ix, iy + 1, nx, ny, gd)
+    Ug = op_U_gauge(gd)
+    Ud = Ug.conj().T
+    WA = np.kron(np.kron(_I(2), Ud), Ug)
+    WB = np.kron(_I(2), Ud) if gRB == 1 else np.kron(np.kron(_I(2), _I(gRB)), Ud)
+    WC = np.kron(_I(2), Ug) if gUC == 1 else np.kron(np.kron(_I(2), Ug), _I(gUC))
+    return WA, WB, WC
+
+
 def plaquette_energy(gp, ix, iy, g):
     """plaquette_energy, finite_peps_ground_state.jl:589-684: -(1/g^2) Re<W> on
     the L-shape A=(ix,iy), B=(ix+1,iy), C=(ix,iy+1) with sqrt(lambda) on
@@ -1387,11 +1436,7 @@ def plaquette_energy(gp, ix, iy, g):
                  sw("v", ix + 1, iy - 1, iy > 1))
     TC = _scale5(gp.T(ix, iy + 1), sw("h", ix - 1, iy + 1, ix > 1), sw("h", ix, iy + 1, ix < nx),
                  sw("v", ix, iy + 1, iy + 1 < ny), sq_AC)
-    Ug = op_U_gauge(gd)
-    Ud = Ug.conj().T
-    WA = np.kron(np.kron(_I(2), Ud), Ug)
-    WB = np.kron(_I(2), Ud) if gRB == 1 else np.kron(np.kron(_I(2), _I(gRB)), Ud)
-    WC = np.kron(_I(2), Ug) if gUC == 1 else np.kron(np.kron(_I(2), Ug), _I(gUC))
+    WA, WB, WC = wilson_site_factors(nx, ny, gd, ix, iy)
     app = lambda T, O: jr(O @ jr(T, T.shape[0], T.size // T.shape[0]), *T.shape)
     MA = lambda k, b: np.einsum("plrud,plsvd->rsuv", k, b.conj(), optimize=True)
     MB = lambda k, b: np.einsum("plrud,pmrud->lm", k, b.conj(), optimize=True)

@@ -170,3 +170,18 @@ def test_measure_all_finite(ctx, cplx):
     np.testing.assert_allclose(mg["E2_grid"], mo["E2_grid"], atol=1e-12)
     lam = go.LH(1, 1)
     assert abs(peps.bond_entropy(lam) - O.bond_entropy(lam)) < 1e-14
+
+
+@pytest.mark.parametrize("nx,ny,gd,D,chi,cplx", [(3, 3, 2, 2, 64, False), (3, 3, 2, 2, 3, False),
+                                                (3, 4, 2, 3, 12, False), (2, 3, 3, 2, 64, True)])
+def test_plaquette_expect(ctx, nx, ny, gd, D, chi, cplx):
+    """Boundary-MPS <W> on every plaquette against the oracle (exact and
+    truncated chi)."""
+    from lgt_amd import peps
+    go = random_peps(nx, ny, gd, D, 61, cplx)
+    gdv = to_device(go, ctx)
+    for iy in range(1, ny):
+        for ix in range(1, nx):
+            wo = O.plaquette_expect(go, ix, iy, chi)
+            wg = peps.plaquette_expect(gdv, ix, iy, chi)
+            assert abs(wg - wo) < 1e-10 * max(1.0, abs(wo)), (ix, iy, wg, wo)

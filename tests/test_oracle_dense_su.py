@@ -279,3 +279,34 @@ def test_measurements_on_basis_product_states():
     assert abs(m["nf_mean"] - nf_exp.mean()) < 1e-14 and m["S_mean"] == 0.0 and m["t"] == 0.5
     even = [nf_exp[ix - 1, iy - 1] for iy in range(1, ny + 1) for ix in range(1, nx + 1) if (ix + iy) % 2 == 0]
     assert abs(m["nf_even"] - np.mean(even)) < 1e-14
+
+
+def _wilson_matrix(gd):
+    """W = Ud (x) U (x) Ud (x) U on the links (A.R, A.U, B.U, C.R), flat index
+    column-major over them (A.R fastest), built factor by factor."""
+    U = O.op_U_gauge(gd)
+    Ud = U.conj().T
+    W8 = np.einsum("ae,bf,cg,dh->abcdefgh", Ud, U, Ud, U)
+    return O.jr(W8, gd ** 4, gd ** 4)
+
+
+@pytest.mark.parametrize("nx,ny,gd,D,cplx", [(2, 2, 2, 2, False), (3, 2, 2, 2, True), (3, 3, 2, 2, False),
+                                            (2, 3, 3, 2, True)])
+def test_plaquette_expect_against_dense(nx, ny, gd, D, cplx):
+    """Boundary-MPS <W> at exact chi equals <psi|W|psi>/<psi|psi> of the dense
+    state with W applied link by link (an independent embedding), for every
+    plaquette; also -(1/g^2) Re<W> matches plaquette_energy on a D=1 state."""
+    gp = _random_peps(nx, ny, gd, D, 12, cplx)
+    psi = O.dense_psi(gp)
+    nrm = np.vdot(psi, psi)
+    Wm = _wilson_matrix(gd)
+    for iy in range(1, ny):
+        for ix in range(1, nx):
+            expect = np.vdot(psi, _apply_links(psi, gp, ix, iy, Wm)) / nrm
+            got = O.plaquette_expect(gp, ix, iy, chi=4096)
+            assert abs(got - expect) < 1e-12 * max(1.0, abs(expect)), (ix, iy, got, expect)
+    g1 = _random_peps(nx, ny, gd, 1, 3, cplx)
+    for iy in range(1, ny):
+        for ix in range(1, nx):
+            e_su = O.plaquette_energy(g1, ix, iy, 1.0)
+            assert abs(-O.plaquette_expect(g1, ix, iy).real - e_su) < 1e-12
