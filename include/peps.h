@@ -181,6 +181,15 @@ int32_t peps_fu_bond_v4(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
    g, t_hop, m, tau and D_max as D_trunc) or, for PEPS_STEP_FULL_V2,
    trotter_step_full! (finite_peps_full_update.jl:550-575, use_gauss=false) */
 int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p);
+/* One peps_trotter_step on each of n independent states at once (no
+   reference counterpart: the reference runs independent instances as a
+   SLURM array).  Every state must belong to its own context: each context's
+   stream runs in its own host thread, so the latency-bound factorisations of
+   one instance overlap the work of the others on the same GPU.  status[i]
+   (optional) receives each state's result; the return value is the first
+   non-OK status, or PEPS_EARG for shared contexts. */
+int32_t peps_batch_trotter_step(peps_state* const* states, int32_t n, int32_t kind, const peps_params* p,
+                                int32_t* status);
 /* ALS statistics of the last PEPS_STEP_V4 step: bonds, accepted, total sweeps */
 int32_t peps_last_als_stats(peps_state* st, int64_t out[3]);
 /* compute_energy_v4 (finite_peps_fullupdate_v4.jl:390): total energy */

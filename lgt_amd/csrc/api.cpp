@@ -2,6 +2,8 @@
 #include "peps.h"
 
 #include <cstring>
+#include <set>
+#include <thread>
 
 #include "engine.h"
 
@@ -491,6 +493,36 @@ int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p) {
       throw peps::Error(peps::PS_EARG, "unknown step kind");
     }
   });
+}
+
+int32_t peps_batch_trotter_step(peps_state* const* states, int32_t n, int32_t kind, const peps_params* p,
+                                int32_t* status) {
+  if (!states || n < 1 || !p) return PEPS_EARG;
+  std::set<const peps::Ctx*> ctxs;
+  for (int32_t i = 0; i < n; ++i) {
+    if (!states[i]) return PEPS_EARG;
+    ctxs.insert(states[i]->ref.get());
+  }
+  if (int32_t(ctxs.size()) != n) return PEPS_EARG;  // one context (stream, handles, pool) per state
+  std::vector<int32_t> st(size_t(n), PEPS_OK);
+  std::vector<std::thread> workers;
+  workers.reserve(size_t(n));
+  for (int32_t i = 0; i < n; ++i)
+    workers.emplace_back([&, i] {
+      // the device is a per-thread setting
+      if (hipSetDevice(states[i]->ref->device) != hipSuccess) {
+        st[size_t(i)] = PEPS_EHIP;
+        return;
+      }
+      st[size_t(i)] = peps_trotter_step(states[i], kind, p);
+    });
+  for (auto& w : workers) w.join();
+  int32_t first = PEPS_OK;
+  for (int32_t i = 0; i < n; ++i) {
+    if (status) status[i] = st[size_t(i)];
+    if (first == PEPS_OK) first = st[size_t(i)];
+  }
+  return first;
 }
 
 int32_t peps_last_als_stats(peps_state* st, int64_t out[3]) {

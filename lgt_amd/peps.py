@@ -83,6 +83,7 @@ _SIGS = {
     "peps_su_bond": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32]),
     "peps_fu_bond_v4": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, C.c_int32]),
     "peps_trotter_step": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params)]),
+    "peps_batch_trotter_step": (C.c_int32, [C.POINTER(_vp), C.c_int32, C.c_int32, C.POINTER(Params), _i32p]),
     "peps_last_als_stats": (C.c_int32, [_vp, _i64p]),
     "peps_energy_v4": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
     "peps_su_bond_dense": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32]),
@@ -633,6 +634,24 @@ def trotter_step_v4(gp: GaugePEPS, g, t_hop, m, tau, D_max, chi=64, n_als=30, al
     st = np.zeros(3, np.int64)
     gp.ctx.check(lib().peps_last_als_stats(gp.h, _iptr(st)))
     return dict(bonds=int(st[0]), accepted=int(st[1]), sweeps=int(st[2]))
+
+
+def trotter_step_batch(states, kind, g, t_hop, m, tau, D_max, chi=64, n_als=30, als_tol=0.0, renorm=False,
+                       positive_cost=False):
+    """One Trotter step (kind PEPS_STEP_V4 / _GAUGE / _DENSE / _FULL_V2) on
+    each independent state at once, every state on its own Context.  Returns
+    the per-state status codes; raises on the first failure."""
+    flags = (PEPS_STEP_RENORM if renorm else 0) | (PEPS_STEP_POSITIVE_COST if positive_cost else 0)
+    p = _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol, flags)
+    n = len(states)
+    hs = (_vp * n)(*[gp.h for gp in states])
+    st = np.zeros(n, np.int32)
+    rc = lib().peps_batch_trotter_step(hs, n, kind, C.byref(p), st.ctypes.data_as(_i32p))
+    if rc:
+        bad = [i for i in range(n) if st[i]]
+        i = bad[0] if bad else 0
+        raise PepsError(rc, f"state {i}: " + (lib().peps_last_error(states[i].ctx.h) or b"").decode())
+    return st
 
 
 def compute_energy_v4(gp: GaugePEPS, g, t_hop, m, chi=64, renorm=False) -> float:

@@ -822,9 +822,12 @@ def spectator_left_v(Lenv, topj, botj, Eiy, Eiy1):
     e1 = split_dl(Eiy1, Dl1, Dr1, Du1, Dd1)
     tp = split_mps(topj, Du1)
     bt = split_mps(botj, Dd)
-    # Lenv[tL,bL,liyk,liyb,liy1k,liy1b]
-    return np.einsum("ABkbmn,BedF,bkxyvued,nmzwsrvu,AsrC->CFyxwz",
-                     Lenv, bt, e0, e1, tp, optimize=True)
+    # Lenv[tL,bL,liyk,liyb,liy1k,liy1b]; the five-tensor sum taken pairwise in a
+    # fixed order (numpy's greedy path can build a huge intermediate here)
+    x = np.einsum("ABkbmn,BedF->AkbmnedF", Lenv, bt, optimize=True)
+    x = np.einsum("AkbmnedF,bkxyvued->AmnFxyvu", x, e0, optimize=True)
+    x = np.einsum("AmnFxyvu,nmzwsrvu->AFxyzwsr", x, e1, optimize=True)
+    return np.einsum("AFxyzwsr,AsrC->CFyxwz", x, tp, optimize=True)
 
 
 def spectator_right_v(Renv, topj, botj, Eiy, Eiy1):
@@ -835,8 +838,10 @@ def spectator_right_v(Renv, topj, botj, Eiy, Eiy1):
     e1 = split_dl(Eiy1, Dl1, Dr1, Du1, Dd1)
     tp = split_mps(topj, Du1)
     bt = split_mps(botj, Dd)
-    return np.einsum("CFyxwz,BedF,bkxyvued,nmzwsrvu,AsrC->ABkbmn",
-                     Renv, bt, e0, e1, tp, optimize=True)
+    x = np.einsum("CFyxwz,BedF->CyxwzBed", Renv, bt, optimize=True)
+    x = np.einsum("CyxwzBed,bkxyvued->CwzBbkvu", x, e0, optimize=True)
+    x = np.einsum("CwzBbkvu,nmzwsrvu->CBbknmsr", x, e1, optimize=True)
+    return np.einsum("CBbknmsr,AsrC->ABkbmn", x, tp, optimize=True)
 
 
 def bond_environment_v(gp, ix, iy, chi=64, top=None, bot=None):

@@ -185,3 +185,34 @@ def test_plaquette_expect(ctx, nx, ny, gd, D, chi, cplx):
             wo = O.plaquette_expect(go, ix, iy, chi)
             wg = peps.plaquette_expect(gdv, ix, iy, chi)
             assert abs(wg - wo) < 1e-10 * max(1.0, abs(wo)), (ix, iy, wg, wo)
+
+
+def test_batch_trotter_step_matches_sequential(ctx):
+    """Independent instances stepped concurrently (one context and host
+    thread each) give the same states as stepping them one by one; states
+    sharing a context are refused."""
+    import time
+    from lgt_amd import peps
+    from tests._util import su_state
+    specs = [(3, 3, 2, 2, 11), (3, 4, 2, 3, 3), (4, 3, 2, 2, 7), (3, 3, 2, 3, 5)]
+    olds = [su_state(nx, ny, gd, D, 3, seed) for nx, ny, gd, D, seed in specs]
+    ctxs = [peps.Context(0) for _ in specs]
+    batch = [to_device(go, c) for go, c in zip(olds, ctxs)]
+    seq = [to_device(go, ctx) for go in olds]
+    t0 = time.perf_counter()
+    st = peps.trotter_step_batch(batch, peps.PEPS_STEP_V4, G, T_HOP, M, 0.05, 3, 12, n_als=2)
+    t_batch = time.perf_counter() - t0
+    assert list(st) == [0] * len(specs)
+    t0 = time.perf_counter()
+    for gp in seq:
+        peps.trotter_step_v4(gp, G, T_HOP, M, 0.05, 3, 12, n_als=2)
+    t_seq = time.perf_counter() - t0
+    print(f"batch of {len(specs)}: {t_batch:.3f} s, one by one: {t_seq:.3f} s", flush=True)
+    for gb, gs in zip(batch, seq):
+        a, b = to_oracle(gb), to_oracle(gs)
+        assert _bond_dims(a) == _bond_dims(b)
+        for iy in range(1, a.ny + 1):
+            for ix in range(1, a.nx + 1):
+                assert rel(a.T(ix, iy), b.T(ix, iy)) < 1e-12
+    with pytest.raises(peps.PepsError):
+        peps.trotter_step_batch([seq[0], seq[1]], peps.PEPS_STEP_V4, G, T_HOP, M, 0.05, 3, 12, n_als=2)
