@@ -147,7 +147,11 @@ def summarize_rocprof(d, log_path=None):
 
     def gname(name):
         m = re.search(r"gemm_kernel<(\w+), (\d+),", name)
-        return "gemm_kernel PREC=" + m.group(2) if m else re.sub(r"\(.*", "", name).replace("void ", "")[:80]
+        if m:
+            return "gemm_kernel PREC=" + m.group(2)
+        if "gemm_f32_tiled" in name:
+            return "gemm_f32_tiled"
+        return re.sub(r"\(.*", "", name).replace("void ", "")[:80]
 
     win = None
     for lf in ([log_path] if log_path else []):
