@@ -336,8 +336,11 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
   Tensor Q = qr_thin(c, have ? Q0 : matmul(c, rh, Q0)).Q;
   std::vector<double> theta;
   Tensor X;
-  constexpr int kCheb = 8;  // filter degree per outer iteration
-  for (int it = 0; it < 40; ++it) {
+  // filter degree per outer iteration: one product rho Y per degree is cheap
+  // next to the QR and Rayleigh-Ritz of an outer iteration, so a high degree
+  // pays; T_16 on the interval floor 1e-10 theta_0 stays below ~1e165
+  constexpr int kCheb = 16, kMaxIt = 25;
+  for (int it = 0; it < kMaxIt; ++it) {
     // Rayleigh-Ritz on span(Q)
     Tensor Z = matmul(c, rh, Q);
     Tensor T = matmul(c, Q, Z, true, false, true, false);
@@ -361,14 +364,14 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
     const double res = std::sqrt(norm2sq(c, R));
     c.count("top_eigs[iterations]", 1.0, 0);
     if (res <= 1e-12 * std::max(std::fabs(theta[0]), 1e-300)) break;
-    if (it == 39) {
+    if (it == kMaxIt - 1) {
       c.count("top_eigs[unconverged]", 1.0, 0);
       break;
     }
     // Chebyshev filter of degree kCheb on the unwanted interval [0, theta_min]
     // applied to the Ritz vectors (Zhou & Saad): eigencomponents above the
     // interval grow like T_k, those inside stay bounded by 1.
-    // (the interval end is kept >= 1e-10 theta_0 so that T_8 stays far from overflow)
+    // (the interval end is kept >= 1e-10 theta_0 so that T_16 stays far from overflow)
     const double hi = std::max(theta[p - 1], 1e-10 * std::max(theta[0], 1e-300));
     const double e = 0.5 * hi, c0 = 0.5 * hi;
     Tensor Y0 = X;

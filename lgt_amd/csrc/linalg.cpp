@@ -7,6 +7,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "peps_internal.h"
 
@@ -207,7 +208,12 @@ static SVD svd_dense(Ctx& c, const Tensor& A) {
     W = copy_tensor(c, B);
   }
   Tensor V = c.alloc({nb, nb}, A.cplx), sg = c.alloc({nb}, false);
-  jacobi_svd(c, W, V, sg, 1e-15, 80);
+  // convergence threshold of LAPACK's dgesvj, sqrt(m) eps on the cosine of a
+  // column pair (a fixed 1e-15 lies below the rounding floor for m > ~80 and
+  // keeps pairs rotating until the sweep limit)
+  const double tol = std::max(1e-15, std::sqrt(double(W.dims[0])) * 1.1102230246251565e-16);
+  const int sweeps = jacobi_svd(c, W, V, sg, tol, 80);
+  c.count("svd[sweeps]", double(sweeps), 0);
   std::vector<double> s = c.to_host(sg);
   std::vector<int64_t> ord(nb);
   for (int64_t i = 0; i < nb; ++i) ord[i] = i;
