@@ -43,7 +43,7 @@ GEMM_KERNEL = {"fp64": "gemm_kernel (FP64 MFMA)", "fp32": "gemm_f32_tiled (FP32 
 # FETCH_SIZE and WRITE_SIZE over the timed step of this workload (copy-
 # calibrated factors 2.0 and 1.0), keyed by (nx, ny, D, chi, gd, tier);
 # profiles/ holds the summary.
-PROFILED_TRAFFIC = {}
+PROFILED_TRAFFIC = {(8, 8, 6, 144, 2, "fp32"): (2.0 * 2.2424e6 + 1.0 * 5.4983e5) * 1024.0}
 
 
 def log(msg):
@@ -221,6 +221,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--energy", action="store_true", help="also report E/site after the timed steps")
+    ap.add_argument("--compare-fp64", action="store_true",
+                    help="after the timed steps, run the same steps from the same initial state in the FP64 "
+                         "tier (the plain FP64 GPU form) and report its step time, the speedup and |dE/site|")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (the known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -263,6 +266,7 @@ def main():
     bond_dims = {"mean": float(np.mean(bd)), "max": int(max(bd)), "at_D": int(sum(1 for x in bd if x == a.D)),
                  "bonds": len(bd)}
     log(f"bond dimensions: {bond_dims}")
+    start = gp.to_arrays() if a.compare_fp64 else None
     ctx.set_tier(a.tier)
     for w in range(a.warmup):
         tw = time.perf_counter()
@@ -329,9 +333,27 @@ def main():
                      "algorithmic_flop_per_launch": gk["flops"] / max(1, gk["launches"]),
                      "algorithmic_bytes_per_launch": gk["bytes"] / max(1, gk["launches"])},
     }
-    if a.energy:
+    if a.energy or a.compare_fp64:
         e = peps.compute_energy_v4(gp, g, t, m, a.chi, renorm=True)
         out["config"]["energy_per_site"] = e / (a.nx * a.ny)
+        log(f"E/site after the timed steps ({a.tier}): {e / (a.nx * a.ny):.12f}")
+    if a.compare_fp64:
+        g64 = peps.GaugePEPS.from_arrays(start["tensors"], start["lh"], start["lv"], start["qh"], start["qv"],
+                                         gp.gch, gp.gd, is_complex=gp.is_complex, ctx=ctx)
+        ctx.set_tier("fp64")
+        ctx.sync()
+        t64 = time.perf_counter()
+        for s64 in range(a.warmup + a.steps):
+            peps.trotter_step_v4(g64, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
+                                 positive_cost=True)
+            ctx.sync()
+            log(f"fp64 step {s64 + 1}/{a.warmup + a.steps}: {time.perf_counter() - t64:.2f} s")
+        ms64 = 1e3 * (time.perf_counter() - t64) / (a.warmup + a.steps)
+        e64 = peps.compute_energy_v4(g64, g, t, m, a.chi, renorm=True)
+        log(f"E/site after the fp64 steps: {e64 / (a.nx * a.ny):.12f}")
+        out["config"]["fp64_gpu"] = {"ms_per_step": ms64, "speedup": ms64 / out["ms_per_step"],
+                                     "energy_per_site": e64 / (a.nx * a.ny),
+                                     "abs_dE_per_site": abs(e - e64) / (a.nx * a.ny)}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(total_flops / a.steps)
     print(json.dumps(out), flush=True)
