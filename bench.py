@@ -343,17 +343,22 @@ def main():
         ctx.set_tier("fp64")
         ctx.sync()
         t64 = time.perf_counter()
+        als64 = dict(bonds=0, accepted=0, sweeps=0)
         for s64 in range(a.warmup + a.steps):
-            peps.trotter_step_v4(g64, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
-                                 positive_cost=True)
+            st64 = peps.trotter_step_v4(g64, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
+                                        positive_cost=True)
+            for k in als64:
+                als64[k] += st64[k]
             ctx.sync()
-            log(f"fp64 step {s64 + 1}/{a.warmup + a.steps}: {time.perf_counter() - t64:.2f} s")
+            log(f"fp64 step {s64 + 1}/{a.warmup + a.steps}: {time.perf_counter() - t64:.2f} s  als={st64}")
         ms64 = 1e3 * (time.perf_counter() - t64) / (a.warmup + a.steps)
         e64 = peps.compute_energy_v4(g64, g, t, m, a.chi, renorm=True)
         log(f"E/site after the fp64 steps: {e64 / (a.nx * a.ny):.12f}")
         out["config"]["fp64_gpu"] = {"ms_per_step": ms64, "speedup": ms64 / out["ms_per_step"],
                                      "energy_per_site": e64 / (a.nx * a.ny),
-                                     "abs_dE_per_site": abs(e - e64) / (a.nx * a.ny)}
+                                     "abs_dE_per_site": abs(e - e64) / (a.nx * a.ny),
+                                     "als_sweeps_per_step": als64["sweeps"] / max(1, a.warmup + a.steps),
+                                     "als_accept_rate": als64["accepted"] / max(1, als64["bonds"])}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(total_flops / a.steps)
     print(json.dumps(out), flush=True)

@@ -35,6 +35,7 @@ extern "C" {
 typedef struct peps_ctx peps_ctx;
 typedef struct peps_state peps_state;
 typedef struct peps_tensor_list peps_tensor_list;
+typedef struct peps_mlp peps_mlp;
 
 enum {
   PEPS_OK = 0,
@@ -168,6 +169,18 @@ int32_t peps_als_truncate(peps_ctx* ctx, int32_t is_complex, const int64_t th_di
                           const double* theta, const double* env, int32_t D_max,
                           const int64_t* eR_of_pL, int32_t n_als, double reg, int64_t* Dp,
                           int64_t* bond_flux, double info[3], peps_tensor_list** out);
+/* als_truncate with an optional initial bond and a tolerance (no reference
+   counterpart for these two): X0 (nA, dpL, Dp) and Y0 (nB, dpR, Dp), both
+   NULL for the reference's SVD initialisation, otherwise shaped like it
+   (the mask of X is applied); als_tol > 0 stops when the relative cost
+   change of a sweep is below it.  The accept rule is the reference's: the
+   result is kept only if its cost is below that of the SVD initialisation.
+   info = cost0 (SVD init), cost1, accepted, sweeps run. */
+int32_t peps_als_truncate_init(peps_ctx* ctx, int32_t is_complex, const int64_t th_dims[4],
+                               const double* theta, const double* env, int32_t D_max,
+                               const int64_t* eR_of_pL, int32_t n_als, double reg, double als_tol,
+                               const double* X0, const double* Y0, int64_t* Dp, int64_t* bond_flux,
+                               double info[4], peps_tensor_list** out);
 /* simple_update_bond_h! / _v! (finite_peps_gauge_invariant.jl:296,412);
    gate: (dL*dR)^2, kron(L,R) ordering (right site fast) */
 int32_t peps_su_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
@@ -245,6 +258,30 @@ int32_t peps_bond_entropy(const double* lambda, int64_t n, double* out);
    stamp: out = nf_mean, nf_even, nf_odd, E2_mean, S_h_mean, S_v_mean,
    S_mean; nf_grid / E2_grid (nx x ny, ix fastest) may be NULL */
 int32_t peps_measure_all(peps_state* st, double out[7], double* nf_grid, double* E2_grid);
+
+/* ── learned ALS warm start (no counterpart in the reference) ──────────────
+   A multilayer perceptron in -> width -> width -> out (ReLU hidden layers,
+   linear output) trained and evaluated on the GPU by the library's own
+   kernels (products in the FP32 MFMA tier).  The batch runs along columns:
+   X (n_in x batch), Y and T (n_out x batch), column-major.  Parameters are
+   flattened as W1 (width x n_in), b1, W2 (width x width), b2, W3 (n_out x
+   width), b3; loss = 1/(2 batch) ||Y - T||^2; training uses Adam (0.9,
+   0.999, 1e-8).  It proposes the initial bond of the ALS solve
+   (finite_peps_fullupdate_v4.jl:99-111 is the slot it replaces); the
+   reference's SVD initialisation stays the fallback. */
+int32_t peps_mlp_create(peps_ctx* ctx, int32_t n_in, int32_t width, int32_t n_out, uint64_t seed,
+                        peps_mlp** out);
+int32_t peps_mlp_destroy(peps_mlp* m);
+int32_t peps_mlp_param_count(const peps_mlp* m, int64_t* n);
+int32_t peps_mlp_get_params(peps_mlp* m, double* out);
+int32_t peps_mlp_set_params(peps_mlp* m, const double* in);
+int32_t peps_mlp_forward(peps_mlp* m, int64_t batch, const double* X, double* Y);
+/* loss and the gradient of every parameter (flattened as above) */
+int32_t peps_mlp_gradients(peps_mlp* m, int64_t batch, const double* X, const double* T, double* loss,
+                           double* grad);
+/* one Adam step; loss = the loss before the step */
+int32_t peps_mlp_train_step(peps_mlp* m, int64_t batch, const double* X, const double* T, double lr,
+                            double* loss);
 
 /* The contraction GEMM itself (no counterpart in the reference; exposed so
    callers can validate the MFMA path of the current tier):

@@ -6,6 +6,7 @@
 #include <thread>
 
 #include "engine.h"
+#include "mlp.h"
 
 // States and result lists hold a reference to the context, so device
 // buffers always return to a live allocator whatever order the caller
@@ -20,6 +21,13 @@ struct peps_state {
   peps::State s;
   peps::GateCache gc;
   int64_t als_bonds = 0, als_accepted = 0, als_sweeps = 0;
+};
+
+struct peps_mlp {
+  std::shared_ptr<peps::Ctx> ref;
+  peps::Mlp m;
+  peps_mlp(std::shared_ptr<peps::Ctx> c, int nin, int w, int nout, uint64_t seed)
+      : ref(std::move(c)), m(*ref, nin, w, nout, seed) {}
 };
 
 struct peps_tensor_list {
@@ -433,6 +441,43 @@ int32_t peps_als_truncate(peps_ctx* ctx, int32_t is_complex, const int64_t th_di
   });
 }
 
+int32_t peps_als_truncate_init(peps_ctx* ctx, int32_t is_complex, const int64_t th_dims[4],
+                               const double* theta, const double* env, int32_t D_max,
+                               const int64_t* eR_of_pL, int32_t n_als, double reg, double als_tol,
+                               const double* X0, const double* Y0, int64_t* Dp, int64_t* bond_flux,
+                               double info[4], peps_tensor_list** out) {
+  if (!ctx || !th_dims || !theta || !env || !eR_of_pL || !Dp || !bond_flux || !out || D_max < 1 || n_als < 0 ||
+      als_tol < 0.0 || (!X0) != (!Y0))
+    return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    bool cp = is_complex != 0;
+    int64_t nA = th_dims[0], nB = th_dims[3];
+    Tensor th = ctx->c->upload(std::vector<int64_t>(th_dims, th_dims + 4), cp, theta);
+    Tensor en = ctx->c->upload({nA, nB, nA, nB}, cp, env);
+    std::vector<int64_t> eR(eR_of_pL, eR_of_pL + th_dims[1]);
+    Tensor X, Y, Xi, Yi;
+    std::vector<int64_t> bf;
+    peps::ALSInfo inf;
+    if (X0) {
+      // the warm start has the shapes of the SVD initialisation: find them first
+      peps::als_truncate(*ctx->c, th, en, D_max, eR, 0, reg, 0.0, &X, &Y, &bf, &inf);
+      Xi = ctx->c->upload(X.dims, cp, X0);
+      Yi = ctx->c->upload(Y.dims, cp, Y0);
+    }
+    peps::als_truncate(*ctx->c, th, en, D_max, eR, n_als, reg, als_tol, &X, &Y, &bf, &inf, X0 ? &Xi : nullptr,
+                       X0 ? &Yi : nullptr);
+    *Dp = int64_t(bf.size());
+    std::memcpy(bond_flux, bf.data(), bf.size() * sizeof(int64_t));
+    if (info) {
+      info[0] = inf.cost0;
+      info[1] = inf.cost1;
+      info[2] = inf.accepted ? 1.0 : 0.0;
+      info[3] = double(inf.sweeps);
+    }
+    *out = make_list(ctx->c, {X, Y});
+  });
+}
+
 int32_t peps_su_bond(peps_state* st, int32_t orient, int32_t ix, int32_t iy, const double* gate,
                      int32_t D_max) {
   if (!st || D_max < 1) return PEPS_EARG;
@@ -648,6 +693,72 @@ int32_t peps_measure_all(peps_state* st, double out[7], double* nf_grid, double*
     std::copy(v, v + 7, out);
     if (nf_grid) std::copy(m.nf_grid.begin(), m.nf_grid.end(), nf_grid);
     if (E2_grid) std::copy(m.E2_grid.begin(), m.E2_grid.end(), E2_grid);
+  });
+}
+
+// ── learned ALS warm start: the small MLP ───────────────────────────────────
+
+int32_t peps_mlp_create(peps_ctx* ctx, int32_t n_in, int32_t width, int32_t n_out, uint64_t seed,
+                        peps_mlp** out) {
+  if (!ctx || !out) return PEPS_EARG;
+  *out = nullptr;
+  return guard(ctx->c.get(), [&] { *out = new peps_mlp(ctx->c, n_in, width, n_out, seed); });
+}
+
+int32_t peps_mlp_destroy(peps_mlp* m) {
+  delete m;
+  return PEPS_OK;
+}
+
+int32_t peps_mlp_param_count(const peps_mlp* m, int64_t* n) {
+  if (!m || !n) return PEPS_EARG;
+  *n = m->m.param_count();
+  return PEPS_OK;
+}
+
+int32_t peps_mlp_get_params(peps_mlp* m, double* out) {
+  if (!m || !out) return PEPS_EARG;
+  return guard(m->ref.get(), [&] { m->m.get_params(out); });
+}
+
+int32_t peps_mlp_set_params(peps_mlp* m, const double* in) {
+  if (!m || !in) return PEPS_EARG;
+  return guard(m->ref.get(), [&] { m->m.set_params(in); });
+}
+
+int32_t peps_mlp_forward(peps_mlp* m, int64_t batch, const double* X, double* Y) {
+  if (!m || !X || !Y || batch < 1) return PEPS_EARG;
+  return guard(m->ref.get(), [&] {
+    Tensor x = m->ref->upload({m->m.nin, batch}, false, X);
+    m->ref->download(m->m.forward(x), Y);
+  });
+}
+
+int32_t peps_mlp_gradients(peps_mlp* m, int64_t batch, const double* X, const double* T, double* loss,
+                           double* grad) {
+  if (!m || !X || !T || !loss || !grad || batch < 1) return PEPS_EARG;
+  return guard(m->ref.get(), [&] {
+    Tensor x = m->ref->upload({m->m.nin, batch}, false, X);
+    Tensor t = m->ref->upload({m->m.nout, batch}, false, T);
+    Tensor gW[3], gb[3];
+    *loss = m->m.gradients(x, t, gW, gb);
+    int64_t off = 0;
+    for (int l = 0; l < 3; ++l) {
+      m->ref->download(gW[l], grad + off);
+      off += gW[l].numel();
+      m->ref->download(gb[l], grad + off);
+      off += gb[l].numel();
+    }
+  });
+}
+
+int32_t peps_mlp_train_step(peps_mlp* m, int64_t batch, const double* X, const double* T, double lr,
+                            double* loss) {
+  if (!m || !X || !T || !loss || batch < 1 || !(lr > 0.0)) return PEPS_EARG;
+  return guard(m->ref.get(), [&] {
+    Tensor x = m->ref->upload({m->m.nin, batch}, false, X);
+    Tensor t = m->ref->upload({m->m.nout, batch}, false, T);
+    *loss = m->m.train_step(x, t, lr);
   });
 }
 

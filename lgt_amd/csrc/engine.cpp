@@ -885,7 +885,11 @@ void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,
                   const std::vector<int64_t>& eR, int n_als, double reg, double tol, Tensor* Xo,
                   Tensor* Yo, std::vector<int64_t>* bflux, ALSInfo* info, const Tensor* X_init,
                   const Tensor* Y_init, bool positive_cost) {
-  // als_truncate :94-154
+  // als_truncate :94-154.  The ALS runs in FP64 whatever the contraction
+  // tier: its normal equations, right-hand sides and costs are small next to
+  // the environment that feeds them, and tier-precision costs (FP32: ~1e-7
+  // relative noise) could never meet a sweep tolerance like 1e-10.
+  TierGuard fp64(c, TIER_FP64);
   const int64_t nA = th.dims[0], dpL = th.dims[1], dpR = th.dims[2], nB = th.dims[3];
   Tensor thm = permute(c, th, {0, 1, 3, 2}).reshaped({nA * dpL, nB * dpR});
   std::vector<int64_t> rq(nA * dpL);

@@ -1929,3 +1929,99 @@ std::vector<double> col_norm2(Ctx& c, const Tensor& M) {
 }
 
 }  // namespace peps
+
+namespace peps {
+
+// ════════════════════════════════════════════════════════════════════════════
+//  Small-MLP kernels (learned ALS warm start): batch in columns, column-major
+// ════════════════════════════════════════════════════════════════════════════
+
+// Z (n x B) += b (per row); ReLU when relu != 0.
+__global__ void bias_act_kernel(double* __restrict__ Z, const double* __restrict__ b, int64_t n, int64_t B,
+                                int relu) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n * B; i += int64_t(gridDim.x) * blockDim.x) {
+    double z = Z[i] + b[i % n];
+    Z[i] = relu ? (z > 0.0 ? z : 0.0) : z;
+  }
+}
+
+// dH (n x B) *= (H > 0)
+__global__ void relu_grad_kernel(double* __restrict__ dH, const double* __restrict__ H, int64_t nB) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nB; i += int64_t(gridDim.x) * blockDim.x)
+    if (!(H[i] > 0.0)) dH[i] = 0.0;
+}
+
+// db[i] = sum_j dZ[i + j n]: one wave per row, fixed-order lane sums
+__global__ void row_sum_kernel(const double* __restrict__ dZ, double* __restrict__ db, int64_t n, int64_t B) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = blockIdx.x * int64_t(blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  double s = 0.0;
+  for (int64_t j = lane; j < B; j += 64) s += dZ[row + j * n];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) db[row] = s;
+}
+
+// D = (Y - T) * scale
+__global__ void scaled_diff_kernel(const double* __restrict__ Y, const double* __restrict__ T, double* __restrict__ D,
+                                   int64_t n, double scale) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    D[i] = (Y[i] - T[i]) * scale;
+}
+
+// Adam (Kingma & Ba) with bias correction c1 = 1 - b1^t, c2 = 1 - b2^t
+__global__ void adam_kernel(double* __restrict__ p, const double* __restrict__ g, double* __restrict__ m,
+                            double* __restrict__ v, int64_t n, double lr, double b1, double b2, double eps, double c1,
+                            double c2) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double gi = g[i];
+    const double mi = b1 * m[i] + (1.0 - b1) * gi;
+    const double vi = b2 * v[i] + (1.0 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrt(vi / c2) + eps);
+  }
+}
+
+static unsigned grid_for(int64_t n) { return unsigned(std::max<int64_t>(1, std::min<int64_t>(cdiv(n, 256), 8192))); }
+
+void mlp_bias_act(Ctx& c, Tensor& Z, const Tensor& b, bool relu) {
+  const int64_t n = Z.dims[0], B = Z.dims[1];
+  PEPS_CHECK(!Z.cplx && b.numel() == n, PS_ESHAPE, "mlp_bias_act: shapes");
+  hipLaunchKernelGGL(bias_act_kernel, dim3(grid_for(n * B)), dim3(256), 0, c.stream, Z.data(), b.data(), n, B,
+                     relu ? 1 : 0);
+  PEPS_HIP(hipGetLastError());
+}
+
+void mlp_relu_grad(Ctx& c, Tensor& dH, const Tensor& H) {
+  PEPS_CHECK(dH.numel() == H.numel(), PS_ESHAPE, "mlp_relu_grad: shapes");
+  hipLaunchKernelGGL(relu_grad_kernel, dim3(grid_for(H.numel())), dim3(256), 0, c.stream, dH.data(), H.data(),
+                     H.numel());
+  PEPS_HIP(hipGetLastError());
+}
+
+void mlp_row_sum(Ctx& c, const Tensor& dZ, Tensor& db) {
+  const int64_t n = dZ.dims[0], B = dZ.dims[1];
+  PEPS_CHECK(db.numel() == n, PS_ESHAPE, "mlp_row_sum: shapes");
+  hipLaunchKernelGGL(row_sum_kernel, dim3(unsigned(cdiv(n, 4))), dim3(256), 0, c.stream, dZ.data(), db.data(), n, B);
+  PEPS_HIP(hipGetLastError());
+}
+
+void mlp_scaled_diff(Ctx& c, const Tensor& Y, const Tensor& T, Tensor& D, double scale) {
+  PEPS_CHECK(Y.numel() == T.numel() && D.numel() == Y.numel(), PS_ESHAPE, "mlp_scaled_diff: shapes");
+  hipLaunchKernelGGL(scaled_diff_kernel, dim3(grid_for(Y.numel())), dim3(256), 0, c.stream, Y.data(), T.data(),
+                     D.data(), Y.numel(), scale);
+  PEPS_HIP(hipGetLastError());
+}
+
+void mlp_adam(Ctx& c, Tensor& p, const Tensor& g, Tensor& m, Tensor& v, double lr, double b1, double b2, double eps,
+              int64_t t) {
+  PEPS_CHECK(p.numel() == g.numel() && m.numel() == p.numel() && v.numel() == p.numel(), PS_ESHAPE,
+             "mlp_adam: shapes");
+  const double c1 = 1.0 - std::pow(b1, double(t)), c2 = 1.0 - std::pow(b2, double(t));
+  hipLaunchKernelGGL(adam_kernel, dim3(grid_for(p.numel())), dim3(256), 0, c.stream, p.data(), g.data(), m.data(),
+                     v.data(), p.numel(), lr, b1, b2, eps, c1, c2);
+  PEPS_HIP(hipGetLastError());
+}
+
+}  // namespace peps

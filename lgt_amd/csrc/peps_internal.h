@@ -261,6 +261,14 @@ void qr_form_q(Ctx& c, const Tensor& W, const Tensor& tau, Tensor& Q);
 void scan_values(Ctx& c, const Tensor& t, bool* finite, bool* nonzero);
 // The first k columns of the m x m identity.
 Tensor eye(Ctx& c, int64_t m, int64_t k, bool cplx);
+// Small-MLP pieces (real, batch in columns): Z += b per row (+ ReLU);
+// dH *= (H > 0); db = row sums of dZ; D = (Y - T) scale; one Adam step.
+void mlp_bias_act(Ctx& c, Tensor& Z, const Tensor& b, bool relu);
+void mlp_relu_grad(Ctx& c, Tensor& dH, const Tensor& H);
+void mlp_row_sum(Ctx& c, const Tensor& dZ, Tensor& db);
+void mlp_scaled_diff(Ctx& c, const Tensor& Y, const Tensor& T, Tensor& D, double scale);
+void mlp_adam(Ctx& c, Tensor& p, const Tensor& g, Tensor& m, Tensor& v, double lr, double b1, double b2, double eps,
+              int64_t t);
 // Deterministic uniform(-1,1) entries (complex: both parts).
 void fill_random(Ctx& c, Tensor& T, uint64_t seed);
 

@@ -80,6 +80,8 @@ _SIGS = {
     "peps_bond_env": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
     "peps_charge_svd": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _i64p, _i64p, C.c_int32, C.c_double, _i64p, _i64p, C.POINTER(_vp)]),
     "peps_als_truncate": (C.c_int32, [_vp, C.c_int32, _i64p, _dp, _dp, C.c_int32, _i64p, C.c_int32, C.c_double, _i64p, _i64p, _dp, C.POINTER(_vp)]),
+    "peps_als_truncate_init": (C.c_int32, [_vp, C.c_int32, _i64p, _dp, _dp, C.c_int32, _i64p, C.c_int32, C.c_double,
+                                           C.c_double, _dp, _dp, _i64p, _i64p, _dp, C.POINTER(_vp)]),
     "peps_su_bond": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32]),
     "peps_fu_bond_v4": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, C.c_int32]),
     "peps_trotter_step": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params)]),
@@ -98,6 +100,14 @@ _SIGS = {
     "peps_plaquette_expect": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp]),
     "peps_bond_entropy": (C.c_int32, [_dp, C.c_int64, _dp]),
     "peps_measure_all": (C.c_int32, [_vp, _dp, _dp, _dp]),
+    "peps_mlp_create": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_vp)]),
+    "peps_mlp_destroy": (C.c_int32, [_vp]),
+    "peps_mlp_param_count": (C.c_int32, [_vp, _i64p]),
+    "peps_mlp_get_params": (C.c_int32, [_vp, _dp]),
+    "peps_mlp_set_params": (C.c_int32, [_vp, _dp]),
+    "peps_mlp_forward": (C.c_int32, [_vp, C.c_int64, _dp, _dp]),
+    "peps_mlp_gradients": (C.c_int32, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
+    "peps_mlp_train_step": (C.c_int32, [_vp, C.c_int64, _dp, _dp, C.c_double, _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
     "peps_permute": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p, _i32p, _dp, _dp]),
     "peps_gemm_acc": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp,
@@ -496,6 +506,64 @@ def measure_all_finite(gp: GaugePEPS, t=0.0) -> dict:
     return dict(t=t, **dict(zip(keys, out.tolist())), nf_grid=nf, E2_grid=e2)
 
 
+# ── learned ALS warm start: the GPU MLP ─────────────────────────────────────
+
+class Mlp:
+    """in -> width -> width -> out perceptron (ReLU hidden layers) trained and
+    evaluated on the GPU; batch along columns (numpy arrays (n, batch))."""
+
+    def __init__(self, n_in, width, n_out, seed=0, ctx=None):
+        self.ctx = ctx or default_context()
+        self.n_in, self.width, self.n_out = n_in, width, n_out
+        h = _vp()
+        self.ctx.check(lib().peps_mlp_create(self.ctx.h, n_in, width, n_out, seed, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().peps_mlp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def param_count(self):
+        n = C.c_int64()
+        self.ctx.check(lib().peps_mlp_param_count(self.h, C.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.zeros(self.param_count())
+        self.ctx.check(lib().peps_mlp_get_params(self.h, _dptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        assert p.size == self.param_count()
+        self.ctx.check(lib().peps_mlp_set_params(self.h, _dptr(p)))
+
+    def forward(self, X):
+        X = np.asfortranarray(X, dtype=np.float64)
+        Y = np.zeros((self.n_out, X.shape[1]), order="F")
+        self.ctx.check(lib().peps_mlp_forward(self.h, X.shape[1], _dptr(X), _dptr(Y)))
+        return Y
+
+    def gradients(self, X, T):
+        X = np.asfortranarray(X, dtype=np.float64)
+        T = np.asfortranarray(T, dtype=np.float64)
+        g = np.zeros(self.param_count())
+        loss = C.c_double()
+        self.ctx.check(lib().peps_mlp_gradients(self.h, X.shape[1], _dptr(X), _dptr(T), C.byref(loss), _dptr(g)))
+        return loss.value, g
+
+    def train_step(self, X, T, lr=1e-3):
+        X = np.asfortranarray(X, dtype=np.float64)
+        T = np.asfortranarray(T, dtype=np.float64)
+        loss = C.c_double()
+        self.ctx.check(lib().peps_mlp_train_step(self.h, X.shape[1], _dptr(X), _dptr(T), lr, C.byref(loss)))
+        return loss.value
+
+
 # ── finite_peps_full_update.jl / finite_peps_boundary_mps.jl ────────────────
 
 def dl_traced(T, ctx=None):
@@ -605,6 +673,30 @@ def als_truncate(theta, env, D_max, eRofpL, n_als=30, reg=1e-10, ctx=None, retur
     if return_info:
         return out + (dict(cost0=info[0], cost1=info[1], accepted=bool(info[2])),)
     return out
+
+
+def als_truncate_init(theta, env, D_max, eRofpL, n_als=30, reg=1e-10, als_tol=0.0, init=None, ctx=None):
+    """als_truncate with an optional initial bond (X0, Y0) shaped like the SVD
+    initialisation and an optional tolerance -> (X, Y, bflux, info)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(theta) or np.iscomplexobj(env) or (init is not None and any(np.iscomplexobj(a) for a in init))
+    th = _as_array(theta, cp)
+    en = _as_array(env, cp)
+    d = np.array(th.shape, dtype=np.int64)
+    eR = np.ascontiguousarray(eRofpL, dtype=np.int64)
+    Dp = C.c_int64()
+    bf = np.zeros(D_max, np.int64)
+    info = np.zeros(4)
+    h = _vp()
+    if init is None:
+        x0 = y0 = C.cast(None, _dp)
+    else:
+        X0, Y0 = (_as_array(a, cp) for a in init)
+        x0, y0 = _dptr(X0), _dptr(Y0)
+    ctx.check(lib().peps_als_truncate_init(ctx.h, int(cp), _iptr(d), _dptr(th), _dptr(en), D_max, _iptr(eR), n_als,
+                                           reg, als_tol, x0, y0, C.byref(Dp), _iptr(bf), _dptr(info), C.byref(h)))
+    X, Y = _take_list(ctx, h)
+    return X, Y, bf[:Dp.value], dict(cost0=info[0], cost1=info[1], accepted=bool(info[2]), sweeps=int(info[3]))
 
 
 def full_update_bond_h_v4(gp: GaugePEPS, ix, iy, gate, D_max, chi=64, n_als=30):

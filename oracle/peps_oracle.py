@@ -1545,6 +1545,46 @@ def measure_all_finite(gp, t=0.0):
 
 
 # ════════════════════════════════════════════════════════════════════════════
+#  The small MLP of the learned ALS warm start (no reference counterpart)
+# ════════════════════════════════════════════════════════════════════════════
+
+def mlp_unpack(p, n_in, width, n_out):
+    """Flat parameters W1 (width x n_in), b1, W2 (width x width), b2, W3
+    (n_out x width), b3, each column-major."""
+    shapes = [(width, n_in), (width,), (width, width), (width,), (n_out, width), (n_out,)]
+    out, off = [], 0
+    for shp in shapes:
+        n = int(np.prod(shp))
+        out.append(np.reshape(p[off:off + n], shp, order="F"))
+        off += n
+    assert off == p.size
+    return out
+
+
+def mlp_forward(p, X, n_in, width, n_out):
+    """Y = W3 relu(W2 relu(W1 X + b1) + b2) + b3, batch along columns."""
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p, n_in, width, n_out)
+    H1 = np.maximum(W1 @ X + b1[:, None], 0.0)
+    H2 = np.maximum(W2 @ H1 + b2[:, None], 0.0)
+    return W3 @ H2 + b3[:, None]
+
+
+def mlp_loss_grads(p, X, T, n_in, width, n_out):
+    """loss 1/(2B) ||Y - T||^2 and its gradient (flattened like p), by backprop."""
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p, n_in, width, n_out)
+    B = X.shape[1]
+    H1 = np.maximum(W1 @ X + b1[:, None], 0.0)
+    H2 = np.maximum(W2 @ H1 + b2[:, None], 0.0)
+    Y = W3 @ H2 + b3[:, None]
+    D3 = (Y - T) / B
+    loss = 0.5 * np.sum((Y - T) ** 2) / B
+    D2 = (W3.T @ D3) * (H2 > 0)
+    D1 = (W2.T @ D2) * (H1 > 0)
+    grads = [D1 @ X.T, D1.sum(1), D2 @ H1.T, D2.sum(1), D3 @ H2.T, D3.sum(1)]
+    return loss, np.concatenate([np.reshape(g, -1, order="F") for g in grads])
+
+
+# ════════════════════════════════════════════════════════════════════════════
 #  Brute-force references (tiny lattices only): dense |psi>, <psi|psi>, <H>
 # ════════════════════════════════════════════════════════════════════════════
 
