@@ -279,10 +279,14 @@ __global__ void splitk_reduce(double* __restrict__ C, const double* __restrict__
 // the blocks the hardware sends to one XCD (round robin over 8) are
 // consecutive in that order.  The epilogue transposes the accumulators
 // through LDS so that C is read and written in whole column segments.
-template <bool AK, bool BKF>
+// TIN is the operand storage type: double (cast to FP32 on the way into LDS)
+// or float (operands cast once by cast_f32_kernel, half the L2 -> CU bytes
+// per k-tile); both give the same bits, since the cast is the same
+// round-to-nearest conversion and the accumulation order does not change.
+template <typename TIN, bool AK, bool BKF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_f32_tiled(int64_t M, int64_t N, int64_t K,
-                                                      const double* __restrict__ A, int64_t lda,
-                                                      const double* __restrict__ B, int64_t ldb,
+                                                      const TIN* __restrict__ A, int64_t lda,
+                                                      const TIN* __restrict__ B, int64_t ldb,
                                                       double* __restrict__ C, int64_t ldc, double beta,
                                                       int64_t kchunk, int64_t wstride, int tilesM,
                                                       int tilesN) {
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // Each thread moves 4 groups of 4 consecutive k of one row (m or n) per
   // operand and k-tile: four coalesced 8-byte loads per group (lanes run
   // along the contiguous global dim) and one 16-byte LDS store.
-  double ra[4][4], rb[4][4];
+  TIN ra[4][4], rb[4][4];
   // group e of this thread in a 128 x 32 operand tile: (row, first k)
   auto arow = [&](int e) { return AK ? (tid >> 3) + 32 * e : (tid & 127); };
   auto acol = [&](int e) { return AK ? 4 * (tid & 7) : 4 * ((tid >> 7) + 2 * e); };
@@ -336,8 +340,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int64_t gk = k0 + acol(e) + j, gkb = k0 + bcol(e) + j;
-        ra[e][j] = (gm < M && gk < kend) ? A[AK ? gk + gm * lda : gm + gk * lda] : 0.0;
-        rb[e][j] = (gn < N && gkb < kend) ? B[BKF ? gkb + gn * ldb : gn + gkb * ldb] : 0.0;
+        ra[e][j] = (gm < M && gk < kend) ? A[AK ? gk + gm * lda : gm + gk * lda] : TIN(0);
+        rb[e][j] = (gn < N && gkb < kend) ? B[BKF ? gkb + gn * ldb : gn + gkb * ldb] : TIN(0);
       }
     }
   };
@@ -406,20 +410,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
 }
 
+template <typename TIN>
 static void launch_gemm_f32_tiled(bool ak, bool bk, hipStream_t st, int64_t M, int64_t N, int64_t K,
-                                  const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                                  const TIN* A, int64_t lda, const TIN* B, int64_t ldb, double* C,
                                   int64_t ldc, double beta, int splits, int64_t kchunk, int64_t wstride) {
   const int tm = int(cdiv(M, 128)), tn = int(cdiv(N, 128));
   dim3 grid(unsigned(tm) * unsigned(tn), 1, splits);
   if (ak && bk)
-    hipLaunchKernelGGL((gemm_f32_tiled<true, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+    hipLaunchKernelGGL((gemm_f32_tiled<TIN, true, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
   else if (ak)
-    hipLaunchKernelGGL((gemm_f32_tiled<true, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+    hipLaunchKernelGGL((gemm_f32_tiled<TIN, true, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
   else if (bk)
-    hipLaunchKernelGGL((gemm_f32_tiled<false, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+    hipLaunchKernelGGL((gemm_f32_tiled<TIN, false, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
   else
-    hipLaunchKernelGGL((gemm_f32_tiled<false, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+    hipLaunchKernelGGL((gemm_f32_tiled<TIN, false, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
 }
+
+// FP64 -> FP32 (round to nearest) copy of a packed operand.
+__global__ void cast_f32_kernel(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] = float(x[i]);
+}
+
+// Operands of a tiled GEMM with both extents >= this are cast to FP32 once:
+// each operand element is then read by >= 4 tiles, so the cast (12 bytes of
+// HBM traffic per element) costs less than the FP64 bytes it saves.
+constexpr int64_t kCastMin = 512;
 
 template <bool CPLX, int PREC, bool AK, bool BKF, int WM, int WN>
 static void launch_gemm_t(hipStream_t st, int64_t M, int64_t N, int64_t K, const double* A,
@@ -501,7 +517,29 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
   if (c.gemm_shapes)
     cls += " " + std::to_string(M) + "x" + std::to_string(N) + "x" + std::to_string(K) + (cplx ? "c" : "") +
            (a_kfast ? " ak" : " am") + (b_kfast ? " bk" : " bn") + " s" + std::to_string(splits);
-  c.count(cls, flops, bytes);
+  const bool cast = tiled && M >= kCastMin && N >= kCastMin;
+  // algorithmic bytes of the GEMM kernel itself: FP32 operands when cast
+  c.count(cls, flops, cast ? 4.0 * (double(M) * K + double(K) * N) + 8.0 * double(M) * N : bytes);
+  std::shared_ptr<Buffer> a32, b32;
+  if (cast) {
+    // A and B are packed (lda = M or K, ldb = K or N): cast M*K and K*N
+    // contiguous elements; the cast kernel is timed as its own class
+    a32 = std::make_shared<Buffer>();
+    b32 = std::make_shared<Buffer>();
+    a32->pool = b32->pool = c.pool.get();
+    a32->bytes = size_t(M * K) * sizeof(float);
+    b32->bytes = size_t(K * N) * sizeof(float);
+    a32->ptr = c.pool->get(a32->bytes);
+    b32->ptr = c.pool->get(b32->bytes);
+    hipEvent_t evc = nullptr;
+    c.tbegin("cast_f32", &evc);
+    c.count("cast_f32", 0.0, 12.0 * double(M * K + K * N));
+    const unsigned gc = unsigned(std::min<int64_t>(cdiv(std::max(M, N) * K, 256), 8192));
+    hipLaunchKernelGGL(cast_f32_kernel, dim3(gc), dim3(256), 0, c.stream, A, (float*)a32->ptr, M * K);
+    hipLaunchKernelGGL(cast_f32_kernel, dim3(gc), dim3(256), 0, c.stream, B, (float*)b32->ptr, K * N);
+    PEPS_HIP(hipGetLastError());
+    c.tend("cast_f32", evc);
+  }
   hipEvent_t ev = nullptr;
   c.tbegin(cls, &ev);
   std::shared_ptr<Buffer> ws;
@@ -526,9 +564,12 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
   using T0 = std::integral_constant<int, 0>;
   using T1 = std::integral_constant<int, 1>;
   using T2 = std::integral_constant<int, 2>;
-  if (tiled) {
-    launch_gemm_f32_tiled(a_kfast, b_kfast, c.stream, M, N, K, A, lda, B, ldb, out, M, kbeta, splits, kchunk,
-                          wstride);
+  if (a32) {
+    launch_gemm_f32_tiled<float>(a_kfast, b_kfast, c.stream, M, N, K, (const float*)a32->ptr, lda,
+                                 (const float*)b32->ptr, ldb, out, M, kbeta, splits, kchunk, wstride);
+  } else if (tiled) {
+    launch_gemm_f32_tiled<double>(a_kfast, b_kfast, c.stream, M, N, K, A, lda, B, ldb, out, M, kbeta, splits,
+                                  kchunk, wstride);
   } else if (cplx) {
     if (c.tier == TIER_FP64) go(std::true_type{}, T0{});
     else if (c.tier == TIER_FP32) go(std::true_type{}, T1{});

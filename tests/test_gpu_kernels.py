@@ -8,7 +8,10 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(7, 5, 3), (64, 64, 16), (130, 129, 40), (200, 144, 37), (48, 40, 5000), (16, 300, 2048),
           # large real outputs (the tiled FP32 kernel): ragged edges, K not a
           # multiple of the k-tile, split-K, more tile rows than one group
-          (300, 260, 70), (257, 1000, 33), (512, 384, 4096), (1100, 300, 5)]
+          (300, 260, 70), (257, 1000, 33), (512, 384, 4096), (1100, 300, 5),
+          # both extents >= 512: operands cast to FP32 before the tiled kernel
+          # (ragged, K not a multiple of the k-tile; split-K)
+          (640, 520, 77), (1030, 700, 1500)]
 TOL = {"fp64": 1e-13, "fp32": 3e-6, "bf16": 2e-2}
 
 
@@ -50,7 +53,7 @@ def test_gemm_accumulate(ctx, tier):
     ctx.set_tier(tier)
     try:
         for (M, N, K, beta) in [(300, 260, 70, 1.0), (512, 384, 4096, 0.5), (40, 30, 20, -2.0),
-                                (1100, 300, 64, 1.0)]:
+                                (1100, 300, 64, 1.0), (600, 530, 300, -0.5), (700, 640, 2048, 1.0)]:
             for ak in (False, True):
                 for bk in (False, True):
                     A = rng.standard_normal((M, K))
