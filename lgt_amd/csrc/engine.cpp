@@ -789,18 +789,24 @@ std::vector<int64_t> iota_v(int64_t n) {
 
 CSVD charge_svd(Ctx& c, const Tensor& M, const std::vector<int64_t>& rq,
                 const std::vector<int64_t>& cq, int Dmax, double reg) {
+  // the sector blocks are independent: their SVDs run concurrently
   std::vector<SectorSVD> secs;
+  std::vector<Tensor> blocks;
   for (int64_t s : sectors_of(rq, cq)) {
     SectorSVD sc;
     sc.s = s;
     sc.rows = where_eq(rq, s);
     sc.cols = where_eq(cq, s);
     if (sc.rows.empty() || sc.cols.empty()) continue;
-    SVD f = svd_thin(c, gather2d(c, M, sc.rows, sc.cols));
-    sc.U = f.U;
-    sc.Vt = f.Vt;
-    sc.S = f.S;
+    blocks.push_back(gather2d(c, M, sc.rows, sc.cols));
     secs.push_back(std::move(sc));
+  }
+  std::vector<SVD> fs = svd_thin_batch(c, blocks);
+  blocks.clear();
+  for (size_t i = 0; i < secs.size(); ++i) {
+    secs[i].U = fs[i].U;
+    secs[i].Vt = fs[i].Vt;
+    secs[i].S = fs[i].S;
   }
   return assemble(c, M.dims[0], M.dims[1], secs, Dmax, reg, M.cplx);
 }
@@ -1118,17 +1124,22 @@ static CSVD su_charge_svd(Ctx& c, const Tensor& core, int64_t dpL, int64_t a,
   std::vector<SectorSVD> secs;
   std::vector<int64_t> rq_core(dpL * a);
   for (int64_t r = 0; r < dpL * a; ++r) rq_core[r] = eR_of_p[r % dpL];
+  std::vector<Tensor> blocks;
   for (int64_t s : sectors_of(rq_full, cq_full)) {
     SectorSVD sc;
     sc.s = s;
     sc.rows = where_eq(rq_core, s);
     sc.cols = iota_v(ncc);
     if (sc.rows.empty() || where_eq(cq_full, s).empty()) continue;
-    SVD f = svd_thin(c, gather2d(c, core, sc.rows, sc.cols));
-    sc.U = f.U;
-    sc.Vt = f.Vt;
-    sc.S = f.S;
+    blocks.push_back(gather2d(c, core, sc.rows, sc.cols));
     secs.push_back(std::move(sc));
+  }
+  std::vector<SVD> fs = svd_thin_batch(c, blocks);
+  blocks.clear();
+  for (size_t i = 0; i < secs.size(); ++i) {
+    secs[i].U = fs[i].U;
+    secs[i].Vt = fs[i].Vt;
+    secs[i].S = fs[i].S;
   }
   return assemble(c, dpL * a, ncc, secs, Dmax, GI_REG, core.cplx);
 }

@@ -1573,11 +1573,10 @@ __global__ __launch_bounds__(1024) void jacobi_svd_kernel(typename JT<CP>::T* __
 // held in registers, R entries per lane: one load and one store per
 // rotation instead of three passes over global memory.  Needs m, n <= L*R.
 template <bool CP, int L, int R>
-__global__ __launch_bounds__(1024) void jacobi_svd_reg_kernel(typename JT<CP>::T* __restrict__ A,
-                                                              typename JT<CP>::T* __restrict__ V,
-                                                              double* __restrict__ sig, int m, int n,
-                                                              double tol, int max_sweeps,
-                                                              int* __restrict__ sweeps_out) {
+__device__ __forceinline__ void jacobi_reg_body(typename JT<CP>::T* __restrict__ A,
+                                                typename JT<CP>::T* __restrict__ V,
+                                                double* __restrict__ sig, int m, int n, double tol,
+                                                int max_sweeps, int* __restrict__ sweeps_out) {
   typedef typename JT<CP>::T T;
   __shared__ int rot;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
@@ -1671,6 +1670,138 @@ __global__ __launch_bounds__(1024) void jacobi_svd_reg_kernel(typename JT<CP>::T
   if (tid == 0 && sweeps_out) *sweeps_out = sweep;
 }
 
+template <bool CP, int L, int R>
+__global__ __launch_bounds__(1024) void jacobi_svd_reg_kernel(typename JT<CP>::T* __restrict__ A,
+                                                              typename JT<CP>::T* __restrict__ V,
+                                                              double* __restrict__ sig, int m, int n,
+                                                              double tol, int max_sweeps,
+                                                              int* __restrict__ sweeps_out) {
+  jacobi_reg_body<CP, L, R>(A, V, sig, m, n, tol, max_sweeps, sweeps_out);
+}
+
+// One matrix of a batched Jacobi launch (workgroup blockIdx.x takes item
+// blockIdx.x); the pointers are FP64 (real) or interleaved complex storage.
+struct JacobiItem {
+  double* A;
+  double* V;
+  double* sig;
+  int* sweeps;
+  int m, n;
+  double tol;
+};
+
+template <bool CP, int L, int R>
+__global__ __launch_bounds__(1024) void jacobi_svd_reg_batched_kernel(const JacobiItem* __restrict__ items,
+                                                                      int max_sweeps) {
+  typedef typename JT<CP>::T T;
+  const JacobiItem it = items[blockIdx.x];
+  jacobi_reg_body<CP, L, R>(reinterpret_cast<T*>(it.A), reinterpret_cast<T*>(it.V), it.sig, it.m, it.n, it.tol,
+                            max_sweeps, it.sweeps);
+}
+
+// Register-resident variant for a real or complex matrix with m, n >= 2:
+// 0: real m, n <= 160 (16 lanes x 10 rows per pair, up to 64 pairs at once),
+// 1: real <= 256 (32 x 8), 2: real <= 512 (64 x 8), 3: complex <= 256
+// (64 x 4); -1: the global-memory kernel.
+static int jacobi_variant(bool cplx, int64_t m, int64_t n) {
+  const int64_t mx = std::max(m, n);
+  if (n < 2) return -1;
+  if (!cplx && mx <= 160) return 0;
+  if (!cplx && mx <= 256) return 1;
+  if (!cplx && mx <= 512) return 2;
+  if (cplx && mx <= 256) return 3;
+  return -1;
+}
+
+// lanes per column pair of a variant
+static int jacobi_lanes(int v) { return v == 0 ? 16 : v == 1 ? 32 : 64; }
+
+// threads per workgroup for a matrix with n columns under variant v: enough
+// pair slots for the ceil(n/2) pairs of a round, at most 1024 threads
+static int jacobi_threads(int v, int64_t n) {
+  const int half = int((n + 1) / 2);
+  const int per_wave = 64 / jacobi_lanes(v);
+  return 64 * std::max(1, std::min(16, int(cdiv(half, per_wave))));
+}
+
+static void launch_jacobi_batched(int v, hipStream_t st, unsigned nb, unsigned threads, const JacobiItem* items,
+                                  int max_sweeps) {
+  switch (v) {
+    case 0:
+      hipLaunchKernelGGL((jacobi_svd_reg_batched_kernel<false, 16, 10>), dim3(nb), dim3(threads), 0, st, items,
+                         max_sweeps);
+      break;
+    case 1:
+      hipLaunchKernelGGL((jacobi_svd_reg_batched_kernel<false, 32, 8>), dim3(nb), dim3(threads), 0, st, items,
+                         max_sweeps);
+      break;
+    case 2:
+      hipLaunchKernelGGL((jacobi_svd_reg_batched_kernel<false, 64, 8>), dim3(nb), dim3(threads), 0, st, items,
+                         max_sweeps);
+      break;
+    default:
+      hipLaunchKernelGGL((jacobi_svd_reg_batched_kernel<true, 64, 4>), dim3(nb), dim3(threads), 0, st, items,
+                         max_sweeps);
+      break;
+  }
+}
+
+void jacobi_svd_batch(Ctx& c, std::vector<JacobiJob>& jobs, int max_sweeps) {
+  // matrices the register kernels take run together, one launch per variant
+  // (one workgroup each); the rest run one by one on the global-memory kernel
+  std::vector<int> var(jobs.size());
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    JacobiJob& j = jobs[i];
+    const int64_t m = j.W->dims[0], n = j.W->dims[1];
+    PEPS_CHECK(m >= n && n >= 1 && n <= 4096 && m < (1ll << 31), PS_EUNSUPPORTED,
+               "jacobi_svd: need m >= n, 1 <= n <= 4096");
+    PEPS_CHECK(j.V->numel() == n * n && j.V->cplx == j.W->cplx && j.sig->numel() == n && !j.sig->cplx, PS_ESHAPE,
+               "jacobi_svd: buffer sizes");
+    var[i] = jacobi_variant(j.W->cplx, m, n);
+    if (var[i] < 0) j.sweeps = jacobi_svd(c, *j.W, *j.V, *j.sig, j.tol, max_sweeps);
+  }
+  auto sw = std::make_shared<Buffer>();
+  sw->pool = c.pool.get();
+  sw->bytes = std::max<size_t>(jobs.size(), 1) * sizeof(int);
+  sw->ptr = c.pool->get(sw->bytes);
+  int* swd = static_cast<int*>(sw->ptr);
+  bool any = false;
+  std::vector<std::shared_ptr<Buffer>> keep;
+  for (int v = 0; v < 4; ++v) {
+    std::vector<JacobiItem> items;
+    int threads = 64, nmax = 0;
+    double flops = 0.0;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+      if (var[i] != v) continue;
+      JacobiJob& j = jobs[i];
+      const int64_t m = j.W->dims[0], n = j.W->dims[1];
+      items.push_back({j.W->data(), j.V->data(), j.sig->data(), swd + i, int(m), int(n), j.tol});
+      threads = std::max(threads, jacobi_threads(v, n));
+      nmax = std::max(nmax, int(n));
+      flops += (j.W->cplx ? 4.0 : 1.0) * 10.0 * 6.0 * double(m) * n * n;
+    }
+    if (items.empty()) continue;
+    std::shared_ptr<Buffer> kb;
+    const JacobiItem* dev =
+        static_cast<const JacobiItem*>(c.scratch_upload(items.data(), items.size() * sizeof(JacobiItem), &kb));
+    keep.push_back(kb);
+    const std::string cls = Ctx::bucket("svd", nmax);
+    c.count(cls, flops, 0);
+    hipEvent_t ev = nullptr;
+    c.tbegin(cls, &ev);
+    launch_jacobi_batched(v, c.stream, unsigned(items.size()), unsigned(threads), dev, max_sweeps);
+    PEPS_HIP(hipGetLastError());
+    c.tend(cls, ev);
+    any = true;
+  }
+  if (!any) return;
+  std::vector<int> h(jobs.size(), 0);
+  PEPS_HIP(hipMemcpyAsync(h.data(), swd, jobs.size() * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  PEPS_HIP(hipStreamSynchronize(c.stream));
+  for (size_t i = 0; i < jobs.size(); ++i)
+    if (var[i] >= 0) jobs[i].sweeps = h[i];
+}
+
 int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sweeps) {
   PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "jacobi_svd: matrix expected");
   const int64_t m = A.dims[0], n = A.dims[1];
@@ -1689,21 +1820,22 @@ int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sw
   c.tbegin(cls, &ev);
   c.count(cls, (A.cplx ? 4.0 : 1.0) * 10.0 * 6.0 * double(m) * n * n, 0);
   // register-resident variants (4 columns x R entries per lane in VGPRs,
-  // within the 128-VGPR budget of 1024-thread blocks): real m, n <= 256 with
-  // 32 lanes per pair (two pairs per wave), real m, n <= 512 and complex
-  // m, n <= 256 with 64 lanes per pair
-  const int64_t mx = std::max(m, n);
-  const int rw2 = std::max(1, std::min(16, int(cdiv(half, 2)))), rw1 = std::max(1, std::min(16, half));
-  if (n >= 2 && !A.cplx && mx <= 256)
-    hipLaunchKernelGGL((jacobi_svd_reg_kernel<false, 32, 8>), dim3(1), dim3(64 * rw2), 0, c.stream, A.data(),
-                       V.data(), sig.data(), int(m), int(n), tol, max_sweeps, (int*)buf->ptr);
-  else if (n >= 2 && !A.cplx && mx <= 512)
-    hipLaunchKernelGGL((jacobi_svd_reg_kernel<false, 64, 8>), dim3(1), dim3(64 * rw1), 0, c.stream, A.data(),
-                       V.data(), sig.data(), int(m), int(n), tol, max_sweeps, (int*)buf->ptr);
-  else if (n >= 2 && A.cplx && mx <= 256)
-    hipLaunchKernelGGL((jacobi_svd_reg_kernel<true, 64, 4>), dim3(1), dim3(64 * rw1), 0, c.stream,
-                       (double2*)A.data(), (double2*)V.data(), sig.data(), int(m), int(n), tol, max_sweeps,
-                       (int*)buf->ptr);
+  // within the 128-VGPR budget of 1024-thread blocks), see jacobi_variant
+  const int v = jacobi_variant(A.cplx, m, n);
+  const unsigned th = v >= 0 ? unsigned(jacobi_threads(v, n)) : 64u;
+  int* swp = (int*)buf->ptr;
+  if (v == 0)
+    hipLaunchKernelGGL((jacobi_svd_reg_kernel<false, 16, 10>), dim3(1), dim3(th), 0, c.stream, A.data(), V.data(),
+                       sig.data(), int(m), int(n), tol, max_sweeps, swp);
+  else if (v == 1)
+    hipLaunchKernelGGL((jacobi_svd_reg_kernel<false, 32, 8>), dim3(1), dim3(th), 0, c.stream, A.data(), V.data(),
+                       sig.data(), int(m), int(n), tol, max_sweeps, swp);
+  else if (v == 2)
+    hipLaunchKernelGGL((jacobi_svd_reg_kernel<false, 64, 8>), dim3(1), dim3(th), 0, c.stream, A.data(), V.data(),
+                       sig.data(), int(m), int(n), tol, max_sweeps, swp);
+  else if (v == 3)
+    hipLaunchKernelGGL((jacobi_svd_reg_kernel<true, 64, 4>), dim3(1), dim3(th), 0, c.stream, (double2*)A.data(),
+                       (double2*)V.data(), sig.data(), int(m), int(n), tol, max_sweeps, swp);
   else if (A.cplx)
     hipLaunchKernelGGL(jacobi_svd_kernel<true>, dim3(1), dim3(64 * waves), 0, c.stream,
                        (double2*)A.data(), (double2*)V.data(), sig.data(), int(m), int(n), tol,

@@ -145,91 +145,75 @@ static Tensor conj_transpose(Ctx& c, const Tensor& A) {
   return T;
 }
 
-static SVD svd_dense(Ctx& c, const Tensor& A);
+namespace {
+// The matrix the Jacobi kernel diagonalises for one dense SVD: B = A or A^H
+// (tall), reduced to R by a Householder QR where that shrinks the work.
+struct DensePlan {
+  bool tr = false, useq = false;
+  int64_t mb = 0, nb = 0;
+  Tensor W, Q, V, sg;
+  double tol = 0.0;
+};
 
-SVD svd_thin(Ctx& c, const Tensor& A) {
-  // Thin SVD A = U diag(S) Vt with S descending.  Wide inputs are handled
-  // through their conjugate transpose; inputs much taller than wide are
-  // first reduced by a Householder QR (rocSOLVER geqrf/orgqr); the one-sided
-  // Jacobi kernel (kernels.hip) diagonalises what remains.  Left vectors of
-  // zero singular values are zero, except for A = 0 (identity vectors).
-  PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "svd: matrix expected");
-  const int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);
-  check_int_range(m, n);
-  SVD r;
-  if (k == 0) {
-    r.U = c.alloc({m, 0}, A.cplx);
-    r.Vt = c.alloc({0, n}, A.cplx);
-    return r;
-  }
-  if (!check_input(c, A, "svd")) {  // A = 0: S = 0 with identity vectors
-    r.U = eye(c, m, k, A.cplx);
-    r.S.assign(k, 0.0);
-    r.Vt = conj_transpose(c, eye(c, n, k, A.cplx));
-    return r;
-  }
-  // rows and columns that are exactly zero (Gauss-masked blocks have many)
-  // do not change the non-zero singular triplets: factorise the compacted
-  // block and pad with zero triplets
+// One thin SVD: either answered directly (empty or zero input) or a dense
+// plan on the compacted block (rows / columns that are exactly zero dropped).
+struct ThinPlan {
+  bool direct = false;
+  SVD out;
+  int64_t m = 0, n = 0, k = 0;
+  bool compact = false;
   std::vector<int64_t> nzr, nzc;
-  nonzero_lines(c, A, &nzr, &nzc);
-  if (int64_t(nzr.size()) < m || int64_t(nzc.size()) < n) {
-    SVD f = svd_dense(c, gather2d(c, A, nzr, nzc));
-    const int64_t kc = int64_t(f.S.size());
-    std::vector<int64_t> jk(kc);
-    for (int64_t i = 0; i < kc; ++i) jk[i] = i;
-    r.U = c.zeros({m, k}, A.cplx);
-    r.Vt = c.zeros({k, n}, A.cplx);
-    scatter_cols(c, r.U, f.U, nzr, jk, jk);
-    scatter_rows(c, r.Vt, f.Vt, jk, jk, nzc);
-    r.S = f.S;
-    r.S.resize(k, 0.0);
-    return r;
-  }
-  return svd_dense(c, A);
-}
+  DensePlan d;
+};
 
-static SVD svd_dense(Ctx& c, const Tensor& A) {
+DensePlan dense_prepare(Ctx& c, const Tensor& A) {
   // A has no zero row or column
   TierGuard fp64(c, TIER_FP64);
+  DensePlan p;
   const int64_t m = A.dims[0], n = A.dims[1];
-  SVD r;
-  const bool tr = m < n;
-  Tensor B = tr ? conj_transpose(c, A) : A;
-  const int64_t mb = B.dims[0], nb = B.dims[1];
+  p.tr = m < n;
+  Tensor B = p.tr ? conj_transpose(c, A) : A;
+  p.mb = B.dims[0];
+  p.nb = B.dims[1];
   // a Householder QR first only where it shrinks the Jacobi work markedly
-  const bool useq = mb > 2 * nb && mb * nb > 65536;
-  Tensor W, Q;
-  if (useq) {
+  p.useq = p.mb > 2 * p.nb && p.mb * p.nb > 65536;
+  if (p.useq) {
     QR f = qr_thin(c, B);
-    Q = f.Q;
-    W = f.R;
+    p.Q = f.Q;
+    p.W = f.R;
   } else {
-    W = copy_tensor(c, B);
+    p.W = copy_tensor(c, B);
   }
-  Tensor V = c.alloc({nb, nb}, A.cplx), sg = c.alloc({nb}, false);
+  p.V = c.alloc({p.nb, p.nb}, A.cplx);
+  p.sg = c.alloc({p.nb}, false);
   // convergence threshold of LAPACK's dgesvj, sqrt(m) eps on the cosine of a
   // column pair (a fixed 1e-15 lies below the rounding floor for m > ~80 and
   // keeps pairs rotating until the sweep limit)
-  const double tol = std::max(1e-15, std::sqrt(double(W.dims[0])) * 1.1102230246251565e-16);
-  const int sweeps = jacobi_svd(c, W, V, sg, tol, 80);
+  p.tol = std::max(1e-15, std::sqrt(double(p.W.dims[0])) * 1.1102230246251565e-16);
+  return p;
+}
+
+SVD dense_finish(Ctx& c, DensePlan& p, int sweeps) {
+  TierGuard fp64(c, TIER_FP64);
+  const int64_t mb = p.mb, nb = p.nb;
   c.count("svd[sweeps]", double(sweeps), 0);
-  std::vector<double> s = c.to_host(sg);
+  std::vector<double> s = c.to_host(p.sg);
   std::vector<int64_t> ord(nb);
   for (int64_t i = 0; i < nb; ++i) ord[i] = i;
   std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return s[x] > s[y]; });
   std::vector<double> inv(nb);
   for (int64_t i = 0; i < nb; ++i) inv[i] = s[i] > 1e-140 ? 1.0 / s[i] : 0.0;  // columns below 1e-140 are left unrotated: zero
-  scale_cols(c, W, inv);
-  Tensor Ub = useq ? matmul(c, Q, W) : W;
+  scale_cols(c, p.W, inv);
+  Tensor Ub = p.useq ? matmul(c, p.Q, p.W) : p.W;
   std::vector<int64_t> rows_u(mb), rows_v(nb);
   for (int64_t i = 0; i < mb; ++i) rows_u[i] = i;
   for (int64_t i = 0; i < nb; ++i) rows_v[i] = i;
   Ub = gather2d(c, Ub, rows_u, ord);
-  Tensor Vb = gather2d(c, V, rows_v, ord);
+  Tensor Vb = gather2d(c, p.V, rows_v, ord);
+  SVD r;
   r.S.resize(nb);
   for (int64_t i = 0; i < nb; ++i) r.S[i] = s[ord[i]];
-  if (!tr) {
+  if (!p.tr) {
     r.U = Ub;
     r.Vt = conj_transpose(c, Vb);
   } else {
@@ -237,6 +221,91 @@ static SVD svd_dense(Ctx& c, const Tensor& A) {
     r.Vt = conj_transpose(c, Ub);
   }
   return r;
+}
+
+ThinPlan thin_prepare(Ctx& c, const Tensor& A) {
+  PEPS_CHECK(A.rank() == 2, PS_ESHAPE, "svd: matrix expected");
+  ThinPlan t;
+  t.m = A.dims[0];
+  t.n = A.dims[1];
+  t.k = std::min(t.m, t.n);
+  check_int_range(t.m, t.n);
+  if (t.k == 0) {
+    t.direct = true;
+    t.out.U = c.alloc({t.m, 0}, A.cplx);
+    t.out.Vt = c.alloc({0, t.n}, A.cplx);
+    return t;
+  }
+  if (!check_input(c, A, "svd")) {  // A = 0: S = 0 with identity vectors
+    t.direct = true;
+    t.out.U = eye(c, t.m, t.k, A.cplx);
+    t.out.S.assign(t.k, 0.0);
+    t.out.Vt = conj_transpose(c, eye(c, t.n, t.k, A.cplx));
+    return t;
+  }
+  // rows and columns that are exactly zero (Gauss-masked blocks have many)
+  // do not change the non-zero singular triplets: factorise the compacted
+  // block and pad with zero triplets
+  nonzero_lines(c, A, &t.nzr, &t.nzc);
+  t.compact = int64_t(t.nzr.size()) < t.m || int64_t(t.nzc.size()) < t.n;
+  t.d = dense_prepare(c, t.compact ? gather2d(c, A, t.nzr, t.nzc) : A);
+  return t;
+}
+
+SVD thin_finish(Ctx& c, ThinPlan& t, int sweeps) {
+  if (t.direct) return t.out;
+  SVD f = dense_finish(c, t.d, sweeps);
+  if (!t.compact) return f;
+  const bool cp = f.U.cplx;
+  const int64_t kc = int64_t(f.S.size());
+  std::vector<int64_t> jk(kc);
+  for (int64_t i = 0; i < kc; ++i) jk[i] = i;
+  SVD r;
+  r.U = c.zeros({t.m, t.k}, cp);
+  r.Vt = c.zeros({t.k, t.n}, cp);
+  scatter_cols(c, r.U, f.U, t.nzr, jk, jk);
+  scatter_rows(c, r.Vt, f.Vt, jk, jk, t.nzc);
+  r.S = f.S;
+  r.S.resize(t.k, 0.0);
+  return r;
+}
+}  // namespace
+
+// Thin SVD A = U diag(S) Vt with S descending.  Wide inputs are handled
+// through their conjugate transpose; inputs much taller than wide are first
+// reduced by a Householder QR (rocSOLVER geqrf/orgqr); the one-sided Jacobi
+// kernel (kernels.hip) diagonalises what remains.  Left vectors of zero
+// singular values are zero, except for A = 0 (identity vectors).
+SVD svd_thin(Ctx& c, const Tensor& A) {
+  ThinPlan t = thin_prepare(c, A);
+  int sweeps = 0;
+  if (!t.direct) sweeps = jacobi_svd(c, t.d.W, t.d.V, t.d.sg, t.d.tol, 80);
+  return thin_finish(c, t, sweeps);
+}
+
+// svd_thin of several independent matrices; their Jacobi diagonalisations
+// run concurrently (one workgroup each).  Same results as one by one.
+std::vector<SVD> svd_thin_batch(Ctx& c, const std::vector<Tensor>& As) {
+  std::vector<ThinPlan> plans;
+  plans.reserve(As.size());
+  for (const Tensor& A : As) plans.push_back(thin_prepare(c, A));
+  std::vector<JacobiJob> jobs;
+  std::vector<size_t> of;
+  for (size_t i = 0; i < plans.size(); ++i)
+    if (!plans[i].direct) {
+      jobs.push_back({&plans[i].d.W, &plans[i].d.V, &plans[i].d.sg, plans[i].d.tol, 0});
+      of.push_back(i);
+    }
+  std::vector<int> sweeps(plans.size(), 0);
+  if (!jobs.empty()) {
+    TierGuard fp64(c, TIER_FP64);
+    jacobi_svd_batch(c, jobs, 80);
+    for (size_t j = 0; j < jobs.size(); ++j) sweeps[of[j]] = jobs[j].sweeps;
+  }
+  std::vector<SVD> out;
+  out.reserve(plans.size());
+  for (size_t i = 0; i < plans.size(); ++i) out.push_back(thin_finish(c, plans[i], sweeps[i]));
+  return out;
 }
 
 EIG eigh(Ctx& c, const Tensor& A) {

@@ -294,6 +294,7 @@ struct SVD {
   std::vector<double> S;   // descending
 };
 SVD svd_thin(Ctx& c, const Tensor& A);
+std::vector<SVD> svd_thin_batch(Ctx& c, const std::vector<Tensor>& As);
 struct EIG {
   std::vector<double> w;   // ascending
   Tensor V;
@@ -304,5 +305,16 @@ EIG eigh(Ctx& c, const Tensor& A);
 // by its orthogonalised columns u_j sigma_j, V gets the right vectors and sig
 // the column norms (unsorted).
 int jacobi_svd(Ctx& c, Tensor& A, Tensor& V, Tensor& sig, double tol, int max_sweeps);
+// One one-sided Jacobi SVD per job (W m x n with m >= n, V n x n, sig n):
+// the matrices the register kernels take run concurrently, one workgroup
+// each; sweeps of each job are returned in its `sweeps`.
+struct JacobiJob {
+  Tensor* W;
+  Tensor* V;
+  Tensor* sig;
+  double tol;
+  int sweeps;
+};
+void jacobi_svd_batch(Ctx& c, std::vector<JacobiJob>& jobs, int max_sweeps);
 
 }  // namespace peps

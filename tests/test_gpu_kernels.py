@@ -136,7 +136,7 @@ def test_svd(ctx, cplx):
     from lgt_amd import peps
     rng = np.random.default_rng(8)
     for (m, n) in [(1, 1), (1, 7), (7, 1), (3, 3), (8, 24), (24, 8), (16, 32), (40, 40), (130, 70),
-                   (70, 200), (300, 288), (2000, 60)]:
+                   (70, 200), (160, 150), (150, 161), (171, 171), (300, 288), (2000, 60)]:
         A = rng.standard_normal((m, n)) + (1j * rng.standard_normal((m, n)) if cplx else 0)
         if min(m, n) > 3:
             A[:, 1] = 0

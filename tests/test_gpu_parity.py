@@ -123,6 +123,26 @@ def test_charge_svd(ctx):
             assert rel((U * S) @ Vt, (Ur * Sr) @ Vr) < 1e-12
 
 
+def test_charge_svd_large_sectors(ctx):
+    """Sector blocks of 40..260 rows: their SVDs run concurrently in one
+    batched launch per register-kernel variant (real <= 160, <= 256, complex)."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(15)
+    rq = rng.integers(0, 4, 470)
+    cq = rng.integers(0, 4, 430)
+    rq[:40] = 3
+    rq[40:] = np.where(rq[40:] == 3, 0, rq[40:])
+    for cp in (False, True):
+        Th = rng.standard_normal((470, 430)) + (1j * rng.standard_normal((470, 430)) if cp else 0)
+        Th[rq[:, None] != cq[None, :]] = 0
+        for D in (6, 40):
+            U, S, Vt, q = peps.charge_svd(Th, rq, cq, D, ctx=ctx)
+            Ur, Sr, Vr, qr = O.charge_svd(Th.astype(complex), rq, cq, D)
+            np.testing.assert_allclose(S, Sr, rtol=1e-12)
+            np.testing.assert_array_equal(q, qr)
+            assert rel((U * S) @ Vt, (Ur * Sr) @ Vr) < 1e-11
+
+
 def test_als_truncate(ctx, s33):
     from lgt_amd import peps
     for (ix, iy, orient) in [(1, 1, 0), (2, 2, 0), (2, 1, 1)]:
