@@ -345,6 +345,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       }
     }
   };
+  // Interior tiles (all 128 rows of A and B in range) and full k-tiles take
+  // a branch-free path: the tile base is uniform, each thread adds one
+  // constant lane offset, and the (e, j) steps are uniform multiples of the
+  // leading dimension (lane group tid >> 7 is the same across a wave).
+  const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+  const int kgrp = __builtin_amdgcn_readfirstlane(tid >> 7);
+  const int64_t laneA = AK ? int64_t(tid >> 3) * lda + 4 * (tid & 7) : int64_t(tid & 127);
+  const int64_t laneB = BKF ? int64_t(tid >> 3) * ldb + 4 * (tid & 7) : int64_t(tid & 127);
+  auto gload_fast = [&](int64_t k0) {
+    const TIN* Ab = A + (AK ? m0 * lda + k0 : m0 + k0 * lda) + laneA;
+    const TIN* Bb = B + (BKF ? n0 * ldb + k0 : n0 + k0 * ldb) + laneB;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        ra[e][j] = Ab[AK ? int64_t(32 * e) * lda + j : int64_t(4 * (kgrp + 2 * e) + j) * lda];
+        rb[e][j] = Bb[BKF ? int64_t(32 * e) * ldb + j : int64_t(4 * (kgrp + 2 * e) + j) * ldb];
+      }
+  };
+  auto load = [&](int64_t k0) {
+    if (interior && k0 + BK <= kend) gload_fast(k0);
+    else gload(k0);
+  };
   auto lstore = [&](int buf) {
     float* As = smem + buf * BUF;
     float* Bs = As + BM * LDK;
@@ -359,13 +382,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
   const int64_t ntk = (kend - kbeg + BK - 1) / BK;
   if (ntk > 0) {
-    gload(kbeg);
+    load(kbeg);
     lstore(0);
   }
   __syncthreads();
   for (int64_t t = 0; t < ntk; ++t) {
     const bool more = t + 1 < ntk;
-    if (more) gload(kbeg + (t + 1) * BK);
+    if (more) load(kbeg + (t + 1) * BK);
     const float* As = smem + (t & 1) * BUF;
     const float* Bs = As + BM * LDK;
     v4f a[4][2], b[4][2];
