@@ -43,7 +43,7 @@ GEMM_KERNEL = {"fp64": "gemm_kernel (FP64 MFMA)", "fp32": "gemm_f32_tiled (FP32 
 # FETCH_SIZE and WRITE_SIZE over the timed step of this workload (copy-
 # calibrated factors 2.0 and 1.0), keyed by (nx, ny, D, chi, gd, tier);
 # profiles/ holds the summary.
-PROFILED_TRAFFIC = {(8, 8, 6, 144, 2, "fp32"): (2.0 * 1.8405e6 + 1.0 * 9.2119e5) * 1024.0}
+PROFILED_TRAFFIC = {(8, 8, 6, 144, 2, "fp32"): (2.0 * 2.2456e6 + 1.0 * 9.2521e5) * 1024.0}
 
 
 def log(msg):
