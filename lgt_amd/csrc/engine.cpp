@@ -927,6 +927,24 @@ void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,
     X = copy_tensor(c, X0);
     Y = copy_tensor(c, Y0);
   }
+  // The refinement is kept only if cost1 < cost0, and cost1 >= 0 with
+  // positive_cost.  When no cost1 can pass -- cost0 NaN or -inf, or
+  // cost0 <= 0 under positive_cost -- the sweeps cannot change the result
+  // and are not run: the SVD initialisation is returned, with cost1 = cost0
+  // (its cost) and no sweeps.
+  if (std::isnan(cost0) || cost0 == -HUGE_VAL || (positive_cost && cost0 <= 0.0)) {
+    if (info) {
+      info->cost0 = cost0;
+      info->cost1 = cost0;
+      info->accepted = false;
+      info->sweeps = 0;
+    }
+    *Xo = X0;
+    *Yo = Y0;
+    *bflux = cs.q;
+    c.count("als[skipped: no acceptable outcome]", 1.0, 0);
+    return;
+  }
   double prev = tol > 0 ? als_cost(c, th, env, X, Y, &Eth) : 0.0;
   int sweeps = 0;
   double cost1 = std::nan("");

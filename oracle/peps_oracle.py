@@ -947,14 +947,18 @@ def als_cost(th, env, X, Y):
     return thth - 2 * np.real(XYth) + XYXY
 
 
-def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=False, init=None):
+def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=False, init=None,
+                 positive_cost=False):
     """finite_peps_fullupdate_v4.jl:94-154.
 
     init=(X0, Y0) replaces the charge-SVD initial bond tensors (the SVD still
     fixes the bond fluxes and the mask).  For complex tensors the sweeps are
     not covariant under the phase freedom of the singular vectors (NX pairs
     Y with conj(Y) in the reference's index order), so comparing two
-    implementations of the sweeps needs a shared init."""
+    implementations of the sweeps needs a shared init.
+
+    positive_cost (no reference counterpart) also requires cost1 >= 0 for
+    the refinement to be kept; the sweeps always run here."""
     nA, dpL, dpR, nB = th.shape
     thm = jr(pd(th, 1, 2, 4, 3), nA * dpL, nB * dpR)
     row_charge = np.array([eRofpL[r // nA] for r in range(nA * dpL)])
@@ -985,7 +989,7 @@ def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=F
             s = jr(SY[:, pR, :], nB * Dp, 1)
             Y[:, pR, :] = jr(solve_herm(Nm, s, reg), nB, Dp)
     cost1 = als_cost(th, env, X, Y)
-    accepted = bool(np.isfinite(cost1) and cost1 < cost0)
+    accepted = bool(np.isfinite(cost1) and cost1 < cost0 and (not positive_cost or cost1 >= 0))
     if accepted:
         out = (X, Y, bflux)
     else:
@@ -1060,8 +1064,10 @@ def recanonicalise_v(gp, ix, iy, Dt, Ut, D_max):
     gp.qv[ix - 1][iy - 1] = newq
 
 
-def full_update_bond_h_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=None):
-    """finite_peps_fullupdate_v4.jl:165-193."""
+def full_update_bond_h_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=None, positive_cost=False,
+                          infos=None):
+    """finite_peps_fullupdate_v4.jl:165-193 (infos: a list that receives the
+    ALS info dict of the bond)."""
     gd = gp.gd
     apply_mask(gp, ix, iy); apply_mask(gp, ix + 1, iy)
     E = bond_environment_h(gp, ix, iy, chi, top=top, bot=bot)
@@ -1070,15 +1076,19 @@ def full_update_bond_h_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=
     th = np.einsum("apk,bqk->apqb", E["RA"], E["RB"], optimize=True)
     th = apply_gate(th, G, dpL, dpR)
     eR = [decode_phys(p, dgRL, dgUL, gd)[1] for p in range(1, dpL + 1)]
-    X, Y, bflux = als_truncate(th, E["env"], D_max, eR, n_als=n_als)
+    X, Y, bflux, info = als_truncate(th, E["env"], D_max, eR, n_als=n_als, return_info=True,
+                                     positive_cost=positive_cost)
+    if infos is not None:
+        infos.append(info)
     Afull = np.einsum("luda,apr->plrud", E["QA"], X, optimize=True)
     Bfull = np.einsum("rudb,bpl->plrud", E["QB"], Y, optimize=True)
     recanonicalise_h(gp, ix, iy, Afull, Bfull, D_max)
     apply_mask(gp, ix, iy); apply_mask(gp, ix + 1, iy)
 
 
-def full_update_bond_v_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=None):
-    """finite_peps_fullupdate_v4.jl:274-299."""
+def full_update_bond_v_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=None, positive_cost=False,
+                          infos=None):
+    """finite_peps_fullupdate_v4.jl:274-299 (infos as full_update_bond_h_v4)."""
     gd = gp.gd
     apply_mask(gp, ix, iy); apply_mask(gp, ix, iy + 1)
     E = bond_environment_v(gp, ix, iy, chi, top=top, bot=bot)
@@ -1087,28 +1097,32 @@ def full_update_bond_v_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=
     th = np.einsum("apk,bqk->apqb", E["RA"], E["RB"], optimize=True)
     th = apply_gate(th, G, dpD, dpU)
     eU = [decode_phys(p, dgRD, dgUD, gd)[2] for p in range(1, dpD + 1)]
-    X, Y, bflux = als_truncate(th, E["env"], D_max, eU, n_als=n_als)
+    X, Y, bflux, info = als_truncate(th, E["env"], D_max, eU, n_als=n_als, return_info=True,
+                                     positive_cost=positive_cost)
+    if infos is not None:
+        infos.append(info)
     Afull = np.einsum("lrda,apu->plrud", E["QA"], X, optimize=True)
     Bfull = np.einsum("lrub,bpd->plrud", E["QB"], Y, optimize=True)
     recanonicalise_v(gp, ix, iy, Afull, Bfull, D_max)
     apply_mask(gp, ix, iy); apply_mask(gp, ix, iy + 1)
 
 
-def trotter_step_v4(gp, g, t_hop, m, tau, D_max, chi=64, n_als=30):
-    """trotter_step_v4!, finite_peps_fullupdate_v4.jl:415-432."""
+def trotter_step_v4(gp, g, t_hop, m, tau, D_max, chi=64, n_als=30, positive_cost=False, infos=None):
+    """trotter_step_v4!, finite_peps_fullupdate_v4.jl:415-432 (positive_cost
+    and infos are passed to every bond update)."""
     nxv, nyv, gd = gp.nx, gp.ny, gp.gd
     for iy in range(1, nyv + 1):
         for ix in range(1, nxv):
             G = gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2)
-            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als)
+            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos)
     for iy in range(1, nyv):
         for ix in range(1, nxv + 1):
             G = gate(H_merged_v_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau)
-            full_update_bond_v_v4(gp, ix, iy, G, D_max, chi, n_als)
+            full_update_bond_v_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos)
     for iy in range(1, nyv + 1):
         for ix in range(nxv - 1, 0, -1):
             G = gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2)
-            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als)
+            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos)
 
 
 def bond_expect(E, h, dpL, dpR):

@@ -143,6 +143,39 @@ def test_charge_svd_large_sectors(ctx):
             assert rel((U * S) @ Vt, (Ur * Sr) @ Vr) < 1e-11
 
 
+@pytest.mark.parametrize("name,D,chi", [("s33", 2, 64), ("s34", 3, 10)])
+def test_trotter_step_v4_positive_cost_skips_decided_bonds(ctx, request, name, D, chi):
+    """With PEPS_STEP_POSITIVE_COST a refinement is kept only if
+    0 <= cost1 < cost0, so a bond whose SVD initialisation already has
+    cost0 <= 0 keeps it whatever the sweeps do: the library skips those
+    sweeps, the oracle runs them all.  The steps must agree as in
+    test_trotter_step_v4_with_als, and the library must report fewer sweeps
+    than bonds x n_als (most bonds of these states start at cost0 <= 0)."""
+    from lgt_amd import peps
+    go0 = request.getfixturevalue(name)
+    n_als = 4
+    go = go0.copy()
+    gd = to_device(go, ctx)
+    st = peps.trotter_step_v4(gd, G, T_HOP, M, 0.05, D, chi, n_als, positive_cost=True)
+    infos = []
+    O.trotter_step_v4(go, G, T_HOP, M, 0.05, D, chi, n_als, positive_cost=True, infos=infos)
+    e_ref = O.compute_energy_v4(go, G, T_HOP, M, chi)
+    spread = 0.0
+    for seed in (1, 2, 3):
+        gp = _perturbed(go0, 1e-14, seed)
+        O.trotter_step_v4(gp, G, T_HOP, M, 0.05, D, chi, n_als, positive_cost=True)
+        spread = max(spread, abs(O.compute_energy_v4(gp, G, T_HOP, M, chi) - e_ref) / abs(e_ref))
+    e_gpu = peps.compute_energy_v4(gd, G, T_HOP, M, chi)
+    de = abs(e_gpu - e_ref) / abs(e_ref)
+    decided = sum(1 for i in infos if not i["cost0"] > 0)
+    print(f"|dE|/|E| {de:.3e} (oracle spread {spread:.3e}); bonds {st['bonds']}, sweeps {st['sweeps']}, "
+          f"oracle bonds with cost0 <= 0: {decided}")
+    assert de <= max(5e-9, 10.0 * spread)
+    assert st["bonds"] == len(infos)
+    assert st["sweeps"] < st["bonds"] * n_als
+    assert O.forbidden_norm(to_oracle(gd)) < 1e-12
+
+
 def test_als_truncate(ctx, s33):
     from lgt_amd import peps
     for (ix, iy, orient) in [(1, 1, 0), (2, 2, 0), (2, 1, 1)]:
