@@ -19,6 +19,12 @@ PEPS_STEP_POSITIVE_COST (an ALS refinement is kept only if its cost is >= 0:
 the chi-truncated environments are not positive semidefinite, and accepting
 negative-cost bonds zeroes the 8x8 state within a sweep).  Prints ONE JSON
 line.
+
+Precision: the default tier is FP64, the reference's precision.  It is also
+the fastest tier on this workload: in the FP32 tier the contraction rounding
+noise (~1e-7 relative) lies far above the boundary compression's 1e-14 cut,
+so every boundary MPS stays at chi=144, and the ALS needs about twice the
+sweeps; --tier fp32 --compare-tier fp64 measures both.
 """
 from __future__ import annotations
 
@@ -37,13 +43,14 @@ if ROOT not in sys.path:
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 # the dominant kernel of each tier: its launches are timed as one class
 GEMM_CLASS = {"fp64": "gemm_f64", "fp32": "gemm_f32_tiled", "bf16": "gemm_bf16"}
-GEMM_KERNEL = {"fp64": "gemm_kernel (FP64 MFMA)", "fp32": "gemm_f32_tiled (FP32 MFMA, 128x128x32 tiles)",
+GEMM_KERNEL = {"fp64": "gemm_kernel (FP64 MFMA, 128x128x16 tiles)", "fp32": "gemm_f32_tiled (FP32 MFMA, 128x128x32 tiles)",
                "bf16": "gemm_kernel (BF16 MFMA)"}
 # HBM bytes per launch of the dominant kernel measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE over the timed step of this workload (copy-
 # calibrated factors 2.0 and 1.0), keyed by (nx, ny, D, chi, gd, tier);
 # profiles/ holds the summary.
-PROFILED_TRAFFIC = {(8, 8, 6, 144, 2, "fp32"): (2.0 * 2.2456e6 + 1.0 * 9.2521e5) * 1024.0}
+PROFILED_TRAFFIC = {(8, 8, 6, 144, 2, "fp32"): (2.0 * 2.2456e6 + 1.0 * 9.2521e5) * 1024.0,
+                    (8, 8, 6, 144, 2, "fp64"): (2.0 * 1.2191e5 + 1.0 * 4.6419e4) * 1024.0}
 
 
 def log(msg):
@@ -208,7 +215,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--tier", default="fp32", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--tier", default="fp64", choices=["fp64", "fp32", "bf16"])
     ap.add_argument("--nx", type=int, default=8)
     ap.add_argument("--ny", type=int, default=8)
     ap.add_argument("--D", type=int, default=6)
@@ -221,6 +228,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--energy", action="store_true", help="also report E/site after the timed steps")
+    ap.add_argument("--compare-tier", default=None, choices=["fp64", "fp32", "bf16"],
+                    help="after the timed steps, run the same steps from the same initial state in this tier "
+                         "(own context) and report its step time, the ratio and |dE/site|")
     ap.add_argument("--compare-fp64", action="store_true",
                     help="after the timed steps, run the same steps from the same initial state in the FP64 "
                          "tier (the plain FP64 GPU form) and report its step time, the speedup and |dE/site|")
@@ -232,6 +242,8 @@ def main():
                          "the profiled run, whose clock window selects the timed region) and exit")
     ap.add_argument("--bench-log", default=None)
     a = ap.parse_args()
+    if a.compare_fp64 and a.compare_tier is None:
+        a.compare_tier = "fp64"
     if a.summarize_rocprof:
         summarize_rocprof(a.summarize_rocprof, a.bench_log)
         return
@@ -266,7 +278,7 @@ def main():
     bond_dims = {"mean": float(np.mean(bd)), "max": int(max(bd)), "at_D": int(sum(1 for x in bd if x == a.D)),
                  "bonds": len(bd)}
     log(f"bond dimensions: {bond_dims}")
-    start = gp.to_arrays() if a.compare_fp64 else None
+    start = gp.to_arrays() if a.compare_tier else None
     ctx.set_tier(a.tier)
     for w in range(a.warmup):
         tw = time.perf_counter()
@@ -298,6 +310,8 @@ def main():
     gk = stats.get(GEMM_CLASS[a.tier], dict(flops=0.0, ms=0.0, launches=0, bytes=0.0))
     achieved = gk["flops"] / (gk["ms"] / 1e3) / 1e12 if gk["ms"] > 0 else 0.0
     total_flops = sum(v["flops"] for v in stats.values())
+    svd_ms = sum(v["ms"] for k, v in stats.items() if k.startswith("svd") and k != "svd[sweeps]")
+    svd_n = sum(v["launches"] for k, v in stats.items() if k.startswith("svd") and k != "svd[sweeps]")
     for name, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
         rate = f" {v['bytes'] / v['ms'] / 1e6:8.1f} GB/s" if v["ms"] > 0 and v["bytes"] > 0 else ""
         log(f"  {name:12s} launches={v['launches']:8d} ms={v['ms']:10.1f} flop={v['flops']:.3e}{rate}")
@@ -332,16 +346,32 @@ def main():
                      "avg_launch_ms": gk["ms"] / max(1, gk["launches"]),
                      "algorithmic_flop_per_launch": gk["flops"] / max(1, gk["launches"]),
                      "algorithmic_bytes_per_launch": gk["bytes"] / max(1, gk["launches"])},
+        # the one-sided Jacobi SVD kernels (one workgroup per matrix:
+        # latency bound, no HBM or MFMA roofline applies) and their share of
+        # the step, beside the GEMM's
+        "svd_kernel": {"kernel": "jacobi_svd_reg_batched_kernel / jacobi_svd_reg_kernel", "bound": "latency",
+                       "ms_per_step": svd_ms / a.steps, "launches": svd_n,
+                       "share_of_step": svd_ms / max(1e-9, 1e3 * elapsed),
+                       "gemm_share_of_step": gk["ms"] / max(1e-9, 1e3 * elapsed)},
     }
-    if a.energy or a.compare_fp64:
+    if a.energy or a.compare_tier:
         e = peps.compute_energy_v4(gp, g, t, m, a.chi, renorm=True)
         out["config"]["energy_per_site"] = e / (a.nx * a.ny)
         log(f"E/site after the timed steps ({a.tier}): {e / (a.nx * a.ny):.12f}")
-    if a.compare_fp64:
+    if a.compare_tier:
+        # a context of its own (an empty Ritz-basis cache), warmed like the
+        # timed run's: one norm evaluation of the start state, as normalise()
+        # ends with.  The timed run's state and context (and the memory its
+        # pool caches) are released first.
+        gch, gd, cplx = gp.gch, gp.gd, gp.is_complex
+        gp.__del__()
+        ctx.close()
+        ctx64 = peps.Context(0)
         g64 = peps.GaugePEPS.from_arrays(start["tensors"], start["lh"], start["lv"], start["qh"], start["qv"],
-                                         gp.gch, gp.gd, is_complex=gp.is_complex, ctx=ctx)
-        ctx.set_tier("fp64")
-        ctx.sync()
+                                         gch, gd, is_complex=cplx, ctx=ctx64)
+        peps.peps_norm2(g64, a.chi)
+        ctx64.set_tier(a.compare_tier)
+        ctx64.sync()
         t64 = time.perf_counter()
         als64 = dict(bonds=0, accepted=0, sweeps=0)
         for s64 in range(a.warmup + a.steps):
@@ -349,12 +379,12 @@ def main():
                                         positive_cost=True)
             for k in als64:
                 als64[k] += st64[k]
-            ctx.sync()
-            log(f"fp64 step {s64 + 1}/{a.warmup + a.steps}: {time.perf_counter() - t64:.2f} s  als={st64}")
+            ctx64.sync()
+            log(f"{a.compare_tier} step {s64 + 1}/{a.warmup + a.steps}: {time.perf_counter() - t64:.2f} s  als={st64}")
         ms64 = 1e3 * (time.perf_counter() - t64) / (a.warmup + a.steps)
         e64 = peps.compute_energy_v4(g64, g, t, m, a.chi, renorm=True)
-        log(f"E/site after the fp64 steps: {e64 / (a.nx * a.ny):.12f}")
-        out["config"]["fp64_gpu"] = {"ms_per_step": ms64, "speedup": ms64 / out["ms_per_step"],
+        log(f"E/site after the {a.compare_tier} steps: {e64 / (a.nx * a.ny):.12f}")
+        out["config"][f"{a.compare_tier}_gpu"] = {"ms_per_step": ms64, "time_ratio": ms64 / out["ms_per_step"],
                                      "energy_per_site": e64 / (a.nx * a.ny),
                                      "abs_dE_per_site": abs(e - e64) / (a.nx * a.ny),
                                      "als_sweeps_per_step": als64["sweeps"] / max(1, a.warmup + a.steps),
