@@ -20,9 +20,12 @@ Generalisation beyond the reference (recorded in the design notes):
   * Gauss law in label units: lR + lU - fl - fd == 2 (n_f + q).
 
 Pins (the reference ships no golden vectors and its Julia code cannot run
-here): each function is checked in tests/test_oracle.py against the
+here): each function is checked in tests/test_oracle.py,
+tests/test_oracle_dense_su.py and tests/test_oracle_mlp.py against the
 reference's own loop formulas, dense brute force on tiny lattices, library
-routines (numpy/scipy SVD, eigh, pinvh) or invariants.  Parity unpinned:
+routines (numpy/scipy SVD, eigh, pinvh), closed forms, finite differences or
+invariants; the positive-cost accept rule (no reference counterpart) by
+test_als_positive_cost_rule.  Parity unpinned:
 ``trotter_step_v4`` / ``_recanonicalise`` beyond one bond — their multi-step
 trajectories depend on LAPACK phase and tie choices that only the Julia
 reference could fix; they are pinned only through the per-bond pieces

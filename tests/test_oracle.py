@@ -252,6 +252,28 @@ def test_als_identity_env_is_truncated_svd():
     assert abs(info["cost0"] - np.sum(np.abs(thm - (U * S) @ Vt) ** 2)) < 1e-10
 
 
+def test_als_positive_cost_rule():
+    """env = -δ⊗δ makes every cost -||θ - XY||² <= 0.  The floored solve then
+    zeroes X (no eigenvalue lies above the floor), so the sweeps end at
+    cost1 = -||θ||² < cost0: the reference's rule keeps that refinement, the
+    positive-cost rule must keep the SVD initialisation instead."""
+    rng = np.random.default_rng(17)
+    nA, dp, nB = 3, 2, 3
+    eR = [-1, 1]
+    th = rng.standard_normal((nA, dp, dp, nB))
+    env = -np.einsum("ac,bd->abcd", np.eye(nA), np.eye(nB)).astype(complex)
+    X0, Y0, _, i0 = O.als_truncate(th, env, 2, eR, n_als=0, return_info=True)
+    assert i0["cost0"] < 0 and not i0["accepted"]
+    X, Y, _, info = O.als_truncate(th, env, 2, eR, n_als=2, return_info=True)
+    assert info["accepted"]
+    assert abs(info["cost1"] + np.sum(th ** 2)) < 1e-10 and info["cost1"] < info["cost0"]
+    assert np.abs(X).max() == 0.0
+    Xp, Yp, _, ip = O.als_truncate(th, env, 2, eR, n_als=2, return_info=True, positive_cost=True)
+    assert not ip["accepted"]
+    np.testing.assert_array_equal(Xp, X0)
+    np.testing.assert_array_equal(Yp, Y0)
+
+
 def test_lattice_init_matches_oracle_init():
     from lgt_amd import lattice
     nx, ny = 3, 4
