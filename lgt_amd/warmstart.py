@@ -8,7 +8,8 @@ and their sector labels and rescales the kept weights: a small MLP
 (``peps.Mlp``, trained and evaluated on the GPU) maps features of the bond --
 the top ``K`` singular values of theta and the environment norms of the
 ``K`` rank-one terms U_k S_k V_k^H, each normalised -- to log-ratios r_k
-between the weights of the ALS-converged bond and the SVD weights.
+between the environment-optimal weights of those rank-one terms (the
+training targets, ``env_optimal_targets``) and the SVD weights.
 
 Fidelity gate: the proposed initial bond is used only if its environment
 cost ||theta - X Y||_env is not above that of the SVD initialisation; the
