@@ -40,17 +40,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+METRIC = "full-update steps/s at 8x8 D=6 chi=144; speedup vs plain FP64 GPU at dE/site<1e-6"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+PEAK_HBM_GBS = 8000.0
 # the dominant kernel of each tier: its launches are timed as one class
 GEMM_CLASS = {"fp64": "gemm_f64", "fp32": "gemm_f32_tiled", "bf16": "gemm_bf16"}
-GEMM_KERNEL = {"fp64": "gemm_kernel (FP64 MFMA, 128x128x16 tiles)", "fp32": "gemm_f32_tiled (FP32 MFMA, 128x128x32 tiles)",
-               "bf16": "gemm_kernel (BF16 MFMA)"}
-# HBM bytes per launch of the dominant kernel measured with rocprofv3
-# FETCH_SIZE and WRITE_SIZE over the timed step of this workload (copy-
-# calibrated factors 2.0 and 1.0), keyed by (nx, ny, D, chi, gd, tier);
-# profiles/ holds the summary.
-PROFILED_TRAFFIC = {(8, 8, 6, 144, 2, "fp32"): (2.0 * 2.2456e6 + 1.0 * 9.2521e5) * 1024.0,
-                    (8, 8, 6, 144, 2, "fp64"): (2.0 * 1.2191e5 + 1.0 * 4.6419e4) * 1024.0}
+# HBM bytes per launch of a kernel class measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE over the timed step of a workload (copy-calibrated factors 2.0
+# and 1.0), keyed by (nx, ny, D, chi, gd, tier, class); profiles/ holds the
+# summaries.
+PROFILED_TRAFFIC = {}
 
 
 def log(msg):
@@ -115,30 +114,101 @@ def normalise(gp, nx, ny, chi):
     return first, peps.peps_norm2(gp, chi).real
 
 
-def cpu_baseline(flops_per_step, seconds=12.0):
-    """numpy (OpenBLAS, all host threads) on the dominant contraction shape of
-    the workload — the vertical-bond environment closure, an (a^2 x K) by
-    (K x b^2) product with a=b=48 — timed for a bounded ~10 s sample; the
-    CPU step time is the library's per-step algorithmic flop count divided
-    by that measured rate."""
-    from oracle import peps_oracle  # noqa: F401  (the oracle's numpy stack)
-    threads = os.cpu_count() or 1
-    rng = np.random.default_rng(0)
-    M = N = 48 * 48
-    K = 144 * 36  # one slice of the chi^2 D^4 closure cut
-    A = rng.standard_normal((M, K))
-    B = rng.standard_normal((K, N))
-    A @ B
-    n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        A @ B
-        n += 1
-    dt = time.perf_counter() - t0
-    rate = 2.0 * M * N * K * n / dt
-    return dict(value=rate / max(flops_per_step, 1.0), unit="steps/s", cores=threads, kind="port",
-                sample=(f"numpy/OpenBLAS FP64 GEMM {M}x{K}x{N} (v-bond environment closure slice) for "
-                        f"{dt:.1f} s = {rate / 1e9:.1f} GFLOP/s on {threads} threads; step time = "
-                        f"{flops_per_step:.3e} algorithmic flop/step / rate"))
+def cpu_baseline(gp, a, per_step, budget_s=25.0):
+    """The oracle (numpy FP64 complex128 on OpenBLAS, all its threads) timed on
+    pieces of the same workload, taken from the device state after the timed
+    steps, then scaled to one full-update step.
+
+    Pieces (interior bond (nx/2, ny/2); boundaries handed in from the library,
+    as the exactly cached schedule does):
+      * h-bond update: one spectator_left step, the environment closure, the
+        ALS (n_als sweeps, the same tolerance and accept rules) and the
+        recanonicalisation (the reference's charge SVD of the full d D^3
+        square Theta);
+      * boundary absorption + compress_mps! (the reference's QR/SVD sweep on
+        the explicit product) of one row at a boundary bond chi' small enough
+        for the sample, scaled by (chi_eff / chi')^3 (QR and SVD cost grows as
+        chi^3 at fixed D).
+    One step = (h + v bonds) x h-bond time + (6 ny - 14) x row time; the
+    vertical bonds are charged an h-bond's time (a lower bound: their strip
+    environments are larger), and the ALS part is rescaled from the sample's
+    sweeps to the step's measured sweeps per bond."""
+    import threadpoolctl
+    from lgt_amd import peps
+    from oracle import peps_oracle as O
+    threads = max([p.get("num_threads", 1) for p in threadpoolctl.threadpool_info()] or [1])
+    nx, ny, D, chi = a.nx, a.ny, a.D, a.chi
+    ix, iy = max(1, nx // 2), max(2, ny // 2)
+    arr = gp.to_arrays()
+    go = O.GaugePEPS(nx, ny, gp.gd, gp.gch)
+    go.tensors = [[np.asarray(t, dtype=np.complex128) for t in col] for col in arr["tensors"]]
+    go.lh, go.lv, go.qh, go.qv = arr["lh"], arr["lv"], arr["qh"], arr["qv"]
+    top = [np.asarray(t, np.complex128) for t in peps.boundary_top(gp, iy, chi)]
+    bot = [np.asarray(t, np.complex128) for t in peps.boundary_bottom(gp, iy, chi)]
+    t_start = time.perf_counter()
+    row = O.row_dl(go, iy)
+    # strip environments up to the bond; the last spectator step is the timed one
+    L = np.ones((1, 1, 1, 1), np.complex128)
+    t_spec = 0.0
+    for j in range(1, ix):
+        t0 = time.perf_counter()
+        L = O.spectator_left(L, top[j - 1], bot[j - 1], row[j - 1])
+        t_spec = time.perf_counter() - t0
+    R = np.ones((1, 1, 1, 1), np.complex128)
+    for j in range(nx, ix + 1, -1):
+        R = O.spectator_right(R, top[j - 1], bot[j - 1], row[j - 1])
+    O.apply_mask(go, ix, iy); O.apply_mask(go, ix + 1, iy)
+    t0 = time.perf_counter()
+    E = O.bond_environment_h(go, ix, iy, chi, top=top, bot=bot, Lenv=L, Renv=R)
+    t_close = time.perf_counter() - t0
+    dpL, dpR = E["RA"].shape[1], E["RB"].shape[1]
+    _, dgR, dgU = O.site_dims(ix, iy, nx, ny, gp.gd)
+    G = O.gate(O.H_merged_h_site(ix, iy, nx, ny, gp.gd, 1.0, 1.0, 0.5), a.tau / 2)
+    th = O.apply_gate(np.einsum("apk,bqk->apqb", E["RA"], E["RB"]), G, dpL, dpR)
+    eR = [O.decode_phys(p, dgR, dgU, gp.gd)[1] for p in range(1, dpL + 1)]
+    t0 = time.perf_counter()
+    X, Y, bf, info = O.als_truncate(th, E["env"], D, eR, n_als=a.n_als, return_info=True, positive_cost=True,
+                                    als_tol=a.als_tol)
+    t_als = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.als_truncate(th, E["env"], D, eR, n_als=0, positive_cost=True)
+    t_als0 = time.perf_counter() - t0
+    t_sweep = max(t_als - t_als0, 0.0) / max(info["sweeps"], 1)
+    Afull = np.einsum("luda,apr->plrud", E["QA"], X)
+    Bfull = np.einsum("rudb,bpl->plrud", E["QB"], Y)
+    t0 = time.perf_counter()
+    O.recanonicalise_h(go, ix, iy, Afull, Bfull, D)
+    t_recan = time.perf_counter() - t0
+    # one boundary row: absorb row iy-1 into the bottom boundary and compress,
+    # at a boundary bond chi' that keeps the sample bounded
+    chi_eff = max(1, int(round(per_step["bmps_bond_mean"])))
+    left = budget_s - (time.perf_counter() - t_start)
+    chi_s = chi_eff
+    # predicted complex QR+SVD cost of one column at bond c: ~3 x 8 (c D^4)(c D^2)^2
+    rate = 2.0e11
+    while chi_s > 4 and nx * 3 * 8.0 * (chi_s * D ** 4) * (chi_s * D * D) ** 2 / rate > max(left, 2.0):
+        chi_s = max(4, int(chi_s * 0.8))
+    bs = [t[:min(t.shape[0], chi_s), :, :min(t.shape[2], chi_s)] for t in bot]
+    t0 = time.perf_counter()
+    mps = O.absorb_row_from_bottom(bs, O.row_dl(go, iy))
+    O.compress_mps(mps, chi_s, relative=True)
+    t_row_s = time.perf_counter() - t0
+    t_row = t_row_s * (chi_eff / max(chi_s, 1)) ** 3
+    bonds = per_step["bonds"]
+    sweeps_per_bond = per_step["als_sweeps"] / max(bonds, 1)
+    t_bond = t_spec + t_close + t_als0 + sweeps_per_bond * t_sweep + t_recan
+    step_s = bonds * t_bond + (6 * ny - 14) * t_row
+    sample_s = time.perf_counter() - t_start
+    return dict(value=1.0 / step_s, unit="steps/s", cores=int(threads), kind="port",
+                seconds_per_step=step_s,
+                sample=(f"oracle (numpy FP64 complex128, {threads} OpenBLAS threads) on the state after the timed "
+                        f"steps, {sample_s:.1f} s of CPU work: h-bond ({ix},{iy}) spectator step {t_spec:.3f} s, "
+                        f"environment closure {t_close:.3f} s, ALS init {t_als0:.3f} s + {info['sweeps']} sweeps "
+                        f"({t_sweep:.3f} s each), recanonicalisation {t_recan:.3f} s; one boundary row absorbed and "
+                        f"compressed at chi'={chi_s}: {t_row_s:.3f} s, scaled by (chi_eff/chi')^3 to chi_eff="
+                        f"{chi_eff}: {t_row:.2f} s.  Step = {bonds} bonds x {t_bond:.3f} s (ALS at the step's "
+                        f"{sweeps_per_bond:.2f} sweeps/bond; v-bonds charged an h-bond's time, a lower bound) + "
+                        f"{6 * ny - 14} rows x {t_row:.2f} s = {step_s:.1f} s"))
 
 
 def summarize_rocprof(d, log_path=None):
@@ -210,11 +280,64 @@ def summarize_rocprof(d, log_path=None):
             print("  %-80s %-10s dispatches %7d mean %.4e KiB total %.4e KiB" % (name, cn, n, v / n, v))
 
 
+# kernel classes of the library's statistics -> (bound, peak, unit) of the
+# hardware resource that bounds them (MI355X published peaks)
+def class_roof(name, tier):
+    if name.startswith("gemm_f32") or name == "splitk_reduce_f32":
+        return "mfma", PEAK_TFLOPS["fp32"], "TFLOP/s"
+    if name.startswith("gemm_bf16"):
+        return "mfma", PEAK_TFLOPS["bf16"], "TFLOP/s"
+    if name.startswith(("permute", "transpose", "mask", "scale", "cast", "copy", "elementwise", "reduce",
+                        "splitk")):
+        return "hbm", PEAK_HBM_GBS, "GB/s"
+    return "mfma", PEAK_TFLOPS["fp64"], "TFLOP/s"
+
+
+def roofline_of(name, v, tier):
+    bound, peak, unit = class_roof(name, tier)
+    secs = v["ms"] / 1e3
+    if bound == "hbm":
+        achieved = v["bytes"] / secs / 1e9 if secs > 0 else 0.0
+    else:
+        achieved = v["flops"] / secs / 1e12 if secs > 0 else 0.0
+    return {"kernel_class": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "launches": v["launches"],
+            "avg_launch_ms": v["ms"] / max(1, v["launches"]),
+            "algorithmic_flop_per_launch": v["flops"] / max(1, v["launches"]),
+            "algorithmic_bytes_per_launch": v["bytes"] / max(1, v["launches"])}
+
+
+def run_steps(gp, ctx, a, count, label, deadline):
+    """Run up to `count` full-update steps, stopping early when another step
+    (at the mean step time so far) would pass `deadline` (perf_counter);
+    returns (steps run, ALS totals, step times)."""
+    from lgt_amd import peps
+    g, t, m = 1.0, 1.0, 0.5
+    als = dict(bonds=0, accepted=0, sweeps=0)
+    times = []
+    for s in range(count):
+        if times and time.perf_counter() + float(np.mean(times)) > deadline:
+            log(f"{label}: stopping after {len(times)} of {count} steps (wall-clock budget)")
+            break
+        ts = time.perf_counter()
+        st = peps.trotter_step_v4(gp, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
+                                  positive_cost=True)
+        ctx.sync()
+        times.append(time.perf_counter() - ts)
+        for k in als:
+            als[k] += st[k]
+        log(f"{label} step {s + 1}/{count}: {times[-1]:.2f} s  als={st}")
+    return len(times), als, times
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--max-seconds", type=float, default=420.0,
+                    help="wall-clock budget of the warm-up plus timed steps: warm-up steps are cut first, then "
+                         "timed steps stop once another one would pass it (the JSON line reports the steps run)")
     ap.add_argument("--tier", default="fp64", choices=["fp64", "fp32", "bf16"])
     ap.add_argument("--nx", type=int, default=8)
     ap.add_argument("--ny", type=int, default=8)
@@ -262,8 +385,6 @@ def main():
               "is a design note, not a measured configuration", file=sys.stderr)
         sys.exit(2)
 
-    # one progress line per bond update on stderr (a C3 step runs for minutes)
-    os.environ.setdefault("PEPS_HIP_VERBOSE", "1")
     from lgt_amd import peps
     g, t, m = 1.0, 1.0, 0.5
     ctx = peps.Context(0)
@@ -272,125 +393,127 @@ def main():
     gp = make_state(ctx, a.nx, a.ny, a.gd, a.D, a.seed, a.su_steps, g, t, m)
     n_before, n_after = normalise(gp, a.nx, a.ny, a.chi)
     log(f"setup done in {time.perf_counter() - t0:.1f} s (<psi|psi> {n_before:.3e} -> {n_after:.6f} "
-        f"after site-uniform rescaling; bond dims h {[len(gp.bond(peps.PEPS_H, 1, iy)[0]) for iy in range(1, a.ny + 1)]})")
+        f"after site-uniform rescaling)")
     bd = [len(gp.bond(peps.PEPS_H, ix, iy)[0]) for iy in range(1, a.ny + 1) for ix in range(1, a.nx)] + \
          [len(gp.bond(peps.PEPS_V, ix, iy)[0]) for iy in range(1, a.ny) for ix in range(1, a.nx + 1)]
     bond_dims = {"mean": float(np.mean(bd)), "max": int(max(bd)), "at_D": int(sum(1 for x in bd if x == a.D)),
                  "bonds": len(bd)}
-    log(f"bond dimensions: {bond_dims}")
+    log(f"PEPS bond dimensions at start: {bond_dims}")
     start = gp.to_arrays() if a.compare_tier else None
     ctx.set_tier(a.tier)
+    t_begin = time.perf_counter()
+    deadline = t_begin + a.max_seconds
+    warm = 0
+    wtimes = []
     for w in range(a.warmup):
-        tw = time.perf_counter()
-        peps.trotter_step_v4(gp, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
-                                 positive_cost=True)
-        ctx.sync()
-        log(f"warmup step {w + 1}/{a.warmup}: {time.perf_counter() - tw:.2f} s")
+        # keep room for at least as many timed steps as warm-up steps run so far
+        if wtimes and time.perf_counter() + 2 * float(np.mean(wtimes)) > deadline:
+            log(f"warm-up: stopping after {warm} of {a.warmup} steps (wall-clock budget)")
+            break
+        n1, _, tw = run_steps(gp, ctx, a, 1, "warm-up", deadline)
+        warm += n1
+        wtimes += tw
     ctx.stats_reset()
     ctx.set_timing(True)
     ctx.sync()
     c0 = _clocks()
     t0 = time.perf_counter()
-    als = dict(bonds=0, accepted=0, sweeps=0)
-    for s in range(a.steps):
-        ts = time.perf_counter()
-        st = peps.trotter_step_v4(gp, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
-                                 positive_cost=True)
-        for k in als:
-            als[k] += st[k]
-        ctx.sync()
-        log(f"step {s + 1}/{a.steps}: {time.perf_counter() - ts:.2f} s  als={st}")
+    steps, als, times = run_steps(gp, ctx, a, a.steps, "timed", deadline)
     ctx.sync()
     elapsed = time.perf_counter() - t0
     c1 = _clocks()
     log(f"timed region clock_boottime_ns={c0[0]}..{c1[0]} clock_monotonic_ns={c0[1]}..{c1[1]}")
     stats = ctx.stats()
     ctx.set_timing(False)
-    value = a.steps / elapsed
-    gk = stats.get(GEMM_CLASS[a.tier], dict(flops=0.0, ms=0.0, launches=0, bytes=0.0))
-    achieved = gk["flops"] / (gk["ms"] / 1e3) / 1e12 if gk["ms"] > 0 else 0.0
-    total_flops = sum(v["flops"] for v in stats.values())
-    svd_ms = sum(v["ms"] for k, v in stats.items() if k.startswith("svd") and k != "svd[sweeps]")
-    svd_n = sum(v["launches"] for k, v in stats.items() if k.startswith("svd") and k != "svd[sweeps]")
-    for name, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
-        rate = f" {v['bytes'] / v['ms'] / 1e6:8.1f} GB/s" if v["ms"] > 0 and v["bytes"] > 0 else ""
-        log(f"  {name:12s} launches={v['launches']:8d} ms={v['ms']:10.1f} flop={v['flops']:.3e}{rate}")
+    value = steps / elapsed
+    kern = {k: v for k, v in stats.items() if v["ms"] > 0}
+    for name, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"]):
+        log(f"  {name:24s} launches={v['launches']:8d} ms={v['ms']:10.1f} flop={v['flops']:.3e} "
+            f"bytes={v['bytes']:.3e}")
+    ranked = sorted(kern.items(), key=lambda kv: -kv[1]["ms"])
+    dominant = ranked[0] if ranked else ("none", dict(flops=0.0, bytes=0.0, ms=0.0, launches=0))
+    gcls = GEMM_CLASS[a.tier]
+    gk = stats.get(gcls, dict(flops=0.0, ms=0.0, launches=0, bytes=0.0))
+    total_flops = sum(v["flops"] for k, v in stats.items() if not k.startswith(("bmps", "top_eigs", "als", "svd[s")))
+    bsum = stats.get("bmps[bond]", dict(flops=0.0, bytes=0.0))
+    bmax = stats.get("bmps[max bond]", dict(flops=0.0))
+    bmps_mean = bsum["flops"] / max(bsum["bytes"], 1.0)
+    roof = roofline_of(dominant[0], dominant[1], a.tier)
+    roof["traffic"] = PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier, dominant[0]))
+    groof = roofline_of(gcls, gk, a.tier)
+    groof["traffic"] = PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier, gcls))
+    kernel_time = sum(v["ms"] for v in kern.values())
     out = {
-        "metric": "full-update steps/s at 8x8 D=6 chi=144",
+        "metric": METRIC,
         "value": value,
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": 1e3 * elapsed / a.steps,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": 1e3 * elapsed / max(steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32 (MFMA, FP32 accumulate; FP64 factorisations)",
                   "bf16": "bf16 (MFMA, FP32 accumulate; FP64 factorisations)"}[a.tier],
         "data": "synthetic (seeded random gauge-invariant PEPS, simple-update initialised)",
-        "config": {"workload": f"finite-PEPS full update, {a.nx}x{a.ny} U(1) LGT, spin-1/2 links",
-                   "nx": a.nx, "ny": a.ny, "D": a.D, "chi": a.chi, "gd": a.gd, "tau": a.tau,
+        "config": {"workload": f"finite-PEPS full update, {a.nx}x{a.ny} U(1) LGT, spin-1/2 links, D={a.D}, "
+                               f"boundary chi<={a.chi}",
+                   "nx": a.nx, "ny": a.ny, "D": a.D, "chi_max": a.chi, "gd": a.gd, "tau": a.tau,
+                   "steps_requested": a.steps, "warmup_requested": a.warmup, "max_seconds": a.max_seconds,
+                   "step_seconds": [round(x, 3) for x in times],
                    "n_als_max": a.n_als, "als_tol": a.als_tol, "tier": a.tier,
                    "step_flags": "PEPS_STEP_RENORM|PEPS_STEP_POSITIVE_COST",
                    "bond_updates_per_step": 2 * (a.nx - 1) * a.ny + a.nx * (a.ny - 1),
-                   "als_sweeps_per_step": als["sweeps"] / max(1, a.steps),
+                   "als_sweeps_per_step": als["sweeps"] / max(1, steps),
                    "als_accept_rate": als["accepted"] / max(1, als["bonds"]),
-                   "flop_per_step": total_flops / a.steps,
-                   "bond_dims_at_start": bond_dims},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[a.tier],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[a.tier],
-                     "traffic": PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier)),
-                     "kernel": GEMM_KERNEL[a.tier],
-                     "launches": gk["launches"],
-                     "avg_launch_ms": gk["ms"] / max(1, gk["launches"]),
-                     "algorithmic_flop_per_launch": gk["flops"] / max(1, gk["launches"]),
-                     "algorithmic_bytes_per_launch": gk["bytes"] / max(1, gk["launches"])},
-        # the one-sided Jacobi SVD kernels (one workgroup per matrix:
-        # latency bound, no HBM or MFMA roofline applies) and their share of
-        # the step, beside the GEMM's
-        "svd_kernel": {"kernel": "jacobi_svd_reg_batched_kernel / jacobi_svd_reg_kernel", "bound": "latency",
-                       "ms_per_step": svd_ms / a.steps, "launches": svd_n,
-                       "share_of_step": svd_ms / max(1e-9, 1e3 * elapsed),
-                       "gemm_share_of_step": gk["ms"] / max(1e-9, 1e3 * elapsed)},
+                   "boundary_bond_mean": bmps_mean,
+                   "boundary_bond_max": int(bmax["flops"]),
+                   "flop_per_step": total_flops / max(steps, 1),
+                   "tflops_per_s": total_flops / max(elapsed, 1e-9) / 1e12,
+                   "kernel_time_share": kernel_time / max(1e-9, 1e3 * elapsed),
+                   "peps_bond_dims_at_start": bond_dims,
+                   "speedup_vs_plain_fp64_gpu": 1.0 if a.tier == "fp64" else None},
+        "roofline": roof,
+        "gemm_roofline": groof,
+        "kernel_classes": [dict(roofline_of(k, v, a.tier), share_of_step=v["ms"] / max(1e-9, 1e3 * elapsed))
+                           for k, v in ranked[:8]],
     }
+    e = None
     if a.energy or a.compare_tier:
         e = peps.compute_energy_v4(gp, g, t, m, a.chi, renorm=True)
         out["config"]["energy_per_site"] = e / (a.nx * a.ny)
         log(f"E/site after the timed steps ({a.tier}): {e / (a.nx * a.ny):.12f}")
+    if not a.no_cpu_baseline:
+        per_step = dict(bonds=als["bonds"] / max(steps, 1), als_sweeps=als["sweeps"] / max(steps, 1),
+                        bmps_bond_mean=bmps_mean)
+        out["cpu_baseline"] = cpu_baseline(gp, a, per_step)
+        log(f"cpu baseline: {out['cpu_baseline']['sample']}")
     if a.compare_tier:
         # a context of its own (an empty Ritz-basis cache), warmed like the
         # timed run's: one norm evaluation of the start state, as normalise()
-        # ends with.  The timed run's state and context (and the memory its
-        # pool caches) are released first.
+        # ends with.  The timed run's state and context are released first.
         gch, gd, cplx = gp.gch, gp.gd, gp.is_complex
         gp.__del__()
         ctx.close()
-        ctx64 = peps.Context(0)
-        g64 = peps.GaugePEPS.from_arrays(start["tensors"], start["lh"], start["lv"], start["qh"], start["qv"],
-                                         gch, gd, is_complex=cplx, ctx=ctx64)
-        peps.peps_norm2(g64, a.chi)
-        ctx64.set_tier(a.compare_tier)
-        ctx64.sync()
-        t64 = time.perf_counter()
-        als64 = dict(bonds=0, accepted=0, sweeps=0)
-        for s64 in range(a.warmup + a.steps):
-            st64 = peps.trotter_step_v4(g64, g, t, m, a.tau, a.D, a.chi, a.n_als, a.als_tol, renorm=True,
-                                        positive_cost=True)
-            for k in als64:
-                als64[k] += st64[k]
-            ctx64.sync()
-            log(f"{a.compare_tier} step {s64 + 1}/{a.warmup + a.steps}: {time.perf_counter() - t64:.2f} s  als={st64}")
-        ms64 = 1e3 * (time.perf_counter() - t64) / (a.warmup + a.steps)
-        e64 = peps.compute_energy_v4(g64, g, t, m, a.chi, renorm=True)
-        log(f"E/site after the {a.compare_tier} steps: {e64 / (a.nx * a.ny):.12f}")
-        out["config"][f"{a.compare_tier}_gpu"] = {"ms_per_step": ms64, "time_ratio": ms64 / out["ms_per_step"],
-                                     "energy_per_site": e64 / (a.nx * a.ny),
-                                     "abs_dE_per_site": abs(e - e64) / (a.nx * a.ny),
-                                     "als_sweeps_per_step": als64["sweeps"] / max(1, a.warmup + a.steps),
-                                     "als_accept_rate": als64["accepted"] / max(1, als64["bonds"])}
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(total_flops / a.steps)
+        ctx2 = peps.Context(0)
+        g2 = peps.GaugePEPS.from_arrays(start["tensors"], start["lh"], start["lv"], start["qh"], start["qv"],
+                                        gch, gd, is_complex=cplx, ctx=ctx2)
+        peps.peps_norm2(g2, a.chi)
+        ctx2.set_tier(a.compare_tier)
+        ctx2.sync()
+        t2 = time.perf_counter()
+        n2, als2, times2 = run_steps(g2, ctx2, a, warm + steps, a.compare_tier, float("inf"))
+        ms2 = 1e3 * float(np.mean(times2[warm:] if len(times2) > warm else times2))
+        e2 = peps.compute_energy_v4(g2, g, t, m, a.chi, renorm=True)
+        log(f"E/site after the {a.compare_tier} steps: {e2 / (a.nx * a.ny):.12f} ({time.perf_counter() - t2:.1f} s)")
+        out["config"][f"{a.compare_tier}_gpu"] = {"ms_per_step": ms2, "time_ratio": ms2 / out["ms_per_step"],
+                                                  "energy_per_site": e2 / (a.nx * a.ny),
+                                                  "abs_dE_per_site": abs(e - e2) / (a.nx * a.ny),
+                                                  "als_sweeps_per_step": als2["sweeps"] / max(1, n2),
+                                                  "als_accept_rate": als2["accepted"] / max(1, als2["bonds"])}
+        if a.compare_tier == "fp64":
+            out["config"]["speedup_vs_plain_fp64_gpu"] = ms2 / out["ms_per_step"]
     print(json.dumps(out), flush=True)
 
 

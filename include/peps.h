@@ -87,8 +87,9 @@ typedef struct peps_params {
 int32_t peps_ctx_create(int32_t device, peps_ctx** out);
 int32_t peps_ctx_destroy(peps_ctx* ctx);
 const char* peps_last_error(const peps_ctx* ctx);
-/* number of reference-@warn conditions met (‖Θ‖≈0 in _recanonicalise_*!,
-   finite_peps_fullupdate_v4.jl:236-238,336-338) */
+/* number of warnings: the reference's @warn conditions (‖Θ‖≈0 in
+   _recanonicalise_*!, finite_peps_fullupdate_v4.jl:236-238,336-338) and the
+   factorisation iteration caps reached (peps_set_solver_limits) */
 int64_t peps_warning_count(const peps_ctx* ctx);
 /* contraction precision: FP64 MFMA, FP32 MFMA, BF16 MFMA (FP32 accumulate) */
 int32_t peps_set_tier(peps_ctx* ctx, int32_t tier);
@@ -98,6 +99,13 @@ int32_t peps_set_tier(peps_ctx* ctx, int32_t tier);
    factored product and top-chi eigenvectors of Y^H G Y — the same truncated
    MPS without forming the chi D^2 x D^2 x chi D^2 product. */
 int32_t peps_set_compress_mode(peps_ctx* ctx, int32_t mode);
+/* Iteration caps of the factorisations (no reference counterpart; the
+   reference's LAPACK calls have their own): Chebyshev subspace iterations
+   per top-k eigenproblem of the Gram-form compression (default 25) and
+   one-sided Jacobi sweeps per SVD (default 80).  Reaching a cap counts a
+   warning (peps_warning_count); an unconverged subspace iteration is then
+   answered by a direct eigendecomposition (matrices up to 4096). */
+int32_t peps_set_solver_limits(peps_ctx* ctx, int32_t subspace_max_it, int32_t jacobi_max_sweeps);
 /* per-kernel-class HIP-event timing on/off, and the collected counters */
 int32_t peps_set_timing(peps_ctx* ctx, int32_t on);
 int32_t peps_stats_count(peps_ctx* ctx, int32_t* n);
@@ -107,7 +115,9 @@ int32_t peps_stats_reset(peps_ctx* ctx);
 int32_t peps_sync(peps_ctx* ctx);
 
 /* ── state: GaugePEPS (finite_peps_gauge_invariant.jl:82-88) ─────────────── */
-/* g_charges: nx*ny background charges, ix fastest */
+/* g_charges: nx*ny background charges, ix fastest.  gd in [2, 5]
+   (PEPS_EUNSUPPORTED otherwise: the interior physical dimension 2 gd^2 is
+   limited to 64 by the Gauss-mask kernels). */
 int32_t peps_state_create(peps_ctx* ctx, int32_t nx, int32_t ny, int32_t gd,
                           const int64_t* g_charges, int32_t is_complex, peps_state** out);
 int32_t peps_state_destroy(peps_state* st);

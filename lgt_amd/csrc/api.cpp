@@ -141,6 +141,13 @@ int32_t peps_set_compress_mode(peps_ctx* ctx, int32_t mode) {
   return PEPS_OK;
 }
 
+int32_t peps_set_solver_limits(peps_ctx* ctx, int32_t subspace_max_it, int32_t jacobi_max_sweeps) {
+  if (!ctx || subspace_max_it < 1 || jacobi_max_sweeps < 1) return PEPS_EARG;
+  ctx->c->subspace_max_it = subspace_max_it;
+  ctx->c->jacobi_max_sweeps = jacobi_max_sweeps;
+  return PEPS_OK;
+}
+
 int32_t peps_set_timing(peps_ctx* ctx, int32_t on) {
   if (!ctx) return PEPS_EARG;
   return guard(ctx->c.get(), [&] {

@@ -35,7 +35,9 @@ namespace peps {
 
 void State::init(Ctx* ctx, int nx_, int ny_, int gd_, const int64_t* g, bool cplx_) {
   PEPS_CHECK(nx_ >= 2 && ny_ >= 2, PS_EARG, "lattice must be at least 2x2");
-  PEPS_CHECK(gd_ >= 2 && gd_ <= 8, PS_EARG, "link dimension gd must be in [2, 8]");
+  // the interior physical dimension 2 gd^2 must fit the 64 physical labels
+  // the Gauss-mask kernels take per site (gd <= 5)
+  PEPS_CHECK(gd_ >= 2 && gd_ <= 5, PS_EUNSUPPORTED, "link dimension gd must be in [2, 5]");
   c = ctx;
   nx = nx_;
   ny = ny_;
@@ -339,7 +341,9 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
   // filter degree per outer iteration: one product rho Y per degree is cheap
   // next to the QR and Rayleigh-Ritz of an outer iteration, so a high degree
   // pays; T_16 on the interval floor 1e-10 theta_0 stays below ~1e165
-  constexpr int kCheb = 16, kMaxIt = 25;
+  constexpr int kCheb = 16;
+  constexpr int64_t kDirectFallbackMax = 4096;
+  const int kMaxIt = std::max(1, c.subspace_max_it);
   for (int it = 0; it < kMaxIt; ++it) {
     // Rayleigh-Ritz on span(Q)
     Tensor Z = matmul(c, rh, Q);
@@ -365,7 +369,21 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
     c.count("top_eigs[iterations]", 1.0, 0);
     if (res <= 1e-12 * std::max(std::fabs(theta[0]), 1e-300)) break;
     if (it == kMaxIt - 1) {
+      // not converged within the iteration cap: count a warning and, while
+      // a direct solve is affordable, answer with the full eigendecomposition
       c.count("top_eigs[unconverged]", 1.0, 0);
+      ++c.warnings;
+      if (N <= kDirectFallbackMax) {
+        c.count("top_eigs[direct fallback]", 1.0, 0);
+        EIG fd = eigh(c, rh);
+        std::vector<int64_t> ordd(N);
+        std::iota(ordd.begin(), ordd.end(), 0);
+        std::stable_sort(ordd.begin(), ordd.end(), [&](int64_t x, int64_t y) { return fd.w[x] > fd.w[y]; });
+        ordd.resize(p);
+        theta.resize(p);
+        for (int64_t i = 0; i < p; ++i) theta[i] = fd.w[ordd[i]];
+        X = gather2d(c, fd.V, seq(N), ordd);
+      }
       break;
     }
     // Chebyshev filter of degree kCheb on the unwanted interval [0, theta_min]
@@ -470,6 +488,22 @@ static void normalise_mps(Ctx& c, MPS& m, bool canonical) {
   }
 }
 
+// Bond dimensions reached by compressed boundaries (statistics only):
+// "bmps[bond]" sums the internal bonds (flops field) over the bonds counted
+// (bytes field); "bmps[max bond]" keeps the largest (flops field).
+static void record_boundary_bonds(Ctx& c, const MPS& m) {
+  double sum = 0.0, cnt = 0.0, mx = 0.0;
+  for (size_t i = 0; i + 1 < m.size(); ++i) {
+    sum += double(m[i].dims[2]);
+    cnt += 1.0;
+    mx = std::max(mx, double(m[i].dims[2]));
+  }
+  c.count("bmps[bond]", sum, cnt);
+  KernelStat& s = c.stats["bmps[max bond]"];
+  s.flops = std::max(s.flops, mx);
+  s.launches += 1;
+}
+
 MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int chi, int key_row,
                     bool normalise) {
   int64_t maxbond = 1;
@@ -480,6 +514,7 @@ MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int c
     MPS out = from_top ? absorb_row_from_top(c, mps, row) : absorb_row_from_bottom(c, mps, row);
     compress_mps(c, out, chi, CMPS_CUT, normalise);
     if (normalise) normalise_mps(c, out, true);
+    record_boundary_bonds(c, out);
     return out;
   }
   std::vector<Column> cols(mps.size());
@@ -495,6 +530,7 @@ MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int c
   MPS out = compress_columns(c, cols, chi, CMPS_CUT, mps[0].cplx, key_row > 0 ? &prefix : nullptr,
                              normalise);
   if (normalise) normalise_mps(c, out, true);
+  record_boundary_bonds(c, out);
   if (c.verbose) {
     int64_t mx = 1;
     for (auto& t : out) mx = std::max(mx, t.dims[2]);

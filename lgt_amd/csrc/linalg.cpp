@@ -197,6 +197,12 @@ SVD dense_finish(Ctx& c, DensePlan& p, int sweeps) {
   TierGuard fp64(c, TIER_FP64);
   const int64_t mb = p.mb, nb = p.nb;
   c.count("svd[sweeps]", double(sweeps), 0);
+  if (sweeps >= c.jacobi_max_sweeps) {
+    // the sweep cap was reached with rotations still applied: the factors
+    // are returned as they are and the condition is counted as a warning
+    c.count("svd[unconverged]", 1.0, 0);
+    ++c.warnings;
+  }
   std::vector<double> s = c.to_host(p.sg);
   std::vector<int64_t> ord(nb);
   for (int64_t i = 0; i < nb; ++i) ord[i] = i;
@@ -279,7 +285,7 @@ SVD thin_finish(Ctx& c, ThinPlan& t, int sweeps) {
 SVD svd_thin(Ctx& c, const Tensor& A) {
   ThinPlan t = thin_prepare(c, A);
   int sweeps = 0;
-  if (!t.direct) sweeps = jacobi_svd(c, t.d.W, t.d.V, t.d.sg, t.d.tol, 80);
+  if (!t.direct) sweeps = jacobi_svd(c, t.d.W, t.d.V, t.d.sg, t.d.tol, c.jacobi_max_sweeps);
   return thin_finish(c, t, sweeps);
 }
 
@@ -299,7 +305,7 @@ std::vector<SVD> svd_thin_batch(Ctx& c, const std::vector<Tensor>& As) {
   std::vector<int> sweeps(plans.size(), 0);
   if (!jobs.empty()) {
     TierGuard fp64(c, TIER_FP64);
-    jacobi_svd_batch(c, jobs, 80);
+    jacobi_svd_batch(c, jobs, c.jacobi_max_sweeps);
     for (size_t j = 0; j < jobs.size(); ++j) sweeps[of[j]] = jobs[j].sweeps;
   }
   std::vector<SVD> out;

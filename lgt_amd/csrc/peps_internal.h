@@ -142,6 +142,12 @@ class Ctx {
   // Hermitian top-k eigenproblems up to this size are solved directly, larger
   // ones by subspace iteration (environment variable PEPS_HIP_EIG_DIRECT_MAX)
   int eig_direct_max = 512;
+  // iteration caps of the factorisations: Chebyshev subspace iterations per
+  // top-k eigenproblem and one-sided Jacobi sweeps per SVD.  Reaching a cap
+  // counts a warning (peps_warning_count); an unconverged subspace iteration
+  // is then answered by a direct eigendecomposition (N <= 4096).
+  int subspace_max_it = 25;
+  int jacobi_max_sweeps = 80;
   // account GEMMs per exact shape ("gemm_f32 MxNxK ...") instead of per
   // precision (environment variable PEPS_HIP_GEMM_SHAPES=1)
   bool gemm_shapes = false;

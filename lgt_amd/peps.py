@@ -52,6 +52,7 @@ _SIGS = {
     "peps_warning_count": (C.c_int64, [_vp]),
     "peps_set_tier": (C.c_int32, [_vp, C.c_int32]),
     "peps_set_compress_mode": (C.c_int32, [_vp, C.c_int32]),
+    "peps_set_solver_limits": (C.c_int32, [_vp, C.c_int32, C.c_int32]),
     "peps_set_timing": (C.c_int32, [_vp, C.c_int32]),
     "peps_stats_count": (C.c_int32, [_vp, _i32p]),
     "peps_stats_get": (C.c_int32, [_vp, C.c_int32, C.c_char_p, C.c_int32, _dp, _dp, _dp, _i64p]),
@@ -183,6 +184,11 @@ class Context:
 
     def sync(self):
         self.check(lib().peps_sync(self.h))
+
+    def set_solver_limits(self, subspace_max_it: int = 25, jacobi_max_sweeps: int = 80):
+        """Iteration caps of the subspace iteration and the Jacobi SVD; a cap
+        reached counts a warning."""
+        self.check(lib().peps_set_solver_limits(self.h, int(subspace_max_it), int(jacobi_max_sweeps)))
 
     def warning_count(self) -> int:
         return int(lib().peps_warning_count(self.h))

@@ -611,8 +611,12 @@ def dl_traced_site(gp, ix, iy):
     return dl_traced(sqrt_absorbed_site(gp, ix, iy))
 
 
-def compress_mps(mps, chi, cutoff=CMPS_CUT):
-    """finite_peps_boundary_mps.jl:70-105 (in place, returns the list)."""
+def compress_mps(mps, chi, cutoff=CMPS_CUT, relative=False):
+    """finite_peps_boundary_mps.jl:70-105 (in place, returns the list).
+
+    relative=True (the library's PEPS_STEP_RENORM, no reference counterpart)
+    keeps singular values above cutoff * S[0] of each bond instead of above
+    the absolute cutoff."""
     n = len(mps)
     if n == 1:
         return mps
@@ -626,7 +630,8 @@ def compress_mps(mps, chi, cutoff=CMPS_CUT):
     for i in range(n - 1, 0, -1):
         Dl, dp, Dr = mps[i].shape
         U, S, Vt = np.linalg.svd(jr(mps[i], Dl, dp * Dr), full_matrices=False)
-        k = max(min(chi, int(np.sum(S > cutoff)), len(S)), 1)
+        thr = cutoff * S[0] if (relative and len(S)) else cutoff
+        k = max(min(chi, int(np.sum(S > thr)), len(S)), 1)
         mps[i] = jr(Vt[:k, :], k, dp, Dr)
         Dl2, dp2, Dr2 = mps[i - 1].shape
         mps[i - 1] = jr(jr(mps[i - 1], Dl2 * dp2, Dr2) @ (U[:, :k] * S[None, :k]), Dl2, dp2, k)
@@ -676,25 +681,48 @@ def _trivial_mps(n):
     return [np.ones((1, 1, 1), np.complex128) for _ in range(n)]
 
 
-def boundary_top(gp, stop_row, chi):
-    """finite_peps_boundary_mps.jl:183-193."""
-    if stop_row >= gp.ny:
-        return _trivial_mps(gp.nx)
-    mps = row_as_mps_down(row_dl(gp, gp.ny))
-    for r in range(gp.ny - 1, stop_row, -1):
-        mps = absorb_row_from_top(mps, row_dl(gp, r))
-        compress_mps(mps, chi)
+def normalise_mps(mps, canonical):
+    """The rescaling of PEPS_STEP_RENORM (no reference counterpart): a
+    compressed boundary is right-canonical (every tensor but the first is a
+    right isometry, compress_mps's right-to-left sweep), so its norm is
+    ||mps[0]|| and mps[0] alone is divided by it; a raw row MPS has every
+    tensor divided by its own norm.  Boundaries enter the full update and the
+    energy only through ratios, so the scale never changes a result."""
+    for i in range(1 if canonical else len(mps)):
+        n = np.linalg.norm(mps[i])
+        if np.isfinite(n) and n > 0:
+            mps[i] = mps[i] / n
     return mps
 
 
-def boundary_bottom(gp, stop_row, chi):
-    """finite_peps_boundary_mps.jl:207-216."""
+def boundary_top(gp, stop_row, chi, renorm=False):
+    """finite_peps_boundary_mps.jl:183-193.  renorm: relative compression cut
+    and normalisation after every absorption (PEPS_STEP_RENORM)."""
+    if stop_row >= gp.ny:
+        return _trivial_mps(gp.nx)
+    mps = row_as_mps_down(row_dl(gp, gp.ny))
+    if renorm:
+        normalise_mps(mps, False)
+    for r in range(gp.ny - 1, stop_row, -1):
+        mps = absorb_row_from_top(mps, row_dl(gp, r))
+        compress_mps(mps, chi, relative=renorm)
+        if renorm:
+            normalise_mps(mps, True)
+    return mps
+
+
+def boundary_bottom(gp, stop_row, chi, renorm=False):
+    """finite_peps_boundary_mps.jl:207-216 (renorm as boundary_top)."""
     if stop_row <= 1:
         return _trivial_mps(gp.nx)
     mps = row_as_mps_up(row_dl(gp, 1))
+    if renorm:
+        normalise_mps(mps, False)
     for r in range(2, stop_row):
         mps = absorb_row_from_bottom(mps, row_dl(gp, r))
-        compress_mps(mps, chi)
+        compress_mps(mps, chi, relative=renorm)
+        if renorm:
+            normalise_mps(mps, True)
     return mps
 
 
@@ -789,13 +817,15 @@ def spectator_right(Renv, topj, botj, Ej):
     return np.einsum("CFyx,AvuC,BedF,bkxyvued->ABkb", Renv, tp, bt, E8, optimize=True)
 
 
-def bond_environment_h(gp, ix, iy, chi=64, top=None, bot=None):
-    """finite_peps_boundary_mps.jl:389-436."""
+def bond_environment_h(gp, ix, iy, chi=64, top=None, bot=None, renorm=False, Lenv=None, Renv=None):
+    """finite_peps_boundary_mps.jl:389-436 (renorm: boundaries as boundary_top;
+    Lenv / Renv: the strip environments left of ix and right of ix+1 when
+    already known, otherwise built from the boundaries as the reference does)."""
     nxv = gp.nx
     if top is None:
-        top = boundary_top(gp, iy, chi)
+        top = boundary_top(gp, iy, chi, renorm)
     if bot is None:
-        bot = boundary_bottom(gp, iy, chi)
+        bot = boundary_bottom(gp, iy, chi, renorm)
     row = row_dl(gp, iy)
     A = sqrt_absorbed_site(gp, ix, iy)
     B = sqrt_absorbed_site(gp, ix + 1, iy)
@@ -803,12 +833,14 @@ def bond_environment_h(gp, ix, iy, chi=64, top=None, bot=None):
     QB, RB = qr_reduce_h_right(B)
     DuA, DdA = A.shape[3], A.shape[4]
     DuB, DdB = B.shape[3], B.shape[4]
-    Lenv = np.ones((1, 1, 1, 1), np.complex128)
-    for j in range(1, ix):
-        Lenv = spectator_left(Lenv, top[j - 1], bot[j - 1], row[j - 1])
-    Renv = np.ones((1, 1, 1, 1), np.complex128)
-    for j in range(nxv, ix + 1, -1):
-        Renv = spectator_right(Renv, top[j - 1], bot[j - 1], row[j - 1])
+    if Lenv is None:
+        Lenv = np.ones((1, 1, 1, 1), np.complex128)
+        for j in range(1, ix):
+            Lenv = spectator_left(Lenv, top[j - 1], bot[j - 1], row[j - 1])
+    if Renv is None:
+        Renv = np.ones((1, 1, 1, 1), np.complex128)
+        for j in range(nxv, ix + 1, -1):
+            Renv = spectator_right(Renv, top[j - 1], bot[j - 1], row[j - 1])
     tpA, btA = split_mps(top[ix - 1], DuA), split_mps(bot[ix - 1], DdA)
     GA = np.einsum("ABkb,AvuC,BedF,kuda,bvec->acCF", Lenv, tpA, btA, QA, QA.conj(), optimize=True)
     tpB, btB = split_mps(top[ix], DuB), split_mps(bot[ix], DdB)
@@ -847,13 +879,14 @@ def spectator_right_v(Renv, topj, botj, Eiy, Eiy1):
     return np.einsum("CBbknmsr,AsrC->ABkbmn", x, tp, optimize=True)
 
 
-def bond_environment_v(gp, ix, iy, chi=64, top=None, bot=None):
-    """finite_peps_boundary_mps.jl:507-547 (same contraction, optimal pairwise order)."""
+def bond_environment_v(gp, ix, iy, chi=64, top=None, bot=None, renorm=False):
+    """finite_peps_boundary_mps.jl:507-547 (same contraction, optimal pairwise
+    order; renorm: boundaries as boundary_top)."""
     nxv = gp.nx
     if top is None:
-        top = boundary_top(gp, iy + 1, chi)
+        top = boundary_top(gp, iy + 1, chi, renorm)
     if bot is None:
-        bot = boundary_bottom(gp, iy, chi)
+        bot = boundary_bottom(gp, iy, chi, renorm)
     row0, row1 = row_dl(gp, iy), row_dl(gp, iy + 1)
     A = sqrt_absorbed_site(gp, ix, iy)
     B = sqrt_absorbed_site(gp, ix, iy + 1)
@@ -951,7 +984,7 @@ def als_cost(th, env, X, Y):
 
 
 def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=False, init=None,
-                 positive_cost=False):
+                 positive_cost=False, als_tol=0.0):
     """finite_peps_fullupdate_v4.jl:94-154.
 
     init=(X0, Y0) replaces the charge-SVD initial bond tensors (the SVD still
@@ -961,7 +994,12 @@ def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=F
     implementations of the sweeps needs a shared init.
 
     positive_cost (no reference counterpart) also requires cost1 >= 0 for
-    the refinement to be kept; the sweeps always run here."""
+    the refinement to be kept; the sweeps always run here.
+
+    als_tol > 0 (no reference counterpart; the reference runs exactly n_als
+    sweeps) stops after the first sweep whose cost differs from the previous
+    one (the initial bond's before the first sweep) by at most
+    als_tol * |previous|; the accept rule is unchanged."""
     nA, dpL, dpR, nB = th.shape
     thm = jr(pd(th, 1, 2, 4, 3), nA * dpL, nB * dpR)
     row_charge = np.array([eRofpL[r // nA] for r in range(nA * dpL)])
@@ -977,7 +1015,10 @@ def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=F
         X, Y = np.array(init[0], dtype=complex), np.array(init[1], dtype=complex)
     X0, Y0 = X.copy(), Y.copy()
     cost0 = als_cost(th, env, X0, Y0)
+    prev = cost0
+    sweeps = 0
     for _ in range(n_als):
+        sweeps += 1
         NX = np.einsum("bqk,abcd,dql->akcl", Y, env, Y.conj(), optimize=True)
         SX = np.einsum("ebcd,epqb,dqk->cpk", env, th, Y.conj(), optimize=True)
         Nm = jr(NX, nA * Dp, nA * Dp)
@@ -991,6 +1032,12 @@ def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=F
         for pR in range(dpR):
             s = jr(SY[:, pR, :], nB * Dp, 1)
             Y[:, pR, :] = jr(solve_herm(Nm, s, reg), nB, Dp)
+        if als_tol > 0:
+            cur = als_cost(th, env, X, Y)
+            conv = abs(prev - cur) <= als_tol * max(abs(prev), 1e-300)
+            prev = cur
+            if conv:
+                break
     cost1 = als_cost(th, env, X, Y)
     accepted = bool(np.isfinite(cost1) and cost1 < cost0 and (not positive_cost or cost1 >= 0))
     if accepted:
@@ -998,7 +1045,7 @@ def als_truncate(th, env, D_max, eRofpL, n_als=30, reg=V4_ALS_REG, return_info=F
     else:
         out = (X0, Y0, bflux)
     if return_info:
-        return out + (dict(cost0=cost0, cost1=cost1, accepted=accepted),)
+        return out + (dict(cost0=cost0, cost1=cost1, accepted=accepted, sweeps=sweeps),)
     return out
 
 
@@ -1068,19 +1115,19 @@ def recanonicalise_v(gp, ix, iy, Dt, Ut, D_max):
 
 
 def full_update_bond_h_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=None, positive_cost=False,
-                          infos=None):
+                          infos=None, renorm=False, als_tol=0.0):
     """finite_peps_fullupdate_v4.jl:165-193 (infos: a list that receives the
-    ALS info dict of the bond)."""
+    ALS info dict of the bond; renorm: PEPS_STEP_RENORM boundaries)."""
     gd = gp.gd
     apply_mask(gp, ix, iy); apply_mask(gp, ix + 1, iy)
-    E = bond_environment_h(gp, ix, iy, chi, top=top, bot=bot)
+    E = bond_environment_h(gp, ix, iy, chi, top=top, bot=bot, renorm=renorm)
     dpL, dpR = E["RA"].shape[1], E["RB"].shape[1]
     _, dgRL, dgUL = site_dims(ix, iy, gp.nx, gp.ny, gd)
     th = np.einsum("apk,bqk->apqb", E["RA"], E["RB"], optimize=True)
     th = apply_gate(th, G, dpL, dpR)
     eR = [decode_phys(p, dgRL, dgUL, gd)[1] for p in range(1, dpL + 1)]
     X, Y, bflux, info = als_truncate(th, E["env"], D_max, eR, n_als=n_als, return_info=True,
-                                     positive_cost=positive_cost)
+                                     positive_cost=positive_cost, als_tol=als_tol)
     if infos is not None:
         infos.append(info)
     Afull = np.einsum("luda,apr->plrud", E["QA"], X, optimize=True)
@@ -1090,18 +1137,18 @@ def full_update_bond_h_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=
 
 
 def full_update_bond_v_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=None, positive_cost=False,
-                          infos=None):
-    """finite_peps_fullupdate_v4.jl:274-299 (infos as full_update_bond_h_v4)."""
+                          infos=None, renorm=False, als_tol=0.0):
+    """finite_peps_fullupdate_v4.jl:274-299 (infos, renorm as full_update_bond_h_v4)."""
     gd = gp.gd
     apply_mask(gp, ix, iy); apply_mask(gp, ix, iy + 1)
-    E = bond_environment_v(gp, ix, iy, chi, top=top, bot=bot)
+    E = bond_environment_v(gp, ix, iy, chi, top=top, bot=bot, renorm=renorm)
     dpD, dpU = E["RA"].shape[1], E["RB"].shape[1]
     _, dgRD, dgUD = site_dims(ix, iy, gp.nx, gp.ny, gd)
     th = np.einsum("apk,bqk->apqb", E["RA"], E["RB"], optimize=True)
     th = apply_gate(th, G, dpD, dpU)
     eU = [decode_phys(p, dgRD, dgUD, gd)[2] for p in range(1, dpD + 1)]
     X, Y, bflux, info = als_truncate(th, E["env"], D_max, eU, n_als=n_als, return_info=True,
-                                     positive_cost=positive_cost)
+                                     positive_cost=positive_cost, als_tol=als_tol)
     if infos is not None:
         infos.append(info)
     Afull = np.einsum("lrda,apu->plrud", E["QA"], X, optimize=True)
@@ -1110,22 +1157,26 @@ def full_update_bond_v_v4(gp, ix, iy, G, D_max, chi=64, n_als=30, top=None, bot=
     apply_mask(gp, ix, iy); apply_mask(gp, ix, iy + 1)
 
 
-def trotter_step_v4(gp, g, t_hop, m, tau, D_max, chi=64, n_als=30, positive_cost=False, infos=None):
-    """trotter_step_v4!, finite_peps_fullupdate_v4.jl:415-432 (positive_cost
-    and infos are passed to every bond update)."""
+def trotter_step_v4(gp, g, t_hop, m, tau, D_max, chi=64, n_als=30, positive_cost=False, infos=None,
+                    renorm=False, als_tol=0.0):
+    """trotter_step_v4!, finite_peps_fullupdate_v4.jl:415-432 (positive_cost,
+    infos, renorm and als_tol are passed to every bond update)."""
     nxv, nyv, gd = gp.nx, gp.ny, gp.gd
     for iy in range(1, nyv + 1):
         for ix in range(1, nxv):
             G = gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2)
-            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos)
+            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos,
+                                  renorm=renorm, als_tol=als_tol)
     for iy in range(1, nyv):
         for ix in range(1, nxv + 1):
             G = gate(H_merged_v_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau)
-            full_update_bond_v_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos)
+            full_update_bond_v_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos,
+                                  renorm=renorm, als_tol=als_tol)
     for iy in range(1, nyv + 1):
         for ix in range(nxv - 1, 0, -1):
             G = gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2)
-            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos)
+            full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos,
+                                  renorm=renorm, als_tol=als_tol)
 
 
 def bond_expect(E, h, dpL, dpR):
@@ -1138,19 +1189,20 @@ def bond_expect(E, h, dpL, dpR):
     return np.real(np.sum(Th * c)), np.real(np.sum(Tn * c))
 
 
-def compute_energy_v4(gp, g, t_hop, m, chi=64):
-    """finite_peps_fullupdate_v4.jl:390-409 (total energy)."""
+def compute_energy_v4(gp, g, t_hop, m, chi=64, renorm=False):
+    """finite_peps_fullupdate_v4.jl:390-409 (total energy; renorm:
+    PEPS_STEP_RENORM boundaries)."""
     nxv, nyv, gd = gp.nx, gp.ny, gp.gd
     E_tot = 0.0
     for iy in range(1, nyv + 1):
-        top, bot = boundary_top(gp, iy, chi), boundary_bottom(gp, iy, chi)
+        top, bot = boundary_top(gp, iy, chi, renorm), boundary_bottom(gp, iy, chi, renorm)
         for ix in range(1, nxv):
             Eb = bond_environment_h(gp, ix, iy, chi, top=top, bot=bot)
             h = H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m)
             hv, nr = bond_expect(Eb, h, Eb["RA"].shape[1], Eb["RB"].shape[1])
             E_tot += hv / nr
     for iy in range(1, nyv):
-        top, bot = boundary_top(gp, iy + 1, chi), boundary_bottom(gp, iy, chi)
+        top, bot = boundary_top(gp, iy + 1, chi, renorm), boundary_bottom(gp, iy, chi, renorm)
         for ix in range(1, nxv + 1):
             Eb = bond_environment_v(gp, ix, iy, chi, top=top, bot=bot)
             h = H_merged_v_site(ix, iy, nxv, nyv, gd, g, t_hop, m)
