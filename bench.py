@@ -114,21 +114,21 @@ def normalise(gp, nx, ny, chi):
     return first, peps.peps_norm2(gp, chi).real
 
 
-def cpu_baseline(gp, a, per_step, budget_s=25.0):
+def cpu_baseline(gp, a, per_step):
     """The oracle (numpy FP64 complex128 on OpenBLAS, all its threads) timed on
     pieces of the same workload, taken from the device state after the timed
-    steps, then scaled to one full-update step.
-
-    Pieces (interior bond (nx/2, ny/2); boundaries handed in from the library,
-    as the exactly cached schedule does):
-      * h-bond update: one spectator_left step, the environment closure, the
-        ALS (n_als sweeps, the same tolerance and accept rules) and the
-        recanonicalisation (the reference's charge SVD of the full d D^3
-        square Theta);
-      * boundary absorption + compress_mps! (the reference's QR/SVD sweep on
-        the explicit product) of one row at a boundary bond chi' small enough
-        for the sample, scaled by (chi_eff / chi')^3 (QR and SVD cost grows as
-        chi^3 at fixed D).
+    steps, then scaled to one full-update step.  Interior h-bond
+    (nx/2, ny/2):
+      * ALS (n_als sweeps, the step's tolerance and accept rules) on the
+        library's environment of the bond, and the recanonicalisation (the
+        reference's charge SVD of the full d D^3 square Theta): timed at full
+        size (their cost does not depend on chi);
+      * one spectator step plus the environment closure, and one boundary
+        row absorbed and compressed (the reference's QR/SVD sweep on the
+        explicit product): timed with the boundaries cut to chi' = 16 and 32
+        (environment) and 8 and 12 (row), extrapolated to the step's mean
+        boundary bond by the power law through the two timings (exponent
+        clamped to [2, 3]).
     One step = (h + v bonds) x h-bond time + (6 ny - 14) x row time; the
     vertical bonds are charged an h-bond's time (a lower bound: their strip
     environments are larger), and the ALS part is rescaled from the sample's
@@ -143,70 +143,87 @@ def cpu_baseline(gp, a, per_step, budget_s=25.0):
     go = O.GaugePEPS(nx, ny, gp.gd, gp.gch)
     go.tensors = [[np.asarray(t, dtype=np.complex128) for t in col] for col in arr["tensors"]]
     go.lh, go.lv, go.qh, go.qv = arr["lh"], arr["lv"], arr["qh"], arr["qv"]
+    Eg = peps.bond_environment_h(gp, ix, iy, chi)
     top = [np.asarray(t, np.complex128) for t in peps.boundary_top(gp, iy, chi)]
     bot = [np.asarray(t, np.complex128) for t in peps.boundary_bottom(gp, iy, chi)]
     t_start = time.perf_counter()
-    row = O.row_dl(go, iy)
-    # strip environments up to the bond; the last spectator step is the timed one
-    L = np.ones((1, 1, 1, 1), np.complex128)
-    t_spec = 0.0
-    for j in range(1, ix):
-        t0 = time.perf_counter()
-        L = O.spectator_left(L, top[j - 1], bot[j - 1], row[j - 1])
-        t_spec = time.perf_counter() - t0
-    R = np.ones((1, 1, 1, 1), np.complex128)
-    for j in range(nx, ix + 1, -1):
-        R = O.spectator_right(R, top[j - 1], bot[j - 1], row[j - 1])
-    O.apply_mask(go, ix, iy); O.apply_mask(go, ix + 1, iy)
-    t0 = time.perf_counter()
-    E = O.bond_environment_h(go, ix, iy, chi, top=top, bot=bot, Lenv=L, Renv=R)
-    t_close = time.perf_counter() - t0
+    E = {k: np.asarray(v, np.complex128) for k, v in Eg.items()}
     dpL, dpR = E["RA"].shape[1], E["RB"].shape[1]
     _, dgR, dgU = O.site_dims(ix, iy, nx, ny, gp.gd)
     G = O.gate(O.H_merged_h_site(ix, iy, nx, ny, gp.gd, 1.0, 1.0, 0.5), a.tau / 2)
     th = O.apply_gate(np.einsum("apk,bqk->apqb", E["RA"], E["RB"]), G, dpL, dpR)
     eR = [O.decode_phys(p, dgR, dgU, gp.gd)[1] for p in range(1, dpL + 1)]
     t0 = time.perf_counter()
+    O.als_truncate(th, E["env"], D, eR, n_als=0, positive_cost=True)
+    t_als0 = time.perf_counter() - t0
+    t0 = time.perf_counter()
     X, Y, bf, info = O.als_truncate(th, E["env"], D, eR, n_als=a.n_als, return_info=True, positive_cost=True,
                                     als_tol=a.als_tol)
     t_als = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.als_truncate(th, E["env"], D, eR, n_als=0, positive_cost=True)
-    t_als0 = time.perf_counter() - t0
     t_sweep = max(t_als - t_als0, 0.0) / max(info["sweeps"], 1)
+    log(f"cpu baseline: ALS init {t_als0:.2f} s, {info['sweeps']} sweeps {t_als:.2f} s")
     Afull = np.einsum("luda,apr->plrud", E["QA"], X)
     Bfull = np.einsum("rudb,bpl->plrud", E["QB"], Y)
+    O.apply_mask(go, ix, iy); O.apply_mask(go, ix + 1, iy)
+    g2 = O.GaugePEPS(nx, ny, gp.gd, gp.gch)
+    g2.tensors = [list(col) for col in go.tensors]
+    g2.lh, g2.lv, g2.qh, g2.qv = [list(c) for c in go.lh], [list(c) for c in go.lv], \
+        [list(c) for c in go.qh], [list(c) for c in go.qv]
     t0 = time.perf_counter()
-    O.recanonicalise_h(go, ix, iy, Afull, Bfull, D)
+    O.recanonicalise_h(g2, ix, iy, Afull, Bfull, D)
     t_recan = time.perf_counter() - t0
-    # one boundary row: absorb row iy-1 into the bottom boundary and compress,
-    # at a boundary bond chi' that keeps the sample bounded
-    chi_eff = max(1, int(round(per_step["bmps_bond_mean"])))
-    left = budget_s - (time.perf_counter() - t_start)
-    chi_s = chi_eff
-    # predicted complex QR+SVD cost of one column at bond c: ~3 x 8 (c D^4)(c D^2)^2
-    rate = 2.0e11
-    while chi_s > 4 and nx * 3 * 8.0 * (chi_s * D ** 4) * (chi_s * D * D) ** 2 / rate > max(left, 2.0):
-        chi_s = max(4, int(chi_s * 0.8))
-    bs = [t[:min(t.shape[0], chi_s), :, :min(t.shape[2], chi_s)] for t in bot]
-    t0 = time.perf_counter()
-    mps = O.absorb_row_from_bottom(bs, O.row_dl(go, iy))
-    O.compress_mps(mps, chi_s, relative=True)
-    t_row_s = time.perf_counter() - t0
-    t_row = t_row_s * (chi_eff / max(chi_s, 1)) ** 3
+    log(f"cpu baseline: recanonicalisation {t_recan:.2f} s")
+    row = O.row_dl(go, iy)
+
+    def cut(mps, c):
+        return [t[:min(t.shape[0], c), :, :min(t.shape[2], c)].copy() for t in mps]
+
+    def env_piece(c):
+        tp, bt = cut(top, c), cut(bot, c)
+        L = np.ones((1, 1, 1, 1), np.complex128)
+        for j in range(1, ix - 1):
+            L = O.spectator_left(L, tp[j - 1], bt[j - 1], row[j - 1])
+        t0 = time.perf_counter()
+        L = O.spectator_left(L, tp[ix - 2], bt[ix - 2], row[ix - 2])
+        ts = time.perf_counter() - t0
+        R = np.ones((1, 1, 1, 1), np.complex128)
+        for j in range(nx, ix + 1, -1):
+            R = O.spectator_right(R, tp[j - 1], bt[j - 1], row[j - 1])
+        t0 = time.perf_counter()
+        O.bond_environment_h(go, ix, iy, c, top=tp, bot=bt, Lenv=L, Renv=R)
+        return ts + time.perf_counter() - t0
+
+    def row_piece(c):
+        bt = cut(bot, c)
+        t0 = time.perf_counter()
+        m = O.absorb_row_from_bottom(bt, O.row_dl(go, iy))
+        O.compress_mps(m, c, relative=True)
+        return time.perf_counter() - t0
+
+    chi_eff = max(1.0, per_step["bmps_bond_mean"])
+    CE, CR = (16, 32), (8, 12)
+    e1, e2 = env_piece(CE[0]), env_piece(CE[1])
+    r1, r2 = row_piece(CR[0]), row_piece(CR[1])
+    pe = min(3.0, max(2.0, np.log(max(e2, 1e-9) / max(e1, 1e-9)) / np.log(CE[1] / CE[0])))
+    pr = min(3.0, max(2.0, np.log(max(r2, 1e-9) / max(r1, 1e-9)) / np.log(CR[1] / CR[0])))
+    t_env = e2 * (chi_eff / CE[1]) ** pe
+    t_row = r2 * (chi_eff / CR[1]) ** pr
+    log(f"cpu baseline: env {e1:.3f}/{e2:.3f} s at chi' {CE}, row {r1:.3f}/{r2:.3f} s at chi' {CR}")
     bonds = per_step["bonds"]
     sweeps_per_bond = per_step["als_sweeps"] / max(bonds, 1)
-    t_bond = t_spec + t_close + t_als0 + sweeps_per_bond * t_sweep + t_recan
+    t_bond = t_env + t_als0 + sweeps_per_bond * t_sweep + t_recan
     step_s = bonds * t_bond + (6 * ny - 14) * t_row
     sample_s = time.perf_counter() - t_start
     return dict(value=1.0 / step_s, unit="steps/s", cores=int(threads), kind="port",
                 seconds_per_step=step_s,
                 sample=(f"oracle (numpy FP64 complex128, {threads} OpenBLAS threads) on the state after the timed "
-                        f"steps, {sample_s:.1f} s of CPU work: h-bond ({ix},{iy}) spectator step {t_spec:.3f} s, "
-                        f"environment closure {t_close:.3f} s, ALS init {t_als0:.3f} s + {info['sweeps']} sweeps "
-                        f"({t_sweep:.3f} s each), recanonicalisation {t_recan:.3f} s; one boundary row absorbed and "
-                        f"compressed at chi'={chi_s}: {t_row_s:.3f} s, scaled by (chi_eff/chi')^3 to chi_eff="
-                        f"{chi_eff}: {t_row:.2f} s.  Step = {bonds} bonds x {t_bond:.3f} s (ALS at the step's "
+                        f"steps, {sample_s:.1f} s of CPU work.  h-bond ({ix},{iy}): ALS init {t_als0:.3f} s + "
+                        f"{info['sweeps']} sweeps at {t_sweep:.3f} s, recanonicalisation {t_recan:.3f} s (full "
+                        f"size); spectator step + environment closure {e1:.3f}/{e2:.3f} s at chi'={CE[0]}/{CE[1]} "
+                        f"and one boundary row absorbed+compressed {r1:.3f}/{r2:.3f} s at chi'={CR[0]}/{CR[1]}, "
+                        f"extrapolated as chi^{pe:.2f} / "
+                        f"chi^{pr:.2f} to the step's mean boundary bond {chi_eff:.1f}: {t_env:.2f} s / "
+                        f"{t_row:.2f} s.  Step = {bonds:.0f} bonds x {t_bond:.2f} s (ALS at the step's "
                         f"{sweeps_per_bond:.2f} sweeps/bond; v-bonds charged an h-bond's time, a lower bound) + "
                         f"{6 * ny - 14} rows x {t_row:.2f} s = {step_s:.1f} s"))
 

@@ -314,7 +314,9 @@ int32_t peps_gemm_acc(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, i
 
 /* Hermitian eigendecomposition used by the ALS solve (_solve_herm,
    finite_peps_fullupdate_v4.jl:37-45): Householder tridiagonalisation and
-   divide and conquer (rocSOLVER syevd/heevd), one matrix after the other.
+   divide and conquer, one matrix after the other (real n <= 1024: the
+   library's one-launch tridiagonal reduction and reflector kernels around
+   rocSOLVER stedc; otherwise rocSOLVER syevd/heevd).
    A: nb contiguous n x n Hermitian matrices (lower triangles read); outputs
    V (same layout) and w (nb*n, ascending per matrix). */
 int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, const double* A,

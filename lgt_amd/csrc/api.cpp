@@ -518,11 +518,8 @@ int32_t peps_fu_bond_v4(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
   });
 }
 
-int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p) {
-  if (!st || !p) return PEPS_EARG;
-  return guard(st->ref.get(), [&] {
-    peps::Params q = to_params(p);
-    for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
+namespace {
+void run_step(peps_state* st, int32_t kind, const peps::Params& q) {
     if (kind == PEPS_STEP_V4) {
       require_labels(st->s);
       std::vector<peps::ALSInfo> infos;
@@ -543,6 +540,60 @@ int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p) {
       peps::trotter_step_full_v2(st->s, q, st->gc);
     } else {
       throw peps::Error(peps::PS_EARG, "unknown step kind");
+    }
+}
+
+// The imaginary-time tau annealing of ite_ground_state_v4 / _v3 / _v2
+// (finite_peps_fullupdate_v4.jl:447-449, finite_peps_gauge_invariant.jl:
+// 556-559, finite_peps_full_update.jl:595-608): n/8 steps at 0.1, n/8 at
+// 0.05, n/4 at 0.02, n/6 at 0.01, n/6 at 0.005 and the rest at 0.001
+// (integer division).
+std::vector<std::pair<double, int64_t>> ite_tau_stages(int64_t n) {
+  const int64_t s1 = n / 8, s2 = n / 8, s3 = n / 4, s4 = n / 6, s5 = n / 6;
+  return {{0.1, s1}, {0.05, s2}, {0.02, s3}, {0.01, s4}, {0.005, s5}, {0.001, n - (s1 + s2 + s3 + s4 + s5)}};
+}
+}  // namespace
+
+int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p) {
+  if (!st || !p) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    peps::Params q = to_params(p);
+    for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
+    run_step(st, kind, q);
+  });
+}
+
+int32_t peps_ite_ground_state(peps_state* st, int32_t kind, const peps_params* p, int64_t n_ite,
+                              int64_t* steps_done) {
+  if (!st || !p || n_ite < 0) return PEPS_EARG;
+  if (steps_done) *steps_done = 0;
+  return guard(st->ref.get(), [&] {
+    PEPS_CHECK(kind == PEPS_STEP_V4 || kind == PEPS_STEP_GAUGE || kind == PEPS_STEP_FULL_V2, peps::PS_EARG,
+               "ite_ground_state: kind must be PEPS_STEP_V4, PEPS_STEP_GAUGE or PEPS_STEP_FULL_V2");
+    peps::Params q = to_params(p);
+    for (auto& t : st->s.T) PEPS_CHECK(!t.empty(), peps::PS_EARG, "state has unset site tensors");
+    int64_t step = 0, bonds = 0, accepted = 0, sweeps = 0;
+    for (const auto& stage : ite_tau_stages(n_ite)) {
+      q.tau = stage.first;
+      for (int64_t i = 0; i < stage.second; ++i) {
+        run_step(st, kind, q);
+        ++step;
+        if (steps_done) *steps_done = step;
+        bonds += st->als_bonds;
+        accepted += st->als_accepted;
+        sweeps += st->als_sweeps;
+        if (step % 25 == 0 || step == 1) {
+          char buf[96];
+          std::snprintf(buf, sizeof buf, "ite step %lld/%lld (tau=%.4f)", (long long)step, (long long)n_ite,
+                        stage.first);
+          st->ref->note(buf);
+        }
+      }
+    }
+    if (kind == PEPS_STEP_V4) {  // ALS statistics over the whole run
+      st->als_bonds = bonds;
+      st->als_accepted = accepted;
+      st->als_sweeps = sweeps;
     }
   });
 }

@@ -892,6 +892,39 @@ static Tensor solve_herm_multi(Ctx& c, const Tensor& N, const Tensor& S, double 
   TierGuard fp64(c, TIER_FP64);
   Tensor Nh = copy_tensor(c, N);
   hermitize(c, Nh);
+  const int64_t n = Nh.dims[0];
+  if (!Nh.cplx && n >= 2 && n <= sym_tridiag_max_n()) {
+    // N = Q Z diag(w) Z^T Q^T: the reflectors are applied to the d_p
+    // right-hand sides and the solution instead of to all n eigenvectors
+    bool finite, nonzero;
+    scan_values(c, Nh, &finite, &nonzero);
+    PEPS_CHECK(finite, PS_ESOLVER, "non-finite input to eigh");
+    if (!nonzero) {  // N = 0: every mode is floored, x = 0
+      c.count("als_solve[floored]", double(n), 0);
+      return c.zeros({n, S.dims[1]}, S.cplx);
+    }
+    Tridiag t = tridiagonalize(c, Nh);
+    std::vector<double> w;
+    Tensor Z;
+    tridiag_eig(c, t, &w, &Z);
+    double wmax = 0.0;
+    for (double x : w) wmax = std::max(wmax, x);
+    const double floor = reg * std::max(wmax, 1e-30);
+    std::vector<double> inv(w.size());
+    int64_t floored = 0;
+    for (size_t i = 0; i < w.size(); ++i) {
+      inv[i] = w[i] > floor ? 1.0 / w[i] : 0.0;
+      floored += w[i] > floor ? 0 : 1;
+    }
+    c.count(floored ? "als_solve[floored]" : "als_solve[full rank]", double(floored), 0);
+    Tensor Sq = copy_tensor(c, S);
+    apply_q(c, t, Sq, true);                                  // Q^T S
+    Tensor T = matmul(c, Z, Sq, false, false, true, false);   // Z^T Q^T S
+    scale_rows(c, T, inv);
+    Tensor X = matmul(c, Z, T);
+    apply_q(c, t, X, false);
+    return X;
+  }
   EIG e = eigh(c, Nh);
   double wmax = 0.0;
   for (double w : e.w) wmax = std::max(wmax, w);

@@ -1,9 +1,11 @@
 // Dense FP64 factorisations on the GPU: thin QR (rocSOLVER Householder
 // geqrf + orgqr/ungqr, the LAPACK semantics of Julia's `qr`), the
-// Hermitian eigendecomposition (rocSOLVER syevd/heevd: Householder
-// tridiagonalisation and divide and conquer, the algorithm class of
-// Julia's `eigen(Hermitian)`), and the project's one-sided Jacobi kernel for
-// the thin SVD (after a QR reduction of very tall inputs).
+// Hermitian eigendecomposition (real n <= 1024: the project's one-launch
+// Householder tridiagonalisation and reflector kernels around a divide and
+// conquer tridiagonal solve, eigh.hip; complex: rocSOLVER heevd -- the
+// algorithm class of Julia's `eigen(Hermitian)`), and the project's
+// one-sided Jacobi kernel for the thin SVD (after a QR reduction of very
+// tall inputs).
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -328,6 +330,14 @@ EIG eigh(Ctx& c, const Tensor& A) {
   if (!check_input(c, A, "eigh")) {  // A = 0
     r.V = eye(c, n, n, A.cplx);
     r.w.assign(n, 0.0);
+    return r;
+  }
+  if (!A.cplx && n >= 2 && n <= sym_tridiag_max_n()) {
+    // the project's one-launch tridiagonal reduction, divide and conquer on
+    // the tridiagonal matrix, the reflectors applied to its eigenvectors
+    Tridiag t = tridiagonalize(c, A);
+    tridiag_eig(c, t, &r.w, &r.V);
+    apply_q(c, t, r.V, false);
     return r;
   }
   r.V = copy_tensor(c, A);

@@ -307,6 +307,21 @@ struct EIG {
 };
 EIG eigh(Ctx& c, const Tensor& A);
 
+// Real symmetric tridiagonal reduction and its pieces (eigh.hip), n <= 1024:
+// A = Q T Q^T with T = tridiag(d, e); R holds the reflectors (LAPACK dsytrd
+// lower layout) and tau their scalars.
+struct Tridiag {
+  int64_t n = 0;
+  Tensor R, tau, d, e;
+};
+int64_t sym_tridiag_max_n();
+// Reads the lower triangle of A (real, n x n); one kernel launch.
+Tridiag tridiagonalize(Ctx& c, const Tensor& A);
+// Z (n x k) <- Q Z, or Q^T Z when trans.
+void apply_q(Ctx& c, const Tridiag& t, Tensor& Z, bool trans);
+// Eigenvalues of T (ascending, host) and its eigenvectors Z (n x n, device).
+void tridiag_eig(Ctx& c, const Tridiag& t, std::vector<double>* w, Tensor* Z);
+
 // One-sided Jacobi SVD kernel (kernels.hip): A (m x n, m >= n) is overwritten
 // by its orthogonalised columns u_j sigma_j, V gets the right vectors and sig
 // the column norms (unsorted).

@@ -808,7 +808,7 @@ def gemm_acc(A, B, Cin, beta, a_kfast=False, conjA=False, b_kfast=True, conjB=Fa
 
 
 def eigh(A, ctx=None):
-    """Batched Hermitian eigendecomposition (Jacobi kernel): A (..., n, n) -> (w, V), w unsorted."""
+    """Batched Hermitian eigendecomposition (lower triangles read): A (..., n, n) -> (w, V), w ascending."""
     ctx = ctx or default_context()
     A = np.asarray(A)
     cp = np.iscomplexobj(A)

@@ -805,7 +805,12 @@ def spectator_left(Lenv, topj, botj, Ej):
     E8 = split_dl(Ej, Dl, Dr, Du, Dd)
     tp = split_mps(topj, Du)
     bt = split_mps(botj, Dd)
-    return np.einsum("ABkb,AvuC,BedF,bkxyvued->CFyx", Lenv, tp, bt, E8, optimize=True)
+    # the four-tensor sum taken pairwise in a fixed order (chi^3 D^4 + chi^2
+    # D^8 + chi^3 D^4 MACs; numpy's greedy path can build a chi^2 D^6
+    # intermediate here)
+    x = np.einsum("ABkb,AvuC->BkbvuC", Lenv, tp, optimize=True)
+    x = np.einsum("BkbvuC,bkxyvued->BCxyed", x, E8, optimize=True)
+    return np.einsum("BCxyed,BedF->CFyx", x, bt, optimize=True)
 
 
 def spectator_right(Renv, topj, botj, Ej):
@@ -814,7 +819,9 @@ def spectator_right(Renv, topj, botj, Ej):
     E8 = split_dl(Ej, Dl, Dr, Du, Dd)
     tp = split_mps(topj, Du)
     bt = split_mps(botj, Dd)
-    return np.einsum("CFyx,AvuC,BedF,bkxyvued->ABkb", Renv, tp, bt, E8, optimize=True)
+    x = np.einsum("CFyx,AvuC->FyxAvu", Renv, tp, optimize=True)
+    x = np.einsum("FyxAvu,bkxyvued->FAbked", x, E8, optimize=True)
+    return np.einsum("FAbked,BedF->ABkb", x, bt, optimize=True)
 
 
 def bond_environment_h(gp, ix, iy, chi=64, top=None, bot=None, renorm=False, Lenv=None, Renv=None):
@@ -842,9 +849,18 @@ def bond_environment_h(gp, ix, iy, chi=64, top=None, bot=None, renorm=False, Len
         for j in range(nxv, ix + 1, -1):
             Renv = spectator_right(Renv, top[j - 1], bot[j - 1], row[j - 1])
     tpA, btA = split_mps(top[ix - 1], DuA), split_mps(bot[ix - 1], DdA)
-    GA = np.einsum("ABkb,AvuC,BedF,kuda,bvec->acCF", Lenv, tpA, btA, QA, QA.conj(), optimize=True)
+    # GA[a,c,C,F] = Lenv[A,B,k,b] tpA[A,v,u,C] btA[B,e,d,F] QA[k,u,d,a] QA*[b,v,e,c],
+    # pairwise in a fixed order
+    x = np.einsum("ABkb,AvuC->BkbvuC", Lenv, tpA, optimize=True)
+    x = np.einsum("BkbvuC,kuda->BbvCda", x, QA, optimize=True)
+    x = np.einsum("BbvCda,BedF->bvCaeF", x, btA, optimize=True)
+    GA = np.einsum("bvCaeF,bvec->acCF", x, QA.conj(), optimize=True)
     tpB, btB = split_mps(top[ix], DuB), split_mps(bot[ix], DdB)
-    GB = np.einsum("CFkb,AvuC,BedF,kuda,bvec->acAB", Renv, tpB, btB, QB, QB.conj(), optimize=True)
+    # GB[a,c,A,B] = Renv[C,F,k,b] tpB[A,v,u,C] btB[B,e,d,F] QB[k,u,d,a] QB*[b,v,e,c]
+    x = np.einsum("CFkb,AvuC->FkbAvu", Renv, tpB, optimize=True)
+    x = np.einsum("FkbAvu,kuda->FbAvda", x, QB, optimize=True)
+    x = np.einsum("FbAvda,BedF->bAvaBe", x, btB, optimize=True)
+    GB = np.einsum("bAvaBe,bvec->acAB", x, QB.conj(), optimize=True)
     env = np.einsum("acTB,bdTB->abcd", GA, GB, optimize=True)
     return dict(env=env, RA=RA, RB=RB, QA=QA, QB=QB)
 

@@ -97,8 +97,10 @@ def test_dl_traced_matches_oracle(ctx):
 
 @pytest.mark.parametrize("cplx", [False, True])
 def test_eigh(ctx, cplx):
-    """The ALS eigensolver (rocSOLVER syevd/heevd) against numpy, including
-    rank-deficient inputs."""
+    """The ALS eigensolver (real: the project's one-launch tridiagonal
+    reduction and reflector kernels around a divide-and-conquer tridiagonal
+    solve; complex: rocSOLVER heevd) against numpy, including rank-deficient
+    inputs."""
     from lgt_amd import peps
     rng = np.random.default_rng(4)
     for n in (1, 2, 3, 7, 16, 33, 64, 130, 288):
@@ -179,3 +181,24 @@ def test_svd(ctx, cplx):
         assert S.shape == (k,) and np.all(S == 0)
         assert np.abs(np.conj(U.T) @ U - np.eye(k)).max() < 1e-15
         assert np.abs(Vt @ np.conj(Vt.T) - np.eye(k)).max() < 1e-15
+
+
+@pytest.mark.gpu
+def test_eigh_large_and_lower_triangle_only(ctx):
+    """Real sizes that take the 16- and 32-row register variants of the
+    reflector kernel (n = 300, 512, 700), badly scaled entries (the
+    tridiagonal reduction rescales by a power of two), and the LAPACK
+    convention that only the lower triangle is read: garbage in the strict
+    upper triangle must not change the result."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(8)
+    for n, scale in ((300, 1e-120), (512, 1.0), (700, 3e90)):
+        B = rng.standard_normal((n, n))
+        H = (B + B.T) * scale
+        G = np.tril(H) + np.triu(rng.standard_normal((n, n)) * 1e3 * scale, 1)
+        w, V = peps.eigh(G[None], ctx=ctx)
+        w, V = w[0], V[0]
+        ref = np.linalg.eigvalsh(H)
+        assert np.abs(w - ref).max() < 1e-13 * n * np.abs(ref).max()
+        assert np.abs(V.T @ V - np.eye(n)).max() < 1e-13 * n
+        assert np.abs(H @ V - V * w).max() < 1e-13 * n * np.abs(ref).max()
