@@ -423,8 +423,8 @@ def main():
     warm = 0
     wtimes = []
     for w in range(a.warmup):
-        # keep room for at least as many timed steps as warm-up steps run so far
-        if wtimes and time.perf_counter() + 2 * float(np.mean(wtimes)) > deadline:
+        # warm-up takes at most a quarter of the budget
+        if wtimes and time.perf_counter() + float(np.mean(wtimes)) > t_begin + 0.25 * a.max_seconds:
             log(f"warm-up: stopping after {warm} of {a.warmup} steps (wall-clock budget)")
             break
         n1, _, tw = run_steps(gp, ctx, a, 1, "warm-up", deadline)

@@ -213,6 +213,19 @@ int32_t peps_trotter_step(peps_state* st, int32_t kind, const peps_params* p);
    non-OK status, or PEPS_EARG for shared contexts. */
 int32_t peps_batch_trotter_step(peps_state* const* states, int32_t n, int32_t kind, const peps_params* p,
                                 int32_t* status);
+/* Imaginary-time ground-state search with the reference's tau annealing:
+   ite_ground_state_v4 (finite_peps_fullupdate_v4.jl:440-467, kind
+   PEPS_STEP_V4), ite_ground_state_v3 (finite_peps_gauge_invariant.jl:548-577,
+   PEPS_STEP_GAUGE) and ite_ground_state_v2 (finite_peps_full_update.jl:
+   582-629, PEPS_STEP_FULL_V2).  n_ite steps: n/8 at tau 0.1, n/8 at 0.05,
+   n/4 at 0.02, n/6 at 0.01, n/6 at 0.005 and the rest at 0.001 (integer
+   division); p->tau is ignored.  The state is the initial PEPS
+   (init_gauge_peps / init_finite_peps_v2, built by the caller) and is
+   stepped in place; steps_done (optional) receives the steps completed,
+   also on failure.  After a PEPS_STEP_V4 run peps_last_als_stats covers the
+   whole run.  Progress lines (PEPS_HIP_VERBOSE=1) every 25 steps. */
+int32_t peps_ite_ground_state(peps_state* st, int32_t kind, const peps_params* p, int64_t n_ite,
+                              int64_t* steps_done);
 /* ALS statistics of the last PEPS_STEP_V4 step: bonds, accepted, total sweeps */
 int32_t peps_last_als_stats(peps_state* st, int64_t out[3]);
 /* compute_energy_v4 (finite_peps_fullupdate_v4.jl:390): total energy */

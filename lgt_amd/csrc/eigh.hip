@@ -24,6 +24,8 @@ namespace peps {
 namespace {
 
 constexpr int kTrdThreads = 1024;
+constexpr int kTrdBatch = 8;     // loads in flight per thread in the trailing-block sweep
+constexpr int kQChunkBytes = 65536;  // LDS for the reflectors staged by apply_q_kernel
 
 // Sum over the workgroup; every thread gets the same value (fixed order).
 __device__ __forceinline__ double trd_block_sum(double x, double* red) {
@@ -48,12 +50,13 @@ __device__ __forceinline__ double trd_block_max(double x, double* red) {
 
 // A (n x n, leading dimension lda): the lower triangle is read; on exit it
 // holds the reflectors below the subdiagonal.  d (n), e (n-1), tau (n-1).
-// The matrix is first scaled by a power of two that brings its largest
-// entry into [1, 2) (exact), and d, e are scaled back on output.
+// The matrix is first scaled by the power of two 2^-ex that brings its
+// largest entry into [1, 2) (exact); d and e are those of the scaled matrix
+// (so that the tridiagonal solver sees O(1) numbers) and ex goes to *exo.
 // Dynamic LDS: 4 n + 1024 + 64 doubles.
 __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(double* __restrict__ A, int n, int lda,
                                                                   double* __restrict__ dd, double* __restrict__ ee,
-                                                                  double* __restrict__ tt) {
+                                                                  double* __restrict__ tt, int* __restrict__ exo) {
   extern __shared__ double sm[];
   double* bufA = sm;
   double* bufB = sm + n;
@@ -72,7 +75,8 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(double* __rest
   }
   mx = trd_block_max(mx, red0);
   const int ex = mx > 0.0 ? ilogb(mx) : 0;
-  const double s = ldexp(1.0, -ex), sinv = ldexp(1.0, ex);
+  const double s = ldexp(1.0, -ex);
+  if (threadIdx.x == 0) *exo = ex;
   __syncthreads();  // every thread has read A's lower triangle for the maximum
   for (int64_t q = tid; q < nn; q += nt) {
     const int i = int(q % n), j = int(q / n);
@@ -106,8 +110,8 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(double* __rest
       scl = 1.0 / (alpha - beta);
     }
     if (tid == 0) {
-      dd[k] = col[k] * sinv;
-      ee[k] = beta * sinv;
+      dd[k] = col[k];
+      ee[k] = beta;
       tt[k] = tau;
     }
     for (int i = tid; i < n; i += nt) {
@@ -126,20 +130,25 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(double* __rest
       const int i = k + 1 + r;
       const double vpi = vp[i], wpi = wp[i];
       double* Ai = A + i;
-      if (g == 0) {
-        double* p = Ai + int64_t(k + 1) * lda;
-        const double a = *p - (vpi * wp[k + 1] + wpi * vp[k + 1]);
-        *p = a;
-        y += a * vk[k + 1];
-        col[i] = a;
-      }
-#pragma unroll 4
-      for (int jj = (g == 0 ? G : g); jj < m; jj += G) {
-        const int j = k + 1 + jj;
-        double* p = Ai + int64_t(j) * lda;
-        const double a = *p - (vpi * wp[j] + wpi * vp[j]);
-        *p = a;
-        y += a * vk[j];
+      // kTrdBatch independent loads in flight per thread, then their updates
+      for (int jj0 = g; jj0 < m; jj0 += kTrdBatch * G) {
+        double a[kTrdBatch];
+#pragma unroll
+        for (int u = 0; u < kTrdBatch; ++u) {
+          const int jj = jj0 + u * G;
+          a[u] = jj < m ? Ai[int64_t(k + 1 + jj) * lda] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kTrdBatch; ++u) {
+          const int jj = jj0 + u * G;
+          if (jj < m) {
+            const int j = k + 1 + jj;
+            const double x = a[u] - (vpi * wp[j] + wpi * vp[j]);
+            Ai[int64_t(j) * lda] = x;
+            y += x * vk[j];
+            if (jj == 0) col[i] = x;
+          }
+        }
       }
     }
     ypart[tid] = y;
@@ -162,17 +171,23 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(double* __rest
     vk = t;
     __syncthreads();
   }
-  if (tid == 0) dd[n - 1] = col[n - 1] * sinv;
+  if (tid == 0) dd[n - 1] = col[n - 1];
 }
 
 // Z (n x ncols, leading dimension ldz) <- Q Z (trans = 0) or Q^T Z (trans =
 // 1), Q = H(0) H(1) ... H(n-2).  Column c of Z lives in the registers of
 // the 32 lanes (tid & 31) of half-wave c, NR rows per lane; the dot product
-// with each reflector is a butterfly over those 32 lanes.
+// with each reflector is a butterfly over those 32 lanes.  The reflectors
+// are staged through LDS in chunks of ch (ch * n doubles, loaded by the
+// whole workgroup with many loads in flight), so the per-reflector work
+// reads only LDS.
 template <int NR>
 __global__ __launch_bounds__(1024) void apply_q_kernel(const double* __restrict__ R, int lda,
                                                        const double* __restrict__ tau, int n,
-                                                       double* __restrict__ Z, int ldz, int ncols, int trans) {
+                                                       double* __restrict__ Z, int ldz, int ncols, int trans,
+                                                       int ch) {
+  extern __shared__ double vs[];  // ch reflectors of length n, then ch scalars
+  double* ts = vs + int64_t(ch) * n;
   const int hl = threadIdx.x & 31;
   const int colid = blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5);
   const bool active = colid < ncols;
@@ -184,24 +199,38 @@ __global__ __launch_bounds__(1024) void apply_q_kernel(const double* __restrict_
     z[s] = (active && i < n) ? zc[i] : 0.0;
   }
   const int nref = n - 1;
-  for (int t = 0; t < nref; ++t) {
-    const int k = trans ? t : nref - 1 - t;
-    const double tk = tau[k];
-    if (tk == 0.0) continue;
-    const double* vcol = R + int64_t(k) * lda;
-    double vv[NR];
-    double part = 0.0;
-#pragma unroll
-    for (int s = 0; s < NR; ++s) {
-      const int i = hl + 32 * s;
-      const double v = i <= k ? 0.0 : (i == k + 1 ? 1.0 : (i < n ? vcol[i] : 0.0));
-      vv[s] = v;
-      part += v * z[s];
+  for (int t0 = 0; t0 < nref; t0 += ch) {
+    const int cnt = min(ch, nref - t0);
+    __syncthreads();  // the previous chunk is consumed
+    for (int q = threadIdx.x; q < cnt * n; q += blockDim.x) {
+      const int c = q / n, i = q - c * n;
+      const int t = t0 + c;
+      const int k = trans ? t : nref - 1 - t;
+      vs[q] = i <= k ? 0.0 : (i == k + 1 ? 1.0 : R[i + int64_t(k) * lda]);
     }
-    for (int o = 16; o > 0; o >>= 1) part += __shfl_xor(part, o);
-    const double f = tk * part;
+    for (int c = threadIdx.x; c < cnt; c += blockDim.x) {
+      const int t = t0 + c;
+      ts[c] = tau[trans ? t : nref - 1 - t];
+    }
+    __syncthreads();
+    for (int c = 0; c < cnt; ++c) {
+      const double tk = ts[c];
+      if (tk == 0.0) continue;
+      const double* v = vs + int64_t(c) * n;
+      double part = 0.0;
 #pragma unroll
-    for (int s = 0; s < NR; ++s) z[s] -= f * vv[s];
+      for (int s = 0; s < NR; ++s) {
+        const int i = hl + 32 * s;
+        part += (i < n ? v[i] : 0.0) * z[s];
+      }
+      for (int o = 16; o > 0; o >>= 1) part += __shfl_xor(part, o);
+      const double f = tk * part;
+#pragma unroll
+      for (int s = 0; s < NR; ++s) {
+        const int i = hl + 32 * s;
+        if (i < n) z[s] -= f * v[i];
+      }
+    }
   }
   if (active) {
 #pragma unroll
@@ -231,13 +260,17 @@ Tridiag tridiagonalize(Ctx& c, const Tensor& A) {
   t.d = c.alloc({n}, false);
   t.e = c.alloc({std::max<int64_t>(n - 1, 1)}, false);
   t.tau = c.alloc({std::max<int64_t>(n - 1, 1)}, false);
+  t.ex = std::make_shared<Buffer>();
+  t.ex->pool = c.pool.get();
+  t.ex->bytes = 256;
+  t.ex->ptr = c.pool->get(256);
   const std::string cls = Ctx::bucket("eigh_trd", n);
   hipEvent_t ev = nullptr;
   c.tbegin(cls, &ev);
   c.count(cls, 4.0 / 3.0 * double(n) * n * n, 16.0 * double(n) * n * n / 3.0);
   const size_t lds = size_t(4 * n + kTrdThreads + 64) * sizeof(double);
   hipLaunchKernelGGL(sytrd_lower_kernel, dim3(1), dim3(kTrdThreads), lds, c.stream, t.R.data(), int(n), int(n),
-                     t.d.data(), t.e.data(), t.tau.data());
+                     t.d.data(), t.e.data(), t.tau.data(), static_cast<int*>(t.ex->ptr));
   PEPS_HIP(hipGetLastError());
   c.tend(cls, ev);
   return t;
@@ -254,15 +287,18 @@ void apply_q(Ctx& c, const Tridiag& t, Tensor& Z, bool trans) {
   const int cols_per_block = int(std::min<int64_t>(32, (k + 1) & ~int64_t(1)));
   const dim3 grid(unsigned((k + cols_per_block - 1) / cols_per_block)), block(unsigned(32 * cols_per_block));
   const int tr = trans ? 1 : 0;
+  // reflectors per LDS chunk: ch (n + 1) doubles within kQChunkBytes
+  const int ch = int(std::max<int64_t>(1, std::min<int64_t>(n - 1, kQChunkBytes / (8 * (n + 1)))));
+  const size_t lds = size_t(ch) * size_t(n + 1) * sizeof(double);
   if (n <= 256)
-    hipLaunchKernelGGL(apply_q_kernel<8>, grid, block, 0, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
-                       Z.data(), int(n), int(k), tr);
+    hipLaunchKernelGGL(apply_q_kernel<8>, grid, block, lds, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
+                       Z.data(), int(n), int(k), tr, ch);
   else if (n <= 512)
-    hipLaunchKernelGGL(apply_q_kernel<16>, grid, block, 0, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
-                       Z.data(), int(n), int(k), tr);
+    hipLaunchKernelGGL(apply_q_kernel<16>, grid, block, lds, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
+                       Z.data(), int(n), int(k), tr, ch);
   else
-    hipLaunchKernelGGL(apply_q_kernel<32>, grid, block, 0, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
-                       Z.data(), int(n), int(k), tr);
+    hipLaunchKernelGGL(apply_q_kernel<32>, grid, block, lds, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
+                       Z.data(), int(n), int(k), tr, ch);
   PEPS_HIP(hipGetLastError());
   c.tend(cls, ev);
 }
@@ -283,10 +319,12 @@ void tridiag_eig(Ctx& c, const Tridiag& t, std::vector<double>* w, Tensor* Z) {
   ck_solver(rocsolver_dstedc(c.blas, rocblas_evect_tridiagonal, int(n), d.data(), e.data(), Z->data(), int(n), info),
             "dstedc");
   c.tend(cls, ev);
-  int hinfo = 0;
+  int hinfo = 0, ex = 0;
   PEPS_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  PEPS_HIP(hipMemcpyAsync(&ex, t.ex->ptr, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   *w = c.to_host(d);  // synchronises the stream
   PEPS_CHECK(hinfo == 0, PS_ESOLVER, "stedc did not converge (info=" + std::to_string(hinfo) + ")");
+  for (double& x : *w) x = std::ldexp(x, ex);  // undo the power-of-two scaling
 }
 
 }  // namespace peps

@@ -312,7 +312,8 @@ EIG eigh(Ctx& c, const Tensor& A);
 // lower layout) and tau their scalars.
 struct Tridiag {
   int64_t n = 0;
-  Tensor R, tau, d, e;
+  Tensor R, tau, d, e;            // d, e: tridiagonal of 2^-ex A
+  std::shared_ptr<Buffer> ex;     // device int: the scaling exponent ex
 };
 int64_t sym_tridiag_max_n();
 // Reads the lower triangle of A (real, n x n); one kernel launch.

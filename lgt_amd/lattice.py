@@ -92,3 +92,69 @@ def random_gauge_peps_arrays(nx, ny, gd, gch, D, seed, complex_entries=False):
                 T /= nrm
             tens[ix - 1][iy - 1] = T
     return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv, gch=gch, gd=gd)
+
+
+def init_gauge_peps_arrays(nx, ny, gd, gch, noise=0.0, seed=None):
+    """init_gauge_peps (finite_peps_gauge_invariant.jl:180-218): the bond
+    dimension 1 product state in which every link carries the same flux
+    label b0 (0 for odd gd, the reference's zero-flux state; +1, i.e.
+    E = +1/2, for even gd, which has no zero label) and n_f = -g(ix, iy) on
+    every site, which satisfies the Gauss law.  noise > 0 adds noise * z to
+    the site tensor (z complex Gaussian of unit variance, the reference's
+    randn(ComplexF64), here from numpy.random.default_rng(seed)), re-applies
+    the Gauss mask and renormalises each site."""
+    gch = np.asarray(gch).reshape(nx, ny)
+    b0 = boundary_label(gd)
+    lh = [[np.ones(1) for _ in range(ny)] for _ in range(nx - 1)]
+    lv = [[np.ones(1) for _ in range(ny - 1)] for _ in range(nx)]
+    qh = [[np.array([b0], np.int64) for _ in range(ny)] for _ in range(nx - 1)]
+    qv = [[np.array([b0], np.int64) for _ in range(ny - 1)] for _ in range(nx)]
+    rng = np.random.default_rng(seed)
+    cplx = noise > 0
+    tens = [[None] * ny for _ in range(nx)]
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d, dR, dU = site_dims(ix, iy, nx, ny, gd)
+            nf0 = -int(gch[ix - 1, iy - 1])
+            if not 0 <= nf0 <= 1:
+                raise ValueError(f"init_gauge_peps: n_f = {nf0} out of range at ({ix},{iy})")
+            T = np.zeros((d, 1, 1, 1, 1), np.complex128 if cplx else np.float64)
+            nf, lR, lU = phys_labels(ix, iy, nx, ny, gd)
+            T[np.where((nf == nf0) & (lR == b0) & (lU == b0))[0][0], 0, 0, 0, 0] = 1.0
+            if cplx:
+                T = T + noise * (rng.standard_normal(T.shape) + 1j * rng.standard_normal(T.shape)) / np.sqrt(2.0)
+                T[~gauss_mask(ix, iy, nx, ny, gd, gch[ix - 1, iy - 1], [b0], [b0], [b0], [b0])] = 0
+                nrm = np.sqrt(np.sum(np.abs(T) ** 2))
+                if nrm > 1e-15:
+                    T /= nrm
+            tens[ix - 1][iy - 1] = T
+    return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv, gch=gch, gd=gd)
+
+
+def init_finite_peps_v2_arrays(nx, ny, gd, noise=0.0, seed=None):
+    """init_finite_peps_v2 (finite_peps_full_update.jl:29-62): the label-free
+    bond dimension 1 product state with n_f = 0 on even and 1 on odd ix+iy
+    and links at the flux label b0 (as init_gauge_peps_arrays), plus
+    noise * z on every entry (no mask), each site normalised."""
+    b0 = boundary_label(gd)
+    lh = [[np.ones(1) for _ in range(ny)] for _ in range(nx - 1)]
+    lv = [[np.ones(1) for _ in range(ny - 1)] for _ in range(nx)]
+    qh = [[np.array([b0], np.int64) for _ in range(ny)] for _ in range(nx - 1)]
+    qv = [[np.array([b0], np.int64) for _ in range(ny - 1)] for _ in range(nx)]
+    rng = np.random.default_rng(seed)
+    cplx = noise > 0
+    tens = [[None] * ny for _ in range(nx)]
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d, dR, dU = site_dims(ix, iy, nx, ny, gd)
+            nf0 = 0 if (ix + iy) % 2 == 0 else 1
+            T = np.zeros((d, 1, 1, 1, 1), np.complex128 if cplx else np.float64)
+            nf, lR, lU = phys_labels(ix, iy, nx, ny, gd)
+            T[np.where((nf == nf0) & (lR == b0) & (lU == b0))[0][0], 0, 0, 0, 0] = 1.0
+            if cplx:
+                T = T + noise * (rng.standard_normal(T.shape) + 1j * rng.standard_normal(T.shape)) / np.sqrt(2.0)
+            nrm = np.sqrt(np.sum(np.abs(T) ** 2))
+            if nrm > 1e-15:
+                T /= nrm
+            tens[ix - 1][iy - 1] = T
+    return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv, gch=staggered_charges(nx, ny), gd=gd)

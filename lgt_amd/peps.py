@@ -88,6 +88,7 @@ _SIGS = {
     "peps_trotter_step": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params)]),
     "peps_batch_trotter_step": (C.c_int32, [C.POINTER(_vp), C.c_int32, C.c_int32, C.POINTER(Params), _i32p]),
     "peps_last_als_stats": (C.c_int32, [_vp, _i64p]),
+    "peps_ite_ground_state": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params), C.c_int64, _i64p]),
     "peps_energy_v4": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
     "peps_su_bond_dense": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32]),
     "peps_fu_bond_v2": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32]),
@@ -750,6 +751,64 @@ def trotter_step_batch(states, kind, g, t_hop, m, tau, D_max, chi=64, n_als=30, 
         i = bad[0] if bad else 0
         raise PepsError(rc, f"state {i}: " + (lib().peps_last_error(states[i].ctx.h) or b"").decode())
     return st
+
+
+def tau_stages(n_ite):
+    """The tau annealing of the ite_ground_state drivers: [(tau, steps), ...]."""
+    s1, s2, s3, s4, s5 = n_ite // 8, n_ite // 8, n_ite // 4, n_ite // 6, n_ite // 6
+    return [(0.1, s1), (0.05, s2), (0.02, s3), (0.01, s4), (0.005, s5), (0.001, n_ite - (s1 + s2 + s3 + s4 + s5))]
+
+
+def init_gauge_peps(nx, ny, gd, g_charges, noise=0.0, seed=None, ctx=None) -> GaugePEPS:
+    """init_gauge_peps (finite_peps_gauge_invariant.jl:180-218) on the device."""
+    from . import lattice
+    a = lattice.init_gauge_peps_arrays(nx, ny, gd, g_charges, noise, seed)
+    return GaugePEPS.from_arrays(a["tensors"], a["lh"], a["lv"], a["qh"], a["qv"], a["gch"], gd,
+                                 is_complex=noise > 0, ctx=ctx)
+
+
+def init_finite_peps_v2(nx, ny, gd, noise=0.0, seed=None, ctx=None) -> GaugePEPS:
+    """init_finite_peps_v2 (finite_peps_full_update.jl:29-62) on the device."""
+    from . import lattice
+    a = lattice.init_finite_peps_v2_arrays(nx, ny, gd, noise, seed)
+    return GaugePEPS.from_arrays(a["tensors"], a["lh"], a["lv"], a["qh"], a["qv"], a["gch"], gd,
+                                 is_complex=noise > 0, ctx=ctx)
+
+
+def ite_ground_state(gp: GaugePEPS, kind, g, t_hop, m, D_max, n_ite, chi=64, n_als=30, als_tol=0.0,
+                     renorm=False, positive_cost=False) -> int:
+    """Run the tau-annealed imaginary-time schedule of the ite_ground_state
+    drivers on an existing state in place (kind PEPS_STEP_V4 / _GAUGE /
+    _FULL_V2).  Returns the steps completed."""
+    flags = (PEPS_STEP_RENORM if renorm else 0) | (PEPS_STEP_POSITIVE_COST if positive_cost else 0)
+    p = _params(g, t_hop, m, 0.0, D_max, chi, n_als, als_tol, flags)
+    done = C.c_int64()
+    gp.ctx.check(lib().peps_ite_ground_state(gp.h, kind, C.byref(p), int(n_ite), C.byref(done)))
+    return int(done.value)
+
+
+def ite_ground_state_v4(nx, ny, gd, D_max, g, t_hop, m, g_charges, n_ite=300, noise=0.1, chi=64, n_als=30,
+                        seed=None, ctx=None, **opts) -> GaugePEPS:
+    """ite_ground_state_v4 (finite_peps_fullupdate_v4.jl:440-467)."""
+    gp = init_gauge_peps(nx, ny, gd, g_charges, noise, seed, ctx)
+    ite_ground_state(gp, PEPS_STEP_V4, g, t_hop, m, D_max, n_ite, chi, n_als, **opts)
+    return gp
+
+
+def ite_ground_state_v3(nx, ny, gd, D_max, g, t_hop, m, g_charges, n_ite=600, noise=0.1, seed=None,
+                        ctx=None) -> GaugePEPS:
+    """ite_ground_state_v3 (finite_peps_gauge_invariant.jl:548-577)."""
+    gp = init_gauge_peps(nx, ny, gd, g_charges, noise, seed, ctx)
+    ite_ground_state(gp, PEPS_STEP_GAUGE, g, t_hop, m, D_max, n_ite)
+    return gp
+
+
+def ite_ground_state_v2(nx, ny, gd, D_max, g, t_hop, m, n_ite=600, noise=0.01, seed=None,
+                        ctx=None) -> GaugePEPS:
+    """ite_ground_state_v2 (finite_peps_full_update.jl:582-629, use_gauss=false)."""
+    gp = init_finite_peps_v2(nx, ny, gd, noise, seed, ctx)
+    ite_ground_state(gp, PEPS_STEP_FULL_V2, g, t_hop, m, D_max, n_ite)
+    return gp
 
 
 def compute_energy_v4(gp: GaugePEPS, g, t_hop, m, chi=64, renorm=False) -> float:

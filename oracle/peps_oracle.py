@@ -378,24 +378,26 @@ def init_gauge_peps_random(nx, ny, gd, gch, D, seed, complex_entries=False):
 
 
 def init_gauge_peps(nx, ny, gd, gch, noise=0.0, seed=None):
-    """finite_peps_gauge_invariant.jl:180-218 (odd gd: product state at zero flux)."""
-    assert gd % 2 == 1, "zero-flux product state needs an odd link dimension"
+    """finite_peps_gauge_invariant.jl:180-218: every link at the flux label
+    b0 (zero for odd gd, the reference's state; +1, E = +1/2, for even gd,
+    which has no zero label), n_f = -g on every site."""
+    b0 = boundary_label(gd)
     gp = GaugePEPS(nx, ny, gd, gch)
     for iy in range(1, ny + 1):
         for ix in range(1, nx):
             gp.lh[ix - 1][iy - 1] = np.array([1.0])
-            gp.qh[ix - 1][iy - 1] = np.array([0], np.int64)
+            gp.qh[ix - 1][iy - 1] = np.array([b0], np.int64)
     for iy in range(1, ny):
         for ix in range(1, nx + 1):
             gp.lv[ix - 1][iy - 1] = np.array([1.0])
-            gp.qv[ix - 1][iy - 1] = np.array([0], np.int64)
+            gp.qv[ix - 1][iy - 1] = np.array([b0], np.int64)
     for iy in range(1, ny + 1):
         for ix in range(1, nx + 1):
             d, dR, dU = site_dims(ix, iy, nx, ny, gd)
             arr = np.zeros((d, 1, 1, 1, 1), np.complex128)
             nf0 = -int(gp.gch[ix - 1, iy - 1])
             assert 0 <= nf0 <= 1
-            arr[site_idx(nf0, 0, 0, dR, dU, gd) - 1, 0, 0, 0, 0] = 1.0
+            arr[site_idx(nf0, b0, b0, dR, dU, gd) - 1, 0, 0, 0, 0] = 1.0
             gp.setT(ix, iy, arr)
     if noise > 0:
         rng = np.random.default_rng(seed)
@@ -1193,6 +1195,39 @@ def trotter_step_v4(gp, g, t_hop, m, tau, D_max, chi=64, n_als=30, positive_cost
             G = gate(H_merged_h_site(ix, iy, nxv, nyv, gd, g, t_hop, m), tau / 2)
             full_update_bond_h_v4(gp, ix, iy, G, D_max, chi, n_als, positive_cost=positive_cost, infos=infos,
                                   renorm=renorm, als_tol=als_tol)
+
+
+def tau_stages(n_ite):
+    """The tau annealing of ite_ground_state_v4/_v3/_v2
+    (finite_peps_fullupdate_v4.jl:447-449, finite_peps_gauge_invariant.jl:
+    556-559, finite_peps_full_update.jl:595-608)."""
+    s1, s2, s3, s4, s5 = n_ite // 8, n_ite // 8, n_ite // 4, n_ite // 6, n_ite // 6
+    return [(0.1, s1), (0.05, s2), (0.02, s3), (0.01, s4), (0.005, s5), (0.001, n_ite - (s1 + s2 + s3 + s4 + s5))]
+
+
+def ite_ground_state_v4(gp, g, t_hop, m, D_max, n_ite=300, chi=64, n_als=30, **opts):
+    """ite_ground_state_v4 (finite_peps_fullupdate_v4.jl:440-467) from the
+    initial state gp (init_gauge_peps), stepped in place."""
+    for tau, ns in tau_stages(n_ite):
+        for _ in range(ns):
+            trotter_step_v4(gp, g, t_hop, m, tau, D_max, chi, n_als, **opts)
+    return gp
+
+
+def ite_ground_state_v3(gp, g, t_hop, m, D_max, n_ite=600):
+    """ite_ground_state_v3 (finite_peps_gauge_invariant.jl:548-577) from gp."""
+    for tau, ns in tau_stages(n_ite):
+        for _ in range(ns):
+            trotter_step_gauge(gp, g, t_hop, m, tau, D_max)
+    return gp
+
+
+def ite_ground_state_v2(gp, g, t_hop, m, D_max, n_ite=600):
+    """ite_ground_state_v2 (finite_peps_full_update.jl:582-629, use_gauss=false) from gp."""
+    for tau, ns in tau_stages(n_ite):
+        for _ in range(ns):
+            trotter_step_full(gp, g, t_hop, m, tau, D_max)
+    return gp
 
 
 def bond_expect(E, h, dpL, dpR):

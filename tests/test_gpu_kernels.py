@@ -183,8 +183,9 @@ def test_svd(ctx, cplx):
         assert np.abs(Vt @ np.conj(Vt.T) - np.eye(k)).max() < 1e-15
 
 
-@pytest.mark.gpu
-def test_eigh_large_and_lower_triangle_only(ctx):
+@pytest.mark.parametrize("n,scale,garbage", [(300, 1.0, False), (300, 1e-120, False), (300, 1.0, True),
+                                             (512, 1.0, True), (700, 3e90, True)])
+def test_eigh_large_and_lower_triangle_only(ctx, n, scale, garbage):
     """Real sizes that take the 16- and 32-row register variants of the
     reflector kernel (n = 300, 512, 700), badly scaled entries (the
     tridiagonal reduction rescales by a power of two), and the LAPACK
@@ -192,13 +193,12 @@ def test_eigh_large_and_lower_triangle_only(ctx):
     upper triangle must not change the result."""
     from lgt_amd import peps
     rng = np.random.default_rng(8)
-    for n, scale in ((300, 1e-120), (512, 1.0), (700, 3e90)):
-        B = rng.standard_normal((n, n))
-        H = (B + B.T) * scale
-        G = np.tril(H) + np.triu(rng.standard_normal((n, n)) * 1e3 * scale, 1)
-        w, V = peps.eigh(G[None], ctx=ctx)
-        w, V = w[0], V[0]
-        ref = np.linalg.eigvalsh(H)
-        assert np.abs(w - ref).max() < 1e-13 * n * np.abs(ref).max()
-        assert np.abs(V.T @ V - np.eye(n)).max() < 1e-13 * n
-        assert np.abs(H @ V - V * w).max() < 1e-13 * n * np.abs(ref).max()
+    B = rng.standard_normal((n, n))
+    H = (B + B.T) * scale
+    G = np.tril(H) + (np.triu(rng.standard_normal((n, n)) * 1e3 * scale, 1) if garbage else np.triu(H, 1))
+    w, V = peps.eigh(G[None], ctx=ctx)
+    w, V = w[0], V[0]
+    ref = np.linalg.eigvalsh(H)
+    assert np.abs(w - ref).max() < 1e-13 * n * np.abs(ref).max()
+    assert np.abs(V.T @ V - np.eye(n)).max() < 1e-13 * n
+    assert np.abs(H @ V - V * w).max() < 1e-13 * n * np.abs(ref).max()
