@@ -84,6 +84,7 @@ Ctx::Ctx(int dev) : device(dev) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   if (const char* v = std::getenv("PEPS_HIP_VERBOSE")) verbose = std::atoi(v);
+  if (const char* v = std::getenv("PEPS_HIP_TRD")) trd_backend = std::string(v) == "rocsolver" ? 1 : 0;
   if (const char* v = std::getenv("PEPS_HIP_EIG_DIRECT_MAX")) eig_direct_max = std::max(0, std::atoi(v));
   if (const char* v = std::getenv("PEPS_HIP_GEMM_SHAPES")) gemm_shapes = std::atoi(v) != 0;
 }

@@ -1,10 +1,11 @@
 // Real symmetric eigensolver for n <= 1024 on one MI355X:
-//   1. sytrd_lower_kernel: Householder reduction to tridiagonal form in ONE
-//      workgroup (1024 threads) and one launch.  The trailing matrix is read
-//      and written once per reflector: each pass applies the previous
-//      step's rank-2 update and forms the next matrix-vector product in the
-//      same sweep over memory, and the column the next reflector needs is
-//      kept in LDS, so every step touches global memory exactly once.
+//   1. Householder reduction to tridiagonal form in ONE workgroup and one
+//      launch: for n <= 208 sytrd_lower_reg_kernel keeps the lower triangle
+//      in registers (no global memory in the reflector loop); above that
+//      sytrd_lower_kernel reads and writes the trailing matrix once per
+//      reflector: each pass applies the previous step's rank-2 update and
+//      forms the next matrix-vector product in the same sweep over memory,
+//      and the column the next reflector needs is kept in LDS.
 //   2. the symmetric tridiagonal eigenproblem (divide and conquer, rocSOLVER
 //      stedc on the n-vectors d, e);
 //   3. apply_q_kernel: Z <- Q Z or Q^T Z with the n - 1 reflectors, each
@@ -174,29 +175,282 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(double* __rest
   if (tid == 0) dd[n - 1] = col[n - 1];
 }
 
+// Cross-lane sums of a double within DPP rows (VALU moves, no LDS):
+// over the 4 lanes of a quad (lane bits 0, 1), and over the lanes l, l+4,
+// l+8, l+12 of a 16-lane row (lane bits 2, 3).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double quad_sum(double x) {
+  x += dpp_f64<0xB1>(x);  // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E>(x);  // quad_perm [2,3,0,1]
+  return x;
+}
+__device__ __forceinline__ double row4_sum(double x) {
+  x += dpp_f64<0x124>(x);  // row_ror:4
+  x += dpp_f64<0x128>(x);  // row_ror:8
+  return x;
+}
+
+// Register-resident variant of the same reduction for n <= 16 * NT (NT tile
+// rows): the lower triangle of A lives in the registers of 8 waves as 16x16
+// tiles (tile (I, J), I >= J, numbered column by column; a diagonal tile
+// holds its full symmetric block).  Each 16-lane group of a wave owns one
+// tile per slot (tile 4 (8 s + wave) + group), each lane a 4x4 block of it:
+// rows 4 (l >> 2) + {0..3}, columns 4 (l & 3) + {0..3} of the tile (l the
+// lane within the group).  With LDS0 the first slot -- tiles of the first
+// two block columns, which are finished after 32 reflectors -- is kept in
+// LDS instead of registers.  Per reflector: the column is gathered into
+// LDS, y = A v is formed from the tiles -- row sums over the 4 lanes of a
+// quad, column sums of the off-diagonal tiles over the 4 lanes 4 apart,
+// both with DPP -- into one LDS buffer per (wave, group), summed in a fixed
+// order, and the rank-2 update is applied in registers.  No global memory is touched between the initial load and the
+// reflectors' store.
+template <int NT, int SLOTS, bool LDS0>
+__global__ __launch_bounds__(512) void sytrd_lower_reg_kernel(const double* __restrict__ A, int n, int lda,
+                                                               double* __restrict__ R, double* __restrict__ dd,
+                                                               double* __restrict__ ee, double* __restrict__ tt,
+                                                               int* __restrict__ exo) {
+  constexpr int NP = 16 * NT;     // padded size
+  constexpr int NW = 8;           // waves
+  constexpr int NTILES = NT * (NT + 1) / 2;
+  constexpr int RS = LDS0 ? SLOTS - 1 : SLOTS;   // slots held in registers
+  static_assert((NTILES + 4 * NW - 1) / (4 * NW) <= SLOTS, "tile slots");
+  static_assert(NP <= 512, "one index per thread");
+  extern __shared__ double dsm[];
+  double (*colb)[NP] = reinterpret_cast<double (*)[NP]>(dsm);         // [4][NP]
+  double* vk = dsm + 4 * NP;
+  double* wv = vk + NP;
+  double (*ybuf)[NP] = reinterpret_cast<double (*)[NP]>(wv + NP);     // [4 NW][NP]
+  double* red0 = wv + NP + 4 * NW * NP;
+  double* red1 = red0 + 32;
+  double* slot0 = red1 + 32;                                           // [16][512] when LDS0
+  __shared__ unsigned char tI[NTILES], tJ[NTILES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int grp = lane >> 4, l16 = lane & 15;
+  const int rb = 4 * (l16 >> 2), cb = 4 * (l16 & 3);
+  if (tid == 0) {
+    int t = 0;
+    for (int J = 0; J < NT; ++J)
+      for (int I = J; I < NT; ++I, ++t) {
+        tI[t] = (unsigned char)I;
+        tJ[t] = (unsigned char)J;
+      }
+  }
+  __syncthreads();
+  auto tile_of = [&](int s) { return 4 * (NW * s + wave) + grp; };
+  // load: lower triangle, mirrored into diagonal tiles; padding is zero
+  double a[RS > 0 ? RS : 1][4][4];
+  double b[4][4];
+  double mx = 0.0;
+  auto load_tile = [&](int t, double (&blk)[4][4]) {
+#pragma unroll
+    for (int ri = 0; ri < 4; ++ri)
+#pragma unroll
+      for (int ci = 0; ci < 4; ++ci) {
+        double x = 0.0;
+        if (t < NTILES) {
+          const int i = 16 * tI[t] + rb + ri, j = 16 * tJ[t] + cb + ci;
+          if (i < n && j < n) x = i >= j ? A[i + int64_t(j) * lda] : A[j + int64_t(i) * lda];
+        }
+        blk[ri][ci] = x;
+        mx = fmax(mx, fabs(x));
+      }
+  };
+#pragma unroll
+  for (int s = 0; s < RS; ++s) load_tile(tile_of(s + (LDS0 ? 1 : 0)), a[s]);
+  if (LDS0) {
+    load_tile(tile_of(0), b);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) slot0[e * 512 + tid] = b[e >> 2][e & 3];
+  }
+  mx = trd_block_max(mx, red0);
+  const int ex = mx > 0.0 ? ilogb(mx) : 0;
+  const double sc = ldexp(1.0, -ex);
+  if (tid == 0) *exo = ex;
+#pragma unroll
+  for (int s = 0; s < RS; ++s)
+#pragma unroll
+    for (int ri = 0; ri < 4; ++ri)
+#pragma unroll
+      for (int ci = 0; ci < 4; ++ci) a[s][ri][ci] *= sc;
+  if (LDS0) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) slot0[e * 512 + tid] *= sc;
+  }
+  __syncthreads();
+  auto active = [&](int t, int k) { return t < NTILES && 16 * tJ[t] + 15 > k; };
+  for (int k = 0; k < n; ++k) {
+    const int kJ = k >> 4, kc = k & 15;
+    double (*cb4)[NP] = colb;
+    for (int q = tid; q < 4 * NW * NP; q += 512) (&ybuf[0][0])[q] = 0.0;
+    // gather column k: the owners write all four columns of their block
+    auto gather = [&](int t, const double (&blk)[4][4]) {
+      if (t < NTILES && tJ[t] == kJ && cb == (kc & ~3)) {
+        const int i = 16 * tI[t] + rb;
+#pragma unroll
+        for (int ri = 0; ri < 4; ++ri)
+#pragma unroll
+          for (int ci = 0; ci < 4; ++ci) cb4[ci][i + ri] = blk[ri][ci];
+      }
+    };
+    if (LDS0 && kJ <= 1) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) b[e >> 2][e & 3] = slot0[e * 512 + tid];
+      gather(tile_of(0), b);
+    }
+#pragma unroll
+    for (int s = 0; s < RS; ++s) gather(tile_of(s + (LDS0 ? 1 : 0)), a[s]);
+    __syncthreads();
+    const double* col = cb4[kc & 3];
+    if (k == n - 1) {
+      if (tid == 0) dd[n - 1] = col[n - 1];
+      break;
+    }
+    double part = 0.0;
+    for (int i = k + 2 + tid; i < n; i += 512) part += col[i] * col[i];
+    const double xn2 = trd_block_sum(part, red0);
+    const double alpha = col[k + 1];
+    double tau = 0.0, beta = alpha, scl = 0.0;
+    if (xn2 > 0.0) {
+      const double nrm = sqrt(alpha * alpha + xn2);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      tau = (beta - alpha) / beta;
+      scl = 1.0 / (alpha - beta);
+    }
+    if (tid == 0) {
+      dd[k] = col[k];
+      ee[k] = beta;
+      tt[k] = tau;
+    }
+    for (int i = tid; i < NP; i += 512) {
+      const double v = (i <= k || i >= n) ? 0.0 : (i == k + 1 ? 1.0 : col[i] * scl);
+      vk[i] = v;
+      if (i >= k + 2 && i < n) R[i + int64_t(k) * lda] = v;
+    }
+    __syncthreads();
+    auto vof = [&](int j) { return vk[j]; };
+    // y = A v (a tile is active while its columns reach k + 1); each
+    // (wave, group) adds into its own buffer: tiles of one group are visited
+    // in order, and no two groups share a buffer
+    double* yb = ybuf[4 * wave + grp];
+    auto matvec = [&](int t, const double (&blk)[4][4]) {
+      const int I0 = 16 * tI[t] + rb, J0 = 16 * tJ[t] + cb;
+      double vj[4], vi[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        vj[q] = vof(J0 + q);
+        vi[q] = vof(I0 + q);
+      }
+#pragma unroll
+      for (int ri = 0; ri < 4; ++ri) {
+        double y = 0.0;
+#pragma unroll
+        for (int ci = 0; ci < 4; ++ci) y += blk[ri][ci] * vj[ci];
+        y = quad_sum(y);
+        if (cb == 0) yb[I0 + ri] += y;
+      }
+      if (tI[t] != tJ[t]) {
+#pragma unroll
+        for (int ci = 0; ci < 4; ++ci) {
+          double y = 0.0;
+#pragma unroll
+          for (int ri = 0; ri < 4; ++ri) y += blk[ri][ci] * vi[ri];
+          y = row4_sum(y);
+          if (rb == 0) yb[J0 + ci] += y;
+        }
+      }
+    };
+    if (LDS0 && active(tile_of(0), k)) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) b[e >> 2][e & 3] = slot0[e * 512 + tid];
+      matvec(tile_of(0), b);
+    }
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+      const int t = tile_of(s + (LDS0 ? 1 : 0));
+      if (active(t, k)) matvec(t, a[s]);
+    }
+    __syncthreads();
+    // thread i owns index i (NP <= 512): y_i, w_i = tau y_i - tau^2 (y.v)/2 v_i
+    double yi = 0.0, pdot = 0.0;
+    const int i = tid;
+    if (i > k && i < n) {
+#pragma unroll 8
+      for (int w = 0; w < 4 * NW; ++w) yi += ybuf[w][i];
+      yi *= tau;
+      pdot = yi * vof(i);
+    }
+    const double K = trd_block_sum(pdot, red1);
+    const double half = -0.5 * tau * K;
+    if (i < NP) wv[i] = (i > k && i < n) ? yi + half * vof(i) : 0.0;
+    __syncthreads();
+    // rank-2 update A -= v w^T + w v^T on the active tiles
+    auto update = [&](int t, double (&blk)[4][4]) {
+      const int I0 = 16 * tI[t] + rb, J0 = 16 * tJ[t] + cb;
+      double vj[4], wj[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        vj[q] = vof(J0 + q);
+        wj[q] = wv[J0 + q];
+      }
+#pragma unroll
+      for (int ri = 0; ri < 4; ++ri) {
+        const double vi = vof(I0 + ri), wi = wv[I0 + ri];
+#pragma unroll
+        for (int ci = 0; ci < 4; ++ci) blk[ri][ci] -= vi * wj[ci] + wi * vj[ci];
+      }
+    };
+    if (LDS0 && active(tile_of(0), k)) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) b[e >> 2][e & 3] = slot0[e * 512 + tid];
+      update(tile_of(0), b);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) slot0[e * 512 + tid] = b[e >> 2][e & 3];
+    }
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+      const int t = tile_of(s + (LDS0 ? 1 : 0));
+      if (active(t, k)) update(t, a[s]);
+    }
+    __syncthreads();
+  }
+}
+
+template <int NT, bool LDS0>
+constexpr size_t trd_reg_lds_bytes() {
+  return sizeof(double) * (size_t(4 + 2 + 32) * 16 * NT + 64 + (LDS0 ? 16 * 512 : 0));
+}
+
 // Z (n x ncols, leading dimension ldz) <- Q Z (trans = 0) or Q^T Z (trans =
-// 1), Q = H(0) H(1) ... H(n-2).  Column c of Z lives in the registers of
-// the 32 lanes (tid & 31) of half-wave c, NR rows per lane; the dot product
-// with each reflector is a butterfly over those 32 lanes.  The reflectors
-// are staged through LDS in chunks of ch (ch * n doubles, loaded by the
-// whole workgroup with many loads in flight), so the per-reflector work
-// reads only LDS.
-template <int NR>
-__global__ __launch_bounds__(1024) void apply_q_kernel(const double* __restrict__ R, int lda,
-                                                       const double* __restrict__ tau, int n,
-                                                       double* __restrict__ Z, int ldz, int ncols, int trans,
-                                                       int ch) {
+// 1), Q = H(0) H(1) ... H(n-2).  Each half-wave owns CPH columns of Z, held
+// in the registers of its 32 lanes (NR rows per lane); the dot products with
+// each reflector are butterflies over those 32 lanes.  The reflectors are
+// staged through LDS in chunks of ch (ch * n doubles, loaded by the whole
+// workgroup), and each lane reads a reflector's entries once for all its
+// columns.
+template <int NR, int CPH>
+__global__ __launch_bounds__(256) void apply_q_kernel(const double* __restrict__ R, int lda,
+                                                      const double* __restrict__ tau, int n,
+                                                      double* __restrict__ Z, int ldz, int ncols, int trans,
+                                                      int ch) {
   extern __shared__ double vs[];  // ch reflectors of length n, then ch scalars
   double* ts = vs + int64_t(ch) * n;
   const int hl = threadIdx.x & 31;
-  const int colid = blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5);
-  const bool active = colid < ncols;
-  double* zc = Z + int64_t(active ? colid : 0) * ldz;
-  double z[NR];
+  const int col0 = (blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5)) * CPH;
+  double z[CPH][NR];
 #pragma unroll
-  for (int s = 0; s < NR; ++s) {
-    const int i = hl + 32 * s;
-    z[s] = (active && i < n) ? zc[i] : 0.0;
+  for (int c = 0; c < CPH; ++c) {
+    const bool act = col0 + c < ncols;
+    const double* zc = Z + int64_t(act ? col0 + c : 0) * ldz;
+#pragma unroll
+    for (int s = 0; s < NR; ++s) {
+      const int i = hl + 32 * s;
+      z[c][s] = (act && i < n) ? zc[i] : 0.0;
+    }
   }
   const int nref = n - 1;
   for (int t0 = 0; t0 < nref; t0 += ch) {
@@ -217,26 +471,32 @@ __global__ __launch_bounds__(1024) void apply_q_kernel(const double* __restrict_
       const double tk = ts[c];
       if (tk == 0.0) continue;
       const double* v = vs + int64_t(c) * n;
-      double part = 0.0;
+      double vv[NR];
 #pragma unroll
       for (int s = 0; s < NR; ++s) {
         const int i = hl + 32 * s;
-        part += (i < n ? v[i] : 0.0) * z[s];
+        vv[s] = i < n ? v[i] : 0.0;
       }
-      for (int o = 16; o > 0; o >>= 1) part += __shfl_xor(part, o);
-      const double f = tk * part;
 #pragma unroll
-      for (int s = 0; s < NR; ++s) {
-        const int i = hl + 32 * s;
-        if (i < n) z[s] -= f * v[i];
+      for (int cc = 0; cc < CPH; ++cc) {
+        double part = 0.0;
+#pragma unroll
+        for (int s = 0; s < NR; ++s) part += vv[s] * z[cc][s];
+        for (int o = 16; o > 0; o >>= 1) part += __shfl_xor(part, o);
+        const double f = tk * part;
+#pragma unroll
+        for (int s = 0; s < NR; ++s) z[cc][s] -= f * vv[s];
       }
     }
   }
-  if (active) {
+#pragma unroll
+  for (int c = 0; c < CPH; ++c) {
+    if (col0 + c >= ncols) break;
+    double* zc = Z + int64_t(col0 + c) * ldz;
 #pragma unroll
     for (int s = 0; s < NR; ++s) {
       const int i = hl + 32 * s;
-      if (i < n) zc[i] = z[s];
+      if (i < n) zc[i] = z[c][s];
     }
   }
 }
@@ -268,9 +528,43 @@ Tridiag tridiagonalize(Ctx& c, const Tensor& A) {
   hipEvent_t ev = nullptr;
   c.tbegin(cls, &ev);
   c.count(cls, 4.0 / 3.0 * double(n) * n * n, 16.0 * double(n) * n * n / 3.0);
-  const size_t lds = size_t(4 * n + kTrdThreads + 64) * sizeof(double);
-  hipLaunchKernelGGL(sytrd_lower_kernel, dim3(1), dim3(kTrdThreads), lds, c.stream, t.R.data(), int(n), int(n),
-                     t.d.data(), t.e.data(), t.tau.data(), static_cast<int*>(t.ex->ptr));
+  int* exo = static_cast<int*>(t.ex->ptr);
+  const int ni = int(n);
+#define PEPS_TRD_REG(NT, SLOTS, L0)                                                                           \
+  do {                                                                                                      \
+    const size_t lds = trd_reg_lds_bytes<NT, L0>();                                                         \
+    static bool attr = false;                                                                               \
+    if (!attr) {                                                                                            \
+      PEPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sytrd_lower_reg_kernel<NT, SLOTS, L0>),     \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                  \
+      attr = true;                                                                                          \
+    }                                                                                                       \
+    hipLaunchKernelGGL((sytrd_lower_reg_kernel<NT, SLOTS, L0>), dim3(1), dim3(512), lds, c.stream, t.R.data(), \
+                       ni, ni, t.R.data(), t.d.data(), t.e.data(), t.tau.data(), exo);                     \
+  } while (0)
+  if (n <= 64)
+    PEPS_TRD_REG(4, 1, false);
+  else if (n <= 144)
+    PEPS_TRD_REG(9, 2, false);
+  else if (n <= 208)
+    PEPS_TRD_REG(13, 3, false);
+  else if (c.trd_backend == 1) {
+#undef PEPS_TRD_REG
+    // rocSOLVER sytrd (same LAPACK reflector layout) on the matrix scaled by
+    // the power of two that brings its Frobenius norm into [1, 2)
+    const double fro = std::sqrt(norm2sq(c, t.R));
+    const int ex = fro > 0.0 ? std::ilogb(fro) : 0;
+    if (ex != 0) scale(c, t.R, std::ldexp(1.0, -ex));
+    PEPS_HIP(hipMemcpyAsync(exo, &ex, sizeof(int), hipMemcpyHostToDevice, c.stream));
+    PEPS_HIP(hipStreamSynchronize(c.stream));  // ex lives on this stack frame
+    ck_solver(rocsolver_dsytrd(c.blas, rocblas_fill_lower, ni, t.R.data(), ni, t.d.data(), t.e.data(),
+                               t.tau.data()),
+              "dsytrd");
+  } else {
+    const size_t lds = size_t(4 * n + kTrdThreads + 64) * sizeof(double);
+    hipLaunchKernelGGL(sytrd_lower_kernel, dim3(1), dim3(kTrdThreads), lds, c.stream, t.R.data(), ni, ni,
+                       t.d.data(), t.e.data(), t.tau.data(), exo);
+  }
   PEPS_HIP(hipGetLastError());
   c.tend(cls, ev);
   return t;
@@ -284,21 +578,28 @@ void apply_q(Ctx& c, const Tridiag& t, Tensor& Z, bool trans) {
   hipEvent_t ev = nullptr;
   c.tbegin(cls, &ev);
   c.count(cls, 4.0 * double(n) * n * k, 8.0 * (double(n) * n / 2.0 + 2.0 * double(n) * k));
-  const int cols_per_block = int(std::min<int64_t>(32, (k + 1) & ~int64_t(1)));
-  const dim3 grid(unsigned((k + cols_per_block - 1) / cols_per_block)), block(unsigned(32 * cols_per_block));
   const int tr = trans ? 1 : 0;
   // reflectors per LDS chunk: ch (n + 1) doubles within kQChunkBytes
   const int ch = int(std::max<int64_t>(1, std::min<int64_t>(n - 1, kQChunkBytes / (8 * (n + 1)))));
   const size_t lds = size_t(ch) * size_t(n + 1) * sizeof(double);
-  if (n <= 256)
-    hipLaunchKernelGGL(apply_q_kernel<8>, grid, block, lds, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
-                       Z.data(), int(n), int(k), tr, ch);
-  else if (n <= 512)
-    hipLaunchKernelGGL(apply_q_kernel<16>, grid, block, lds, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
-                       Z.data(), int(n), int(k), tr, ch);
-  else
-    hipLaunchKernelGGL(apply_q_kernel<32>, grid, block, lds, c.stream, t.R.data(), int(n), t.tau.data(), int(n),
-                       Z.data(), int(n), int(k), tr, ch);
+  // one column per half-wave for a few right-hand sides, four for many
+  const int cph = k <= 32 ? 1 : 4;
+  const int64_t halfwaves = (k + cph - 1) / cph;
+  const int hw_per_block = int(std::min<int64_t>(8, halfwaves + (halfwaves & 1)));
+  const dim3 grid(unsigned((halfwaves + hw_per_block - 1) / hw_per_block)), block(unsigned(32 * hw_per_block));
+#define PEPS_APPLY_Q(NR, CPH)                                                                                   \
+  hipLaunchKernelGGL((apply_q_kernel<NR, CPH>), grid, block, lds, c.stream, t.R.data(), int(n), t.tau.data(), \
+                     int(n), Z.data(), int(n), int(k), tr, ch)
+  if (cph == 1) {
+    if (n <= 256) PEPS_APPLY_Q(8, 1);
+    else if (n <= 512) PEPS_APPLY_Q(16, 1);
+    else PEPS_APPLY_Q(32, 1);
+  } else {
+    if (n <= 256) PEPS_APPLY_Q(8, 4);
+    else if (n <= 512) PEPS_APPLY_Q(16, 4);
+    else PEPS_APPLY_Q(32, 4);
+  }
+#undef PEPS_APPLY_Q
   PEPS_HIP(hipGetLastError());
   c.tend(cls, ev);
 }

@@ -147,6 +147,10 @@ class Ctx {
   // counts a warning (peps_warning_count); an unconverged subspace iteration
   // is then answered by a direct eigendecomposition (N <= 4096).
   int subspace_max_it = 25;
+  // tridiagonal reduction above the register kernel's size (n > 208): 0 the
+  // library's one-launch kernel, 1 rocSOLVER sytrd (environment variable
+  // PEPS_HIP_TRD=rocsolver)
+  int trd_backend = 0;
   int jacobi_max_sweeps = 80;
   // account GEMMs per exact shape ("gemm_f32 MxNxK ...") instead of per
   // precision (environment variable PEPS_HIP_GEMM_SHAPES=1)
