@@ -150,6 +150,10 @@ int32_t peps_list_destroy(peps_tensor_list* l);
 /* _dl_traced (finite_peps_full_update.jl:79-91): T (p,l,r,u,d) -> E (l²,r²,u²,d²) */
 int32_t peps_dl_traced(peps_ctx* ctx, int32_t is_complex, const int64_t dims[5], const double* T,
                        double* E);
+/* _dl_open (finite_peps_full_update.jl:97-109): T (p,l,r,u,d) ->
+   E (p, p', l², r², u², d²) = T[p,..] conj(T[p',..]) with the bra index
+   fast in every merged pair (the physical indices left open) */
+int32_t peps_dl_open(peps_ctx* ctx, int32_t is_complex, const int64_t dims[5], const double* T, double* E);
 /* absorb_row_from_top / _bottom (finite_peps_boundary_mps.jl:128,156).
    mps_dims: 3n, row_dims: 4n. */
 int32_t peps_absorb_row(peps_ctx* ctx, int32_t from_top, int32_t is_complex, int32_t n,

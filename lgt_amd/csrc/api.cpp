@@ -340,6 +340,15 @@ int32_t peps_dl_traced(peps_ctx* ctx, int32_t is_complex, const int64_t dims[5],
   });
 }
 
+int32_t peps_dl_open(peps_ctx* ctx, int32_t is_complex, const int64_t dims[5], const double* T, double* E) {
+  if (!ctx || !dims || !T || !E) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    for (int i = 0; i < 5; ++i) PEPS_CHECK(dims[i] >= 1, peps::PS_ESHAPE, "dl_open: dims must be >= 1");
+    Tensor t = ctx->c->upload(std::vector<int64_t>(dims, dims + 5), is_complex != 0, T);
+    ctx->c->download(peps::dl_open(*ctx->c, t), E);
+  });
+}
+
 static peps::MPS upload_list(peps::Ctx& c, bool cplx, int n, int rank, const int64_t* dims,
                              const double* const* data) {
   peps::MPS m;

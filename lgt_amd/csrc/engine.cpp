@@ -823,26 +823,87 @@ std::vector<int64_t> iota_v(int64_t n) {
 }
 }  // namespace
 
+namespace {
+// The leading k singular triplets of B (m x n) by randomized subspace
+// iteration (Halko, Martinsson & Tropp 2011, Alg. 4.4) with a Rayleigh-Ritz
+// SVD of Q^H B (p x n, p = k + 16): U = Q Ub.  Accepted only when every
+// kept triplet satisfies ||B^H u_i - sigma_i v_i|| <= 1e-13 sigma_0 -- the
+// triplets of the full SVD to rounding -- else returns false and the caller
+// runs the full SVD.  Worth it only for blocks much larger than k.
+bool leading_svd(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
+  TierGuard fp64(c, TIER_FP64);
+  const int64_t m = B.dims[0], n = B.dims[1], kmin = std::min(m, n);
+  k = std::min(k, kmin);
+  const int64_t p = std::min<int64_t>(kmin, k + 16);
+  if (k < 1 || kmin < 64 || 2 * p > kmin) return false;
+  Tensor Om = c.alloc({n, p}, B.cplx);
+  fill_random(c, Om, 0x5eedull + uint64_t(m) * 1315423911ull + uint64_t(n));
+  Tensor Q = qr_thin(c, matmul(c, B, Om)).Q;  // m x p
+  constexpr int kMaxIt = 6;
+  for (int it = 0; it < kMaxIt; ++it) {
+    Tensor Qz = qr_thin(c, matmul(c, B, Q, true, false, true, false)).Q;  // n x p, span B^H Q
+    Q = qr_thin(c, matmul(c, B, Qz)).Q;                                    // m x p, span B B^H Q
+    if (it == 0) continue;
+    Tensor Bt = matmul(c, Q, B, true, false, true, false);  // p x n
+    SVD f = svd_thin(c, Bt);
+    if (f.S.empty() || !(f.S[0] > 0.0)) return false;
+    Tensor Uk = matmul(c, Q, f.U.slice_last(0, k));                  // m x k
+    Tensor Vtk = gather2d(c, f.Vt, iota_v(k), iota_v(n));            // k x n
+    // residuals B^H u_i - sigma_i v_i (n x k)
+    Tensor Rz = matmul(c, B, Uk, true, false, true, false);
+    Tensor Vk = permute(c, Vtk, {1, 0});
+    if (Vk.data() == Vtk.data()) Vk = copy_tensor(c, Vk);
+    if (B.cplx) conj_inplace(c, Vk);
+    std::vector<double> ms(k);
+    for (int64_t i = 0; i < k; ++i) ms[i] = -f.S[i];
+    scale_cols(c, Vk, ms);
+    axpy_inplace(c, Rz, Vk, 1.0);
+    std::vector<double> r2 = col_norm2(c, Rz);
+    bool ok = true;
+    for (double x : r2) ok = ok && std::isfinite(x) && std::sqrt(x) <= 1e-13 * f.S[0];
+    c.count("svd_leading[iterations]", double(it + 1), 0);
+    if (ok) {
+      out->U = Uk;
+      out->Vt = Vtk;
+      out->S.assign(f.S.begin(), f.S.begin() + k);
+      return true;
+    }
+  }
+  c.count("svd_leading[fallback]", 1.0, 0);
+  return false;
+}
+}  // namespace
+
 CSVD charge_svd(Ctx& c, const Tensor& M, const std::vector<int64_t>& rq,
-                const std::vector<int64_t>& cq, int Dmax, double reg) {
+                const std::vector<int64_t>& cq, int Dmax, double reg, bool leading_only) {
   // the sector blocks are independent: their SVDs run concurrently
   std::vector<SectorSVD> secs;
   std::vector<Tensor> blocks;
+  std::vector<size_t> full;  // sectors that take the full SVD
   for (int64_t s : sectors_of(rq, cq)) {
     SectorSVD sc;
     sc.s = s;
     sc.rows = where_eq(rq, s);
     sc.cols = where_eq(cq, s);
     if (sc.rows.empty() || sc.cols.empty()) continue;
-    blocks.push_back(gather2d(c, M, sc.rows, sc.cols));
+    Tensor blk = gather2d(c, M, sc.rows, sc.cols);
+    SVD lead;
+    if (leading_only && leading_svd(c, blk, Dmax, &lead)) {
+      sc.U = lead.U;
+      sc.Vt = lead.Vt;
+      sc.S = lead.S;
+    } else {
+      blocks.push_back(blk);
+      full.push_back(secs.size());
+    }
     secs.push_back(std::move(sc));
   }
   std::vector<SVD> fs = svd_thin_batch(c, blocks);
   blocks.clear();
-  for (size_t i = 0; i < secs.size(); ++i) {
-    secs[i].U = fs[i].U;
-    secs[i].Vt = fs[i].Vt;
-    secs[i].S = fs[i].S;
+  for (size_t i = 0; i < full.size(); ++i) {
+    secs[full[i]].U = fs[i].U;
+    secs[full[i]].Vt = fs[i].Vt;
+    secs[full[i]].S = fs[i].S;
   }
   return assemble(c, M.dims[0], M.dims[1], secs, Dmax, reg, M.cplx);
 }
@@ -971,7 +1032,8 @@ void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,
   for (int64_t r = 0; r < nA * dpL; ++r) rq[r] = eR[r / nA];
   std::vector<int64_t> am = col_argmax(c, thm), cq(am.size());
   for (size_t i = 0; i < am.size(); ++i) cq[i] = rq[am[i]];
-  CSVD cs = charge_svd(c, thm, rq, cq, Dmax, GI_REG);
+  // only the leading D_max triplets of each sector enter the initialisation
+  CSVD cs = charge_svd(c, thm, rq, cq, Dmax, GI_REG, true);
   const int64_t Dp = int64_t(cs.S.size());
   std::vector<double> sq(Dp);
   for (int64_t i = 0; i < Dp; ++i) sq[i] = std::sqrt(cs.S[i]);

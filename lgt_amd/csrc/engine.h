@@ -70,6 +70,8 @@ void apply_mask(State& s, int ix, int iy);
 double forbidden_norm(State& s);
 Tensor sqrt_absorbed(State& s, int ix, int iy);
 Tensor dl_traced(Ctx& c, const Tensor& T);
+// _dl_open: (p,l,r,u,d) -> (p, p', l², r², u², d²), bra (') fast (kernels.hip)
+Tensor dl_open(Ctx& c, const Tensor& T);
 Tensor gate_tensor(Ctx& c, const Mat& G, bool cplx);
 
 // ── boundary MPS ─────────────────────────────────────────────────────────────
@@ -111,8 +113,11 @@ Env bond_environment_h(State& s, int ix, int iy, int chi);
 Env bond_environment_v(State& s, int ix, int iy, int chi);
 
 // ── truncations ──────────────────────────────────────────────────────────────
+// leading_only: sectors much larger than D_max get only their leading D_max
+// triplets (randomized subspace iteration, residual-checked, full SVD as
+// the fallback) -- all that the global top-D_max selection reads.
 CSVD charge_svd(Ctx& c, const Tensor& M, const std::vector<int64_t>& rq,
-                const std::vector<int64_t>& cq, int Dmax, double reg = GI_REG);
+                const std::vector<int64_t>& cq, int Dmax, double reg = GI_REG, bool leading_only = false);
 Tensor apply_gate(Ctx& c, const Tensor& th, const Tensor& G);
 double als_cost(Ctx& c, const Tensor& th, const Tensor& env, const Tensor& X, const Tensor& Y,
                 const Tensor* Eth = nullptr);

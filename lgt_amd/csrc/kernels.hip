@@ -2221,3 +2221,55 @@ void mlp_adam(Ctx& c, Tensor& p, const Tensor& g, Tensor& m, Tensor& v, double l
 }
 
 }  // namespace peps
+
+namespace peps {
+
+// _dl_open (finite_peps_full_update.jl:97-109): the double layer with the
+// physical indices left open, E[p1, p2, (l1 l2), (r1 r2), (u1 u2), (d1 d2)] =
+// T[p1, l1, r1, u1, d1] conj(T[p2, l2, r2, u2, d2]), the bra index (2) fast
+// in every merged pair.  One output element per thread (grid-stride): an
+// outer product, bound by the HBM write of E.
+template <bool CP>
+__global__ void dl_open_kernel(const typename JT<CP>::T* __restrict__ T, typename JT<CP>::T* __restrict__ E, int dp,
+                               int Dl, int Dr, int Du, int Dd, int64_t total) {
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < total;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    int64_t q = idx;
+    const int p1 = int(q % dp); q /= dp;
+    const int p2 = int(q % dp); q /= dp;
+    const int l2 = int(q % Dl); q /= Dl;
+    const int l1 = int(q % Dl); q /= Dl;
+    const int r2 = int(q % Dr); q /= Dr;
+    const int r1 = int(q % Dr); q /= Dr;
+    const int u2 = int(q % Du); q /= Du;
+    const int u1 = int(q % Du); q /= Du;
+    const int d2 = int(q % Dd); q /= Dd;
+    const int d1 = int(q);
+    const int64_t a = p1 + int64_t(dp) * (l1 + int64_t(Dl) * (r1 + int64_t(Dr) * (u1 + int64_t(Du) * d1)));
+    const int64_t b = p2 + int64_t(dp) * (l2 + int64_t(Dl) * (r2 + int64_t(Dr) * (u2 + int64_t(Du) * d2)));
+    E[idx] = mulc(T[a], conjv(T[b]));
+  }
+}
+
+Tensor dl_open(Ctx& c, const Tensor& T) {
+  PEPS_CHECK(T.rank() == 5, PS_ESHAPE, "dl_open: (p,l,r,u,d) tensor expected");
+  const int64_t dp = T.dims[0], Dl = T.dims[1], Dr = T.dims[2], Du = T.dims[3], Dd = T.dims[4];
+  Tensor E = c.alloc({dp, dp, Dl * Dl, Dr * Dr, Du * Du, Dd * Dd}, T.cplx);
+  const int64_t total = E.numel();
+  if (total == 0) return E;
+  hipEvent_t ev = nullptr;
+  c.tbegin("elementwise", &ev);
+  c.count("elementwise", 0, double(E.bytes()) + double(T.bytes()));
+  const unsigned blocks = unsigned(std::min<int64_t>(cdiv(total, 256), 4096));
+  if (T.cplx)
+    hipLaunchKernelGGL(dl_open_kernel<true>, dim3(blocks), dim3(256), 0, c.stream, (const double2*)T.data(),
+                       (double2*)E.data(), int(dp), int(Dl), int(Dr), int(Du), int(Dd), total);
+  else
+    hipLaunchKernelGGL(dl_open_kernel<false>, dim3(blocks), dim3(256), 0, c.stream, T.data(), E.data(), int(dp),
+                       int(Dl), int(Dr), int(Du), int(Dd), total);
+  PEPS_HIP(hipGetLastError());
+  c.tend("elementwise", ev);
+  return E;
+}
+
+}  // namespace peps

@@ -89,6 +89,7 @@ _SIGS = {
     "peps_batch_trotter_step": (C.c_int32, [C.POINTER(_vp), C.c_int32, C.c_int32, C.POINTER(Params), _i32p]),
     "peps_last_als_stats": (C.c_int32, [_vp, _i64p]),
     "peps_ite_ground_state": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params), C.c_int64, _i64p]),
+    "peps_dl_open": (C.c_int32, [_vp, C.c_int32, _i64p, _dp, _dp]),
     "peps_energy_v4": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
     "peps_su_bond_dense": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32]),
     "peps_fu_bond_v2": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32]),
@@ -581,6 +582,17 @@ def dl_traced(T, ctx=None):
     d = np.array(a.shape, dtype=np.int64)
     E = np.empty((d[1] ** 2, d[2] ** 2, d[3] ** 2, d[4] ** 2), dtype=a.dtype, order="F")
     ctx.check(lib().peps_dl_traced(ctx.h, int(cp), _iptr(d), _dptr(a), _dptr(E)))
+    return E
+
+
+def dl_open(T, ctx=None):
+    """_dl_open (finite_peps_full_update.jl:97-109): (p, p', l², r², u², d²)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(T)
+    a = _as_array(T, cp)
+    d = np.array(a.shape, dtype=np.int64)
+    E = np.empty((d[0], d[0], d[1] ** 2, d[2] ** 2, d[3] ** 2, d[4] ** 2), dtype=a.dtype, order="F")
+    ctx.check(lib().peps_dl_open(ctx.h, int(cp), _iptr(d), _dptr(a), _dptr(E)))
     return E
 
 

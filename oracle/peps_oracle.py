@@ -608,6 +608,14 @@ def dl_traced(T):
     return jr(E8, Dl * Dl, Dr * Dr, Du * Du, Dd * Dd)
 
 
+def dl_open(T):
+    """_dl_open, finite_peps_full_update.jl:97-109: E[p1,p2,(l1,l2),...] =
+    T[p1,l1,..] conj(T[p2,l2,..]) with l2 (bra) the fast sub-index."""
+    dp, Dl, Dr, Du, Dd = T.shape
+    E10 = np.einsum("pabcd,qefgh->pqeafbgchd", T, T.conj(), optimize=True)
+    return jr(E10, dp, dp, Dl * Dl, Dr * Dr, Du * Du, Dd * Dd)
+
+
 def dl_traced_site(gp, ix, iy):
     """finite_peps_boundary_mps.jl:56-57."""
     return dl_traced(sqrt_absorbed_site(gp, ix, iy))

@@ -61,3 +61,20 @@ def mps_dense(mps):
 def rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def fixture_arrays(z, prefix, nx, ny):
+    """Tensors, weights and labels stored by tests/golden/make_config_fixtures.py."""
+    tens = [[z[f"{prefix}T_{ix}_{iy}"] for iy in range(1, ny + 1)] for ix in range(1, nx + 1)]
+    lh = [[z[f"{prefix}lh_{ix}_{iy}"] for iy in range(1, ny + 1)] for ix in range(1, nx)]
+    qh = [[z[f"{prefix}qh_{ix}_{iy}"] for iy in range(1, ny + 1)] for ix in range(1, nx)]
+    lv = [[z[f"{prefix}lv_{ix}_{iy}"] for iy in range(1, ny)] for ix in range(1, nx + 1)]
+    qv = [[z[f"{prefix}qv_{ix}_{iy}"] for iy in range(1, ny)] for ix in range(1, nx + 1)]
+    return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv)
+
+
+def fixture_state(z, prefix, nx, ny, gd, ctx):
+    from lgt_amd import lattice, peps
+    a = fixture_arrays(z, prefix, nx, ny)
+    return peps.GaugePEPS.from_arrays(a["tensors"], a["lh"], a["lv"], a["qh"], a["qv"],
+                                      lattice.staggered_charges(nx, ny), gd, ctx=ctx)

@@ -286,3 +286,28 @@ def test_lattice_init_matches_oracle_init():
                 np.testing.assert_allclose(a.tensors[ix][iy], b["tensors"][ix][iy], rtol=0, atol=1e-15)
         assert all(np.array_equal(a.qh[i][j], b["qh"][i][j]) for i in range(nx - 1) for j in range(ny))
         np.testing.assert_array_equal(lattice.staggered_charges(nx, ny), g)
+
+
+def test_dl_open_matches_reference_loop_and_traces_to_dl_traced():
+    """_dl_open (finite_peps_full_update.jl:97-109): the reference's explicit
+    loop (il = (l1-1) Dl + l2 in 1-based form, i.e. the bra copy fast), and
+    its physical trace is _dl_traced."""
+    rng = np.random.default_rng(3)
+    T = rng.standard_normal((3, 2, 3, 1, 2)) + 1j * rng.standard_normal((3, 2, 3, 1, 2))
+    dp, Dl, Dr, Du, Dd = T.shape
+    E = np.zeros((dp, dp, Dl * Dl, Dr * Dr, Du * Du, Dd * Dd), complex)
+    for d2 in range(Dd):
+        for d1 in range(Dd):
+            for u2 in range(Du):
+                for u1 in range(Du):
+                    for r2 in range(Dr):
+                        for r1 in range(Dr):
+                            for l2 in range(Dl):
+                                for l1 in range(Dl):
+                                    il, ir, iu, idd = l1 * Dl + l2, r1 * Dr + r2, u1 * Du + u2, d1 * Dd + d2
+                                    for p1 in range(dp):
+                                        for p2 in range(dp):
+                                            E[p1, p2, il, ir, iu, idd] = T[p1, l1, r1, u1, d1] * np.conj(
+                                                T[p2, l2, r2, u2, d2])
+    np.testing.assert_allclose(O.dl_open(T), E, atol=1e-14)
+    np.testing.assert_allclose(np.einsum("ppabcd->abcd", O.dl_open(T)), O.dl_traced(T), atol=1e-13)
