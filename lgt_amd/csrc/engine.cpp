@@ -192,6 +192,16 @@ MPS absorb_row_from_bottom(Ctx& c, const MPS& mps, const MPS& row) {
   return out;
 }
 
+// PEPS_HIP_DEBUG_FINITE: raise at the first non-finite intermediate, naming it
+static void debug_finite(Ctx& c, const Tensor& t, const char* what, size_t site) {
+  if (!c.debug_finite) return;
+  bool finite, nonzero;
+  scan_values(c, t, &finite, &nonzero);
+  PEPS_CHECK(finite, PS_ESOLVER, std::string("compress_mps: non-finite ") + what + " at site " +
+                                     std::to_string(site + 1) + " (" + std::to_string(t.dims[0]) + "x" +
+                                     std::to_string(t.numel() / std::max<int64_t>(1, t.dims[0])) + ")");
+}
+
 void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff, bool relative) {
   // :70-105  left->right QR, then right->left truncated SVD
   const size_t n = mps.size();
@@ -199,14 +209,19 @@ void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff, bool relative) {
   for (size_t i = 0; i + 1 < n; ++i) {
     int64_t Dl = mps[i].dims[0], dp = mps[i].dims[1], Dr = mps[i].dims[2];
     QR f = qr_thin(c, mps[i].reshaped({Dl * dp, Dr}));
+    debug_finite(c, f.Q, "QR factor Q", i);
+    debug_finite(c, f.R, "QR factor R", i);
     int64_t k = f.Q.dims[1];
     mps[i] = f.Q.reshaped({Dl, dp, k});
     int64_t Dl2 = mps[i + 1].dims[0], dp2 = mps[i + 1].dims[1], Dr2 = mps[i + 1].dims[2];
     mps[i + 1] = matmul(c, f.R, mps[i + 1].reshaped({Dl2, dp2 * Dr2})).reshaped({k, dp2, Dr2});
+    debug_finite(c, mps[i + 1], "R*M product", i + 1);
   }
   for (size_t i = n - 1; i >= 1; --i) {
     int64_t Dl = mps[i].dims[0], dp = mps[i].dims[1], Dr = mps[i].dims[2];
     SVD f = svd_thin(c, mps[i].reshaped({Dl, dp * Dr}));
+    debug_finite(c, f.U, "SVD factor U", i);
+    debug_finite(c, f.Vt, "SVD factor Vt", i);
     int64_t cnt = 0;
     const double thr = relative && !f.S.empty() ? cutoff * f.S[0] : cutoff;
     for (double x : f.S) cnt += x > thr;
@@ -221,6 +236,7 @@ void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff, bool relative) {
     scale_cols(c, US, std::vector<double>(f.S.begin(), f.S.begin() + k));
     int64_t Dl2 = mps[i - 1].dims[0], dp2 = mps[i - 1].dims[1], Dr2 = mps[i - 1].dims[2];
     mps[i - 1] = matmul(c, mps[i - 1].reshaped({Dl2 * dp2, Dr2}), US).reshaped({Dl2, dp2, k});
+    debug_finite(c, mps[i - 1], "M*US product", i - 1);
   }
 }
 
@@ -830,12 +846,11 @@ namespace {
 // kept triplet satisfies ||B^H u_i - sigma_i v_i|| <= 1e-13 sigma_0 -- the
 // triplets of the full SVD to rounding -- else returns false and the caller
 // runs the full SVD.  Worth it only for blocks much larger than k.
-bool leading_svd(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
+bool leading_svd_try(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
   TierGuard fp64(c, TIER_FP64);
   const int64_t m = B.dims[0], n = B.dims[1], kmin = std::min(m, n);
   k = std::min(k, kmin);
   const int64_t p = std::min<int64_t>(kmin, k + 16);
-  if (k < 1 || kmin < 64 || 2 * p > kmin) return false;
   Tensor Om = c.alloc({n, p}, B.cplx);
   fill_random(c, Om, 0x5eedull + uint64_t(m) * 1315423911ull + uint64_t(n));
   Tensor Q = qr_thin(c, matmul(c, B, Om)).Q;  // m x p
@@ -859,8 +874,13 @@ bool leading_svd(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
     scale_cols(c, Vk, ms);
     axpy_inplace(c, Rz, Vk, 1.0);
     std::vector<double> r2 = col_norm2(c, Rz);
-    bool ok = true;
+    // the kept triplets are exact to rounding, and the (k+1)-th Ritz value
+    // is at most half the k-th: a clear gap, so the subspace iteration has
+    // found the leading k (a small residual alone would also accept another
+    // genuine triplet in place of a missed larger one)
+    bool ok = int64_t(f.S.size()) > k && f.S[k] <= 0.5 * f.S[k - 1];
     for (double x : r2) ok = ok && std::isfinite(x) && std::sqrt(x) <= 1e-13 * f.S[0];
+    ok = ok && all_finite(c, Uk) && all_finite(c, Vtk);
     c.count("svd_leading[iterations]", double(it + 1), 0);
     if (ok) {
       out->U = Uk;
@@ -871,6 +891,20 @@ bool leading_svd(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
   }
   c.count("svd_leading[fallback]", 1.0, 0);
   return false;
+}
+
+bool leading_svd(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
+  const int64_t kmin = std::min(B.dims[0], B.dims[1]);
+  const int64_t kk = std::min(k, kmin), p = std::min<int64_t>(kmin, kk + 16);
+  if (!c.leading_svd || kk < 1 || kmin < 64 || 2 * p > kmin) return false;
+  try {
+    return leading_svd_try(c, B, k, out);
+  } catch (const Error& e) {
+    // a solver failure inside the sketch: the full SVD decides
+    if (e.code != PS_ESOLVER) throw;
+    c.count("svd_leading[fallback]", 1.0, 0);
+    return false;
+  }
 }
 }  // namespace
 

@@ -1395,6 +1395,25 @@ void scan_values(Ctx& c, const Tensor& t, bool* finite, bool* nonzero) {
   *nonzero = h[1] != 0;
 }
 
+bool all_finite(Ctx& c, std::initializer_list<const Tensor*> ts) {
+  auto buf = std::make_shared<Buffer>();
+  buf->pool = c.pool.get();
+  buf->bytes = 8;
+  buf->ptr = c.pool->get(8);
+  PEPS_HIP(hipMemsetAsync(buf->ptr, 0, 8, c.stream));
+  for (const Tensor* t : ts) {
+    int64_t n = t->numel() * (t->cplx ? 2 : 1);
+    if (!n) continue;
+    unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 4096));
+    hipLaunchKernelGGL(scan_kernel, dim3(g), dim3(256), 0, c.stream, t->data(), n, (int*)buf->ptr);
+    PEPS_HIP(hipGetLastError());
+  }
+  int h[2] = {0, 0};
+  PEPS_HIP(hipMemcpyAsync(h, buf->ptr, 8, hipMemcpyDeviceToHost, c.stream));
+  PEPS_HIP(hipStreamSynchronize(c.stream));
+  return h[0] == 0;
+}
+
 // out = the first columns of the identity (m x k).
 __global__ void eye_kernel(double* out, int64_t m, int64_t k, int esz) {
   int64_t n = m * k;

@@ -1,5 +1,7 @@
-// Dense FP64 factorisations on the GPU: thin QR (rocSOLVER Householder
-// geqrf + orgqr/ungqr, the LAPACK semantics of Julia's `qr`), the
+// Dense FP64 factorisations on the GPU: thin QR (the project's one-workgroup
+// Householder panel kernel up to 64 columns; wider, rocSOLVER Householder
+// geqrf + orgqr/ungqr, the LAPACK semantics of Julia's `qr`, with complex
+// factors checked and refactored unblocked where the blocked form fails), the
 // Hermitian eigendecomposition (real n <= 1024: the project's one-launch
 // Householder tridiagonalisation and reflector kernels around a divide and
 // conquer tridiagonal solve, eigh.hip; complex: rocSOLVER heevd -- the
@@ -56,6 +58,48 @@ void check_int_range(int64_t m, int64_t n) {
              "factorisation: matrix too large");
 }
 
+// Q = the first k columns of the m-row factor W (column-major: a prefix).
+Tensor q_prefix(Ctx& c, const Tensor& W, int64_t m, int64_t k) {
+  if (W.dims[1] == k) return W.reshaped({m, k});
+  Tensor Q = c.alloc({m, k}, W.cplx);
+  if (Q.bytes()) PEPS_HIP(hipMemcpyAsync(Q.data(), W.data(), Q.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  return Q;
+}
+
+// Complex QR of a matrix wider than the panel kernel takes.  rocSOLVER's
+// blocked zgeqrf returns non-finite values in every trailing column once a
+// column is exactly zero when its reflector is formed (tau = 0), which
+// charge-structured matrices produce all the time, while the unblocked
+// zgeqr2 / zung2r factor the same matrices to rounding.  The blocked form runs
+// first; a non-finite R or Q repeats the factorisation of A unblocked.
+QR zqr_large(Ctx& c, const Tensor& A, bool want_q) {
+  const int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);
+  QR r;
+  for (int blocked = 1; blocked >= 0; --blocked) {
+    Tensor W = copy_tensor(c, A);
+    Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, true);
+    auto* w = (rocblas_double_complex*)W.data();
+    auto* t = (rocblas_double_complex*)tau.data();
+    if (k > 0) {
+      if (blocked) ck(rocsolver_zgeqrf(c.blas, int(m), int(n), w, int(m), t), "zgeqrf");
+      else ck(rocsolver_zgeqr2(c.blas, int(m), int(n), w, int(m), t), "zgeqr2");
+    }
+    r.R = upper_trapezoid(c, W, k);
+    if (want_q) {
+      if (k > 0) {
+        if (blocked) ck(rocsolver_zungqr(c.blas, int(m), int(k), int(k), w, int(m), t), "zungqr");
+        else ck(rocsolver_zung2r(c.blas, int(m), int(k), int(k), w, int(m), t), "zung2r");
+      }
+      r.Q = q_prefix(c, W, m, k);
+    }
+    if (want_q ? all_finite(c, {&r.R, &r.Q}) : all_finite(c, {&r.R})) return r;
+    if (blocked) c.count("qr_unblocked", 0.0, 0);
+  }
+  PEPS_CHECK(false, PS_ESOLVER, "qr: non-finite factors of a finite " + std::to_string(m) + "x" +
+                                    std::to_string(n) + " complex matrix");
+  return r;
+}
+
 }  // namespace
 
 QR qr_thin(Ctx& c, const Tensor& A) {
@@ -68,44 +112,27 @@ QR qr_thin(Ctx& c, const Tensor& A) {
     r.R = c.zeros({k, n}, A.cplx);
     return r;
   }
-  Tensor W = copy_tensor(c, A);
-  Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
   hipEvent_t ev = nullptr;
   const std::string cls = c.gemm_shapes ? "qr " + std::to_string(m) + "x" + std::to_string(n) : Ctx::bucket("qr", n);
   c.tbegin(cls, &ev);
   c.count(cls, (A.cplx ? 4.0 : 1.0) * (4.0 * m * n * k - 2.0 * (m + n) * k * k + 4.0 * k * k * k / 3.0), 0);
-  if (n <= qr_small_max_cols()) {  // narrow panel: one-workgroup Householder kernel
-    QR r;
-    if (k > 0) qr_panel(c, W, tau);
-    r.R = upper_trapezoid(c, W, k);
-    r.Q = c.alloc({m, k}, A.cplx);
-    if (k > 0) qr_form_q(c, W, tau, r.Q);
-    c.tend(cls, ev);
-    return r;
-  }
-  if (k > 0) {
-    if (A.cplx)
-      ck(rocsolver_zgeqrf(c.blas, int(m), int(n), (rocblas_double_complex*)W.data(), int(m),
-                          (rocblas_double_complex*)tau.data()), "zgeqrf");
-    else
-      ck(rocsolver_dgeqrf(c.blas, int(m), int(n), W.data(), int(m), tau.data()), "dgeqrf");
-  }
   QR r;
-  r.R = upper_trapezoid(c, W, k);
-  if (k > 0) {
-    if (A.cplx)
-      ck(rocsolver_zungqr(c.blas, int(m), int(k), int(k), (rocblas_double_complex*)W.data(), int(m),
-                          (rocblas_double_complex*)tau.data()), "zungqr");
-    else
-      ck(rocsolver_dorgqr(c.blas, int(m), int(k), int(k), W.data(), int(m), tau.data()), "dorgqr");
-  }
-  // Q = first k columns of W (column-major: a prefix of the buffer)
-  if (k == n) {
-    r.Q = W.reshaped({m, n});
+  if (n > qr_small_max_cols() && A.cplx) {
+    r = zqr_large(c, A, true);
   } else {
-    r.Q = c.alloc({m, k}, A.cplx);
-    if (r.Q.bytes())
-      PEPS_HIP(hipMemcpyAsync(r.Q.data(), W.data(), r.Q.bytes(), hipMemcpyDeviceToDevice, c.stream));
+    Tensor W = copy_tensor(c, A);
+    Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
+    if (n <= qr_small_max_cols()) {  // narrow panel: one-workgroup Householder kernel
+      if (k > 0) qr_panel(c, W, tau);
+      r.R = upper_trapezoid(c, W, k);
+      r.Q = c.alloc({m, k}, A.cplx);
+      if (k > 0) qr_form_q(c, W, tau, r.Q);
+    } else {
+      if (k > 0) ck(rocsolver_dgeqrf(c.blas, int(m), int(n), W.data(), int(m), tau.data()), "dgeqrf");
+      r.R = upper_trapezoid(c, W, k);
+      if (k > 0) ck(rocsolver_dorgqr(c.blas, int(m), int(k), int(k), W.data(), int(m), tau.data()), "dorgqr");
+      r.Q = q_prefix(c, W, m, k);
+    }
   }
   c.tend(cls, ev);
   return r;
@@ -116,26 +143,22 @@ Tensor qr_r_only(Ctx& c, const Tensor& A) {
   int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n);
   check_int_range(m, n);
   if (!check_input(c, A, "qr")) return c.zeros({k, n}, A.cplx);
-  Tensor W = copy_tensor(c, A);
-  Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
   hipEvent_t ev = nullptr;
   const std::string cls = c.gemm_shapes ? "qr " + std::to_string(m) + "x" + std::to_string(n) : Ctx::bucket("qr", n);
   c.tbegin(cls, &ev);
   c.count(cls, (A.cplx ? 4.0 : 1.0) * (2.0 * m * n * k - (m + n) * k * k + 2.0 * k * k * k / 3.0), 0);
-  if (n <= qr_small_max_cols()) {
-    if (k > 0) qr_panel(c, W, tau);
-    Tensor R = upper_trapezoid(c, W, k);
-    c.tend(cls, ev);
-    return R;
+  Tensor R;
+  if (n > qr_small_max_cols() && A.cplx) {
+    R = zqr_large(c, A, false).R;
+  } else {
+    Tensor W = copy_tensor(c, A);
+    Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
+    if (k > 0) {
+      if (n <= qr_small_max_cols()) qr_panel(c, W, tau);
+      else ck(rocsolver_dgeqrf(c.blas, int(m), int(n), W.data(), int(m), tau.data()), "dgeqrf");
+    }
+    R = upper_trapezoid(c, W, k);
   }
-  if (k > 0) {
-    if (A.cplx)
-      ck(rocsolver_zgeqrf(c.blas, int(m), int(n), (rocblas_double_complex*)W.data(), int(m),
-                          (rocblas_double_complex*)tau.data()), "zgeqrf");
-    else
-      ck(rocsolver_dgeqrf(c.blas, int(m), int(n), W.data(), int(m), tau.data()), "dgeqrf");
-  }
-  Tensor R = upper_trapezoid(c, W, k);
   c.tend(cls, ev);
   return R;
 }

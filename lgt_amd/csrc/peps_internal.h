@@ -17,6 +17,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <initializer_list>
 #include <vector>
 
 namespace peps {
@@ -151,6 +152,14 @@ class Ctx {
   // library's one-launch kernel, 1 rocSOLVER sytrd (environment variable
   // PEPS_HIP_TRD=rocsolver)
   int trd_backend = 0;
+  // leading-triplet (randomized, residual-checked) SVD of large charge
+  // sectors in the ALS initialisation (environment variable
+  // PEPS_HIP_LEADING_SVD=0 turns it off: every sector takes the full SVD)
+  bool leading_svd = true;
+  // check every intermediate of the boundary compression for non-finite
+  // values and name the first step that produced one (environment variable
+  // PEPS_HIP_DEBUG_FINITE=1; one device scan per factor)
+  bool debug_finite = false;
   int jacobi_max_sweeps = 80;
   // account GEMMs per exact shape ("gemm_f32 MxNxK ...") instead of per
   // precision (environment variable PEPS_HIP_GEMM_SHAPES=1)
@@ -269,6 +278,8 @@ void qr_panel(Ctx& c, Tensor& W, Tensor& tau);
 void qr_form_q(Ctx& c, const Tensor& W, const Tensor& tau, Tensor& Q);
 // One pass: are all entries finite, is any entry non-zero (synchronises).
 void scan_values(Ctx& c, const Tensor& t, bool* finite, bool* nonzero);
+// every entry of every tensor finite (one stream synchronisation)
+bool all_finite(Ctx& c, std::initializer_list<const Tensor*> ts);
 // The first k columns of the m x m identity.
 Tensor eye(Ctx& c, int64_t m, int64_t k, bool cplx);
 // Small-MLP pieces (real, batch in columns): Z += b per row (+ ReLU);

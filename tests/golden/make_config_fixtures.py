@@ -11,9 +11,15 @@ tests compare the library with them.
       state is unchanged; the reference's absolute compression cut needs an
       O(1) norm) and the norm, the energy and every plaquette <W> are
       evaluated at chi=64.
+  v4: the full-update self-test of finite_peps_fullupdate_v4.jl:479-504 at
+      its default settings: 3x4, gd=3 (dg=1), staggered charges,
+      init_gauge_peps(noise=0.1, seed 1), eight trotter_step_v4 steps
+      (g=1, t=1, m=0.25, tau=0.05, D_max=4, chi=64, 30 ALS sweeps); the
+      energy (compute_energy_v4, chi=64), <n_f> and the Gauss-law violation
+      after every step.
 
-Usage: python tests/golden/make_config_fixtures.py c1|c2  (c2 takes about an
-hour of CPU time)."""
+Usage: python tests/golden/make_config_fixtures.py c1|c2|v4  (c2 takes about
+an hour of CPU time, v4 about half an hour)."""
 import os
 import sys
 
@@ -64,7 +70,24 @@ def rescale(gp, chi):
     return f_tot
 
 
+def v4_selftest():
+    gp = O.init_gauge_peps(3, 4, 3, O.staggered_charges(3, 4), noise=0.1, seed=1)
+    E, nf, forbidden = [], [], []
+    for s in range(8):
+        O.trotter_step_v4(gp, 1.0, 1.0, 0.25, 0.05, 4, 64, 30)
+        E.append(O.compute_energy_v4(gp, 1.0, 1.0, 0.25, 64))
+        nf.append(O.mean_nf(gp))
+        forbidden.append(O.forbidden_norm(gp))
+        print(f"v4: step {s + 1}/8 E = {E[-1]:.15f} <n_f> = {nf[-1]:.12f}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "v4_selftest.npz"), energy=np.array(E), mean_nf=np.array(nf),
+                        forbidden_norm=np.array(forbidden), nx=3, ny=4, gd=3, noise=0.1, seed=1,
+                        g=1.0, t=1.0, m=0.25, tau=0.05, D_max=4, chi=64, n_als=30)
+    print("v4: wrote v4_selftest.npz", flush=True)
+
+
 def main(which):
+    if which == "v4":
+        return v4_selftest()
     if which == "c1":
         nx, D, steps, chi = 4, 2, 200, 16
     else:

@@ -154,18 +154,34 @@ def test_c5_full_update_step_properties(ctx):
 
 
 def test_v4_update_selftest_reference_settings(ctx):
-    """v4_update_selftest (finite_peps_fullupdate_v4.jl:479-504) with the
-    reference's settings: 3x4, dg=1 (gd=3), m=0.25, D_max=4, chi=64, 30 ALS
-    sweeps, 8 full-update steps at tau=0.05 from init_gauge_peps(noise=0.1):
-    Gauss law (forbidden_norm < 1e-9), half filling (|<n_f> - 0.5| < 1e-6)
-    and E_last <= E_first + 1e-6."""
+    """v4_update_selftest (finite_peps_fullupdate_v4.jl:479-504) at its
+    default settings: 3x4, dg=1 (gd=3), m=0.25, D_max=4, chi=64, 30 ALS
+    sweeps, 8 full-update steps at tau=0.05 from init_gauge_peps(noise=0.1,
+    seed 1).  The reference's checks -- Gauss law (forbidden_norm < 1e-9) and
+    E_last <= E_first + 1e-6 -- plus the energy after every step against the
+    oracle's run from the same state (tests/golden/v4_selftest.npz): the
+    first step to 1e-10, the later ones to 1e-4 relative (rounding differences
+    of the two FP64 runs grow through the ALS sweeps; measured <= 1e-5).
+    The reference's third check, |<n_f> - 0.5| < 1e-6, holds for its own
+    random draw only: the lambda-weighted local density
+    (finite_peps_ground_state.jl:550-580) is not conserved by the update, and
+    the oracle's run from seed 1 drifts to 5e-5 of half filling, so <n_f> is
+    held to the oracle's values to 1e-5 and to half filling to 1e-4."""
     from lgt_amd import lattice, peps
+    gold = _load("v4_selftest.npz")
     gp = peps.init_gauge_peps(3, 4, 3, lattice.staggered_charges(3, 4), noise=0.1, seed=1, ctx=ctx)
-    Es = []
+    Es, nfs = [], []
     for _ in range(8):
         peps.trotter_step_v4(gp, 1.0, 1.0, 0.25, 0.05, 4, 64, 30)
         Es.append(peps.compute_energy_v4(gp, 1.0, 1.0, 0.25, 64))
-    print(f"v4 self-test energies: {Es}, <n_f> = {peps.mean_nf(gp)}")
+        nfs.append(peps.mean_nf(gp))
+    print(f"v4 self-test energies: {Es}, <n_f>: {nfs}")
     assert peps.forbidden_norm(gp) < 1e-9
-    assert abs(peps.mean_nf(gp) - 0.5) < 1e-6
     assert Es[-1] <= Es[0] + 1e-6
+    E_ref = gold["energy"]
+    assert abs(Es[0] - E_ref[0]) < 1e-10 * abs(E_ref[0])
+    for e, r in zip(Es, E_ref):
+        assert abs(e - r) < 1e-4 * abs(r)
+    for n, r in zip(nfs, gold["mean_nf"]):
+        assert abs(n - r) < 1e-5
+        assert abs(n - 0.5) < 1e-4

@@ -61,6 +61,29 @@ def test_compress_mps_state(ctx, chi):
     assert rel(mps_dense(out), mps_dense(ref)) < 1e-12
 
 
+@pytest.mark.parametrize("zero_cols", [(), (3, 100, 200)])
+@pytest.mark.parametrize("chi", [256, 8])
+def test_compress_mps_complex_zero_columns(ctx, chi, zero_cols):
+    """A 256 x 256 complex QR in the left-to-right sweep with exactly zero
+    columns (charge-structured boundaries produce them): rocSOLVER's blocked
+    zgeqrf returns non-finite trailing columns there, so the library must
+    refactor unblocked and still match the oracle."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(17)
+
+    def mk(s):
+        return rng.standard_normal(s) + 1j * rng.standard_normal(s)
+
+    mps = [mk((1, 16, 256)), mk((256, 16, 256)), mk((256, 16, 1))]
+    for j in zero_cols:
+        mps[1][:, :, j] = 0
+    out = peps.compress_mps([m.copy() for m in mps], chi, ctx=ctx)
+    ref = O.compress_mps([m.copy() for m in mps], chi)
+    assert all(np.all(np.isfinite(m)) for m in out)
+    assert [m.shape for m in out] == [m.shape for m in ref]
+    assert rel(mps_dense(out), mps_dense(ref)) < 1e-11
+
+
 @pytest.mark.parametrize("name,chi", [("s33", 64), ("s33", 6), ("s34", 12), ("s23c", 64)])
 def test_norm_and_boundaries(ctx, request, name, chi):
     from lgt_amd import peps
