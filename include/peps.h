@@ -318,6 +318,24 @@ int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64
                   const double* A, int32_t a_kfast, int32_t conjA, const double* B,
                   int32_t b_kfast, int32_t conjB, double* Cout);
 
+/* peps_gemm for callers that cannot allocate device memory: code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc.  No
+   counterpart in the reference.  A, B and Cout are DEVICE addresses on the
+   context's device, owned by the caller (checked against their allocations:
+   a host pointer or a short buffer returns PEPS_EARG); same layouts and
+   result as peps_gemm.  The call only enqueues kernels on the context's
+   stream (peps_sync waits) and allocates nothing: the split-K partial
+   products go to a fixed 64 MiB array that the library holds on each device
+   from the context's creation until the process ends (the split count is
+   lowered until they fit, which can change the rounding of the sum relative
+   to peps_gemm), and the FP32 tier converts operands inside the GEMM kernel
+   instead of casting them into buffers first.  Calls that overlap on one
+   device share that array and race: serialise them (one stream, or events).
+   Extents must stay below 2^31 (PEPS_EUNSUPPORTED otherwise). */
+int32_t peps_gemm_noalloc(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,
+                          const double* A, int32_t a_kfast, int32_t conjA, const double* B,
+                          int32_t b_kfast, int32_t conjB, double* Cout);
+
 /* The engine's tensor permutation (no counterpart in the reference; exposed
    for validation): out.dims[i] = dims[perm[i]], 0-based perm. */
 int32_t peps_permute(peps_ctx* ctx, int32_t is_complex, int32_t rank, const int64_t* dims,

@@ -844,6 +844,41 @@ int32_t peps_gemm(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64
   });
 }
 
+// [ptr, ptr + bytes) inside one device allocation of the context's device
+static void check_device_span(const peps::Ctx& c, const void* ptr, int64_t bytes, const char* what) {
+  hipPointerAttribute_t at;
+  PEPS_CHECK(hipPointerGetAttributes(&at, ptr) == hipSuccess && at.type == hipMemoryTypeDevice &&
+                 at.device == c.device,
+             peps::PS_EARG, std::string("gemm_noalloc: ") + what + " is not device memory of the context's device");
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  PEPS_CHECK(hipMemGetAddressRange(&base, &size, const_cast<void*>(ptr)) == hipSuccess, peps::PS_EARG,
+             std::string("gemm_noalloc: no allocation holds ") + what);
+  const char* b = static_cast<const char*>(base);
+  const char* p = static_cast<const char*>(ptr);
+  PEPS_CHECK(p >= b && uint64_t(bytes) <= size - size_t(p - b), peps::PS_EARG,
+             std::string("gemm_noalloc: ") + what + " extends past the end of its allocation");
+}
+
+int32_t peps_gemm_noalloc(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, int64_t K,
+                          const double* A, int32_t a_kfast, int32_t conjA, const double* B,
+                          int32_t b_kfast, int32_t conjB, double* Cout) {
+  if (!ctx || !A || !B || !Cout || M < 0 || N < 0 || K < 0) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    PEPS_CHECK(M < (int64_t(1) << 31) && N < (int64_t(1) << 31) && K < (int64_t(1) << 31), peps::PS_EUNSUPPORTED,
+               "gemm_noalloc: extents must stay below 2^31");
+    const int64_t esz = is_complex ? 16 : 8;
+    if (M * N == 0) return;
+    check_device_span(*ctx->c, Cout, M * N * esz, "C");
+    if (K > 0) {
+      check_device_span(*ctx->c, A, M * K * esz, "A");
+      check_device_span(*ctx->c, B, K * N * esz, "B");
+    }
+    peps::gemm_noalloc(*ctx->c, is_complex != 0, M, N, K, A, a_kfast != 0, conjA != 0, B, b_kfast != 0,
+                       conjB != 0, Cout);
+  });
+}
+
 int32_t peps_permute(peps_ctx* ctx, int32_t is_complex, int32_t rank, const int64_t* dims,
                      const int32_t* perm, const double* in, double* out) {
   if (!ctx || !dims || !perm || !in || !out || rank < 1 || rank > 16) return PEPS_EARG;

@@ -76,6 +76,7 @@ Tensor Tensor::slice_last(int64_t i0, int64_t i1) const {
 
 Ctx::Ctx(int dev) : device(dev) {
   PEPS_HIP(hipSetDevice(dev));
+  scratch_attach(dev);  // the device address of the no-allocation GEMM's fixed workspace
   PEPS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   if (rocblas_create_handle(&blas) != rocblas_status_success)
     throw Error(PS_EHIP, "rocblas_create_handle failed");

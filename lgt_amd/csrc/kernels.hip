@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace peps {
 
@@ -496,15 +497,42 @@ static void dispatch_tile(bool big, bool ak, bool bk, hipStream_t st, int64_t M,
     dispatch_layout<CPLX, PREC, 2, 2>(ak, bk, st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
 }
 
-void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, bool a_kfast,
-          bool conjA, const double* B, bool b_kfast, bool conjB, double* C, double beta) {
+__global__ void zero_kernel(double* __restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = 0.0;
+}
+
+namespace {
+// Where the split-K partial products of one GEMM go: ws_fixed == nullptr
+// takes them from the context's pool (at most ws_cap bytes), otherwise they
+// are written to ws_fixed, which holds ws_cap bytes.  cast allows the FP32
+// tier's one-time operand cast (two pool buffers); timed brackets the
+// kernels with the context's timing events.
+struct GemmWorkspace {
+  double* ws_fixed = nullptr;
+  double ws_cap = double(1ull << 30);
+  bool cast = true;
+  bool timed = true;
+};
+
+void gemm_impl(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, bool a_kfast,
+               bool conjA, const double* B, bool b_kfast, bool conjB, double* C, double beta,
+               const GemmWorkspace& wsp) {
   if (M <= 0 || N <= 0) return;
   const int64_t lda = a_kfast ? K : M;
   const int64_t ldb = b_kfast ? K : N;
   const int esz = cplx ? 2 : 1;
   if (K <= 0) {
-    if (beta == 0.0) PEPS_HIP(hipMemsetAsync(C, 0, size_t(M * N) * esz * 8, c.stream));
-    else if (beta != 1.0) {
+    if (beta == 0.0) {
+      if (wsp.ws_fixed) {  // a kernel, not a memset node, for callers under graph capture
+        const int64_t n = M * N * esz;
+        hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min<int64_t>(cdiv(n, 256), 8192))), dim3(256), 0,
+                           c.stream, C, n);
+        PEPS_HIP(hipGetLastError());
+      } else {
+        PEPS_HIP(hipMemsetAsync(C, 0, size_t(M * N) * esz * 8, c.stream));
+      }
+    } else if (beta != 1.0) {
       Tensor t;  // scale existing C
       t.buf = std::make_shared<Buffer>();
       t.buf->ptr = C;
@@ -523,8 +551,8 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
   if (tiles < target && K >= 1024) {
     splits = int(std::min<int64_t>((target + tiles - 1) / tiles, K / 256));
     splits = std::max(1, std::min(splits, 512));
-    // bound the workspace to 1 GiB
-    while (splits > 1 && double(splits) * M * N * esz * 8 > double(1ull << 30)) splits /= 2;
+    // bound the workspace
+    while (splits > 1 && double(splits) * M * N * esz * 8 > wsp.ws_cap) splits /= 2;
   }
   int64_t kchunk = K;
   if (splits > 1) {
@@ -535,14 +563,17 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
   const double bytes = double(esz * 8) * (double(M) * K + double(K) * N + double(M) * N);
   const bool tiled = !cplx && c.tier == TIER_FP32 && M >= 128 && N >= 128 &&
                      int64_t(cdiv(M, 128)) * cdiv(N, 128) < (int64_t(1) << 31);
-  std::string cls = tiled ? "gemm_f32_tiled"
-                          : (c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16"));
-  if (c.gemm_shapes)
-    cls += " " + std::to_string(M) + "x" + std::to_string(N) + "x" + std::to_string(K) + (cplx ? "c" : "") +
-           (a_kfast ? " ak" : " am") + (b_kfast ? " bk" : " bn") + " s" + std::to_string(splits);
-  const bool cast = tiled && M >= kCastMin && N >= kCastMin;
-  // algorithmic bytes of the GEMM kernel itself: FP32 operands when cast
-  c.count(cls, flops, cast ? 4.0 * (double(M) * K + double(K) * N) + 8.0 * double(M) * N : bytes);
+  std::string cls;
+  const bool cast = wsp.cast && tiled && M >= kCastMin && N >= kCastMin;
+  if (wsp.timed) {
+    cls = tiled ? "gemm_f32_tiled"
+                : (c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16"));
+    if (c.gemm_shapes)
+      cls += " " + std::to_string(M) + "x" + std::to_string(N) + "x" + std::to_string(K) + (cplx ? "c" : "") +
+             (a_kfast ? " ak" : " am") + (b_kfast ? " bk" : " bn") + " s" + std::to_string(splits);
+    // algorithmic bytes of the GEMM kernel itself: FP32 operands when cast
+    c.count(cls, flops, cast ? 4.0 * (double(M) * K + double(K) * N) + 8.0 * double(M) * N : bytes);
+  }
   std::shared_ptr<Buffer> a32, b32;
   if (cast) {
     // A and B are packed (lda = M or K, ldb = K or N): cast M*K and K*N
@@ -564,18 +595,24 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
     c.tend("cast_f32", evc);
   }
   hipEvent_t ev = nullptr;
-  c.tbegin(cls, &ev);
+  if (wsp.timed) c.tbegin(cls, &ev);
   std::shared_ptr<Buffer> ws;
   double* out = C;
   int64_t wstride = 0;
   double kbeta = beta;
   if (splits > 1) {
     wstride = M * N * esz;
-    ws = std::make_shared<Buffer>();
-    ws->pool = c.pool.get();
-    ws->bytes = size_t(splits) * wstride * 8;
-    ws->ptr = c.pool->get(ws->bytes);
-    out = static_cast<double*>(ws->ptr);
+    const size_t need = size_t(splits) * wstride * 8;
+    if (wsp.ws_fixed) {
+      PEPS_CHECK(double(need) <= wsp.ws_cap, PS_EUNSUPPORTED, "gemm: split-K workspace exceeds the fixed buffer");
+      out = wsp.ws_fixed;
+    } else {
+      ws = std::make_shared<Buffer>();
+      ws->pool = c.pool.get();
+      ws->bytes = need;
+      ws->ptr = c.pool->get(ws->bytes);
+      out = static_cast<double*>(ws->ptr);
+    }
     kbeta = 0.0;
   }
   auto go = [&](auto cplx_tag, auto prec_tag) {
@@ -603,17 +640,68 @@ void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, b
     else go(std::false_type{}, T2{});
   }
   PEPS_HIP(hipGetLastError());
-  c.tend(cls, ev);  // the GEMM kernel alone (the split-K reduce is its own class)
+  if (wsp.timed) c.tend(cls, ev);  // the GEMM kernel alone (the split-K reduce is its own class)
   if (splits > 1) {
     int64_t n = M * N * esz;
     unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 8192));
     hipEvent_t ev2 = nullptr;
-    c.tbegin("splitk_reduce", &ev2);
-    c.count("splitk_reduce", 0.0, double(n) * 8.0 * (splits + 2));
+    if (wsp.timed) {
+      c.tbegin("splitk_reduce", &ev2);
+      c.count("splitk_reduce", 0.0, double(n) * 8.0 * (splits + 2));
+    }
     hipLaunchKernelGGL(splitk_reduce, dim3(g), dim3(256), 0, c.stream, C, out, n, splits, wstride, beta);
     PEPS_HIP(hipGetLastError());
-    c.tend("splitk_reduce", ev2);
+    if (wsp.timed) c.tend("splitk_reduce", ev2);
   }
+}
+}  // namespace
+
+void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, bool a_kfast,
+          bool conjA, const double* B, bool b_kfast, bool conjB, double* C, double beta) {
+  gemm_impl(c, cplx, M, N, K, A, a_kfast, conjA, B, b_kfast, conjB, C, beta, GemmWorkspace{});
+}
+
+// ── the GEMM without allocation ─────────────────────────────────────────────
+//
+// The split-K partial products go to `scratch`, one array per device that the
+// library's code object holds from its first use on that device until the
+// process ends.  Host code never passes the array's name to a kernel (on the
+// host it is a stand-in, not a device address): scratch_attach, called once
+// per device by the context's constructor with that device current, looks up
+// the device address with hipGetSymbolAddress and keeps it per device.
+// Calls that overlap on one device share the array and race.
+static __device__ __align__(256) double scratch[kGemmScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+void scratch_attach(int device) {
+  PEPS_CHECK(device >= 0 && device < kMaxDevices, PS_EARG, "scratch_attach: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  PEPS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  PEPS_CHECK(p != nullptr, PS_EHIP, "scratch_attach: no device address for the scratch array");
+  g_scratch[device] = static_cast<double*>(p);
+}
+
+void gemm_noalloc(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, bool a_kfast,
+                  bool conjA, const double* B, bool b_kfast, bool conjB, double* C) {
+  double* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ws = c.device >= 0 && c.device < kMaxDevices ? g_scratch[c.device] : nullptr;
+  }
+  PEPS_CHECK(ws != nullptr, PS_EHIP, "gemm_noalloc: the device's scratch array is not attached");
+  GemmWorkspace wsp;
+  wsp.ws_fixed = ws;
+  wsp.ws_cap = double(kGemmScratchBytes);
+  wsp.cast = false;
+  wsp.timed = false;
+  gemm_impl(c, cplx, M, N, K, A, a_kfast, conjA, B, b_kfast, conjB, C, 0.0, wsp);
 }
 
 // ════════════════════════════════════════════════════════════════════════════

@@ -224,6 +224,15 @@ Tensor permute(Ctx& c, const Tensor& in, const std::vector<int>& perm);
 //   B is (K x N) if b_kfast (B[k + n*K]) or stored (N x K) if !b_kfast (B[n + k*N])
 void gemm(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, bool a_kfast,
           bool conjA, const double* B, bool b_kfast, bool conjB, double* C, double beta);
+// The same product with no allocation: split-K partial products in the
+// library's fixed per-device array of kGemmScratchBytes (the split count is
+// lowered until they fit), no FP32 operand cast, no timing events; C =
+// op(A) op(B).  scratch_attach(device) must have run on the device (the
+// context's constructor does it).
+constexpr size_t kGemmScratchBytes = size_t(64) << 20;
+void scratch_attach(int device);
+void gemm_noalloc(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double* A, bool a_kfast,
+                  bool conjA, const double* B, bool b_kfast, bool conjB, double* C);
 void scale_legs(Ctx& c, Tensor& T, const std::vector<double>* w /*[4]: l,r,u,d*/);
 void scale(Ctx& c, Tensor& T, double s);
 void axpy_inplace(Ctx& c, Tensor& Y, const Tensor& X, double a);  // Y += a X

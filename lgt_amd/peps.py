@@ -112,6 +112,8 @@ _SIGS = {
     "peps_mlp_gradients": (C.c_int32, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
     "peps_mlp_train_step": (C.c_int32, [_vp, C.c_int64, _dp, _dp, C.c_double, _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
+    "peps_gemm_noalloc": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _vp, C.c_int32, C.c_int32,
+                                      _vp, C.c_int32, C.c_int32, _vp]),
     "peps_permute": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p, _i32p, _dp, _dp]),
     "peps_gemm_acc": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp,
                                   C.c_int32, C.c_int32, C.c_double, _dp]),
@@ -849,6 +851,17 @@ def gemm(A, B, a_kfast=False, conjA=False, b_kfast=True, conjB=False, ctx=None):
     ctx.check(lib().peps_gemm(ctx.h, int(cp), M, N, K, _dptr(a), int(a_kfast), int(conjA), _dptr(b),
                               int(b_kfast), int(conjB), _dptr(out)))
     return out
+
+
+def gemm_noalloc(M, N, K, a_ptr, b_ptr, c_ptr, is_complex=False, a_kfast=False, conjA=False, b_kfast=True,
+                 conjB=False, ctx=None):
+    """peps_gemm_noalloc: C = op(A) op(B) on DEVICE buffers the caller owns
+    (integer device addresses, column-major, the layouts of gemm), enqueued on
+    the context's stream without any allocation; ctx.sync() waits for it."""
+    ctx = ctx or default_context()
+    ctx.check(lib().peps_gemm_noalloc(ctx.h, int(bool(is_complex)), int(M), int(N), int(K), C.c_void_p(a_ptr),
+                                      int(a_kfast), int(conjA), C.c_void_p(b_ptr), int(b_kfast), int(conjB),
+                                      C.c_void_p(c_ptr)))
 
 
 def permute(T, perm, ctx=None):
