@@ -447,6 +447,9 @@ def main():
     for name, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"]):
         log(f"  {name:24s} launches={v['launches']:8d} ms={v['ms']:10.1f} flop={v['flops']:.3e} "
             f"bytes={v['bytes']:.3e}")
+    for name, v in sorted(stats.items()):
+        if v["ms"] <= 0:  # counters without kernel time (solve ranks, fallbacks, bond sizes)
+            log(f"  counter {name:24s} events={v['launches']:8d} value={v['flops']:.6g} aux={v['bytes']:.6g}")
     ranked = sorted(kern.items(), key=lambda kv: -kv[1]["ms"])
     dominant = ranked[0] if ranked else ("none", dict(flops=0.0, bytes=0.0, ms=0.0, launches=0))
     gcls = GEMM_CLASS[a.tier]

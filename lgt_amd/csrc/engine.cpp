@@ -847,6 +847,15 @@ namespace {
 // triplets of the full SVD to rounding -- else returns false and the caller
 // runs the full SVD.  Worth it only for blocks much larger than k.
 bool leading_svd_try(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
+  // Randomized subspace iteration (Halko, Martinsson & Tropp 2011, alg. 4.4)
+  // on a (k + 16)-column sketch; from the second iteration on, the leading k
+  // Ritz triplets of Q^H B are accepted once they are certified to be the
+  // leading k singular triplets of B:
+  //  * each kept pair has a rounding-level residual ||B v_i - s_i u_i||
+  //    (the other residual, B^H u_i - s_i v_i, vanishes by construction);
+  //  * with R = B - Q Q^H B, every singular value of B is within ||R||_2 <=
+  //    ||R||_F of one of Q Q^H B (Weyl), so s_{k+1}(B) <= s_{k+1}^Ritz +
+  //    ||R||_F: below the k-th kept value, no larger triplet was missed.
   TierGuard fp64(c, TIER_FP64);
   const int64_t m = B.dims[0], n = B.dims[1], kmin = std::min(m, n);
   k = std::min(k, kmin);
@@ -854,7 +863,7 @@ bool leading_svd_try(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
   Tensor Om = c.alloc({n, p}, B.cplx);
   fill_random(c, Om, 0x5eedull + uint64_t(m) * 1315423911ull + uint64_t(n));
   Tensor Q = qr_thin(c, matmul(c, B, Om)).Q;  // m x p
-  constexpr int kMaxIt = 6;
+  constexpr int kMaxIt = 10;
   for (int it = 0; it < kMaxIt; ++it) {
     Tensor Qz = qr_thin(c, matmul(c, B, Q, true, false, true, false)).Q;  // n x p, span B^H Q
     Q = qr_thin(c, matmul(c, B, Qz)).Q;                                    // m x p, span B B^H Q
@@ -862,27 +871,27 @@ bool leading_svd_try(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
     Tensor Bt = matmul(c, Q, B, true, false, true, false);  // p x n
     SVD f = svd_thin(c, Bt);
     if (f.S.empty() || !(f.S[0] > 0.0)) return false;
-    Tensor Uk = matmul(c, Q, f.U.slice_last(0, k));                  // m x k
-    Tensor Vtk = gather2d(c, f.Vt, iota_v(k), iota_v(n));            // k x n
-    // residuals B^H u_i - sigma_i v_i (n x k)
-    Tensor Rz = matmul(c, B, Uk, true, false, true, false);
-    Tensor Vk = permute(c, Vtk, {1, 0});
-    if (Vk.data() == Vtk.data()) Vk = copy_tensor(c, Vk);
-    if (B.cplx) conj_inplace(c, Vk);
+    if (int64_t(f.S.size()) <= k) return false;
+    Tensor Uk = matmul(c, Q, f.U.slice_last(0, k));        // m x k
+    Tensor Vtk = gather2d(c, f.Vt, iota_v(k), iota_v(n));  // k x n
+    // residuals B v_i - s_i u_i (m x k)
+    Tensor Rv = matmul(c, B, Vtk, false, true, false, true);  // B V_k, V_k = Vtk^H
+    Tensor Us = copy_tensor(c, Uk);
     std::vector<double> ms(k);
     for (int64_t i = 0; i < k; ++i) ms[i] = -f.S[i];
-    scale_cols(c, Vk, ms);
-    axpy_inplace(c, Rz, Vk, 1.0);
-    std::vector<double> r2 = col_norm2(c, Rz);
-    // the kept triplets are exact to rounding, and the (k+1)-th Ritz value
-    // is at most half the k-th: a clear gap, so the subspace iteration has
-    // found the leading k (a small residual alone would also accept another
-    // genuine triplet in place of a missed larger one)
-    bool ok = int64_t(f.S.size()) > k && f.S[k] <= 0.5 * f.S[k - 1];
-    for (double x : r2) ok = ok && std::isfinite(x) && std::sqrt(x) <= 1e-13 * f.S[0];
-    ok = ok && all_finite(c, Uk) && all_finite(c, Vtk);
-    c.count("svd_leading[iterations]", double(it + 1), 0);
+    scale_cols(c, Us, ms);
+    axpy_inplace(c, Rv, Us, 1.0);
+    std::vector<double> r2 = col_norm2(c, Rv);
+    // ||B - Q Q^H B||_F, formed explicitly (no cancellation against ||B||_F)
+    Tensor R = copy_tensor(c, B);
+    axpy_inplace(c, R, matmul(c, Q, Bt), -1.0);
+    const double rf = std::sqrt(std::max(0.0, dot(c, R, R, true).re));
+    const double tol = 1e-13 * f.S[0];
+    bool ok = f.S[k] + rf + 2.0 * tol < f.S[k - 1];
+    for (double x : r2) ok = ok && std::isfinite(x) && std::sqrt(x) <= tol;
+    ok = ok && std::isfinite(rf) && all_finite(c, {&Uk, &Vtk});
     if (ok) {
+      c.count("svd_leading[iterations]", double(it + 1), 0);
       out->U = Uk;
       out->Vt = Vtk;
       out->S.assign(f.S.begin(), f.S.begin() + k);
