@@ -357,6 +357,15 @@ int32_t peps_gemm_acc(peps_ctx* ctx, int32_t is_complex, int64_t M, int64_t N, i
 int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, const double* A,
                   double* V, double* w);
 
+/* _solve_herm (finite_peps_fullupdate_v4.jl:37-45): X = V diag(inv) V^H S for
+   the Hermitian part of N (n x n) and the nrhs columns of S, inv_i = 1/w_i
+   where w_i > reg * max(w_max, 1e-30) and 0 otherwise (the ALS solve).  Real
+   matrices whose couplings between groups of rows total at most eps ||N||_F
+   (the gauge sectors of the ALS normal equations) are solved block by block
+   with the floor taken against the largest eigenvalue of all blocks. */
+int32_t peps_solve_herm(peps_ctx* ctx, int32_t is_complex, int64_t n, int64_t nrhs, const double* N,
+                        const double* S, double reg, double* X);
+
 /* Thin SVD used by compress_mps!/charge_svd (LinearAlgebra.svd in the
    reference): QR reduction + one-sided Jacobi kernel.  A (m x n); outputs
    U (m x k), S (k, descending), Vt (k x n), k = min(m, n). */

@@ -926,6 +926,17 @@ int32_t peps_eigh(peps_ctx* ctx, int32_t is_complex, int32_t n, int32_t nb, cons
   });
 }
 
+int32_t peps_solve_herm(peps_ctx* ctx, int32_t is_complex, int64_t n, int64_t nrhs, const double* N,
+                        const double* S, double reg, double* X) {
+  if (!ctx || !N || !S || !X || n < 1 || nrhs < 1 || !(reg >= 0.0)) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    const bool cp = is_complex != 0;
+    Tensor a = ctx->c->upload({n, n}, cp, N);
+    Tensor b = ctx->c->upload({n, nrhs}, cp, S);
+    ctx->c->download(peps::solve_herm(*ctx->c, a, b, reg), X);
+  });
+}
+
 int32_t peps_svd(peps_ctx* ctx, int32_t is_complex, int64_t m, int64_t n, const double* A,
                  double* U, double* S, double* Vt) {
   if (!ctx || !A || !U || !S || !Vt || m < 1 || n < 1) return PEPS_EARG;

@@ -88,6 +88,7 @@ Ctx::Ctx(int dev) : device(dev) {
   if (const char* v = std::getenv("PEPS_HIP_TRD")) trd_backend = std::string(v) == "rocsolver" ? 1 : 0;
   if (const char* v = std::getenv("PEPS_HIP_LEADING_SVD")) leading_svd = std::atoi(v) != 0;
   if (const char* v = std::getenv("PEPS_HIP_DEBUG_FINITE")) debug_finite = std::atoi(v) != 0;
+  if (const char* v = std::getenv("PEPS_HIP_ALS_BLOCKS")) als_blocks = std::atoi(v) != 0;
   if (const char* v = std::getenv("PEPS_HIP_EIG_DIRECT_MAX")) eig_direct_max = std::max(0, std::atoi(v));
   if (const char* v = std::getenv("PEPS_HIP_GEMM_SHAPES")) gemm_shapes = std::atoi(v) != 0;
 }

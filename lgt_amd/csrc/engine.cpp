@@ -992,6 +992,136 @@ Tensor apply_gate(Ctx& c, const Tensor& th, const Tensor& G) {
 }
 
 // _solve_herm :37-45 applied to every column of S at once.
+// ── block structure of the ALS normal equations ─────────────────────────────
+
+constexpr int64_t kAlsBlocksMinN = 96;  // smaller matrices take the one dense solve
+//
+// The gauge symmetry makes the normal matrices block diagonal up to rounding
+// (couplings between charge sectors come out of the contractions as sums of
+// products that cancel to ~1e-17 of the largest entry).  Dropping couplings
+// whose total Frobenius norm is at most eps ||N||_F leaves a matrix whose
+// exact eigendecomposition is a backward-stable one of N, at a smaller error
+// than LAPACK's bound for its dense eigensolvers (p(n) eps ||N||_2), so the
+// floored solve of the independent blocks stands in for the dense one.
+//
+// The host copy h is column-major n x n (interleaved if complex).  Couplings
+// are dropped by binary-exponent class of |N_ij|^2, smallest first, while the
+// running sum stays within the budget; the kept ones join rows into
+// components (union-find).  Returns the components (rows ascending) or an
+// empty list when the largest holds more than 85% of the rows.
+static std::vector<std::vector<int64_t>> sector_blocks(const std::vector<double>& h, int64_t n, bool cplx) {
+  auto mag2 = [&](int64_t i, int64_t j) {
+    const int64_t k = i + j * n;
+    if (!cplx) return h[k] * h[k];
+    return h[2 * k] * h[2 * k] + h[2 * k + 1] * h[2 * k + 1];
+  };
+  constexpr int kOff = 1100, kBins = 2200;
+  std::vector<double> sums(kBins, 0.0);
+  double fro2 = 0.0;
+  auto bin = [&](double m2) { return std::min(kBins - 1, std::max(0, std::ilogb(m2) + kOff)); };
+  for (int64_t j = 0; j < n; ++j) {
+    fro2 += mag2(j, j);
+    for (int64_t i = j + 1; i < n; ++i) {
+      const double m2 = mag2(i, j);
+      fro2 += 2.0 * m2;
+      if (m2 > 0.0) sums[bin(m2)] += 2.0 * m2;
+    }
+  }
+  if (!(fro2 > 0.0) || !std::isfinite(fro2)) return {};
+  const double eps = 2.220446049250313e-16, budget = eps * eps * fro2;
+  int keep_from = 0;  // classes below this one are dropped
+  double cum = 0.0;
+  while (keep_from < kBins && cum + sums[keep_from] <= budget) cum += sums[keep_from++];
+  std::vector<int64_t> parent(n);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto find = [&](int64_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i = j + 1; i < n; ++i) {
+      const double m2 = mag2(i, j);
+      if (m2 > 0.0 && bin(m2) >= keep_from) {
+        const int64_t a = find(i), b = find(j);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+      }
+    }
+  std::map<int64_t, std::vector<int64_t>> comp;
+  for (int64_t i = 0; i < n; ++i) comp[find(i)].push_back(i);
+  size_t largest = 0;
+  for (auto& kv : comp) largest = std::max(largest, kv.second.size());
+  if (double(largest) > 0.85 * double(n)) return {};
+  std::vector<std::vector<int64_t>> out;
+  for (auto& kv : comp) out.push_back(std::move(kv.second));
+  return out;
+}
+
+// The floored solve of a real N that splits into the given blocks: every
+// block is diagonalised on its own (singletons directly), the floor is taken
+// against the largest eigenvalue of all of them, and each block's rows of
+// the solution come from its own eigenpairs.
+static Tensor solve_herm_blocks(Ctx& c, const Tensor& Nh, const std::vector<double>& h, const Tensor& S,
+                                double reg, const std::vector<std::vector<int64_t>>& blocks) {
+  const int64_t n = Nh.dims[0], nrhs = S.dims[1];
+  struct Blk {
+    std::vector<int64_t> idx;
+    Tridiag t;
+    Tensor Z;
+    std::vector<double> w;
+  };
+  std::vector<Blk> big;
+  std::vector<int64_t> single;
+  std::vector<double> wsingle;
+  double wmax = 0.0;
+  for (const auto& b : blocks) {
+    if (b.size() == 1) {
+      single.push_back(b[0]);
+      wsingle.push_back(h[b[0] + b[0] * n]);
+      wmax = std::max(wmax, wsingle.back());
+      continue;
+    }
+    Blk k;
+    k.idx = b;
+    Tensor Nb = gather2d(c, Nh, b, b);
+    k.t = tridiagonalize(c, Nb);
+    tridiag_eig(c, k.t, &k.w, &k.Z);
+    for (double x : k.w) wmax = std::max(wmax, x);
+    big.push_back(std::move(k));
+  }
+  const double floor = reg * std::max(wmax, 1e-30);
+  int64_t floored = 0;
+  auto invert = [&](const std::vector<double>& w) {
+    std::vector<double> inv(w.size());
+    for (size_t i = 0; i < w.size(); ++i) {
+      inv[i] = w[i] > floor ? 1.0 / w[i] : 0.0;
+      floored += w[i] > floor ? 0 : 1;
+    }
+    return inv;
+  };
+  Tensor X = c.zeros({n, nrhs}, S.cplx);
+  const std::vector<int64_t> rhs = iota_v(nrhs);
+  for (auto& k : big) {
+    const int64_t nb = int64_t(k.idx.size());
+    std::vector<double> inv = invert(k.w);
+    Tensor Sb = gather2d(c, S, k.idx, rhs);
+    apply_q(c, k.t, Sb, true);                                    // Q^T S
+    Tensor T = matmul(c, k.Z, Sb, false, false, true, false);     // Z^T Q^T S
+    scale_rows(c, T, inv);
+    Tensor Xb = matmul(c, k.Z, T);
+    apply_q(c, k.t, Xb, false);
+    scatter_rows(c, X, Xb, iota_v(nb), k.idx, rhs);
+  }
+  if (!single.empty()) {
+    std::vector<double> inv = invert(wsingle);
+    Tensor Ss = gather2d(c, S, single, rhs);
+    scale_rows(c, Ss, inv);
+    scatter_rows(c, X, Ss, iota_v(int64_t(single.size())), single, rhs);
+  }
+  c.count(floored ? "als_solve[floored]" : "als_solve[full rank]", double(floored), 0);
+  c.count("als_solve[blocks]", double(blocks.size()), double(big.size()));
+  return X;
+}
+
 static Tensor solve_herm_multi(Ctx& c, const Tensor& N, const Tensor& S, double reg) {
   TierGuard fp64(c, TIER_FP64);
   Tensor Nh = copy_tensor(c, N);
@@ -1006,6 +1136,11 @@ static Tensor solve_herm_multi(Ctx& c, const Tensor& N, const Tensor& S, double 
     if (!nonzero) {  // N = 0: every mode is floored, x = 0
       c.count("als_solve[floored]", double(n), 0);
       return c.zeros({n, S.dims[1]}, S.cplx);
+    }
+    if (c.als_blocks && n >= kAlsBlocksMinN) {
+      const std::vector<double> h = c.to_host(Nh);
+      const auto blocks = sector_blocks(h, n, false);
+      if (!blocks.empty()) return solve_herm_blocks(c, Nh, h, S, reg, blocks);
     }
     Tridiag t = tridiagonalize(c, Nh);
     std::vector<double> w;
@@ -1044,6 +1179,13 @@ static Tensor solve_herm_multi(Ctx& c, const Tensor& N, const Tensor& S, double 
   Tensor T = matmul(c, e.V, S, true, false, true, false);  // V^H S
   scale_rows(c, T, inv);
   return matmul(c, e.V, T);
+}
+
+Tensor solve_herm(Ctx& c, const Tensor& N, const Tensor& S, double reg) {
+  PEPS_CHECK(N.rank() == 2 && N.dims[0] == N.dims[1] && S.rank() == 2 && S.dims[0] == N.dims[0] &&
+                 N.cplx == S.cplx,
+             PS_ESHAPE, "solve_herm: N (n x n) and S (n x k) expected");
+  return solve_herm_multi(c, N, S, reg);
 }
 
 double als_cost(Ctx& c, const Tensor& th, const Tensor& env, const Tensor& X, const Tensor& Y,

@@ -119,6 +119,8 @@ Env bond_environment_v(State& s, int ix, int iy, int chi);
 CSVD charge_svd(Ctx& c, const Tensor& M, const std::vector<int64_t>& rq,
                 const std::vector<int64_t>& cq, int Dmax, double reg = GI_REG, bool leading_only = false);
 Tensor apply_gate(Ctx& c, const Tensor& th, const Tensor& G);
+// _solve_herm (finite_peps_fullupdate_v4.jl:37-45) for the k columns of S.
+Tensor solve_herm(Ctx& c, const Tensor& N, const Tensor& S, double reg);
 double als_cost(Ctx& c, const Tensor& th, const Tensor& env, const Tensor& X, const Tensor& Y,
                 const Tensor* Eth = nullptr);
 void als_truncate(Ctx& c, const Tensor& th, const Tensor& env, int Dmax,

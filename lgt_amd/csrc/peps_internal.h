@@ -156,6 +156,10 @@ class Ctx {
   // sectors in the ALS initialisation (environment variable
   // PEPS_HIP_LEADING_SVD=0 turns it off: every sector takes the full SVD)
   bool leading_svd = true;
+  // ALS normal equations split into independent blocks where the couplings
+  // between them are within the matrix's rounding level (environment
+  // variable PEPS_HIP_ALS_BLOCKS=0 turns it off)
+  bool als_blocks = true;
   // check every intermediate of the boundary compression for non-finite
   // values and name the first step that produced one (environment variable
   // PEPS_HIP_DEBUG_FINITE=1; one device scan per factor)

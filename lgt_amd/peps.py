@@ -118,6 +118,7 @@ _SIGS = {
     "peps_gemm_acc": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp,
                                   C.c_int32, C.c_int32, C.c_double, _dp]),
     "peps_eigh": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
+    "peps_solve_herm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, C.c_double, _dp]),
     "peps_svd": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]),
     "peps_bond_operator": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_double, C.c_double, C.c_double, C.c_double, _i32p, _dp]),
@@ -906,6 +907,22 @@ def eigh(A, ctx=None):
     ctx.check(lib().peps_eigh(ctx.h, int(cp), n, nb, _dptr(a), _dptr(V), _dptr(w)))
     V = np.transpose(V, (0, 2, 1))
     return w.reshape(A.shape[:-1]), V.reshape(A.shape)
+
+
+def solve_herm(N, S, reg=1e-10, ctx=None):
+    """_solve_herm (finite_peps_fullupdate_v4.jl:37-45): the eigenvalue-floored
+    solve of the ALS, for the columns of S (n x k)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(N) or np.iscomplexobj(S)
+    a = _as_array(N, cp)
+    s2 = np.asarray(S)
+    s2 = s2.reshape(-1, 1) if s2.ndim == 1 else s2
+    b = _as_array(s2, cp)
+    n, k = a.shape[0], b.shape[1]
+    assert a.shape == (n, n) and b.shape[0] == n
+    out = np.empty((n, k), dtype=a.dtype, order="F")
+    ctx.check(lib().peps_solve_herm(ctx.h, int(cp), n, k, _dptr(a), _dptr(b), float(reg), _dptr(out)))
+    return out
 
 
 def svd(A, ctx=None):

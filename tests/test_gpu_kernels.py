@@ -247,3 +247,48 @@ def test_als_init_leading_triplets_at_full_size(ctx, noise):
             assert np.abs(XYg - XYo).max() <= 1e-10 * np.abs(XYo).max()
         else:
             assert abs(ig["cost1"] - io["cost1"]) <= 1e-8 * abs(io["cost1"])
+
+
+def _blocky_psd(rng, sizes, coupling):
+    """A symmetric PSD matrix that is block diagonal after a row permutation:
+    each block Q diag(w) Q^T with w log-uniform in [1e-5, 1] times 1, 0.1 or
+    0.01, or exactly 0: none near the solve's 1e-10 floor, and the kept part
+    conditioned to <= 1e7.  Couplings of relative size `coupling` join every
+    pair of rows."""
+    n = sum(sizes)
+    N = np.zeros((n, n))
+    o = 0
+    for b, k in enumerate(sizes):
+        Q, _ = np.linalg.qr(rng.standard_normal((k, k)))
+        w = 10.0 ** rng.uniform(-5, 0, k) * 10.0 ** (-b % 3)
+        w[rng.random(k) < 0.4] = 0.0
+        N[o:o + k, o:o + k] = (Q * w) @ Q.T
+        o += k
+    E = rng.standard_normal((n, n)) * coupling * np.abs(N).max()
+    N = N + 0.5 * (E + E.T)
+    p = rng.permutation(n)
+    return N[np.ix_(p, p)]
+
+
+@pytest.mark.parametrize("case", ["split", "coupled", "dense"])
+def test_solve_herm_sector_blocks(ctx, case):
+    """_solve_herm (finite_peps_fullupdate_v4.jl:37-45) through peps_solve_herm
+    against the oracle's dense eigen-floored solve.  A matrix whose couplings
+    between row groups total far below eps ||N||_F is solved block by block
+    (the library's counter shows it); couplings above that budget, or no
+    block structure, keep the dense solve.  The spectra stay clear of the
+    floor, so both agree to 1e-8 relative."""
+    from lgt_amd import peps
+    from oracle import peps_oracle as O
+    rng = np.random.default_rng({"split": 3, "coupled": 4, "dense": 5}[case])
+    if case == "dense":
+        N = _blocky_psd(rng, [140], 0.0)
+    else:
+        N = _blocky_psd(rng, [70, 45, 30, 9] + [1] * 6, 1e-20 if case == "split" else 1e-12)
+    S = rng.standard_normal((N.shape[0], 8))
+    before = ctx.stats().get("als_solve[blocks]", {"launches": 0})["launches"]
+    X = peps.solve_herm(N, S, 1e-10, ctx=ctx)
+    after = ctx.stats().get("als_solve[blocks]", {"launches": 0})["launches"]
+    Xr = O.solve_herm(N.astype(complex), S.astype(complex), 1e-10)
+    assert (after - before) == (1 if case == "split" else 0)
+    assert np.linalg.norm(X - Xr.real) <= 1e-8 * np.linalg.norm(Xr)
