@@ -36,6 +36,7 @@ typedef struct peps_ctx peps_ctx;
 typedef struct peps_state peps_state;
 typedef struct peps_tensor_list peps_tensor_list;
 typedef struct peps_mlp peps_mlp;
+typedef struct peps_ed peps_ed;
 
 enum {
   PEPS_OK = 0,
@@ -378,6 +379,34 @@ int32_t peps_svd(peps_ctx* ctx, int32_t is_complex, int64_t m, int64_t n, const 
 int32_t peps_bond_operator(int32_t orient, int32_t ix, int32_t iy, int32_t nx, int32_t ny,
                            int32_t gd, double g, double t_hop, double m, double tau, int32_t* n,
                            double* out);
+
+/* ── exact diagonalisation in one gauge sector (finite_ed.jl) ─────────────
+   H = sum (g^2/2)(E_R^2 + E_U^2) + m sum (-1)^(ix+iy) n_f - t sum (hops with
+   the link raised on a forward hop), no plaquette term (finite_ed.jl:6-10).
+   States are keyed as in finite_ed.jl:79-95 (n_f bits, then E + dg in
+   2-bit link fields for dg = 1, wider fields for dg = 2, 3). */
+/* build_basis (finite_ed.jl:125-233): g_charges nx*ny, ix fastest; the
+   basis is enumerated on the GPU in the reference's order.  Lattices from
+   2x2 to 32 sites, dg in [1, 3], keys within 64 bits. */
+int32_t peps_ed_create(peps_ctx* ctx, int32_t nx, int32_t ny, int32_t dg, const int64_t* g_charges,
+                       peps_ed** out);
+int32_t peps_ed_destroy(peps_ed* ed);
+int32_t peps_ed_dim(const peps_ed* ed, int64_t* dim);
+/* the encoded basis states, in basis order (dim entries) */
+int32_t peps_ed_keys(peps_ed* ed, uint64_t* keys);
+/* build_hamiltonian (finite_ed.jl:252-326) */
+int32_t peps_ed_hamiltonian(peps_ed* ed, double g, double t, double m);
+/* y = H x for real vectors of the basis dimension */
+int32_t peps_ed_apply(peps_ed* ed, const double* x, double* y);
+/* find_ground_state (finite_ed.jl:333-341): restarted Lanczos (Krylov
+   dimension 50, at most 300 restarts) until ||H psi - E0 psi|| <= tol;
+   psi (optional, dim doubles) receives the normalised real ground state.
+   PEPS_ESOLVER if the tolerance is not reached. */
+int32_t peps_ed_ground_state(peps_ed* ed, double tol, double* E0, double* psi);
+/* measure_ED (finite_ed.jl:369-395) of psi (NULL: the ground state):
+   out = {nf_mean, nf_even, nf_odd, E2_mean}; nf_grid, E2_grid (optional)
+   nx*ny, ix fastest. */
+int32_t peps_ed_measure(peps_ed* ed, const double* psi, double out[4], double* nf_grid, double* E2_grid);
 
 #ifdef __cplusplus
 }

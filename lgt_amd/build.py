@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libpeps_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("PEPS_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "eigh.hip", "ctx.cpp", "contract.cpp", "linalg.cpp", "model.cpp",
+SOURCES = ["kernels.hip", "eigh.hip", "ed.hip", "ctx.cpp", "contract.cpp", "linalg.cpp", "model.cpp",
            "engine.cpp", "mlp.cpp", "api.cpp"]
 
 CFLAGS = [

@@ -5,6 +5,7 @@
 #include <set>
 #include <thread>
 
+#include "ed.h"
 #include "engine.h"
 #include "mlp.h"
 
@@ -33,6 +34,11 @@ struct peps_mlp {
 struct peps_tensor_list {
   std::shared_ptr<peps::Ctx> ref;
   std::vector<peps::Tensor> t;
+};
+
+struct peps_ed {
+  std::shared_ptr<peps::Ctx> ref;
+  std::unique_ptr<peps::EDBasis> b;  // released before the context
 };
 
 namespace {
@@ -963,6 +969,85 @@ int32_t peps_bond_operator(int32_t orient, int32_t ix, int32_t iy, int32_t nx, i
     if (tau != 0.0) H = peps::expm_sym(H, tau);
     *n = H.n;
     if (out) std::memcpy(out, H.a.data(), H.a.size() * sizeof(double));
+  });
+}
+
+}  // extern "C"
+
+// ── exact diagonalisation (finite_ed.jl) ─────────────────────────────────────
+
+extern "C" {
+
+int32_t peps_ed_create(peps_ctx* ctx, int32_t nx, int32_t ny, int32_t dg, const int64_t* g_charges,
+                       peps_ed** out) {
+  if (!ctx || !g_charges || !out || nx < 1 || ny < 1) return PEPS_EARG;
+  *out = nullptr;
+  return guard(ctx->c.get(), [&] {
+    auto e = std::make_unique<peps_ed>();
+    e->ref = ctx->c;
+    e->b = peps::ed_build_basis(*ctx->c, nx, ny, dg, std::vector<int64_t>(g_charges, g_charges + int64_t(nx) * ny));
+    *out = e.release();
+  });
+}
+
+int32_t peps_ed_destroy(peps_ed* ed) {
+  delete ed;
+  return PEPS_OK;
+}
+
+int32_t peps_ed_dim(const peps_ed* ed, int64_t* dim) {
+  if (!ed || !dim) return PEPS_EARG;
+  *dim = ed->b->dim;
+  return PEPS_OK;
+}
+
+int32_t peps_ed_keys(peps_ed* ed, uint64_t* keys) {
+  if (!ed || !keys) return PEPS_EARG;
+  return guard(ed->ref.get(), [&] {
+    peps::Ctx& c = *ed->ref;
+    PEPS_HIP(hipMemcpyAsync(keys, ed->b->keys->ptr, size_t(ed->b->dim) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            c.stream));
+    PEPS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int32_t peps_ed_hamiltonian(peps_ed* ed, double g, double t, double m) {
+  if (!ed) return PEPS_EARG;
+  return guard(ed->ref.get(), [&] { peps::ed_build_hamiltonian(*ed->b, g, t, m); });
+}
+
+int32_t peps_ed_apply(peps_ed* ed, const double* x, double* y) {
+  if (!ed || !x || !y) return PEPS_EARG;
+  return guard(ed->ref.get(), [&] {
+    peps::Ctx& c = *ed->ref;
+    Tensor xd = c.upload({ed->b->dim}, false, x);
+    Tensor yd = c.alloc({ed->b->dim}, false);
+    peps::ed_apply(*ed->b, xd, yd);
+    c.download(yd, y);
+  });
+}
+
+int32_t peps_ed_ground_state(peps_ed* ed, double tol, double* E0, double* psi) {
+  if (!ed || !E0 || !(tol > 0.0)) return PEPS_EARG;
+  return guard(ed->ref.get(), [&] {
+    *E0 = peps::ed_ground_state(*ed->b, tol, 50, 300);
+    if (psi) ed->ref->download(ed->b->psi0, psi);
+  });
+}
+
+int32_t peps_ed_measure(peps_ed* ed, const double* psi, double out[4], double* nf_grid, double* E2_grid) {
+  if (!ed || !out) return PEPS_EARG;
+  return guard(ed->ref.get(), [&] {
+    peps::Ctx& c = *ed->ref;
+    PEPS_CHECK(psi || ed->b->has_gs, peps::PS_EARG, "ed_measure: no state given and no ground state computed");
+    Tensor p = psi ? c.upload({ed->b->dim}, false, psi) : ed->b->psi0;
+    peps::EDObs o = peps::ed_measure(*ed->b, p);
+    out[0] = o.nf_mean;
+    out[1] = o.nf_even;
+    out[2] = o.nf_odd;
+    out[3] = o.E2_mean;
+    if (nf_grid) std::copy(o.nf_grid.begin(), o.nf_grid.end(), nf_grid);
+    if (E2_grid) std::copy(o.E2_grid.begin(), o.E2_grid.end(), E2_grid);
   });
 }
 

@@ -112,6 +112,14 @@ _SIGS = {
     "peps_mlp_gradients": (C.c_int32, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
     "peps_mlp_train_step": (C.c_int32, [_vp, C.c_int64, _dp, _dp, C.c_double, _dp]),
     "peps_gemm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, C.c_int32, _dp]),
+    "peps_ed_create": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _i64p, C.POINTER(_vp)]),
+    "peps_ed_destroy": (C.c_int32, [_vp]),
+    "peps_ed_dim": (C.c_int32, [_vp, _i64p]),
+    "peps_ed_keys": (C.c_int32, [_vp, C.POINTER(C.c_uint64)]),
+    "peps_ed_hamiltonian": (C.c_int32, [_vp, C.c_double, C.c_double, C.c_double]),
+    "peps_ed_apply": (C.c_int32, [_vp, _dp, _dp]),
+    "peps_ed_ground_state": (C.c_int32, [_vp, C.c_double, _dp, _dp]),
+    "peps_ed_measure": (C.c_int32, [_vp, _dp, _dp, _dp, _dp]),
     "peps_gemm_noalloc": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _vp, C.c_int32, C.c_int32,
                                       _vp, C.c_int32, C.c_int32, _vp]),
     "peps_permute": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p, _i32p, _dp, _dp]),
@@ -953,3 +961,82 @@ def bond_operator(orient, ix, iy, nx, ny, gd, g, t_hop, m, tau=0.0):
     if st != 0:
         raise PepsError(st, "peps_bond_operator failed")
     return out
+
+
+# ── finite_ed.jl: exact diagonalisation in one gauge sector ──────────────────
+
+class ExactDiag:
+    """One Gauss-law sector of the finite U(1) LGT on the GPU (finite_ed.jl):
+    the basis (build_basis), the Hamiltonian (build_hamiltonian), its ground
+    state (find_ground_state) and the observables (measure_ED).
+    g_charges[ix-1][iy-1] is the background charge of site (ix, iy)."""
+
+    def __init__(self, nx, ny, dg, g_charges, ctx=None):
+        self.ctx = ctx or default_context()
+        self.nx, self.ny, self.dg = nx, ny, dg
+        g = np.asarray(g_charges, dtype=np.int64).reshape(nx, ny)
+        gf = np.ascontiguousarray(g.T.reshape(-1))  # ix fastest
+        self.h = _vp()
+        self.ctx.check(lib().peps_ed_create(self.ctx.h, nx, ny, dg, _iptr(gf), C.byref(self.h)))
+        d = C.c_int64()
+        self.ctx.check(lib().peps_ed_dim(self.h, C.byref(d)))
+        self.dim = d.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().peps_ed_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def keys(self):
+        out = np.empty(self.dim, np.uint64)
+        self.ctx.check(lib().peps_ed_keys(self.h, out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
+    def hamiltonian(self, g, t, m):
+        self.ctx.check(lib().peps_ed_hamiltonian(self.h, float(g), float(t), float(m)))
+        return self
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.dim)
+        self.ctx.check(lib().peps_ed_apply(self.h, _dptr(x), _dptr(y)))
+        return y
+
+    def ground_state(self, tol=1e-10):
+        e = C.c_double()
+        psi = np.empty(self.dim)
+        self.ctx.check(lib().peps_ed_ground_state(self.h, float(tol), C.byref(e), _dptr(psi)))
+        return e.value, psi
+
+    def measure(self, psi=None):
+        out = np.zeros(4)
+        nf = np.zeros(self.nx * self.ny)
+        e2 = np.zeros(self.nx * self.ny)
+        p = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
+        self.ctx.check(lib().peps_ed_measure(self.h, None if p is None else _dptr(p), _dptr(out), _dptr(nf),
+                                             _dptr(e2)))
+        return dict(nf_mean=out[0], nf_even=out[1], nf_odd=out[2], E2_mean=out[3],
+                    nf_grid=nf.reshape(self.ny, self.nx).T, E2_grid=e2.reshape(self.ny, self.nx).T)
+
+
+def build_basis(nx, ny, dg, g_charges, ctx=None) -> ExactDiag:
+    """build_basis (finite_ed.jl:125)."""
+    return ExactDiag(nx, ny, dg, g_charges, ctx=ctx)
+
+
+def build_hamiltonian(ed: ExactDiag, g, t, m) -> ExactDiag:
+    """build_hamiltonian (finite_ed.jl:252), kept on the device."""
+    return ed.hamiltonian(g, t, m)
+
+
+def find_ground_state(ed: ExactDiag, tol=1e-10):
+    """find_ground_state (finite_ed.jl:333) -> (E0, psi0)."""
+    return ed.ground_state(tol)
+
+
+def measure_ED(ed: ExactDiag, psi=None):
+    """measure_ED (finite_ed.jl:369) -> dict(nf_mean, nf_even, nf_odd, E2_mean, nf_grid, E2_grid)."""
+    return ed.measure(psi)
