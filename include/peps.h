@@ -51,7 +51,10 @@ enum {
 
 enum { PEPS_TIER_FP64 = 0, PEPS_TIER_FP32 = 1, PEPS_TIER_BF16 = 2 };
 enum { PEPS_H = 0, PEPS_V = 1 };
-enum { PEPS_STEP_V4 = 0, PEPS_STEP_GAUGE = 1, PEPS_STEP_DENSE = 2, PEPS_STEP_FULL_V2 = 3 };
+/* PEPS_STEP_REALTIME: realtime_trotter_step! (finite_peps_quench.jl:81-109),
+   one second-order real-time step exp(-i dt H) with dt = peps_params.tau
+   through the dense update_bond_h!/v! (complex states only). */
+enum { PEPS_STEP_V4 = 0, PEPS_STEP_GAUGE = 1, PEPS_STEP_DENSE = 2, PEPS_STEP_FULL_V2 = 3, PEPS_STEP_REALTIME = 4 };
 /* peps_params.flags bits.  PEPS_STEP_RENORM has no counterpart in the
    reference: every boundary MPS built by a full-update step or an energy
    evaluation is truncated at 1e-14 times its largest singular value (the
@@ -82,6 +85,9 @@ typedef struct peps_params {
   int32_t n_als;   /* ALS sweeps (exactly n_als when als_tol == 0) */
   int32_t flags;    /* PEPS_STEP_* bits, 0 for the reference's behaviour */
   double als_tol;  /* > 0: stop ALS when the relative cost change < als_tol */
+  double mu;       /* chemical potential mu n_f (trotter_step!'s mu,
+                      finite_peps_ground_state.jl:500): PEPS_STEP_DENSE and
+                      PEPS_STEP_REALTIME only, 0 for every other kind */
 } peps_params;
 
 /* ── context ──────────────────────────────────────────────────────────────── */
@@ -231,6 +237,11 @@ int32_t peps_batch_trotter_step(peps_state* const* states, int32_t n, int32_t ki
    whole run.  Progress lines (PEPS_HIP_VERBOSE=1) every 25 steps. */
 int32_t peps_ite_ground_state(peps_state* st, int32_t kind, const peps_params* p, int64_t n_ite,
                               int64_t* steps_done);
+/* apply_site_op! (finite_peps_quench.jl:228-234): T(ix,iy) <- O * reshape(T,
+   dp, :) with O (dp x dp, column-major) in the state's element type.  The
+   flux labels are undefined afterwards (the state is then label-free, as
+   after a dense update). */
+int32_t peps_apply_site_op(peps_state* st, int32_t ix, int32_t iy, const double* O);
 /* ALS statistics of the last PEPS_STEP_V4 step: bonds, accepted, total sweeps */
 int32_t peps_last_als_stats(peps_state* st, int64_t out[3]);
 /* compute_energy_v4 (finite_peps_fullupdate_v4.jl:390): total energy */

@@ -103,6 +103,8 @@ peps::Params to_params(const peps_params* p) {
   q.chi = p->chi;
   q.n_als = p->n_als;
   q.als_tol = p->als_tol;
+  q.mu = p->mu;
+  PEPS_CHECK(std::isfinite(q.mu), peps::PS_EARG, "mu must be finite");
   PEPS_CHECK((p->flags & ~int32_t(PEPS_STEP_RENORM | PEPS_STEP_POSITIVE_COST)) == 0, peps::PS_EARG,
              "unknown step flags");
   q.renorm = (p->flags & PEPS_STEP_RENORM) != 0;
@@ -535,6 +537,8 @@ int32_t peps_fu_bond_v4(peps_state* st, int32_t orient, int32_t ix, int32_t iy,
 
 namespace {
 void run_step(peps_state* st, int32_t kind, const peps::Params& q) {
+    PEPS_CHECK(q.mu == 0.0 || kind == PEPS_STEP_DENSE || kind == PEPS_STEP_REALTIME, peps::PS_EARG,
+               "mu is taken by PEPS_STEP_DENSE and PEPS_STEP_REALTIME only");
     if (kind == PEPS_STEP_V4) {
       require_labels(st->s);
       std::vector<peps::ALSInfo> infos;
@@ -553,6 +557,9 @@ void run_step(peps_state* st, int32_t kind, const peps::Params& q) {
       peps::trotter_step_dense(st->s, q, st->gc);
     } else if (kind == PEPS_STEP_FULL_V2) {
       peps::trotter_step_full_v2(st->s, q, st->gc);
+    } else if (kind == PEPS_STEP_REALTIME) {
+      PEPS_CHECK(st->s.cplx, peps::PS_EARG, "PEPS_STEP_REALTIME needs a complex state");
+      peps::trotter_step_realtime(st->s, q, st->gc);
     } else {
       throw peps::Error(peps::PS_EARG, "unknown step kind");
     }
@@ -653,7 +660,23 @@ int32_t peps_last_als_stats(peps_state* st, int64_t out[3]) {
 
 int32_t peps_energy_v4(peps_state* st, const peps_params* p, double* E) {
   if (!st || !p || !E) return PEPS_EARG;
-  return guard(st->ref.get(), [&] { *E = peps::energy_v4(st->s, to_params(p)); });
+  return guard(st->ref.get(), [&] {
+    peps::Params q = to_params(p);
+    PEPS_CHECK(q.mu == 0.0, peps::PS_EARG, "compute_energy_v4 takes no mu");
+    *E = peps::energy_v4(st->s, q);
+  });
+}
+
+int32_t peps_apply_site_op(peps_state* st, int32_t ix, int32_t iy, const double* O) {
+  if (!st || !O) return PEPS_EARG;
+  return guard(st->ref.get(), [&] {
+    check_site(st->s, ix, iy);
+    Tensor& T = st->s.t(ix, iy);
+    PEPS_CHECK(!T.empty(), peps::PS_EARG, "site tensor not set");
+    const int64_t dp = T.dims[0];
+    Tensor Od = st->s.c->upload({dp, dp}, st->s.cplx, O);
+    peps::apply_site_op(st->s, ix, iy, Od);
+  });
 }
 
 // ── dense simple update with plaquettes ─────────────────────────────────────

@@ -2091,7 +2091,8 @@ Measurements measure_all_finite(State& s) {
 //  Drivers
 // ════════════════════════════════════════════════════════════════════════════
 
-Tensor GateCache::get(State& s, char orient, int ix, int iy, const Params& p, double tau_factor) {
+Tensor GateCache::get(State& s, char orient, int ix, int iy, const Params& p, double tau_factor,
+                      bool real_time) {
   // the gate depends on the bond only through the two sites' dimensions,
   // on-site weights and stagger signs
   int jx = orient == 'h' ? ix + 1 : ix, jy = orient == 'h' ? iy : iy + 1;
@@ -2100,12 +2101,24 @@ Tensor GateCache::get(State& s, char orient, int ix, int iy, const Params& p, do
   k.precision(17);
   k << orient << a.dR << a.dU << b.dR << b.dU << ((ix + iy) % 2) << ':'
     << onsite_weight(ix, iy, s.nx, s.ny) << ':' << onsite_weight(jx, jy, s.nx, s.ny) << ':' << p.g
-    << ':' << p.t << ':' << p.m << ':' << p.tau * tau_factor << ':' << s.cplx;
+    << ':' << p.t << ':' << p.m << ':' << p.mu << ':' << p.tau * tau_factor << ':' << s.cplx << ':'
+    << real_time;
   auto it = g.find(k.str());
   if (it != g.end()) return it->second;
-  Mat H = orient == 'h' ? H_merged_h_site(ix, iy, s.nx, s.ny, s.gd, p.g, p.t, p.m)
-                        : H_merged_v_site(ix, iy, s.nx, s.ny, s.gd, p.g, p.t, p.m);
-  Tensor G = gate_tensor(*s.c, expm_sym(H, p.tau * tau_factor), s.cplx);
+  Mat H = orient == 'h' ? H_merged_h_site(ix, iy, s.nx, s.ny, s.gd, p.g, p.t, p.m, p.mu)
+                        : H_merged_v_site(ix, iy, s.nx, s.ny, s.gd, p.g, p.t, p.m, p.mu);
+  Tensor G;
+  if (real_time) {
+    // exp(-i dt H): complex, so only a complex state can take it
+    PEPS_CHECK(s.cplx, PS_EARG, "real-time gates need a complex state");
+    Mat re, im;
+    expm_sym_imag(H, p.tau * tau_factor, &re, &im);
+    std::vector<double> h(re.a.size() * 2);
+    for (size_t i = 0; i < re.a.size(); ++i) h[2 * i] = re.a[i], h[2 * i + 1] = im.a[i];
+    G = s.c->upload({re.n, re.n}, true, h.data());
+  } else {
+    G = gate_tensor(*s.c, expm_sym(H, p.tau * tau_factor), s.cplx);
+  }
   g[k.str()] = G;
   return G;
 }
@@ -2134,6 +2147,35 @@ void trotter_step_dense(State& s, const Params& p, GateCache& gc) {
     for (int ix = s.nx; ix >= 1; --ix) dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 0.5), p.Dmax, SU_DENSE);
   for (int iy = s.ny; iy >= 1; --iy)
     for (int ix = s.nx - 1; ix >= 1; --ix) dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5), p.Dmax, SU_DENSE);
+}
+
+void apply_site_op(State& s, int ix, int iy, const Tensor& O) {
+  // apply_site_op! (finite_peps_quench.jl:228-234): T <- O * reshape(T, dp, :)
+  Tensor& T = s.t(ix, iy);
+  const int64_t dp = T.dims[0];
+  PEPS_CHECK(O.rank() == 2 && O.dims[0] == dp && O.dims[1] == dp && O.cplx == T.cplx, PS_ESHAPE,
+             "apply_site_op: operator must be dp x dp in the state's element type");
+  const std::vector<int64_t> dims = T.dims;
+  Tensor R = matmul(*s.c, O, T.reshaped({dp, T.numel() / std::max<int64_t>(dp, 1)}));
+  T = R.reshaped(dims);
+  // an arbitrary operator leaves the Gauss-law flux labels undefined
+  s.labelled = false;
+}
+
+void trotter_step_realtime(State& s, const Params& p, GateCache& gc) {
+  // realtime_trotter_step! (finite_peps_quench.jl:81-109): the dense
+  // update_bond_h!/v! with exp(-i dt/2 H_h) (iy ascending, ix ascending),
+  // exp(-i dt H_v) (iy ascending, ix ascending), exp(-i dt/2 H_h) (iy
+  // ascending, ix descending); dt = p.tau
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = 1; ix < s.nx; ++ix)
+      dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5, true), p.Dmax, SU_DENSE);
+  for (int iy = 1; iy < s.ny; ++iy)
+    for (int ix = 1; ix <= s.nx; ++ix)
+      dense_bond_v(s, ix, iy, gc.get(s, 'v', ix, iy, p, 1.0, true), p.Dmax, SU_DENSE);
+  for (int iy = 1; iy <= s.ny; ++iy)
+    for (int ix = s.nx - 1; ix >= 1; --ix)
+      dense_bond_h(s, ix, iy, gc.get(s, 'h', ix, iy, p, 0.5, true), p.Dmax, SU_DENSE);
 }
 
 void trotter_step_full_v2(State& s, const Params& p, GateCache& gc) {

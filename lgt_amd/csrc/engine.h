@@ -21,6 +21,7 @@ using MPS = std::vector<Tensor>;
 
 struct Params {
   double g = 1.0, t = 1.0, m = 0.5, tau = 0.05;
+  double mu = 0.0;  // chemical potential mu n_f in H_onsite_site (dense step only)
   int Dmax = 2, chi = 16, n_als = 30;
   double als_tol = 0.0;  // > 0: stop sweeping when the relative cost change < tol
   bool renorm = false;   // normalise every boundary MPS after each row absorption
@@ -139,7 +140,9 @@ void fu_bond_v(State& s, int ix, int iy, const Tensor& G, int Dmax, int n_als, d
 
 struct GateCache {
   std::map<std::string, Tensor> g;
-  Tensor get(State& s, char orient, int ix, int iy, const Params& p, double tau_factor);
+  // exp(-tau_factor tau H_b) of bond b, or exp(-i tau_factor tau H_b) when
+  // real_time (complex states only)
+  Tensor get(State& s, char orient, int ix, int iy, const Params& p, double tau_factor, bool real_time = false);
 };
 
 void trotter_step_gauge(State& s, const Params& p, GateCache& gc);
@@ -172,6 +175,10 @@ void update_plaquette(State& s, int ix, int iy, const PlaquetteGate& pg, int Dtr
 void trotter_step_dense(State& s, const Params& p, GateCache& gc);
 // trotter_step_full! (finite_peps_full_update.jl:550-575, use_gauss = false)
 void trotter_step_full_v2(State& s, const Params& p, GateCache& gc);
+// apply_site_op! (finite_peps_quench.jl:228-234); marks the state unlabelled
+void apply_site_op(State& s, int ix, int iy, const Tensor& O);
+// realtime_trotter_step! (finite_peps_quench.jl:81-109): dt = p.tau
+void trotter_step_realtime(State& s, const Params& p, GateCache& gc);
 // expect_site (:549-567), O is dp x dp column-major (complex interleaved if o_cplx)
 double expect_site(State& s, int ix, int iy, const std::vector<double>& O, bool o_cplx);
 // plaquette_energy (:589-684) and total_energy (:690-805)

@@ -147,21 +147,21 @@ double onsite_weight(int ix, int iy, int nx, int ny) {
   return 1.0 / nb;
 }
 
-Mat H_merged_h_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m) {
+Mat H_merged_h_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m, double mu) {
   // u1_lgt_hamiltonian.jl:829-841
   int dL = site_dims(ix, iy, nx, ny, gd).d, dR = site_dims(ix + 1, iy, nx, ny, gd).d;
   Mat H = H_hop_h_site(ix, iy, nx, ny, gd, t);
-  axpy(H, onsite_weight(ix, iy, nx, ny), kron(H_onsite_site(ix, iy, nx, ny, gd, g, m), eye(dR)));
-  axpy(H, onsite_weight(ix + 1, iy, nx, ny), kron(eye(dL), H_onsite_site(ix + 1, iy, nx, ny, gd, g, m)));
+  axpy(H, onsite_weight(ix, iy, nx, ny), kron(H_onsite_site(ix, iy, nx, ny, gd, g, m, mu), eye(dR)));
+  axpy(H, onsite_weight(ix + 1, iy, nx, ny), kron(eye(dL), H_onsite_site(ix + 1, iy, nx, ny, gd, g, m, mu)));
   return H;
 }
 
-Mat H_merged_v_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m) {
+Mat H_merged_v_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m, double mu) {
   // u1_lgt_hamiltonian.jl:851-863
   int dD = site_dims(ix, iy, nx, ny, gd).d, dU = site_dims(ix, iy + 1, nx, ny, gd).d;
   Mat H = H_hop_v_site(ix, iy, nx, ny, gd, t);
-  axpy(H, onsite_weight(ix, iy, nx, ny), kron(H_onsite_site(ix, iy, nx, ny, gd, g, m), eye(dU)));
-  axpy(H, onsite_weight(ix, iy + 1, nx, ny), kron(eye(dD), H_onsite_site(ix, iy + 1, nx, ny, gd, g, m)));
+  axpy(H, onsite_weight(ix, iy, nx, ny), kron(H_onsite_site(ix, iy, nx, ny, gd, g, m, mu), eye(dU)));
+  axpy(H, onsite_weight(ix, iy + 1, nx, ny), kron(eye(dD), H_onsite_site(ix, iy + 1, nx, ny, gd, g, m, mu)));
   return H;
 }
 
@@ -233,6 +233,26 @@ static void jacobi_eig(Mat A, std::vector<double>* w, Mat* V) {
   }
   w->resize(n);
   for (int i = 0; i < n; ++i) (*w)[i] = A(i, i);
+}
+
+void expm_sym_imag(const Mat& H, double dt, Mat* re, Mat* im) {
+  // exp(-i dt H) = V diag(cos(dt w) - i sin(dt w)) V^T for real symmetric H
+  std::vector<double> w;
+  Mat V;
+  jacobi_eig(H, &w, &V);
+  *re = zeros(H.n);
+  *im = zeros(H.n);
+  for (int k = 0; k < H.n; ++k) {
+    const double cr = std::cos(dt * w[k]), ci = -std::sin(dt * w[k]);
+    for (int j = 0; j < H.n; ++j) {
+      const double vj = V(j, k);
+      if (vj == 0.0) continue;
+      for (int i = 0; i < H.n; ++i) {
+        (*re)(i, j) += V(i, k) * vj * cr;
+        (*im)(i, j) += V(i, k) * vj * ci;
+      }
+    }
+  }
 }
 
 Mat expm_sym(const Mat& H, double tau) {

@@ -33,8 +33,10 @@ Mat H_onsite_site(int ix, int iy, int nx, int ny, int gd, double g, double m, do
 Mat H_hop_h_site(int ix, int iy, int nx, int ny, int gd, double t);
 Mat H_hop_v_site(int ix, int iy, int nx, int ny, int gd, double t);
 double onsite_weight(int ix, int iy, int nx, int ny);
-Mat H_merged_h_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m);
-Mat H_merged_v_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m);
+// mu: the optional chemical potential mu n_f of H_onsite_site (0 in every
+// full-update and gauge-invariant caller; the dense trotter_step! takes it)
+Mat H_merged_h_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m, double mu = 0.0);
+Mat H_merged_v_site(int ix, int iy, int nx, int ny, int gd, double g, double t, double m, double mu = 0.0);
 // Pure-gauge plaquette term on the links (b, l, r, t), b slowest
 // (u1_lgt_hamiltonian.jl:283-289).
 Mat H_plaquette_gauge(int gd, double g);
@@ -48,5 +50,8 @@ Mat site_E2(int ix, int iy, int nx, int ny, int gd);
 Mat kron(const Mat& A, const Mat& B);
 // exp(-tau H) for real symmetric H (Jacobi eigendecomposition, FP64).
 Mat expm_sym(const Mat& H, double tau);
+// exp(-i dt H) for real symmetric H: real and imaginary parts (the real-time
+// gate of realtime_trotter_step!, finite_peps_quench.jl:81-109).
+void expm_sym_imag(const Mat& H, double dt, Mat* re, Mat* im);
 
 }  // namespace peps

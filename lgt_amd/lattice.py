@@ -158,3 +158,57 @@ def init_finite_peps_v2_arrays(nx, ny, gd, noise=0.0, seed=None):
                 T /= nrm
             tens[ix - 1][iy - 1] = T
     return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv, gch=staggered_charges(nx, ny), gd=gd)
+
+
+def init_finite_peps_arrays(nx, ny, gd, D, noise=0.0, seed=None, nf_of=None):
+    """init_finite_peps (finite_peps_ground_state.jl:122-160): the label-free
+    FinitePEPS at bond dimension D (1 on the open edges) with all weights 1
+    and the product state |n_f, E_R = 0, E_U = 0> in the [., 1, 1, 1, 1]
+    corner of every site tensor, n_f = 0 on even and 1 on odd ix + iy (or
+    nf_of(ix, iy)); noise > 0 adds noise * z to every entry (z complex
+    Gaussian of unit variance, the reference's randn(ComplexF64), here from
+    numpy.random.default_rng(seed)) without renormalising.  The zero field
+    E = 0 needs odd gd (the reference's dg = (gd-1)/2).  Entries are complex,
+    as the reference's ComplexF64 tensors; the flux labels are placeholders
+    (a FinitePEPS carries none)."""
+    if gd % 2 == 0:
+        raise ValueError("init_finite_peps: the E = 0 product state needs an odd link dimension")
+    rng = np.random.default_rng(seed)
+    lh = [[np.ones(D) for _ in range(ny)] for _ in range(nx - 1)]
+    lv = [[np.ones(D) for _ in range(ny - 1)] for _ in range(nx)]
+    qh = [[np.zeros(D, np.int64) for _ in range(ny)] for _ in range(nx - 1)]
+    qv = [[np.zeros(D, np.int64) for _ in range(ny - 1)] for _ in range(nx)]
+    tens = [[None] * ny for _ in range(nx)]
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx + 1):
+            d, dR, dU = site_dims(ix, iy, nx, ny, gd)
+            shp = (d, D if ix > 1 else 1, D if ix < nx else 1, D if iy < ny else 1, D if iy > 1 else 1)
+            T = np.zeros(shp, np.complex128)
+            nf0 = (0 if (ix + iy) % 2 == 0 else 1) if nf_of is None else int(nf_of(ix, iy))
+            nf, lR, lU = phys_labels(ix, iy, nx, ny, gd)
+            T[np.where((nf == nf0) & (lR == 0) & (lU == 0))[0][0], 0, 0, 0, 0] = 1.0
+            if noise > 0:
+                T = T + noise * (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) / np.sqrt(2.0)
+            tens[ix - 1][iy - 1] = T
+    return dict(tensors=tens, lh=lh, lv=lv, qh=qh, qv=qv, gch=staggered_charges(nx, ny), gd=gd)
+
+
+def site_operators(ix, iy, nx, ny, gd):
+    """The single-site operators the quench protocols use, embedded in the
+    site's physical space (fermion slowest, then the right link, the up link
+    fastest: embed_f_site / embed_R_site / embed_U_site,
+    u1_lgt_hamiltonian.jl:629-643): n_f, c^dagger (op_cdag, :147), U on the
+    right link and U on the up link (op_U_gauge, U|l> = |l+2> in label units;
+    None for a missing link), as complex (d, d) arrays."""
+    d, dR, dU = site_dims(ix, iy, nx, ny, gd)
+    c = np.zeros((2, 2))
+    c[0, 1] = 1.0
+    nfo = np.diag([0.0, 1.0])
+    U = np.zeros((gd, gd))
+    for k in range(gd - 1):
+        U[k + 1, k] = 1.0
+    I2, IR, IU = np.eye(2), np.eye(dR), np.eye(dU)
+    out = dict(nf=np.kron(np.kron(nfo, IR), IU), cdag=np.kron(np.kron(c.T, IR), IU),
+               U_R=np.kron(np.kron(I2, U), IU) if dR > 1 else None,
+               U_U=np.kron(np.kron(I2, IR), U) if dU > 1 else None)
+    return {k: (None if v is None else v.astype(np.complex128)) for k, v in out.items()}

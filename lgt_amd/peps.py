@@ -21,7 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PEPS_HIP_LIB", os.path.join(_HERE, "libpeps_hip.so"))
 
 PEPS_H, PEPS_V = 0, 1
-PEPS_STEP_V4, PEPS_STEP_GAUGE, PEPS_STEP_DENSE, PEPS_STEP_FULL_V2 = 0, 1, 2, 3
+PEPS_STEP_V4, PEPS_STEP_GAUGE, PEPS_STEP_DENSE, PEPS_STEP_FULL_V2, PEPS_STEP_REALTIME = 0, 1, 2, 3, 4
 TIERS = {"fp64": 0, "fp32": 1, "bf16": 2}
 STATUS = {0: "PEPS_OK", 1: "PEPS_EARG", 2: "PEPS_ESHAPE", 3: "PEPS_ENOSECTOR", 4: "PEPS_EHIP",
           5: "PEPS_ENOMEM", 6: "PEPS_ESOLVER", 7: "PEPS_EUNSUPPORTED"}
@@ -41,7 +41,7 @@ class PepsError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("g", C.c_double), ("t_hop", C.c_double), ("m", C.c_double),
                 ("tau", C.c_double), ("D_max", C.c_int32), ("chi", C.c_int32),
-                ("n_als", C.c_int32), ("flags", C.c_int32), ("als_tol", C.c_double)]
+                ("n_als", C.c_int32), ("flags", C.c_int32), ("als_tol", C.c_double), ("mu", C.c_double)]
 
 
 # name: (restype, argtypes)
@@ -88,6 +88,7 @@ _SIGS = {
     "peps_trotter_step": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params)]),
     "peps_batch_trotter_step": (C.c_int32, [C.POINTER(_vp), C.c_int32, C.c_int32, C.POINTER(Params), _i32p]),
     "peps_last_als_stats": (C.c_int32, [_vp, _i64p]),
+    "peps_apply_site_op": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp]),
     "peps_ite_ground_state": (C.c_int32, [_vp, C.c_int32, C.POINTER(Params), C.c_int64, _i64p]),
     "peps_dl_open": (C.c_int32, [_vp, C.c_int32, _i64p, _dp, _dp]),
     "peps_energy_v4": (C.c_int32, [_vp, C.POINTER(Params), _dp]),
@@ -354,9 +355,9 @@ PEPS_STEP_RENORM = 1
 PEPS_STEP_POSITIVE_COST = 2
 
 
-def _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol=0.0, flags=0):
+def _params(g, t_hop, m, tau, D_max, chi, n_als, als_tol=0.0, flags=0, mu=0.0):
     return Params(g=g, t_hop=t_hop, m=m, tau=tau, D_max=D_max, chi=chi, n_als=n_als, flags=flags,
-                  als_tol=als_tol)
+                  als_tol=als_tol, mu=mu)
 
 
 # ── finite_peps_gauge_invariant.jl ───────────────────────────────────────────
@@ -467,10 +468,25 @@ def update_plaquette(gp: GaugePEPS, ix, iy, GA, GB, GC, D_trunc):
                                              D_trunc))
 
 
-def trotter_step(gp: GaugePEPS, g, t_hop, m, tau, D_trunc):
+def trotter_step(gp: GaugePEPS, g, t_hop, m, tau, D_trunc, mu=0.0):
     """trotter_step! (finite_peps_ground_state.jl:498): h-half, v-half, plaquettes, v-half, h-half."""
-    p = _params(g, t_hop, m, tau, D_trunc, 1, 0)
+    p = _params(g, t_hop, m, tau, D_trunc, 1, 0, mu=mu)
     gp.ctx.check(lib().peps_trotter_step(gp.h, PEPS_STEP_DENSE, C.byref(p)))
+
+
+def realtime_trotter_step(gp: GaugePEPS, g, t_hop, m, dt, D_trunc):
+    """realtime_trotter_step! (finite_peps_quench.jl:81-109): exp(-i dt/2 H_h),
+    exp(-i dt H_v), exp(-i dt/2 H_h) through the dense bond updates; the
+    state must be complex."""
+    p = _params(g, t_hop, m, dt, D_trunc, 1, 0)
+    gp.ctx.check(lib().peps_trotter_step(gp.h, PEPS_STEP_REALTIME, C.byref(p)))
+
+
+def apply_site_op(gp: GaugePEPS, ix, iy, O):
+    """apply_site_op! (finite_peps_quench.jl:228-234): T(ix,iy) <- O T on the
+    physical index; the flux labels are undefined afterwards."""
+    a = _as_array(O, gp.is_complex)
+    gp.ctx.check(lib().peps_apply_site_op(gp.h, ix, iy, _dptr(a)))
 
 
 def expect_site(gp: GaugePEPS, ix, iy, O):

@@ -1474,13 +1474,14 @@ def update_plaquette(gp, ix, iy, GA, GB, GC, D_trunc, recompress=True):
     update_bond_v(gp, ix, iy, np.eye(dpA * dpC, dtype=np.complex128), D_trunc)
 
 
-def trotter_step_dense(gp, g, t_hop, m, tau, D_trunc):
+def trotter_step_dense(gp, g, t_hop, m, tau, D_trunc, mu=0.0):
     """trotter_step!, finite_peps_ground_state.jl:498-541:
-    h-half (L->R) -> v-half (B->T) -> plaquettes -> v-half (T->B) -> h-half (R->L)."""
+    h-half (L->R) -> v-half (B->T) -> plaquettes -> v-half (T->B) -> h-half (R->L);
+    mu is the chemical potential of H_merged (:507,514,528,535)."""
     nx, ny, gd = gp.nx, gp.ny, gp.gd
     GA, GB, GC = decompose_plaquette_LShape(gate(H_plaquette_gauge(gd, g), tau), gd, cutoff=SVD_CUT)
-    gh = lambda ix, iy: gate(H_merged_h_site(ix, iy, nx, ny, gd, g, t_hop, m), tau / 2)
-    gv = lambda ix, iy: gate(H_merged_v_site(ix, iy, nx, ny, gd, g, t_hop, m), tau / 2)
+    gh = lambda ix, iy: gate(H_merged_h_site(ix, iy, nx, ny, gd, g, t_hop, m, mu), tau / 2)
+    gv = lambda ix, iy: gate(H_merged_v_site(ix, iy, nx, ny, gd, g, t_hop, m, mu), tau / 2)
     for iy in range(1, ny + 1):
         for ix in range(1, nx):
             update_bond_h(gp, ix, iy, gh(ix, iy), D_trunc)
@@ -1496,6 +1497,39 @@ def trotter_step_dense(gp, g, t_hop, m, tau, D_trunc):
     for iy in range(ny, 0, -1):
         for ix in range(nx - 1, 0, -1):
             update_bond_h(gp, ix, iy, gh(ix, iy), D_trunc)
+
+
+def realtime_gate(H, dt):
+    """exp(-i dt H), the real-time gate of realtime_trotter_step!
+    (finite_peps_quench.jl:88-89: exp(-tau H) at tau = i dt), from the
+    spectral decomposition of the real symmetric H (Pade on a complex
+    324 x 324 matrix costs ~50 times more)."""
+    w, V = np.linalg.eigh(np.asarray(H))
+    return (V * np.exp(-1j * dt * w)) @ V.conj().T
+
+
+def realtime_trotter_step(gp, g, t_hop, m, dt, D_trunc):
+    """realtime_trotter_step!, finite_peps_quench.jl:81-109: the dense
+    update_bond_h!/v! with exp(-i dt/2 H_h) (iy ascending, ix ascending),
+    exp(-i dt H_v) (iy ascending, ix ascending) and exp(-i dt/2 H_h) (iy
+    ascending, ix descending)."""
+    nx, ny, gd = gp.nx, gp.ny, gp.gd
+    gh = lambda ix, iy: realtime_gate(H_merged_h_site(ix, iy, nx, ny, gd, g, t_hop, m), dt / 2)
+    for iy in range(1, ny + 1):
+        for ix in range(1, nx):
+            update_bond_h(gp, ix, iy, gh(ix, iy), D_trunc)
+    for iy in range(1, ny):
+        for ix in range(1, nx + 1):
+            update_bond_v(gp, ix, iy, realtime_gate(H_merged_v_site(ix, iy, nx, ny, gd, g, t_hop, m), dt), D_trunc)
+    for iy in range(1, ny + 1):
+        for ix in range(nx - 1, 0, -1):
+            update_bond_h(gp, ix, iy, gh(ix, iy), D_trunc)
+
+
+def apply_site_op(gp, ix, iy, O):
+    """apply_site_op!, finite_peps_quench.jl:228-234: T <- O * reshape(T, dp, :)."""
+    T = gp.T(ix, iy)
+    gp.setT(ix, iy, jr(np.asarray(O) @ jr(T, T.shape[0], -1), *T.shape))
 
 
 def full_update_bond_h_v2(gp, ix, iy, G, D_trunc):

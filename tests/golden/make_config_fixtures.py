@@ -17,9 +17,14 @@ tests compare the library with them.
       (g=1, t=1, m=0.25, tau=0.05, D_max=4, chi=64, 30 ALS sweeps); the
       energy (compute_energy_v4, chi=64), <n_f> and the Gauss-law violation
       after every step.
+  v4spread: the same run from the same start with every site tensor
+      multiplied entrywise by (1 + 1e-14 r), r ~ N(0,1) from seeds 1 and 2
+      (v4_spread_<seed>.npz): the oracle's own sensitivity to rounding-level
+      input changes, which bounds how far two correct FP64 runs of the
+      chaotic ALS trajectory can drift apart.
 
-Usage: python tests/golden/make_config_fixtures.py c1|c2|v4  (c2 takes about
-an hour of CPU time, v4 about half an hour)."""
+Usage: python tests/golden/make_config_fixtures.py c1|c2|v4|v4spread [seed]
+(c2 takes about an hour of CPU time, v4 about half an hour per run)."""
 import os
 import sys
 
@@ -70,8 +75,11 @@ def rescale(gp, chi):
     return f_tot
 
 
-def v4_selftest():
+def v4_selftest(perturb_seed=None):
     gp = O.init_gauge_peps(3, 4, 3, O.staggered_charges(3, 4), noise=0.1, seed=1)
+    if perturb_seed is not None:
+        rng = np.random.default_rng(perturb_seed)
+        gp.tensors = [[t * (1.0 + 1e-14 * rng.standard_normal(t.shape)) for t in col] for col in gp.tensors]
     E, nf, forbidden = [], [], []
     for s in range(8):
         O.trotter_step_v4(gp, 1.0, 1.0, 0.25, 0.05, 4, 64, 30)
@@ -79,15 +87,19 @@ def v4_selftest():
         nf.append(O.mean_nf(gp))
         forbidden.append(O.forbidden_norm(gp))
         print(f"v4: step {s + 1}/8 E = {E[-1]:.15f} <n_f> = {nf[-1]:.12f}", flush=True)
-    np.savez_compressed(os.path.join(HERE, "v4_selftest.npz"), energy=np.array(E), mean_nf=np.array(nf),
+    name = "v4_selftest.npz" if perturb_seed is None else f"v4_spread_{perturb_seed}.npz"
+    np.savez_compressed(os.path.join(HERE, name), energy=np.array(E), mean_nf=np.array(nf),
                         forbidden_norm=np.array(forbidden), nx=3, ny=4, gd=3, noise=0.1, seed=1,
-                        g=1.0, t=1.0, m=0.25, tau=0.05, D_max=4, chi=64, n_als=30)
-    print("v4: wrote v4_selftest.npz", flush=True)
+                        g=1.0, t=1.0, m=0.25, tau=0.05, D_max=4, chi=64, n_als=30,
+                        perturbation=(0.0 if perturb_seed is None else 1e-14))
+    print(f"v4: wrote {name}", flush=True)
 
 
-def main(which):
+def main(which, arg=None):
     if which == "v4":
         return v4_selftest()
+    if which == "v4spread":
+        return v4_selftest(int(arg or 1))
     if which == "c1":
         nx, D, steps, chi = 4, 2, 200, 16
     else:
@@ -118,4 +130,4 @@ def main(which):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "c1")
+    main(sys.argv[1] if len(sys.argv) > 1 else "c1", sys.argv[2] if len(sys.argv) > 2 else None)

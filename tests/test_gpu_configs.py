@@ -158,17 +158,23 @@ def test_v4_update_selftest_reference_settings(ctx):
     default settings: 3x4, dg=1 (gd=3), m=0.25, D_max=4, chi=64, 30 ALS
     sweeps, 8 full-update steps at tau=0.05 from init_gauge_peps(noise=0.1,
     seed 1).  The reference's checks -- Gauss law (forbidden_norm < 1e-9) and
-    E_last <= E_first + 1e-6 -- plus the energy after every step against the
-    oracle's run from the same state (tests/golden/v4_selftest.npz): the
-    first step to 1e-10, the later ones to 1e-4 relative (rounding differences
-    of the two FP64 runs grow through the ALS sweeps; measured <= 1e-5).
-    The reference's third check, |<n_f> - 0.5| < 1e-6, holds for its own
-    random draw only: the lambda-weighted local density
-    (finite_peps_ground_state.jl:550-580) is not conserved by the update, and
-    the oracle's run from seed 1 drifts to 5e-5 of half filling, so <n_f> is
-    held to the oracle's values to 1e-5 and to half filling to 1e-4."""
+    E_last <= E_first + 1e-6 -- plus the energy and <n_f> after every step
+    against the oracle's run from the same state (tests/golden/
+    v4_selftest.npz).  The ALS trajectory is chaotic: the oracle's own runs
+    from the same start perturbed by 1e-14 relative (v4_spread_1/2.npz) drift
+    apart by up to 5e-4 in E and 7e-5 in <n_f> within eight steps, so after
+    step k the GPU must lie within 10x the largest drift those runs showed up
+    to step k (the first step to 1e-10 relative).  The reference's third
+    check, |<n_f> - 0.5| < 1e-6, holds for its own random draw only: the
+    lambda-weighted density (finite_peps_ground_state.jl:550-580) is not
+    conserved by the update and the oracle's runs reach 6e-5 from half
+    filling, so <n_f> is held within 10x that."""
     from lgt_amd import lattice, peps
     gold = _load("v4_selftest.npz")
+    spread = [_load(f"v4_spread_{k}.npz") for k in (1, 2)]
+    dE = np.maximum.accumulate(np.max([np.abs(z["energy"] - gold["energy"]) for z in spread], axis=0))
+    dN = np.maximum.accumulate(np.max([np.abs(z["mean_nf"] - gold["mean_nf"]) for z in spread], axis=0))
+    half = max(np.abs(z["mean_nf"] - 0.5).max() for z in spread + [gold])
     gp = peps.init_gauge_peps(3, 4, 3, lattice.staggered_charges(3, 4), noise=0.1, seed=1, ctx=ctx)
     Es, nfs = [], []
     for _ in range(8):
@@ -178,10 +184,9 @@ def test_v4_update_selftest_reference_settings(ctx):
     print(f"v4 self-test energies: {Es}, <n_f>: {nfs}")
     assert peps.forbidden_norm(gp) < 1e-9
     assert Es[-1] <= Es[0] + 1e-6
-    E_ref = gold["energy"]
+    E_ref, N_ref = gold["energy"], gold["mean_nf"]
     assert abs(Es[0] - E_ref[0]) < 1e-10 * abs(E_ref[0])
-    for e, r in zip(Es, E_ref):
-        assert abs(e - r) < 1e-4 * abs(r)
-    for n, r in zip(nfs, gold["mean_nf"]):
-        assert abs(n - r) < 1e-5
-        assert abs(n - 0.5) < 1e-4
+    for k in range(8):
+        assert abs(Es[k] - E_ref[k]) <= max(1e-10 * abs(E_ref[k]), 10 * dE[k]), (k, Es[k], E_ref[k], dE[k])
+        assert abs(nfs[k] - N_ref[k]) <= max(1e-12, 10 * dN[k]), (k, nfs[k], N_ref[k], dN[k])
+        assert abs(nfs[k] - 0.5) < 10 * half
