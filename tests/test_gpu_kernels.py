@@ -221,13 +221,16 @@ def test_als_init_leading_triplets_at_full_size(ctx, noise):
     """The ALS initialisation at the C3 bond shape (theta 48 x 8 x 8 x 48,
     charge sectors of 192 rows) takes only the leading D_max triplets of each
     sector (randomized subspace iteration, residual-checked).  With a
-    dominant rank-6 part (noise 1e-4, as a gated bond looks) that path is
-    taken; with pure noise (1.0) it falls back to the full SVD.  Either way
-    the result equals the oracle's full-SVD initialisation: same bond fluxes,
-    X Y to 1e-10, the same initial cost, and after 2 ALS sweeps the same
-    cost to 1e-8."""
+    dominant rank-6 part (noise 1e-4, as a gated bond looks) or pure noise
+    (1.0), the cut at D_max falls inside a flat noise spectrum that the
+    randomized iteration cannot certify, and the leading triplets come from
+    the eigenvectors of the Jordan-Wielandt matrix [[0, B], [B^T, 0]]
+    instead (no full Jacobi SVD either way).  The result equals the oracle's
+    full-SVD initialisation: same bond fluxes, X Y to 1e-10, the same
+    initial cost, and after 2 ALS sweeps the same cost to 1e-8."""
     from lgt_amd import peps
     from oracle import peps_oracle as O
+    ctx.stats_reset()
     rng = np.random.default_rng(11)
     nA, dp, D = 48, 8, 6
     eR = np.array([1, -1] * (dp // 2))
@@ -247,6 +250,10 @@ def test_als_init_leading_triplets_at_full_size(ctx, noise):
             assert np.abs(XYg - XYo).max() <= 1e-10 * np.abs(XYo).max()
         else:
             assert abs(ig["cost1"] - io["cost1"]) <= 1e-8 * abs(io["cost1"])
+    st = {k: v["launches"] for k, v in ctx.stats().items() if k.startswith("svd_leading")}
+    print(f"leading-triplet paths (noise {noise}): {st}", flush=True)
+    assert st.get("svd_leading[fallback]", 0) == 0
+    assert st.get("svd_leading[jordan-wielandt]", 0) + st.get("svd_leading[iterations]", 0) > 0
 
 
 def _blocky_psd(rng, sizes, coupling):
