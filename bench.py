@@ -44,7 +44,7 @@ METRIC = "full-update steps/s at 8x8 D=6 chi=144; speedup vs plain FP64 GPU at d
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 PEAK_HBM_GBS = 8000.0
 # the dominant kernel of each tier: its launches are timed as one class
-GEMM_CLASS = {"fp64": "gemm_f64", "fp32": "gemm_f32_tiled", "bf16": "gemm_bf16"}
+GEMM_CLASS = {"fp64": "gemm_f64_tiled", "fp32": "gemm_f32_tiled", "bf16": "gemm_bf16"}
 # HBM bytes per launch of a kernel class measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE over the timed step of a workload (copy-calibrated factors 2.0
 # and 1.0), keyed by (nx, ny, D, chi, gd, tier, class); profiles/ holds the
@@ -243,8 +243,9 @@ def summarize_rocprof(d, log_path=None):
         m = re.search(r"gemm_kernel<(\w+), (\d+),", name)
         if m:
             return "gemm_kernel PREC=" + m.group(2)
-        if "gemm_f32_tiled" in name:
-            return "gemm_f32_tiled"
+        for tiled in ("gemm_f32_tiled", "gemm_f64_tiled"):
+            if tiled in name:
+                return tiled
         return re.sub(r"\(.*", "", name).replace("void ", "")[:80]
 
     win = None

@@ -55,7 +55,7 @@ __device__ __forceinline__ typename LdsT<PREC>::T cvt(double x) {
 // MFMA 16x16 operand layout (lane l): A row = l%16, k-group = l/16;
 // B col = l%16, k-group = l/16; output row = 4*(l/16)+v, col = l%16.
 template <bool CPLX, int PREC, bool AK, bool BKF, int WM, int WN>
-__global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t K,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_kernel(int64_t M, int64_t N, int64_t K,
                                                    const double* __restrict__ A, int64_t lda,
                                                    const double* __restrict__ B, int64_t ldb,
                                                    double* __restrict__ C, int64_t ldc, int conjA,
@@ -79,8 +79,25 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t
   const int wave = tid >> 6;
   const int wm = (wave & 1) * (16 * WM);
   const int wn = (wave >> 1) * (16 * WN);
-  const int64_t m0 = int64_t(blockIdx.x) * BM;
-  const int64_t n0 = int64_t(blockIdx.y) * BN;
+  // tile of this block: the linear block index is remapped so that the
+  // blocks the hardware deals to one XCD (round robin over 8) are
+  // consecutive, then tiles are visited in groups of 8 tile rows: the blocks
+  // resident on one XCD share a few A and B panels, whose current k-slices
+  // stay in that XCD's L2 while the blocks advance through K together
+  int64_t m0, n0;
+  {
+    const int tilesM = int((M + BM - 1) / BM), tilesN = int((N + BN - 1) / BN);
+    const int nt = tilesM * tilesN;
+    int w = blockIdx.x;
+    const int q = nt >> 3, r = nt & 7, x = w & 7, i = w >> 3;
+    w = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+    constexpr int GM = 8;
+    const int per_group = GM * tilesN;
+    const int first = (w / per_group) * GM;
+    const int gs = min(tilesM - first, GM);
+    m0 = int64_t(first + (w % per_group) % gs) * BM;
+    n0 = int64_t((w % per_group) / gs) * BN;
+  }
   const int64_t kbeg = int64_t(blockIdx.z) * kchunk;
   const int64_t kend = min(K, kbeg + kchunk);
   double* Cz = C + int64_t(blockIdx.z) * wstride;
@@ -434,6 +451,177 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
 }
 
+// Real GEMM of the FP64 tier for outputs with M, N >= 128.
+//
+// 128x128 output tile per 256-thread block (2x2 waves of 64x64, 4x4
+// v_mfma_f64_16x16x4f64 each), k-tiles of 16 doubles through two LDS buffers
+// (one barrier per k-tile; the next tile's global loads are in flight during
+// the 64 MFMAs of the current one).  LDS rows hold the 16 consecutive k of
+// one m (or n), padded to 18 doubles (144 bytes, the FP32 kernel's row
+// geometry): lane group g supplies k = 4g + s in MFMA step s, so a lane
+// reads the four operands of one row for the whole k-tile as two 16-byte
+// LDS reads.  Operands are loaded four consecutive k at a time per row
+// (coalesced along the contiguous global dim), stored as two 16-byte LDS
+// writes; interior tiles and full k-tiles load without per-element guards.
+// Tiles are visited in groups of 8 tile rows with the XCD-contiguous block
+// remap of the FP32 kernel.  The accumulators are written straight to C
+// (each lane holds rows 4v + lane/16 of column lane%16 of a 16x16 tile).
+template <bool AK, bool BKF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_f64_tiled(int64_t M, int64_t N, int64_t K,
+                                                      const double* __restrict__ A, int64_t lda,
+                                                      const double* __restrict__ B, int64_t ldb,
+                                                      double* __restrict__ C, int64_t ldc, double beta,
+                                                      int64_t kchunk, int64_t wstride, int tilesM,
+                                                      int tilesN) {
+  constexpr int BM = 128, BN = 128, BK = 16, LDK = BK + 2;
+  constexpr int BUF = (BM + BN) * LDK;
+  __shared__ __align__(16) double smem[2 * BUF];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+
+  const int nt = tilesM * tilesN;
+  int w = blockIdx.x;
+  {
+    const int q = nt >> 3, r = nt & 7, x = w & 7, i = w >> 3;
+    w = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+  }
+  constexpr int GM = 8;
+  const int per_group = GM * tilesN;
+  const int first = (w / per_group) * GM;
+  const int gs = min(tilesM - first, GM);
+  const int64_t m0 = int64_t(first + (w % per_group) % gs) * BM;
+  const int64_t n0 = int64_t((w % per_group) / gs) * BN;
+
+  const int64_t kbeg = int64_t(blockIdx.z) * kchunk;
+  const int64_t kend = min(K, kbeg + kchunk);
+  double* Cz = C + int64_t(blockIdx.z) * wstride;
+
+  v4d acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  // two groups of 4 consecutive k of one row per operand and thread
+  double ra[2][4], rb[2][4];
+  auto arow = [&](int e) { return AK ? (tid >> 2) + 64 * e : (tid & 127); };
+  auto acol = [&](int e) { return AK ? 4 * (tid & 3) : 4 * ((tid >> 7) + 2 * e); };
+  auto brow = [&](int e) { return BKF ? (tid >> 2) + 64 * e : (tid & 127); };
+  auto bcol = [&](int e) { return BKF ? 4 * (tid & 3) : 4 * ((tid >> 7) + 2 * e); };
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int64_t gm = m0 + arow(e), gn = n0 + brow(e);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t gk = k0 + acol(e) + j, gkb = k0 + bcol(e) + j;
+        ra[e][j] = (gm < M && gk < kend) ? A[AK ? gk + gm * lda : gm + gk * lda] : 0.0;
+        rb[e][j] = (gn < N && gkb < kend) ? B[BKF ? gkb + gn * ldb : gn + gkb * ldb] : 0.0;
+      }
+    }
+  };
+  const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+  const int kgrp = __builtin_amdgcn_readfirstlane(tid >> 7);
+  const int64_t laneA = AK ? int64_t(tid >> 2) * lda + 4 * (tid & 3) : int64_t(tid & 127);
+  const int64_t laneB = BKF ? int64_t(tid >> 2) * ldb + 4 * (tid & 3) : int64_t(tid & 127);
+  auto gload_fast = [&](int64_t k0) {
+    const double* Ab = A + (AK ? m0 * lda + k0 : m0 + k0 * lda) + laneA;
+    const double* Bb = B + (BKF ? n0 * ldb + k0 : n0 + k0 * ldb) + laneB;
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        ra[e][j] = Ab[AK ? int64_t(64 * e) * lda + j : int64_t(4 * (kgrp + 2 * e) + j) * lda];
+        rb[e][j] = Bb[BKF ? int64_t(64 * e) * ldb + j : int64_t(4 * (kgrp + 2 * e) + j) * ldb];
+      }
+  };
+  auto load = [&](int64_t k0) {
+    if (interior && k0 + BK <= kend) gload_fast(k0);
+    else gload(k0);
+  };
+  auto lstore = [&](int buf) {
+    double* As = smem + buf * BUF;
+    double* Bs = As + BM * LDK;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      double* pa = As + arow(e) * LDK + acol(e);
+      double* pb = Bs + brow(e) * LDK + bcol(e);
+      *reinterpret_cast<double2*>(pa) = double2{ra[e][0], ra[e][1]};
+      *reinterpret_cast<double2*>(pa + 2) = double2{ra[e][2], ra[e][3]};
+      *reinterpret_cast<double2*>(pb) = double2{rb[e][0], rb[e][1]};
+      *reinterpret_cast<double2*>(pb + 2) = double2{rb[e][2], rb[e][3]};
+    }
+  };
+
+  const int64_t ntk = (kend - kbeg + BK - 1) / BK;
+  if (ntk > 0) {
+    load(kbeg);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int64_t t = 0; t < ntk; ++t) {
+    const bool more = t + 1 < ntk;
+    if (more) load(kbeg + (t + 1) * BK);
+    const double* As = smem + (t & 1) * BUF;
+    const double* Bs = As + BM * LDK;
+    // two halves of the k-tile (steps s = 2h, 2h + 1): one 16-byte LDS read
+    // per operand row and half
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double2 a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = *reinterpret_cast<const double2*>(As + (wm + i * 16 + lr) * LDK + 4 * lg + 2 * h);
+        b[i] = *reinterpret_cast<const double2*>(Bs + (wn + i * 16 + lr) * LDK + 4 * lg + 2 * h);
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(s ? a[i].y : a[i].x, s ? b[j].y : b[j].x, acc[i][j],
+                                                             0, 0, 0);
+    }
+    if (more) lstore((t + 1) & 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t col = n0 + wn + j * 16 + lr;
+      if (col >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = m0 + wm + i * 16 + lg + 4 * v;
+        if (row >= M) continue;
+        const int64_t off = row + col * ldc;
+        double x = acc[i][j][v];
+        if (beta != 0.0) x += beta * Cz[off];
+        Cz[off] = x;
+      }
+    }
+}
+
+static void launch_gemm_f64_tiled(bool ak, bool bk, hipStream_t st, int64_t M, int64_t N, int64_t K,
+                                  const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                                  int64_t ldc, double beta, int splits, int64_t kchunk, int64_t wstride) {
+  const int tm = int(cdiv(M, 128)), tn = int(cdiv(N, 128));
+  dim3 grid(unsigned(tm) * unsigned(tn), 1, splits);
+  if (ak && bk)
+    hipLaunchKernelGGL((gemm_f64_tiled<true, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+  else if (ak)
+    hipLaunchKernelGGL((gemm_f64_tiled<true, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+  else if (bk)
+    hipLaunchKernelGGL((gemm_f64_tiled<false, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+  else
+    hipLaunchKernelGGL((gemm_f64_tiled<false, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk, wstride, tm, tn);
+}
+
 template <typename TIN>
 static void launch_gemm_f32_tiled(bool ak, bool bk, hipStream_t st, int64_t M, int64_t N, int64_t K,
                                   const TIN* A, int64_t lda, const TIN* B, int64_t ldb, double* C,
@@ -466,7 +654,8 @@ static void launch_gemm_t(hipStream_t st, int64_t M, int64_t N, int64_t K, const
                           int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
                           int cA, int cB, double beta, int splits, int64_t kchunk,
                           int64_t wstride) {
-  dim3 grid(cdiv(M, 32 * WM), cdiv(N, 32 * WN), splits);
+  // one linear block index per output tile (the kernel maps it to a tile)
+  dim3 grid(unsigned(int64_t(cdiv(M, 32 * WM)) * cdiv(N, 32 * WN)), 1, splits);
   hipLaunchKernelGGL((gemm_kernel<CPLX, PREC, AK, BKF, WM, WN>), grid, dim3(256), 0, st, M, N, K,
                      A, lda, B, ldb, C, ldc, cA, cB, beta, kchunk, wstride);
 }
@@ -491,10 +680,16 @@ static void dispatch_tile(bool big, bool ak, bool bk, hipStream_t st, int64_t M,
                           int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb,
                           double* C, int64_t ldc, int cA, int cB, double beta, int splits,
                           int64_t kchunk, int64_t wstride) {
-  if (big && !CPLX)
-    dispatch_layout<CPLX, PREC, 4, 4>(ak, bk, st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
-  else
-    dispatch_layout<CPLX, PREC, 2, 2>(ak, bk, st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
+  // complex outputs always take 64x64 tiles (a 128x128 complex tile does
+  // not fit the registers of two waves per SIMD)
+  if constexpr (!CPLX) {
+    if (big) {
+      dispatch_layout<CPLX, PREC, 4, 4>(ak, bk, st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk,
+                                        wstride);
+      return;
+    }
+  }
+  dispatch_layout<CPLX, PREC, 2, 2>(ak, bk, st, M, N, K, A, lda, B, ldb, C, ldc, cA, cB, beta, splits, kchunk, wstride);
 }
 
 __global__ void zero_kernel(double* __restrict__ x, int64_t n) {
@@ -563,11 +758,14 @@ void gemm_impl(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double*
   const double bytes = double(esz * 8) * (double(M) * K + double(K) * N + double(M) * N);
   const bool tiled = !cplx && c.tier == TIER_FP32 && M >= 128 && N >= 128 &&
                      int64_t(cdiv(M, 128)) * cdiv(N, 128) < (int64_t(1) << 31);
+  const bool tiled64 = !cplx && c.tier == TIER_FP64 && M >= 128 && N >= 128 &&
+                       int64_t(cdiv(M, 128)) * cdiv(N, 128) < (int64_t(1) << 31);
   std::string cls;
   const bool cast = wsp.cast && tiled && M >= kCastMin && N >= kCastMin;
   if (wsp.timed) {
     cls = tiled ? "gemm_f32_tiled"
-                : (c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16"));
+                : tiled64 ? "gemm_f64_tiled"
+                          : (c.tier == TIER_FP64 ? "gemm_f64" : (c.tier == TIER_FP32 ? "gemm_f32" : "gemm_bf16"));
     if (c.gemm_shapes)
       cls += " " + std::to_string(M) + "x" + std::to_string(N) + "x" + std::to_string(K) + (cplx ? "c" : "") +
              (a_kfast ? " ak" : " am") + (b_kfast ? " bk" : " bn") + " s" + std::to_string(splits);
@@ -615,6 +813,8 @@ void gemm_impl(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double*
     }
     kbeta = 0.0;
   }
+  PEPS_CHECK(int64_t(cdiv(M, 64)) * cdiv(N, 64) < (int64_t(1) << 31), PS_EUNSUPPORTED,
+             "gemm: more than 2^31 output tiles");
   auto go = [&](auto cplx_tag, auto prec_tag) {
     constexpr bool CP = decltype(cplx_tag)::value;
     constexpr int PR = decltype(prec_tag)::value;
@@ -630,6 +830,9 @@ void gemm_impl(Ctx& c, bool cplx, int64_t M, int64_t N, int64_t K, const double*
   } else if (tiled) {
     launch_gemm_f32_tiled<double>(a_kfast, b_kfast, c.stream, M, N, K, A, lda, B, ldb, out, M, kbeta, splits,
                                   kchunk, wstride);
+  } else if (tiled64) {
+    launch_gemm_f64_tiled(a_kfast, b_kfast, c.stream, M, N, K, A, lda, B, ldb, out, M, kbeta, splits, kchunk,
+                          wstride);
   } else if (cplx) {
     if (c.tier == TIER_FP64) go(std::true_type{}, T0{});
     else if (c.tier == TIER_FP32) go(std::true_type{}, T1{});
