@@ -384,6 +384,14 @@ int32_t peps_solve_herm(peps_ctx* ctx, int32_t is_complex, int64_t n, int64_t nr
 int32_t peps_svd(peps_ctx* ctx, int32_t is_complex, int64_t m, int64_t n, const double* A,
                  double* U, double* S, double* Vt);
 
+/* Thin QR used by the boundary compression, the QR reductions of the bond
+   environments and the subspace iterations (LinearAlgebra.qr in the
+   reference, finite_peps_boundary_mps.jl:78-84, :325-351): Householder
+   reflectors with LAPACK's sign convention; real panels wider than 64
+   columns in blocked form.  A (m x n); outputs Q (m x k), R (k x n),
+   k = min(m, n). */
+int32_t peps_qr(peps_ctx* ctx, int32_t is_complex, int64_t m, int64_t n, const double* A, double* Q, double* R);
+
 /* ── model (u1_lgt_hamiltonian.jl:829-863, :396) ─────────────────────────── */
 /* H_merged_{h,v}_site (tau == 0) or its gate exp(-tau H); pass out == NULL to
    query n.  out is n*n real, column-major. */

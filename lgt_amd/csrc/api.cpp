@@ -978,6 +978,16 @@ int32_t peps_svd(peps_ctx* ctx, int32_t is_complex, int64_t m, int64_t n, const 
   });
 }
 
+int32_t peps_qr(peps_ctx* ctx, int32_t is_complex, int64_t m, int64_t n, const double* A, double* Q, double* R) {
+  if (!ctx || !A || !Q || !R || m < 1 || n < 1) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    Tensor a = ctx->c->upload({m, n}, is_complex != 0, A);
+    peps::QR f = peps::qr_thin(*ctx->c, a);
+    ctx->c->download(f.Q, Q);
+    ctx->c->download(f.R, R);
+  });
+}
+
 int32_t peps_bond_operator(int32_t orient, int32_t ix, int32_t iy, int32_t nx, int32_t ny,
                            int32_t gd, double g, double t_hop, double m, double tau, int32_t* n,
                            double* out) {

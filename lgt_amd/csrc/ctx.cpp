@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 namespace peps {
 
@@ -82,6 +83,8 @@ Ctx::Ctx(int dev) : device(dev) {
     throw Error(PS_EHIP, "rocblas_create_handle failed");
   rocblas_set_stream(blas, stream);
   pool.reset(new Pool(dev));
+  PEPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&stage_host), kStageBytes, hipHostMallocDefault));
+  PEPS_HIP(hipEventCreateWithFlags(&stage_ev, hipEventDisableTiming));
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   if (const char* v = std::getenv("PEPS_HIP_VERBOSE")) verbose = std::atoi(v);
@@ -91,6 +94,7 @@ Ctx::Ctx(int dev) : device(dev) {
   if (const char* v = std::getenv("PEPS_HIP_ALS_BLOCKS")) als_blocks = std::atoi(v) != 0;
   if (const char* v = std::getenv("PEPS_HIP_EIG_DIRECT_MAX")) eig_direct_max = std::max(0, std::atoi(v));
   if (const char* v = std::getenv("PEPS_HIP_GEMM_SHAPES")) gemm_shapes = std::atoi(v) != 0;
+  if (const char* v = std::getenv("PEPS_HIP_QR")) qr_backend = std::string(v) == "blocked" ? 0 : 1;
 }
 
 Ctx::~Ctx() {
@@ -105,6 +109,8 @@ Ctx::~Ctx() {
   pool.reset();
   if (blas) rocblas_destroy_handle(blas);
   if (stream) (void)hipStreamDestroy(stream);
+  if (stage_ev) (void)hipEventDestroy(stage_ev);
+  if (stage_host) (void)hipHostFree(stage_host);
 }
 
 static hipEvent_t take_event(Ctx& c) {
@@ -169,12 +175,29 @@ Tensor Ctx::zeros(std::vector<int64_t> dims, bool cplx) {
   return t;
 }
 
+void Ctx::h2d_async(void* dst, const void* host, size_t bytes) {
+  if (!bytes) return;
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (!stage_host || need > kStageBytes / 4) {
+    PEPS_HIP(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, stream));
+    PEPS_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  if (stage_head + need > kStageBytes) {
+    // every copy staged so far was enqueued before stage_ev's last record
+    if (stage_used) PEPS_HIP(hipEventSynchronize(stage_ev));
+    stage_head = 0;
+  }
+  std::memcpy(stage_host + stage_head, host, bytes);
+  PEPS_HIP(hipMemcpyAsync(dst, stage_host + stage_head, bytes, hipMemcpyHostToDevice, stream));
+  PEPS_HIP(hipEventRecord(stage_ev, stream));
+  stage_used = true;
+  stage_head += need;
+}
+
 Tensor Ctx::upload(std::vector<int64_t> dims, bool cplx, const double* host) {
   Tensor t = alloc(std::move(dims), cplx);
-  if (t.bytes()) {
-    PEPS_HIP(hipMemcpyAsync(t.data(), host, t.bytes(), hipMemcpyHostToDevice, stream));
-    PEPS_HIP(hipStreamSynchronize(stream));
-  }
+  h2d_async(t.data(), host, t.bytes());
   return t;
 }
 
@@ -197,11 +220,7 @@ void* Ctx::scratch_upload(const void* host, size_t bytes, std::shared_ptr<Buffer
   b->pool = pool.get();
   b->bytes = std::max<size_t>(bytes, 8);
   b->ptr = pool->get(b->bytes);
-  if (bytes) {
-    PEPS_HIP(hipMemcpyAsync(b->ptr, host, bytes, hipMemcpyHostToDevice, stream));
-    // the host array may be a temporary: wait for the copy before returning
-    PEPS_HIP(hipStreamSynchronize(stream));
-  }
+  h2d_async(b->ptr, host, bytes);
   *keep = b;
   return b->ptr;
 }

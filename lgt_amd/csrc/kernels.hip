@@ -2373,6 +2373,106 @@ __global__ __launch_bounds__(QR_THREADS) void qr_form_q_kernel(const typename JT
 
 int qr_small_max_cols() { return QR_SMALL_N; }
 
+// V (m x b): the unit lower trapezoid of a factored panel P (m x b, real):
+// V(i, j) = P(i, j) below the diagonal, 1 on it, 0 above.
+__global__ void householder_v_kernel(const double* __restrict__ P, double* __restrict__ V, int64_t m, int b) {
+  const int64_t n = m * b;
+  for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = q % m, j = q / m;
+    V[q] = i > j ? P[q] : (i == j ? 1.0 : 0.0);
+  }
+}
+
+// T (b x b, upper triangular) of the compact WY form H_0 H_1 ... H_{b-1} =
+// I - V T V^T (LAPACK dlarft, forward, columnwise) from G = V^T V and the
+// reflector scalars: T(i, i) = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).
+// One workgroup of QR_SMALL_N threads; T is built in LDS column by column.
+__global__ __launch_bounds__(QR_SMALL_N) void householder_t_kernel(const double* __restrict__ G,
+                                                                   const double* __restrict__ tau,
+                                                                   double* __restrict__ T, int b) {
+  __shared__ double Ts[QR_SMALL_N * QR_SMALL_N];
+  __shared__ double Gs[QR_SMALL_N * QR_SMALL_N];
+  const int r = threadIdx.x;
+  for (int q = r; q < QR_SMALL_N * QR_SMALL_N; q += QR_SMALL_N) {
+    Ts[q] = 0.0;
+    Gs[q] = q < b * b ? G[q] : 0.0;  // G (b x b) column-major, leading dimension b
+  }
+  __syncthreads();
+  for (int i = 0; i < b; ++i) {
+    double w = 0.0;
+    if (r < i)
+      for (int s2 = r; s2 < i; ++s2) w += Ts[r + s2 * QR_SMALL_N] * Gs[s2 + i * b];
+    // column i is written only after every thread has read columns < i
+    __syncthreads();
+    if (r < i) Ts[r + i * QR_SMALL_N] = -tau[i] * w;
+    if (r == i) Ts[i + i * QR_SMALL_N] = tau[i];
+    __syncthreads();
+  }
+  for (int q = r; q < b * b; q += QR_SMALL_N) T[q] = Ts[(q % b) + (q / b) * QR_SMALL_N];
+}
+
+// dst(i, j) = src(i, j) for i < rows, j < cols, both column-major with their
+// own leading dimensions (esz doubles per element).
+__global__ void copy_block_kernel(double* __restrict__ dst, int64_t ldd, const double* __restrict__ src,
+                                  int64_t lds, int64_t rows, int64_t cols, int esz) {
+  const int64_t n = rows * cols * esz;
+  for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t e = q % esz, ij = q / esz, i = ij % rows, j = ij / rows;
+    dst[(i + j * ldd) * esz + e] = src[(i + j * lds) * esz + e];
+  }
+}
+
+static void copy_block(Ctx& c, double* dst, int64_t ldd, const double* src, int64_t lds, int64_t rows,
+                       int64_t cols, bool cplx) {
+  if (rows <= 0 || cols <= 0) return;
+  const int esz = cplx ? 2 : 1;
+  const unsigned g = unsigned(std::min<int64_t>(cdiv(rows * cols * esz, 256), 16384));
+  hipLaunchKernelGGL(copy_block_kernel, dim3(g), dim3(256), 0, c.stream, dst, ldd, src, lds, rows, cols, esz);
+  PEPS_HIP(hipGetLastError());
+}
+
+Tensor sub_block(Ctx& c, const Tensor& M, int64_t r0, int64_t c0, int64_t rows, int64_t cols) {
+  PEPS_CHECK(M.rank() == 2 && r0 >= 0 && c0 >= 0 && r0 + rows <= M.dims[0] && c0 + cols <= M.dims[1], PS_ESHAPE,
+             "sub_block: range out of bounds");
+  Tensor B = c.alloc({rows, cols}, M.cplx);
+  const int esz = M.cplx ? 2 : 1;
+  copy_block(c, B.data(), rows, M.data() + (r0 + c0 * M.dims[0]) * esz, M.dims[0], rows, cols, M.cplx);
+  return B;
+}
+
+void put_block(Ctx& c, Tensor& M, int64_t r0, int64_t c0, const Tensor& B) {
+  PEPS_CHECK(M.rank() == 2 && B.rank() == 2 && M.cplx == B.cplx && r0 >= 0 && c0 >= 0 &&
+                 r0 + B.dims[0] <= M.dims[0] && c0 + B.dims[1] <= M.dims[1],
+             PS_ESHAPE, "put_block: range out of bounds");
+  const int esz = M.cplx ? 2 : 1;
+  copy_block(c, M.data() + (r0 + c0 * M.dims[0]) * esz, M.dims[0], B.data(), B.dims[0], B.dims[0], B.dims[1],
+             M.cplx);
+}
+
+Tensor householder_v(Ctx& c, const Tensor& P) {
+  PEPS_CHECK(P.rank() == 2 && !P.cplx && P.dims[1] <= QR_SMALL_N, PS_ESHAPE, "householder_v: real m x b panel");
+  const int64_t m = P.dims[0];
+  const int b = int(P.dims[1]);
+  Tensor V = c.alloc({m, int64_t(b)}, false);
+  if (m * b == 0) return V;
+  const unsigned g = unsigned(std::min<int64_t>(cdiv(m * b, 256), 16384));
+  hipLaunchKernelGGL(householder_v_kernel, dim3(g), dim3(256), 0, c.stream, P.data(), V.data(), m, b);
+  PEPS_HIP(hipGetLastError());
+  return V;
+}
+
+Tensor householder_t(Ctx& c, const Tensor& G, const Tensor& tau) {
+  const int b = int(G.dims[0]);
+  PEPS_CHECK(G.rank() == 2 && G.dims[1] == b && b <= QR_SMALL_N && !G.cplx && tau.numel() >= b, PS_ESHAPE,
+             "householder_t: real b x b Gram matrix, b <= 64");
+  Tensor T = c.alloc({int64_t(b), int64_t(b)}, false);
+  if (b == 0) return T;
+  hipLaunchKernelGGL(householder_t_kernel, dim3(1), dim3(QR_SMALL_N), 0, c.stream, G.data(), tau.data(), T.data(),
+                     b);
+  PEPS_HIP(hipGetLastError());
+  return T;
+}
+
 void qr_panel(Ctx& c, Tensor& W, Tensor& tau) {
   PEPS_CHECK(W.rank() == 2 && W.dims[1] <= QR_SMALL_N && W.dims[0] < (1ll << 40), PS_EUNSUPPORTED,
              "qr_panel: at most 64 columns");

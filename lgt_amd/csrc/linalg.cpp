@@ -100,6 +100,61 @@ QR zqr_large(Ctx& c, const Tensor& A, bool want_q) {
   return r;
 }
 
+// Blocked Householder QR of a real matrix wider than one panel (LAPACK
+// dgeqrf / dorgqr with the project's kernels): panels of 64 columns are
+// factored by the one-workgroup panel kernel, their reflectors put in the
+// compact WY form H = I - V T V^T (householder_v / householder_t), and the
+// trailing columns updated as A2 - V (T^T (V^T A2)) with three GEMMs; Q is
+// accumulated backwards from the first k columns of I the same way.  The
+// same reflectors as the unblocked algorithm, so R and Q are LAPACK's up to
+// rounding; about a dozen launches per panel where the library's blocked
+// routine makes several hundred small ones per call.
+QR dqr_blocked(Ctx& c, const Tensor& A, bool want_q) {
+  const int64_t m = A.dims[0], n = A.dims[1], k = std::min(m, n), nb = qr_small_max_cols();
+  Tensor W = copy_tensor(c, A);
+  struct Panel {
+    int64_t j0, b;
+    Tensor V, T;
+  };
+  std::vector<Panel> panels;
+  for (int64_t j0 = 0; j0 < k; j0 += nb) {
+    const int64_t b = std::min(nb, k - j0), mp = m - j0, n2 = n - j0 - b;
+    Tensor P = sub_block(c, W, j0, j0, mp, b);
+    Tensor taup = c.alloc({b}, false);
+    qr_panel(c, P, taup);
+    put_block(c, W, j0, j0, P);
+    Panel pn;
+    pn.j0 = j0;
+    pn.b = b;
+    pn.V = householder_v(c, P);
+    Tensor G = matmul(c, pn.V, pn.V, false, false, true, false);  // V^T V
+    pn.T = householder_t(c, G, taup);
+    if (n2 > 0) {
+      Tensor A2 = sub_block(c, W, j0, j0 + b, mp, n2);
+      Tensor X = matmul(c, pn.V, A2, false, false, true, false);  // V^T A2 (b x n2)
+      Tensor Y = matmul(c, pn.T, X, false, false, true, false);   // T^T X
+      scale(c, Y, -1.0);
+      gemm(c, false, mp, n2, b, pn.V.data(), false, false, Y.data(), true, false, A2.data(), 1.0);
+      put_block(c, W, j0, j0 + b, A2);
+    }
+    if (want_q) panels.push_back(std::move(pn));
+  }
+  QR r;
+  r.R = upper_trapezoid(c, W, k);
+  if (!want_q) return r;
+  r.Q = eye(c, m, k, false);
+  for (auto it = panels.rbegin(); it != panels.rend(); ++it) {
+    const int64_t mp = m - it->j0, kq = k - it->j0;
+    Tensor Q2 = sub_block(c, r.Q, it->j0, it->j0, mp, kq);
+    Tensor X = matmul(c, it->V, Q2, false, false, true, false);  // V^T Q2
+    Tensor Y = matmul(c, it->T, X);                             // T X
+    scale(c, Y, -1.0);
+    gemm(c, false, mp, kq, it->b, it->V.data(), false, false, Y.data(), true, false, Q2.data(), 1.0);
+    put_block(c, r.Q, it->j0, it->j0, Q2);
+  }
+  return r;
+}
+
 }  // namespace
 
 QR qr_thin(Ctx& c, const Tensor& A) {
@@ -119,6 +174,8 @@ QR qr_thin(Ctx& c, const Tensor& A) {
   QR r;
   if (n > qr_small_max_cols() && A.cplx) {
     r = zqr_large(c, A, true);
+  } else if (n > qr_small_max_cols() && c.qr_backend == 0) {
+    r = dqr_blocked(c, A, true);
   } else {
     Tensor W = copy_tensor(c, A);
     Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
@@ -151,13 +208,17 @@ Tensor qr_r_only(Ctx& c, const Tensor& A) {
   if (n > qr_small_max_cols() && A.cplx) {
     R = zqr_large(c, A, false).R;
   } else {
-    Tensor W = copy_tensor(c, A);
-    Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
-    if (k > 0) {
-      if (n <= qr_small_max_cols()) qr_panel(c, W, tau);
-      else ck(rocsolver_dgeqrf(c.blas, int(m), int(n), W.data(), int(m), tau.data()), "dgeqrf");
+    if (n > qr_small_max_cols() && c.qr_backend == 0) {
+      R = dqr_blocked(c, A, false).R;
+    } else {
+      Tensor W = copy_tensor(c, A);
+      Tensor tau = c.alloc({std::max<int64_t>(k, 1)}, A.cplx);
+      if (k > 0) {
+        if (n <= qr_small_max_cols()) qr_panel(c, W, tau);
+        else ck(rocsolver_dgeqrf(c.blas, int(m), int(n), W.data(), int(m), tau.data()), "dgeqrf");
+      }
+      R = upper_trapezoid(c, W, k);
     }
-    R = upper_trapezoid(c, W, k);
   }
   c.tend(cls, ev);
   return R;

@@ -168,6 +168,11 @@ class Ctx {
   // account GEMMs per exact shape ("gemm_f32 MxNxK ...") instead of per
   // precision (environment variable PEPS_HIP_GEMM_SHAPES=1)
   bool gemm_shapes = false;
+  // real QR wider than one panel: 1 rocSOLVER geqrf/orgqr (default), 0 the
+  // project's blocked Householder QR (PEPS_HIP_QR=blocked); the blocked form
+  // is ~5x slower on the tall panels of the subspace iteration (5184 x 288:
+  // 58 ms against 11 ms), whose one-workgroup panel kernel is the limit
+  int qr_backend = 1;
   // last Ritz basis of each boundary-MPS column position: the warm start of
   // the next subspace iteration at the same position
   std::map<std::string, Tensor> ritz_cache;
@@ -209,6 +214,17 @@ class Ctx {
 
   // Device scratch for small host→device uploads (label tables, weights).
   void* scratch_upload(const void* host, size_t bytes, std::shared_ptr<Buffer>* keep);
+  // Host -> device copy enqueued on the stream without waiting for it: the
+  // host bytes are first copied into a pinned staging ring (kStageBytes), so
+  // the caller's array may go away at once.  Copies larger than a quarter of
+  // the ring go synchronously.  When the ring wraps, the copies staged on its
+  // previous lap are waited for (one event, recorded after every copy).
+  void h2d_async(void* dst, const void* host, size_t bytes);
+  static constexpr size_t kStageBytes = size_t(16) << 20;
+  char* stage_host = nullptr;
+  size_t stage_head = 0;
+  hipEvent_t stage_ev = nullptr;
+  bool stage_used = false;
 };
 
 // Runs the enclosed GEMMs at a fixed precision tier (the factorisations and
@@ -288,6 +304,14 @@ void nonzero_lines(Ctx& c, const Tensor& M, std::vector<int64_t>* rows, std::vec
 // their scalars; qr_form_q builds the thin Q (m x min(m,n)).
 int qr_small_max_cols();
 void qr_panel(Ctx& c, Tensor& W, Tensor& tau);
+// The reflectors of a factored real panel as an explicit unit lower
+// trapezoid V, and the triangular factor T of I - V T V^T from G = V^T V.
+Tensor householder_v(Ctx& c, const Tensor& P);
+Tensor householder_t(Ctx& c, const Tensor& G, const Tensor& tau);
+// Contiguous sub-block copies of a matrix (no index tables): M(r0:r0+rows,
+// c0:c0+cols) out, and B back into M at (r0, c0).
+Tensor sub_block(Ctx& c, const Tensor& M, int64_t r0, int64_t c0, int64_t rows, int64_t cols);
+void put_block(Ctx& c, Tensor& M, int64_t r0, int64_t c0, const Tensor& B);
 void qr_form_q(Ctx& c, const Tensor& W, const Tensor& tau, Tensor& Q);
 // One pass: are all entries finite, is any entry non-zero (synchronises).
 void scan_values(Ctx& c, const Tensor& t, bool* finite, bool* nonzero);

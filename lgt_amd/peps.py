@@ -129,6 +129,7 @@ _SIGS = {
     "peps_eigh": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "peps_solve_herm": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, C.c_double, _dp]),
     "peps_svd": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]),
+    "peps_qr": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, _dp, _dp, _dp]),
     "peps_bond_operator": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_double, C.c_double, C.c_double, C.c_double, _i32p, _dp]),
 }
@@ -961,6 +962,19 @@ def svd(A, ctx=None):
     S = np.empty(k)
     ctx.check(lib().peps_svd(ctx.h, int(cp), m, n, _dptr(a), _dptr(U), _dptr(S), _dptr(Vt)))
     return U, S, Vt
+
+
+def qr(A, ctx=None):
+    """Thin QR through the library: (Q, R)."""
+    ctx = ctx or default_context()
+    cp = np.iscomplexobj(A)
+    a = _as_array(A, cp)
+    m, n = a.shape
+    k = min(m, n)
+    Q = np.empty((m, k), dtype=a.dtype, order="F")
+    R = np.empty((k, n), dtype=a.dtype, order="F")
+    ctx.check(lib().peps_qr(ctx.h, int(cp), m, n, _dptr(a), _dptr(Q), _dptr(R)))
+    return Q, R
 
 
 # ── u1_lgt_hamiltonian.jl ────────────────────────────────────────────────────

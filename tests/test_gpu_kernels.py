@@ -131,6 +131,58 @@ def test_eigh(ctx, cplx):
         np.testing.assert_allclose(np.sort(w), np.linalg.eigvalsh(H), atol=1e-13 * n)
 
 
+def test_qr_blocked_backend():
+    """The project's blocked Householder QR (PEPS_HIP_QR=blocked at context
+    creation; rocSOLVER geqrf/orgqr is the default for real panels wider than
+    64 columns): the same R rows as LAPACK, Q orthonormal, Q R = A, across
+    several panels, a ragged last panel and m < n."""
+    import os
+    from lgt_amd import peps
+    os.environ["PEPS_HIP_QR"] = "blocked"
+    try:
+        c2 = peps.Context(0)
+    finally:
+        del os.environ["PEPS_HIP_QR"]
+    rng = np.random.default_rng(13)
+    for (m, n) in [(300, 130), (2000, 200), (130, 300), (700, 700)]:
+        A = rng.standard_normal((m, n))
+        Q, R = peps.qr(A, ctx=c2)
+        Qn, Rn = np.linalg.qr(A)
+        k = min(m, n)
+        assert np.abs(Q @ R - A).max() < 1e-13 * np.abs(A).max() * max(m, n) ** 0.5
+        assert np.abs(Q.T @ Q - np.eye(k)).max() < 1e-13 * max(m, n) ** 0.5
+        assert np.abs(R - Rn).max() < 1e-11 * np.abs(A).max() * max(m, n) ** 0.5
+    c2.close()
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_qr(ctx, cplx):
+    """Thin QR against numpy's LAPACK QR: the same R (LAPACK's sign
+    convention, so no sign freedom) and Q to rounding, Q orthonormal, Q R = A.
+    Up to 64 columns the project's panel kernel; wider, rocSOLVER (real) or
+    its unblocked fallback (complex): ragged shapes, wide m < n, exact zero
+    and repeated columns."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(12)
+    for (m, n) in [(40, 30), (200, 64), (300, 130), (2000, 65), (5184, 288), (130, 300), (700, 700)]:
+        A = rng.standard_normal((m, n)) + (1j * rng.standard_normal((m, n)) if cplx else 0)
+        A[:, 3] = 0.0
+        A[:, 70 % n] = 2.0 * A[:, 1]
+        Q, R = peps.qr(A, ctx=ctx)
+        k = min(m, n)
+        Qn, Rn = np.linalg.qr(A)
+        scale = np.abs(A).max()
+        assert np.abs(Q @ R - A).max() < 1e-13 * scale * max(m, n) ** 0.5, (m, n)
+        assert np.abs(np.conj(Q.T) @ Q - np.eye(k)).max() < 1e-13 * max(m, n) ** 0.5, (m, n)
+        assert np.abs(np.tril(R, -1)).max() == 0.0
+        # with LAPACK's sign convention the rows of R before the repeated
+        # column are unique (the exactly zero column gives tau = 0 in both);
+        # the repeated column leaves a rounding-level residual whose
+        # reflector, and every row after it, is arbitrary
+        cut = min(70 % n, k)
+        assert np.abs(R[:cut] - Rn[:cut]).max() < 1e-11 * scale * max(m, n) ** 0.5, (m, n, np.abs(R - Rn).max())
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 def test_svd(ctx, cplx):
     """Thin SVD (QR + one-sided Jacobi) against numpy, including the
