@@ -49,7 +49,11 @@ GEMM_CLASS = {"fp64": "gemm_f64_tiled", "fp32": "gemm_f32_tiled", "bf16": "gemm_
 # WRITE_SIZE over the timed step of a workload (copy-calibrated factors 2.0
 # and 1.0), keyed by (nx, ny, D, chi, gd, tier, class); profiles/ holds the
 # summaries.
-PROFILED_TRAFFIC = {}
+PROFILED_TRAFFIC = {
+    # first full-update step (profiles/gemm_f64_tiled_fu8x8_D6_chi144_hbm_traffic.md):
+    # 2.0 x 4.5306e5 KiB fetched + 1.0 x 1.4377e5 KiB written per launch
+    (8, 8, 6, 144, 2, "fp64", "gemm_f64_tiled"): 1024.0 * (2.0 * 4.5306e5 + 1.4377e5),
+}
 
 
 def log(msg):
@@ -357,6 +361,10 @@ def main():
                     help="wall-clock budget of the warm-up plus timed steps: warm-up steps are cut first, then "
                          "timed steps stop once another one would pass it (the JSON line reports the steps run)")
     ap.add_argument("--tier", default="fp64", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--cut", type=float, default=None,
+                    help="singular-value cut of the boundary compression of the timed run (default 1e-14, the "
+                         "reference's; a reduced-precision tier can cut at its own rounding level); a "
+                         "--compare-tier run keeps the default")
     ap.add_argument("--nx", type=int, default=8)
     ap.add_argument("--ny", type=int, default=8)
     ap.add_argument("--D", type=int, default=6)
@@ -419,6 +427,8 @@ def main():
     log(f"PEPS bond dimensions at start: {bond_dims}")
     start = gp.to_arrays() if a.compare_tier else None
     ctx.set_tier(a.tier)
+    if a.cut is not None:
+        ctx.set_compress_cut(a.cut)
     t_begin = time.perf_counter()
     deadline = t_begin + a.max_seconds
     warm = 0
@@ -461,6 +471,9 @@ def main():
     bmps_mean = bsum["flops"] / max(bsum["bytes"], 1.0)
     roof = roofline_of(dominant[0], dominant[1], a.tier)
     roof["traffic"] = PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier, dominant[0]))
+    if roof["traffic"] is not None:
+        roof["traffic_source"] = ("HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE over the first "
+                                  "full-update step, copy-calibrated (factors 2.0 and 1.0); profiles/")
     groof = roofline_of(gcls, gk, a.tier)
     groof["traffic"] = PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier, gcls))
     kernel_time = sum(v["ms"] for v in kern.values())
@@ -484,6 +497,7 @@ def main():
                    "steps_requested": a.steps, "warmup_requested": a.warmup, "max_seconds": a.max_seconds,
                    "step_seconds": [round(x, 3) for x in times],
                    "n_als_max": a.n_als, "als_tol": a.als_tol, "tier": a.tier,
+                   "compress_cut": a.cut if a.cut is not None else 1e-14,
                    "step_flags": "PEPS_STEP_RENORM|PEPS_STEP_POSITIVE_COST",
                    "bond_updates_per_step": 2 * (a.nx - 1) * a.ny + a.nx * (a.ny - 1),
                    "als_sweeps_per_step": als["sweeps"] / max(1, steps),

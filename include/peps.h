@@ -106,6 +106,13 @@ int32_t peps_set_tier(peps_ctx* ctx, int32_t tier);
    factored product and top-chi eigenvectors of Y^H G Y — the same truncated
    MPS without forming the chi D^2 x D^2 x chi D^2 product. */
 int32_t peps_set_compress_mode(peps_ctx* ctx, int32_t mode);
+/* Singular-value cut of the boundary compression in the steps and energy
+   evaluations (compress_mps!'s 1e-14, finite_peps_boundary_mps.jl:93, the
+   default; relative to each bond's largest singular value under
+   PEPS_STEP_RENORM).  No reference counterpart: a reduced-precision tier
+   can cut at its own rounding level instead of keeping noise above 1e-14.
+   0 < cut < 1. */
+int32_t peps_set_compress_cut(peps_ctx* ctx, double cut);
 /* Iteration caps of the factorisations (no reference counterpart; the
    reference's LAPACK calls have their own): Chebyshev subspace iterations
    per top-k eigenproblem of the Gram-form compression (default 25) and

@@ -149,6 +149,12 @@ int32_t peps_set_compress_mode(peps_ctx* ctx, int32_t mode) {
   return PEPS_OK;
 }
 
+int32_t peps_set_compress_cut(peps_ctx* ctx, double cut) {
+  if (!ctx || !(cut > 0.0) || !std::isfinite(cut) || cut >= 1.0) return PEPS_EARG;
+  ctx->c->bmps_cut = cut;
+  return PEPS_OK;
+}
+
 int32_t peps_set_solver_limits(peps_ctx* ctx, int32_t subspace_max_it, int32_t jacobi_max_sweeps) {
   if (!ctx || subspace_max_it < 1 || jacobi_max_sweeps < 1) return PEPS_EARG;
   ctx->c->subspace_max_it = subspace_max_it;

@@ -528,7 +528,7 @@ MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int c
   bool gram = c.compress_mode == 2 || (c.compress_mode == 0 && maxbond > 1024);
   if (!gram || mps.size() <= 1) {
     MPS out = from_top ? absorb_row_from_top(c, mps, row) : absorb_row_from_bottom(c, mps, row);
-    compress_mps(c, out, chi, CMPS_CUT, normalise);
+    compress_mps(c, out, chi, c.bmps_cut, normalise);
     if (normalise) normalise_mps(c, out, true);
     record_boundary_bonds(c, out);
     return out;
@@ -543,7 +543,7 @@ MPS absorb_compress(Ctx& c, const MPS& mps, const MPS& row, bool from_top, int c
   }
   // the same boundary (side, row) is recomputed every sweep: key its columns
   const std::string prefix = std::string(from_top ? "top" : "bottom") + std::to_string(key_row);
-  MPS out = compress_columns(c, cols, chi, CMPS_CUT, mps[0].cplx, key_row > 0 ? &prefix : nullptr,
+  MPS out = compress_columns(c, cols, chi, c.bmps_cut, mps[0].cplx, key_row > 0 ? &prefix : nullptr,
                              normalise);
   if (normalise) normalise_mps(c, out, true);
   record_boundary_bonds(c, out);

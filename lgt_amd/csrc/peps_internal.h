@@ -168,6 +168,10 @@ class Ctx {
   // account GEMMs per exact shape ("gemm_f32 MxNxK ...") instead of per
   // precision (environment variable PEPS_HIP_GEMM_SHAPES=1)
   bool gemm_shapes = false;
+  // singular-value cut of the boundary-MPS compression of the drivers
+  // (compress_mps!'s 1e-14, finite_peps_boundary_mps.jl:93; relative to the
+  // largest singular value under PEPS_STEP_RENORM); peps_set_compress_cut
+  double bmps_cut = 1e-14;
   // real QR wider than one panel: 1 rocSOLVER geqrf/orgqr (default), 0 the
   // project's blocked Householder QR (PEPS_HIP_QR=blocked); the blocked form
   // is ~5x slower on the tall panels of the subspace iteration (5184 x 288:

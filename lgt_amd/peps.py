@@ -52,6 +52,7 @@ _SIGS = {
     "peps_warning_count": (C.c_int64, [_vp]),
     "peps_set_tier": (C.c_int32, [_vp, C.c_int32]),
     "peps_set_compress_mode": (C.c_int32, [_vp, C.c_int32]),
+    "peps_set_compress_cut": (C.c_int32, [_vp, C.c_double]),
     "peps_set_solver_limits": (C.c_int32, [_vp, C.c_int32, C.c_int32]),
     "peps_set_timing": (C.c_int32, [_vp, C.c_int32]),
     "peps_stats_count": (C.c_int32, [_vp, _i32p]),
@@ -193,6 +194,10 @@ class Context:
     def set_compress_mode(self, mode: str):
         """'auto', 'qr_svd' (explicit product) or 'gram' (factored Gram form)."""
         self.check(lib().peps_set_compress_mode(self.h, {"auto": 0, "qr_svd": 1, "gram": 2}[mode]))
+
+    def set_compress_cut(self, cut: float):
+        """Singular-value cut of the boundary compression (1e-14 by default)."""
+        self.check(lib().peps_set_compress_cut(self.h, float(cut)))
 
     def set_timing(self, on: bool):
         self.check(lib().peps_set_timing(self.h, 1 if on else 0))
