@@ -351,7 +351,17 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
     Tensor rest = Q0.slice_last(have, p);
     fill_random(c, rest, 0x5eed1234abcdull + uint64_t(N) * 7919u + uint64_t(have));
   }
-  Tensor Q = qr_thin(c, have ? Q0 : matmul(c, rh, Q0)).Q;
+  // a full cached basis is the previous sweep's Ritz vectors, orthonormal to
+  // rounding: it is used as it is when its Gram matrix is the identity to
+  // 1e-12 (one p x N x p product instead of a Householder QR of N x p)
+  bool orthonormal = false;
+  if (have == p) {
+    Tensor Gq = matmul(c, Q0, Q0, true, false, true, false);
+    axpy_inplace(c, Gq, eye(c, p, p, Q0.cplx), -1.0);
+    orthonormal = std::sqrt(norm2sq(c, Gq)) <= 1e-12;
+    c.count(orthonormal ? "top_eigs[cached basis]" : "top_eigs[cached basis, reorthonormalised]", 1.0, 0);
+  }
+  Tensor Q = orthonormal ? Q0 : qr_thin(c, have ? Q0 : matmul(c, rh, Q0)).Q;
   std::vector<double> theta;
   Tensor X;
   // filter degree per outer iteration: one product rho Y per degree is cheap
