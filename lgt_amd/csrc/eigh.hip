@@ -1,6 +1,6 @@
 // Real symmetric eigensolver for n <= 1024 on one MI355X:
 //   1. Householder reduction to tridiagonal form in ONE workgroup and one
-//      launch: for n <= 208 sytrd_lower_reg_kernel keeps the lower triangle
+//      launch: for n <= 288 sytrd_lower_reg_kernel keeps the lower triangle
 //      in registers (no global memory in the reflector loop); above that
 //      sytrd_lower_kernel reads and writes the trailing matrix once per
 //      reflector: each pass applies the previous step's rank-2 update and
@@ -194,40 +194,77 @@ __device__ __forceinline__ double row4_sum(double x) {
   x += dpp_f64<0x128>(x);  // row_ror:8
   return x;
 }
+// Sum over the 64 lanes of a wave, the same value in every lane: DPP sums
+// within each 16-lane row, then the four row sums read from lanes 0, 16, 32
+// and 48 and added in that order.  Every lane of the wave must be active.
+__device__ __forceinline__ double wave_sum(double x) {
+  x = row4_sum(quad_sum(x));
+  auto lane_val = [&](int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+  };
+  return ((lane_val(0) + lane_val(16)) + lane_val(32)) + lane_val(48);
+}
 
 // Register-resident variant of the same reduction for n <= 16 * NT (NT tile
-// rows): the lower triangle of A lives in the registers of 8 waves as 16x16
-// tiles (tile (I, J), I >= J, numbered column by column; a diagonal tile
-// holds its full symmetric block).  Each 16-lane group of a wave owns one
-// tile per slot (tile 4 (8 s + wave) + group), each lane a 4x4 block of it:
-// rows 4 (l >> 2) + {0..3}, columns 4 (l & 3) + {0..3} of the tile (l the
-// lane within the group).  With LDS0 the first slot -- tiles of the first
-// two block columns, which are finished after 32 reflectors -- is kept in
-// LDS instead of registers.  Per reflector: the column is gathered into
-// LDS, y = A v is formed from the tiles -- row sums over the 4 lanes of a
-// quad, column sums of the off-diagonal tiles over the 4 lanes 4 apart,
-// both with DPP -- into one LDS buffer per (wave, group), summed in a fixed
-// order, and the rank-2 update is applied in registers.  No global memory is touched between the initial load and the
-// reflectors' store.
-template <int NT, int SLOTS, bool LDS0>
-__global__ __launch_bounds__(512) void sytrd_lower_reg_kernel(const double* __restrict__ A, int n, int lda,
-                                                               double* __restrict__ R, double* __restrict__ dd,
-                                                               double* __restrict__ ee, double* __restrict__ tt,
-                                                               int* __restrict__ exo) {
-  constexpr int NP = 16 * NT;     // padded size
-  constexpr int NW = 8;           // waves
+// rows), one workgroup of NW waves.  The lower triangle of A lives in
+// registers as 16x16 tiles (tile (I, J), I >= J, numbered column by column;
+// a diagonal tile holds its full symmetric block).  Each 16-lane group owns
+// one tile per slot (tile 4 (NW s + wave) + group), each lane a 4x4 block of
+// it: rows 4 (l >> 2) + {0..3}, columns 4 (l & 3) + {0..3} of the tile (l
+// the lane within the group).  With LDS0 the first slot -- the first 4 NW
+// tiles, all in the first block columns and so the first to be finished --
+// is kept in LDS instead of registers.
+// Per reflector k, four barriers:
+//   1. column k (already updated) and the partial norms of its rows >= k+2,
+//      one per tile row, are in LDS: every thread forms the reflector
+//      scalars and thread i stores v_i;
+//   2. y = A v is accumulated from the tiles -- row sums over the 4
+//      lanes of a quad, column sums of the off-diagonal tiles over the 4
+//      lanes 4 apart, both with DPP -- and each tile writes its 16-row
+//      pieces into its own slots part[I][J] (rows) and part[J][I] (columns);
+//   3. thread i sums part[i >> 4][q] over the live q in a fixed order, forms
+//      y_i and its share of y.v;
+//   4. the rank-2 update A -= v w^T + w v^T runs in registers (w formed
+//      from y and v as it is read), and the owners of column k + 1 write
+//      it, with the partial norms of its rows >= k + 3, into the other
+//      column buffer.
+// No global memory is touched between the initial load and the stores of
+// the reflectors, d, e and tau.
+// 4x4 block of a tile as seen by one lane: registers, or LDS strided by the
+// workgroup size (the tile slot kept in LDS).
+struct RegBlk {
+  double (&b)[4][4];
+  __device__ double& operator()(int ri, int ci) const { return b[ri][ci]; }
+};
+struct LdsBlk {
+  double* p;
+  int stride;
+  __device__ double& operator()(int ri, int ci) const { return p[(4 * ri + ci) * stride]; }
+};
+
+template <int NT, int NW, int SLOTS, bool LDS0>
+__global__ __launch_bounds__(64 * NW) void sytrd_lower_reg_kernel(const double* __restrict__ A, int n, int lda,
+                                                                   double* __restrict__ R, double* __restrict__ dd,
+                                                                   double* __restrict__ ee, double* __restrict__ tt,
+                                                                   int* __restrict__ exo) {
+  constexpr int NP = 16 * NT;  // padded size
+  constexpr int NTH = 64 * NW;
   constexpr int NTILES = NT * (NT + 1) / 2;
-  constexpr int RS = LDS0 ? SLOTS - 1 : SLOTS;   // slots held in registers
+  constexpr int RS = LDS0 ? SLOTS - 1 : SLOTS;  // slots held in registers
   static_assert((NTILES + 4 * NW - 1) / (4 * NW) <= SLOTS, "tile slots");
-  static_assert(NP <= 512, "one index per thread");
+  static_assert(NP <= NTH, "one index per thread");
+  static_assert(NT <= 32 && NW <= 32, "norm and reduction buffers");
   extern __shared__ double dsm[];
-  double (*colb)[NP] = reinterpret_cast<double (*)[NP]>(dsm);         // [4][NP]
-  double* vk = dsm + 4 * NP;
-  double* wv = vk + NP;
-  double (*ybuf)[NP] = reinterpret_cast<double (*)[NP]>(wv + NP);     // [4 NW][NP]
-  double* red0 = wv + NP + 4 * NW * NP;
-  double* red1 = red0 + 32;
-  double* slot0 = red1 + 32;                                           // [16][512] when LDS0
+  // [2][4][NP]: the 4-column blocks holding column k and column k + 1
+  double (*colb)[4][NP] = reinterpret_cast<double (*)[4][NP]>(dsm);
+  double* ybuf = dsm + 8 * NP;                                 // [NP]: y = tau A v
+  double* vbuf = ybuf + NP;                                    // [NP]: v of reflector k
+  double (*part)[NT][16] = reinterpret_cast<double (*)[NT][16]>(vbuf + NP);  // [NT][NT][16]
+  double (*nbuf)[32] = reinterpret_cast<double (*)[32]>(vbuf + NP + NT * NT * 16);  // [4][32]
+  double* red = vbuf + NP + NT * NT * 16 + 128;                // [32]: per-wave partial sums
+  double* slot0 = red + 32;                                    // [16][NTH] when LDS0
   __shared__ unsigned char tI[NTILES], tJ[NTILES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int grp = lane >> 4, l16 = lane & 15;
@@ -244,9 +281,9 @@ __global__ __launch_bounds__(512) void sytrd_lower_reg_kernel(const double* __re
   auto tile_of = [&](int s) { return 4 * (NW * s + wave) + grp; };
   // load: lower triangle, mirrored into diagonal tiles; padding is zero
   double a[RS > 0 ? RS : 1][4][4];
-  double b[4][4];
+  const LdsBlk b{slot0 + tid, NTH};
   double mx = 0.0;
-  auto load_tile = [&](int t, double (&blk)[4][4]) {
+  auto load_tile = [&](int t, auto blk) {
 #pragma unroll
     for (int ri = 0; ri < 4; ++ri)
 #pragma unroll
@@ -256,18 +293,14 @@ __global__ __launch_bounds__(512) void sytrd_lower_reg_kernel(const double* __re
           const int i = 16 * tI[t] + rb + ri, j = 16 * tJ[t] + cb + ci;
           if (i < n && j < n) x = i >= j ? A[i + int64_t(j) * lda] : A[j + int64_t(i) * lda];
         }
-        blk[ri][ci] = x;
+        blk(ri, ci) = x;
         mx = fmax(mx, fabs(x));
       }
   };
 #pragma unroll
-  for (int s = 0; s < RS; ++s) load_tile(tile_of(s + (LDS0 ? 1 : 0)), a[s]);
-  if (LDS0) {
-    load_tile(tile_of(0), b);
-#pragma unroll
-    for (int e = 0; e < 16; ++e) slot0[e * 512 + tid] = b[e >> 2][e & 3];
-  }
-  mx = trd_block_max(mx, red0);
+  for (int s = 0; s < RS; ++s) load_tile(tile_of(s + (LDS0 ? 1 : 0)), RegBlk{a[s]});
+  if (LDS0) load_tile(tile_of(0), b);
+  mx = trd_block_max(mx, red);
   const int ex = mx > 0.0 ? ilogb(mx) : 0;
   const double sc = ldexp(1.0, -ex);
   if (tid == 0) *exo = ex;
@@ -279,40 +312,58 @@ __global__ __launch_bounds__(512) void sytrd_lower_reg_kernel(const double* __re
       for (int ci = 0; ci < 4; ++ci) a[s][ri][ci] *= sc;
   if (LDS0) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) slot0[e * 512 + tid] *= sc;
+    for (int e = 0; e < 16; ++e) slot0[e * NTH + tid] *= sc;
   }
-  __syncthreads();
+  // a tile takes part in reflector k while its columns reach k + 1
   auto active = [&](int t, int k) { return t < NTILES && 16 * tJ[t] + 15 > k; };
-  for (int k = 0; k < n; ++k) {
-    const int kJ = k >> 4, kc = k & 15;
-    double (*cb4)[NP] = colb;
-    for (int q = tid; q < 4 * NW * NP; q += 512) (&ybuf[0][0])[q] = 0.0;
-    // gather column k: the owners write all four columns of their block
-    auto gather = [&](int t, const double (&blk)[4][4]) {
-      if (t < NTILES && tJ[t] == kJ && cb == (kc & ~3)) {
-        const int i = 16 * tI[t] + rb;
+  // owners of column c (tiles (I, c >> 4), the lanes whose 4 columns hold
+  // it) write their 4-column block to dst[ci][row] and, per tile row, the
+  // sums of squares of each of the 4 columns over the rows >= (its index) + 2
+  // to nbuf[ci][I]; only column c & 3 of them is used.  Constant register
+  // indices only (a selected column would put the tile array in scratch);
+  // the whole 16-lane group runs this (the DPP sums read lanes 4 apart),
+  // only the owners store.
+  auto gather = [&](int t, auto blk, int c, double (*dst)[NP]) {
+    if (t < NTILES && tJ[t] == (c >> 4)) {
+      const int i0 = 16 * tI[t] + rb, c0 = c & ~3;
+      double s[4];
+#pragma unroll
+      for (int ci = 0; ci < 4; ++ci) {
+        s[ci] = 0.0;
+#pragma unroll
+        for (int ri = 0; ri < 4; ++ri)
+          if (i0 + ri >= c0 + ci + 2) s[ci] += blk(ri, ci) * blk(ri, ci);
+        s[ci] = row4_sum(s[ci]);
+      }
+      if (cb == (c & 12)) {
 #pragma unroll
         for (int ri = 0; ri < 4; ++ri)
 #pragma unroll
-          for (int ci = 0; ci < 4; ++ci) cb4[ci][i + ri] = blk[ri][ci];
+          for (int ci = 0; ci < 4; ++ci) dst[ci][i0 + ri] = blk(ri, ci);
+        if (rb == 0) {
+#pragma unroll
+          for (int ci = 0; ci < 4; ++ci) nbuf[ci][tI[t]] = s[ci];
+        }
       }
-    };
-    if (LDS0 && kJ <= 1) {
-#pragma unroll
-      for (int e = 0; e < 16; ++e) b[e >> 2][e & 3] = slot0[e * 512 + tid];
-      gather(tile_of(0), b);
     }
+  };
+  // column 0
+  __syncthreads();  // the load's block maximum has read red
+  if (LDS0) gather(tile_of(0), b, 0, colb[0]);
 #pragma unroll
-    for (int s = 0; s < RS; ++s) gather(tile_of(s + (LDS0 ? 1 : 0)), a[s]);
-    __syncthreads();
-    const double* col = cb4[kc & 3];
-    if (k == n - 1) {
-      if (tid == 0) dd[n - 1] = col[n - 1];
-      break;
+  for (int s = 0; s < RS; ++s) gather(tile_of(s + (LDS0 ? 1 : 0)), RegBlk{a[s]}, 0, colb[0]);
+  __syncthreads();
+  for (int k = 0; k < n - 1; ++k) {
+    const double* col = colb[k & 1][k & 3];
+    // reflector scalars (dlarfg), computed identically by every thread
+    // (fixed trip counts: the loads are issued together, the order is fixed)
+    double xn2 = 0.0;
+    {
+      const double* nb = nbuf[k & 3];
+      const int I0 = k >> 4;
+#pragma unroll
+      for (int I = 0; I < NT; ++I) xn2 += I >= I0 ? nb[I] : 0.0;
     }
-    double part = 0.0;
-    for (int i = k + 2 + tid; i < n; i += 512) part += col[i] * col[i];
-    const double xn2 = trd_block_sum(part, red0);
     const double alpha = col[k + 1];
     double tau = 0.0, beta = alpha, scl = 0.0;
     if (xn2 > 0.0) {
@@ -326,19 +377,18 @@ __global__ __launch_bounds__(512) void sytrd_lower_reg_kernel(const double* __re
       ee[k] = beta;
       tt[k] = tau;
     }
-    for (int i = tid; i < NP; i += 512) {
-      const double v = (i <= k || i >= n) ? 0.0 : (i == k + 1 ? 1.0 : col[i] * scl);
-      vk[i] = v;
-      if (i >= k + 2 && i < n) R[i + int64_t(k) * lda] = v;
+    if (tid < NP) {
+      const double x = col[tid];
+      const double v = tid == k + 1 ? 1.0 : (tid >= k + 2 && tid < n ? x * scl : 0.0);
+      vbuf[tid] = v;
+      if (tid >= k + 2 && tid < n) R[tid + int64_t(k) * lda] = v;
     }
     __syncthreads();
-    auto vof = [&](int j) { return vk[j]; };
-    // y = A v (a tile is active while its columns reach k + 1); each
-    // (wave, group) adds into its own buffer: tiles of one group are visited
-    // in order, and no two groups share a buffer
-    double* yb = ybuf[4 * wave + grp];
-    auto matvec = [&](int t, const double (&blk)[4][4]) {
-      const int I0 = 16 * tI[t] + rb, J0 = 16 * tJ[t] + cb;
+    auto vof = [&](int j) { return vbuf[j]; };
+    // y = A v into part[][][]
+    auto matvec = [&](int t, auto blk) {
+      const int I = tI[t], J = tJ[t];
+      const int I0 = 16 * I + rb, J0 = 16 * J + cb;
       double vj[4], vi[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -349,80 +399,86 @@ __global__ __launch_bounds__(512) void sytrd_lower_reg_kernel(const double* __re
       for (int ri = 0; ri < 4; ++ri) {
         double y = 0.0;
 #pragma unroll
-        for (int ci = 0; ci < 4; ++ci) y += blk[ri][ci] * vj[ci];
+        for (int ci = 0; ci < 4; ++ci) y += blk(ri, ci) * vj[ci];
         y = quad_sum(y);
-        if (cb == 0) yb[I0 + ri] += y;
+        if (cb == 0) part[I][J][rb + ri] = y;
       }
-      if (tI[t] != tJ[t]) {
+      if (I != J) {
 #pragma unroll
         for (int ci = 0; ci < 4; ++ci) {
           double y = 0.0;
 #pragma unroll
-          for (int ri = 0; ri < 4; ++ri) y += blk[ri][ci] * vi[ri];
+          for (int ri = 0; ri < 4; ++ri) y += blk(ri, ci) * vi[ri];
           y = row4_sum(y);
-          if (rb == 0) yb[J0 + ci] += y;
+          if (rb == 0) part[J][I][cb + ci] = y;
         }
       }
     };
-    if (LDS0 && active(tile_of(0), k)) {
-#pragma unroll
-      for (int e = 0; e < 16; ++e) b[e >> 2][e & 3] = slot0[e * 512 + tid];
-      matvec(tile_of(0), b);
-    }
+    if (LDS0 && active(tile_of(0), k)) matvec(tile_of(0), b);
 #pragma unroll
     for (int s = 0; s < RS; ++s) {
       const int t = tile_of(s + (LDS0 ? 1 : 0));
-      if (active(t, k)) matvec(t, a[s]);
+      if (active(t, k)) matvec(t, RegBlk{a[s]});
+      __builtin_amdgcn_sched_barrier(0);  // one tile's operands live at a time
     }
     __syncthreads();
-    // thread i owns index i (NP <= 512): y_i, w_i = tau y_i - tau^2 (y.v)/2 v_i
-    double yi = 0.0, pdot = 0.0;
-    const int i = tid;
-    if (i > k && i < n) {
-#pragma unroll 8
-      for (int w = 0; w < 4 * NW; ++w) yi += ybuf[w][i];
+    // thread i owns index i: y_i = tau (A v)_i, and y.v over the workgroup;
+    // part[I][q] is live for q >= (k + 1) >> 4 (the tiles still active)
+    double yi = 0.0, pd = 0.0;
+    if (tid > k && tid < n) {
+      const int I = tid >> 4, r = tid & 15, q0 = (k + 1) >> 4;
+#pragma unroll
+      for (int q = 0; q < NT; ++q) yi += q >= q0 ? part[I][q][r] : 0.0;
       yi *= tau;
-      pdot = yi * vof(i);
+      pd = yi * vof(tid);
     }
-    const double K = trd_block_sum(pdot, red1);
-    const double half = -0.5 * tau * K;
-    if (i < NP) wv[i] = (i > k && i < n) ? yi + half * vof(i) : 0.0;
+    if (tid < NP) ybuf[tid] = yi;
+    pd = wave_sum(pd);
+    if (lane == 0) red[wave] = pd;
     __syncthreads();
-    // rank-2 update A -= v w^T + w v^T on the active tiles
-    auto update = [&](int t, double (&blk)[4][4]) {
+    double K = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) K += red[w];
+    // w = y - (tau/2)(y.v) v
+    const double half = -0.5 * tau * K;
+    auto wof = [&](int j) { return ybuf[j] + half * vof(j); };
+    auto update = [&](int t, auto blk) {
       const int I0 = 16 * tI[t] + rb, J0 = 16 * tJ[t] + cb;
       double vj[4], wj[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         vj[q] = vof(J0 + q);
-        wj[q] = wv[J0 + q];
+        wj[q] = wof(J0 + q);
       }
 #pragma unroll
       for (int ri = 0; ri < 4; ++ri) {
-        const double vi = vof(I0 + ri), wi = wv[I0 + ri];
+        const double vi = vof(I0 + ri), wi = wof(I0 + ri);
 #pragma unroll
-        for (int ci = 0; ci < 4; ++ci) blk[ri][ci] -= vi * wj[ci] + wi * vj[ci];
+        for (int ci = 0; ci < 4; ++ci) blk(ri, ci) -= vi * wj[ci] + wi * vj[ci];
       }
     };
+    double (*nxt)[NP] = colb[(k + 1) & 1];
     if (LDS0 && active(tile_of(0), k)) {
-#pragma unroll
-      for (int e = 0; e < 16; ++e) b[e >> 2][e & 3] = slot0[e * 512 + tid];
       update(tile_of(0), b);
-#pragma unroll
-      for (int e = 0; e < 16; ++e) slot0[e * 512 + tid] = b[e >> 2][e & 3];
+      gather(tile_of(0), b, k + 1, nxt);
     }
 #pragma unroll
     for (int s = 0; s < RS; ++s) {
       const int t = tile_of(s + (LDS0 ? 1 : 0));
-      if (active(t, k)) update(t, a[s]);
+      if (active(t, k)) {
+        update(t, RegBlk{a[s]});
+        gather(t, RegBlk{a[s]}, k + 1, nxt);
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
   }
+  if (tid == 0) dd[n - 1] = colb[(n - 1) & 1][(n - 1) & 3][n - 1];
 }
 
-template <int NT, bool LDS0>
+template <int NT, int NW, bool LDS0>
 constexpr size_t trd_reg_lds_bytes() {
-  return sizeof(double) * (size_t(4 + 2 + 32) * 16 * NT + 64 + (LDS0 ? 16 * 512 : 0));
+  return sizeof(double) * (size_t(10) * 16 * NT + size_t(NT) * NT * 16 + 160 + (LDS0 ? size_t(16) * 64 * NW : 0));
 }
 
 // Z (n x ncols, leading dimension ldz) <- Q Z (trans = 0) or Q^T Z (trans =
@@ -524,30 +580,36 @@ Tridiag tridiagonalize(Ctx& c, const Tensor& A) {
   t.ex->pool = c.pool.get();
   t.ex->bytes = 256;
   t.ex->ptr = c.pool->get(256);
-  const std::string cls = Ctx::bucket("eigh_trd", n);
+  const std::string cls = c.gemm_shapes ? "eigh_trd " + std::to_string(n) : Ctx::bucket("eigh_trd", n);
   hipEvent_t ev = nullptr;
   c.tbegin(cls, &ev);
   c.count(cls, 4.0 / 3.0 * double(n) * n * n, 16.0 * double(n) * n * n / 3.0);
   int* exo = static_cast<int*>(t.ex->ptr);
   const int ni = int(n);
-#define PEPS_TRD_REG(NT, SLOTS, L0)                                                                           \
+#define PEPS_TRD_REG(NT, NW, SLOTS, L0)                                                                       \
   do {                                                                                                      \
-    const size_t lds = trd_reg_lds_bytes<NT, L0>();                                                         \
+    const size_t lds = trd_reg_lds_bytes<NT, NW, L0>();                                                     \
     static bool attr = false;                                                                               \
     if (!attr) {                                                                                            \
-      PEPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sytrd_lower_reg_kernel<NT, SLOTS, L0>),     \
+      PEPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sytrd_lower_reg_kernel<NT, NW, SLOTS, L0>), \
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                  \
       attr = true;                                                                                          \
     }                                                                                                       \
-    hipLaunchKernelGGL((sytrd_lower_reg_kernel<NT, SLOTS, L0>), dim3(1), dim3(512), lds, c.stream, t.R.data(), \
-                       ni, ni, t.R.data(), t.d.data(), t.e.data(), t.tau.data(), exo);                     \
+    hipLaunchKernelGGL((sytrd_lower_reg_kernel<NT, NW, SLOTS, L0>), dim3(1), dim3(64 * NW), lds, c.stream,   \
+                       t.R.data(), ni, ni, t.R.data(), t.d.data(), t.e.data(), t.tau.data(), exo);          \
   } while (0)
   if (n <= 64)
-    PEPS_TRD_REG(4, 1, false);
+    PEPS_TRD_REG(4, 8, 1, false);
+  else if (n <= 96)
+    PEPS_TRD_REG(6, 8, 1, false);
   else if (n <= 144)
-    PEPS_TRD_REG(9, 2, false);
+    PEPS_TRD_REG(9, 8, 2, false);
   else if (n <= 208)
-    PEPS_TRD_REG(13, 3, false);
+    PEPS_TRD_REG(13, 8, 3, false);
+  else if (n <= 256)
+    PEPS_TRD_REG(16, 12, 3, true);
+  else if (n <= 288)
+    PEPS_TRD_REG(18, 11, 4, true);
   else if (c.trd_backend == 1) {
 #undef PEPS_TRD_REG
     // rocSOLVER sytrd (same LAPACK reflector layout) on the matrix scaled by
@@ -613,7 +675,7 @@ void tridiag_eig(Ctx& c, const Tridiag& t, std::vector<double>* w, Tensor* Z) {
   ib->bytes = 256;
   ib->ptr = c.pool->get(256);
   rocblas_int* info = static_cast<rocblas_int*>(ib->ptr);
-  const std::string cls = Ctx::bucket("eigh_stedc", n);
+  const std::string cls = c.gemm_shapes ? "eigh_stedc " + std::to_string(n) : Ctx::bucket("eigh_stedc", n);
   hipEvent_t ev = nullptr;
   c.tbegin(cls, &ev);
   c.count(cls, 4.0 * double(n) * n * n, 0);

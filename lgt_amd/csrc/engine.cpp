@@ -332,6 +332,7 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
     return;
   }
   const int64_t p = std::min<int64_t>(512, std::min<int64_t>(N, std::max<int64_t>(want + 64, 2 * want)));
+  c.count("top_eigs[subspace calls]", 1.0, 0);
   // rho comes from tier-precision products: make it exactly Hermitian, or the
   // residuals stall at the level of its anti-Hermitian part
   Tensor rh = copy_tensor(c, rho);
@@ -927,7 +928,11 @@ bool leading_svd_jw(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
   TierGuard fp64(c, TIER_FP64);
   const int64_t m = B.dims[0], n = B.dims[1], N = m + n;
   k = std::min(k, std::min(m, n));
-  if (B.cplx || N > sym_tridiag_max_n() || k < 1) return false;
+  if (B.cplx || k < 1) return false;
+  if (N > sym_tridiag_max_n()) {
+    c.count("svd_leading[jw: order above 1024]", 1.0, double(N));
+    return false;
+  }
   std::vector<int64_t> rows_u = iota_v(m), rows_v(n);
   for (int64_t i = 0; i < n; ++i) rows_v[i] = m + i;
   Tensor J = c.zeros({N, N}, false);
@@ -959,8 +964,12 @@ bool leading_svd_jw(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
   axpy_inplace(c, Rv, Us, 1.0);
   std::vector<double> r2 = col_norm2(c, Rv);
   const double tol = std::max(1e-13, 16.0 * double(N) * 2.220446049250313e-16) * s0;
-  for (double x : r2)
-    if (!(std::isfinite(x) && std::sqrt(x) <= tol)) return false;
+  for (int64_t j = 0; j < k; ++j)
+    if (!(std::isfinite(r2[j]) && std::sqrt(r2[j]) <= tol)) {
+      // value: the failing triplet's s_j / s_0; aux: its index
+      c.count("svd_leading[jw: residual]", S[j] / s0, double(j));
+      return false;
+    }
   c.count("svd_leading[jordan-wielandt]", 1.0, 0);
   out->U = Uk;
   out->Vt = permute(c, Vk, {1, 0});

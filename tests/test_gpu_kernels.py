@@ -103,7 +103,9 @@ def test_eigh(ctx, cplx):
     inputs."""
     from lgt_amd import peps
     rng = np.random.default_rng(4)
-    for n in (1, 2, 3, 7, 16, 33, 64, 130, 288):
+    # every variant of the register-resident reduction and its size limits
+    # (64, 96, 144, 208, 256, 288), the global-memory kernel above them
+    for n in (1, 2, 3, 7, 16, 33, 64, 65, 96, 97, 130, 144, 145, 208, 209, 250, 256, 257, 288, 289, 300):
         nb = 3 if n < 100 else 1
         B = rng.standard_normal((nb, n, n)) + (1j * rng.standard_normal((nb, n, n)) if cplx else 0)
         H = B @ np.conj(np.transpose(B, (0, 2, 1)))
