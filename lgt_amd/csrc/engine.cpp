@@ -202,6 +202,45 @@ static void debug_finite(Ctx& c, const Tensor& t, const char* what, size_t site)
                                      std::to_string(t.numel() / std::max<int64_t>(1, t.dims[0])) + ")");
 }
 
+// Smallest block dimension at which the sweep's SVD takes the Jordan-Wielandt
+// form (below it the one-workgroup Jacobi kernel is the faster one).
+constexpr int64_t kJwSweepMinDim = 128;
+
+// Truncated right factor of a real M (m x n) for the right-to-left SVD
+// sweep: k = max(1, min(kmax, #{s_j > thr})) with thr = cut s_0 (relative)
+// or cut, Vk (n x k) an orthonormal basis of the leading k right singular
+// vectors and P = M Vk (= U_k diag(S_k), which the sweep passes to the
+// left).  The singular triplets come from the symmetric eigenproblem of the
+// Jordan-Wielandt matrix [[0, C], [C^T, 0]] of M's square core C (C = R of
+// M = Q R when m > n, else M): backward stable, singular values to about
+// (m + n) eps s_0 -- the accuracy class of LAPACK's gesdd -- at the cost of
+// one eigendecomposition of order rows(C) + n instead of a one-workgroup
+// Jacobi iteration.  The selected right parts are orthonormalised by a QR
+// (singular values at rounding level mix their +-s and zero eigenvectors).
+static void topk_right_jw(Ctx& c, const Tensor& M, int64_t kmax, double cut, bool relative, Tensor* Vk,
+                          Tensor* P) {
+  TierGuard fp64(c, TIER_FP64);
+  const int64_t m = M.dims[0], n = M.dims[1];
+  const Tensor C = m > n ? qr_r_only(c, M) : M;
+  const int64_t mc = C.dims[0], N = mc + n, nsv = std::min(mc, n);
+  std::vector<int64_t> rows_u = seq(mc), rows_v(n);
+  for (int64_t i = 0; i < n; ++i) rows_v[i] = mc + i;
+  Tensor J = c.zeros({N, N}, false);
+  scatter_cols(c, J, C, rows_u, seq(n), rows_v);                        // J(0:mc, mc:N) = C
+  scatter_cols(c, J, permute(c, C, {1, 0}), rows_v, seq(mc), rows_u);   // J(mc:N, 0:mc) = C^T
+  EIG e = eigh(c, J);  // ascending: s_j = w[N - 1 - j]
+  const double s0 = std::max(e.w[N - 1], 0.0);
+  const double thr = relative ? cut * s0 : cut;
+  int64_t cnt = 0;
+  for (int64_t j = 0; j < nsv; ++j) cnt += e.w[N - 1 - j] > thr;
+  const int64_t k = std::max<int64_t>(1, std::min<int64_t>(kmax, cnt));
+  std::vector<int64_t> top(k);
+  for (int64_t j = 0; j < k; ++j) top[j] = N - 1 - j;
+  *Vk = qr_thin(c, gather2d(c, e.V, rows_v, top)).Q;
+  *P = matmul(c, M, *Vk);
+  c.count("bmps[svd: jordan-wielandt]", 1.0, double(N));
+}
+
 void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff, bool relative) {
   // :70-105  left->right QR, then right->left truncated SVD
   const size_t n = mps.size();
@@ -219,6 +258,17 @@ void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff, bool relative) {
   }
   for (size_t i = n - 1; i >= 1; --i) {
     int64_t Dl = mps[i].dims[0], dp = mps[i].dims[1], Dr = mps[i].dims[2];
+    if (!mps[i].cplx && std::min(Dl, dp * Dr) >= kJwSweepMinDim) {
+      // large real bonds: the Jordan-Wielandt form of the same truncation
+      Tensor Vk, P;
+      topk_right_jw(c, mps[i].reshaped({Dl, dp * Dr}), chi, cutoff, relative, &Vk, &P);
+      const int64_t k = Vk.dims[1];
+      mps[i] = copy_tensor(c, permute(c, Vk, {1, 0})).reshaped({k, dp, Dr});
+      int64_t Dl2 = mps[i - 1].dims[0], dp2 = mps[i - 1].dims[1], Dr2 = mps[i - 1].dims[2];
+      mps[i - 1] = matmul(c, mps[i - 1].reshaped({Dl2 * dp2, Dr2}), P).reshaped({Dl2, dp2, k});
+      debug_finite(c, mps[i - 1], "M*P product", i - 1);
+      continue;
+    }
     SVD f = svd_thin(c, mps[i].reshaped({Dl, dp * Dr}));
     debug_finite(c, f.U, "SVD factor U", i);
     debug_finite(c, f.Vt, "SVD factor Vt", i);
@@ -856,8 +906,11 @@ namespace {
 // SVD of Q^H B (p x n, p = k + 16): U = Q Ub.  Accepted only when every
 // kept triplet satisfies ||B^H u_i - sigma_i v_i|| <= 1e-13 sigma_0 -- the
 // triplets of the full SVD to rounding -- else returns false and the caller
-// runs the full SVD.  Worth it only for blocks much larger than k.
-bool leading_svd_try(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
+// runs the full SVD.  Worth it only for blocks much larger than k.  With
+// floor >= 0 only the triplets above it are wanted (the caller drops the
+// rest): when fewer than k exceed it, the kept ones are certified instead by
+// s_{keff+1}^Ritz + ||R||_F <= floor (no dropped value of B exceeds it).
+bool leading_svd_try(Ctx& c, const Tensor& B, int64_t k, SVD* out, double floor) {
   // Randomized subspace iteration (Halko, Martinsson & Tropp 2011, alg. 4.4)
   // on a (k + 16)-column sketch; from the second iteration on, the leading k
   // Ritz triplets of Q^H B are accepted once they are certified to be the
@@ -898,14 +951,22 @@ bool leading_svd_try(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
     axpy_inplace(c, R, matmul(c, Q, Bt), -1.0);
     const double rf = std::sqrt(std::max(0.0, dot(c, R, R, true).re));
     const double tol = 1e-13 * f.S[0];
-    bool ok = f.S[k] + rf + 2.0 * tol < f.S[k - 1];
-    for (double x : r2) ok = ok && std::isfinite(x) && std::sqrt(x) <= tol;
+    int64_t keff = k;
+    if (floor >= 0.0)
+      while (keff > 0 && !(f.S[keff - 1] > floor)) --keff;
+    bool ok = keff == k ? f.S[k] + rf + 2.0 * tol < f.S[k - 1] : keff > 0 && f.S[keff] + rf <= floor;
+    for (int64_t i = 0; i < keff; ++i) ok = ok && std::isfinite(r2[i]) && std::sqrt(r2[i]) <= tol;
     ok = ok && std::isfinite(rf) && all_finite(c, {&Uk, &Vtk});
     if (ok) {
       c.count("svd_leading[iterations]", double(it + 1), 0);
+      if (keff < k) {
+        c.count("svd_leading[below floor]", double(k - keff), 0);
+        Uk = copy_tensor(c, Uk.slice_last(0, keff));
+        Vtk = gather2d(c, Vtk, iota_v(keff), iota_v(n));
+      }
       out->U = Uk;
       out->Vt = Vtk;
-      out->S.assign(f.S.begin(), f.S.begin() + k);
+      out->S.assign(f.S.begin(), f.S.begin() + keff);
       return true;
     }
   }
@@ -924,7 +985,7 @@ bool leading_svd_try(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
 // iteration cannot certify its answer (a flat spectrum around the cut, as
 // in the noise part of a gated bond), at the cost of one tridiagonal
 // reduction of order m + n instead of ~15 sequential Jacobi sweeps.
-bool leading_svd_jw(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
+bool leading_svd_jw(Ctx& c, const Tensor& B, int64_t k, SVD* out, double floor) {
   TierGuard fp64(c, TIER_FP64);
   const int64_t m = B.dims[0], n = B.dims[1], N = m + n;
   k = std::min(k, std::min(m, n));
@@ -943,6 +1004,16 @@ bool leading_svd_jw(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
   for (int64_t j = 0; j < k; ++j) top[j] = N - 1 - j;
   const double s0 = e.w[N - 1];
   if (!(s0 > 0.0) || !std::isfinite(s0)) return false;
+  // with a floor only the values above it are kept (the caller drops the
+  // rest; their vectors mix with the zero eigenvectors of J)
+  if (floor >= 0.0) {
+    int64_t keff = k;
+    while (keff > 0 && !(e.w[N - keff] > floor)) --keff;
+    if (keff == 0) return false;
+    if (keff < k) c.count("svd_leading[below floor]", double(k - keff), 0);
+    k = keff;
+    top.resize(k);
+  }
   Tensor Uk = gather2d(c, e.V, rows_u, top);  // m x k, each column u_j / sqrt(2)
   Tensor Vk = gather2d(c, e.V, rows_v, top);  // n x k
   std::vector<double> nu = col_norm2(c, Uk), nv = col_norm2(c, Vk), iu(k), iv(k);
@@ -977,12 +1048,12 @@ bool leading_svd_jw(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
   return true;
 }
 
-bool leading_svd(Ctx& c, const Tensor& B, int64_t k, SVD* out) {
+bool leading_svd(Ctx& c, const Tensor& B, int64_t k, SVD* out, double floor) {
   const int64_t kmin = std::min(B.dims[0], B.dims[1]);
   const int64_t kk = std::min(k, kmin), p = std::min<int64_t>(kmin, kk + 16);
   if (!c.leading_svd || kk < 1 || kmin < 64 || 2 * p > kmin) return false;
   try {
-    if (leading_svd_try(c, B, k, out) || leading_svd_jw(c, B, k, out)) return true;
+    if (leading_svd_try(c, B, k, out, floor) || leading_svd_jw(c, B, k, out, floor)) return true;
     c.count("svd_leading[fallback]", 1.0, 0);
     return false;
   } catch (const Error& e) {
@@ -1008,7 +1079,7 @@ CSVD charge_svd(Ctx& c, const Tensor& M, const std::vector<int64_t>& rq,
     if (sc.rows.empty() || sc.cols.empty()) continue;
     Tensor blk = gather2d(c, M, sc.rows, sc.cols);
     SVD lead;
-    if (leading_only && leading_svd(c, blk, Dmax, &lead)) {
+    if (leading_only && leading_svd(c, blk, Dmax, &lead, reg)) {
       sc.U = lead.U;
       sc.Vt = lead.Vt;
       sc.S = lead.S;

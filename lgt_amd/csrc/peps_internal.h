@@ -148,10 +148,10 @@ class Ctx {
   // counts a warning (peps_warning_count); an unconverged subspace iteration
   // is then answered by a direct eigendecomposition (N <= 4096).
   int subspace_max_it = 25;
-  // tridiagonal reduction above the register kernel's size (n > 208): 0 the
-  // library's one-launch kernel, 1 rocSOLVER sytrd (environment variable
-  // PEPS_HIP_TRD=rocsolver)
-  int trd_backend = 0;
+  // tridiagonal reduction above the register kernel's size (n > 288): 1
+  // rocSOLVER sytrd (2-4x faster from n = 512 on), 0 the library's
+  // one-workgroup global-memory kernel (environment variable PEPS_HIP_TRD=own)
+  int trd_backend = 1;
   // leading-triplet (randomized, residual-checked) SVD of large charge
   // sectors in the ALS initialisation (environment variable
   // PEPS_HIP_LEADING_SVD=0 turns it off: every sector takes the full SVD)

@@ -84,6 +84,44 @@ def test_compress_mps_complex_zero_columns(ctx, chi, zero_cols):
     assert rel(mps_dense(out), mps_dense(ref)) < 1e-11
 
 
+@pytest.mark.parametrize("case", ["random", "graded", "zero"])
+def test_compress_mps_real_large_bonds(ctx, case):
+    """Real bonds of 256 (the sweep's SVDs take the Jordan-Wielandt form at
+    block dimensions >= 128): the compressed state and the bond dimensions
+    match the oracle's SVD sweep.  "graded": the middle tensor has rank 40
+    across its right bond with singular values 10^(-j/4), so the 1e-14 cut
+    keeps exactly 40 there; "zero": an all-zero middle tensor keeps one
+    direction and the zero state."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(23)
+    # orthogonal end tensors: the state's spectra across the two bonds are
+    # those of the middle tensor's two matricisations
+    qa, _ = np.linalg.qr(rng.standard_normal((256, 256)))
+    qc, _ = np.linalg.qr(rng.standard_normal((256, 256)))
+    mps = [qa.reshape(1, 256, 256), rng.standard_normal((256, 16, 256)) / 64, qc.T.reshape(256, 256, 1)]
+    if case == "graded":
+        U, _ = np.linalg.qr(rng.standard_normal((256 * 16, 40)))
+        V, _ = np.linalg.qr(rng.standard_normal((256, 40)))
+        mps[1] = ((U * 10.0 ** (-np.arange(40) / 4.0)) @ V.T).reshape(256, 16, 256)
+    if case == "zero":
+        mps[1] = np.zeros((256, 16, 256))
+    for chi in (256, 24):
+        out = peps.compress_mps([m.copy() for m in mps], chi, ctx=ctx)
+        ref = O.compress_mps([m.astype(complex) for m in mps], chi)
+        assert [m.shape for m in out] == [m.shape for m in ref], (chi, [m.shape for m in out])
+        d_ref = mps_dense(ref)
+        if case == "zero":
+            assert np.abs(mps_dense(out)).max() == 0.0
+        else:
+            assert rel(mps_dense(out), d_ref) < 1e-11
+        if case == "graded":
+            assert out[1].shape[2] == min(chi, 40)
+        # right-canonical: every tensor but the first is an isometry
+        for m in out[1:]:
+            a = m.reshape(m.shape[0], -1)
+            assert np.abs(a @ a.T - np.eye(a.shape[0])).max() < 1e-13
+
+
 @pytest.mark.parametrize("name,chi", [("s33", 64), ("s33", 6), ("s34", 12), ("s23c", 64)])
 def test_norm_and_boundaries(ctx, request, name, chi):
     from lgt_amd import peps
