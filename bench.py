@@ -386,6 +386,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (the known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
+    ap.add_argument("--eigh-bench", metavar="SIZES", default=None,
+                    help="only time the real symmetric eigensolver (tridiagonal reduction, divide and conquer, "
+                         "reflectors) on seeded random matrices of these comma-separated orders, 20 calls each, "
+                         "and exit (the ALS normal matrices of the full update are 48..288)")
     ap.add_argument("--summarize-rocprof", metavar="DIR", default=None,
                     help="only summarise the rocprofv3 output under DIR (with --bench-log: the log of "
                          "the profiled run, whose clock window selects the timed region) and exit")
@@ -395,6 +399,24 @@ def main():
         a.compare_tier = "fp64"
     if a.summarize_rocprof:
         summarize_rocprof(a.summarize_rocprof, a.bench_log)
+        return
+    if a.eigh_bench:
+        from lgt_amd import peps
+        ctx = peps.Context(0)
+        rng = np.random.default_rng(1)
+        for n in [int(x) for x in a.eigh_bench.split(",")]:
+            B = rng.standard_normal((n, n))
+            H = (B + B.T)[None]
+            w, V = peps.eigh(H, ctx=ctx)
+            ref = np.linalg.eigvalsh(H[0])
+            err = float(np.abs(np.sort(w[0]) - ref).max() / np.abs(ref).max())
+            orth = float(np.abs(V[0].T @ V[0] - np.eye(n)).max())
+            t0 = time.perf_counter()
+            for _ in range(20):
+                peps.eigh(H, ctx=ctx)
+            dt = (time.perf_counter() - t0) / 20
+            print(json.dumps({"eigh_n": n, "wall_ms": dt * 1e3, "eigval_rel_err": err, "orth_err": orth}),
+                  flush=True)
         return
     if a.copy_calibration:
         import torch

@@ -88,6 +88,7 @@ Ctx::Ctx(int dev) : device(dev) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   if (const char* v = std::getenv("PEPS_HIP_VERBOSE")) verbose = std::atoi(v);
+  if (const char* v = std::getenv("PEPS_HIP_SUBSPACE_OVERSAMPLE")) subspace_oversample = std::max(0, std::atoi(v));
   if (const char* v = std::getenv("PEPS_HIP_TRD")) trd_backend = std::string(v) == "own" ? 0 : 1;
   if (const char* v = std::getenv("PEPS_HIP_LEADING_SVD")) leading_svd = std::atoi(v) != 0;
   if (const char* v = std::getenv("PEPS_HIP_DEBUG_FINITE")) debug_finite = std::atoi(v) != 0;

@@ -381,7 +381,9 @@ void top_eigs(Ctx& c, const Tensor& rho, int64_t want, std::vector<double>* lam,
     *V = gather2d(c, W, seq(N), ord);
     return;
   }
-  const int64_t p = std::min<int64_t>(512, std::min<int64_t>(N, std::max<int64_t>(want + 64, 2 * want)));
+  const int64_t p = std::min<int64_t>(
+      512, std::min<int64_t>(N, c.subspace_oversample > 0 ? want + c.subspace_oversample
+                                                            : std::max<int64_t>(want + 64, 2 * want)));
   c.count("top_eigs[subspace calls]", 1.0, 0);
   // rho comes from tier-precision products: make it exactly Hermitian, or the
   // residuals stall at the level of its anti-Hermitian part

@@ -148,6 +148,10 @@ class Ctx {
   // counts a warning (peps_warning_count); an unconverged subspace iteration
   // is then answered by a direct eigendecomposition (N <= 4096).
   int subspace_max_it = 25;
+  // subspace width of top_eigs for `want` pairs: 0 the default
+  // max(want + 64, 2 want), else want + this many (environment variable
+  // PEPS_HIP_SUBSPACE_OVERSAMPLE)
+  int subspace_oversample = 0;
   // tridiagonal reduction above the register kernel's size (n > 288): 1
   // rocSOLVER sytrd (2-4x faster from n = 512 on), 0 the library's
   // one-workgroup global-memory kernel (environment variable PEPS_HIP_TRD=own)
