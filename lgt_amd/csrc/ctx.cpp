@@ -185,13 +185,16 @@ void Ctx::h2d_async(void* dst, const void* host, size_t bytes) {
     return;
   }
   if (stage_head + need > kStageBytes) {
-    // every copy staged so far was enqueued before stage_ev's last record
-    if (stage_used) PEPS_HIP(hipEventSynchronize(stage_ev));
+    // wrap: every copy staged so far is on the stream before this record, so
+    // once it completes the whole ring is free (one wait per ~16 MiB staged)
+    if (stage_used) {
+      PEPS_HIP(hipEventRecord(stage_ev, stream));
+      PEPS_HIP(hipEventSynchronize(stage_ev));
+    }
     stage_head = 0;
   }
   std::memcpy(stage_host + stage_head, host, bytes);
   PEPS_HIP(hipMemcpyAsync(dst, stage_host + stage_head, bytes, hipMemcpyHostToDevice, stream));
-  PEPS_HIP(hipEventRecord(stage_ev, stream));
   stage_used = true;
   stage_head += need;
 }

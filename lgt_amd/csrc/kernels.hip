@@ -1429,8 +1429,17 @@ void zero_cross_sector(Ctx& c, Tensor& M, const std::vector<int64_t>& rq,
   PEPS_HIP(hipGetLastError());
 }
 
-__global__ void gather2d_kernel(const double* M, int64_t nrow, double* out, int64_t nr,
-                                int64_t nc, int esz, const int64_t* rows, const int64_t* cols) {
+// An index list as a kernel sees it: the uploaded entries, or, for a range
+// off, off+1, ... (p == nullptr), the entry computed.
+struct Idx {
+  const int64_t* p;
+  int64_t off;
+  __device__ __forceinline__ int64_t operator[](int64_t i) const { return p ? p[i] : off + i; }
+};
+
+// out(i, j) = M(rows[i], cols[j]); M column-major nrow x ncol
+__global__ void gather2d_kernel(const double* M, int64_t nrow, double* out, int64_t nr, int64_t nc, int esz,
+                                Idx rows, Idx cols) {
   int64_t n = nr * nc;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
@@ -1449,6 +1458,38 @@ static void check_index(const std::vector<int64_t>& ix, int64_t bound, const cha
 
 static int64_t mat_cols(const Tensor& T) { return T.dims[0] ? T.numel() / T.dims[0] : 0; }
 
+// The index lists of one gather/scatter launch: ranges travel as their first
+// entry, the others are staged to the device together in one copy.
+struct IdxSet {
+  std::shared_ptr<Buffer> keep;
+  Idx ix[3] = {{nullptr, 0}, {nullptr, 0}, {nullptr, 0}};
+};
+static IdxSet device_indices(Ctx& c, std::initializer_list<const std::vector<int64_t>*> lists) {
+  IdxSet s;
+  std::vector<int64_t> packed;
+  size_t at[3] = {0, 0, 0};
+  bool up[3] = {false, false, false};
+  int k = 0;
+  for (const std::vector<int64_t>* v : lists) {
+    bool range = !v->empty();
+    for (size_t i = 1; range && i < v->size(); ++i) range = (*v)[i] == (*v)[0] + int64_t(i);
+    if (range) {
+      s.ix[k].off = (*v)[0];
+    } else {
+      up[k] = true;
+      at[k] = packed.size();
+      packed.insert(packed.end(), v->begin(), v->end());
+    }
+    ++k;
+  }
+  if (!packed.empty()) {
+    auto* base = static_cast<const int64_t*>(c.scratch_upload(packed.data(), packed.size() * 8, &s.keep));
+    for (int j = 0; j < 3; ++j)
+      if (up[j]) s.ix[j].p = base + at[j];
+  }
+  return s;
+}
+
 Tensor gather2d(Ctx& c, const Tensor& M, const std::vector<int64_t>& rows,
                 const std::vector<int64_t>& cols) {
   PEPS_CHECK(M.rank() >= 1, PS_ESHAPE, "gather2d: matrix expected");
@@ -1456,21 +1497,18 @@ Tensor gather2d(Ctx& c, const Tensor& M, const std::vector<int64_t>& rows,
   check_index(cols, mat_cols(M), "gather2d cols");
   Tensor out = c.alloc({int64_t(rows.size()), int64_t(cols.size())}, M.cplx);
   if (rows.empty() || cols.empty()) return out;
-  std::shared_ptr<Buffer> k1, k2;
-  auto* dr = (const int64_t*)c.scratch_upload(rows.data(), rows.size() * 8, &k1);
-  auto* dc = (const int64_t*)c.scratch_upload(cols.data(), cols.size() * 8, &k2);
+  IdxSet ix = device_indices(c, {&rows, &cols});
   int64_t n = int64_t(rows.size() * cols.size());
   unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
   hipLaunchKernelGGL(gather2d_kernel, dim3(g), dim3(256), 0, c.stream, M.data(), M.dims[0],
-                     out.data(), int64_t(rows.size()), int64_t(cols.size()), M.cplx ? 2 : 1, dr, dc);
+                     out.data(), int64_t(rows.size()), int64_t(cols.size()), M.cplx ? 2 : 1, ix.ix[0], ix.ix[1]);
   PEPS_HIP(hipGetLastError());
   return out;
 }
 
 // dst(rows[i], dst_cols[j]) = src(i, src_cols[j])
 __global__ void scatter_cols_kernel(double* dst, int64_t dnrow, const double* src, int64_t snrow,
-                                    int64_t nr, int64_t nc, int esz, const int64_t* rows,
-                                    const int64_t* scols, const int64_t* dcols) {
+                                    int64_t nr, int64_t nc, int esz, Idx rows, Idx scols, Idx dcols) {
   int64_t n = nr * nc;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
@@ -1490,22 +1528,18 @@ void scatter_cols(Ctx& c, Tensor& dst, const Tensor& src, const std::vector<int6
   check_index(src_cols, mat_cols(src), "scatter_cols src");
   check_index(dst_cols, mat_cols(dst), "scatter_cols dst");
   if (rows.empty() || src_cols.empty()) return;
-  std::shared_ptr<Buffer> k1, k2, k3;
-  auto* dr = (const int64_t*)c.scratch_upload(rows.data(), rows.size() * 8, &k1);
-  auto* ds = (const int64_t*)c.scratch_upload(src_cols.data(), src_cols.size() * 8, &k2);
-  auto* dd = (const int64_t*)c.scratch_upload(dst_cols.data(), dst_cols.size() * 8, &k3);
+  IdxSet ix = device_indices(c, {&rows, &src_cols, &dst_cols});
   int64_t n = int64_t(rows.size() * src_cols.size());
   unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
   hipLaunchKernelGGL(scatter_cols_kernel, dim3(g), dim3(256), 0, c.stream, dst.data(), dst.dims[0],
                      src.data(), src.dims[0], int64_t(rows.size()), int64_t(src_cols.size()),
-                     dst.cplx ? 2 : 1, dr, ds, dd);
+                     dst.cplx ? 2 : 1, ix.ix[0], ix.ix[1], ix.ix[2]);
   PEPS_HIP(hipGetLastError());
 }
 
 // dst(dst_rows[i], cols[j]) = src(src_rows[i], j)
 __global__ void scatter_rows_kernel(double* dst, int64_t dnrow, const double* src, int64_t snrow,
-                                    int64_t nr, int64_t nc, int esz, const int64_t* srows,
-                                    const int64_t* drows, const int64_t* cols) {
+                                    int64_t nr, int64_t nc, int esz, Idx srows, Idx drows, Idx cols) {
   int64_t n = nr * nc;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
@@ -1525,15 +1559,12 @@ void scatter_rows(Ctx& c, Tensor& dst, const Tensor& src, const std::vector<int6
   check_index(cols, mat_cols(dst), "scatter_rows cols");
   PEPS_CHECK(int64_t(cols.size()) <= mat_cols(src), PS_ESHAPE, "scatter_rows: cols");
   if (src_rows.empty() || cols.empty()) return;
-  std::shared_ptr<Buffer> k1, k2, k3;
-  auto* ds = (const int64_t*)c.scratch_upload(src_rows.data(), src_rows.size() * 8, &k1);
-  auto* dd = (const int64_t*)c.scratch_upload(dst_rows.data(), dst_rows.size() * 8, &k2);
-  auto* dc = (const int64_t*)c.scratch_upload(cols.data(), cols.size() * 8, &k3);
+  IdxSet ix = device_indices(c, {&src_rows, &dst_rows, &cols});
   int64_t n = int64_t(src_rows.size() * cols.size());
   unsigned g = unsigned(std::min<int64_t>(cdiv(n, 256), 16384));
   hipLaunchKernelGGL(scatter_rows_kernel, dim3(g), dim3(256), 0, c.stream, dst.data(), dst.dims[0],
                      src.data(), src.dims[0], int64_t(src_rows.size()), int64_t(cols.size()),
-                     dst.cplx ? 2 : 1, ds, dd, dc);
+                     dst.cplx ? 2 : 1, ix.ix[0], ix.ix[1], ix.ix[2]);
   PEPS_HIP(hipGetLastError());
 }
 
