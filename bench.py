@@ -376,6 +376,11 @@ def main():
     ap.add_argument("--tau", type=float, default=0.02)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--class-timing", default="gemm", choices=["gemm", "all", "off"],
+                    help="HIP-event timing in the timed steps: 'gemm' (default) times the tier's MFMA GEMM class, "
+                         "the dominant kernel that `roofline` reports; 'all' times every kernel class for the "
+                         "per-class breakdown (`kernel_classes`), at ~4%% of the step; 'off' times the steps only "
+                         "(no roofline)")
     ap.add_argument("--energy", action="store_true", help="also report E/site after the timed steps")
     ap.add_argument("--compare-tier", default=None, choices=["fp64", "fp32", "bf16"],
                     help="after the timed steps, run the same steps from the same initial state in this tier "
@@ -464,7 +469,7 @@ def main():
         warm += n1
         wtimes += tw
     ctx.stats_reset()
-    ctx.set_timing(True)
+    ctx.set_timing(a.class_timing != "off", gemm_only=a.class_timing == "gemm")
     ctx.sync()
     c0 = _clocks()
     t0 = time.perf_counter()
@@ -498,6 +503,8 @@ def main():
                                   "full-update step, copy-calibrated (factors 2.0 and 1.0); profiles/")
     groof = roofline_of(gcls, gk, a.tier)
     groof["traffic"] = PROFILED_TRAFFIC.get((a.nx, a.ny, a.D, a.chi, a.gd, a.tier, gcls))
+    if a.class_timing == "off":
+        roof = groof = None
     kernel_time = sum(v["ms"] for v in kern.values())
     out = {
         "metric": METRIC,
@@ -528,6 +535,7 @@ def main():
                    "boundary_bond_max": int(bmax["flops"]),
                    "flop_per_step": total_flops / max(steps, 1),
                    "tflops_per_s": total_flops / max(elapsed, 1e-9) / 1e12,
+                   "class_timing": a.class_timing,
                    "kernel_time_share": kernel_time / max(1e-9, 1e3 * elapsed),
                    "peps_bond_dims_at_start": bond_dims,
                    "speedup_vs_plain_fp64_gpu": 1.0 if a.tier == "fp64" else None},

@@ -120,7 +120,10 @@ int32_t peps_set_compress_cut(peps_ctx* ctx, double cut);
    warning (peps_warning_count); an unconverged subspace iteration is then
    answered by a direct eigendecomposition (matrices up to 4096). */
 int32_t peps_set_solver_limits(peps_ctx* ctx, int32_t subspace_max_it, int32_t jacobi_max_sweeps);
-/* per-kernel-class HIP-event timing on/off, and the collected counters */
+/* per-kernel-class HIP-event timing and the collected counters: on = 0 off,
+   1 every class, 2 only the MFMA GEMM classes of the precision tiers
+   (gemm_f64_tiled, gemm_f32_tiled, gemm_bf16): two events per timed launch
+   cost about 4% of a full-update step when every class is timed */
 int32_t peps_set_timing(peps_ctx* ctx, int32_t on);
 int32_t peps_stats_count(peps_ctx* ctx, int32_t* n);
 int32_t peps_stats_get(peps_ctx* ctx, int32_t i, char* name, int32_t name_len, double* flops,

@@ -165,8 +165,10 @@ int32_t peps_set_solver_limits(peps_ctx* ctx, int32_t subspace_max_it, int32_t j
 int32_t peps_set_timing(peps_ctx* ctx, int32_t on) {
   if (!ctx) return PEPS_EARG;
   return guard(ctx->c.get(), [&] {
+    PEPS_CHECK(on >= 0 && on <= 2, peps::PS_EARG, "set_timing: mode 0, 1 or 2");
     ctx->c->resolve_timing();
     ctx->c->timing = on != 0;
+    ctx->c->timing_gemm_only = on == 2;
   });
 }
 

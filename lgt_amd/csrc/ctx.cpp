@@ -125,8 +125,10 @@ static hipEvent_t take_event(Ctx& c) {
   return e;
 }
 
-void Ctx::tbegin(const std::string& /*cls*/, hipEvent_t* a) {
+void Ctx::tbegin(const std::string& cls, hipEvent_t* a) {
+  *a = nullptr;
   if (!timing) return;
+  if (timing_gemm_only && cls != "gemm_f64_tiled" && cls != "gemm_f32_tiled" && cls != "gemm_bf16") return;
   *a = take_event(*this);
   PEPS_HIP(hipEventRecord(*a, stream));
 }

@@ -190,6 +190,7 @@ class Ctx {
   // Per kernel-class accounting ("gemm", "permute", "svd", ...).
   std::map<std::string, KernelStat> stats;
   bool timing = false;
+  bool timing_gemm_only = false;  // time only the MFMA GEMM classes of the tiers
   struct PendingEv {
     std::string cls;
     hipEvent_t a, b;

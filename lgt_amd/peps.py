@@ -199,8 +199,11 @@ class Context:
         """Singular-value cut of the boundary compression (1e-14 by default)."""
         self.check(lib().peps_set_compress_cut(self.h, float(cut)))
 
-    def set_timing(self, on: bool):
-        self.check(lib().peps_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, on, gemm_only: bool = False):
+        """Per-kernel-class HIP-event timing; gemm_only times just the MFMA
+        GEMM classes of the tiers (the event pairs of every class cost about
+        4% of a full-update step)."""
+        self.check(lib().peps_set_timing(self.h, 0 if not on else (2 if gemm_only else 1)))
 
     def sync(self):
         self.check(lib().peps_sync(self.h))

@@ -353,3 +353,31 @@ def test_solve_herm_sector_blocks(ctx, case):
     Xr = O.solve_herm(N.astype(complex), S.astype(complex), 1e-10)
     assert (after - before) == (1 if case == "split" else 0)
     assert np.linalg.norm(X - Xr.real) <= 1e-8 * np.linalg.norm(Xr)
+
+
+def test_timing_modes():
+    """peps_set_timing: mode 2 times only the tiers' MFMA GEMM classes (the
+    bench's default, the event pairs of every class cost ~4% of a step), mode
+    1 every class, other modes are refused."""
+    from lgt_amd import peps
+    rng = np.random.default_rng(5)
+    A, B = rng.standard_normal((256, 192)), rng.standard_normal((192, 320))
+    H = rng.standard_normal((64, 64))
+    H = (H + H.T)[None]
+    ctx = peps.Context(0)
+    try:
+        for gemm_only in (True, False):
+            ctx.stats_reset()
+            ctx.set_timing(True, gemm_only=gemm_only)
+            C_ = peps.gemm(A, B, ctx=ctx)
+            peps.eigh(H, ctx=ctx)
+            st = ctx.stats()
+            ctx.set_timing(False)
+            np.testing.assert_allclose(C_, A @ B, rtol=0, atol=1e-12 * np.abs(A @ B).max())
+            assert st["gemm_f64_tiled"]["ms"] > 0 and st["gemm_f64_tiled"]["launches"] == 1
+            timed_other = [k for k, v in st.items() if v["ms"] > 0 and k != "gemm_f64_tiled"]
+            assert bool(timed_other) == (not gemm_only), timed_other
+        with pytest.raises(peps.PepsError):
+            ctx.check(peps.lib().peps_set_timing(ctx.h, 3))
+    finally:
+        ctx.close()
