@@ -352,6 +352,87 @@ def run_steps(gp, ctx, a, count, label, deadline):
     return len(times), als, times
 
 
+def run_c4(a):
+    """The batched-sweep configuration: independent PEPS instances with
+    per-instance sizes and couplings (Lx, Ly uniform in the --c4-L range, D in
+    the --c4-D range, chi = min(4 D^2, 144), m uniform in [0.1, 2], g^2 uniform
+    in [0.5, 2], seeds base+i), each brought to its D by 3 simple-update steps
+    and normalised (not timed), then --c4-steps full-update steps each at
+    tau = 0.02 (ALS <= 20 sweeps, tol 1e-10).  Every instance has its own
+    context and stream; --c4-workers host threads step instances
+    concurrently (the library releases the GIL), each thread an instance at a
+    time; the timed phase opens each instance's context, uploads its state,
+    runs its steps and closes it.  Prints one JSON line: instances/s over the
+    timed phase and the p50/p99 of the per-step wall latencies."""
+    from concurrent.futures import ThreadPoolExecutor
+    from lgt_amd import lattice, peps
+    lo_l, hi_l = [int(x) for x in a.c4_L.split(",")]
+    lo_d, hi_d = [int(x) for x in a.c4_D.split(",")]
+    rng = np.random.default_rng(a.seed)
+    inst = []
+    for i in range(a.c4_n):
+        lx, ly = int(rng.integers(lo_l, hi_l + 1)), int(rng.integers(lo_l, hi_l + 1))
+        D = int(rng.integers(lo_d, hi_d + 1))
+        inst.append(dict(i=i, nx=lx, ny=ly, D=D, chi=min(4 * D * D, 144), m=float(rng.uniform(0.1, 2.0)),
+                         g=float(np.sqrt(rng.uniform(0.5, 2.0))), seed=a.seed + 1 + i))
+
+    def setup(p):
+        # the state goes back to the host and its context is closed, so that
+        # at most `workers` device pools exist at a time
+        ctx = peps.Context(0)
+        try:
+            gp = make_state(ctx, p["nx"], p["ny"], a.gd, p["D"], p["seed"], 3, p["g"], 1.0, p["m"])
+            normalise(gp, p["nx"], p["ny"], p["chi"])
+            return gp.to_arrays()
+        finally:
+            ctx.close()
+
+    def run(p):
+        ctx = peps.Context(0)
+        try:
+            ctx.set_tier(a.tier)
+            s = p["state"]
+            gp = peps.GaugePEPS.from_arrays(s["tensors"], s["lh"], s["lv"], s["qh"], s["qv"],
+                                            lattice.staggered_charges(p["nx"], p["ny"]), a.gd, ctx=ctx)
+            lat = []
+            for _ in range(a.c4_steps):
+                t0 = time.perf_counter()
+                peps.trotter_step_v4(gp, p["g"], 1.0, p["m"], 0.02, p["D"], p["chi"], 20, 1e-10, renorm=True,
+                                     positive_cost=True)
+                ctx.sync()
+                lat.append(time.perf_counter() - t0)
+            return lat
+        finally:
+            ctx.close()
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(a.c4_workers) as ex:
+        for p, st in zip(inst, ex.map(setup, inst)):
+            p["state"] = st
+    log(f"c4: {len(inst)} instances set up in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(a.c4_workers) as ex:
+        lats = list(ex.map(run, inst))
+    wall = time.perf_counter() - t0
+    flat = np.array([x for l in lats for x in l])
+    for p, l in zip(inst, lats):
+        log(f"c4: instance {p['i']:2d} {p['nx']}x{p['ny']} D={p['D']} chi={p['chi']} m={p['m']:.3f} "
+            f"g={p['g']:.3f}: steps {', '.join(f'{x:.2f}' for x in l)} s")
+    print(json.dumps({
+        "metric": "batched full-update sweep: instances/s and per-step latency",
+        "value": len(inst) / wall, "unit": "instances/s", "n_gpus": 1, "higher_is_better": True,
+        "p50_step_ms": float(np.percentile(flat, 50) * 1e3), "p99_step_ms": float(np.percentile(flat, 99) * 1e3),
+        "steps_per_instance": a.c4_steps, "wall_s": wall,
+        "dtype": {"fp64": "f64", "fp32": "f32 (MFMA, FP32 accumulate; FP64 factorisations)",
+                  "bf16": "bf16 (MFMA, FP32 accumulate; FP64 factorisations)"}[a.tier],
+        "data": "synthetic (seeded random gauge-invariant PEPS per instance, simple-update initialised)",
+        "config": {"workload": f"{len(inst)} independent instances, Lx,Ly in [{lo_l},{hi_l}], D in [{lo_d},{hi_d}], "
+                               "chi = min(4 D^2, 144), m in [0.1,2], g^2 in [0.5,2], tau 0.02",
+                   "workers": a.c4_workers, "tier": a.tier,
+                   "instances": [{k: p[k] for k in ("nx", "ny", "D", "chi", "m", "g", "seed")} for p in inst]},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -395,6 +476,14 @@ def main():
                     help="only time the real symmetric eigensolver (tridiagonal reduction, divide and conquer, "
                          "reflectors) on seeded random matrices of these comma-separated orders, 20 calls each, "
                          "and exit (the ALS normal matrices of the full update are 48..288)")
+    ap.add_argument("--c4", action="store_true",
+                    help="only run the batched-sweep configuration (independent instances with per-instance "
+                         "sizes and couplings, see run_c4) and print its own JSON line")
+    ap.add_argument("--c4-n", type=int, default=64)
+    ap.add_argument("--c4-L", default="4,10", help="lattice side range (inclusive)")
+    ap.add_argument("--c4-D", default="2,6", help="bond dimension range (inclusive)")
+    ap.add_argument("--c4-steps", type=int, default=20)
+    ap.add_argument("--c4-workers", type=int, default=8)
     ap.add_argument("--summarize-rocprof", metavar="DIR", default=None,
                     help="only summarise the rocprofv3 output under DIR (with --bench-log: the log of "
                          "the profiled run, whose clock window selects the timed region) and exit")
@@ -404,6 +493,9 @@ def main():
         a.compare_tier = "fp64"
     if a.summarize_rocprof:
         summarize_rocprof(a.summarize_rocprof, a.bench_log)
+        return
+    if a.c4:
+        run_c4(a)
         return
     if a.eigh_bench:
         from lgt_amd import peps

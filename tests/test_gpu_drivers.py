@@ -99,3 +99,24 @@ def test_ite_ground_state_v4_driver(ctx):
     e_gpu = peps.compute_energy_v4(gd, G, T, M, 16)
     e_ora = O.compute_energy_v4(go, G, T, M, 16)
     assert abs(e_gpu - e_ora) <= 1e-8 * abs(e_ora), (e_gpu, e_ora)
+
+
+def test_batched_sweep_driver(capsys):
+    """bench.py --c4 at a tiny size (3 instances, 3x3..4x4, D=2, one step,
+    two workers): every instance steps, the JSON line carries instances/s and
+    the per-step latency percentiles, and the instance list follows the seed."""
+    import argparse
+    import json
+    import bench
+    a = argparse.Namespace(c4_n=3, c4_L="3,4", c4_D="2,2", c4_steps=1, c4_workers=2, seed=5, gd=2,
+                           tier="fp64")
+    print("c4 driver 3 instances", flush=True)
+    bench.run_c4(a)
+    line = [ln for ln in capsys.readouterr().out.splitlines() if ln.startswith("{")][-1]
+    d = json.loads(line)
+    assert d["unit"] == "instances/s" and d["value"] > 0
+    assert 0 < d["p50_step_ms"] <= d["p99_step_ms"]
+    inst = d["config"]["instances"]
+    assert len(inst) == 3 and all(3 <= p["nx"] <= 4 and 3 <= p["ny"] <= 4 and p["D"] == 2 for p in inst)
+    assert all(p["chi"] == 16 and 0.1 <= p["m"] <= 2.0 for p in inst)
+    assert [p["seed"] for p in inst] == [6, 7, 8]
