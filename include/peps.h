@@ -177,6 +177,25 @@ int32_t peps_absorb_row(peps_ctx* ctx, int32_t from_top, int32_t is_complex, int
                         const int64_t* mps_dims, const double* const* mps,
                         const int64_t* row_dims, const double* const* row,
                         peps_tensor_list** out);
+/* dmrg_ground_state (mps_lgt.jl:418-467) with two_site_update! (:384-416):
+   two-site DMRG of a real MPO of n >= 2 sites, W[p] (Dl, Dr, d, d) column-major
+   (bra index, then ket) with dimensions mpo_dims[4p..4p+3], from the start
+   MPS psi0[p] (Dl, Dr, d), dimensions psi_dims[3p..3p+2] (the reference
+   draws a random one when none is given; the caller passes it here).  The
+   start is compressed to bond D and normalised; bonds grow up to D, at most
+   nsweeps sweeps, stopping once a sweep changes the energy by less than etol;
+   each local ground state by restarted Lanczos (krylovdim vectors, residual
+   <= tol, at most maxiter restarts), split by an SVD keeping values above
+   1e-12 of the largest.  Returns the last update's energy, the sweeps run
+   (sweeps may be NULL) and the MPS.  Real tensors only. */
+int32_t peps_dmrg(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
+                  const int64_t* psi_dims, const double* const* psi0, int32_t D, int32_t nsweeps, double tol,
+                  int32_t maxiter, int32_t krylovdim, double etol, double* energy, int32_t* sweeps,
+                  peps_tensor_list** psi_out);
+/* mpo_expect (mps_lgt.jl:308-315): <psi|W|psi> of a real MPS and MPO (not
+   divided by <psi|psi>) */
+int32_t peps_mpo_expect(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
+                        const int64_t* psi_dims, const double* const* psi, double* out);
 /* compress_mps! (finite_peps_boundary_mps.jl:70-105) */
 int32_t peps_compress_mps(peps_ctx* ctx, int32_t is_complex, int32_t n, const int64_t* dims,
                           const double* const* mps, int32_t chi, double cutoff,

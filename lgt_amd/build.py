@@ -20,7 +20,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("PEPS_ARCH", "gfx950")
 
 SOURCES = ["kernels.hip", "eigh.hip", "ed.hip", "ctx.cpp", "contract.cpp", "linalg.cpp", "model.cpp",
-           "engine.cpp", "mlp.cpp", "api.cpp"]
+           "engine.cpp", "mlp.cpp", "dmrg.cpp", "api.cpp"]
 
 CFLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

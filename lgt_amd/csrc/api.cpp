@@ -400,6 +400,33 @@ int32_t peps_compress_mps(peps_ctx* ctx, int32_t is_complex, int32_t n, const in
   });
 }
 
+int32_t peps_dmrg(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
+                  const int64_t* psi_dims, const double* const* psi0, int32_t D, int32_t nsweeps, double tol,
+                  int32_t maxiter, int32_t krylovdim, double etol, double* energy, int32_t* sweeps,
+                  peps_tensor_list** psi_out) {
+  if (!ctx || !mpo_dims || !mpo || !psi_dims || !psi0 || !energy || !psi_out || n < 2 || D < 1 || nsweeps < 1 ||
+      maxiter < 1 || krylovdim < 1 || !(tol > 0.0) || !(etol >= 0.0))
+    return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    auto W = upload_list(*ctx->c, false, n, 4, mpo_dims, mpo);
+    auto P = upload_list(*ctx->c, false, n, 3, psi_dims, psi0);
+    peps::DmrgResult r = peps::dmrg_ground_state(*ctx->c, W, P, D, nsweeps, tol, maxiter, krylovdim, etol);
+    *energy = r.E;
+    if (sweeps) *sweeps = r.sweeps;
+    *psi_out = make_list(ctx->c, r.psi);
+  });
+}
+
+int32_t peps_mpo_expect(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
+                        const int64_t* psi_dims, const double* const* psi, double* out) {
+  if (!ctx || !mpo_dims || !mpo || !psi_dims || !psi || !out || n < 1) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    auto W = upload_list(*ctx->c, false, n, 4, mpo_dims, mpo);
+    auto P = upload_list(*ctx->c, false, n, 3, psi_dims, psi);
+    *out = peps::mpo_expect(*ctx->c, W, P);
+  });
+}
+
 int32_t peps_boundary(peps_state* st, int32_t from_top, int32_t stop_row, int32_t chi,
                       peps_tensor_list** out) {
   if (!st || !out || chi < 1) return PEPS_EARG;

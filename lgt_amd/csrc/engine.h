@@ -85,6 +85,19 @@ MPS absorb_row_from_bottom(Ctx& c, const MPS& mps, const MPS& row);
 void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT, bool relative = false);
 // Same truncated MPS from left Gram matrices and top-chi eigenvectors.
 void compress_mps_gram(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT);
+
+// ── MPS / DMRG (dmrg.cpp; mps_lgt.jl) ──────────────────────────────────────
+// MPS tensors (Dl, Dr, d), MPO tensors (Dl, Dr, k, b); real only.
+struct DmrgResult {
+  double E = 0.0;
+  int sweeps = 0;
+  std::vector<Tensor> psi;
+};
+// dmrg_ground_state (mps_lgt.jl:418-467) with two_site_update! (:384-416)
+DmrgResult dmrg_ground_state(Ctx& c, const std::vector<Tensor>& W, std::vector<Tensor> psi, int D, int nsweeps,
+                             double tol, int maxiter, int krylovdim, double etol);
+// mpo_expect (mps_lgt.jl:308-315): <psi|W|psi>
+double mpo_expect(Ctx& c, const std::vector<Tensor>& W, const std::vector<Tensor>& psi);
 // absorb_row_from_top/bottom + compress_mps!, literal or fused Gram form.
 // key_row > 0 names the row just absorbed: the Gram eigensolves of this
 // boundary warm-start from the previous computation at the same position.

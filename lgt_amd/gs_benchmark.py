@@ -110,14 +110,77 @@ def run_one_v4(task_id=1, outdir=None, ctx=None, n_ite=V4_N_ITE, nx=V4_NX, ny=V4
     return row
 
 
+# gs_benchmark_mps.jl:26-39: the same (m, g) grid as gs_benchmark_v3.jl, the
+# 3x4 lattice at dg = 1, DMRG bond 40, at most 8 sweeps, Lambda = 5
+MPS_PARAM_GRID = [(0.25, 0.25), (0.25, 0.50), (0.25, 1.00), (0.25, 2.00), (0.25, 4.00),
+                  (0.50, 0.25), (0.50, 0.50), (0.50, 2.00), (0.50, 4.00)]
+MPS_NX, MPS_NY, MPS_DG, MPS_T_HOP, MPS_D, MPS_NSW, MPS_LAM = 3, 4, 1, 1.0, 40, 8, 5.0
+MPS_COLUMNS = ("task", "m", "g", "E_ed", "E_mps", "dE_rel", "nf_mean_ed", "nf_mean_mps", "nf_even_ed",
+               "nf_even_mps", "nf_odd_ed", "nf_odd_mps", "E2_mean_ed", "E2_mean_mps", "gauss_viol", "bond")
+
+
+def run_one_mps(task_id=1, outdir=None, ctx=None, nx=MPS_NX, ny=MPS_NY, dg=MPS_DG, D=MPS_D, nsweeps=MPS_NSW,
+                lam=MPS_LAM, start="staggered", seed=1, log=None):
+    """run_one_mps (gs_benchmark_mps.jl:41-96): the sector's ED ground state
+    and the Gauss-penalised DMRG one (GPU), compared in one CSV row keyed by
+    MPS_COLUMNS (gs_mps_task<id>.csv).  start "staggered" is the reference's
+    product start in the target sector; "random" a seeded random MPS."""
+    from . import mps
+    m, g = MPS_PARAM_GRID[(task_id - 1) % len(MPS_PARAM_GRID)]
+    gd = 2 * dg + 1
+    gch = lattice.staggered_charges(nx, ny)
+    ctx = ctx or peps.default_context()
+    t0 = time.time()
+    ed = peps.build_basis(nx, ny, dg, gch, ctx=ctx)
+    peps.build_hamiltonian(ed, g, MPS_T_HOP, m)
+    E_ed, psi_ed = peps.find_ground_state(ed)
+    obs_ed = peps.measure_ED(ed, psi_ed)
+    if log:
+        log(f"  [ED] E0 = {E_ed:.8f}  ({ed.dim} states, {time.time() - t0:.1f}s)")
+    t1 = time.time()
+    dims = mps.node_dims(nx, ny, gd)
+    Hpen = mps.build_penalized_H(nx, ny, gd, g, MPS_T_HOP, m, gch, lam, ctx=ctx)
+    psi0 = mps.staggered_mps(nx, ny, gd) if start == "staggered" else None
+    _, psi, sweeps = mps.dmrg_ground_state(Hpen, dims, D=D, nsweeps=nsweeps, psi0=psi0, seed=seed, ctx=ctx)
+    Hbare = mps.build_H_mpo(nx, ny, gd, g, MPS_T_HOP, m, ctx=ctx)
+    E_mps = peps.mpo_expect(Hbare, psi, ctx=ctx) / mps.mps_overlap(psi, ctx=ctx)
+    obs = mps.measure_mps(psi, nx, ny, gd, ctx=ctx)
+    viol = mps.gauss_violation(psi, nx, ny, gd, gch, ctx=ctx)
+    bond = max(t.shape[1] for t in psi)
+    t_mps = time.time() - t1
+    if log:
+        log(f"  [MPS] E = {E_mps:.8f}  bond={bond}  Gauss-viol={viol:.2e}  ({t_mps:.1f}s, {sweeps} sweeps)")
+        log(f"  dE/|E| = {abs(E_mps - E_ed) / abs(E_ed):.2e}")
+    row = dict(task=int(task_id), m=m, g=g, E_ed=E_ed, E_mps=E_mps,
+               dE_rel=abs(E_mps - E_ed) / max(abs(E_ed), 1e-12),
+               nf_mean_ed=obs_ed["nf_mean"], nf_mean_mps=obs["nf_mean"], nf_even_ed=obs_ed["nf_even"],
+               nf_even_mps=obs["nf_even"], nf_odd_ed=obs_ed["nf_odd"], nf_odd_mps=obs["nf_odd"],
+               E2_mean_ed=obs_ed["E2_mean"], E2_mean_mps=obs["E2_mean"], gauss_viol=viol, bond=int(bond))
+    row = {k: (float(v) if not isinstance(v, int) else v) for k, v in row.items()}
+    row["time_s"] = t_mps
+    if outdir:
+        path = os.path.join(outdir, f"gs_mps_task{int(task_id)}.csv")
+        write_csv([row], MPS_COLUMNS, path)
+        if log:
+            log(f"  Saved: {path}")
+    return row
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("task_id", nargs="?", type=int, default=int(os.environ.get("SLURM_ARRAY_TASK_ID", "1")))
-    ap.add_argument("--outdir", default=os.path.join("results", "gs_bench_v4"))
+    ap.add_argument("--outdir", default=None)
     ap.add_argument("--n-ite", type=int, default=V4_N_ITE)
     ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--mps", action="store_true", help="the MPS/DMRG benchmark task (gs_benchmark_mps.jl)")
+    ap.add_argument("--start", default="staggered", choices=["staggered", "random"])
     a = ap.parse_args(argv)
-    run_one_v4(a.task_id, a.outdir, n_ite=a.n_ite, seed=a.seed, log=lambda s: print(s, flush=True))
+    log = lambda s: print(s, flush=True)  # noqa: E731
+    if a.mps:
+        run_one_mps(a.task_id, a.outdir or "results", start=a.start, seed=a.seed or 1, log=log)
+    else:
+        run_one_v4(a.task_id, a.outdir or os.path.join("results", "gs_bench_v4"), n_ite=a.n_ite, seed=a.seed,
+                   log=log)
 
 
 if __name__ == "__main__":

@@ -77,6 +77,11 @@ _SIGS = {
     "peps_dl_traced": (C.c_int32, [_vp, C.c_int32, _i64p, _dp, _dp]),
     "peps_absorb_row": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _i64p, C.POINTER(_dp), _i64p, C.POINTER(_dp), C.POINTER(_vp)]),
     "peps_compress_mps": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p, C.POINTER(_dp), C.c_int32, C.c_double, C.POINTER(_vp)]),
+    "peps_dmrg": (C.c_int32, [_vp, C.c_int32, _i64p, C.POINTER(_dp), _i64p, C.POINTER(_dp), C.c_int32, C.c_int32,
+                              C.c_double, C.c_int32, C.c_int32, C.c_double, C.POINTER(C.c_double),
+                              C.POINTER(C.c_int32), C.POINTER(_vp)]),
+    "peps_mpo_expect": (C.c_int32, [_vp, C.c_int32, _i64p, C.POINTER(_dp), _i64p, C.POINTER(_dp),
+                                    C.POINTER(C.c_double)]),
     "peps_boundary": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
     "peps_norm2": (C.c_int32, [_vp, C.c_int32, _dp]),
     "peps_bond_env": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
@@ -667,6 +672,29 @@ def compress_mps(mps, chi, cutoff=1e-14, ctx=None):
     h = _vp()
     ctx.check(lib().peps_compress_mps(ctx.h, int(cp), len(a), _iptr(d), p, chi, cutoff, C.byref(h)))
     return _take_list(ctx, h)
+
+
+def dmrg(mpo, psi0, D, nsweeps=8, tol=1e-7, maxiter=60, krylovdim=12, etol=1e-7, ctx=None):
+    """dmrg_ground_state (mps_lgt.jl:418-467) on the GPU for a real MPO
+    (tensors (Dl, Dr, d, d)) from the start MPS psi0 (tensors (Dl, Dr, d)).
+    Returns (energy, MPS, sweeps run)."""
+    ctx = ctx or default_context()
+    wa, wd, wp = _pack(mpo, False)
+    pa, pd, pp = _pack(psi0, False)
+    E, sw, h = C.c_double(), C.c_int32(), _vp()
+    ctx.check(lib().peps_dmrg(ctx.h, len(wa), _iptr(wd), wp, _iptr(pd), pp, int(D), int(nsweeps), float(tol),
+                              int(maxiter), int(krylovdim), float(etol), C.byref(E), C.byref(sw), C.byref(h)))
+    return E.value, _take_list(ctx, h), sw.value
+
+
+def mpo_expect(mpo, psi, ctx=None):
+    """mpo_expect (mps_lgt.jl:308-315): <psi|W|psi> on the GPU (real)."""
+    ctx = ctx or default_context()
+    wa, wd, wp = _pack(mpo, False)
+    pa, pd, pp = _pack(psi, False)
+    out = C.c_double()
+    ctx.check(lib().peps_mpo_expect(ctx.h, len(wa), _iptr(wd), wp, _iptr(pd), pp, C.byref(out)))
+    return out.value
 
 
 def boundary_top(gp: GaugePEPS, stop_row, chi):

@@ -1,0 +1,70 @@
+"""The MPS/DMRG oracle (oracle/mps_oracle.py) against the ED oracle, and the
+library's host-side term lists (lgt_amd/mps.py, mps_lgt.jl:83-116,474-514)
+against the oracle's full-space Hamiltonian.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import ed_oracle as ED
+from oracle import mps_oracle as M
+from oracle import peps_oracle as P
+
+
+def _sector(G, gch):
+    return np.all(np.abs(G - np.asarray(gch)[None]) < 1e-12, axis=(1, 2))
+
+
+@pytest.mark.parametrize("charges", ["staggered", "string"])
+def test_full_hamiltonian_sector_spectrum_matches_ed(charges):
+    """Restricted to a Gauss-law sector, the full-space Hamiltonian of the
+    snake chain has the ED Hamiltonian's spectrum (2x2, gd = 3, i.e. the
+    reference's dg = 1), and the sector has the ED basis' size."""
+    nx, ny, gd = 2, 2, 3
+    if charges == "staggered":
+        gch = np.asarray(P.staggered_charges(nx, ny))
+    else:  # string_charges (mps_lgt.jl:640-643), iy_str = 1
+        gch = np.zeros((nx, ny), np.int64)
+        gch[0, 0], gch[1, 0] = 1, -2
+    H = M.full_hamiltonian(nx, ny, gd, 1.0, 1.0, 0.5)
+    assert np.abs(H - H.T).max() < 1e-14
+    sel = _sector(M.gauss_values(nx, ny, gd), gch)
+    b = ED.build_basis(nx, ny, 1, gch)
+    assert sel.sum() == b.dim > 0
+    w_full = np.linalg.eigvalsh(H[np.ix_(sel, sel)])
+    w_ed = np.linalg.eigvalsh(ED.build_hamiltonian(b, 1.0, 1.0, 0.5).toarray())
+    assert np.abs(w_full - w_ed).max() < 1e-12
+
+
+def test_library_terms_sum_to_the_full_hamiltonian():
+    """lgt_terms: the sum of coef x (product of the listed site operators)
+    is the oracle's full-space Hamiltonian; gauss_penalty_terms sum to
+    Lambda sum_i ((G_i - g_i)^2 - g_i^2), diagonal in the product basis."""
+    from lgt_amd import mps
+    nx, ny, gd, lam = 2, 2, 3, 7.0
+    dims = M.node_dims(nx, ny, gd)
+    assert mps.node_dims(nx, ny, gd) == dims
+    assert mps.snake_nodes(3, 2)[0] == [(1, 1), (2, 1), (3, 1), (3, 2), (2, 2), (1, 2)]
+
+    def total(terms):
+        return sum(c * M._apply_local(dims, ops) for c, ops in terms)
+
+    H = total(mps.lgt_terms(nx, ny, gd, 1.3, 0.7, 0.4))
+    assert np.abs(H - M.full_hamiltonian(nx, ny, gd, 1.3, 0.7, 0.4)).max() < 1e-13
+    gch = np.asarray(P.staggered_charges(nx, ny))
+    Hg = total(mps.gauss_penalty_terms(nx, ny, gd, gch, lam))
+    G = M.gauss_values(nx, ny, gd)
+    want = lam * (((G - gch[None]) ** 2).sum(axis=(1, 2)) - (gch ** 2).sum())
+    assert np.abs(Hg - np.diag(want)).max() < 1e-12
+
+
+def test_mpo_to_dense_of_product_terms():
+    """mpo_to_dense of a bond-1 term MPO is the Kronecker product of its site
+    operators (first site slowest), and of a direct sum the sum."""
+    from lgt_amd import mps
+    rng = np.random.default_rng(3)
+    dims = [2, 3, 2]
+    A, B = rng.standard_normal((2, 2)), rng.standard_normal((2, 2))
+    W1 = mps.term_mpo(dims, (0.5, {0: A, 2: B}))
+    W2 = mps.term_mpo(dims, (-1.5, {1: rng.standard_normal((3, 3))}))
+    D1, D2 = M.mpo_to_dense(W1), M.mpo_to_dense(W2)
+    assert np.abs(D1 - 0.5 * np.kron(np.kron(A, np.eye(3)), B)).max() < 1e-15
+    assert np.abs(M.mpo_to_dense(mps.mpo_axpby(1.0, W1, 2.0, W2)) - (D1 + 2.0 * D2)).max() < 1e-14
