@@ -105,66 +105,90 @@ void small_sym_eig(int n, std::vector<double> A, std::vector<double>* w, std::ve
 }
 
 // Lowest eigenpair of the Hermitian operator `apply` from the start vector
-// x0 (KrylovKit.eigsolve(..., 1, :SR) in two_site_update!): restarted
-// Lanczos with full reorthogonalisation, Krylov spaces of up to `krylovdim`
-// vectors, restarted from the Ritz vector until its residual
-// ||H x - theta x|| is at most tol (or maxiter restarts).  The Krylov vectors
-// are the columns of one device matrix V, so each of the two classical
-// Gram-Schmidt passes is two GEMMs (V^T w, then w -= V c) and a step reads
-// back two numbers (alpha, beta).
+// x0 (KrylovKit.eigsolve(..., 1, :SR) in two_site_update!): a Krylov space
+// of up to `krylovdim` vectors with thick restarts in the Krylov-Schur
+// manner -- the lowest half of the Ritz vectors is kept, with their images,
+// and the space is extended from the lowest pair's residual -- until that
+// residual ||H x - theta x|| is at most tol (or maxiter restarts).  The basis
+// V and its image HV are device matrices: the projection V^T HV, the two
+// Gram-Schmidt passes and the Ritz combinations are GEMMs, and a new vector
+// reads back one number (its norm).
 template <class F>
 double lanczos_lowest(Ctx& c, F&& apply, const Tensor& x0, double tol, int maxiter, int krylovdim, Tensor* xout) {
   const int64_t N = x0.numel();
-  const int kmax = int(std::max<int64_t>(1, std::min<int64_t>(krylovdim, N)));
-  Tensor x = copy_tensor(c, x0);
+  const int kmax = int(std::max<int64_t>(2, std::min<int64_t>(krylovdim, N)));
+  const int nkeep = std::max(1, kmax / 2);
+  Tensor V = c.zeros({N, kmax}, false), HV = c.zeros({N, kmax}, false);
+  auto col = [&](Tensor& M, int j) { return M.slice_last(j, j + 1); };  // N x 1 view
+  auto put = [&](Tensor& M, int j, const Tensor& v) {
+    PEPS_HIP(hipMemcpyAsync(col(M, j).data(), v.data(), size_t(N) * 8, hipMemcpyDeviceToDevice, c.stream));
+  };
+  Tensor x = copy_tensor(c, x0).reshaped({N, 1});
   {
     const double nx = std::sqrt(norm2sq(c, x));
     if (!(nx > 0.0) || !std::isfinite(nx)) fill_random(c, x, 0x6c616e637a6f73ull);
     scale(c, x, 1.0 / std::sqrt(norm2sq(c, x)));
   }
-  Tensor V = c.zeros({N, kmax}, false);
-  auto col = [&](int j) { return V.slice_last(j, j + 1); };  // N x 1 view
+  put(V, 0, x);
+  put(HV, 0, apply(x.reshaped(x0.dims)).reshaped({N, 1}));
+  int m = 1;
+  Tensor r = copy_tensor(c, col(HV, 0));  // next direction before orthogonalisation
   double theta = 0.0;
+  Tensor y = x;
+  int applied = 1;
   for (int restart = 0; restart < std::max(1, maxiter); ++restart) {
-    PEPS_HIP(hipMemcpyAsync(V.data(), x.data(), size_t(N) * 8, hipMemcpyDeviceToDevice, c.stream));
-    std::vector<double> alpha, beta;
-    for (int j = 0; j < kmax; ++j) {
-      Tensor w = apply(col(j).reshaped(x0.dims)).reshaped({N, 1});
-      Tensor Vj = V.slice_last(0, j + 1);                                    // N x (j+1)
-      Tensor cf = matmul(c, Vj, w, false, false, true, false);              // V^T w
-      alpha.push_back(c.to_host(cf)[j]);
-      axpy_inplace(c, w, matmul(c, Vj, cf), -1.0);
-      Tensor cf2 = matmul(c, Vj, w, false, false, true, false);             // second pass
-      axpy_inplace(c, w, matmul(c, Vj, cf2), -1.0);
-      const double bj = std::sqrt(norm2sq(c, w));
-      beta.push_back(bj);
-      if (j + 1 == kmax || !(bj > 1e-14 * std::max(1.0, std::fabs(alpha.back())))) break;
-      scale(c, w, 1.0 / bj);
-      PEPS_HIP(hipMemcpyAsync(col(j + 1).data(), w.data(), size_t(N) * 8, hipMemcpyDeviceToDevice, c.stream));
+    while (m < kmax) {
+      Tensor Vm = V.slice_last(0, m);
+      for (int pass = 0; pass < 2; ++pass)
+        axpy_inplace(c, r, matmul(c, Vm, matmul(c, Vm, r, false, false, true, false)), -1.0);
+      const double b = std::sqrt(norm2sq(c, r));
+      if (!(b > 1e-14) || !std::isfinite(b)) break;  // invariant subspace
+      scale(c, r, 1.0 / b);
+      put(V, m, r);
+      Tensor hr = apply(r.reshaped(x0.dims)).reshaped({N, 1});
+      ++applied;
+      put(HV, m, hr);
+      r = copy_tensor(c, hr);
+      ++m;
     }
-    const int k = int(alpha.size());
-    std::vector<double> T(size_t(k) * k, 0.0), tw, tv;
-    for (int i = 0; i < k; ++i) {
-      T[size_t(i) * k + i] = alpha[i];
-      if (i + 1 < k) T[size_t(i) * k + i + 1] = T[size_t(i + 1) * k + i] = beta[i];
-    }
-    small_sym_eig(k, T, &tw, &tv);
+    Tensor Vm = V.slice_last(0, m), HVm = HV.slice_last(0, m);
+    std::vector<double> T = c.to_host(matmul(c, Vm, HVm, false, false, true, false));  // column-major m x m
+    std::vector<double> Ts(size_t(m) * m);
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) Ts[size_t(i) * m + j] = 0.5 * (T[size_t(i) + size_t(j) * m] + T[size_t(j) + size_t(i) * m]);
+    std::vector<double> tw, tv;
+    small_sym_eig(m, Ts, &tw, &tv);
     theta = tw[0];
-    // Ritz vector V s and its residual norm |beta_k s_k|
-    std::vector<double> s0(k);
-    for (int i = 0; i < k; ++i) s0[i] = tv[size_t(i) * k + 0];
-    Tensor sv = c.upload({k, 1}, false, s0.data());
-    Tensor y = matmul(c, V.slice_last(0, k), sv);
-    const double res = std::fabs(beta[k - 1] * s0[k - 1]);
-    scale(c, y, 1.0 / std::sqrt(norm2sq(c, y)));
-    x = y.reshaped(x0.dims);
-    c.count("dmrg[lanczos iterations]", double(k), 0);
-    if (res <= tol) break;
-    if (restart == std::max(1, maxiter) - 1) c.count("dmrg[lanczos unconverged]", 1.0, 0);
+    const int keep = std::min(nkeep, m);
+    std::vector<double> Sk(size_t(m) * keep);  // column-major m x keep
+    for (int j = 0; j < keep; ++j)
+      for (int i = 0; i < m; ++i) Sk[size_t(i) + size_t(j) * m] = tv[size_t(i) * m + j];
+    Tensor S = c.upload({m, keep}, false, Sk.data());
+    Tensor Y = matmul(c, Vm, S), HY = matmul(c, HVm, S);  // N x keep
+    y = copy_tensor(c, col(Y, 0));
+    Tensor res = copy_tensor(c, col(HY, 0));
+    axpy_inplace(c, res, y, -theta);
+    const double rn = std::sqrt(norm2sq(c, res));
+    c.count("dmrg[lanczos restarts]", 1.0, 0);
+    if (rn <= tol || m < kmax) break;  // converged, or the space is invariant
+    if (restart == std::max(1, maxiter) - 1) {
+      c.count("dmrg[lanczos unconverged]", 1.0, 0);
+      break;
+    }
+    PEPS_HIP(hipMemcpyAsync(V.data(), Y.data(), size_t(N) * keep * 8, hipMemcpyDeviceToDevice, c.stream));
+    PEPS_HIP(hipMemcpyAsync(HV.data(), HY.data(), size_t(N) * keep * 8, hipMemcpyDeviceToDevice, c.stream));
+    m = keep;
+    r = res;
   }
-  *xout = x;
+  c.count("dmrg[lanczos applications]", double(applied), 0);
+  scale(c, y, 1.0 / std::sqrt(norm2sq(c, y)));
+  *xout = y.reshaped(x0.dims);
   return theta;
 }
+
+// Smallest block dimension of the two-site split that takes the
+// Jordan-Wielandt form instead of the one-workgroup Jacobi SVD.
+constexpr int64_t kDmrgJwMinDim = 64;
 
 // two_site_update! (mps_lgt.jl:384-416) at bond (p, p+1).
 double two_site_update(Ctx& c, std::vector<Tensor>& psi, const std::vector<Tensor>& W, std::vector<Tensor>& L,
@@ -175,15 +199,35 @@ double two_site_update(Ctx& c, std::vector<Tensor>& psi, const std::vector<Tenso
   auto heff = [&](const Tensor& v) { return apply_heff2(c, L[p], W[p], W[p + 1], R[p + 1], v); };
   Tensor g;
   const double E = lanczos_lowest(c, heff, th, tol, maxiter, krylovdim, &g);
-  SVD f = svd_thin(c, g.reshaped({a * k1, k2 * cr}));
-  int64_t cnt = 0;
-  for (double s : f.S) cnt += s > 1e-12 * f.S[0];
-  const int64_t keep = std::max<int64_t>(1, std::min<int64_t>({int64_t(D), cnt, int64_t(f.S.size())}));
-  Tensor U = copy_tensor(c, f.U.slice_last(0, keep));                       // (a k1) x keep
-  Tensor Vt = gather2d(c, f.Vt, seq_range(keep), seq_range(k2 * cr));     // keep x (k2 c)
-  std::vector<double> S(f.S.begin(), f.S.begin() + keep);
-  if (to_right) scale_rows(c, Vt, S);
-  else scale_cols(c, U, S);
+  // the split M = U S V^T of Theta (a k1) x (k2 c), keeping at most D
+  // values above 1e-12 s_0: large blocks through the Jordan-Wielandt form
+  // (the side that becomes canonical orthonormalised, the other carrying S),
+  // small ones through the Jacobi SVD
+  const Tensor Mt = g.reshaped({a * k1, k2 * cr});
+  Tensor U, Vt;
+  if (std::min(a * k1, k2 * cr) >= kDmrgJwMinDim) {
+    Tensor Q, P;
+    if (to_right) {  // U orthonormal, S V^T = (M^T U)^T
+      topk_right_jw(c, permute(c, Mt, {1, 0}), D, 1e-12, true, &Q, &P);
+      U = Q;
+      Vt = copy_tensor(c, permute(c, P, {1, 0}));
+    } else {         // V orthonormal, U S = M V
+      topk_right_jw(c, Mt, D, 1e-12, true, &Q, &P);
+      U = P;
+      Vt = copy_tensor(c, permute(c, Q, {1, 0}));
+    }
+  } else {
+    SVD f = svd_thin(c, Mt);
+    int64_t cnt = 0;
+    for (double s : f.S) cnt += s > 1e-12 * f.S[0];
+    const int64_t k = std::max<int64_t>(1, std::min<int64_t>({int64_t(D), cnt, int64_t(f.S.size())}));
+    U = copy_tensor(c, f.U.slice_last(0, k));                       // (a k1) x k
+    Vt = gather2d(c, f.Vt, seq_range(k), seq_range(k2 * cr));     // k x (k2 c)
+    std::vector<double> S(f.S.begin(), f.S.begin() + k);
+    if (to_right) scale_rows(c, Vt, S);
+    else scale_cols(c, U, S);
+  }
+  const int64_t keep = U.dims[1];
   psi[p] = permute(c, U.reshaped({a, k1, keep}), {0, 2, 1});
   psi[p + 1] = permute(c, Vt.reshaped({keep, k2, cr}), {0, 2, 1});
   if (to_right) L[p + 1] = left_env(c, L[p], W[p], psi[p]);

@@ -217,8 +217,7 @@ constexpr int64_t kJwSweepMinDim = 128;
 // one eigendecomposition of order rows(C) + n instead of a one-workgroup
 // Jacobi iteration.  The selected right parts are orthonormalised by a QR
 // (singular values at rounding level mix their +-s and zero eigenvectors).
-static void topk_right_jw(Ctx& c, const Tensor& M, int64_t kmax, double cut, bool relative, Tensor* Vk,
-                          Tensor* P) {
+void topk_right_jw(Ctx& c, const Tensor& M, int64_t kmax, double cut, bool relative, Tensor* Vk, Tensor* P) {
   TierGuard fp64(c, TIER_FP64);
   const int64_t m = M.dims[0], n = M.dims[1];
   const Tensor C = m > n ? qr_r_only(c, M) : M;

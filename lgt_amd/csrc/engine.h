@@ -86,6 +86,12 @@ void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT, bool rela
 // Same truncated MPS from left Gram matrices and top-chi eigenvectors.
 void compress_mps_gram(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT);
 
+// Truncated right factor of a real M (m x n): Vk (n x k) orthonormal, the
+// leading k right singular vectors with k = max(1, min(kmax, #{s > thr})),
+// thr = cut s_0 (relative) or cut, and P = M Vk, from the Jordan-Wielandt
+// eigenproblem of M's square core (engine.cpp).
+void topk_right_jw(Ctx& c, const Tensor& M, int64_t kmax, double cut, bool relative, Tensor* Vk, Tensor* P);
+
 // ── MPS / DMRG (dmrg.cpp; mps_lgt.jl) ──────────────────────────────────────
 // MPS tensors (Dl, Dr, d), MPO tensors (Dl, Dr, k, b); real only.
 struct DmrgResult {
