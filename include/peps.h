@@ -196,6 +196,27 @@ int32_t peps_dmrg(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const doubl
    divided by <psi|psi>) */
 int32_t peps_mpo_expect(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
                         const int64_t* psi_dims, const double* const* psi, double* out);
+/* The same expectation for a real or complex MPS (psi_complex != 0: every
+   tensor interleaved re/im); the real part of <psi|W|psi>. */
+int32_t peps_mpo_expect_c(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
+                          int32_t psi_complex, const int64_t* psi_dims, const double* const* psi, double* out);
+/* krylov_evolve_step (mps_lgt.jl:546-586), nsteps times, as evolve_mps
+   (mps_quench.jl:42-62) calls it: psi <- exp(-i dt H) psi by Lanczos on the
+   MPS manifold with krylov_m vectors (full reorthogonalisation), each H v by
+   the zip-up MPO application (mpo_apply_mps_zipup, mps_lgt.jl:239-257), sums
+   by mps_axpby (:259-281) and compression by mps_compress! (:283-303), every
+   truncation keeping at most Dmax values above eps times the largest; the
+   state is renormalised after each step.  H real (Dl, Dr, d, d), psi0 real
+   or complex (Dl, Dr, d); returns a complex MPS.  nsteps = 0 returns psi0. */
+int32_t peps_krylov_evolve(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
+                           int32_t psi_complex, const int64_t* psi_dims, const double* const* psi0, double dt,
+                           int32_t nsteps, int32_t krylov_m, int32_t Dmax, double eps, peps_tensor_list** psi_out);
+/* mps_local_expect (mps_lgt.jl:317-330) for nops (site, operator) pairs:
+   out[i] = Re <psi|O_i at chain site sites[i] (0-based)|psi> / <psi|psi>,
+   O_i real (d, d) with op_dims[2 i], op_dims[2 i + 1] = d. */
+int32_t peps_mps_local_expect(peps_ctx* ctx, int32_t n, int32_t psi_complex, const int64_t* psi_dims,
+                              const double* const* psi, int32_t nops, const int32_t* sites, const int64_t* op_dims,
+                              const double* const* ops, double* out);
 /* compress_mps! (finite_peps_boundary_mps.jl:70-105) */
 int32_t peps_compress_mps(peps_ctx* ctx, int32_t is_complex, int32_t n, const int64_t* dims,
                           const double* const* mps, int32_t chi, double cutoff,

@@ -1,6 +1,7 @@
 // C ABI of libpeps_hip.so (declared in include/peps.h).
 #include "peps.h"
 
+#include <cmath>
 #include <cstring>
 #include <set>
 #include <thread>
@@ -424,6 +425,43 @@ int32_t peps_mpo_expect(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const
     auto W = upload_list(*ctx->c, false, n, 4, mpo_dims, mpo);
     auto P = upload_list(*ctx->c, false, n, 3, psi_dims, psi);
     *out = peps::mpo_expect(*ctx->c, W, P);
+  });
+}
+
+int32_t peps_mpo_expect_c(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
+                          int32_t psi_complex, const int64_t* psi_dims, const double* const* psi, double* out) {
+  if (!ctx || !mpo_dims || !mpo || !psi_dims || !psi || !out || n < 1) return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    auto W = upload_list(*ctx->c, false, n, 4, mpo_dims, mpo);
+    auto P = upload_list(*ctx->c, psi_complex != 0, n, 3, psi_dims, psi);
+    *out = peps::mpo_expect(*ctx->c, W, P);
+  });
+}
+
+int32_t peps_krylov_evolve(peps_ctx* ctx, int32_t n, const int64_t* mpo_dims, const double* const* mpo,
+                           int32_t psi_complex, const int64_t* psi_dims, const double* const* psi0, double dt,
+                           int32_t nsteps, int32_t krylov_m, int32_t Dmax, double eps, peps_tensor_list** psi_out) {
+  if (!ctx || !mpo_dims || !mpo || !psi_dims || !psi0 || !psi_out || n < 2 || nsteps < 0 || krylov_m < 1 ||
+      Dmax < 1 || !(eps >= 0.0) || !std::isfinite(dt))
+    return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    auto W = upload_list(*ctx->c, false, n, 4, mpo_dims, mpo);
+    auto P = upload_list(*ctx->c, psi_complex != 0, n, 3, psi_dims, psi0);
+    *psi_out = make_list(ctx->c, peps::krylov_evolve(*ctx->c, W, P, dt, nsteps, krylov_m, Dmax, eps));
+  });
+}
+
+int32_t peps_mps_local_expect(peps_ctx* ctx, int32_t n, int32_t psi_complex, const int64_t* psi_dims,
+                              const double* const* psi, int32_t nops, const int32_t* sites, const int64_t* op_dims,
+                              const double* const* ops, double* out) {
+  if (!ctx || !psi_dims || !psi || n < 1 || nops < 0 || (nops > 0 && (!sites || !op_dims || !ops || !out)))
+    return PEPS_EARG;
+  return guard(ctx->c.get(), [&] {
+    auto P = upload_list(*ctx->c, psi_complex != 0, n, 3, psi_dims, psi);
+    auto O = upload_list(*ctx->c, false, nops, 2, op_dims, ops);
+    std::vector<int> st(sites, sites + nops);
+    std::vector<double> v = peps::mps_local_expect(*ctx->c, P, st, O);
+    std::copy(v.begin(), v.end(), out);
   });
 }
 

@@ -1,8 +1,11 @@
 // Two-site DMRG of a matrix-product operator on the GPU (mps_lgt.jl:333-467):
 // environments, the two-site effective Hamiltonian, a Lanczos solve of the
 // local ground state and the truncated SVD split, all on device tensors.
-// Real MPOs and MPS only: the U(1) lattice-gauge Hamiltonian is real
-// symmetric, so its ground state is real up to a global phase.
+// DMRG takes real MPOs and MPS only: the U(1) lattice-gauge Hamiltonian is
+// real symmetric, so its ground state is real up to a global phase.  The
+// real-time part (mps_lgt.jl:204-303, 546-586) -- overlaps, the zip-up MPO
+// application, direct sums, compression and the global Krylov step -- works
+// on complex MPS.
 //
 // Layouts follow the reference: MPS tensors psi[p] (Dl, Dr, d), MPO tensors
 // W[p] (Dl, Dr, k, b) with k the bra and b the ket physical index, all
@@ -239,12 +242,302 @@ double two_site_update(Ctx& c, std::vector<Tensor>& psi, const std::vector<Tenso
 
 double mpo_expect(Ctx& c, const std::vector<Tensor>& W, const std::vector<Tensor>& psi) {
   // mpo_expect (mps_lgt.jl:308-315): <psi|W|psi> by absorbing sites into the
-  // (bra, mpo, ket) sandwich from the left
+  // (bra, mpo, ket) sandwich from the left; a complex psi takes the complex
+  // form of W and the conjugate bra (the real part is returned)
   PEPS_CHECK(W.size() == psi.size() && !W.empty(), PS_ESHAPE, "mpo_expect: lengths");
-  Tensor E = ones111(c);
-  for (size_t p = 0; p < W.size(); ++p) E = left_env(c, E, W[p], psi[p]);
+  const bool cp = psi[0].cplx;
+  Tensor E = cp ? as_complex(c, ones111(c)) : ones111(c);
+  for (size_t p = 0; p < W.size(); ++p) {
+    PEPS_CHECK(psi[p].cplx == cp, PS_EARG, "mpo_expect: mixed real/complex MPS");
+    const Tensor Wp = cp ? as_complex(c, W[p]) : W[p];
+    Tensor t1 = contract(c, E, "xyz", psi[p], "zcq", "xycq");
+    Tensor t2 = contract(c, t1, "xycq", Wp, "ybkq", "xcbk");
+    E = contract(c, t2, "xcbk", psi[p], "xak", "abc", false, true);
+  }
   PEPS_CHECK(E.numel() == 1, PS_ESHAPE, "mpo_expect: open boundary bonds expected");
   return c.to_host(E)[0];
+}
+
+// ── real-time evolution (mps_lgt.jl:204-303, 546-586) ──────────────────────
+//
+// Inside the evolution the MPS tensors are held as (Dl, d, Dr), so that the
+// left matrix (Dl d) x Dr and the right matrix Dl x (d Dr) of every site are
+// free reshapes; the boundary-MPS compression sweep takes the same layout.
+// All tensors are complex.
+
+namespace {
+
+MPS to_ldr(Ctx& c, const std::vector<Tensor>& psi) {
+  MPS m(psi.size());
+  for (size_t p = 0; p < psi.size(); ++p) m[p] = permute(c, as_complex(c, psi[p]), {0, 2, 1});
+  return m;
+}
+
+std::vector<Tensor> from_ldr(Ctx& c, const MPS& m) {
+  std::vector<Tensor> psi(m.size());
+  for (size_t p = 0; p < m.size(); ++p) psi[p] = permute(c, m[p], {0, 2, 1});
+  return psi;
+}
+
+// With the context's finite-value checks on, name the first non-finite
+// intermediate of the evolution.
+void check_finite(Ctx& c, const Tensor& t, const char* what, size_t site) {
+  if (!c.debug_finite) return;
+  PEPS_CHECK(all_finite(c, t), PS_ESOLVER,
+             std::string("krylov_evolve: non-finite ") + what + " at site " + std::to_string(site + 1));
+}
+
+Tensor one11c(Ctx& c) {
+  const double one[2] = {1.0, 0.0};
+  return c.upload({1, 1}, true, one);
+}
+
+// mps_overlap (mps_lgt.jl:204-215): <phi|psi>, v(a, b) over (bra, ket) bonds.
+cplx overlap(Ctx& c, const MPS& phi, const MPS& psi) {
+  Tensor v = one11c(c);
+  for (size_t p = 0; p < psi.size(); ++p) {
+    Tensor t = contract(c, v, "ab", psi[p], "bqc", "aqc");
+    v = contract(c, phi[p], "aqd", t, "aqc", "dc", true, false);
+  }
+  std::vector<double> h = c.to_host(v);
+  return {h[0], h[1]};
+}
+
+double mps_norm(Ctx& c, const MPS& psi) { return std::sqrt(std::max(0.0, overlap(c, psi, psi).re)); }
+
+// psi with its first tensor multiplied by s (the other tensors shared).
+MPS scaled(Ctx& c, const MPS& psi, cplx s) {
+  MPS out = psi;
+  out[0] = copy_tensor(c, psi[0]);
+  scale(c, out[0], s);
+  return out;
+}
+
+// The truncated split M ~= U R of a (m x n) site matrix: U (m x k) with
+// orthonormal columns spanning the leading k left singular vectors, R = U^H M
+// (k x n), k = max(1, min(kmax, #{s > eps s_0})) (the keep rule of
+// mps_lgt.jl:250).  Blocks with min(m, n) >= kEvolveJwMinDim take the
+// Jordan-Wielandt form of M^H, smaller ones the one-workgroup Jacobi SVD.
+constexpr int64_t kEvolveJwMinDim = 32;
+
+void trunc_left(Ctx& c, const Tensor& M, int64_t kmax, double eps, Tensor* U, Tensor* R) {
+  const int64_t m = M.dims[0], n = M.dims[1];
+  if (std::min(m, n) >= kEvolveJwMinDim) {
+    Tensor P;
+    topk_right_jw(c, conj_transpose(c, M), kmax, eps, true, U, &P);  // U: leading left vectors of M, P = M^H U
+    *R = conj_transpose(c, P);
+    return;
+  }
+  SVD f = svd_thin(c, M);
+  int64_t cnt = 0;
+  for (double x : f.S) cnt += x > eps * f.S[0];
+  const int64_t k = std::max<int64_t>(1, std::min<int64_t>({kmax, cnt, int64_t(f.S.size())}));
+  *U = copy_tensor(c, f.U.slice_last(0, k));
+  *R = gather2d(c, f.Vt, seq_range(k), seq_range(n));
+  scale_rows(c, *R, std::vector<double>(f.S.begin(), f.S.begin() + k));
+}
+
+// mpo_apply_mps_zipup (mps_lgt.jl:239-257): one left-to-right sweep that
+// contracts L (dl, aH, apsi) with W[p] and psi[p] and splits the result at
+// once, so the new bond never exceeds Dmax.
+MPS zipup(Ctx& c, const std::vector<Tensor>& W, const MPS& psi, int64_t Dmax, double eps) {
+  const size_t n = psi.size();
+  MPS out(n);
+  Tensor L = as_complex(c, ones111(c));
+  for (size_t p = 0; p < n; ++p) {
+    Tensor t = contract(c, L, "xhs", psi[p], "sqb", "xhqb");
+    Tensor B = contract(c, t, "xhqb", W[p], "hgkq", "xkgb");  // (dl, k, bH, bpsi)
+    const int64_t dl = B.dims[0], d = B.dims[1], bH = B.dims[2], bp = B.dims[3];
+    check_finite(c, B, "zip-up site product", p);
+    if (p + 1 < n) {
+      Tensor U, R;
+      trunc_left(c, B.reshaped({dl * d, bH * bp}), Dmax, eps, &U, &R);
+      check_finite(c, U, "zip-up left factor", p);
+      check_finite(c, R, "zip-up carried factor", p);
+      const int64_t keep = U.dims[1];
+      out[p] = U.reshaped({dl, d, keep});
+      L = R.reshaped({keep, bH, bp});
+    } else {
+      PEPS_CHECK(bH * bp == 1, PS_ESHAPE, "zipup: open right boundary expected");
+      out[p] = B.reshaped({dl, d, 1});
+    }
+  }
+  return out;
+}
+
+// mps_axpby (mps_lgt.jl:259-281): the direct sum a A + b B, block-diagonal
+// in the bonds, with a and b on the first site.
+MPS axpby(Ctx& c, cplx a, const MPS& A, cplx b, const MPS& B) {
+  const size_t n = A.size();
+  MPS C(n);
+  for (size_t p = 0; p < n; ++p) {
+    const int64_t dla = A[p].dims[0], d = A[p].dims[1], dra = A[p].dims[2];
+    const int64_t dlb = B[p].dims[0], drb = B[p].dims[2];
+    const bool first = p == 0, last = p + 1 == n;
+    const int64_t Dl = first ? 1 : dla + dlb, Dr = last ? 1 : dra + drb;
+    Tensor T = c.zeros({Dl, d, Dr}, true);
+    Tensor Tm = T.reshaped({Dl, d * Dr});
+    Tensor Ap = first ? copy_tensor(c, A[p]) : A[p], Bp = first ? copy_tensor(c, B[p]) : B[p];
+    if (first) {
+      scale(c, Ap, a);
+      scale(c, Bp, b);
+    }
+    put_block(c, Tm, 0, 0, Ap.reshaped({dla, d * dra}));
+    put_block(c, Tm, first ? 0 : dla, last ? 0 : d * dra, Bp.reshaped({dlb, d * drb}));
+    check_finite(c, T, "direct sum", p);
+    C[p] = T;
+  }
+  return C;
+}
+
+// mps_compress! (mps_lgt.jl:283-303): the boundary-MPS sweep (orthonormal
+// left factors, then the truncated right-to-left SVD with the relative cut).
+void compress(Ctx& c, MPS& psi, int64_t Dmax, double eps) { compress_mps(c, psi, int(Dmax), eps, true); }
+
+// exp(-i dt T) e_1 for the real symmetric tridiagonal T = tridiag(beta,
+// alpha, beta): T = Q diag(w) Q^T, so the coefficients are
+// sum_l Q(j, l) e^{-i dt w_l} Q(0, l).
+std::vector<cplx> krylov_coeffs(const std::vector<double>& al, const std::vector<double>& be, double dt) {
+  const int k = int(al.size());
+  std::vector<double> T(size_t(k) * k, 0.0), w, Q;
+  for (int i = 0; i < k; ++i) T[size_t(i) * k + i] = al[i];
+  for (int i = 0; i + 1 < k; ++i) T[size_t(i) * k + i + 1] = T[size_t(i + 1) * k + i] = be[i];
+  small_sym_eig(k, T, &w, &Q);
+  std::vector<cplx> out(k, {0.0, 0.0});
+  for (int j = 0; j < k; ++j)
+    for (int l = 0; l < k; ++l) {
+      const double f = Q[size_t(j) * k + l] * Q[size_t(l)];
+      out[j].re += f * std::cos(dt * w[l]);
+      out[j].im -= f * std::sin(dt * w[l]);
+    }
+  return out;
+}
+
+// krylov_evolve_step (mps_lgt.jl:546-586): psi <- exp(-i dt H) psi on the
+// MPS manifold, Lanczos with full reorthogonalisation, renormalised.
+MPS krylov_step(Ctx& c, const std::vector<Tensor>& W, const MPS& psi, double dt, int m, int64_t Dmax, double eps,
+                int* kdim) {
+  const cplx one{1.0, 0.0};
+  std::vector<MPS> V;
+  V.push_back(scaled(c, psi, {1.0 / mps_norm(c, psi), 0.0}));
+  std::vector<double> al, be;
+  MPS w = zipup(c, W, V[0], Dmax, eps);
+  al.push_back(overlap(c, V[0], w).re);
+  w = axpby(c, one, w, {-al[0], 0.0}, V[0]);
+  compress(c, w, Dmax, eps);
+  for (int j = 2; j <= m; ++j) {
+    const double b = mps_norm(c, w);
+    if (b < 1e-12) break;
+    be.push_back(b);
+    V.push_back(scaled(c, w, {1.0 / b, 0.0}));
+    const MPS& vj = V.back();
+    w = zipup(c, W, vj, Dmax, eps);
+    const double aj = overlap(c, vj, w).re;
+    al.push_back(aj);
+    w = axpby(c, one, w, {-aj, 0.0}, vj);
+    w = axpby(c, one, w, {-b, 0.0}, V[V.size() - 2]);
+    compress(c, w, Dmax, eps);
+    for (const MPS& vk : V) {  // full reorthogonalisation
+      const cplx ck = overlap(c, vk, w);
+      w = axpby(c, one, w, {-ck.re, -ck.im}, vk);
+    }
+    compress(c, w, Dmax, eps);
+  }
+  const std::vector<cplx> co = krylov_coeffs(al, be, dt);
+  const int k = int(al.size());
+  MPS out = scaled(c, V[0], co[0]);
+  for (int j = 1; j < k; ++j) {
+    out = axpby(c, one, out, co[j], V[j]);
+    compress(c, out, Dmax, eps);
+  }
+  out = scaled(c, out, {1.0 / mps_norm(c, out), 0.0});
+  c.count("mps_evolve[krylov dimension]", double(k), 0);
+  if (kdim) *kdim = k;
+  return out;
+}
+
+void check_chain(const std::vector<Tensor>& W, const std::vector<Tensor>& psi, const char* who) {
+  const size_t n = W.size();
+  PEPS_CHECK(n >= 2 && psi.size() == n, PS_EARG, std::string(who) + ": at least two sites");
+  for (size_t p = 0; p < n; ++p) {
+    PEPS_CHECK(W[p].rank() == 4 && psi[p].rank() == 3 && !W[p].cplx, PS_ESHAPE,
+               std::string(who) + ": real W (Dl, Dr, d, d) and psi (Dl, Dr, d) expected");
+    PEPS_CHECK(W[p].dims[2] == W[p].dims[3] && W[p].dims[2] == psi[p].dims[2], PS_ESHAPE,
+               std::string(who) + ": physical dims");
+    if (p + 1 < n)
+      PEPS_CHECK(W[p].dims[1] == W[p + 1].dims[0] && psi[p].dims[1] == psi[p + 1].dims[0], PS_ESHAPE,
+                 std::string(who) + ": bond dims");
+  }
+  PEPS_CHECK(W[0].dims[0] == 1 && W[n - 1].dims[1] == 1 && psi[0].dims[0] == 1 && psi[n - 1].dims[1] == 1,
+             PS_ESHAPE, std::string(who) + ": open boundaries expected");
+}
+
+}  // namespace
+
+std::vector<Tensor> krylov_evolve(Ctx& c, const std::vector<Tensor>& W, const std::vector<Tensor>& psi0, double dt,
+                                  int nsteps, int m, int Dmax, double eps) {
+  TierGuard fp64(c, TIER_FP64);
+  check_chain(W, psi0, "krylov_evolve");
+  PEPS_CHECK(nsteps >= 0 && m >= 1 && Dmax >= 1 && eps >= 0.0 && std::isfinite(dt), PS_EARG,
+             "krylov_evolve: nsteps >= 0, m >= 1, Dmax >= 1, eps >= 0, finite dt");
+  std::vector<Tensor> Wc(W.size());
+  for (size_t p = 0; p < W.size(); ++p) Wc[p] = as_complex(c, W[p]);
+  MPS psi = to_ldr(c, psi0);
+  const double nrm = mps_norm(c, psi);
+  PEPS_CHECK(nrm > 0.0 && std::isfinite(nrm), PS_ESOLVER, "krylov_evolve: zero or non-finite state");
+  for (int s = 0; s < nsteps; ++s) {
+    int k = 0;
+    psi = krylov_step(c, Wc, psi, dt, m, Dmax, eps, &k);
+    c.count("mps_evolve[steps]", 1.0, 0);
+    if (c.verbose) {
+      int64_t bmax = 1;
+      for (const Tensor& t : psi) bmax = std::max(bmax, t.dims[2]);
+      c.note("krylov step " + std::to_string(s + 1) + ": Krylov dimension " + std::to_string(k) + ", max bond " +
+             std::to_string(bmax));
+    }
+  }
+  return from_ldr(c, psi);
+}
+
+std::vector<double> mps_local_expect(Ctx& c, const std::vector<Tensor>& psi0, const std::vector<int>& sites,
+                                     const std::vector<Tensor>& ops) {
+  // mps_local_expect (mps_lgt.jl:317-330) for a list of (site, operator)
+  // pairs: the left and right overlap environments are built once and each
+  // <psi|O_p|psi> / <psi|psi> closes at its site
+  const size_t n = psi0.size();
+  PEPS_CHECK(n >= 1 && sites.size() == ops.size(), PS_EARG, "mps_local_expect: lengths");
+  for (size_t p = 0; p < n; ++p) {
+    PEPS_CHECK(psi0[p].rank() == 3, PS_ESHAPE, "mps_local_expect: psi (Dl, Dr, d) expected");
+    if (p + 1 < n) PEPS_CHECK(psi0[p].dims[1] == psi0[p + 1].dims[0], PS_ESHAPE, "mps_local_expect: bond dims");
+  }
+  PEPS_CHECK(psi0[0].dims[0] == 1 && psi0[n - 1].dims[1] == 1, PS_ESHAPE, "mps_local_expect: open boundaries");
+  MPS psi = to_ldr(c, psi0);
+  std::vector<Tensor> EL(n + 1), ER(n + 1);
+  EL[0] = one11c(c);
+  for (size_t p = 0; p < n; ++p) {
+    Tensor t = contract(c, EL[p], "ab", psi[p], "bqc", "aqc");
+    EL[p + 1] = contract(c, psi[p], "aqd", t, "aqc", "dc", true, false);
+  }
+  ER[n] = one11c(c);
+  for (size_t p = n; p-- > 0;) {
+    Tensor t = contract(c, psi[p], "bqc", ER[p + 1], "dc", "bqd");
+    ER[p] = contract(c, psi[p], "aqd", t, "bqd", "ab", true, false);
+  }
+  const double nrm = c.to_host(EL[n])[0];
+  PEPS_CHECK(nrm > 0.0 && std::isfinite(nrm), PS_ESOLVER, "mps_local_expect: zero or non-finite state");
+  std::vector<double> out(sites.size());
+  for (size_t i = 0; i < sites.size(); ++i) {
+    const int p = sites[i];
+    PEPS_CHECK(p >= 0 && size_t(p) < n, PS_EARG, "mps_local_expect: site out of range");
+    const int64_t d = psi[p].dims[1];
+    PEPS_CHECK(ops[i].rank() == 2 && ops[i].dims[0] == d && ops[i].dims[1] == d && !ops[i].cplx, PS_ESHAPE,
+               "mps_local_expect: real (d, d) operator expected");
+    Tensor Op = contract(c, psi[p], "bqc", as_complex(c, ops[i]), "kq", "bkc");  // (O psi)(b, k, c)
+    Tensor t = contract(c, EL[p], "ab", Op, "bkc", "akc");
+    Tensor u = contract(c, t, "akc", ER[p + 1], "dc", "akd");
+    out[i] = dot(c, u, psi[p], true).re / nrm;
+  }
+  return out;
 }
 
 DmrgResult dmrg_ground_state(Ctx& c, const std::vector<Tensor>& W, std::vector<Tensor> psi, int D, int nsweeps,

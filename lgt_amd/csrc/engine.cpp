@@ -205,13 +205,34 @@ static void debug_finite(Ctx& c, const Tensor& t, const char* what, size_t site)
 // Smallest block dimension at which the sweep's SVD takes the Jordan-Wielandt
 // form (below it the one-workgroup Jacobi kernel is the faster one).
 constexpr int64_t kJwSweepMinDim = 128;
+// complex blocks: the one-workgroup complex Jacobi SVD is slower still
+constexpr int64_t kJwSweepMinDimComplex = 32;
 
-// Truncated right factor of a real M (m x n) for the right-to-left SVD
-// sweep: k = max(1, min(kmax, #{s_j > thr})) with thr = cut s_0 (relative)
-// or cut, Vk (n x k) an orthonormal basis of the leading k right singular
-// vectors and P = M Vk (= U_k diag(S_k), which the sweep passes to the
-// left).  The singular triplets come from the symmetric eigenproblem of the
-// Jordan-Wielandt matrix [[0, C], [C^T, 0]] of M's square core C (C = R of
+// The real matrix [[X, -Y], [Y, X]] (2m x 2n) of a complex C = X + i Y.
+static Tensor real_embedding(Ctx& c, const Tensor& C) {
+  const int64_t m = C.dims[0], n = C.dims[1];
+  Tensor R = C;  // the interleaved storage as a real (2, m, n) tensor
+  R.cplx = false;
+  R.off = C.off * 2;
+  R.dims = {2, m, n};
+  const Tensor RI = permute(c, R, {1, 2, 0});  // (m, n, 2): X, then Y
+  const Tensor X = RI.slice_last(0, 1).reshaped({m, n}), Y = RI.slice_last(1, 2).reshaped({m, n});
+  Tensor Yn = copy_tensor(c, Y);
+  scale(c, Yn, -1.0);
+  Tensor E = c.alloc({2 * m, 2 * n}, false);
+  put_block(c, E, 0, 0, X);
+  put_block(c, E, m, 0, Y);
+  put_block(c, E, 0, n, Yn);
+  put_block(c, E, m, n, X);
+  return E;
+}
+
+// Truncated right factor of M (m x n, real or complex) for the right-to-left
+// SVD sweep: k = max(1, min(kmax, #{s_j > thr})) with thr = cut s_0
+// (relative) or cut, Vk (n x k) an orthonormal basis of the leading k right
+// singular vectors and P = M Vk (= U_k diag(S_k), which the sweep passes to
+// the left).  The singular triplets come from the Hermitian eigenproblem of
+// the Jordan-Wielandt matrix [[0, C], [C^H, 0]] of M's square core C (C = R of
 // M = Q R when m > n, else M): backward stable, singular values to about
 // (m + n) eps s_0 -- the accuracy class of LAPACK's gesdd -- at the cost of
 // one eigendecomposition of order rows(C) + n instead of a one-workgroup
@@ -220,22 +241,100 @@ constexpr int64_t kJwSweepMinDim = 128;
 void topk_right_jw(Ctx& c, const Tensor& M, int64_t kmax, double cut, bool relative, Tensor* Vk, Tensor* P) {
   TierGuard fp64(c, TIER_FP64);
   const int64_t m = M.dims[0], n = M.dims[1];
-  const Tensor C = m > n ? qr_r_only(c, M) : M;
-  const int64_t mc = C.dims[0], N = mc + n, nsv = std::min(mc, n);
-  std::vector<int64_t> rows_u = seq(mc), rows_v(n);
-  for (int64_t i = 0; i < n; ++i) rows_v[i] = mc + i;
-  Tensor J = c.zeros({N, N}, false);
-  scatter_cols(c, J, C, rows_u, seq(n), rows_v);                        // J(0:mc, mc:N) = C
-  scatter_cols(c, J, permute(c, C, {1, 0}), rows_v, seq(mc), rows_u);   // J(mc:N, 0:mc) = C^T
-  EIG e = eigh(c, J);  // ascending: s_j = w[N - 1 - j]
-  const double s0 = std::max(e.w[N - 1], 0.0);
-  const double thr = relative ? cut * s0 : cut;
-  int64_t cnt = 0;
-  for (int64_t j = 0; j < nsv; ++j) cnt += e.w[N - 1 - j] > thr;
-  const int64_t k = std::max<int64_t>(1, std::min<int64_t>(kmax, cnt));
-  std::vector<int64_t> top(k);
-  for (int64_t j = 0; j < k; ++j) top[j] = N - 1 - j;
-  *Vk = qr_thin(c, gather2d(c, e.V, rows_v, top)).Q;
+  // the square core: R of M = Q R when tall; for wide complex M, R^H of
+  // M^H = Q R (M = R^H Q^H, so the right vectors of M are Q times those of
+  // R^H and the eigenproblem has order 2m instead of m + n)
+  Tensor Qw;
+  Tensor C0;
+  const bool wide = M.cplx && m < n;
+  if (m > n) {
+    C0 = qr_r_only(c, M);
+  } else if (wide) {
+    QR f = qr_thin(c, conj_transpose(c, M));
+    Qw = f.Q;
+    C0 = conj_transpose(c, f.R);
+  } else {
+    C0 = M;
+  }
+  const int64_t n0 = C0.dims[1];
+  // rows and columns of the core that are exactly zero (Gauss-law blocks
+  // have many) hold no singular weight: the eigenproblem is set up on the
+  // compacted core and the right vectors padded with zeros
+  std::vector<int64_t> nzr, nzc;
+  nonzero_lines(c, C0, &nzr, &nzc);
+  if (nzr.empty()) {  // M = 0: one zero triplet
+    *Vk = eye(c, n, 1, M.cplx);
+    *P = c.zeros({m, 1}, M.cplx);
+    return;
+  }
+  const bool compact = int64_t(nzr.size()) < C0.dims[0] || int64_t(nzc.size()) < n0;
+  const Tensor C = compact ? gather2d(c, C0, nzr, nzc) : C0;
+  const int64_t mc = C.dims[0], nc = C.dims[1], N = mc + nc, nsv = std::min(mc, nc);
+  Tensor Vs;  // nc x k, spanning the leading k right singular vectors of C
+  int64_t k = 1;
+  if (M.cplx) {
+    // complex: the real Jordan-Wielandt problem of the real embedding
+    // Cr = [[X, -Y], [Y, X]] of C = X + i Y.  Each singular value of C is a
+    // pair of Cr, with right vectors (p; q) and (-q; p) for v = p + i q, so
+    // the leading 2k real right vectors E give Z = E_p + i E_q (nc x 2k)
+    // with (1/2) Z Z^H the projector onto the leading k complex ones; its
+    // range is Z Z^H Omega for a fixed random Omega (nc x k).  (rocSOLVER's
+    // complex Householder tridiagonalisation has returned non-finite
+    // vectors for these matrices, whose reduction meets exactly zero
+    // columns.)
+    const Tensor Cr = real_embedding(c, C);
+    const int64_t Nr = 2 * N;
+    std::vector<int64_t> ru = seq(2 * mc), rv(2 * nc), rp(nc), rq(nc);
+    for (int64_t i = 0; i < 2 * nc; ++i) rv[i] = 2 * mc + i;
+    for (int64_t i = 0; i < nc; ++i) {
+      rp[i] = 2 * mc + i;
+      rq[i] = 2 * mc + nc + i;
+    }
+    Tensor J = c.zeros({Nr, Nr}, false);
+    scatter_cols(c, J, Cr, ru, seq(2 * nc), rv);
+    scatter_cols(c, J, permute(c, Cr, {1, 0}), rv, seq(2 * mc), ru);
+    EIG e = eigh(c, J);
+    debug_finite(c, e.V, "Jordan-Wielandt eigenvectors", 0);
+    const double s0 = std::max(e.w[Nr - 1], 0.0);
+    const double thr = relative ? cut * s0 : cut;
+    int64_t cnt = 0;
+    for (int64_t j = 0; j < 2 * nsv; ++j) cnt += e.w[Nr - 1 - j] > thr;
+    k = std::max<int64_t>(1, std::min<int64_t>(kmax, (cnt + 1) / 2));
+    std::vector<int64_t> top(2 * k);
+    for (int64_t j = 0; j < 2 * k; ++j) top[j] = Nr - 1 - j;
+    Tensor Z = as_complex(c, gather2d(c, e.V, rp, top));
+    Tensor Zq = as_complex(c, gather2d(c, e.V, rq, top));
+    scale(c, Zq, cplx{0.0, 1.0});
+    axpy_inplace(c, Z, Zq, 1.0);
+    Tensor Om = c.alloc({nc, k}, true);
+    fill_random(c, Om, 0x6a77ull + uint64_t(nc) * 2654435761ull + uint64_t(k));
+    Vs = matmul(c, Z, matmul(c, Z, Om, true, false, true, false));
+  } else {
+    std::vector<int64_t> rows_u = seq(mc), rows_v(nc);
+    for (int64_t i = 0; i < nc; ++i) rows_v[i] = mc + i;
+    const Tensor Ch = conj_transpose(c, C);
+    Tensor J = c.zeros({N, N}, M.cplx);
+    scatter_cols(c, J, C, rows_u, seq(nc), rows_v);   // J(0:mc, mc:N) = C
+    scatter_cols(c, J, Ch, rows_v, seq(mc), rows_u);  // J(mc:N, 0:mc) = C^H
+    debug_finite(c, J, "Jordan-Wielandt matrix", 0);
+    EIG e = eigh(c, J);  // ascending: s_j = w[N - 1 - j]
+    debug_finite(c, e.V, "Jordan-Wielandt eigenvectors", 0);
+    const double s0 = std::max(e.w[N - 1], 0.0);
+    const double thr = relative ? cut * s0 : cut;
+    int64_t cnt = 0;
+    for (int64_t j = 0; j < nsv; ++j) cnt += e.w[N - 1 - j] > thr;
+    k = std::max<int64_t>(1, std::min<int64_t>(kmax, cnt));
+    std::vector<int64_t> top(k);
+    for (int64_t j = 0; j < k; ++j) top[j] = N - 1 - j;
+    Vs = gather2d(c, e.V, rows_v, top);
+  }
+  if (compact) {
+    Tensor Vf = c.zeros({n0, k}, M.cplx);
+    scatter_cols(c, Vf, Vs, nzc, seq(k), seq(k));
+    Vs = Vf;
+  }
+  if (wide) Vs = matmul(c, Qw, Vs);
+  *Vk = qr_thin(c, Vs).Q;
   *P = matmul(c, M, *Vk);
   c.count("bmps[svd: jordan-wielandt]", 1.0, double(N));
 }
@@ -257,12 +356,12 @@ void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff, bool relative) {
   }
   for (size_t i = n - 1; i >= 1; --i) {
     int64_t Dl = mps[i].dims[0], dp = mps[i].dims[1], Dr = mps[i].dims[2];
-    if (!mps[i].cplx && std::min(Dl, dp * Dr) >= kJwSweepMinDim) {
-      // large real bonds: the Jordan-Wielandt form of the same truncation
+    if (std::min(Dl, dp * Dr) >= (mps[i].cplx ? kJwSweepMinDimComplex : kJwSweepMinDim)) {
+      // large bonds: the Jordan-Wielandt form of the same truncation
       Tensor Vk, P;
       topk_right_jw(c, mps[i].reshaped({Dl, dp * Dr}), chi, cutoff, relative, &Vk, &P);
       const int64_t k = Vk.dims[1];
-      mps[i] = copy_tensor(c, permute(c, Vk, {1, 0})).reshaped({k, dp, Dr});
+      mps[i] = conj_transpose(c, Vk).reshaped({k, dp, Dr});
       int64_t Dl2 = mps[i - 1].dims[0], dp2 = mps[i - 1].dims[1], Dr2 = mps[i - 1].dims[2];
       mps[i - 1] = matmul(c, mps[i - 1].reshaped({Dl2 * dp2, Dr2}), P).reshaped({Dl2, dp2, k});
       debug_finite(c, mps[i - 1], "M*P product", i - 1);

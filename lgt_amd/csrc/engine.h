@@ -86,14 +86,15 @@ void compress_mps(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT, bool rela
 // Same truncated MPS from left Gram matrices and top-chi eigenvectors.
 void compress_mps_gram(Ctx& c, MPS& mps, int chi, double cutoff = CMPS_CUT);
 
-// Truncated right factor of a real M (m x n): Vk (n x k) orthonormal, the
-// leading k right singular vectors with k = max(1, min(kmax, #{s > thr})),
-// thr = cut s_0 (relative) or cut, and P = M Vk, from the Jordan-Wielandt
-// eigenproblem of M's square core (engine.cpp).
+// Truncated right factor of M (m x n, real or complex): Vk (n x k)
+// orthonormal, the leading k right singular vectors with k = max(1, min(kmax,
+// #{s > thr})), thr = cut s_0 (relative) or cut, and P = M Vk, from the
+// Jordan-Wielandt eigenproblem [[0, C], [C^H, 0]] of M's square core C
+// (engine.cpp).
 void topk_right_jw(Ctx& c, const Tensor& M, int64_t kmax, double cut, bool relative, Tensor* Vk, Tensor* P);
 
 // ── MPS / DMRG (dmrg.cpp; mps_lgt.jl) ──────────────────────────────────────
-// MPS tensors (Dl, Dr, d), MPO tensors (Dl, Dr, k, b); real only.
+// MPS tensors (Dl, Dr, d), MPO tensors (Dl, Dr, k, b); DMRG real only.
 struct DmrgResult {
   double E = 0.0;
   int sweeps = 0;
@@ -102,8 +103,16 @@ struct DmrgResult {
 // dmrg_ground_state (mps_lgt.jl:418-467) with two_site_update! (:384-416)
 DmrgResult dmrg_ground_state(Ctx& c, const std::vector<Tensor>& W, std::vector<Tensor> psi, int D, int nsweeps,
                              double tol, int maxiter, int krylovdim, double etol);
-// mpo_expect (mps_lgt.jl:308-315): <psi|W|psi>
+// mpo_expect (mps_lgt.jl:308-315): <psi|W|psi> (real part; psi real or complex)
 double mpo_expect(Ctx& c, const std::vector<Tensor>& W, const std::vector<Tensor>& psi);
+// krylov_evolve_step (mps_lgt.jl:546-586) nsteps times with the zip-up MPO
+// application (:239-257), direct sums (:259-281) and compression (:283-303):
+// psi <- exp(-i dt W) psi, renormalised; returns a complex MPS.
+std::vector<Tensor> krylov_evolve(Ctx& c, const std::vector<Tensor>& W, const std::vector<Tensor>& psi0, double dt,
+                                  int nsteps, int m, int Dmax, double eps);
+// mps_local_expect (mps_lgt.jl:317-330) for each (site, real operator) pair.
+std::vector<double> mps_local_expect(Ctx& c, const std::vector<Tensor>& psi, const std::vector<int>& sites,
+                                     const std::vector<Tensor>& ops);
 // absorb_row_from_top/bottom + compress_mps!, literal or fused Gram form.
 // key_row > 0 names the row just absorbed: the Gram eigensolves of this
 // boundary warm-start from the previous computation at the same position.

@@ -1184,6 +1184,43 @@ void axpy_inplace(Ctx& c, Tensor& Y, const Tensor& X, double a) {
   PEPS_HIP(hipGetLastError());
 }
 
+__global__ void scale_cplx_kernel(double* x, int64_t n, double re, double im) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double a = x[2 * i], b = x[2 * i + 1];
+    x[2 * i] = a * re - b * im;
+    x[2 * i + 1] = a * im + b * re;
+  }
+}
+void scale(Ctx& c, Tensor& T, cplx s) {
+  if (!T.cplx) {
+    PEPS_CHECK(s.im == 0.0, PS_EARG, "scale: complex factor on a real tensor");
+    scale(c, T, s.re);
+    return;
+  }
+  if (!T.numel()) return;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(T.numel(), 256), 16384));
+  hipLaunchKernelGGL(scale_cplx_kernel, dim3(g), dim3(256), 0, c.stream, T.data(), T.numel(), s.re, s.im);
+  PEPS_HIP(hipGetLastError());
+}
+
+__global__ void real_to_cplx_kernel(double* out, const double* in, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    out[2 * i] = in[i];
+    out[2 * i + 1] = 0.0;
+  }
+}
+Tensor as_complex(Ctx& c, const Tensor& T) {
+  if (T.cplx) return T;
+  Tensor out = c.alloc(T.dims, true);
+  if (!T.numel()) return out;
+  unsigned g = unsigned(std::min<int64_t>(cdiv(T.numel(), 256), 16384));
+  hipLaunchKernelGGL(real_to_cplx_kernel, dim3(g), dim3(256), 0, c.stream, out.data(), T.data(), T.numel());
+  PEPS_HIP(hipGetLastError());
+  return out;
+}
+
 __global__ void conj_kernel(double* x, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)

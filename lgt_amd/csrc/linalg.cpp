@@ -224,7 +224,7 @@ Tensor qr_r_only(Ctx& c, const Tensor& A) {
   return R;
 }
 
-static Tensor conj_transpose(Ctx& c, const Tensor& A) {
+Tensor conj_transpose(Ctx& c, const Tensor& A) {
   Tensor T = permute(c, A, {1, 0});
   if (T.data() == A.data()) T = copy_tensor(c, T);  // vector-shaped: permute returned a view
   conj_inplace(c, T);

@@ -266,6 +266,10 @@ void scale_legs(Ctx& c, Tensor& T, const std::vector<double>* w /*[4]: l,r,u,d*/
 void scale(Ctx& c, Tensor& T, double s);
 void axpy_inplace(Ctx& c, Tensor& Y, const Tensor& X, double a);  // Y += a X
 void conj_inplace(Ctx& c, Tensor& T);
+// T *= s for a complex factor (a real tensor accepts only s.im == 0).
+void scale(Ctx& c, Tensor& T, cplx s);
+// A complex copy of a real tensor (imaginary parts zero); complex T itself.
+Tensor as_complex(Ctx& c, const Tensor& T);
 // Zero the entries of a (p,l,r,u,d) tensor that violate the Gauss law.
 void gauss_mask(Ctx& c, Tensor& T, const std::vector<int64_t>& nf, const std::vector<int64_t>& lR,
                 const std::vector<int64_t>& lU, const std::vector<int64_t>& fl,
@@ -355,6 +359,8 @@ struct QR {
   Tensor Q, R;  // thin: Q (m x k), R (k x n), k = min(m, n)
 };
 QR qr_thin(Ctx& c, const Tensor& A);
+// A^H as a new tensor (never a view of A).
+Tensor conj_transpose(Ctx& c, const Tensor& A);
 Tensor qr_r_only(Ctx& c, const Tensor& A);
 struct SVD {
   Tensor U, Vt;            // U (m x k), Vt (k x n)

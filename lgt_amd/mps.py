@@ -222,6 +222,30 @@ def staggered_mps(nx, ny, gd):
     return product_mps(cfg, node_dims(nx, ny, gd))
 
 
+def string_breaking_mps(nx, ny, gd, iy_str=None):
+    """string_breaking_mps (mps_lgt.jl:624-637): the vacuum with n_f(2, iy_str)
+    = 1 and E_R(1, iy_str) = 1 (the sector of string_charges)."""
+    iy_str = ny // 2 if iy_str is None else iy_str
+    chain, _ = snake_nodes(nx, ny)
+    cfg = []
+    for (ix, iy) in chain:
+        _, dR, dU = lattice.site_dims(ix, iy, nx, ny, gd)
+        nf = 1 if (ix == 2 and iy == iy_str) else 0
+        eR = 1 if (ix == 1 and iy == iy_str) else 0
+        iR = 1 if dR == 1 else eR + (gd - 1) // 2 + 1
+        iU = 1 if dU == 1 else (gd - 1) // 2 + 1
+        cfg.append(nf * dR * dU + (iR - 1) * dU + iU)
+    return product_mps(cfg, node_dims(nx, ny, gd))
+
+
+def string_charges(nx, ny, iy_str=None):
+    """string_charges (mps_lgt.jl:640-643): g(1, iy_str) = 1, g(2, iy_str) = -2."""
+    iy_str = ny // 2 if iy_str is None else iy_str
+    g = np.zeros((nx, ny), dtype=np.int64)
+    g[0, iy_str - 1], g[1, iy_str - 1] = 1, -2
+    return g
+
+
 def random_mps(dims, D, seed):
     """A seeded random MPS with bonds min(D, growth) (the reference's default
     start, real entries)."""
@@ -259,20 +283,27 @@ def mps_local_expect(psi, p, O, ctx=None):
 
 
 def measure_mps(psi, nx, ny, gd, t_now=0.0, ctx=None):
-    """measure_mps (mps_lgt.jl:600-612): site-resolved <n_f>, <E^2>."""
+    """measure_mps (mps_lgt.jl:600-612): site-resolved <n_f>, <E^2> (node_nf_op,
+    node_E2_op, :589-598), all in one pass of overlap environments; psi real
+    or complex."""
     _, pos = snake_nodes(nx, ny)
     E, _, _, nf = _ops(gd)
-    nfg, e2g = np.zeros((nx, ny)), np.zeros((nx, ny))
+    sites, ops, where = [], [], []
     for iy in range(1, ny + 1):
         for ix in range(1, nx + 1):
             p = pos[(ix, iy)]
-            nfg[ix - 1, iy - 1] = mps_local_expect(psi, p, _embed(ix, iy, nx, ny, gd, f=nf), ctx=ctx)
             O = np.zeros((psi[p].shape[2],) * 2)
             if ix < nx:
                 O += _embed(ix, iy, nx, ny, gd, R=E @ E)
             if iy < ny:
                 O += _embed(ix, iy, nx, ny, gd, Up=E @ E)
-            e2g[ix - 1, iy - 1] = mps_local_expect(psi, p, O, ctx=ctx)
+            sites += [p, p]
+            ops += [_embed(ix, iy, nx, ny, gd, f=nf), O]
+            where.append((ix - 1, iy - 1))
+    vals = peps.mps_local_expect_many(psi, sites, ops, ctx=ctx)
+    nfg, e2g = np.zeros((nx, ny)), np.zeros((nx, ny))
+    for i, (a, b) in enumerate(where):
+        nfg[a, b], e2g[a, b] = vals[2 * i], vals[2 * i + 1]
     par = (np.add.outer(np.arange(1, nx + 1), np.arange(1, ny + 1)) % 2 == 0)
     return dict(t=t_now, nf_mean=float(nfg.mean()), nf_even=float(nfg[par].mean()), nf_odd=float(nfg[~par].mean()),
                 E2_mean=float(e2g.mean()), nf_grid=nfg, E2_grid=e2g)

@@ -82,6 +82,12 @@ _SIGS = {
                               C.POINTER(C.c_int32), C.POINTER(_vp)]),
     "peps_mpo_expect": (C.c_int32, [_vp, C.c_int32, _i64p, C.POINTER(_dp), _i64p, C.POINTER(_dp),
                                     C.POINTER(C.c_double)]),
+    "peps_mpo_expect_c": (C.c_int32, [_vp, C.c_int32, _i64p, C.POINTER(_dp), C.c_int32, _i64p, C.POINTER(_dp),
+                                      C.POINTER(C.c_double)]),
+    "peps_krylov_evolve": (C.c_int32, [_vp, C.c_int32, _i64p, C.POINTER(_dp), C.c_int32, _i64p, C.POINTER(_dp),
+                                       C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.POINTER(_vp)]),
+    "peps_mps_local_expect": (C.c_int32, [_vp, C.c_int32, C.c_int32, _i64p, C.POINTER(_dp), C.c_int32, _i32p,
+                                          _i64p, C.POINTER(_dp), _dp]),
     "peps_boundary": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
     "peps_norm2": (C.c_int32, [_vp, C.c_int32, _dp]),
     "peps_bond_env": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
@@ -687,14 +693,54 @@ def dmrg(mpo, psi0, D, nsweeps=8, tol=1e-7, maxiter=60, krylovdim=12, etol=1e-7,
     return E.value, _take_list(ctx, h), sw.value
 
 
+def _is_complex(arrs):
+    return any(np.iscomplexobj(a) for a in arrs)
+
+
 def mpo_expect(mpo, psi, ctx=None):
-    """mpo_expect (mps_lgt.jl:308-315): <psi|W|psi> on the GPU (real)."""
+    """mpo_expect (mps_lgt.jl:308-315): <psi|W|psi> on the GPU for a real
+    MPO; the real part for a complex MPS."""
     ctx = ctx or default_context()
     wa, wd, wp = _pack(mpo, False)
-    pa, pd, pp = _pack(psi, False)
+    cp = _is_complex(psi)
+    pa, pd, pp = _pack(psi, cp)
     out = C.c_double()
-    ctx.check(lib().peps_mpo_expect(ctx.h, len(wa), _iptr(wd), wp, _iptr(pd), pp, C.byref(out)))
+    if cp:
+        ctx.check(lib().peps_mpo_expect_c(ctx.h, len(wa), _iptr(wd), wp, 1, _iptr(pd), pp, C.byref(out)))
+    else:
+        ctx.check(lib().peps_mpo_expect(ctx.h, len(wa), _iptr(wd), wp, _iptr(pd), pp, C.byref(out)))
     return out.value
+
+
+def krylov_evolve(mpo, psi0, dt, nsteps=1, m=8, Dmax=100, eps=1e-10, ctx=None):
+    """krylov_evolve_step (mps_lgt.jl:546-586) nsteps times on the GPU:
+    psi <- exp(-i dt H) psi with m Krylov vectors, bonds <= Dmax, relative
+    cut eps, renormalised each step.  Returns the complex MPS."""
+    ctx = ctx or default_context()
+    wa, wd, wp = _pack(mpo, False)
+    cp = _is_complex(psi0)
+    pa, pd, pp = _pack(psi0, cp)
+    h = _vp()
+    ctx.check(lib().peps_krylov_evolve(ctx.h, len(wa), _iptr(wd), wp, int(cp), _iptr(pd), pp, float(dt), int(nsteps),
+                                       int(m), int(Dmax), float(eps), C.byref(h)))
+    return _take_list(ctx, h)
+
+
+def mps_local_expect_many(psi, sites, ops, ctx=None):
+    """mps_local_expect (mps_lgt.jl:317-330) for every (site, operator) pair
+    (sites 0-based chain positions, real (d, d) operators) in one pass of
+    overlap environments: Re <psi|O|psi> / <psi|psi>."""
+    ctx = ctx or default_context()
+    cp = _is_complex(psi)
+    pa, pd, pp = _pack(psi, cp)
+    oa, od, op = _pack(ops, False)
+    st = np.ascontiguousarray(sites, dtype=np.int32)
+    out = np.zeros(len(oa))
+    if len(oa) != len(st):
+        raise ValueError("mps_local_expect_many: one site per operator")
+    ctx.check(lib().peps_mps_local_expect(ctx.h, len(pa), int(cp), _iptr(pd), pp, len(oa),
+                                          st.ctypes.data_as(_i32p), _iptr(od), op, _dptr(out)))
+    return out
 
 
 def boundary_top(gp: GaugePEPS, stop_row, chi):

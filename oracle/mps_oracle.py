@@ -16,11 +16,22 @@ MPO to the same space.  Small lattices only (2x2 at gd = 3 is 1296 states).
   * ``gauss_values``                G_i = E_R(i) - E_R(i-x) + E_U(i) - E_U(i-y)
                                     - n_f(i) per product state (mps_lgt.jl:466-514)
   * ``mpo_to_dense``                mps_lgt.jl:649-661
+  * ``krylov_evolve_dense``         krylov_evolve_step (mps_lgt.jl:546-586)
+                                    on full-space vectors: the same Lanczos
+                                    recurrence, full reorthogonalisation and
+                                    exp(-i dt T) e_1, with exact sums in place
+                                    of MPS direct sums and compressions
+  * ``product_vector``              product_mps (mps_lgt.jl:200-202) as a
+                                    full-space vector
 
 Pins (tests/test_oracle_mps.py): the full-space Hamiltonian restricted to a
 Gauss-law sector has exactly the spectrum of the ED oracle's Hamiltonian of
 that sector (finite_ed.jl, itself pinned against the reference's fixtures),
 and the Gauss values reproduce the ED basis size of every sector tested.
+The Krylov step is pinned against exp(-i dt H) psi from the eigenvectors of
+H (agreement to the Lanczos truncation error, which shrinks as m grows and
+vanishes once m reaches the dimension of the Krylov space), and the product
+vector against the Gauss sector of its charges.
 """
 from __future__ import annotations
 
@@ -129,3 +140,42 @@ def mpo_to_dense(W):
                 new[:, :, r] += np.kron(op[:, :, l], Wp[l, r])
         op = new
     return op[:, :, 0]
+
+
+def product_vector(configs, dims):
+    """product_mps (mps_lgt.jl:200-202) as a full-space vector: configs[p]
+    in 1..dims[p], first chain site slowest."""
+    idx = np.ravel_multi_index(tuple(int(c) - 1 for c in configs), dims)
+    v = np.zeros(int(np.prod(dims)), dtype=complex)
+    v[idx] = 1.0
+    return v
+
+
+def krylov_evolve_dense(H, psi, dt, m=8):
+    """krylov_evolve_step (mps_lgt.jl:546-586) on a full-space vector: Lanczos
+    with the three-term recurrence, full reorthogonalisation against every
+    basis vector, stop on beta < 1e-12; psi_new = V exp(-i dt T) e_1,
+    renormalised."""
+    V = [psi / np.linalg.norm(psi)]
+    al, be = [], []
+    w = H @ V[0]
+    al.append(float(np.real(np.vdot(V[0], w))))
+    w = w - al[0] * V[0]
+    for j in range(2, m + 1):
+        b = float(np.linalg.norm(w))
+        if b < 1e-12:
+            break
+        be.append(b)
+        V.append(w / b)
+        w = H @ V[-1]
+        a = float(np.real(np.vdot(V[-1], w)))
+        al.append(a)
+        w = w - a * V[-1] - b * V[-2]
+        for vk in V:
+            w = w - np.vdot(vk, w) * vk
+    k = len(al)
+    T = np.diag(al) + np.diag(be[:k - 1], 1) + np.diag(be[:k - 1], -1)
+    wT, QT = np.linalg.eigh(T)
+    coeffs = QT @ (np.exp(-1j * dt * wT) * QT[0])
+    out = sum(c * v for c, v in zip(coeffs, V))
+    return out / np.linalg.norm(out)
