@@ -476,6 +476,16 @@ int32_t peps_ed_ground_state(peps_ed* ed, double tol, double* E0, double* psi);
    out = {nf_mean, nf_even, nf_odd, E2_mean}; nf_grid, E2_grid (optional)
    nx*ny, ix fastest. */
 int32_t peps_ed_measure(peps_ed* ed, const double* psi, double out[4], double* nf_grid, double* E2_grid);
+/* evolve_ED! (finite_ed.jl:405-430): psi <- exp(-i dt H) psi, renormalised,
+   nsteps times, each step by KrylovKit's exponentiate as the reference calls
+   it (Lanczos projections of at most krylovdim vectors, error estimate <= tol,
+   the interval halved until it is); psi0 and psi_out (may be NULL) are
+   interleaved complex vectors of the basis dimension.  obs gets (nsteps + 1)
+   rows of {nf_mean, nf_even, nf_odd, E2_mean} (t = 0 first); nf_grids and
+   E2_grids (may be NULL) (nsteps + 1) site grids of nx*ny (ix fastest);
+   projections (may be NULL) the Krylov projections used in all. */
+int32_t peps_ed_evolve(peps_ed* ed, const double* psi0, double dt, int32_t nsteps, int32_t krylovdim, double tol,
+                       double* psi_out, double* obs, double* nf_grids, double* E2_grids, int32_t* projections);
 
 #ifdef __cplusplus
 }

@@ -1141,6 +1141,37 @@ int32_t peps_ed_ground_state(peps_ed* ed, double tol, double* E0, double* psi) {
   });
 }
 
+int32_t peps_ed_evolve(peps_ed* ed, const double* psi0, double dt, int32_t nsteps, int32_t krylovdim, double tol,
+                       double* psi_out, double* obs, double* nf_grids, double* E2_grids, int32_t* projections) {
+  if (!ed || !psi0 || !obs || nsteps < 0 || krylovdim < 2 || !(tol > 0.0) || !std::isfinite(dt)) return PEPS_EARG;
+  return guard(ed->ref.get(), [&] {
+    peps::Ctx& c = *ed->ref;
+    peps::EDBasis& b = *ed->b;
+    const size_t ns = size_t(b.nx) * b.ny;
+    Tensor psi = c.upload({b.dim}, true, psi0);
+    int total = 0;
+    for (int32_t s = 0; s <= nsteps; ++s) {
+      if (s > 0) {
+        int sub = 0;
+        psi = peps::ed_exp_step(b, psi, dt, krylovdim, tol, &sub);
+        total += sub;
+        const double nrm = std::sqrt(peps::norm2sq(c, psi));  // psi ./= norm(psi)
+        PEPS_CHECK(nrm > 0.0 && std::isfinite(nrm), peps::PS_ESOLVER, "ed_evolve: zero or non-finite state");
+        peps::scale(c, psi, 1.0 / nrm);
+      }
+      peps::EDObs o = peps::ed_measure(b, psi);
+      obs[4 * size_t(s)] = o.nf_mean;
+      obs[4 * size_t(s) + 1] = o.nf_even;
+      obs[4 * size_t(s) + 2] = o.nf_odd;
+      obs[4 * size_t(s) + 3] = o.E2_mean;
+      if (nf_grids) std::copy(o.nf_grid.begin(), o.nf_grid.end(), nf_grids + ns * s);
+      if (E2_grids) std::copy(o.E2_grid.begin(), o.E2_grid.end(), E2_grids + ns * s);
+    }
+    if (psi_out) c.download(psi, psi_out);
+    if (projections) *projections = total;
+  });
+}
+
 int32_t peps_ed_measure(peps_ed* ed, const double* psi, double out[4], double* nf_grid, double* E2_grid) {
   if (!ed || !out) return PEPS_EARG;
   return guard(ed->ref.get(), [&] {

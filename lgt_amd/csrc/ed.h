@@ -45,7 +45,14 @@ void ed_apply(EDBasis& b, const Tensor& x, Tensor& y);
 // find_ground_state (finite_ed.jl:333-341): restarted Lanczos with full
 // reorthogonalisation until ||H psi - E0 psi|| <= tol.
 double ed_ground_state(EDBasis& b, double tol, int krylov, int max_restarts);
-// measure_ED (finite_ed.jl:369-395) of a real state.
+// measure_ED (finite_ed.jl:369-395) of a real or complex state.
 EDObs ed_measure(EDBasis& b, const Tensor& psi);
+// y = H x for interleaved complex vectors.
+void ed_apply_complex(EDBasis& b, const Tensor& x, Tensor& y);
+// exp(-i dt H) psi (complex psi), the KrylovKit exponentiate call of
+// evolve_ED! (finite_ed.jl:417-419): Lanczos projections of at most krylov
+// vectors, each interval halved until the error estimate is <= tol;
+// substeps (may be null) gets the number of projections used.
+Tensor ed_exp_step(EDBasis& b, const Tensor& psi, double dt, int krylov, double tol, int* substeps);
 
 }  // namespace peps

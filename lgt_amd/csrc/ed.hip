@@ -226,6 +226,30 @@ __global__ void ed_spmv_kernel(int64_t dim, int W, const double* __restrict__ di
   }
 }
 
+// The same product for an interleaved complex x (H is real).
+__global__ void ed_spmv_cplx_kernel(int64_t dim, int W, const double* __restrict__ diag,
+                                    const int* __restrict__ cols, double t, const double* __restrict__ x,
+                                    double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < dim; i += int64_t(gridDim.x) * blockDim.x) {
+    double hr = 0.0, hi = 0.0;
+    for (int w = 0; w < W; ++w) {
+      const int c = cols[int64_t(w) * dim + i];
+      if (c >= 0) {
+        hr += x[2 * int64_t(c)];
+        hi += x[2 * int64_t(c) + 1];
+      }
+    }
+    y[2 * i] = diag[i] * x[2 * i] - t * hr;
+    y[2 * i + 1] = diag[i] * x[2 * i + 1] - t * hi;
+  }
+}
+
+// |x_i| of an interleaved complex vector.
+__global__ void ed_cabs_kernel(int64_t dim, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < dim; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] = sqrt(x[2 * i] * x[2 * i] + x[2 * i + 1] * x[2 * i + 1]);
+}
+
 __global__ __launch_bounds__(256) void ed_measure_kernel(EdGeom G, const uint64_t* __restrict__ keys,
                                                          const double* __restrict__ psi, int64_t dim,
                                                          double* __restrict__ part) {
@@ -418,6 +442,100 @@ void ed_apply(EDBasis& b, const Tensor& x, Tensor& y) {
   PEPS_HIP(hipGetLastError());
 }
 
+void ed_apply_complex(EDBasis& b, const Tensor& x, Tensor& y) {
+  PEPS_CHECK(b.has_h, PS_EARG, "ed: build the Hamiltonian first");
+  PEPS_CHECK(x.numel() == b.dim && y.numel() == b.dim && x.cplx && y.cplx, PS_ESHAPE, "ed: complex vector size");
+  Ctx& c = *b.c;
+  hipLaunchKernelGGL(ed_spmv_cplx_kernel, dim3(grid_for(b.dim)), dim3(256), 0, c.stream, b.dim, b.W, b.diag.data(),
+                     (const int*)b.cols->ptr, b.t_hop, x.data(), y.data());
+  PEPS_HIP(hipGetLastError());
+}
+
+Tensor ed_exp_step(EDBasis& b, const Tensor& psi0, double dt, int krylov, double tol, int* substeps) {
+  // exponentiate(H, -i dt, psi; krylovdim, tol, ishermitian = true), the
+  // call of evolve_ED! (finite_ed.jl:417-419): a Lanczos basis of psi with
+  // full reorthogonalisation, psi(h) = |psi| V exp(-i h T) e_1, and the
+  // error estimate |psi| beta_k |e_k^T exp(-i h T) e_1| of the truncated
+  // projection; while it exceeds tol the interval is halved (the basis does
+  // not depend on h), and the rest of dt is covered by further projections.
+  PEPS_CHECK(b.has_h, PS_EARG, "ed: build the Hamiltonian first");
+  PEPS_CHECK(psi0.cplx && psi0.numel() == b.dim && krylov >= 2 && tol > 0.0, PS_EARG, "ed_evolve: bad input");
+  Ctx& c = *b.c;
+  TierGuard fp64(c, TIER_FP64);
+  const int64_t n = b.dim;
+  const int m = int(std::min<int64_t>(krylov, n));
+  Tensor V = c.alloc({n, int64_t(m)}, true);
+  Tensor w = c.alloc({n}, true);
+  Tensor psi = copy_tensor(c, psi0.reshaped({n}));
+  double left = dt;
+  int nsub = 0;
+  while (left > 0.0) {
+    const double nu = std::sqrt(norm2sq(c, psi));
+    PEPS_CHECK(nu > 0.0 && std::isfinite(nu), PS_ESOLVER, "ed_evolve: zero or non-finite state");
+    Tensor v0 = V.slice_last(0, 1);
+    PEPS_HIP(hipMemcpyAsync(v0.data(), psi.data(), size_t(n) * 16, hipMemcpyDeviceToDevice, c.stream));
+    scale(c, v0, 1.0 / nu);
+    std::vector<double> alpha, beta;
+    int k = 0;
+    double bk = 0.0;
+    for (int j = 0; j < m; ++j) {
+      Tensor vj = V.slice_last(j, j + 1).reshaped({n});
+      ed_apply_complex(b, vj, w);
+      double a = dot(c, w, vj, true).re;
+      axpy_inplace(c, w, vj, -a);
+      if (j > 0) axpy_inplace(c, w, V.slice_last(j - 1, j).reshaped({n}), -beta[j - 1]);
+      Tensor Vj = V.slice_last(0, j + 1);
+      Tensor wm = w.reshaped({n, 1});
+      for (int pass = 0; pass < 2; ++pass) {  // full reorthogonalisation
+        Tensor h = matmul(c, Vj, wm, true, false, true, false);
+        a += c.to_host(h)[2 * size_t(j)];
+        axpy_inplace(c, wm, matmul(c, Vj, h), -1.0);
+      }
+      bk = std::sqrt(norm2sq(c, w));
+      alpha.push_back(a);
+      k = j + 1;
+      if (j + 1 == m || bk <= 1e-14 * std::max(1.0, std::fabs(a))) break;
+      beta.push_back(bk);
+      Tensor vn = V.slice_last(j + 1, j + 2);
+      PEPS_HIP(hipMemcpyAsync(vn.data(), w.data(), size_t(n) * 16, hipMemcpyDeviceToDevice, c.stream));
+      scale(c, vn, 1.0 / bk);
+    }
+    const bool exhausted = bk <= 1e-14 * std::max(1.0, std::fabs(alpha.back()));
+    std::vector<double> T(size_t(k) * k, 0.0), ev, Z;
+    for (int i = 0; i < k; ++i) {
+      T[size_t(i) * k + i] = alpha[i];
+      if (i + 1 < k) T[size_t(i) * k + i + 1] = T[size_t(i + 1) * k + i] = beta[i];
+    }
+    jacobi_host(T, k, ev, Z);
+    auto coeffs = [&](double h) {
+      std::vector<double> co(2 * size_t(k), 0.0);
+      for (int i = 0; i < k; ++i)
+        for (int l = 0; l < k; ++l) {
+          const double f = Z[size_t(i) * k + l] * Z[size_t(l)];
+          co[2 * i] += f * std::cos(h * ev[l]);
+          co[2 * i + 1] -= f * std::sin(h * ev[l]);
+        }
+      return co;
+    };
+    double h = left;
+    std::vector<double> co = coeffs(h);
+    int halvings = 0;
+    while (!exhausted && nu * bk * std::hypot(co[2 * (k - 1)], co[2 * (k - 1) + 1]) > tol) {
+      PEPS_CHECK(++halvings <= 40, PS_ESOLVER, "ed_evolve: the Krylov error estimate does not fall below tol");
+      h *= 0.5;
+      co = coeffs(h);
+    }
+    for (double& x : co) x *= nu;
+    Tensor cv = c.upload({int64_t(k), 1}, true, co.data());
+    psi = matmul(c, V.slice_last(0, k), cv).reshaped({n});
+    left = (h >= left) ? 0.0 : left - h;
+    ++nsub;
+  }
+  if (substeps) *substeps = nsub;
+  c.count("ed_evolve[projections]", double(nsub), 0);
+  return psi;
+}
+
 double ed_ground_state(EDBasis& b, double tol, int krylov, int max_restarts) {
   PEPS_CHECK(b.has_h, PS_EARG, "ed: build the Hamiltonian first");
   PEPS_CHECK(tol > 0.0 && krylov >= 2 && max_restarts >= 1, PS_EARG, "ed: bad Lanczos parameters");
@@ -487,9 +605,18 @@ double ed_ground_state(EDBasis& b, double tol, int krylov, int max_restarts) {
   return theta;
 }
 
-EDObs ed_measure(EDBasis& b, const Tensor& psi) {
-  PEPS_CHECK(psi.numel() == b.dim && !psi.cplx, PS_ESHAPE, "ed: state size");
+EDObs ed_measure(EDBasis& b, const Tensor& psi_in) {
+  PEPS_CHECK(psi_in.numel() == b.dim, PS_ESHAPE, "ed: state size");
   Ctx& c = *b.c;
+  // the observables are diagonal in the basis: a complex state is measured
+  // through |psi|
+  Tensor psi = psi_in;
+  if (psi_in.cplx) {
+    psi = c.alloc({b.dim}, false);
+    hipLaunchKernelGGL(ed_cabs_kernel, dim3(grid_for(b.dim)), dim3(256), 0, c.stream, b.dim, psi_in.data(),
+                       psi.data());
+    PEPS_HIP(hipGetLastError());
+  }
   const EdGeom G = geom_of(b);
   const unsigned nb = grid_for(b.dim, 1024);
   const int ns = b.nsite;

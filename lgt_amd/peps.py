@@ -132,6 +132,8 @@ _SIGS = {
     "peps_ed_hamiltonian": (C.c_int32, [_vp, C.c_double, C.c_double, C.c_double]),
     "peps_ed_apply": (C.c_int32, [_vp, _dp, _dp]),
     "peps_ed_ground_state": (C.c_int32, [_vp, C.c_double, _dp, _dp]),
+    "peps_ed_evolve": (C.c_int32, [_vp, _dp, C.c_double, C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp, _dp,
+                                   _i32p]),
     "peps_ed_measure": (C.c_int32, [_vp, _dp, _dp, _dp, _dp]),
     "peps_gemm_noalloc": (C.c_int32, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _vp, C.c_int32, C.c_int32,
                                       _vp, C.c_int32, C.c_int32, _vp]),
@@ -1132,6 +1134,28 @@ class ExactDiag:
                                              _dptr(e2)))
         return dict(nf_mean=out[0], nf_even=out[1], nf_odd=out[2], E2_mean=out[3],
                     nf_grid=nf.reshape(self.ny, self.nx).T, E2_grid=e2.reshape(self.ny, self.nx).T)
+
+
+    def evolve(self, psi0, dt, n_steps, krylovdim=30, tol=1e-10):
+        """evolve_ED! (finite_ed.jl:405-430) on the GPU: returns (list of
+        measure dicts at t = 0, dt, ..., the final complex state, Krylov
+        projections used)."""
+        p = np.ascontiguousarray(psi0, dtype=np.complex128)
+        if p.shape != (self.dim,):
+            raise ValueError("evolve: state of the basis dimension expected")
+        ns = self.nx * self.ny
+        obs = np.zeros((n_steps + 1, 4))
+        nf = np.zeros((n_steps + 1, ns))
+        e2 = np.zeros((n_steps + 1, ns))
+        out = np.empty(self.dim, np.complex128)
+        proj = C.c_int32()
+        self.ctx.check(lib().peps_ed_evolve(self.h, p.ctypes.data_as(_dp), float(dt), int(n_steps),
+                                            int(min(krylovdim, self.dim)), float(tol), out.ctypes.data_as(_dp),
+                                            _dptr(obs), _dptr(nf), _dptr(e2), C.byref(proj)))
+        data = [dict(t=s * dt, nf_mean=obs[s, 0], nf_even=obs[s, 1], nf_odd=obs[s, 2], E2_mean=obs[s, 3],
+                     nf_grid=nf[s].reshape(self.ny, self.nx).T, E2_grid=e2[s].reshape(self.ny, self.nx).T)
+                for s in range(n_steps + 1)]
+        return data, out, proj.value
 
 
 def build_basis(nx, ny, dg, g_charges, ctx=None) -> ExactDiag:

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import ed_oracle as ED
+from oracle import peps_oracle as P
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +92,69 @@ def test_4x4_properties(ctx):
     assert E0 <= diag.min()
     obs = ed.measure()
     assert obs["nf_grid"].sum() == pytest.approx(8.0, abs=1e-9)
+
+
+def test_evolve_matches_expm_multiply(ctx):
+    """evolve_ED! (finite_ed.jl:405-430) on the GPU: ten steps of dt = 0.05
+    from a seeded complex state of the 3x2 staggered sector equal the
+    oracle's expm_multiply evolution to 1e-9 (state and observables), with
+    the state renormalised every step."""
+    from lgt_amd import peps
+    nx, ny, dg = 3, 2, 1
+    print("ED evolve 3x2", flush=True)
+    gch = np.asarray(P.staggered_charges(nx, ny))
+    b = ED.build_basis(nx, ny, dg, gch)
+    H = ED.build_hamiltonian(b, 1.0, 1.0, 0.5)
+    rng = np.random.default_rng(2)
+    psi0 = rng.standard_normal(b.dim) + 1j * rng.standard_normal(b.dim)
+    psi0 /= np.linalg.norm(psi0)
+    ed = peps.ExactDiag(nx, ny, dg, gch, ctx=ctx).hamiltonian(1.0, 1.0, 0.5)
+    data, psi, proj = ed.evolve(psi0, 0.05, 10)
+    want = ED.evolve_ed(psi0, H, 0.05, 10)
+    assert np.abs(psi - want[-1]).max() < 1e-9
+    assert len(data) == 11 and proj >= 10
+    for s, d in enumerate(data):
+        obs = ED.measure_ed(psi0 if s == 0 else want[s - 1], b)
+        assert np.abs(d["nf_grid"] - obs["nf_grid"]).max() < 1e-9, s
+        assert np.abs(d["E2_grid"] - obs["E2_grid"]).max() < 1e-9, s
+        assert d["t"] == pytest.approx(0.05 * s)
+
+
+def test_quench_A_ED_matches_oracle_and_mps():
+    """run_quench_A_ED (finite_ed.jl:491-551) at 2x2: the string state found
+    through its key, evolved 8 steps of dt = 0.05; the observables equal the
+    oracle ED evolution's (1e-9) and the MPS Krylov quench's on the same
+    lattice (mps_quench.jl's protocol, 1e-7), and the CSV has the reference's
+    columns."""
+    import tempfile
+    from lgt_amd import ed_quench, mps, mps_quench, peps
+    nx, ny, dg = 2, 2, 1
+    print("ED quench A 2x2", flush=True)
+    ctx = peps.Context(0)
+    try:
+        with tempfile.TemporaryDirectory() as d:
+            data, path = ed_quench.run_quench_A_ED(nx=nx, ny=ny, dt=0.05, n_steps=8, results_dir=d, ctx=ctx,
+                                                   log=None)
+            lines = open(path).read().splitlines()
+        assert lines[0] == ",".join(ed_quench.ED_CSV_COLUMNS) and len(lines) == 10
+        H = mps.build_H_mpo(nx, ny, 2 * dg + 1, 1.0, 1.0, 2.0, ctx=ctx)
+        mdata, _ = mps_quench.evolve_mps(mps.string_breaking_mps(nx, ny, 2 * dg + 1), H, nx, ny, 2 * dg + 1,
+                                         0.05, 8, 200, ctx=ctx)
+    finally:
+        ctx.close()
+    g = ed_quench.string_charges(nx, ny)
+    b = ED.build_basis(nx, ny, dg, g)
+    n_f = np.zeros((1, nx, ny), np.int8)
+    n_f[0, 1, 0] = 1
+    e_R = np.zeros((1, nx - 1, ny), np.int8)
+    e_R[0, 0, 0] = 1
+    k0 = ED.encode(n_f, e_R, np.zeros((1, nx, ny - 1), np.int8), dg)[0]
+    psi0 = np.zeros(b.dim, complex)
+    psi0[int(np.nonzero(b.keys == k0)[0][0])] = 1.0
+    states = [psi0] + ED.evolve_ed(psi0, ED.build_hamiltonian(b, 1.0, 1.0, 2.0), 0.05, 8)
+    for s in range(9):
+        obs = ED.measure_ed(states[s], b)
+        for key in ("nf_grid", "E2_grid"):
+            assert np.abs(data[s][key] - obs[key]).max() < 1e-9, (s, key)
+            assert np.abs(mdata[s][key] - obs[key]).max() < 1e-7, (s, key)
+    assert np.abs(data[-1]["nf_grid"] - data[0]["nf_grid"]).max() > 1e-3

@@ -196,3 +196,18 @@ def test_evolve_matches_the_dense_exponential():
     out = ED.evolve_ed(psi, H, 0.05, 3)
     U = sla.expm(-1j * 0.05 * H.toarray())
     np.testing.assert_allclose(out[-1], U @ U @ U @ psi, atol=1e-12)
+
+
+def test_library_state_key_matches_the_oracle_encoding():
+    """lgt_amd.ed_quench.state_key (_encode, finite_ed.jl:79-96) gives the
+    oracle's key for every state of two sectors (3x2 string sector at
+    dg = 1, 2x2 staggered at dg = 2)."""
+    from lgt_amd import ed_quench
+    from oracle import peps_oracle as P
+    for nx, ny, dg, g in ((3, 2, 1, ed_quench.string_charges(3, 2)),
+                          (2, 2, 2, np.asarray(P.staggered_charges(2, 2)))):
+        b = ED.build_basis(nx, ny, dg, g)
+        assert b.dim > 0
+        for s in range(0, b.dim, max(1, b.dim // 50)):
+            k = ed_quench.state_key(b.n_f[s], b.e_R[s], b.e_U[s], dg)
+            assert k == int(b.keys[s]), (nx, ny, dg, s)
