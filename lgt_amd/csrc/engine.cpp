@@ -272,7 +272,68 @@ void topk_right_jw(Ctx& c, const Tensor& M, int64_t kmax, double cut, bool relat
   const int64_t mc = C.dims[0], nc = C.dims[1], N = mc + nc, nsv = std::min(mc, nc);
   Tensor Vs;  // nc x k, spanning the leading k right singular vectors of C
   int64_t k = 1;
+  bool done = false;
   if (M.cplx) {
+    // complex: the Hermitian Jordan-Wielandt matrix J, mixed by a random
+    // real Householder reflector H = I - 2 w w^T (J' = H J H has J's
+    // spectrum, eigenvectors V = H V', and no exactly zero columns for the
+    // tridiagonal reduction to meet); rocSOLVER's complex eigensolver on J'
+    const Tensor Ch = conj_transpose(c, C);
+    Tensor J = c.zeros({N, N}, true);
+    std::vector<int64_t> ru = seq(mc), rv(nc);
+    for (int64_t i = 0; i < nc; ++i) rv[i] = mc + i;
+    scatter_cols(c, J, C, ru, seq(nc), rv);
+    scatter_cols(c, J, Ch, rv, seq(mc), ru);
+    Tensor wv = c.alloc({N, 1}, false);
+    fill_random(c, wv, 0x4d1eull + uint64_t(N));
+    scale(c, wv, 1.0 / std::sqrt(norm2sq(c, wv)));
+    const Tensor w = as_complex(c, wv);
+    // J' = J - 2 w (w^T J) - 2 (J w) w^T + 4 (w^T J w) w w^T
+    //    = J - 2 w (w^T J) - 2 u w^T,  u = J w - 2 (w^T J w) w
+    const Tensor Jw = matmul(c, J, w);                            // N x 1
+    const Tensor wJ = matmul(c, w, J, false, false, true, false);  // 1 x N
+    const cplx q = dot(c, Jw, w, false);                         // w^T J w
+    Tensor u = copy_tensor(c, Jw);
+    Tensor wq = copy_tensor(c, w);
+    scale(c, wq, cplx{-2.0 * q.re, -2.0 * q.im});
+    axpy_inplace(c, u, wq, 1.0);
+    axpy_inplace(c, J, matmul(c, w, wJ), -2.0);
+    axpy_inplace(c, J, matmul(c, u, w, false, false, false, true), -2.0);
+    EIG e = eigh(c, J);
+    const double s0 = e.w.empty() ? 0.0 : std::max(e.w[N - 1], 0.0);
+    const double thr = relative ? cut * s0 : cut;
+    int64_t cnt = 0;
+    for (int64_t j = 0; j < nsv; ++j) cnt += e.w[N - 1 - j] > thr;
+    const int64_t kk = std::max<int64_t>(1, std::min<int64_t>(kmax, cnt));
+    std::vector<int64_t> top(kk);
+    std::vector<double> mw(kk);
+    for (int64_t j = 0; j < kk; ++j) {
+      top[j] = N - 1 - j;
+      mw[j] = -e.w[N - 1 - j];
+    }
+    // the kept pairs are accepted only with finite values and residuals
+    // ||J' v - w v|| at the eigensolver's backward error; otherwise the
+    // real embedding below decides
+    Tensor Vk = gather2d(c, e.V, seq(N), top);  // N x kk
+    Tensor R = matmul(c, J, Vk);
+    Tensor Vw = copy_tensor(c, Vk);
+    scale_cols(c, Vw, mw);
+    axpy_inplace(c, R, Vw, 1.0);
+    const std::vector<double> r2 = col_norm2(c, R);
+    const double tol = std::max(1e-13, 16.0 * double(N) * 2.220446049250313e-16) * std::max(s0, 1e-300);
+    bool ok = std::isfinite(s0) && s0 > 0.0;
+    for (double x : r2) ok = ok && std::isfinite(x) && std::sqrt(x) <= tol;
+    if (ok) {
+      // V = H V' = V' - 2 w (w^T V')
+      axpy_inplace(c, Vk, matmul(c, w, matmul(c, w, Vk, false, false, true, false)), -2.0);
+      k = kk;
+      Vs = gather2d(c, Vk, rv, seq(k));
+      done = true;
+    } else {
+      c.count("bmps[svd: jordan-wielandt complex rejected, real embedding]", 1.0, double(N));
+    }
+  }
+  if (M.cplx && !done) {
     // complex: the real Jordan-Wielandt problem of the real embedding
     // Cr = [[X, -Y], [Y, X]] of C = X + i Y.  Each singular value of C is a
     // pair of Cr, with right vectors (p; q) and (-q; p) for v = p + i q, so
@@ -281,7 +342,8 @@ void topk_right_jw(Ctx& c, const Tensor& M, int64_t kmax, double cut, bool relat
     // range is Z Z^H Omega for a fixed random Omega (nc x k).  (rocSOLVER's
     // complex Householder tridiagonalisation has returned non-finite
     // vectors for these matrices, whose reduction meets exactly zero
-    // columns.)
+    // columns; the mixed form above falls back to this one when its
+    // vectors are not finite.)
     const Tensor Cr = real_embedding(c, C);
     const int64_t Nr = 2 * N;
     std::vector<int64_t> ru = seq(2 * mc), rv(2 * nc), rp(nc), rq(nc);
