@@ -124,15 +124,19 @@ def ite_ground_state(nx, ny, gd, D_bond, D_max, g, t_hop, m, tau_ite, n_ite, mu=
     return gp
 
 
-def evolve(gp, g, t_hop, m, dt, n_steps, D_trunc, measure_every=10, log=None):
+def evolve(gp, g, t_hop, m, dt, n_steps, D_trunc, measure_every=10, log=None, progress=None):
     """evolve! (finite_peps_quench.jl:322-353): measure at t = 0, then after
     every realtime_trotter_step; returns the list of measure_all_finite
-    dicts (t, scalar observables, nf_grid, E2_grid)."""
+    dicts (t, scalar observables, nf_grid, E2_grid).  `log` gets the
+    reference's table rows every measure_every steps; `progress` (optional)
+    is called after every step with (step, t, observables)."""
     data = [peps.measure_all_finite(gp, 0.0)]
     for step in range(1, n_steps + 1):
         peps.realtime_trotter_step(gp, g, t_hop, m, dt, D_trunc)
         obs = peps.measure_all_finite(gp, step * dt)
         data.append(obs)
+        if progress:
+            progress(step, step * dt, obs)
         if log and (step % measure_every == 0 or step == n_steps):
             D_now = max([len(gp.bond(peps.PEPS_H, ix, iy)[0]) for iy in range(1, gp.ny + 1)
                          for ix in range(1, gp.nx)], default=1)
