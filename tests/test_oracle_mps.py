@@ -127,3 +127,24 @@ def test_string_product_vector_sector():
             v = M.product_vector(cfg, dims)
             G = M.gauss_values(nx, ny, gd)[int(np.argmax(np.abs(v)))]
         assert np.array_equal(G, mps.string_charges(nx, ny))
+
+
+def test_quench_csv_writers(tmp_path):
+    """write_mps_csv (mps_quench.jl:64-75) and data_to_df_ED
+    (finite_ed.jl:437-445) write the reference's five columns, one row per
+    measurement, with Julia's float printing; the drivers' CLIs refuse an
+    unknown protocol."""
+    from lgt_amd import ed_quench, mps_quench
+    rows = [dict(t=0.0, nf_mean=1 / 12, nf_even=1 / 6, nf_odd=0.0, E2_mean=1 / 12, extra=1),
+            dict(t=0.05, nf_mean=0.0833, nf_even=0.1654, nf_odd=0.0012, E2_mean=0.084, extra=2)]
+    p1 = mps_quench.write_mps_csv(rows, "mps_quench_A", str(tmp_path))
+    p2 = ed_quench.data_to_csv(rows, str(tmp_path / "finite_ed_quench_A_data.csv"))
+    for p in (p1, p2):
+        lines = open(p).read().splitlines()
+        assert lines[0] == "t,nf_mean,nf_even,nf_odd,E2_mean" and len(lines) == 3
+        assert lines[1].split(",")[0] == "0.0" and lines[2].split(",")[3] == "0.0012"
+    assert p1.endswith("mps_quench_A_data.csv")
+    with pytest.raises(SystemExit):
+        mps_quench.main(["4", "--results", str(tmp_path)])
+    with pytest.raises(SystemExit):
+        ed_quench.main(["0", "--results", str(tmp_path)])
